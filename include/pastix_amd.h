@@ -1,0 +1,73 @@
+/**
+ * @file pastix_amd.h
+ *
+ * Extensions of the MI355X engine that have no counterpart in the reference
+ * API.  They live in their own parameter array so that the reference
+ * iparm/dparm numbering (include/pastix/api.h) stays untouched.
+ */
+#ifndef PASTIX_AMD_EXT_H
+#define PASTIX_AMD_EXT_H
+
+#include "pastix.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pastix_amd_iparm_e {
+    /* Arithmetic of the factor: PastixDouble (default: that of the matrix) or
+     * PastixFloat (FP32 factors refined in FP64 by pastix_task_refine).      */
+    PXA_IPARM_FACTOR_PRECISION = 0,
+    /* HIP device ordinal used by this instance (default 0).                  */
+    PXA_IPARM_DEVICE           = 1,
+    /* Apply the reference's level reordering of supernodes (default 0).      */
+    PXA_IPARM_LEVEL_ORDER      = 2,
+    PXA_IPARM_SIZE             = 8
+} pastix_amd_iparm_t;
+
+/* Set / get an engine parameter.  Returns PASTIX_SUCCESS or
+ * PASTIX_ERR_BADPARAMETER. */
+int pastix_amd_set_iparm(pastix_data_t *pastix_data, int key, pastix_int_t value);
+int pastix_amd_get_iparm(const pastix_data_t *pastix_data, int key, pastix_int_t *value);
+
+/* Sizes of the analysed structure:
+ * out[0]=n, [1]=cblknbr, [2]=bloknbr, [3]=coefnbr (entries of one panel
+ * table), [4]=browtab size, [5]=number of dependency levels,
+ * [6]=update tiles, [7]=update entries. */
+int pastix_amd_symbol_sizes(const pastix_data_t *pastix_data, int64_t out[8]);
+
+/* Copy the analysed structure out (any pointer may be NULL).  Row/column
+ * indices are in the permuted numbering; permtab maps user -> permuted.
+ * cblk_* have cblknbr+1 entries (fcol, bloknum, coef) or cblknbr (stride);
+ * blok_* have bloknbr entries. */
+int pastix_amd_symbol_export(const pastix_data_t *pastix_data,
+                             int64_t *cblk_fcol, int64_t *cblk_bloknum,
+                             int64_t *cblk_stride, int64_t *cblk_coef,
+                             int64_t *blok_frow, int64_t *blok_lrow,
+                             int64_t *blok_fcblk, int64_t *blok_roff,
+                             int64_t *permtab, int64_t *peritab);
+
+/* Copy the device factor to host buffers of coefnbr entries each, in the
+ * factor precision (U only for LU). */
+int pastix_amd_coeftab_export(const pastix_data_t *pastix_data, void *L, void *U);
+
+/* Solve with right-hand sides already resident in device memory, user
+ * ordering, in the factor precision; ldb >= n. */
+int pastix_amd_solve_device(pastix_data_t *pastix_data, pastix_int_t nrhs,
+                            void *d_b, pastix_int_t ldb);
+
+/* Normwise backward error max|b-Ax| / (||A||_inf max|x| + max|b|) for host
+ * FP64 vectors in the user ordering, computed on the device. */
+int pastix_amd_backward_error(pastix_data_t *pastix_data, const double *b,
+                              const double *x, double *berr);
+
+/* Device time of the last numerical factorisation split by kernel family:
+ * out[0]=total, [1]=diagonal, [2]=panel TRSM, [3]=update (seconds; the
+ * split is filled only when profiling is enabled). */
+int pastix_amd_fact_times(const pastix_data_t *pastix_data, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

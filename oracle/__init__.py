@@ -1,0 +1,137 @@
+"""CPU oracle of the MI355X engine -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+package, as the checker and as the timed CPU baseline.  The product
+(pastix_amd, libpastix_amd.so) never imports it.
+
+`Oracle` wraps liboracle.so (oracle.c: a plain-C restatement of the
+reference sequential supernodal factorisation, solve, SpMV, flop count and
+CG refinement) over the block structure exported by the engine's analysis.
+`bcsc()` restates the reference permuted block-CSC construction
+(bcsc/bcsc_zinit.c:485-541) with scipy: B = P A P^T with its full pattern.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+import scipy.sparse as sps
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+
+class ora_symbol_t(ct.Structure):
+    _fields_ = [("n", ct.c_int64), ("cblknbr", ct.c_int64), ("bloknbr", ct.c_int64), ("coefnbr", ct.c_int64)] + \
+               [(k, ct.POINTER(ct.c_int64)) for k in ("cblk_fcol", "cblk_bloknum", "cblk_stride", "cblk_coef",
+                                                       "blok_frow", "blok_lrow", "blok_fcblk", "blok_roff")]
+
+
+def _lib():
+    if not os.path.exists(_LIB):
+        raise ImportError(f"{_LIB} missing: run make -C oracle")
+    lib = ct.CDLL(_LIB)
+    P = ct.c_void_p
+    lib.ora_fill.argtypes = [P] * 9
+    lib.ora_fill.restype = ct.c_int
+    lib.ora_potrf.argtypes = [P, P, ct.c_double, P]
+    lib.ora_potrf.restype = ct.c_int
+    lib.ora_getrf.argtypes = [P, P, P, ct.c_double, P]
+    lib.ora_getrf.restype = ct.c_int
+    lib.ora_solve.argtypes = [P, ct.c_int, P, P, P, ct.c_int64, ct.c_int]
+    lib.ora_solve.restype = ct.c_int
+    lib.ora_spmv.argtypes = [ct.c_int64, P, P, P, ct.c_double, P, ct.c_double, P]
+    lib.ora_spmv.restype = None
+    lib.ora_thflops.argtypes = [P, ct.c_int]
+    lib.ora_thflops.restype = ct.c_double
+    lib.ora_cg.argtypes = [P, P, ct.c_int64, P, P, P, P, P, ct.c_int, ct.c_double, P]
+    lib.ora_cg.restype = ct.c_int
+    return lib
+
+
+def bcsc(A, permtab):
+    """B = P A P^T (full pattern) as (csc of B, csc of B^T) index/value arrays."""
+    n = A.shape[0]
+    p = np.asarray(permtab, dtype=np.int64)
+    Pm = sps.csr_matrix((np.ones(n), (p, np.arange(n))), shape=(n, n))
+    B = sps.csc_matrix(Pm @ sps.csc_matrix(A) @ Pm.T)
+    B.sum_duplicates()
+    B.sort_indices()
+    Bt = sps.csc_matrix(B.T)
+    Bt.sort_indices()
+    as64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)
+    return {"B": B, "colptr": as64(B.indptr), "rows": as64(B.indices), "vals": np.ascontiguousarray(B.data),
+            "tcolptr": as64(Bt.indptr), "trows": as64(Bt.indices), "tvals": np.ascontiguousarray(Bt.data)}
+
+
+class Oracle:
+    def __init__(self, sym):
+        self.lib = _lib()
+        self.sym = {k: np.ascontiguousarray(v, dtype=np.int64) for k, v in sym.items() if isinstance(v, np.ndarray)}
+        s = ora_symbol_t()
+        s.n, s.cblknbr, s.bloknbr, s.coefnbr = sym["n"], sym["cblknbr"], sym["bloknbr"], sym["coefnbr"]
+        for k in ("cblk_fcol", "cblk_bloknum", "cblk_stride", "cblk_coef", "blok_frow", "blok_lrow", "blok_fcblk",
+                  "blok_roff"):
+            setattr(s, k, self.sym[k].ctypes.data_as(ct.POINTER(ct.c_int64)))
+        self.s = s
+        self.n = sym["n"]
+        self.coefnbr = sym["coefnbr"]
+
+    def fill(self, bc, lu=False):
+        L = np.zeros(self.coefnbr)
+        U = np.zeros(self.coefnbr) if lu else None
+        rc = self.lib.ora_fill(ct.byref(self.s), bc["colptr"].ctypes.data, bc["rows"].ctypes.data,
+                               bc["vals"].ctypes.data, bc["tcolptr"].ctypes.data, bc["trows"].ctypes.data,
+                               bc["tvals"].ctypes.data, L.ctypes.data, U.ctypes.data if lu else None)
+        return L, U, rc
+
+    def potrf(self, L, crit):
+        npiv = ct.c_int64()
+        rc = self.lib.ora_potrf(ct.byref(self.s), L.ctypes.data, crit, ct.byref(npiv))
+        return rc, npiv.value
+
+    def getrf(self, L, U, crit):
+        npiv = ct.c_int64()
+        rc = self.lib.ora_getrf(ct.byref(self.s), L.ctypes.data, U.ctypes.data, crit, ct.byref(npiv))
+        return rc, npiv.value
+
+    def solve(self, L, U, x):
+        """In-place solve of a Fortran-ordered (n, nrhs) block in the permuted ordering."""
+        x = np.asfortranarray(x, dtype=np.float64)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        rc = self.lib.ora_solve(ct.byref(self.s), 1 if U is not None else 0, L.ctypes.data,
+                                U.ctypes.data if U is not None else None, x.ctypes.data, self.n, nrhs)
+        assert rc == 0
+        return x
+
+    def spmv(self, bc, x, alpha=1.0, beta=0.0, y=None):
+        """y = alpha * B x + beta * y with B from bcsc(): rows of B are the columns of B^T."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n) if y is None else np.ascontiguousarray(y, dtype=np.float64).copy()
+        self.lib.ora_spmv(self.n, bc["tcolptr"].ctypes.data, bc["trows"].ctypes.data, bc["tvals"].ctypes.data,
+                          alpha, x.ctypes.data, beta, y.ctypes.data)
+        return y
+
+    def thflops(self, lu=False):
+        return self.lib.ora_thflops(ct.byref(self.s), 1 if lu else 0)
+
+    def cg(self, L, bc, b, x, itermax=250, eps=1e-12):
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        res = ct.c_double()
+        it = self.lib.ora_cg(ct.byref(self.s), L.ctypes.data, self.n, bc["tcolptr"].ctypes.data,
+                             bc["trows"].ctypes.data, bc["tvals"].ctypes.data, b.ctypes.data, x.ctypes.data,
+                             itermax, eps, ct.byref(res))
+        return x, it, res.value
+
+    # ---- helpers on the 1D panel layout ----
+    def to_dense(self, L, lower_only=True):
+        """Scatter a panel table into a dense n x n matrix (permuted numbering)."""
+        s = self.sym
+        D = np.zeros((self.n, self.n))
+        for c in range(len(s["cblk_stride"])):
+            fc, lc, st = s["cblk_fcol"][c], s["cblk_fcol"][c + 1], s["cblk_stride"][c]
+            P = L[s["cblk_coef"][c]:s["cblk_coef"][c + 1]].reshape((lc - fc, st)).T
+            for b in range(s["cblk_bloknum"][c], s["cblk_bloknum"][c + 1]):
+                r0, nr = s["blok_roff"][b], s["blok_lrow"][b] - s["blok_frow"][b] + 1
+                D[s["blok_frow"][b]:s["blok_frow"][b] + nr, fc:lc] = P[r0:r0 + nr, :]
+        return np.tril(D) if lower_only else D

@@ -1,0 +1,393 @@
+/*
+ * CPU oracle for the MI355X engine -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Plain-C restatement of the reference PaStiX numerical path, used by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker and
+ * CPU baseline.  It is never linked into or called by libpastix_amd.so.
+ *
+ * It runs on the same block structure as the engine (1D panel layout: cblk c
+ * owns coef[cblk_coef[c] .. + stride*width), column-major, ld = stride) and
+ * follows, function by function:
+ *   ora_fill      kernels/cpucblk_zinit.c:229-309      (cpucblk_zfillin_fr)
+ *   potf2/potrf   kernels/core_zpotrfsp.c:66-189        (64-blocked, static pivot)
+ *   getf2/getrf   kernels/core_zgetrfsp.c:63-259, core_zgetmo.c:48-60, core_zgeadd.c
+ *   trsm panel    kernels/core_ztrsmsp.c:147-181,357-398
+ *   gemmsp        kernels/core_zgemmsp.c:396-465        (2d2d loop, 1D addressing)
+ *   sequential    sopalin/sequential_zpotrf.c:34-68, sequential_zgetrf.c (same shape)
+ *   solve         kernels/solve_ztrsmsp.c:67-100,332-557, sopalin/sequential_ztrsm.c:34-103
+ *   spmv          bcsc/bcsc_zspmv.c:55-202
+ *   thflops       symbol/symbol_cost.c:53-88, symbol_cost_flops.c, common/flops.h
+ *   cg / gmres    refinement/z_refine_grad.c:46-156, z_refine_gmres.c:46-352
+ *
+ * Parity status of each function is recorded in the repository design notes;
+ * every function here is pinned by a test in tests/ (dense factorisations,
+ * closed-form flop counts, scipy SpMV, dense solves).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define BLK 64 /* MAXSIZEOFBLOCKS, kernels/core_zpotrfsp.c:25 */
+
+static inline int64_t imin(int64_t a, int64_t b) { return a < b ? a : b; }
+static inline int64_t width(const ora_symbol_t *s, int64_t c) { return s->cblk_fcol[c + 1] - s->cblk_fcol[c]; }
+static inline int64_t brows(const ora_symbol_t *s, int64_t b) { return s->blok_lrow[b] - s->blok_frow[b] + 1; }
+
+/* C(m x n, ldc) += alpha * A(m x k, lda) * op(B), op(B)(l, j) = B[j + l*ldb] (B^T, NT) */
+static void gemm_nt(int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda, const double *B,
+                    int64_t ldb, double *C, int64_t ldc) {
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t l = 0; l < k; l++) {
+            const double b = alpha * B[j + l * ldb];
+            if (b == 0.0) continue;
+            const double *a = A + l * lda;
+            double *c = C + j * ldc;
+            for (int64_t i = 0; i < m; i++) c[i] += a[i] * b;
+        }
+}
+/* C += alpha * A * B with B(l, j) = B[l + j*ldb] (NN) */
+static void gemm_nn(int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda, const double *B,
+                    int64_t ldb, double *C, int64_t ldc) {
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t l = 0; l < k; l++) {
+            const double b = alpha * B[l + j * ldb];
+            if (b == 0.0) continue;
+            const double *a = A + l * lda;
+            double *c = C + j * ldc;
+            for (int64_t i = 0; i < m; i++) c[i] += a[i] * b;
+        }
+}
+
+/* X * L^T = B in place, L lower non-unit n x n: dtrsm(Right, Lower, Trans, NonUnit) */
+static void trsm_rltn(int64_t m, int64_t n, const double *L, int64_t ldl, double *X, int64_t ldx) {
+    for (int64_t j = 0; j < n; j++) {
+        double *xj = X + j * ldx;
+        for (int64_t k = 0; k < j; k++) {
+            const double l = L[j + k * ldl];
+            if (l == 0.0) continue;
+            const double *xk = X + k * ldx;
+            for (int64_t i = 0; i < m; i++) xj[i] -= l * xk[i];
+        }
+        const double d = 1.0 / L[j + j * ldl];
+        for (int64_t i = 0; i < m; i++) xj[i] *= d;
+    }
+}
+/* X * U = B in place, U upper n x n stored in A: dtrsm(Right, Upper, NoTrans, diag) */
+static void trsm_run(int64_t m, int64_t n, const double *U, int64_t ldu, int unit, double *X, int64_t ldx) {
+    for (int64_t j = 0; j < n; j++) {
+        double *xj = X + j * ldx;
+        for (int64_t k = 0; k < j; k++) {
+            const double u = U[k + j * ldu];
+            if (u == 0.0) continue;
+            const double *xk = X + k * ldx;
+            for (int64_t i = 0; i < m; i++) xj[i] -= u * xk[i];
+        }
+        if (!unit) {
+            const double d = 1.0 / U[j + j * ldu];
+            for (int64_t i = 0; i < m; i++) xj[i] *= d;
+        }
+    }
+}
+/* L X = B in place, L unit lower n x n: dtrsm(Left, Lower, NoTrans, Unit), B n x m */
+static void trsm_llnu(int64_t n, int64_t m, const double *L, int64_t ldl, double *X, int64_t ldx) {
+    for (int64_t j = 0; j < m; j++) {
+        double *x = X + j * ldx;
+        for (int64_t k = 0; k < n; k++) {
+            const double xk = x[k];
+            if (xk == 0.0) continue;
+            for (int64_t i = k + 1; i < n; i++) x[i] -= xk * L[i + k * ldl];
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- fill --- */
+int ora_fill(const ora_symbol_t *s, const int64_t *colptr, const int64_t *rows, const double *vals,
+             const int64_t *tcolptr, const int64_t *trows, const double *tvals, double *L, double *U) {
+    memset(L, 0, s->coefnbr * sizeof(double));
+    if (U) memset(U, 0, s->coefnbr * sizeof(double));
+    int64_t dropped = 0;
+    for (int64_t c = 0; c < s->cblknbr; c++) {
+        const int64_t fc = s->cblk_fcol[c], lc = s->cblk_fcol[c + 1], st = s->cblk_stride[c];
+        for (int64_t j = fc; j < lc; j++) {
+            int64_t b = s->cblk_bloknum[c];
+            for (int64_t k = colptr[j]; k < colptr[j + 1]; k++) {
+                const int64_t i = rows[k];
+                if (i < j) continue;
+                while (b < s->cblk_bloknum[c + 1] && s->blok_lrow[b] < i) b++;
+                if (b >= s->cblk_bloknum[c + 1] || s->blok_frow[b] > i) { dropped++; continue; }
+                L[s->cblk_coef[c] + s->blok_roff[b] + (i - s->blok_frow[b]) + (j - fc) * st] = vals[k];
+            }
+            if (U) {
+                int64_t bu = s->cblk_bloknum[c];
+                for (int64_t k = tcolptr[j]; k < tcolptr[j + 1]; k++) {
+                    const int64_t i = trows[k];
+                    if (i <= j) continue;
+                    while (bu < s->cblk_bloknum[c + 1] && s->blok_lrow[bu] < i) bu++;
+                    if (bu >= s->cblk_bloknum[c + 1] || s->blok_frow[bu] > i) { dropped++; continue; }
+                    U[s->cblk_coef[c] + s->blok_roff[bu] + (i - s->blok_frow[bu]) + (j - fc) * st] = tvals[k];
+                }
+            }
+        }
+    }
+    return (int)(dropped > 0);
+}
+
+/* ------------------------------------------------------- diagonal LLt --- */
+static int potf2(int64_t n, double *A, int64_t lda, int64_t *npiv, double crit) {
+    for (int64_t k = 0; k < n; k++) {
+        double *akk = A + k * (lda + 1);
+        if (fabs(*akk) < crit) { *akk = crit; (*npiv)++; }
+        if (*akk < 0.0) return -1;
+        *akk = sqrt(*akk);
+        const double alpha = 1.0 / *akk;
+        for (int64_t i = k + 1; i < n; i++) A[i + k * lda] *= alpha;
+        /* dsyr lower: A[k+1:, k+1:] -= a a^T */
+        for (int64_t j = k + 1; j < n; j++) {
+            const double aj = A[j + k * lda];
+            for (int64_t i = j; i < n; i++) A[i + j * lda] -= A[i + k * lda] * aj;
+        }
+    }
+    return 0;
+}
+
+static int potrfsp(int64_t n, double *A, int64_t lda, int64_t *npiv, double crit) {
+    for (int64_t k0 = 0; k0 < n; k0 += BLK) {
+        const int64_t bs = imin(BLK, n - k0);
+        double *akk = A + k0 * (lda + 1);
+        if (potf2(bs, akk, lda, npiv, crit)) return -1;
+        const int64_t rest = n - k0 - bs;
+        if (rest > 0) {
+            double *a1 = akk + bs, *a2 = a1 + bs * lda;
+            trsm_rltn(rest, bs, akk, lda, a1, lda);
+            /* dsyrk lower: a2 -= a1 a1^T */
+            for (int64_t j = 0; j < rest; j++)
+                for (int64_t l = 0; l < bs; l++) {
+                    const double b = a1[j + l * lda];
+                    for (int64_t i = j; i < rest; i++) a2[i + j * lda] -= a1[i + l * lda] * b;
+                }
+        }
+    }
+    return 0;
+}
+
+/* -------------------------------------------------------- diagonal LU --- */
+static void getf2(int64_t m, int64_t n, double *A, int64_t lda, int64_t *npiv, double crit) {
+    const int64_t mn = imin(m, n);
+    for (int64_t k = 0; k < mn; k++) {
+        double *akk = A + k * (lda + 1);
+        if (fabs(*akk) < crit) { *akk = (*akk < 0.0) ? -crit : crit; (*npiv)++; }
+        const double alpha = 1.0 / *akk;
+        for (int64_t i = k + 1; i < m; i++) A[i + k * lda] *= alpha;
+        if (k + 1 < mn)
+            for (int64_t j = k + 1; j < n; j++) {
+                const double u = A[k + j * lda];
+                for (int64_t i = k + 1; i < m; i++) A[i + j * lda] -= A[i + k * lda] * u;
+            }
+    }
+}
+
+static void getrfsp(int64_t n, double *A, int64_t lda, int64_t *npiv, double crit) {
+    for (int64_t k0 = 0; k0 < n; k0 += BLK) {
+        const int64_t tm = n - k0, bs = imin(BLK, tm);
+        double *akk = A + k0 * (lda + 1);
+        getf2(tm, bs, akk, lda, npiv, crit);
+        const int64_t rest = tm - bs;
+        if (rest > 0) {
+            double *lik = akk + bs, *ukj = akk + bs * lda, *aij = ukj + bs;
+            trsm_llnu(bs, rest, akk, lda, ukj, lda);
+            gemm_nn(rest, rest, bs, -1.0, lik, lda, ukj, lda, aij, lda);
+        }
+    }
+}
+
+/* --------------------------------------------- Schur update (gemmsp) --- */
+/* shift = 0: iterblok >= blok (L side), 1: iterblok > blok (U side) */
+static void gemmsp(const ora_symbol_t *s, int shift, int64_t c, int64_t b, const double *A, const double *B,
+                   double *C) {
+    const int64_t f = s->blok_fcblk[b];
+    const int64_t K = width(s, c), st = s->cblk_stride[c], stf = s->cblk_stride[f];
+    const int64_t N = brows(s, b);
+    const double *Bb = B + s->cblk_coef[c] + s->blok_roff[b];
+    int64_t fb = s->cblk_bloknum[f];
+    for (int64_t i = b + shift; i < s->cblk_bloknum[c + 1]; i++) {
+        while (!(s->blok_frow[fb] <= s->blok_frow[i] && s->blok_lrow[i] <= s->blok_lrow[fb])) fb++;
+        const int64_t M = brows(s, i);
+        const double *Ai = A + s->cblk_coef[c] + s->blok_roff[i];
+        double *Ci = C + s->cblk_coef[f] + s->blok_roff[fb] + (s->blok_frow[i] - s->blok_frow[fb]) +
+                     (s->blok_frow[b] - s->cblk_fcol[f]) * stf;
+        gemm_nt(M, N, K, -1.0, Ai, st, Bb, st, Ci, stf);
+    }
+}
+
+/* ------------------------------------------------ sequential numfact --- */
+int ora_potrf(const ora_symbol_t *s, double *L, double crit, int64_t *npiv) {
+    *npiv = 0;
+    for (int64_t c = 0; c < s->cblknbr; c++) {
+        const int64_t K = width(s, c), st = s->cblk_stride[c];
+        double *P = L + s->cblk_coef[c];
+        if (potrfsp(K, P, st, npiv, crit)) return -1;
+        if (st > K) trsm_rltn(st - K, K, P, st, P + K, st);
+        for (int64_t b = s->cblk_bloknum[c] + 1; b < s->cblk_bloknum[c + 1]; b++) gemmsp(s, 0, c, b, L, L, L);
+    }
+    return 0;
+}
+
+int ora_getrf(const ora_symbol_t *s, double *L, double *U, double crit, int64_t *npiv) {
+    *npiv = 0;
+    for (int64_t c = 0; c < s->cblknbr; c++) {
+        const int64_t K = width(s, c), st = s->cblk_stride[c];
+        double *P = L + s->cblk_coef[c], *Q = U + s->cblk_coef[c];
+        for (int64_t j = 0; j < K; j++) /* core_zgeadd(Trans): L += U^T */
+            for (int64_t i = 0; i < K; i++) P[i + j * st] += Q[j + i * st];
+        getrfsp(K, P, st, npiv, crit);
+        for (int64_t j = 0; j < K; j++) /* core_zgetmo: U_diag = L_diag^T */
+            for (int64_t i = 0; i < K; i++) Q[i + j * st] = P[j + i * st];
+        if (st > K) {
+            trsm_run(st - K, K, P, st, 0, P + K, st);
+            trsm_run(st - K, K, Q, st, 1, Q + K, st);
+        }
+        for (int64_t b = s->cblk_bloknum[c] + 1; b < s->cblk_bloknum[c + 1]; b++) {
+            gemmsp(s, 0, c, b, L, U, L);
+            if (b + 1 < s->cblk_bloknum[c + 1]) gemmsp(s, 1, c, b, U, L, U);
+        }
+    }
+    return 0;
+}
+
+/* ----------------------------------------------------------- solve ---- */
+int ora_solve(const ora_symbol_t *s, int lu, const double *L, const double *U, double *x, int64_t ldx, int nrhs) {
+    const double *Bk = lu ? U : L;
+    /* browtab for the backward sweep */
+    int64_t *cnt = calloc(s->cblknbr + 1, sizeof(int64_t));
+    int64_t *brow = malloc((s->bloknbr + 1) * sizeof(int64_t));
+    if (!cnt || !brow) { free(cnt); free(brow); return -1; }
+    for (int64_t c = 0; c < s->cblknbr; c++)
+        for (int64_t b = s->cblk_bloknum[c] + 1; b < s->cblk_bloknum[c + 1]; b++) cnt[s->blok_fcblk[b] + 1]++;
+    for (int64_t c = 0; c < s->cblknbr; c++) cnt[c + 1] += cnt[c];
+    int64_t *pos = malloc((s->cblknbr + 1) * sizeof(int64_t));
+    int64_t *owner = malloc((s->bloknbr + 1) * sizeof(int64_t));
+    memcpy(pos, cnt, (s->cblknbr + 1) * sizeof(int64_t));
+    for (int64_t c = 0; c < s->cblknbr; c++)
+        for (int64_t b = s->cblk_bloknum[c]; b < s->cblk_bloknum[c + 1]; b++) {
+            owner[b] = c;
+            if (b > s->cblk_bloknum[c]) brow[pos[s->blok_fcblk[b]]++] = b;
+        }
+    for (int r = 0; r < nrhs; r++) {
+        double *xr = x + r * ldx;
+        /* forward: solve_cblk_ztrsmsp_forward, Left Lower NoTrans (Unit for LU) */
+        for (int64_t c = 0; c < s->cblknbr; c++) {
+            const int64_t K = width(s, c), st = s->cblk_stride[c], fc = s->cblk_fcol[c];
+            const double *P = L + s->cblk_coef[c];
+            double *xc = xr + fc;
+            for (int64_t k = 0; k < K; k++) {
+                if (!lu) xc[k] /= P[k + k * st];
+                const double v = xc[k];
+                for (int64_t i = k + 1; i < K; i++) xc[i] -= P[i + k * st] * v;
+            }
+            for (int64_t b = s->cblk_bloknum[c] + 1; b < s->cblk_bloknum[c + 1]; b++) {
+                const int64_t nr = brows(s, b);
+                double *xf = xr + s->blok_frow[b];
+                const double *A = P + s->blok_roff[b];
+                for (int64_t k = 0; k < K; k++) {
+                    const double v = xc[k];
+                    for (int64_t i = 0; i < nr; i++) xf[i] -= A[i + k * st] * v;
+                }
+            }
+        }
+        /* backward: solve_cblk_ztrsmsp_backward, Left Lower Trans on L (or on U^T for LU) */
+        for (int64_t c = s->cblknbr - 1; c >= 0; c--) {
+            const int64_t K = width(s, c), st = s->cblk_stride[c], fc = s->cblk_fcol[c];
+            const double *P = Bk + s->cblk_coef[c];
+            double *xc = xr + fc;
+            for (int64_t k = K - 1; k >= 0; k--) {
+                double v = xc[k];
+                for (int64_t i = k + 1; i < K; i++) v -= P[i + k * st] * xc[i];
+                xc[k] = v / P[k + k * st];
+            }
+            for (int64_t e = cnt[c + 1] - 1; e >= cnt[c]; e--) {
+                const int64_t b = brow[e], sc = owner[b];
+                const int64_t Ks = width(s, sc), sts = s->cblk_stride[sc], nr = brows(s, b);
+                const double *A = Bk + s->cblk_coef[sc] + s->blok_roff[b];
+                double *xs = xr + s->cblk_fcol[sc];
+                const double *xb = xr + s->blok_frow[b];
+                for (int64_t k = 0; k < Ks; k++) {
+                    double v = 0.0;
+                    for (int64_t i = 0; i < nr; i++) v += A[i + k * sts] * xb[i];
+                    xs[k] -= v;
+                }
+            }
+        }
+    }
+    free(cnt); free(brow); free(pos); free(owner);
+    return 0;
+}
+
+/* ------------------------------------------------------------ spmv ---- */
+void ora_spmv(int64_t n, const int64_t *rowptr, const int64_t *cols, const double *vals, double alpha,
+              const double *x, double beta, double *y) {
+    for (int64_t i = 0; i < n; i++) {
+        double v = 0.0;
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) v += vals[k] * x[cols[k]];
+        y[i] = (beta == 0.0) ? alpha * v : alpha * v + beta * y[i];
+    }
+}
+
+/* --------------------------------------------------------- thflops ---- */
+double ora_thflops(const ora_symbol_t *s, int lu) {
+    double total = 0.0;
+    for (int64_t c = 0; c < s->cblknbr; c++) {
+        const double N = (double)width(s, c);
+        double M = 0.0;
+        for (int64_t b = s->cblk_bloknum[c] + 1; b < s->cblk_bloknum[c + 1]; b++) M += (double)brows(s, b);
+        if (lu) {
+            /* FLOPS_DGETRF(N,N) (m >= n branch) + 2 FLOPS_DTRSM(Right,M,N) + FLOPS_DGEMM(M,M,N) */
+            const double mul = 0.5 * N * (N * (N - N / 3.0 - 1.0) + N) + (2.0 / 3.0) * N;
+            const double add = 0.5 * N * (N * (N - N / 3.0) - N) + (1.0 / 6.0) * N;
+            total += mul + add;
+            if (M > 0) total += 2.0 * (M * N * (N + 1.0)) + 2.0 * M * M * N;
+        } else {
+            /* FLOPS_DPOTRF(N) + FLOPS_DTRSM(Right,M,N) + FLOPS_DSYRK(N,M) */
+            total += N * ((N / 6.0 + 0.5) * N + 1.0 / 3.0) + N * ((N / 6.0) * N - 1.0 / 6.0);
+            if (M > 0) total += M * N * (N + 1.0) + N * M * (M + 1.0);
+        }
+    }
+    return total;
+}
+
+/* ------------------------------------------------------ refinement ---- */
+static double dot(int64_t n, const double *a, const double *b) {
+    double v = 0.0;
+    for (int64_t i = 0; i < n; i++) v += a[i] * b[i];
+    return v;
+}
+
+int ora_cg(const ora_symbol_t *s, const double *L, int64_t n, const int64_t *rowptr, const int64_t *cols,
+           const double *vals, const double *b, double *x, int itermax, double eps, double *resid) {
+    double *r = malloc(4 * n * sizeof(double));
+    if (!r) return -1;
+    double *p = r + n, *z = r + 2 * n, *Ap = r + 3 * n;
+    const double normb = sqrt(dot(n, b, b)), normx = sqrt(dot(n, x, x));
+    memcpy(r, b, n * sizeof(double));
+    if (normx > 0.0) ora_spmv(n, rowptr, cols, vals, -1.0, x, 1.0, r);
+    double rb = sqrt(dot(n, r, r)) / normb;
+    memcpy(z, r, n * sizeof(double));
+    ora_solve(s, 0, L, NULL, z, n, 1);
+    memcpy(p, z, n * sizeof(double));
+    int it = 0;
+    while (rb > eps && it < itermax) {
+        it++;
+        ora_spmv(n, rowptr, cols, vals, 1.0, p, 0.0, Ap);
+        const double rz = dot(n, r, z), alpha = rz / dot(n, Ap, p);
+        for (int64_t i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; }
+        memcpy(z, r, n * sizeof(double));
+        ora_solve(s, 0, L, NULL, z, n, 1);
+        const double beta = dot(n, r, z) / rz;
+        for (int64_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+        rb = sqrt(dot(n, r, r)) / normb;
+    }
+    free(r);
+    if (resid) *resid = rb;
+    return it;
+}

@@ -1,0 +1,871 @@
+// PaStiX C API on top of the host analysis and the device engine.
+//
+// Task/subtask functions keep the reference contracts: the steps bitmask and
+// its prerequisite checks (common/pastixdata.h:25-33,
+// sopalin/pastix_task_sopalin.c:90-101,206-217,367-388), iparm/dparm inputs and
+// outputs, in-place right-hand sides, and PASTIX_ERR_* return codes.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "analysis.h"
+#include "engine.h"
+#include "plan.h"
+
+extern "C" {
+#include "pastix.h"
+#include "pastix_amd.h"
+}
+
+#define STEP_INIT (1 << 0)
+#define STEP_ORDERING (1 << 1)
+#define STEP_SYMBFACT (1 << 2)
+#define STEP_ANALYSE (1 << 3)
+#define STEP_CSC2BCSC (1 << 4)
+#define STEP_BCSC2CTAB (1 << 5)
+#define STEP_NUMFACT (1 << 6)
+#define STEP_SOLVE (1 << 7)
+#define STEP_REFINE (1 << 8)
+
+struct pastix_data_s {
+    pastix_int_t *iparm = nullptr;
+    double *dparm = nullptr;
+    pastix_int_t steps = 0;
+    pastix_int_t ext[PXA_IPARM_SIZE] = {0};
+    // analysis
+    pxa::Graph graph;
+    pxa::Order order;
+    pxa::Symbol symbol;
+    pxa::Layout layout;
+    pxa::Plan plan;
+    int64_t n = 0;
+    int mtxtype = SpmGeneral;
+    int flttype = SpmDouble;
+    const spmatrix_t *csc = nullptr;
+    // numerics
+    pxa::HostBcsc bcsc;
+    std::unique_ptr<pxa::Engine> engine;
+    std::vector<char> hostL, hostU;
+    double fact_times[4] = {0, 0, 0, 0};
+    double *dvec = nullptr;  // 2n FP64 device vectors for refinement / error checks
+    pxa::Engine *dvec_owner = nullptr;
+};
+
+static double *device_pair(pastix_data_t *d) {
+    if (d->dvec && d->dvec_owner == d->engine.get()) return d->dvec;
+    d->dvec = (double *)d->engine->alloc(2 * d->n * sizeof(double));
+    d->dvec_owner = d->engine.get();
+    return d->dvec;
+}
+
+static void err_print(const char *fn, const char *msg) { fprintf(stderr, "%s: %s\n", fn, msg); }
+
+static double wall() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// --------------------------------------------------------------------------
+// Parameters (reference common/api.c:363-538)
+// --------------------------------------------------------------------------
+extern "C" void pastixInitParam(pastix_int_t *iparm, double *dparm) {
+    memset(iparm, 0, IPARM_SIZE * sizeof(pastix_int_t));
+    memset(dparm, 0, DPARM_SIZE * sizeof(double));
+    iparm[IPARM_VERBOSE] = PastixVerboseNo;
+    iparm[IPARM_IO_STRATEGY] = PastixIONo;
+    iparm[IPARM_ORDERING] = PastixOrderScotch;
+    iparm[IPARM_ORDERING_DEFAULT] = 1;
+    iparm[IPARM_SCOTCH_SWITCH_LEVEL] = 120;
+    iparm[IPARM_SCOTCH_CMIN] = 0;
+    iparm[IPARM_SCOTCH_CMAX] = 100000;
+    iparm[IPARM_SCOTCH_FRAT] = 8;
+    iparm[IPARM_METIS_CTYPE] = -1;
+    iparm[IPARM_METIS_RTYPE] = -1;
+    iparm[IPARM_METIS_NSEPS] = 1;
+    iparm[IPARM_METIS_NITER] = 10;
+    iparm[IPARM_METIS_UFACTOR] = 200;
+    iparm[IPARM_METIS_COMPRESS] = 1;
+    iparm[IPARM_METIS_SEED] = 3452;
+    iparm[IPARM_AMALGAMATION_LVLCBLK] = 5;
+    iparm[IPARM_AMALGAMATION_LVLBLAS] = 5;
+    iparm[IPARM_REORDERING_SPLIT] = 0;
+    iparm[IPARM_REORDERING_STOP] = PASTIX_INT_MAX;
+    iparm[IPARM_SPLITTING_STRATEGY] = PastixSplitNot;
+    iparm[IPARM_SPLITTING_LEVELS_PROJECTIONS] = 2;
+    iparm[IPARM_SPLITTING_LEVELS_KWAY] = 2;
+    iparm[IPARM_SPLITTING_PROJECTIONS_DEPTH] = 3;
+    iparm[IPARM_SPLITTING_PROJECTIONS_DISTANCE] = 2;
+    iparm[IPARM_SPLITTING_PROJECTIONS_WIDTH] = 1;
+    iparm[IPARM_MIN_BLOCKSIZE] = 160;
+    iparm[IPARM_MAX_BLOCKSIZE] = 320;
+    iparm[IPARM_TASKS2D_LEVEL] = -1;
+    iparm[IPARM_TASKS2D_WIDTH] = iparm[IPARM_MIN_BLOCKSIZE];
+    iparm[IPARM_FACTORIZATION] = PastixFactLU;
+    iparm[IPARM_SCHUR_FACT_MODE] = PastixFactModeLocal;
+    iparm[IPARM_SCHUR_SOLV_MODE] = PastixSolvModeLocal;
+    iparm[IPARM_APPLYPERM_WS] = 1;
+    iparm[IPARM_REFINEMENT] = PastixRefineGMRES;
+    iparm[IPARM_ITERMAX] = 250;
+    iparm[IPARM_GMRES_IM] = 25;
+    iparm[IPARM_SCHEDULER] = PastixSchedStatic;
+    iparm[IPARM_THREAD_NBR] = -1;
+    iparm[IPARM_GPU_NBR] = 0;
+    iparm[IPARM_GPU_MEMORY_PERCENTAGE] = 95;
+    iparm[IPARM_GPU_MEMORY_BLOCK_SIZE] = 32 * 1024;
+    iparm[IPARM_COMPRESS_MIN_WIDTH] = 120;
+    iparm[IPARM_COMPRESS_MIN_HEIGHT] = 20;
+    iparm[IPARM_COMPRESS_WHEN] = PastixCompressNever;
+    iparm[IPARM_COMPRESS_METHOD] = PastixCompressMethodPQRCP;
+    iparm[IPARM_COMPRESS_ORTHO] = PastixCompressOrthoCGS;
+    iparm[IPARM_COMPRESS_PRESELECT] = -1;
+    iparm[IPARM_MODIFY_PARAMETER] = 1;
+    iparm[IPARM_START_TASK] = PastixTaskOrdering;
+    iparm[IPARM_END_TASK] = PastixTaskClean;
+    iparm[IPARM_FLOAT] = -1;
+    iparm[IPARM_MTX_TYPE] = -1;
+    iparm[IPARM_DOF_NBR] = 1;
+    dparm[DPARM_EPSILON_REFINEMENT] = -1.;
+    dparm[DPARM_RELATIVE_ERROR] = -1.;
+    dparm[DPARM_EPSILON_MAGN_CTRL] = 0.;
+    dparm[DPARM_A_NORM] = -1.;
+    dparm[DPARM_COMPRESS_TOLERANCE] = 0.01;
+    dparm[DPARM_COMPRESS_MIN_RATIO] = 1.;
+}
+
+extern "C" void pastixInitWithAffinity(pastix_data_t **pastix_data, PASTIX_Comm, pastix_int_t *iparm, double *dparm,
+                                       const int *) {
+    pastix_data_t *d = new pastix_data_s();
+    if (iparm[IPARM_MODIFY_PARAMETER] == 0) pastixInitParam(iparm, dparm);
+    d->iparm = iparm;
+    d->dparm = dparm;
+    d->ext[PXA_IPARM_FACTOR_PRECISION] = -1;
+    d->ext[PXA_IPARM_DEVICE] = 0;
+    d->ext[PXA_IPARM_LEVEL_ORDER] = 0;
+    d->steps = STEP_INIT;
+    *pastix_data = d;
+}
+
+extern "C" void pastixInit(pastix_data_t **pastix_data, PASTIX_Comm comm, pastix_int_t *iparm, double *dparm) {
+    pastixInitWithAffinity(pastix_data, comm, iparm, dparm, nullptr);
+}
+
+extern "C" void pastixFinalize(pastix_data_t **pastix_data) {
+    if (!pastix_data || !*pastix_data) return;
+    delete *pastix_data;
+    *pastix_data = nullptr;
+}
+
+// --------------------------------------------------------------------------
+// spm helpers
+// --------------------------------------------------------------------------
+struct Triplets {
+    std::vector<int64_t> i, j;
+    std::vector<double> v;
+};
+
+// Expand the user matrix to 0-based triplets of the full matrix (a symmetric
+// matrix stores one triangle, as the SPM library does).
+static int spm_triplets(const spmatrix_t *spm, Triplets &t, bool values) {
+    const int64_t n = spm->n, nnz = spm->nnz;
+    if (spm->dof != 1) return PASTIX_ERR_NOTIMPLEMENTED;
+    if (spm->flttype != SpmDouble && spm->flttype != SpmFloat && spm->flttype != SpmPattern)
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    const spm_int_t *cp = spm->colptr, *rp = spm->rowptr;
+    t.i.reserve(2 * nnz);
+    t.j.reserve(2 * nnz);
+    if (values) t.v.reserve(2 * nnz);
+    auto val = [&](int64_t k) -> double {
+        if (!values || spm->flttype == SpmPattern || !spm->values) return 1.0;
+        return spm->flttype == SpmDouble ? ((const double *)spm->values)[k] : ((const float *)spm->values)[k];
+    };
+    auto add = [&](int64_t i, int64_t j, double v) {
+        t.i.push_back(i);
+        t.j.push_back(j);
+        if (values) t.v.push_back(v);
+        if (spm->mtxtype != SpmGeneral && i != j) {
+            t.i.push_back(j);
+            t.j.push_back(i);
+            if (values) t.v.push_back(v);
+        }
+    };
+    if (spm->fmttype == SpmCSC) {
+        const int64_t base = cp[0];
+        for (int64_t j = 0; j < n; j++)
+            for (int64_t k = cp[j] - base; k < cp[j + 1] - base; k++) add(rp[k] - base, j, val(k));
+    } else if (spm->fmttype == SpmCSR) {
+        const int64_t base = rp[0];
+        for (int64_t i = 0; i < n; i++)
+            for (int64_t k = rp[i] - base; k < rp[i + 1] - base; k++) add(i, cp[k] - base, val(k));
+    } else {
+        int64_t base = INT64_MAX;
+        for (int64_t k = 0; k < nnz; k++) base = std::min<int64_t>(base, std::min<int64_t>(cp[k], rp[k]));
+        if (nnz == 0) base = 0;
+        for (int64_t k = 0; k < nnz; k++) add(rp[k] - base, cp[k] - base, val(k));
+    }
+    for (size_t k = 0; k < t.i.size(); k++)
+        if (t.i[k] < 0 || t.i[k] >= n || t.j[k] < 0 || t.j[k] >= n) return PASTIX_ERR_BADPARAMETER;
+    return PASTIX_SUCCESS;
+}
+
+// CSC (sorted, duplicates summed) of the n x n matrix given by triplets,
+// optionally permuted: B(perm[i], perm[j]) = A(i, j).
+static void triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
+                            std::vector<int32_t> &rows, std::vector<double> &vals, bool transpose) {
+    const size_t m = t.i.size();
+    colptr.assign(n + 1, 0);
+    std::vector<int64_t> I(m), J(m);
+    for (size_t k = 0; k < m; k++) {
+        int64_t i = perm ? perm[t.i[k]] : t.i[k];
+        int64_t j = perm ? perm[t.j[k]] : t.j[k];
+        if (transpose) std::swap(i, j);
+        I[k] = i;
+        J[k] = j;
+        colptr[j + 1]++;
+    }
+    for (int64_t j = 0; j < n; j++) colptr[j + 1] += colptr[j];
+    std::vector<int64_t> pos(colptr.begin(), colptr.end() - 1);
+    std::vector<int64_t> tr(m);
+    std::vector<double> tv(m);
+    for (size_t k = 0; k < m; k++) {
+        int64_t p = pos[J[k]]++;
+        tr[p] = I[k];
+        tv[p] = t.v.empty() ? 1.0 : t.v[k];
+    }
+    // sort each column by row and sum duplicates
+    rows.clear();
+    vals.clear();
+    rows.reserve(m);
+    vals.reserve(m);
+    std::vector<int64_t> newptr(n + 1, 0);
+    std::vector<std::pair<int64_t, double>> col;
+    for (int64_t j = 0; j < n; j++) {
+        col.clear();
+        for (int64_t p = colptr[j]; p < colptr[j + 1]; p++) col.emplace_back(tr[p], tv[p]);
+        std::sort(col.begin(), col.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        for (size_t q = 0; q < col.size(); q++) {
+            if (q > 0 && col[q].first == col[q - 1].first) vals.back() += col[q].second;
+            else {
+                rows.push_back((int32_t)col[q].first);
+                vals.push_back(col[q].second);
+            }
+        }
+        newptr[j + 1] = (int64_t)rows.size();
+    }
+    colptr = newptr;
+}
+
+// --------------------------------------------------------------------------
+// Analysis subtasks
+// --------------------------------------------------------------------------
+extern "C" int pastix_subtask_order(pastix_data_t *d, const spmatrix_t *spm, pastix_order_t *myorder) {
+    if (!d) { err_print("pastix_subtask_order", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!spm) { err_print("pastix_subtask_order", "wrong spm parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_INIT)) { err_print("pastix_subtask_order", "pastixInit() has to be called before"); return PASTIX_ERR_BADPARAMETER; }
+    Triplets t;
+    int rc = spm_triplets(spm, t, false);
+    if (rc) return rc;
+    const int64_t n = spm->n;
+    d->n = n;
+    d->mtxtype = spm->mtxtype;
+    d->flttype = spm->flttype;
+    {
+        std::vector<int64_t> cp;
+        std::vector<int32_t> rw;
+        std::vector<double> vv;
+        Triplets tt = t;
+        tt.v.clear();
+        triplets_to_csc(n, tt, nullptr, cp, rw, vv, false);
+        std::vector<int64_t> rows64(rw.begin(), rw.end());
+        d->graph = pxa::graph_from_csc(n, cp.data(), rows64.data(), 0);
+    }
+    pxa::Order o;
+    o.n = n;
+    const pastix_int_t ord = d->iparm[IPARM_ORDERING];
+    if (ord == PastixOrderPersonal) {
+        o.permtab.resize(n);
+        o.peritab.resize(n);
+        const int64_t base = (myorder && myorder->baseval) ? myorder->baseval : 0;
+        if (myorder && myorder->permtab) {
+            for (int64_t i = 0; i < n; i++) o.permtab[i] = myorder->permtab[i] - base;
+            for (int64_t i = 0; i < n; i++) o.peritab[o.permtab[i]] = i;
+        } else if (myorder && myorder->peritab) {
+            for (int64_t i = 0; i < n; i++) o.peritab[i] = myorder->peritab[i] - base;
+            for (int64_t i = 0; i < n; i++) o.permtab[o.peritab[i]] = i;
+        } else {
+            for (int64_t i = 0; i < n; i++) o.permtab[i] = o.peritab[i] = i;
+        }
+        if (myorder && myorder->rangtab && myorder->treetab && myorder->cblknbr > 0) {
+            o.cblknbr = myorder->cblknbr;
+            o.rangtab.resize(o.cblknbr + 1);
+            o.treetab.resize(o.cblknbr);
+            for (int64_t c = 0; c <= o.cblknbr; c++) o.rangtab[c] = myorder->rangtab[c] - base;
+            for (int64_t c = 0; c < o.cblknbr; c++)
+                o.treetab[c] = myorder->treetab[c] < 0 ? -1 : myorder->treetab[c] - base;
+        } else {
+            pxa::order_find_supernodes(d->graph, o);
+        }
+    } else {
+        // Scotch / METIS are not part of this build: graph nested dissection.
+        if (d->iparm[IPARM_VERBOSE] > PastixVerboseNo)
+            fprintf(stderr, "pastix_subtask_order: ordering library not available, using level-structure nested dissection\n");
+        o = pxa::order_nested_dissection(d->graph);
+    }
+    std::string err;
+    if (!pxa::check_order(o, err)) { err_print("pastix_subtask_order", err.c_str()); return PASTIX_ERR_BADPARAMETER; }
+    if (d->ext[PXA_IPARM_LEVEL_ORDER]) pxa::order_apply_level_order(o);
+    d->order = std::move(o);
+    d->steps |= STEP_ORDERING;
+    d->steps &= ~(STEP_SYMBFACT | STEP_ANALYSE | STEP_CSC2BCSC | STEP_BCSC2CTAB | STEP_NUMFACT | STEP_SOLVE | STEP_REFINE);
+    d->engine.reset();
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_subtask_symbfact(pastix_data_t *d) {
+    if (!d) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_ORDERING)) { err_print("pastix_subtask_symbfact", "pastix_subtask_order() has to be called before"); return PASTIX_ERR_BADPARAMETER; }
+    d->symbol = pxa::symbol_fax(d->graph, d->order);
+    std::string err;
+    if (!pxa::check_symbol(d->symbol, err)) { err_print("pastix_subtask_symbfact", err.c_str()); return PASTIX_ERR_INTERNAL; }
+    d->iparm[IPARM_NNZEROS] = pxa::symbol_nnz(d->symbol);
+    d->steps |= STEP_SYMBFACT;
+    return PASTIX_SUCCESS;
+}
+
+// Reference symbol/pastix_subtask_reordering.c reorders rows inside
+// supernodes; this engine keeps the ordering as computed (documented divergence).
+extern "C" int pastix_subtask_reordering(pastix_data_t *d) {
+    if (!d) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_SYMBFACT)) { err_print("pastix_subtask_reordering", "pastix_subtask_symbfact() has to be called before"); return PASTIX_ERR_BADPARAMETER; }
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_subtask_blend(pastix_data_t *d) {
+    if (!d) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_SYMBFACT)) { err_print("pastix_subtask_blend", "pastix_subtask_symbfact() has to be called before"); return PASTIX_ERR_BADPARAMETER; }
+    double t0 = wall();
+    pastix_int_t bmin = d->iparm[IPARM_MIN_BLOCKSIZE], bmax = d->iparm[IPARM_MAX_BLOCKSIZE];
+    if (bmax < 1 || bmin < 1 || bmin > bmax) { err_print("pastix_subtask_blend", "invalid block sizes"); return PASTIX_ERR_BADPARAMETER; }
+    d->symbol = pxa::symbol_split(d->symbol, bmin, bmax);
+    std::string err;
+    if (!pxa::check_symbol(d->symbol, err)) { err_print("pastix_subtask_blend", err.c_str()); return PASTIX_ERR_INTERNAL; }
+    d->layout = pxa::build_layout(d->symbol);
+    d->plan = pxa::build_plan(d->symbol, d->layout);
+    int facto = (int)d->iparm[IPARM_FACTORIZATION];
+    int ft = (facto == PastixFactGETRF) ? 2 : 0;
+    d->dparm[DPARM_FACT_THFLOPS] = pxa::symbol_thflops(d->symbol, ft);
+    if (facto == PastixFactLU) d->iparm[IPARM_NNZEROS] *= 2;
+    d->dparm[DPARM_SOLV_FLOPS] = (double)d->iparm[IPARM_NNZEROS];
+    d->iparm[IPARM_NNZEROS_BLOCK_LOCAL] = d->layout.coefnbr;
+    d->dparm[DPARM_ANALYZE_TIME] = wall() - t0;
+    d->steps |= STEP_ANALYSE;
+    d->engine.reset();
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_task_analyze(pastix_data_t *d, spmatrix_t *spm) {
+    if (!d) { err_print("pastix_task_analyze", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!spm) { err_print("pastix_task_analyze", "wrong spm parameter"); return PASTIX_ERR_BADPARAMETER; }
+    double t0 = wall();
+    int rc = pastix_subtask_order(d, spm, nullptr);
+    if (rc) return rc;
+    if ((rc = pastix_subtask_symbfact(d))) return rc;
+    if ((rc = pastix_subtask_reordering(d))) return rc;
+    if ((rc = pastix_subtask_blend(d))) return rc;
+    d->dparm[DPARM_ANALYZE_TIME] = wall() - t0;
+    return PASTIX_SUCCESS;
+}
+
+// --------------------------------------------------------------------------
+// Numerical factorisation subtasks
+// --------------------------------------------------------------------------
+extern "C" int pastix_subtask_spm2bcsc(pastix_data_t *d, spmatrix_t *spm) {
+    if (!d) { err_print("pastix_subtask_spm2bcsc", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!spm) { err_print("pastix_subtask_spm2bcsc", "wrong spm parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_ANALYSE)) {
+        err_print("pastix_subtask_spm2bcsc", "All steps from pastix_task_init() to pastix_task_blend() have to be called before calling this function");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if (spm->n != d->n) { err_print("pastix_subtask_spm2bcsc", "matrix size differs from the analysed one"); return PASTIX_ERR_BADPARAMETER; }
+    Triplets t;
+    int rc = spm_triplets(spm, t, true);
+    if (rc) return rc;
+    pxa::HostBcsc &bc = d->bcsc;
+    bc.n = d->n;
+    bc.mtxtype = spm->mtxtype;
+    triplets_to_csc(d->n, t, d->order.permtab.data(), bc.colptr, bc.rows, bc.vals, false);
+    triplets_to_csc(d->n, t, d->order.permtab.data(), bc.rowptr, bc.cols, bc.tvals, true);
+    double f2 = 0;
+    for (double v : bc.vals) f2 += v * v;
+    bc.normF = std::sqrt(f2);
+    double ninf = 0;
+    for (int64_t i = 0; i < d->n; i++) {
+        double s = 0;
+        for (int64_t k = bc.rowptr[i]; k < bc.rowptr[i + 1]; k++) s += std::fabs(bc.tvals[k]);
+        ninf = std::max(ninf, s);
+    }
+    bc.normInf = ninf;
+    d->dparm[DPARM_A_NORM] = bc.normF;
+    d->flttype = spm->flttype == SpmPattern ? SpmDouble : spm->flttype;
+    d->csc = spm;
+    d->steps |= STEP_CSC2BCSC;
+    return PASTIX_SUCCESS;
+}
+
+template <typename T> static void fill_coeftab(pastix_data_t *d, int facto) {
+    const pxa::Symbol &s = d->symbol;
+    const pxa::Layout &L = d->layout;
+    const pxa::HostBcsc &bc = d->bcsc;
+    d->hostL.assign(L.coefnbr * sizeof(T), 0);
+    T *Lc = (T *)d->hostL.data();
+    T *Uc = nullptr;
+    if (facto == 2) {
+        d->hostU.assign(L.coefnbr * sizeof(T), 0);
+        Uc = (T *)d->hostU.data();
+    }
+    // reference kernels/cpucblk_zinit.c:229-309 (cpucblk_zfillin_fr)
+    for (int64_t c = 0; c < s.cblknbr(); c++) {
+        const int64_t fc = s.cblk_fcol[c], lc = s.cblk_fcol[c + 1];
+        const int64_t st = L.cblk_stride[c];
+        T *P = Lc + L.cblk_coef[c];
+        for (int64_t j = fc; j < lc; j++) {
+            int64_t b = s.cblk_bloknum[c];
+            for (int64_t k = bc.colptr[j]; k < bc.colptr[j + 1]; k++) {
+                const int64_t i = bc.rows[k];
+                if (i < j) continue;
+                while (b < s.cblk_bloknum[c + 1] && s.blok_lrow[b] < i) b++;
+                if (b >= s.cblk_bloknum[c + 1] || s.blok_frow[b] > i) continue;  // dropped (outside the symbol)
+                P[L.blok_roff[b] + (i - s.blok_frow[b]) + (j - fc) * st] = (T)bc.vals[k];
+            }
+            if (Uc) {
+                T *Q = Uc + L.cblk_coef[c];
+                int64_t bu = s.cblk_bloknum[c];
+                for (int64_t k = bc.rowptr[j]; k < bc.rowptr[j + 1]; k++) {
+                    const int64_t i = bc.cols[k];
+                    if (i <= j) continue;
+                    while (bu < s.cblk_bloknum[c + 1] && s.blok_lrow[bu] < i) bu++;
+                    if (bu >= s.cblk_bloknum[c + 1] || s.blok_frow[bu] > i) continue;
+                    Q[L.blok_roff[bu] + (i - s.blok_frow[bu]) + (j - fc) * st] = (T)bc.tvals[k];
+                }
+            }
+        }
+    }
+}
+
+static int factor_prec(pastix_data_t *d) {
+    pastix_int_t p = d->ext[PXA_IPARM_FACTOR_PRECISION];
+    if (p == SpmFloat || p == SpmDouble) return (int)p;
+    return d->flttype == SpmFloat ? SpmFloat : SpmDouble;
+}
+
+extern "C" int pastix_subtask_bcsc2ctab(pastix_data_t *d) {
+    if (!d) { err_print("pastix_subtask_bcsc2ctab", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_CSC2BCSC)) {
+        err_print("pastix_subtask_bcsc2ctab", "All steps from pastix_task_init() to pastix_stask_spm2bcsc() have to be called before calling this function");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    const int facto = (d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF) ? 2 : 0;
+    if (d->iparm[IPARM_FACTORIZATION] == PastixFactSYTRF || d->iparm[IPARM_FACTORIZATION] == PastixFactHETRF) {
+        err_print("pastix_subtask_bcsc2ctab", "LDL^t factorizations are not provided by this engine");
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    }
+    const int prec = factor_prec(d);
+    if (!d->engine || !d->engine->ready() || d->engine->prec() != prec || d->engine->factotype() != facto) {
+        d->engine.reset(new pxa::Engine());
+        if (d->engine->init((int)d->ext[PXA_IPARM_DEVICE], d->symbol, d->layout, d->plan, facto, prec)) {
+            d->engine.reset();
+            return PASTIX_ERR_INTERNAL;
+        }
+    }
+    if (prec == SpmFloat) fill_coeftab<float>(d, facto);
+    else fill_coeftab<double>(d, facto);
+    if (d->engine->upload_coef(d->hostL.data(), facto == 2 ? d->hostU.data() : nullptr)) return PASTIX_ERR_INTERNAL;
+    if (d->engine->upload_matrix(d->bcsc)) return PASTIX_ERR_INTERNAL;
+    std::vector<char>().swap(d->hostL);
+    std::vector<char>().swap(d->hostU);
+    d->steps |= STEP_BCSC2CTAB;
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_subtask_sopalin(pastix_data_t *d) {
+    if (!d) { err_print("pastix_subtask_sopalin", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_ANALYSE)) { err_print("pastix_subtask_sopalin", "All steps from pastix_task_init() to pastix_task_analyze() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_BCSC2CTAB)) { err_print("pastix_subtask_sopalin", "All steps from pastix_task_init() to pastix_task_bcsc2ctab() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
+    // static pivoting threshold, reference sopalin/pastix_task_sopalin.c:401-425
+    double thr;
+    const double magn = d->dparm[DPARM_EPSILON_MAGN_CTRL];
+    const int prec = d->engine->prec();
+    if (magn < 0.) thr = -magn;
+    else if (magn == 0.) {
+        double eps = (prec == SpmFloat) ? std::ldexp(1.0, -24) : std::ldexp(1.0, -53);
+        thr = std::sqrt(eps) * d->dparm[DPARM_A_NORM];
+    } else thr = magn * d->dparm[DPARM_A_NORM];
+    int64_t npiv = 0;
+    int neg = 0;
+    double secs = 0;
+    if (d->engine->factorize(thr, &npiv, &neg, &secs)) return PASTIX_ERR_INTERNAL;
+    d->dparm[DPARM_FACT_TIME] = secs;
+    d->dparm[DPARM_FACT_FLOPS] = (secs > 0) ? d->dparm[DPARM_FACT_THFLOPS] / secs / (1 << 30) : 0.;
+    d->iparm[IPARM_STATIC_PIVOTING] = npiv;
+    d->fact_times[0] = secs;
+    if (neg && d->iparm[IPARM_FACTORIZATION] != PastixFactGETRF) {
+        // The reference aborts on a negative pivot (kernels/core_zpotrfsp.c:84-89);
+        // this engine reports the failure instead.
+        err_print("pastix_subtask_sopalin", "Negative diagonal term");
+        return PASTIX_ERR_INTERNAL;
+    }
+    d->steps |= STEP_NUMFACT;
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_task_numfact(pastix_data_t *d, spmatrix_t *spm) {
+    if (!d) { err_print("pastix_task_sopalin", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!spm) { err_print("pastix_task_sopalin", "wrong spm parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_ANALYSE)) {
+        err_print("pastix_task_sopalin", "All steps from pastix_task_init() to pastix_task_blend() have to be called before calling this function");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    d->steps &= ~(STEP_CSC2BCSC | STEP_BCSC2CTAB | STEP_NUMFACT | STEP_SOLVE | STEP_REFINE);
+    int rc;
+    if ((rc = pastix_subtask_spm2bcsc(d, spm))) return rc;
+    if ((rc = pastix_subtask_bcsc2ctab(d))) return rc;
+    if ((rc = pastix_subtask_sopalin(d))) return rc;
+    return PASTIX_SUCCESS;
+}
+
+// --------------------------------------------------------------------------
+// Solve
+// --------------------------------------------------------------------------
+extern "C" int pastix_subtask_applyorder(pastix_data_t *d, pastix_coeftype_t flttype, pastix_dir_t dir, pastix_int_t m,
+                                         pastix_int_t n, void *b, pastix_int_t ldb) {
+    if (!d || !b || m != d->n || ldb < m) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_ORDERING)) return PASTIX_ERR_BADPARAMETER;
+    const size_t es = (flttype == SpmFloat) ? 4 : 8;
+    std::vector<char> tmp((size_t)m * es);
+    for (pastix_int_t r = 0; r < n; r++) {
+        char *col = (char *)b + (size_t)r * ldb * es;
+        for (pastix_int_t i = 0; i < m; i++) {
+            // forward: xp[perm[i]] = b[i]; backward: b[i] = xp[perm[i]]
+            if (dir == PastixDirForward) memcpy(&tmp[(size_t)d->order.permtab[i] * es], col + (size_t)i * es, es);
+            else memcpy(&tmp[(size_t)i * es], col + (size_t)d->order.permtab[i] * es, es);
+        }
+        memcpy(col, tmp.data(), (size_t)m * es);
+    }
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_subtask_solve(pastix_data_t *d, pastix_int_t nrhs, void *b, pastix_int_t ldb) {
+    if (!d || !b || nrhs < 0 || ldb < d->n) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_NUMFACT)) { err_print("pastix_subtask_solve", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
+    // b is already in the permuted ordering: identity permutation for the engine
+    std::vector<int64_t> id(d->n);
+    for (int64_t i = 0; i < d->n; i++) id[i] = i;
+    double secs = 0;
+    if (d->engine->solve_host(id.data(), d->flttype, b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
+    d->dparm[DPARM_SOLV_TIME] = secs;
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_task_solve(pastix_data_t *d, pastix_int_t nrhs, void *b, pastix_int_t ldb) {
+    if (!d) { err_print("pastix_task_solve", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_NUMFACT)) { err_print("pastix_task_solve", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
+    if (!b || nrhs < 0 || ldb < d->n) return PASTIX_ERR_BADPARAMETER;
+    if (nrhs == 0) return PASTIX_SUCCESS;
+    double secs = 0;
+    if (d->engine->solve_host(d->order.permtab.data(), d->flttype, b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
+    d->dparm[DPARM_SOLV_TIME] = secs;
+    d->steps |= STEP_SOLVE;
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_subtask_trsm(pastix_data_t *, pastix_coeftype_t, pastix_side_t, pastix_uplo_t, pastix_trans_t,
+                                   pastix_diag_t, pastix_int_t, void *, pastix_int_t) {
+    err_print("pastix_subtask_trsm", "single sweeps are not exposed by this engine; use pastix_subtask_solve");
+    return PASTIX_ERR_NOTIMPLEMENTED;
+}
+
+extern "C" int pastix_subtask_diag(pastix_data_t *, pastix_coeftype_t, pastix_int_t, void *, pastix_int_t) {
+    err_print("pastix_subtask_diag", "LDL^t diagonal solves are not provided by this engine");
+    return PASTIX_ERR_NOTIMPLEMENTED;
+}
+
+// --------------------------------------------------------------------------
+// Refinement (reference refinement/pastix_task_refine.c:115-288)
+// --------------------------------------------------------------------------
+static int refine_common(pastix_data_t *d, pastix_int_t nrhs, const void *b, pastix_int_t ldb, void *x,
+                         pastix_int_t ldx, bool permute) {
+    pxa::Engine &E = *d->engine;
+    const int64_t n = d->n;
+    double eps = d->dparm[DPARM_EPSILON_REFINEMENT];
+    if (eps < 0.) eps = (d->flttype == SpmFloat) ? 1e-6 : 1e-12;
+    const int itermax = (int)d->iparm[IPARM_ITERMAX];
+    const int im = (int)std::max<pastix_int_t>(1, d->iparm[IPARM_GMRES_IM]);
+    double *db = device_pair(d);
+    if (!db) return PASTIX_ERR_OUTOFMEMORY;
+    double *dx = db + n;
+    std::vector<double> hb(n), hx(n);
+    const bool isf = d->flttype == SpmFloat;
+    int total = 0;
+    double worst = 0;
+    for (pastix_int_t r = 0; r < nrhs; r++) {
+        for (int64_t i = 0; i < n; i++) {
+            int64_t p = permute ? d->order.permtab[i] : i;
+            hb[p] = isf ? ((const float *)b)[i + r * ldb] : ((const double *)b)[i + r * ldb];
+            hx[p] = isf ? ((const float *)x)[i + r * ldx] : ((const double *)x)[i + r * ldx];
+        }
+        hipMemcpy(db, hb.data(), n * sizeof(double), hipMemcpyHostToDevice);
+        hipMemcpy(dx, hx.data(), n * sizeof(double), hipMemcpyHostToDevice);
+        int it = 0;
+        double res = 0;
+        int rc;
+        pastix_int_t method = d->iparm[IPARM_REFINEMENT];
+        if (method == PastixRefineCG) rc = E.refine_cg(dx, db, itermax, eps, &it, &res);
+        else rc = E.refine_gmres(dx, db, itermax, im, eps, &it, &res);
+        if (rc) return PASTIX_ERR_INTERNAL;
+        hipMemcpy(hx.data(), dx, n * sizeof(double), hipMemcpyDeviceToHost);
+        for (int64_t i = 0; i < n; i++) {
+            int64_t p = permute ? d->order.permtab[i] : i;
+            if (isf) ((float *)x)[i + r * ldx] = (float)hx[p];
+            else ((double *)x)[i + r * ldx] = hx[p];
+        }
+        total = std::max(total, it);
+        worst = std::max(worst, res);
+    }
+    d->iparm[IPARM_NBITER] = total;
+    d->dparm[DPARM_RELATIVE_ERROR] = worst;
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_subtask_refine(pastix_data_t *d, pastix_int_t n, pastix_int_t nrhs, const void *b,
+                                     pastix_int_t ldb, void *x, pastix_int_t ldx) {
+    if (!d || !b || !x || n != d->n) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_NUMFACT)) return PASTIX_ERR_BADPARAMETER;
+    double t0 = wall();
+    int rc = refine_common(d, nrhs, b, ldb, x, ldx, false);
+    d->dparm[DPARM_REFINE_TIME] = wall() - t0;
+    return rc;
+}
+
+extern "C" int pastix_task_refine(pastix_data_t *d, pastix_int_t n, pastix_int_t nrhs, void *b, pastix_int_t ldb,
+                                  void *x, pastix_int_t ldx) {
+    if (!d) { err_print("pastix_task_refine", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_NUMFACT)) { err_print("pastix_task_refine", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
+    if (!b || !x || n != d->n || ldb < n || ldx < n) return PASTIX_ERR_BADPARAMETER;
+    double t0 = wall();
+    int rc = refine_common(d, nrhs, b, ldb, x, ldx, true);
+    d->dparm[DPARM_REFINE_TIME] = wall() - t0;
+    if (rc == PASTIX_SUCCESS) d->steps |= STEP_REFINE;
+    return rc;
+}
+
+// --------------------------------------------------------------------------
+// Legacy and auxiliary entry points
+// --------------------------------------------------------------------------
+extern "C" int pastix(pastix_data_t **, PASTIX_Comm, pastix_int_t, pastix_int_t *, pastix_int_t *, void *,
+                      pastix_int_t *, pastix_int_t *, void *, pastix_int_t, pastix_int_t *, double *) {
+    err_print("pastix", "the former single-call interface is not provided; use the task interface");
+    return PASTIX_ERR_NOTIMPLEMENTED;
+}
+extern "C" void pastixSetSchurUnknownList(pastix_data_t *, pastix_int_t, const pastix_int_t *) {
+    err_print("pastixSetSchurUnknownList", "Schur complement is not provided by this engine");
+}
+extern "C" int pastixGetSchur(const pastix_data_t *, void *, pastix_int_t) { return PASTIX_ERR_NOTIMPLEMENTED; }
+extern "C" void pastixExpand(const pastix_data_t *, spmatrix_t *) {}
+extern "C" int pastixGetDiag(const pastix_data_t *, void *, pastix_int_t) { return PASTIX_ERR_NOTIMPLEMENTED; }
+extern "C" void pastixGetOptions(int, char **, pastix_int_t *, double *, int *check, spm_driver_t *driver,
+                                 char **filename) {
+    if (check) *check = 1;
+    if (driver) *driver = SpmDriverLaplacian;
+    if (filename) *filename = nullptr;
+}
+
+// --------------------------------------------------------------------------
+// Ordering structure helpers (reference order/order.c, order/order_grids.c)
+// --------------------------------------------------------------------------
+extern "C" int pastixOrderAlloc(pastix_order_t *const o, pastix_int_t vertnbr, pastix_int_t cblknbr) {
+    if (!o) return PASTIX_ERR_BADPARAMETER;
+    memset(o, 0, sizeof(*o));
+    o->vertnbr = vertnbr;
+    o->cblknbr = cblknbr;
+    if (vertnbr > 0) {
+        o->permtab = (pastix_int_t *)malloc(vertnbr * sizeof(pastix_int_t));
+        o->peritab = (pastix_int_t *)malloc(vertnbr * sizeof(pastix_int_t));
+    }
+    if (cblknbr > 0) {
+        o->rangtab = (pastix_int_t *)calloc(cblknbr + 1, sizeof(pastix_int_t));
+        o->treetab = (pastix_int_t *)calloc(cblknbr, sizeof(pastix_int_t));
+    }
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastixOrderInit(pastix_order_t *const o, pastix_int_t baseval, pastix_int_t vertnbr,
+                               pastix_int_t cblknbr, pastix_int_t *const permtab, pastix_int_t *const peritab,
+                               pastix_int_t *const rangtab, pastix_int_t *const treetab) {
+    if (!o) return PASTIX_ERR_BADPARAMETER;
+    memset(o, 0, sizeof(*o));
+    o->baseval = baseval;
+    o->vertnbr = vertnbr;
+    o->cblknbr = cblknbr;
+    o->permtab = permtab;
+    o->peritab = peritab;
+    o->rangtab = rangtab;
+    o->treetab = treetab;
+    return PASTIX_SUCCESS;
+}
+extern "C" void pastixOrderExit(pastix_order_t *const o) {
+    if (!o) return;
+    free(o->permtab);
+    free(o->peritab);
+    free(o->rangtab);
+    free(o->treetab);
+    free(o->selevtx);
+    free(o->sndetab);
+    memset(o, 0, sizeof(*o));
+}
+extern "C" void pastixOrderBase(pastix_order_t *const o, pastix_int_t baseval) {
+    if (!o || o->baseval == baseval) return;
+    pastix_int_t d = baseval - o->baseval;
+    for (pastix_int_t i = 0; i < o->vertnbr; i++) {
+        if (o->permtab) o->permtab[i] += d;
+        if (o->peritab) o->peritab[i] += d;
+    }
+    if (o->rangtab)
+        for (pastix_int_t i = 0; i <= o->cblknbr; i++) o->rangtab[i] += d;
+    if (o->treetab)
+        for (pastix_int_t i = 0; i < o->cblknbr; i++)
+            if (o->treetab[i] >= 0) o->treetab[i] += d;
+    o->baseval = baseval;
+}
+extern "C" int pastixOrderCheck(const pastix_order_t *const o) {
+    if (!o) return PASTIX_ERR_BADPARAMETER;
+    pxa::Order t;
+    t.n = o->vertnbr;
+    t.cblknbr = o->cblknbr;
+    t.permtab.assign(o->permtab, o->permtab + o->vertnbr);
+    t.peritab.assign(o->peritab, o->peritab + o->vertnbr);
+    t.rangtab.assign(o->rangtab, o->rangtab + o->cblknbr + 1);
+    t.treetab.assign(o->treetab, o->treetab + o->cblknbr);
+    for (auto &v : t.permtab) v -= o->baseval;
+    for (auto &v : t.peritab) v -= o->baseval;
+    for (auto &v : t.rangtab) v -= o->baseval;
+    for (auto &v : t.treetab)
+        if (v >= 0) v -= o->baseval;
+    std::string err;
+    return pxa::check_order(t, err) ? PASTIX_SUCCESS : PASTIX_ERR_BADPARAMETER;
+}
+extern "C" int pastixOrderGrid(pastix_order_t **myorder, pastix_int_t nx, pastix_int_t ny, pastix_int_t nz) {
+    if (!myorder || !*myorder || nx < 1 || ny < 1 || nz < 1) return PASTIX_ERR_BADPARAMETER;
+    pxa::Order o = pxa::order_grid(nx, ny, nz);
+    pastix_order_t *r = *myorder;
+    pastixOrderAlloc(r, o.n, o.cblknbr);
+    for (int64_t i = 0; i < o.n; i++) {
+        r->permtab[i] = o.permtab[i];
+        r->peritab[i] = o.peritab[i];
+    }
+    for (int64_t c = 0; c <= o.cblknbr; c++) r->rangtab[c] = o.rangtab[c];
+    for (int64_t c = 0; c < o.cblknbr; c++) r->treetab[c] = o.treetab[c];
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastixOrderApplyLevelOrder(pastix_order_t *o, pastix_int_t, pastix_int_t) {
+    if (!o || !o->rangtab || !o->treetab) return PASTIX_ERR_BADPARAMETER;
+    pxa::Order t;
+    t.n = o->vertnbr;
+    t.cblknbr = o->cblknbr;
+    t.permtab.assign(o->permtab, o->permtab + o->vertnbr);
+    t.peritab.assign(o->peritab, o->peritab + o->vertnbr);
+    t.rangtab.assign(o->rangtab, o->rangtab + o->cblknbr + 1);
+    t.treetab.assign(o->treetab, o->treetab + o->cblknbr);
+    pxa::order_apply_level_order(t);
+    std::copy(t.permtab.begin(), t.permtab.end(), o->permtab);
+    std::copy(t.peritab.begin(), t.peritab.end(), o->peritab);
+    std::copy(t.rangtab.begin(), t.rangtab.end(), o->rangtab);
+    std::copy(t.treetab.begin(), t.treetab.end(), o->treetab);
+    return PASTIX_SUCCESS;
+}
+
+// --------------------------------------------------------------------------
+// Engine extensions (pastix_amd.h)
+// --------------------------------------------------------------------------
+extern "C" int pastix_amd_set_iparm(pastix_data_t *d, int key, pastix_int_t value) {
+    if (!d || key < 0 || key >= PXA_IPARM_SIZE) return PASTIX_ERR_BADPARAMETER;
+    if (key == PXA_IPARM_FACTOR_PRECISION && value != -1 && value != SpmFloat && value != SpmDouble)
+        return PASTIX_ERR_BADPARAMETER;
+    d->ext[key] = value;
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_get_iparm(const pastix_data_t *d, int key, pastix_int_t *value) {
+    if (!d || !value || key < 0 || key >= PXA_IPARM_SIZE) return PASTIX_ERR_BADPARAMETER;
+    *value = d->ext[key];
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_symbol_sizes(const pastix_data_t *d, int64_t out[8]) {
+    if (!d || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
+    out[0] = d->n;
+    out[1] = d->symbol.cblknbr();
+    out[2] = d->symbol.bloknbr();
+    out[3] = d->layout.coefnbr;
+    out[4] = (int64_t)d->layout.browtab.size();
+    out[5] = d->layout.nlevels;
+    out[6] = (int64_t)d->plan.upd.size();
+    out[7] = (int64_t)d->plan.ents.size();
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_symbol_export(const pastix_data_t *d, int64_t *cblk_fcol, int64_t *cblk_bloknum,
+                                        int64_t *cblk_stride, int64_t *cblk_coef, int64_t *blok_frow,
+                                        int64_t *blok_lrow, int64_t *blok_fcblk, int64_t *blok_roff, int64_t *permtab,
+                                        int64_t *peritab) {
+    if (!d || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
+    const pxa::Symbol &s = d->symbol;
+    const pxa::Layout &L = d->layout;
+    auto cp = [](const std::vector<int64_t> &v, int64_t *o) {
+        if (o) std::copy(v.begin(), v.end(), o);
+    };
+    cp(s.cblk_fcol, cblk_fcol);
+    cp(s.cblk_bloknum, cblk_bloknum);
+    cp(L.cblk_stride, cblk_stride);
+    cp(L.cblk_coef, cblk_coef);
+    cp(s.blok_frow, blok_frow);
+    cp(s.blok_lrow, blok_lrow);
+    cp(s.blok_fcblk, blok_fcblk);
+    cp(L.blok_roff, blok_roff);
+    cp(d->order.permtab, permtab);
+    cp(d->order.peritab, peritab);
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_coeftab_export(const pastix_data_t *d, void *L, void *U) {
+    if (!d || !(d->steps & STEP_NUMFACT) || !d->engine) return PASTIX_ERR_BADPARAMETER;
+    return d->engine->download_coef(L, U) ? PASTIX_ERR_INTERNAL : PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_solve_device(pastix_data_t *d, pastix_int_t nrhs, void *d_b, pastix_int_t ldb) {
+    if (!d || !d_b || nrhs < 0 || ldb < d->n) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_NUMFACT)) return PASTIX_ERR_BADPARAMETER;
+    (void)d_b;
+    return PASTIX_ERR_NOTIMPLEMENTED;
+}
+extern "C" int pastix_amd_backward_error(pastix_data_t *d, const double *b, const double *x, double *berr) {
+    if (!d || !b || !x || !berr || !d->engine) return PASTIX_ERR_BADPARAMETER;
+    pxa::Engine &E = *d->engine;
+    const int64_t n = d->n;
+    double *db = device_pair(d);
+    if (!db) return PASTIX_ERR_OUTOFMEMORY;
+    std::vector<double> hb(n), hx(n);
+    for (int64_t i = 0; i < n; i++) {
+        hb[d->order.permtab[i]] = b[i];
+        hx[d->order.permtab[i]] = x[i];
+    }
+    hipMemcpy(db, hb.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(db + n, hx.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    double o[3];
+    if (E.berr(db + n, db, o)) return PASTIX_ERR_INTERNAL;
+    double den = d->bcsc.normInf * o[1] + o[2];
+    *berr = den > 0 ? o[0] / den : o[0];
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_fact_times(const pastix_data_t *d, double out[4]) {
+    if (!d) return PASTIX_ERR_BADPARAMETER;
+    for (int i = 0; i < 4; i++) out[i] = d->fact_times[i];
+    return PASTIX_SUCCESS;
+}

@@ -1,0 +1,89 @@
+// Device helpers shared by the CDNA4 kernels: MFMA wrappers and the
+// wave-level 32x32 tile product used by every GEMM-shaped kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace pxa {
+
+// v_mfma_{f64,f32}_16x16x4: A is 16x4, B is 4x16, D is 16x16.  Lane l holds
+// A[l%16][l/16] and B[l/16][l%16] for both types; the accumulator layouts
+// differ (measured on gfx950):
+//   f32: D[4*(l/16) + r][l%16] in register r
+//   f64: D[4*r + l/16][l%16]   in register r
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// Strided read-only view: element (r, k) at p[r*rs + k*cs].
+template <typename T> struct View {
+    const T *p;
+    int64_t rs, cs;
+    __device__ __forceinline__ T at(int r, int k) const { return p[(int64_t)r * rs + (int64_t)k * cs]; }
+};
+
+// 32x32 accumulator of one wave, as 2x2 MFMA blocks of 16x16.
+template <typename T> struct Quad {
+    typename Mfma<T>::v4 c[2][2];
+    __device__ __forceinline__ void zero() {
+        for (int m = 0; m < 2; m++)
+            for (int n = 0; n < 2; n++)
+                for (int r = 0; r < 4; r++) c[m][n][r] = (T)0;
+    }
+    // Element of block (m,n), register r, owned by `lane`:
+    //   row = m*16 + Mfma<T>::drow(lane, r), col = n*16 + (lane&15)
+};
+
+// acc += A[0:32, 0:K] * B[0:32, 0:K]^T where rows >= ma of A and rows >= nb
+// of B read as zero.  Operand loads for step k+4 are issued before the MFMAs
+// of step k so that the global/LDS latency overlaps the matrix core.
+template <typename T>
+__device__ __forceinline__ void quad_gemm_nt(Quad<T> &q, View<T> A, int ma, View<T> B, int nb, int K, int lane) {
+    const int i = lane & 15, kq = lane >> 4;
+    const bool ra0 = i < ma, ra1 = i + 16 < ma, rb0 = i < nb, rb1 = i + 16 < nb;
+    T a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+    if (kq < K) {
+        if (ra0) a0 = A.at(i, kq);
+        if (ra1) a1 = A.at(i + 16, kq);
+        if (rb0) b0 = B.at(i, kq);
+        if (rb1) b1 = B.at(i + 16, kq);
+    }
+    for (int k = 0; k < K; k += 4) {
+        const int kn = k + 4 + kq;
+        T na0 = 0, na1 = 0, nb0 = 0, nb1 = 0;
+        if (kn < K) {
+            if (ra0) na0 = A.at(i, kn);
+            if (ra1) na1 = A.at(i + 16, kn);
+            if (rb0) nb0 = B.at(i, kn);
+            if (rb1) nb1 = B.at(i + 16, kn);
+        }
+        q.c[0][0] = Mfma<T>::mma(a0, b0, q.c[0][0]);
+        q.c[0][1] = Mfma<T>::mma(a0, b1, q.c[0][1]);
+        q.c[1][0] = Mfma<T>::mma(a1, b0, q.c[1][0]);
+        q.c[1][1] = Mfma<T>::mma(a1, b1, q.c[1][1]);
+        a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
+    }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of work items so neighbouring items share its L2.  The grid
+// size must be a multiple of 8 (the map is then a bijection of [0, grid));
+// callers drop ids beyond their item count.
+__device__ __forceinline__ int xcd_remap(int bid, int grid) {
+    int per = grid >> 3;
+    return (bid & 7) * per + (bid >> 3);
+}
+
+}  // namespace pxa

@@ -1,0 +1,58 @@
+// Device-side views of the solver structure and the per-level work lists.
+// Shared by the kernels (device) and the engine (host) translation units.
+#pragma once
+#include <cstdint>
+
+namespace pxa {
+
+// Flattened SolverMatrix (reference blend/solver.h:83-213) in device memory.
+// Panels use the 1D layout: cblk c owns coef[cblk_coef[c] .. +stride*width),
+// column-major with ld = stride; block b is rows [blok_roff[b], +nrows).
+struct DevSymbol {
+    const int32_t *cblk_fcol;     // cblknbr+1
+    const int32_t *cblk_bloknum;  // cblknbr+1
+    const int32_t *cblk_stride;   // cblknbr
+    const int64_t *cblk_coef;     // cblknbr+1
+    const int64_t *cblk_rowbase;  // cblknbr+1: prefix sum of strides
+    const int64_t *cblk_winv;     // cblknbr+1: offsets of the width x width diagonal inverses
+    const int32_t *blok_frow;
+    const int32_t *blok_lrow;
+    const int32_t *blok_fcblk;
+    const int32_t *blok_lcblk;
+    const int32_t *blok_roff;
+    const int32_t *panel_grow;    // permuted global row of every panel row
+    const int32_t *panel_blk;     // block index of every panel row
+    const int32_t *cblk_brow;     // cblknbr+1
+    const int32_t *browtab;
+};
+
+// One target tile of a Schur-complement update: rows [r0,r0+64) x columns
+// [c0,c0+64) of cblk f's panel, fed by entries [e0,e1).
+struct UpdItem {
+    int32_t f, r0, c0, e0, e1, pad0, pad1, pad2;
+};
+
+// One source contribution to a tile: rows [p0,p1) and columns [q0,q1) are
+// row ranges of source cblk s's panel; row p lands on target row
+// panel_grow[p] + shift[sb + panel_blk[p] - ib0].
+struct UpdEntry {
+    int32_t s, p0, p1, q0, q1, sb, ib0, pad;
+};
+
+// Forward-solve gather: browtab entries [e0,e1) of target cblk c, partial
+// sums written at slot (offset into the partial buffer, in width*nrhs units).
+struct FwdItem {
+    int32_t c, e0, e1, slot;
+};
+
+// Backward-solve gather: off-diagonal panel rows [r0,r1) of cblk c.
+struct BwdItem {
+    int32_t c, r0, r1, slot;
+};
+
+// Panel TRSM tile: off-diagonal rows [r0, r0+64) of cblk c (absolute panel rows).
+struct TrsmItem {
+    int32_t c, r0;
+};
+
+}  // namespace pxa

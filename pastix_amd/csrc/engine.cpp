@@ -1,0 +1,452 @@
+// Device engine implementation: memory, uploads, and the level-by-level
+// drivers of numfact and solve.  Replaces the reference schedulers
+// (sopalin/sequential_zpotrf.c:34-286, sequential_zgetrf.c,
+// sequential_ztrsm.c:34-322) for one GPU.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "kernels.h"
+
+namespace pxa {
+
+#define HIPCHK(x)                                                                   \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+Engine::~Engine() {
+    if (stream_) hipStreamSynchronize(stream_);
+    for (void *p : owned_) hipFree(p);
+    if (ev0_) hipEventDestroy(ev0_);
+    if (ev1_) hipEventDestroy(ev1_);
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+int Engine::fail(const std::string &m) {
+    err_ = m;
+    fprintf(stderr, "pastix_amd: %s\n", m.c_str());
+    return -1;
+}
+
+void *Engine::alloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        fail("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return nullptr;
+    }
+    owned_.push_back(p);
+    return p;
+}
+
+template <typename T> static T *upload(Engine &E, const std::vector<T> &v) {
+    T *d = (T *)E.alloc(v.size() * sizeof(T));
+    if (d && !v.empty()) hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return d;
+}
+template <typename T, typename S> static T *upload_as(Engine &E, const std::vector<S> &v) {
+    std::vector<T> t(v.begin(), v.end());
+    return upload(E, t);
+}
+
+int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, int factotype, int prec) {
+    dev_ = device;
+    HIPCHK(hipSetDevice(dev_));
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ev0_));
+    HIPCHK(hipEventCreate(&ev1_));
+    facto_ = factotype;
+    prec_ = prec;
+    n_ = s.n;
+    cblknbr_ = s.cblknbr();
+    coefnbr_ = L.coefnbr;
+    winvnbr_ = P.winvnbr;
+    maxwidth_ = (int)L.maxwidth;
+    if (n_ >= (int64_t)INT32_MAX) return fail("matrix order exceeds the 32-bit device index range");
+
+    S_.cblk_fcol = upload_as<int32_t>(*this, s.cblk_fcol);
+    S_.cblk_bloknum = upload_as<int32_t>(*this, s.cblk_bloknum);
+    S_.cblk_stride = upload_as<int32_t>(*this, L.cblk_stride);
+    S_.cblk_coef = upload(*this, L.cblk_coef);
+    S_.cblk_rowbase = upload(*this, P.rowbase);
+    S_.cblk_winv = upload(*this, P.winv_off);
+    S_.blok_frow = upload_as<int32_t>(*this, s.blok_frow);
+    S_.blok_lrow = upload_as<int32_t>(*this, s.blok_lrow);
+    S_.blok_fcblk = upload_as<int32_t>(*this, s.blok_fcblk);
+    S_.blok_lcblk = upload_as<int32_t>(*this, L.blok_lcblk);
+    S_.blok_roff = upload_as<int32_t>(*this, L.blok_roff);
+    S_.panel_grow = upload(*this, P.panel_grow);
+    S_.panel_blk = upload(*this, P.panel_blk);
+    S_.cblk_brow = upload_as<int32_t>(*this, L.cblk_brow);
+    S_.browtab = upload_as<int32_t>(*this, L.browtab);
+
+    lv_cblk_ptr_ = P.lv_cblk_ptr;
+    lv_trsm_ptr_ = P.lv_trsm_ptr;
+    lv_upd_ptr_ = P.lv_upd_ptr;
+    lv_fwd_ptr_ = P.lv_fwd_ptr;
+    lv_bwd_ptr_ = P.lv_bwd_ptr;
+    d_lv_cblks_ = upload(*this, P.lv_cblks);
+    d_fwd_itemptr_ = upload(*this, P.fwd_itemptr);
+    d_bwd_itemptr_ = upload(*this, P.bwd_itemptr);
+    d_shift_ = upload(*this, P.shift);
+    d_trsm_ = upload(*this, P.trsm);
+    d_upd_ = upload(*this, P.upd);
+    d_ents_ = upload(*this, P.ents);
+    d_fwd_ = upload(*this, P.fwd);
+    d_bwd_ = upload(*this, P.bwd);
+    fwd_part_ = P.fwd_part;
+    bwd_part_ = P.bwd_part;
+
+    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    d_L_ = alloc(coefnbr_ * fsz);
+    d_W1_ = alloc(winvnbr_ * fsz);
+    if (!d_L_ || !d_W1_) return -1;
+    HIPCHK(hipMemsetAsync(d_W1_, 0, winvnbr_ * fsz, stream_));
+    if (facto_ == 2) {
+        d_U_ = alloc(coefnbr_ * fsz);
+        d_W2_ = alloc(winvnbr_ * fsz);
+        if (!d_U_ || !d_W2_) return -1;
+        HIPCHK(hipMemsetAsync(d_W2_, 0, winvnbr_ * fsz, stream_));
+    } else {
+        d_W2_ = d_W1_;
+    }
+    d_stat_ = (int *)alloc(4 * sizeof(int));
+    // nrhs chunk bounded by the dynamic LDS of the solve kernels (48 KiB)
+    nrhs_chunk_ = (int)std::max<int64_t>(1, std::min<int64_t>(64, (48 * 1024) / (std::max(maxwidth_, 256) * (int64_t)fsz)));
+    d_part_ = alloc((size_t)std::max<int64_t>(std::max(fwd_part_, bwd_part_), 1) * nrhs_chunk_ * fsz);
+    d_red_ = (double *)alloc((size_t)(512 * 32 + 64) * sizeof(double));
+    if (!d_part_ || !d_red_ || !d_stat_) return -1;
+    HIPCHK(hipStreamSynchronize(stream_));
+    inited_ = true;
+    return 0;
+}
+
+int Engine::upload_coef(const void *hL, const void *hU) {
+    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    HIPCHK(hipMemcpyAsync(d_L_, hL, coefnbr_ * fsz, hipMemcpyHostToDevice, stream_));
+    if (facto_ == 2) HIPCHK(hipMemcpyAsync(d_U_, hU, coefnbr_ * fsz, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Engine::download_coef(void *hL, void *hU) {
+    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    if (hL) HIPCHK(hipMemcpyAsync(hL, d_L_, coefnbr_ * fsz, hipMemcpyDeviceToHost, stream_));
+    if (hU && facto_ == 2) HIPCHK(hipMemcpyAsync(hU, d_U_, coefnbr_ * fsz, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+// Level L: factor the diagonal blocks of its cblks, solve their panels, then
+// apply every update these cblks send to later levels.
+template <typename T> int Engine::factorize_t(double thr) {
+    T *L = (T *)d_L_, *U = (T *)d_U_, *W1 = (T *)d_W1_, *W2 = (T *)d_W2_;
+    for (size_t lv = 0; lv + 1 < lv_cblk_ptr_.size(); lv++) {
+        const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
+        if (facto_ == 2)
+            launch_getrf_diag<T>(S_, d_lv_cblks_ + c0, nc, L, U, W1, W2, thr, d_stat_, stream_);
+        else
+            launch_potrf_diag<T>(S_, d_lv_cblks_ + c0, nc, L, W1, thr, d_stat_, stream_);
+        const int t0 = lv_trsm_ptr_[lv], nt = lv_trsm_ptr_[lv + 1] - t0;
+        if (facto_ == 2) {
+            launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, L, W2, stream_);
+            launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, U, W1, stream_);
+        } else {
+            launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, L, W1, stream_);
+        }
+        const int u0 = lv_upd_ptr_[lv], nu = lv_upd_ptr_[lv + 1] - u0;
+        if (facto_ == 2) {
+            launch_update<T>(S_, 0, d_upd_ + u0, nu, d_ents_, d_shift_, L, U, L, stream_);
+            launch_update<T>(S_, 1, d_upd_ + u0, nu, d_ents_, d_shift_, U, L, U, stream_);
+        } else {
+            launch_update<T>(S_, 0, d_upd_ + u0, nu, d_ents_, d_shift_, L, L, L, stream_);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) {
+    if (!inited_) return fail("engine not initialised");
+    HIPCHK(hipMemsetAsync(d_stat_, 0, 4 * sizeof(int), stream_));
+    HIPCHK(hipEventRecord(ev0_, stream_));
+    int rc = (prec_ == 2) ? factorize_t<float>(thr) : factorize_t<double>(thr);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ev1_, stream_));
+    HIPCHK(hipEventSynchronize(ev1_));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    int st[4];
+    HIPCHK(hipMemcpy(st, d_stat_, sizeof(st), hipMemcpyDeviceToHost));
+    if (npiv) *npiv = st[0];
+    if (negdiag) *negdiag = st[1];
+    if (seconds) *seconds = ms * 1e-3;
+    return 0;
+}
+
+template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
+    const T *L = (const T *)d_L_;
+    const T *U = (facto_ == 2) ? (const T *)d_U_ : L;
+    const T *W1 = (const T *)d_W1_, *W2 = (const T *)d_W2_;
+    T *part = (T *)d_part_;
+    const int nl = (int)lv_cblk_ptr_.size() - 1;
+    for (int r0 = 0; r0 < nrhs; r0 += nrhs_chunk_) {
+        const int nr = std::min(nrhs_chunk_, nrhs - r0);
+        T *xx = x + (int64_t)r0 * ldx;
+        for (int lv = 0; lv < nl; lv++) {
+            const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
+            const int f0 = lv_fwd_ptr_[lv], nf = lv_fwd_ptr_[lv + 1] - f0;
+            launch_fwd_gather<T>(S_, d_fwd_ + f0, nf, L, xx, ldx, nr, part, maxwidth_, stream_);
+            // item slots are level-relative; the diag kernel reads items through absolute indices
+            launch_fwd_diag<T>(S_, d_lv_cblks_ + c0, nc, d_fwd_itemptr_ + c0 + lv, d_fwd_, xx, ldx, nr, part, W1,
+                               maxwidth_, stream_);
+        }
+        for (int lv = nl - 1; lv >= 0; lv--) {
+            const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
+            const int b0 = lv_bwd_ptr_[lv], nb = lv_bwd_ptr_[lv + 1] - b0;
+            launch_bwd_gather<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, stream_);
+            launch_bwd_diag<T>(S_, d_lv_cblks_ + c0, nc, d_bwd_itemptr_ + c0 + lv, d_bwd_, xx, ldx, nr, part, W2,
+                               maxwidth_, stream_);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int Engine::solve_device(void *dx, int64_t ldx, int nrhs, double *seconds) {
+    if (!inited_) return fail("engine not initialised");
+    HIPCHK(hipEventRecord(ev0_, stream_));
+    int rc = (prec_ == 2) ? solve_t<float>((float *)dx, ldx, nrhs) : solve_t<double>((double *)dx, ldx, nrhs);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ev1_, stream_));
+    HIPCHK(hipEventSynchronize(ev1_));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    if (seconds) *seconds = ms * 1e-3;
+    return 0;
+}
+
+int Engine::solve_host(const int64_t *perm, int flttype, void *b, int64_t ldb, int nrhs, double *seconds) {
+    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    if (rhs_cap_ < (int64_t)nrhs * n_) {
+        rhs_cap_ = (int64_t)nrhs * n_;
+        d_rhs_ = alloc(rhs_cap_ * fsz);
+        if (!d_rhs_) return -1;
+    }
+    // permute on the host into the factor precision: xp[perm[i]] = b[i]
+    std::vector<char> tmp((size_t)n_ * nrhs * fsz);
+    for (int r = 0; r < nrhs; r++) {
+        for (int64_t i = 0; i < n_; i++) {
+            const int64_t p = perm[i] + (int64_t)r * n_;
+            if (flttype == 3) {
+                double v = ((const double *)b)[i + r * ldb];
+                if (prec_ == 2) ((float *)tmp.data())[p] = (float)v;
+                else ((double *)tmp.data())[p] = v;
+            } else {
+                float v = ((const float *)b)[i + r * ldb];
+                if (prec_ == 2) ((float *)tmp.data())[p] = v;
+                else ((double *)tmp.data())[p] = v;
+            }
+        }
+    }
+    HIPCHK(hipMemcpyAsync(d_rhs_, tmp.data(), tmp.size(), hipMemcpyHostToDevice, stream_));
+    int rc = solve_device(d_rhs_, n_, nrhs, seconds);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(tmp.data(), d_rhs_, tmp.size(), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int r = 0; r < nrhs; r++) {
+        for (int64_t i = 0; i < n_; i++) {
+            const int64_t p = perm[i] + (int64_t)r * n_;
+            double v = (prec_ == 2) ? (double)((float *)tmp.data())[p] : ((double *)tmp.data())[p];
+            if (flttype == 3) ((double *)b)[i + r * ldb] = v;
+            else ((float *)b)[i + r * ldb] = (float)v;
+        }
+    }
+    return 0;
+}
+
+int Engine::upload_matrix(const HostBcsc &bc) {
+    if (!have_matrix_) {
+        d_rowptr_ = upload(*this, bc.rowptr);
+        d_cols_ = upload(*this, bc.cols);
+        d_vals_ = (double *)alloc(bc.tvals.size() * sizeof(double));
+        if (!d_rowptr_ || !d_cols_ || !d_vals_) return -1;
+        have_matrix_ = true;
+    }
+    HIPCHK(hipMemcpyAsync(d_vals_, bc.tvals.data(), bc.tvals.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Engine::spmv(double alpha, const double *x, double beta, double *y) {
+    if (!have_matrix_) return fail("matrix not uploaded");
+    launch_spmv<double>(n_, d_rowptr_, d_cols_, d_vals_, alpha, x, beta, y, stream_);
+    return 0;
+}
+
+double Engine::dot(const double *x, const double *y) {
+    launch_dot(n_, x, y, d_red_, d_red_ + 512 * 32, stream_);
+    double v = 0;
+    hipMemcpyAsync(&v, d_red_ + 512 * 32, sizeof(double), hipMemcpyDeviceToHost, stream_);
+    hipStreamSynchronize(stream_);
+    return v;
+}
+
+int Engine::precond(double *dv) {
+    if (prec_ == 3) return solve_t<double>(dv, n_, 1);
+    launch_d2s(n_, dv, d_svec_, stream_);
+    int rc = solve_t<float>(d_svec_, n_, 1);
+    if (rc) return rc;
+    launch_s2d(n_, d_svec_, dv, stream_);
+    return 0;
+}
+
+int Engine::berr(const double *dx, const double *db, double out[3]) {
+    if (!have_matrix_) return fail("matrix not uploaded");
+    launch_berr_parts(n_, d_rowptr_, d_cols_, d_vals_, dx, db, d_red_, d_red_ + 512 * 32, stream_);
+    HIPCHK(hipMemcpyAsync(out, d_red_ + 512 * 32, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+// Preconditioned conjugate gradient, reference refinement/z_refine_grad.c:46-156,
+// on device vectors; M^{-1} is one supernodal solve in the factor precision.
+int Engine::refine_cg(double *x, const double *b, int itermax, double eps, int *niter, double *resid) {
+    if (vec_cap_ < 4 * n_) {
+        vec_cap_ = 4 * n_;
+        d_vec_ = (double *)alloc(vec_cap_ * sizeof(double));
+        d_svec_ = (float *)alloc(n_ * sizeof(float));
+        if (!d_vec_ || !d_svec_) return -1;
+    }
+    double *r = d_vec_, *p = d_vec_ + n_, *z = d_vec_ + 2 * n_, *Ap = d_vec_ + 3 * n_;
+    const double normb = std::sqrt(dot(b, b));
+    const double normx = std::sqrt(dot(x, x));
+    launch_copy(n_, b, r, stream_);
+    if (normx > 0.) spmv(-1.0, x, 1.0, r);
+    double normr = std::sqrt(dot(r, r));
+    double rb = (normb > 0) ? normr / normb : normr;
+    launch_copy(n_, r, z, stream_);
+    if (precond(z)) return -1;
+    launch_copy(n_, z, p, stream_);
+    int it = 0;
+    while (rb > eps && it < itermax) {
+        it++;
+        spmv(1.0, p, 0.0, Ap);
+        double rz = dot(r, z);
+        double pAp = dot(Ap, p);
+        double alpha = rz / pAp;
+        launch_axpy(n_, alpha, p, x, stream_);
+        launch_axpy(n_, -alpha, Ap, r, stream_);
+        launch_copy(n_, r, z, stream_);
+        if (precond(z)) return -1;
+        double rz2 = dot(r, z);
+        double beta = rz2 / rz;
+        launch_scal(n_, beta, p, stream_);
+        launch_axpy(n_, 1.0, z, p, stream_);
+        normr = std::sqrt(dot(r, r));
+        rb = (normb > 0) ? normr / normb : normr;
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (niter) *niter = it;
+    if (resid) *resid = rb;
+    return 0;
+}
+
+// Right-preconditioned restarted GMRES with classical Gram-Schmidt and Givens
+// rotations (reference refinement/z_refine_gmres.c:46-352).
+int Engine::refine_gmres(double *x, const double *b, int itermax, int im, double eps, int *niter, double *resid) {
+    const int64_t need = (int64_t)(2 * im + 3) * n_;
+    if (vec_cap_ < need) {
+        vec_cap_ = need;
+        d_vec_ = (double *)alloc(vec_cap_ * sizeof(double));
+        d_svec_ = (float *)alloc(n_ * sizeof(float));
+        if (!d_vec_ || !d_svec_) return -1;
+    }
+    double *V = d_vec_;                    // n x (im+1)
+    double *Wz = d_vec_ + (int64_t)(im + 1) * n_;  // n x im: preconditioned directions
+    double *w = Wz + (int64_t)im * n_;     // n
+    double *hd = d_red_ + 512 * 32 + 8;    // device h column
+    std::vector<double> H((size_t)(im + 1) * im, 0.0), cs(im), sn(im), g(im + 1);
+    const double normb = std::sqrt(dot(b, b));
+    int it = 0;
+    double rb = 1.0;
+    bool done = false;
+    while (!done) {
+        // r = b - A x
+        launch_copy(n_, b, V, stream_);
+        spmv(-1.0, x, 1.0, V);
+        double beta = std::sqrt(dot(V, V));
+        rb = (normb > 0) ? beta / normb : beta;
+        if (rb <= eps || it >= itermax) break;
+        launch_scal(n_, 1.0 / beta, V, stream_);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        for (; j < im && it < itermax; j++) {
+            it++;
+            double *vj = V + (int64_t)j * n_;
+            double *zj = Wz + (int64_t)j * n_;
+            launch_copy(n_, vj, zj, stream_);
+            if (precond(zj)) return -1;
+            spmv(1.0, zj, 0.0, w);
+            // classical Gram-Schmidt: h = V^T w ; w -= V h
+            launch_gemv_t(n_, j + 1, V, n_, w, d_red_, hd, stream_);
+            std::vector<double> h(j + 2);
+            HIPCHK(hipMemcpyAsync(h.data(), hd, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            launch_gemv_n(n_, j + 1, V, n_, hd, -1.0, w, stream_);
+            h[j + 1] = std::sqrt(dot(w, w));
+            if (h[j + 1] > 0) {
+                launch_copy(n_, w, V + (int64_t)(j + 1) * n_, stream_);
+                launch_scal(n_, 1.0 / h[j + 1], V + (int64_t)(j + 1) * n_, stream_);
+            }
+            for (int k = 0; k < j; k++) {
+                double t = cs[k] * h[k] + sn[k] * h[k + 1];
+                h[k + 1] = -sn[k] * h[k] + cs[k] * h[k + 1];
+                h[k] = t;
+            }
+            double den = std::sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+            cs[j] = (den > 0) ? h[j] / den : 1.0;
+            sn[j] = (den > 0) ? h[j + 1] / den : 0.0;
+            h[j] = cs[j] * h[j] + sn[j] * h[j + 1];
+            h[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int k = 0; k <= j; k++) H[(size_t)k + (size_t)j * (im + 1)] = h[k];
+            rb = (normb > 0) ? std::fabs(g[j + 1]) / normb : std::fabs(g[j + 1]);
+            if (rb <= eps) {
+                j++;
+                break;
+            }
+        }
+        // y = H^{-1} g (upper triangular j x j), x += Wz y
+        std::vector<double> y(j);
+        for (int k = j - 1; k >= 0; k--) {
+            double s = g[k];
+            for (int l = k + 1; l < j; l++) s -= H[(size_t)k + (size_t)l * (im + 1)] * y[l];
+            y[k] = s / H[(size_t)k + (size_t)k * (im + 1)];
+        }
+        if (j > 0) {
+            HIPCHK(hipMemcpyAsync(hd, y.data(), j * sizeof(double), hipMemcpyHostToDevice, stream_));
+            launch_gemv_n(n_, j, Wz, n_, hd, 1.0, x, stream_);
+        }
+        if (rb <= eps || it >= itermax) {
+            launch_copy(n_, b, w, stream_);
+            spmv(-1.0, x, 1.0, w);
+            double nr = std::sqrt(dot(w, w));
+            rb = (normb > 0) ? nr / normb : nr;
+            done = true;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (niter) *niter = it;
+    if (resid) *resid = rb;
+    return 0;
+}
+
+}  // namespace pxa

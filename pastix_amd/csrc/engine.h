@@ -1,0 +1,118 @@
+// Device engine: owns the HBM-resident solver structure, factor and
+// workspaces of one pastix instance and runs numfact / solve / SpMV / BLAS-1
+// on one HIP stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "analysis.h"
+#include "plan.h"
+
+namespace pxa {
+
+struct HostBcsc;
+
+class Engine {
+  public:
+    Engine() = default;
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    // factotype: 0 POTRF/LLT, 2 GETRF.  prec: 2 (float) or 3 (double) for the factor.
+    int init(int device, const Symbol &s, const Layout &L, const Plan &P, int factotype, int prec);
+    bool ready() const { return inited_; }
+    int prec() const { return prec_; }
+    int factotype() const { return facto_; }
+
+    // Host coefficient tables (1D layout, factor precision) to the device.
+    int upload_coef(const void *hL, const void *hU);
+    int download_coef(void *hL, void *hU);
+    // Numerical factorisation over all levels; seconds = device time.
+    int factorize(double thr, int64_t *npiv, int *negdiag, double *seconds);
+    // Solve in place on a device block (factor precision, permuted order).
+    int solve_device(void *dx, int64_t ldx, int nrhs, double *seconds);
+    // Host right-hand sides in the user's ordering (FP64 or factor precision).
+    int solve_host(const int64_t *perm, int flttype, void *b, int64_t ldb, int nrhs, double *seconds);
+
+    // Permuted matrix in CSR form for the residual SpMV (always FP64).
+    int upload_matrix(const HostBcsc &bc);
+    // Device refinement drivers (vectors in the permuted ordering, FP64).
+    int refine_cg(double *dx, const double *db, int itermax, double eps, int *niter, double *resid);
+    int refine_gmres(double *dx, const double *db, int itermax, int im, double eps, int *niter, double *resid);
+    // Normwise backward error pieces: [max|b-Ax|, max|x|, max|b|].
+    int berr(const double *dx, const double *db, double out[3]);
+
+    hipStream_t stream() const { return stream_; }
+    int64_t n() const { return n_; }
+    void *alloc(size_t bytes);
+    const std::string &last_error() const { return err_; }
+    int device() const { return dev_; }
+    // applies M^{-1} (one solve in factor precision) to an FP64 device vector
+    int precond(double *dv);
+    // y = alpha*A*x + beta*y on FP64 device vectors
+    int spmv(double alpha, const double *x, double beta, double *y);
+    double dot(const double *x, const double *y);
+
+    // timing of the last factorize / solve split by kernel family (ms)
+    double t_diag = 0, t_trsm = 0, t_upd = 0;
+
+  private:
+    template <typename T> int factorize_t(double thr);
+    template <typename T> int solve_t(T *dx, int64_t ldx, int nrhs);
+    int fail(const std::string &m);
+
+    bool inited_ = false;
+    int dev_ = 0, facto_ = 0, prec_ = 3;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    std::vector<void *> owned_;
+    std::string err_;
+
+    int64_t n_ = 0, cblknbr_ = 0, coefnbr_ = 0, winvnbr_ = 0;
+    int maxwidth_ = 0;
+    DevSymbol S_{};
+    // host copies of the level pointers
+    std::vector<int32_t> lv_cblk_ptr_, lv_trsm_ptr_, lv_upd_ptr_, lv_fwd_ptr_, lv_bwd_ptr_;
+    // device plan
+    int32_t *d_lv_cblks_ = nullptr, *d_fwd_itemptr_ = nullptr, *d_bwd_itemptr_ = nullptr, *d_shift_ = nullptr;
+    TrsmItem *d_trsm_ = nullptr;
+    UpdItem *d_upd_ = nullptr;
+    UpdEntry *d_ents_ = nullptr;
+    FwdItem *d_fwd_ = nullptr;
+    BwdItem *d_bwd_ = nullptr;
+    int32_t *d_perm_ = nullptr;  // permtab as int32 for device permutations
+    int64_t fwd_part_ = 0, bwd_part_ = 0;
+    // numerics
+    void *d_L_ = nullptr, *d_U_ = nullptr, *d_W1_ = nullptr, *d_W2_ = nullptr;
+    int *d_stat_ = nullptr;
+    void *d_part_ = nullptr;
+    int nrhs_chunk_ = 8;
+    void *d_rhs_ = nullptr;  // n x nrhs_cap work block in factor precision
+    int64_t rhs_cap_ = 0;
+    double *d_red_ = nullptr;  // reduction partials + scalars
+    // matrix (CSR of the permuted matrix), FP64
+    int64_t *d_rowptr_ = nullptr;
+    int32_t *d_cols_ = nullptr;
+    double *d_vals_ = nullptr;
+    bool have_matrix_ = false;
+    // refinement vectors
+    double *d_vec_ = nullptr;
+    int64_t vec_cap_ = 0;
+    float *d_svec_ = nullptr;
+};
+
+// Host-side permuted matrix (reference bcsc/bcsc.h:28-47 keeps the same data
+// grouped by cblk).  colptr/rows/vals: CSC of B = P A P^T with the full
+// pattern; trowptr/tcols/tvals: CSR of B (= CSC of B^T).
+struct HostBcsc {
+    int64_t n = 0;
+    int mtxtype = 0;
+    std::vector<int64_t> colptr, rowptr;
+    std::vector<int32_t> rows, cols;
+    std::vector<double> vals, tvals;
+    double normF = 0, normInf = 0;
+};
+
+}  // namespace pxa

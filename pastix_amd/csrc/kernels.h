@@ -1,0 +1,61 @@
+// Host-callable launchers of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "dev_types.h"
+
+namespace pxa {
+
+template <typename T>
+void launch_potrf_diag(const DevSymbol &S, const int32_t *cblks, int n, T *coef, T *winv, double thr, int *stat,
+                       hipStream_t st);
+template <typename T>
+void launch_getrf_diag(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, T *w1, T *w2, double thr,
+                       int *stat, hipStream_t st);
+template <typename T>
+void launch_trsm_panel(const DevSymbol &S, const TrsmItem *items, int n, T *coef, const T *winv, hipStream_t st);
+template <typename T>
+void launch_update(const DevSymbol &S, int mode, const UpdItem *items, int n, const UpdEntry *ents,
+                   const int32_t *shift, const T *A, const T *B, T *C, hipStream_t st);
+
+// Supernodal triangular solves (reference kernels/solve_ztrsmsp.c:332-557).
+template <typename T>
+void launch_fwd_gather(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
+                       int nrhs, T *part, int maxwidth, hipStream_t st);
+template <typename T>
+void launch_fwd_diag(const DevSymbol &S, const int32_t *cblks, int n, const int32_t *itemptr, const FwdItem *items,
+                     T *x, int64_t ldx, int nrhs, const T *part, const T *w1, int maxwidth, hipStream_t st);
+template <typename T>
+void launch_bwd_gather(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
+                       int nrhs, T *part, hipStream_t st);
+template <typename T>
+void launch_bwd_diag(const DevSymbol &S, const int32_t *cblks, int n, const int32_t *itemptr, const BwdItem *items,
+                     T *x, int64_t ldx, int nrhs, const T *part, const T *w2, int maxwidth, hipStream_t st);
+
+// Sparse matrix-vector product on the permuted block CSC
+// (reference bcsc/bcsc_zspmv.c:55-202): y = alpha*A*x + beta*y.
+template <typename T>
+void launch_spmv(int64_t n, const int64_t *colptr, const int32_t *rows, const T *vals, T alpha, const T *x, T beta,
+                 T *y, hipStream_t st);
+
+// BLAS-1 on device vectors (reference bcsc/bvec_zcompute.c).
+void launch_dot(int64_t n, const double *x, const double *y, double *partials, double *out, hipStream_t st);
+void launch_nrm2sq(int64_t n, const double *x, double *partials, double *out, hipStream_t st);
+void launch_axpy(int64_t n, double alpha, const double *x, double *y, hipStream_t st);
+void launch_axpy_dev(int64_t n, const double *alpha, double sign, const double *x, double *y, hipStream_t st);
+void launch_scal(int64_t n, double alpha, double *x, hipStream_t st);
+void launch_copy(int64_t n, const double *x, double *y, hipStream_t st);
+void launch_d2s(int64_t n, const double *x, float *y, hipStream_t st);
+void launch_s2d(int64_t n, const float *x, double *y, hipStream_t st);
+void launch_gemv_t(int64_t n, int m, const double *V, int64_t ldv, const double *x, double *partials, double *out,
+                   hipStream_t st);
+void launch_gemv_n(int64_t n, int m, const double *V, int64_t ldv, const double *y, double alpha, double *x,
+                   hipStream_t st);
+void launch_berr_parts(int64_t n, const int64_t *colptr, const int32_t *rows, const double *vals, const double *x,
+                       const double *b, double *partials, double *out, hipStream_t st);
+// Permute rows of a column-major m x nrhs block: y[perm[i]] = x[i] (scatter) or y[i] = x[perm[i]] (gather).
+template <typename T>
+void launch_permute(int64_t m, int nrhs, const int32_t *perm, bool scatter, const T *x, int64_t ldx, T *y,
+                    int64_t ldy, hipStream_t st);
+
+}  // namespace pxa

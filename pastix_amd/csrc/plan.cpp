@@ -1,0 +1,201 @@
+// Level schedule builder (host).
+//
+// The reference walks cblks in order and lets every source lock its target
+// while it adds an update (kernels/core_zgemmsp.c:455-463).  Here the update
+// of a level is reorganised by target: every 64x64 tile of a target panel
+// becomes one work item that owns the tile and lists, in source order, the
+// (source rows, source columns) ranges that land in it.
+#include "plan.h"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace pxa {
+
+namespace {
+struct Range {
+    int32_t t, a, b;  // tile index, [a, b) source-local rows
+};
+struct KeyEnt {
+    uint64_t key;
+    UpdEntry e;
+};
+}  // namespace
+
+Plan build_plan(const Symbol &s, const Layout &L) {
+    const int64_t cb = s.cblknbr(), nb = s.bloknbr();
+    Plan P;
+    // panel row maps
+    P.rowbase.assign(cb + 1, 0);
+    for (int64_t c = 0; c < cb; c++) P.rowbase[c + 1] = P.rowbase[c] + L.cblk_stride[c];
+    P.panel_grow.assign(P.rowbase[cb], 0);
+    P.panel_blk.assign(P.rowbase[cb], 0);
+    for (int64_t c = 0; c < cb; c++)
+        for (int64_t b = s.cblk_bloknum[c]; b < s.cblk_bloknum[c + 1]; b++) {
+            int64_t nr = s.blok_lrow[b] - s.blok_frow[b] + 1;
+            for (int64_t t = 0; t < nr; t++) {
+                P.panel_grow[P.rowbase[c] + L.blok_roff[b] + t] = (int32_t)(s.blok_frow[b] + t);
+                P.panel_blk[P.rowbase[c] + L.blok_roff[b] + t] = (int32_t)b;
+            }
+        }
+    P.winv_off.assign(cb + 1, 0);
+    for (int64_t c = 0; c < cb; c++) {
+        int64_t w = s.cblk_fcol[c + 1] - s.cblk_fcol[c];
+        P.winv_off[c + 1] = P.winv_off[c] + w * w;
+    }
+    P.winvnbr = P.winv_off[cb];
+
+    // levels
+    const int32_t nl = L.nlevels;
+    P.nlevels = nl;
+    P.lv_cblk_ptr.assign(nl + 1, 0);
+    for (int64_t c = 0; c < cb; c++) P.lv_cblk_ptr[L.cblk_level[c] + 1]++;
+    for (int32_t l = 0; l < nl; l++) P.lv_cblk_ptr[l + 1] += P.lv_cblk_ptr[l];
+    P.lv_cblks.assign(cb, 0);
+    {
+        std::vector<int32_t> pos(P.lv_cblk_ptr.begin(), P.lv_cblk_ptr.end() - 1);
+        for (int64_t c = 0; c < cb; c++) P.lv_cblks[pos[L.cblk_level[c]]++] = (int32_t)c;
+    }
+    auto width = [&](int64_t c) { return s.cblk_fcol[c + 1] - s.cblk_fcol[c]; };
+    auto rows = [&](int64_t b) { return s.blok_lrow[b] - s.blok_frow[b] + 1; };
+
+    P.lv_trsm_ptr.assign(nl + 1, 0);
+    P.lv_upd_ptr.assign(nl + 1, 0);
+    P.lv_fwd_ptr.assign(nl + 1, 0);
+    P.lv_bwd_ptr.assign(nl + 1, 0);
+    P.fwd_itemptr.assign(cb + nl, 0);
+    P.bwd_itemptr.assign(cb + nl, 0);
+    std::vector<KeyEnt> ke;
+    std::vector<Range> rr, cr;
+    const int64_t FWD_LIMIT = 1 << 16;  // factor entries read by one forward item
+    const int32_t BWD_ROWS = 256;       // panel rows read by one backward item
+
+    for (int32_t lv = 0; lv < nl; lv++) {
+        const int32_t l0 = P.lv_cblk_ptr[lv], l1 = P.lv_cblk_ptr[lv + 1];
+        // TRSM tiles
+        for (int32_t k = l0; k < l1; k++) {
+            int32_t c = P.lv_cblks[k];
+            int64_t K = width(c), st = L.cblk_stride[c];
+            for (int64_t r0 = K; r0 < st; r0 += 64) P.trsm.push_back(TrsmItem{c, (int32_t)r0});
+        }
+        P.lv_trsm_ptr[lv + 1] = (int32_t)P.trsm.size();
+
+        // update tiles from the sources of this level
+        ke.clear();
+        for (int32_t k = l0; k < l1; k++) {
+            const int64_t src = P.lv_cblks[k];
+            const int64_t last = s.cblk_bloknum[src + 1];
+            int64_t b = s.cblk_bloknum[src] + 1;
+            while (b < last) {
+                const int64_t f = s.blok_fcblk[b];
+                const int64_t i0 = b;
+                while (b < last && s.blok_fcblk[b] == f) b++;
+                const int64_t i1 = b;
+                const int32_t sb = (int32_t)P.shift.size();
+                int64_t fb = s.cblk_bloknum[f];
+                for (int64_t i = i0; i < last; i++) {
+                    while (!(s.blok_frow[fb] <= s.blok_frow[i] && s.blok_lrow[i] <= s.blok_lrow[fb])) fb++;
+                    P.shift.push_back((int32_t)(L.blok_roff[fb] - s.blok_frow[fb]));
+                }
+                // row ranges per target row tile
+                rr.clear();
+                for (int64_t i = i0; i < last; i++) {
+                    int64_t pb = L.blok_roff[i], pe = pb + rows(i);
+                    int64_t flr0 = s.blok_frow[i] + P.shift[sb + (i - i0)];
+                    int64_t p = pb;
+                    while (p < pe) {
+                        int64_t tr = (flr0 + (p - pb)) / 64;
+                        int64_t pn = std::min(pe, pb + ((tr + 1) * 64 - flr0));
+                        if (!rr.empty() && rr.back().t == tr && rr.back().b == p) rr.back().b = (int32_t)pn;
+                        else rr.push_back(Range{(int32_t)tr, (int32_t)p, (int32_t)pn});
+                        p = pn;
+                    }
+                }
+                // column ranges per target column tile
+                cr.clear();
+                for (int64_t i = i0; i < i1; i++) {
+                    int64_t qb = L.blok_roff[i], qe = qb + rows(i);
+                    int64_t fcl0 = s.blok_frow[i] - s.cblk_fcol[f];
+                    int64_t q = qb;
+                    while (q < qe) {
+                        int64_t tc = (fcl0 + (q - qb)) / 64;
+                        int64_t qn = std::min(qe, qb + ((tc + 1) * 64 - fcl0));
+                        if (!cr.empty() && cr.back().t == tc && cr.back().b == q) cr.back().b = (int32_t)qn;
+                        else cr.push_back(Range{(int32_t)tc, (int32_t)q, (int32_t)qn});
+                        q = qn;
+                    }
+                }
+                const int64_t rb = P.rowbase[src];
+                for (const Range &r : rr)
+                    for (const Range &cc : cr) {
+                        // keep only tiles with at least one pair block(p) >= block(q)
+                        if (P.panel_blk[rb + r.b - 1] < P.panel_blk[rb + cc.a]) continue;
+                        uint64_t key = ((uint64_t)f << 40) | ((uint64_t)r.t << 16) | (uint64_t)cc.t;
+                        ke.push_back(KeyEnt{key, UpdEntry{(int32_t)src, r.a, r.b, cc.a, cc.b, sb, (int32_t)i0, 0}});
+                    }
+            }
+        }
+        std::stable_sort(ke.begin(), ke.end(), [](const KeyEnt &a, const KeyEnt &b) { return a.key < b.key; });
+        for (size_t k = 0; k < ke.size();) {
+            size_t e = k;
+            while (e < ke.size() && ke[e].key == ke[k].key) e++;
+            UpdItem it;
+            it.f = (int32_t)(ke[k].key >> 40);
+            it.r0 = (int32_t)((ke[k].key >> 16) & 0xFFFFFF) * 64;
+            it.c0 = (int32_t)(ke[k].key & 0xFFFF) * 64;
+            it.e0 = (int32_t)P.ents.size();
+            for (size_t t = k; t < e; t++) P.ents.push_back(ke[t].e);
+            it.e1 = (int32_t)P.ents.size();
+            it.pad0 = it.pad1 = it.pad2 = 0;
+            P.upd.push_back(it);
+            k = e;
+        }
+        P.lv_upd_ptr[lv + 1] = (int32_t)P.upd.size();
+
+        // forward gather items for the targets of this level
+        {
+            int64_t slot = 0;
+            const int32_t seg = l0 + lv;
+            for (int32_t k = l0; k < l1; k++) {
+                const int64_t c = P.lv_cblks[k];
+                P.fwd_itemptr[seg + (k - l0)] = (int32_t)P.fwd.size();
+                int64_t e = L.cblk_brow[c];
+                const int64_t ee = L.cblk_brow[c + 1];
+                while (e < ee) {
+                    int64_t e0 = e, cost = 0;
+                    while (e < ee && (e == e0 || cost < FWD_LIMIT)) {
+                        int64_t bb = L.browtab[e];
+                        cost += rows(bb) * width(L.blok_lcblk[bb]);
+                        e++;
+                    }
+                    P.fwd.push_back(FwdItem{(int32_t)c, (int32_t)e0, (int32_t)e, (int32_t)slot});
+                    slot += width(c);
+                }
+            }
+            P.fwd_itemptr[seg + (l1 - l0)] = (int32_t)P.fwd.size();
+            P.fwd_part = std::max(P.fwd_part, slot);
+            P.lv_fwd_ptr[lv + 1] = (int32_t)P.fwd.size();
+        }
+        // backward gather items for the cblks of this level
+        {
+            int64_t slot = 0;
+            const int32_t seg = l0 + lv;
+            for (int32_t k = l0; k < l1; k++) {
+                const int64_t c = P.lv_cblks[k];
+                P.bwd_itemptr[seg + (k - l0)] = (int32_t)P.bwd.size();
+                const int64_t K = width(c), st = L.cblk_stride[c];
+                for (int64_t r0 = K; r0 < st; r0 += BWD_ROWS) {
+                    int64_t r1 = std::min<int64_t>(st, r0 + BWD_ROWS);
+                    P.bwd.push_back(BwdItem{(int32_t)c, (int32_t)r0, (int32_t)r1, (int32_t)slot});
+                    slot += K;
+                }
+            }
+            P.bwd_itemptr[seg + (l1 - l0)] = (int32_t)P.bwd.size();
+            P.bwd_part = std::max(P.bwd_part, slot);
+            P.lv_bwd_ptr[lv + 1] = (int32_t)P.bwd.size();
+        }
+    }
+    return P;
+}
+
+}  // namespace pxa

@@ -1,0 +1,319 @@
+"""Python binding of the PaStiX C API served by libpastix_amd.so.
+
+Function names and argument meaning follow the reference pypastix module
+(wrappers/python/examples/pypastix/pastix.py): initParam, init, finalize,
+task_analyze, task_numfact, task_solve, task_refine and the subtasks.  Calls
+go through the C-ABI with ctypes; errors from the library raise
+PastixError with the PASTIX_ERR_* code.
+
+The library must have been built (``python -c "import __graft_entry__ as g;
+g.build()"``); importing this module without it raises ImportError, there is
+no fallback implementation.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+from . import spm as _spm
+
+pastix_int = ct.c_int64
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libpastix_amd.so")
+
+if not os.path.exists(LIBPATH):
+    raise ImportError(f"{LIBPATH} is missing: build it with __graft_entry__.build()")
+lib = ct.CDLL(LIBPATH)
+
+# ---------------------------------------------------------------- enums ----
+class iparm:
+    verbose, io_strategy, nnzeros, nnzeros_block_local, allocated_terms, produce_stats = range(6)
+    mc64, ordering, ordering_default = 6, 7, 8
+    amalgamation_lvlblas, amalgamation_lvlcblk = 24, 25
+    reordering_split, reordering_stop = 26, 27
+    min_blocksize, max_blocksize, tasks2d_level, tasks2d_width, allcand = 34, 35, 36, 37, 38
+    incomplete, level_of_fill = 39, 40
+    factorization, static_pivoting, free_cscuser = 41, 42, 43
+    schur_fact_mode, schur_solv_mode, applyperm_ws = 44, 45, 46
+    refinement, nbiter, itermax, gmres_im = 47, 48, 49, 50
+    scheduler, thread_nbr, autosplit_comm = 51, 52, 53
+    gpu_nbr, gpu_memory_percentage, gpu_memory_block_size = 54, 55, 56
+    modify_parameter = 65
+    dof_nbr = 70
+    size = 71
+
+
+class dparm:
+    fill_in, epsilon_refinement, relative_error, epsilon_magn_ctrl = 0, 1, 2, 3
+    analyze_time, pred_fact_time, fact_time, solv_time = 4, 5, 6, 7
+    fact_flops, fact_thflops, fact_rlflops = 8, 9, 10
+    solv_flops, solv_thflops, solv_rlflops, refine_time, a_norm = 11, 12, 13, 14, 15
+    compress_tolerance, compress_min_ratio = 16, 17
+    size = 18
+
+
+class factotype:
+    POTRF, SYTRF, GETRF, PXTRF, HETRF = 0, 1, 2, 3, 4
+    LLH, LDLT, LU, LLT, LDLH = 0, 1, 2, 3, 4
+
+
+class scheduler:
+    Sequential, Static, Parsec, StarPU, Dynamic = 0, 1, 2, 3, 4
+
+
+class refine:
+    GMRES, CG, SR, BiCGSTAB = 0, 1, 2, 3
+
+
+class order:
+    Scotch, Metis, Personal, PtScotch, ParMetis = 0, 1, 2, 3, 4
+
+
+class dir:
+    Forward, Backward = 391, 392
+
+
+class error:
+    SUCCESS, UNKNOWN, ALLOC, NOTIMPLEMENTED, OUTOFMEMORY, THREAD, INTERNAL, BADPARAMETER = range(8)
+
+
+class amd_iparm:
+    factor_precision, device, level_order = 0, 1, 2
+
+
+class PastixError(RuntimeError):
+    def __init__(self, fn, code):
+        super().__init__(f"{fn} returned {code}")
+        self.code = code
+
+
+class pastix_order_t(ct.Structure):
+    _fields_ = [
+        ("baseval", pastix_int), ("vertnbr", pastix_int), ("cblknbr", pastix_int),
+        ("permtab", ct.POINTER(pastix_int)), ("peritab", ct.POINTER(pastix_int)),
+        ("rangtab", ct.POINTER(pastix_int)), ("treetab", ct.POINTER(pastix_int)),
+        ("selevtx", ct.c_void_p), ("sndenbr", pastix_int), ("sndetab", ct.POINTER(pastix_int)),
+    ]
+
+
+_P = ct.c_void_p
+_I = pastix_int
+_A = np.ctypeslib.ndpointer
+_ip = _A(dtype=np.int64, flags="C_CONTIGUOUS")
+_dp = _A(dtype=np.float64, flags="C_CONTIGUOUS")
+
+lib.pastixInitParam.argtypes = [_ip, _dp]
+lib.pastixInitParam.restype = None
+lib.pastixInit.argtypes = [ct.POINTER(_P), ct.c_uint64, _ip, _dp]
+lib.pastixInit.restype = None
+lib.pastixFinalize.argtypes = [ct.POINTER(_P)]
+lib.pastixFinalize.restype = None
+for _f in ("pastix_task_analyze", "pastix_task_numfact", "pastix_subtask_spm2bcsc"):
+    getattr(lib, _f).argtypes = [_P, ct.c_void_p]
+    getattr(lib, _f).restype = ct.c_int
+for _f in ("pastix_subtask_symbfact", "pastix_subtask_reordering", "pastix_subtask_blend",
+           "pastix_subtask_bcsc2ctab", "pastix_subtask_sopalin"):
+    getattr(lib, _f).argtypes = [_P]
+    getattr(lib, _f).restype = ct.c_int
+lib.pastix_subtask_order.argtypes = [_P, ct.c_void_p, ct.c_void_p]
+lib.pastix_subtask_order.restype = ct.c_int
+lib.pastix_task_solve.argtypes = [_P, _I, ct.c_void_p, _I]
+lib.pastix_task_solve.restype = ct.c_int
+lib.pastix_subtask_solve.argtypes = [_P, _I, ct.c_void_p, _I]
+lib.pastix_subtask_solve.restype = ct.c_int
+lib.pastix_task_refine.argtypes = [_P, _I, _I, ct.c_void_p, _I, ct.c_void_p, _I]
+lib.pastix_task_refine.restype = ct.c_int
+lib.pastix_subtask_refine.argtypes = [_P, _I, _I, ct.c_void_p, _I, ct.c_void_p, _I]
+lib.pastix_subtask_refine.restype = ct.c_int
+lib.pastix_subtask_applyorder.argtypes = [_P, ct.c_int, ct.c_int, _I, _I, ct.c_void_p, _I]
+lib.pastix_subtask_applyorder.restype = ct.c_int
+lib.pastixOrderGrid.argtypes = [ct.POINTER(ct.POINTER(pastix_order_t)), _I, _I, _I]
+lib.pastixOrderGrid.restype = ct.c_int
+lib.pastixOrderExit.argtypes = [ct.POINTER(pastix_order_t)]
+lib.pastixOrderExit.restype = None
+lib.pastix_amd_set_iparm.argtypes = [_P, ct.c_int, _I]
+lib.pastix_amd_set_iparm.restype = ct.c_int
+lib.pastix_amd_symbol_sizes.argtypes = [_P, _ip]
+lib.pastix_amd_symbol_sizes.restype = ct.c_int
+lib.pastix_amd_symbol_export.argtypes = [_P] + [ct.c_void_p] * 10
+lib.pastix_amd_symbol_export.restype = ct.c_int
+lib.pastix_amd_coeftab_export.argtypes = [_P, ct.c_void_p, ct.c_void_p]
+lib.pastix_amd_coeftab_export.restype = ct.c_int
+lib.pastix_amd_backward_error.argtypes = [_P, _dp, _dp, ct.POINTER(ct.c_double)]
+lib.pastix_amd_backward_error.restype = ct.c_int
+
+
+def _check(fn, rc):
+    if rc != 0:
+        raise PastixError(fn, rc)
+
+
+def _getnrhs(nrhs, x):
+    if nrhs == -1:
+        return 1 if x.ndim == 1 else x.shape[1]
+    return nrhs
+
+
+# ---------------------------------------------------------- main API ----
+def initParam():
+    ip = np.zeros(iparm.size, dtype=np.int64)
+    dp = np.zeros(dparm.size, dtype=np.float64)
+    lib.pastixInitParam(ip, dp)
+    return ip, dp
+
+
+class _Handle:
+    """Keeps the iparm/dparm arrays alive as long as the instance (the library
+    stores the caller's pointers, reference common/api.c:733-734)."""
+
+    def __init__(self, ptr, ip, dp):
+        self.ptr = ptr
+        self.iparm = ip
+        self.dparm = dp
+        self.order = None
+
+
+def init(iparm_, dparm_):
+    p = _P()
+    lib.pastixInit(ct.byref(p), 0, iparm_, dparm_)
+    return _Handle(p, iparm_, dparm_)
+
+
+def finalize(pastix_data, iparm_=None, dparm_=None):
+    if pastix_data.ptr:
+        lib.pastixFinalize(ct.byref(pastix_data.ptr))
+    if pastix_data.order is not None:
+        lib.pastixOrderExit(pastix_data.order)
+        pastix_data.order = None
+
+
+def task_analyze(pastix_data, spmA):
+    _check("pastix_task_analyze", lib.pastix_task_analyze(pastix_data.ptr, spmA.id_ptr))
+
+
+def task_numfact(pastix_data, spmA):
+    _check("pastix_task_numfact", lib.pastix_task_numfact(pastix_data.ptr, spmA.id_ptr))
+
+
+def task_solve(pastix_data, spmA, x, nrhs=-1):
+    n = x.shape[0]
+    nrhs = _getnrhs(nrhs, x)
+    assert x.flags.f_contiguous and x.dtype == spmA.dtype
+    _check("pastix_task_solve", lib.pastix_task_solve(pastix_data.ptr, nrhs, x.ctypes.data, n))
+
+
+def task_refine(pastix_data, spmA, b, x, nrhs=-1):
+    n = x.shape[0]
+    nrhs = _getnrhs(nrhs, x)
+    if b.dtype != x.dtype:
+        raise TypeError("b and x must use the same arithmetic")
+    assert x.flags.f_contiguous and b.flags.f_contiguous
+    _check("pastix_task_refine", lib.pastix_task_refine(pastix_data.ptr, n, nrhs, b.ctypes.data, b.shape[0],
+                                                        x.ctypes.data, x.shape[0]))
+
+
+def subtask_order(pastix_data, spmA, myorder=None):
+    ptr = ct.byref(myorder) if myorder is not None else None
+    _check("pastix_subtask_order", lib.pastix_subtask_order(pastix_data.ptr, spmA.id_ptr, ptr))
+
+
+def subtask_symbfact(pastix_data):
+    _check("pastix_subtask_symbfact", lib.pastix_subtask_symbfact(pastix_data.ptr))
+
+
+def subtask_reordering(pastix_data):
+    _check("pastix_subtask_reordering", lib.pastix_subtask_reordering(pastix_data.ptr))
+
+
+def subtask_blend(pastix_data):
+    _check("pastix_subtask_blend", lib.pastix_subtask_blend(pastix_data.ptr))
+
+
+def subtask_spm2bcsc(pastix_data, spmA):
+    _check("pastix_subtask_spm2bcsc", lib.pastix_subtask_spm2bcsc(pastix_data.ptr, spmA.id_ptr))
+
+
+def subtask_bcsc2ctab(pastix_data):
+    _check("pastix_subtask_bcsc2ctab", lib.pastix_subtask_bcsc2ctab(pastix_data.ptr))
+
+
+def subtask_sopalin(pastix_data):
+    _check("pastix_subtask_sopalin", lib.pastix_subtask_sopalin(pastix_data.ptr))
+
+
+def subtask_applyorder(pastix_data, direction, b):
+    flt = _spm.SpmDouble if b.dtype == np.float64 else _spm.SpmFloat
+    m = b.shape[0]
+    n = 1 if b.ndim == 1 else b.shape[1]
+    _check("pastix_subtask_applyorder",
+           lib.pastix_subtask_applyorder(pastix_data.ptr, flt, direction, m, n, b.ctypes.data, m))
+
+
+def subtask_solve(pastix_data, b, nrhs=-1):
+    nrhs = _getnrhs(nrhs, b)
+    _check("pastix_subtask_solve", lib.pastix_subtask_solve(pastix_data.ptr, nrhs, b.ctypes.data, b.shape[0]))
+
+
+# -------------------------------------------------------- orderings ----
+def orderGrid(nx, ny, nz):
+    """pastixOrderGrid (reference order/order_grids.c:275-383): returns a
+    pastix_order_t owned by the caller (free with orderExit)."""
+    o = pastix_order_t()
+    po = ct.pointer(o)
+    _check("pastixOrderGrid", lib.pastixOrderGrid(ct.byref(po), nx, ny, nz))
+    return o
+
+
+def orderExit(o):
+    lib.pastixOrderExit(ct.byref(o))
+
+
+def order_arrays(o):
+    n, c = o.vertnbr, o.cblknbr
+    return (np.ctypeslib.as_array(o.permtab, (n,)).copy(), np.ctypeslib.as_array(o.peritab, (n,)).copy(),
+            np.ctypeslib.as_array(o.rangtab, (c + 1,)).copy(), np.ctypeslib.as_array(o.treetab, (c,)).copy())
+
+
+# -------------------------------------------------------- extensions ----
+def set_amd_iparm(pastix_data, key, value):
+    _check("pastix_amd_set_iparm", lib.pastix_amd_set_iparm(pastix_data.ptr, key, value))
+
+
+def symbol(pastix_data):
+    """Analysed structure as a dict of numpy arrays (permuted numbering)."""
+    sz = np.zeros(8, dtype=np.int64)
+    _check("pastix_amd_symbol_sizes", lib.pastix_amd_symbol_sizes(pastix_data.ptr, sz))
+    n, cb, nb = int(sz[0]), int(sz[1]), int(sz[2])
+    out = {
+        "cblk_fcol": np.zeros(cb + 1, np.int64), "cblk_bloknum": np.zeros(cb + 1, np.int64),
+        "cblk_stride": np.zeros(cb, np.int64), "cblk_coef": np.zeros(cb + 1, np.int64),
+        "blok_frow": np.zeros(nb, np.int64), "blok_lrow": np.zeros(nb, np.int64),
+        "blok_fcblk": np.zeros(nb, np.int64), "blok_roff": np.zeros(nb, np.int64),
+        "permtab": np.zeros(n, np.int64), "peritab": np.zeros(n, np.int64),
+    }
+    keys = ["cblk_fcol", "cblk_bloknum", "cblk_stride", "cblk_coef", "blok_frow", "blok_lrow", "blok_fcblk",
+            "blok_roff", "permtab", "peritab"]
+    _check("pastix_amd_symbol_export",
+           lib.pastix_amd_symbol_export(pastix_data.ptr, *[out[k].ctypes.data for k in keys]))
+    out["n"], out["cblknbr"], out["bloknbr"], out["coefnbr"] = n, cb, nb, int(sz[3])
+    out["nlevels"], out["upd_items"], out["upd_entries"] = int(sz[5]), int(sz[6]), int(sz[7])
+    return out
+
+
+def coeftab(pastix_data, dtype=np.float64, lu=False):
+    """Device factor copied to host: (L, U or None) in the 1D panel layout."""
+    sz = np.zeros(8, dtype=np.int64)
+    _check("pastix_amd_symbol_sizes", lib.pastix_amd_symbol_sizes(pastix_data.ptr, sz))
+    L = np.zeros(int(sz[3]), dtype=dtype)
+    U = np.zeros(int(sz[3]), dtype=dtype) if lu else None
+    _check("pastix_amd_coeftab_export",
+           lib.pastix_amd_coeftab_export(pastix_data.ptr, L.ctypes.data, U.ctypes.data if lu else None))
+    return L, U
+
+
+def backward_error(pastix_data, b, x):
+    out = ct.c_double()
+    _check("pastix_amd_backward_error",
+           lib.pastix_amd_backward_error(pastix_data.ptr, np.ascontiguousarray(b, dtype=np.float64),
+                                         np.ascontiguousarray(x, dtype=np.float64), ct.byref(out)))
+    return out.value

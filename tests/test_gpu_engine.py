@@ -1,0 +1,170 @@
+"""GPU parity tests: the HIP engine against the CPU oracle through the C-ABI.
+
+Bars: the factor within 1e-12 relative Frobenius of the oracle's (FP64),
+solves within the solution's conditioning, normwise backward error
+max|b-Ax| / (||A||_inf max|x| + max|b|) <= 1e-12 after refinement.
+"""
+import numpy as np
+import pytest
+
+from pastix_amd import pastix, spm
+from tests.helpers import analyse, oracle_factor, rel_frob, lower_mask_panel
+
+pytestmark = pytest.mark.gpu
+
+LLT = pastix.factotype.LLT
+LU = pastix.factotype.LU
+
+
+def _berr(A, x, b):
+    r = b - A @ x
+    return float(np.abs(r).max() / (abs(A).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max()))
+
+
+@pytest.mark.parametrize("grid,bs", [((5, 5, 5), (160, 320)), ((8, 8, 8), (4, 8)), ((10, 10, 10), (8, 16)),
+                                     ((20, 20, 20), (160, 320)), ((12, 12, 12), (40, 70))])
+def test_potrf_factor_parity(grid, bs):
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid, min_bs=bs[0], max_bs=bs[1])
+    pastix.task_numfact(h, S)
+    Lg, _ = pastix.coeftab(h)
+    O, bc, Lo, _ = oracle_factor(A, sym)
+    err = rel_frob(lower_mask_panel(sym, Lg), lower_mask_panel(sym, Lo))
+    print(grid, bs, "cblk", sym["cblknbr"], "levels", sym["nlevels"], "rel frob", err)
+    assert err <= 1e-12
+    assert h.iparm[pastix.iparm.static_pivoting] == 0
+    pastix.finalize(h)
+
+
+@pytest.mark.parametrize("nrhs", [1, 4, 16, 64])
+def test_potrf_solve(nrhs):
+    grid = (12, 11, 10)
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid, min_bs=16, max_bs=32)
+    pastix.task_numfact(h, S)
+    x0, b = spm.rhs_from_x(A, nrhs, seed=7)
+    x = b.copy(order="F")
+    pastix.task_solve(h, S, x)
+    assert np.abs(x - x0).max() <= 1e-12 * np.abs(x0).max() * 10
+    for r in range(nrhs):
+        assert _berr(A, x[:, r], b[:, r]) <= 1e-15
+    # oracle solve on the oracle factor agrees
+    O, bc, Lo, _ = oracle_factor(A, sym)
+    p = sym["permtab"]
+    xp = np.zeros_like(b)
+    xp[p, :] = b
+    xo = O.solve(Lo, None, xp)[p, :]
+    assert np.abs(x - xo).max() <= 1e-13 * np.abs(xo).max()
+    pastix.finalize(h)
+
+
+def test_potrf_generic_ordering():
+    # Scotch requested: the engine orders with its own nested dissection
+    A = spm.laplacian7(9, 7, 6, coef=(1.0, 1.0, 0.3))
+    h, S, sym = analyse(A, True, LLT)
+    pastix.task_numfact(h, S)
+    Lg, _ = pastix.coeftab(h)
+    O, bc, Lo, _ = oracle_factor(A, sym)
+    assert rel_frob(lower_mask_panel(sym, Lg), lower_mask_panel(sym, Lo)) <= 1e-12
+    x0, b = spm.rhs_from_x(A, 1, seed=3)
+    x = b.copy(order="F")
+    pastix.task_solve(h, S, x)
+    assert _berr(A, x[:, 0], b[:, 0]) <= 1e-15
+    pastix.finalize(h)
+
+
+@pytest.mark.parametrize("grid,bs", [((6, 6, 6), (160, 320)), ((8, 7, 6), (8, 16)), ((10, 10, 10), (30, 60))])
+def test_getrf_factor_parity(grid, bs):
+    A = spm.convection_diffusion27(*grid, seed=11)
+    h, S, sym = analyse(A, False, LU, grid=grid, min_bs=bs[0], max_bs=bs[1])
+    h.dparm[pastix.dparm.epsilon_magn_ctrl] = np.ldexp(1.0, -52)
+    pastix.task_numfact(h, S)
+    Lg, Ug = pastix.coeftab(h, lu=True)
+    O, bc, Lo, Uo = oracle_factor(A, sym, lu=True, crit=np.ldexp(1.0, -52) * np.linalg.norm(A.data))
+    eL, eU = rel_frob(Lg, Lo), rel_frob(Ug, Uo)
+    print(grid, bs, "L", eL, "U", eU)
+    assert eL <= 1e-12 and eU <= 1e-12
+    x0, b = spm.rhs_from_x(A, 2, seed=5)
+    x = b.copy(order="F")
+    pastix.task_solve(h, S, x)
+    for r in range(2):
+        assert _berr(A, x[:, r], b[:, r]) <= 1e-14
+    pastix.finalize(h)
+
+
+def test_refine_cg_fp64():
+    grid = (10, 10, 10)
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid)
+    h.iparm[pastix.iparm.refinement] = pastix.refine.CG
+    pastix.task_numfact(h, S)
+    x0, b = spm.rhs_from_x(A, 1, seed=1)
+    x = b.copy(order="F")
+    pastix.task_solve(h, S, x)
+    pastix.task_refine(h, S, b, x)
+    assert _berr(A, x[:, 0], b[:, 0]) <= 1e-15
+    assert pastix.backward_error(h, b[:, 0], x[:, 0]) <= 1e-15
+    pastix.finalize(h)
+
+
+@pytest.mark.parametrize("facto", [LLT, LU])
+def test_fp32_factor_fp64_refinement(facto):
+    grid = (12, 12, 12)
+    if facto == LLT:
+        A = spm.laplacian7(*grid, coef=(1.0, 1.0, 0.01))
+        sym_ = True
+    else:
+        A = spm.convection_diffusion27(*grid, seed=2)
+        sym_ = False
+    S = spm.Spm(A, symmetric=sym_)
+    ip, dp = pastix.initParam()
+    ip[pastix.iparm.factorization] = facto
+    ip[pastix.iparm.ordering] = pastix.order.Personal
+    ip[pastix.iparm.refinement] = pastix.refine.CG if facto == LLT else pastix.refine.GMRES
+    h = pastix.init(ip, dp)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.factor_precision, spm.SpmFloat)
+    o = pastix.orderGrid(*grid)
+    h.order = o
+    pastix.subtask_order(h, S, o)
+    pastix.subtask_symbfact(h)
+    pastix.subtask_blend(h)
+    if facto == LU:
+        dp[pastix.dparm.epsilon_magn_ctrl] = np.ldexp(1.0, -52)
+    pastix.task_numfact(h, S)
+    x0, b = spm.rhs_from_x(A, 1, seed=9)
+    x = b.copy(order="F")
+    pastix.task_solve(h, S, x)
+    e0 = _berr(A, x[:, 0], b[:, 0])
+    pastix.task_refine(h, S, b, x)
+    e1 = _berr(A, x[:, 0], b[:, 0])
+    print("facto", facto, "berr fp32 solve", e0, "refined", e1, "iters", ip[pastix.iparm.nbiter])
+    assert e0 < 1e-4
+    assert e1 <= 1e-12
+    pastix.finalize(h)
+
+
+def test_refactorization_same_pattern():
+    grid = (9, 9, 9)
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid)
+    pastix.task_numfact(h, S)
+    for shift in (0.05, 0.09):
+        A2 = spm.laplacian7(*grid, diag=6.0 + shift)
+        S.set_values(A2)
+        pastix.task_numfact(h, S)
+        Lg, _ = pastix.coeftab(h)
+        O, bc, Lo, _ = oracle_factor(A2, sym)
+        assert rel_frob(lower_mask_panel(sym, Lg), lower_mask_panel(sym, Lo)) <= 1e-12
+    pastix.finalize(h)
+
+
+def test_factor_bitwise_reproducible():
+    grid = (12, 12, 12)
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid, min_bs=20, max_bs=40)
+    pastix.task_numfact(h, S)
+    L1, _ = pastix.coeftab(h)
+    pastix.task_numfact(h, S)
+    L2, _ = pastix.coeftab(h)
+    assert np.array_equal(L1, L2)
+    pastix.finalize(h)
