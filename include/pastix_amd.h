@@ -22,6 +22,12 @@ typedef enum pastix_amd_iparm_e {
     PXA_IPARM_DEVICE           = 1,
     /* Apply the reference's level reordering of supernodes (default 0).      */
     PXA_IPARM_LEVEL_ORDER      = 2,
+    /* Keep a device copy of the filled coefficients so that
+     * pastix_amd_refactorize() can rerun the factorisation from HBM
+     * (default 0; costs one more coefficient table in device memory).      */
+    PXA_IPARM_RESIDENT_INPUT   = 3,
+    /* Record per-kernel-family device times of numfact (default 0).        */
+    PXA_IPARM_PROFILE          = 4,
     PXA_IPARM_SIZE             = 8
 } pastix_amd_iparm_t;
 
@@ -62,9 +68,20 @@ int pastix_amd_backward_error(pastix_data_t *pastix_data, const double *b,
                               const double *x, double *berr);
 
 /* Device time of the last numerical factorisation split by kernel family:
- * out[0]=total, [1]=diagonal, [2]=panel TRSM, [3]=update (seconds; the
- * split is filled only when profiling is enabled). */
-int pastix_amd_fact_times(const pastix_data_t *pastix_data, double out[4]);
+ * out[0]=total, [1]=diagonal, [2]=panel TRSM, [3]=update (seconds), and the
+ * number of launches of each family out[4..6]; the split is filled only
+ * when PXA_IPARM_PROFILE is set. */
+int pastix_amd_fact_times(const pastix_data_t *pastix_data, double out[8]);
+
+/* Algorithmic flops of the analysed factorisation: out[0]=DPARM_FACT_THFLOPS,
+ * [1]=diagonal factorisations, [2]=panel TRSMs, [3]=Schur updates counted
+ * as the reference's per-block GEMMs (kernels/core_zgemmsp.c:1352-1395). */
+int pastix_amd_flops(const pastix_data_t *pastix_data, double out[4]);
+
+/* Rerun the numerical factorisation from the device copy of the filled
+ * coefficients (needs PXA_IPARM_RESIDENT_INPUT before pastix_task_numfact);
+ * updates DPARM_FACT_TIME / DPARM_FACT_FLOPS like pastix_subtask_sopalin. */
+int pastix_amd_refactorize(pastix_data_t *pastix_data);
 
 #ifdef __cplusplus
 }

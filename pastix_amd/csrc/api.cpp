@@ -51,7 +51,7 @@ struct pastix_data_s {
     pxa::HostBcsc bcsc;
     std::unique_ptr<pxa::Engine> engine;
     std::vector<char> hostL, hostU;
-    double fact_times[4] = {0, 0, 0, 0};
+    double fact_times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double *dvec = nullptr;  // 2n FP64 device vectors for refinement / error checks
     pxa::Engine *dvec_owner = nullptr;
 };
@@ -485,6 +485,9 @@ extern "C" int pastix_subtask_bcsc2ctab(pastix_data_t *d) {
     else fill_coeftab<double>(d, facto);
     if (d->engine->upload_coef(d->hostL.data(), facto == 2 ? d->hostU.data() : nullptr)) return PASTIX_ERR_INTERNAL;
     if (d->engine->upload_matrix(d->bcsc)) return PASTIX_ERR_INTERNAL;
+    if (d->engine->upload_perm(d->order.permtab)) return PASTIX_ERR_INTERNAL;
+    if (d->ext[PXA_IPARM_RESIDENT_INPUT] && d->engine->snapshot_coef()) return PASTIX_ERR_INTERNAL;
+    d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
     std::vector<char>().swap(d->hostL);
     std::vector<char>().swap(d->hostU);
     d->steps |= STEP_BCSC2CTAB;
@@ -512,6 +515,12 @@ extern "C" int pastix_subtask_sopalin(pastix_data_t *d) {
     d->dparm[DPARM_FACT_FLOPS] = (secs > 0) ? d->dparm[DPARM_FACT_THFLOPS] / secs / (1 << 30) : 0.;
     d->iparm[IPARM_STATIC_PIVOTING] = npiv;
     d->fact_times[0] = secs;
+    d->fact_times[1] = d->engine->t_diag;
+    d->fact_times[2] = d->engine->t_trsm;
+    d->fact_times[3] = d->engine->t_upd;
+    d->fact_times[4] = d->engine->n_diag;
+    d->fact_times[5] = d->engine->n_trsm;
+    d->fact_times[6] = d->engine->n_upd;
     if (neg && d->iparm[IPARM_FACTORIZATION] != PastixFactGETRF) {
         // The reference aborts on a negative pivot (kernels/core_zpotrfsp.c:84-89);
         // this engine reports the failure instead.
@@ -842,8 +851,26 @@ extern "C" int pastix_amd_coeftab_export(const pastix_data_t *d, void *L, void *
 extern "C" int pastix_amd_solve_device(pastix_data_t *d, pastix_int_t nrhs, void *d_b, pastix_int_t ldb) {
     if (!d || !d_b || nrhs < 0 || ldb < d->n) return PASTIX_ERR_BADPARAMETER;
     if (!(d->steps & STEP_NUMFACT)) return PASTIX_ERR_BADPARAMETER;
-    (void)d_b;
-    return PASTIX_ERR_NOTIMPLEMENTED;
+    if (nrhs == 0) return PASTIX_SUCCESS;
+    double secs = 0;
+    if (d->engine->solve_user_device(d_b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
+    d->dparm[DPARM_SOLV_TIME] = secs;
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_flops(const pastix_data_t *d, double out[4]) {
+    if (!d || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
+    const bool lu = d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF;
+    out[0] = d->dparm[DPARM_FACT_THFLOPS];
+    out[1] = d->plan.flops_diag;
+    out[2] = d->plan.flops_trsm * (lu ? 2.0 : 1.0);
+    out[3] = lu ? d->plan.flops_upd_lu : d->plan.flops_upd_potrf;
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_refactorize(pastix_data_t *d) {
+    if (!d || !d->engine || !(d->steps & STEP_BCSC2CTAB)) return PASTIX_ERR_BADPARAMETER;
+    if (d->engine->restore_coef()) return PASTIX_ERR_BADPARAMETER;
+    d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
+    return pastix_subtask_sopalin(d);
 }
 extern "C" int pastix_amd_backward_error(pastix_data_t *d, const double *b, const double *x, double *berr) {
     if (!d || !b || !x || !berr || !d->engine) return PASTIX_ERR_BADPARAMETER;
@@ -864,8 +891,8 @@ extern "C" int pastix_amd_backward_error(pastix_data_t *d, const double *b, cons
     *berr = den > 0 ? o[0] / den : o[0];
     return PASTIX_SUCCESS;
 }
-extern "C" int pastix_amd_fact_times(const pastix_data_t *d, double out[4]) {
+extern "C" int pastix_amd_fact_times(const pastix_data_t *d, double out[8]) {
     if (!d) return PASTIX_ERR_BADPARAMETER;
-    for (int i = 0; i < 4; i++) out[i] = d->fact_times[i];
+    for (int i = 0; i < 8; i++) out[i] = d->fact_times[i];
     return PASTIX_SUCCESS;
 }

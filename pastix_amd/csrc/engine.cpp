@@ -20,11 +20,12 @@ namespace pxa {
     } while (0)
 
 Engine::~Engine() {
-    if (stream_) hipStreamSynchronize(stream_);
-    for (void *p : owned_) hipFree(p);
-    if (ev0_) hipEventDestroy(ev0_);
-    if (ev1_) hipEventDestroy(ev1_);
-    if (stream_) hipStreamDestroy(stream_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void *p : owned_) (void)hipFree(p);
+    for (hipEvent_t e : pev_) (void)hipEventDestroy(e);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 int Engine::fail(const std::string &m) {
@@ -146,28 +147,97 @@ int Engine::download_coef(void *hL, void *hU) {
 // apply every update these cblks send to later levels.
 template <typename T> int Engine::factorize_t(double thr) {
     T *L = (T *)d_L_, *U = (T *)d_U_, *W1 = (T *)d_W1_, *W2 = (T *)d_W2_;
-    for (size_t lv = 0; lv + 1 < lv_cblk_ptr_.size(); lv++) {
+    const size_t nl = lv_cblk_ptr_.size() - 1;
+    if (profile && pev_.size() < 6 * nl) {
+        for (hipEvent_t e : pev_) hipEventDestroy(e);
+        pev_.assign(6 * nl, nullptr);
+        for (auto &e : pev_) HIPCHK(hipEventCreate(&e));
+    }
+    auto mark = [&](size_t k) {
+        if (profile) (void)hipEventRecord(pev_[k], stream_);
+    };
+    for (size_t lv = 0; lv < nl; lv++) {
         const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
+        mark(6 * lv + 0);
         if (facto_ == 2)
             launch_getrf_diag<T>(S_, d_lv_cblks_ + c0, nc, L, U, W1, W2, thr, d_stat_, stream_);
         else
             launch_potrf_diag<T>(S_, d_lv_cblks_ + c0, nc, L, W1, thr, d_stat_, stream_);
+        mark(6 * lv + 1);
         const int t0 = lv_trsm_ptr_[lv], nt = lv_trsm_ptr_[lv + 1] - t0;
+        mark(6 * lv + 2);
         if (facto_ == 2) {
             launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, L, W2, stream_);
             launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, U, W1, stream_);
         } else {
             launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, L, W1, stream_);
         }
+        mark(6 * lv + 3);
         const int u0 = lv_upd_ptr_[lv], nu = lv_upd_ptr_[lv + 1] - u0;
+        mark(6 * lv + 4);
         if (facto_ == 2) {
             launch_update<T>(S_, 0, d_upd_ + u0, nu, d_ents_, d_shift_, L, U, L, stream_);
             launch_update<T>(S_, 1, d_upd_ + u0, nu, d_ents_, d_shift_, U, L, U, stream_);
         } else {
             launch_update<T>(S_, 0, d_upd_ + u0, nu, d_ents_, d_shift_, L, L, L, stream_);
         }
+        mark(6 * lv + 5);
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int Engine::snapshot_coef() {
+    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    if (!d_L0_) {
+        d_L0_ = alloc(coefnbr_ * fsz);
+        if (!d_L0_) return -1;
+        if (facto_ == 2) {
+            d_U0_ = alloc(coefnbr_ * fsz);
+            if (!d_U0_) return -1;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(d_L0_, d_L_, coefnbr_ * fsz, hipMemcpyDeviceToDevice, stream_));
+    if (facto_ == 2) HIPCHK(hipMemcpyAsync(d_U0_, d_U_, coefnbr_ * fsz, hipMemcpyDeviceToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Engine::restore_coef() {
+    if (!d_L0_) return fail("no coefficient snapshot");
+    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    HIPCHK(hipMemcpyAsync(d_L_, d_L0_, coefnbr_ * fsz, hipMemcpyDeviceToDevice, stream_));
+    if (facto_ == 2) HIPCHK(hipMemcpyAsync(d_U_, d_U0_, coefnbr_ * fsz, hipMemcpyDeviceToDevice, stream_));
+    return 0;
+}
+
+int Engine::upload_perm(const std::vector<int64_t> &permtab) {
+    if (!d_perm_) {
+        std::vector<int32_t> p(permtab.begin(), permtab.end());
+        d_perm_ = upload(*this, p);
+        if (!d_perm_) return -1;
+    }
+    return 0;
+}
+
+// Device right-hand sides in the user ordering: permute into the work block,
+// solve, permute back (reference sopalin/pastix_task_solve.c:550-577 with
+// bvec_zlapmr replaced by a gather/scatter kernel).
+int Engine::solve_user_device(void *db, int64_t ldb, int nrhs, double *seconds) {
+    if (!d_perm_) return fail("permutation not uploaded");
+    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    if (rhs_cap_ < (int64_t)nrhs * n_) {
+        rhs_cap_ = (int64_t)nrhs * n_;
+        d_rhs_ = alloc(rhs_cap_ * fsz);
+        if (!d_rhs_) return -1;
+    }
+    if (prec_ == 2) launch_permute<float>(n_, nrhs, d_perm_, true, (const float *)db, ldb, (float *)d_rhs_, n_, stream_);
+    else launch_permute<double>(n_, nrhs, d_perm_, true, (const double *)db, ldb, (double *)d_rhs_, n_, stream_);
+    int rc = solve_device(d_rhs_, n_, nrhs, seconds);
+    if (rc) return rc;
+    if (prec_ == 2) launch_permute<float>(n_, nrhs, d_perm_, false, (const float *)d_rhs_, n_, (float *)db, ldb, stream_);
+    else launch_permute<double>(n_, nrhs, d_perm_, false, (const double *)d_rhs_, n_, (double *)db, ldb, stream_);
+    HIPCHK(hipStreamSynchronize(stream_));
     return 0;
 }
 
@@ -181,6 +251,22 @@ int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) 
     HIPCHK(hipEventSynchronize(ev1_));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    if (profile) {
+        t_diag = t_trsm = t_upd = 0;
+        n_diag = n_trsm = n_upd = 0;
+        for (size_t lv = 0; lv + 1 < lv_cblk_ptr_.size(); lv++) {
+            float a = 0, b = 0, c = 0;
+            HIPCHK(hipEventElapsedTime(&a, pev_[6 * lv + 0], pev_[6 * lv + 1]));
+            HIPCHK(hipEventElapsedTime(&b, pev_[6 * lv + 2], pev_[6 * lv + 3]));
+            HIPCHK(hipEventElapsedTime(&c, pev_[6 * lv + 4], pev_[6 * lv + 5]));
+            t_diag += a * 1e-3;
+            t_trsm += b * 1e-3;
+            t_upd += c * 1e-3;
+            n_diag += (lv_cblk_ptr_[lv + 1] > lv_cblk_ptr_[lv]);
+            n_trsm += (lv_trsm_ptr_[lv + 1] > lv_trsm_ptr_[lv]) * (facto_ == 2 ? 2 : 1);
+            n_upd += (lv_upd_ptr_[lv + 1] > lv_upd_ptr_[lv]) * (facto_ == 2 ? 2 : 1);
+        }
+    }
     int st[4];
     HIPCHK(hipMemcpy(st, d_stat_, sizeof(st), hipMemcpyDeviceToHost));
     if (npiv) *npiv = st[0];

@@ -55,8 +55,19 @@ class Engine {
     int spmv(double alpha, const double *x, double beta, double *y);
     double dot(const double *x, const double *y);
 
-    // timing of the last factorize / solve split by kernel family (ms)
+    // Keep a device copy of the freshly filled coefficients so that the
+    // factorisation can be re-run from HBM-resident input (restore_coef).
+    int snapshot_coef();
+    int restore_coef();
+    // Solve with device right-hand sides in the user ordering.
+    int solve_user_device(void *db, int64_t ldb, int nrhs, double *seconds);
+    int upload_perm(const std::vector<int64_t> &permtab);
+
+    // Per-family device time of the last factorize (seconds) when profiling
+    // is on (PASTIX_AMD_PROFILE=1): diag, trsm, update; launch counts.
+    bool profile = false;
     double t_diag = 0, t_trsm = 0, t_upd = 0;
+    int n_diag = 0, n_trsm = 0, n_upd = 0;
 
   private:
     template <typename T> int factorize_t(double thr);
@@ -83,6 +94,8 @@ class Engine {
     FwdItem *d_fwd_ = nullptr;
     BwdItem *d_bwd_ = nullptr;
     int32_t *d_perm_ = nullptr;  // permtab as int32 for device permutations
+    void *d_L0_ = nullptr, *d_U0_ = nullptr;  // snapshot of the filled coefficients
+    std::vector<hipEvent_t> pev_;
     int64_t fwd_part_ = 0, bwd_part_ = 0;
     // numerics
     void *d_L_ = nullptr, *d_U_ = nullptr, *d_W1_ = nullptr, *d_W2_ = nullptr;

@@ -45,6 +45,21 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     }
     P.winvnbr = P.winv_off[cb];
 
+    // flop accounting
+    for (int64_t c = 0; c < cb; c++) {
+        const double K = (double)(s.cblk_fcol[c + 1] - s.cblk_fcol[c]);
+        const double M = (double)(L.cblk_stride[c]) - K;
+        P.flops_diag += K * ((K / 6.0 + 0.5) * K + 1.0 / 3.0) + K * ((K / 6.0) * K - 1.0 / 6.0);
+        P.flops_trsm += M * K * (K + 1.0);
+        for (int64_t b = s.cblk_bloknum[c] + 1; b < s.cblk_bloknum[c + 1]; b++) {
+            const double nrow = (double)(s.blok_lrow[b] - s.blok_frow[b] + 1);
+            const double m = (double)(L.cblk_stride[c] - L.blok_roff[b]);
+            P.flops_upd_potrf += 2.0 * m * nrow * K;
+            P.flops_upd_lu += 2.0 * m * nrow * K + ((b + 1 < s.cblk_bloknum[c + 1]) ? 2.0 * (m - nrow) * nrow * K : 0.0);
+        }
+    }
+    P.flops_upd = P.flops_upd_potrf;
+
     // levels
     const int32_t nl = L.nlevels;
     P.nlevels = nl;

@@ -34,8 +34,12 @@ struct Plan {
     std::vector<int32_t> panel_grow, panel_blk;
     std::vector<int64_t> rowbase, winv_off;
     int64_t winvnbr = 0;
-    // statistics
-    int64_t upd_flops_tiles = 0;
+    // algorithmic flops per kernel family (reference flop conventions):
+    // diag = FLOPS_DPOTRF/GETRF(K), trsm = FLOPS_DTRSM(Right, M, K) per cblk,
+    // update = sum over (cblk, blok) of FLOPS_DGEMM(m, n, k) with the m, n, k
+    // of kernels/core_zgemmsp.c:1352-1357 (x2 sides for LU)
+    double flops_diag = 0, flops_trsm = 0, flops_upd = 0;
+    double flops_upd_potrf = 0, flops_upd_lu = 0;
 };
 
 Plan build_plan(const Symbol &s, const Layout &L);

@@ -78,7 +78,7 @@ class error:
 
 
 class amd_iparm:
-    factor_precision, device, level_order = 0, 1, 2
+    factor_precision, device, level_order, resident_input, profile = 0, 1, 2, 3, 4
 
 
 class PastixError(RuntimeError):
@@ -317,3 +317,85 @@ def backward_error(pastix_data, b, x):
            lib.pastix_amd_backward_error(pastix_data.ptr, np.ascontiguousarray(b, dtype=np.float64),
                                          np.ascontiguousarray(x, dtype=np.float64), ct.byref(out)))
     return out.value
+
+
+lib.pastix_amd_fact_times.argtypes = [_P, _dp]
+lib.pastix_amd_fact_times.restype = ct.c_int
+lib.pastix_amd_flops.argtypes = [_P, _dp]
+lib.pastix_amd_flops.restype = ct.c_int
+lib.pastix_amd_refactorize.argtypes = [_P]
+lib.pastix_amd_refactorize.restype = ct.c_int
+lib.pastix_amd_solve_device.argtypes = [_P, _I, ct.c_void_p, _I]
+lib.pastix_amd_solve_device.restype = ct.c_int
+
+
+def fact_times(pastix_data):
+    """[total, diag, trsm, update] seconds and launch counts of the last numfact."""
+    out = np.zeros(8)
+    _check("pastix_amd_fact_times", lib.pastix_amd_fact_times(pastix_data.ptr, out))
+    return out
+
+
+def flops(pastix_data):
+    """[THFLOPS, diag, trsm, update] algorithmic flops of the analysed problem."""
+    out = np.zeros(4)
+    _check("pastix_amd_flops", lib.pastix_amd_flops(pastix_data.ptr, out))
+    return out
+
+
+def refactorize(pastix_data):
+    _check("pastix_amd_refactorize", lib.pastix_amd_refactorize(pastix_data.ptr))
+
+
+def solve_device(pastix_data, dbuf, n, nrhs):
+    """Solve in place on a DeviceArray holding an n x nrhs column-major block."""
+    _check("pastix_amd_solve_device", lib.pastix_amd_solve_device(pastix_data.ptr, nrhs, dbuf.ptr, n))
+
+
+class DeviceArray:
+    """Minimal HIP device buffer (hipMalloc/hipMemcpy through ctypes) used to
+    hand device-resident right-hand sides to pastix_amd_solve_device."""
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            h = ct.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+            h.hipFree.argtypes = [ct.c_void_p]
+            h.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, host):
+        self.nbytes = host.nbytes
+        self.dtype = host.dtype
+        self.shape = host.shape
+        self.ptr = ct.c_void_p()
+        rc = self.hip().hipMalloc(ct.byref(self.ptr), self.nbytes)
+        if rc != 0:
+            raise MemoryError(f"hipMalloc failed ({rc})")
+        self.upload(host)
+
+    def upload(self, host):
+        h = np.asfortranarray(host)
+        rc = self.hip().hipMemcpy(self.ptr, h.ctypes.data, self.nbytes, 1)
+        assert rc == 0
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        rc = self.hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2)
+        assert rc == 0
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
