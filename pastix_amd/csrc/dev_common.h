@@ -47,33 +47,46 @@ template <typename T> struct Quad {
 };
 
 // acc += A[0:32, 0:K] * B[0:32, 0:K]^T where rows >= ma of A and rows >= nb
-// of B read as zero.  Operand loads for step k+4 are issued before the MFMAs
-// of step k so that the global/LDS latency overlaps the matrix core.
+// of B read as zero.  K is consumed in groups of QG k-steps (4*QG columns):
+// all operand loads of group g+1 are issued before the MFMAs of group g, so
+// 16*QG independent loads per lane are in flight while the matrix core works.
+template <typename T>
+__device__ __forceinline__ void quad_load_group(View<T> A, View<T> B, int k0, int K, int i, int kq, bool ra0,
+                                                bool ra1, bool rb0, bool rb1, T (&a0)[4], T (&a1)[4], T (&b0)[4],
+                                                T (&b1)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int k = k0 + 4 * u + kq;
+        const bool ok = k < K;
+        a0[u] = (ok && ra0) ? A.at(i, k) : (T)0;
+        a1[u] = (ok && ra1) ? A.at(i + 16, k) : (T)0;
+        b0[u] = (ok && rb0) ? B.at(i, k) : (T)0;
+        b1[u] = (ok && rb1) ? B.at(i + 16, k) : (T)0;
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void quad_gemm_nt(Quad<T> &q, View<T> A, int ma, View<T> B, int nb, int K, int lane) {
     const int i = lane & 15, kq = lane >> 4;
     const bool ra0 = i < ma, ra1 = i + 16 < ma, rb0 = i < nb, rb1 = i + 16 < nb;
-    T a0 = 0, a1 = 0, b0 = 0, b1 = 0;
-    if (kq < K) {
-        if (ra0) a0 = A.at(i, kq);
-        if (ra1) a1 = A.at(i + 16, kq);
-        if (rb0) b0 = B.at(i, kq);
-        if (rb1) b1 = B.at(i + 16, kq);
-    }
-    for (int k = 0; k < K; k += 4) {
-        const int kn = k + 4 + kq;
-        T na0 = 0, na1 = 0, nb0 = 0, nb1 = 0;
-        if (kn < K) {
-            if (ra0) na0 = A.at(i, kn);
-            if (ra1) na1 = A.at(i + 16, kn);
-            if (rb0) nb0 = B.at(i, kn);
-            if (rb1) nb1 = B.at(i + 16, kn);
+    T a0[4], a1[4], b0[4], b1[4];
+    quad_load_group(A, B, 0, K, i, kq, ra0, ra1, rb0, rb1, a0, a1, b0, b1);
+    for (int k = 0; k < K; k += 16) {
+        T na0[4], na1[4], nb0[4], nb1[4];
+        if (k + 16 < K) quad_load_group(A, B, k + 16, K, i, kq, ra0, ra1, rb0, rb1, na0, na1, nb0, nb1);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (k + 4 * u < K) {  // uniform across the wave
+                q.c[0][0] = Mfma<T>::mma(a0[u], b0[u], q.c[0][0]);
+                q.c[0][1] = Mfma<T>::mma(a0[u], b1[u], q.c[0][1]);
+                q.c[1][0] = Mfma<T>::mma(a1[u], b0[u], q.c[1][0]);
+                q.c[1][1] = Mfma<T>::mma(a1[u], b1[u], q.c[1][1]);
+            }
         }
-        q.c[0][0] = Mfma<T>::mma(a0, b0, q.c[0][0]);
-        q.c[0][1] = Mfma<T>::mma(a0, b1, q.c[0][1]);
-        q.c[1][0] = Mfma<T>::mma(a1, b0, q.c[1][0]);
-        q.c[1][1] = Mfma<T>::mma(a1, b1, q.c[1][1]);
-        a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a0[u] = na0[u]; a1[u] = na1[u]; b0[u] = nb0[u]; b1[u] = nb1[u];
+        }
     }
 }
 

@@ -39,10 +39,10 @@ struct UpdEntry {
     int32_t s, p0, p1, q0, q1, sb, ib0, pad;
 };
 
-// Forward-solve gather: browtab entries [e0,e1) of target cblk c, partial
-// sums written at slot (offset into the partial buffer, in width*nrhs units).
+// Forward-solve product: off-diagonal panel rows [r0,r1) of cblk c times x_c
+// into the update buffer at row offset `slot` (ybuf row of panel row r0).
 struct FwdItem {
-    int32_t c, e0, e1, slot;
+    int32_t c, r0, r1, slot;
 };
 
 // Backward-solve gather: off-diagonal panel rows [r0,r1) of cblk c.
@@ -53,6 +53,12 @@ struct BwdItem {
 // Panel TRSM tile: off-diagonal rows [r0, r0+64) of cblk c (absolute panel rows).
 struct TrsmItem {
     int32_t c, r0;
+};
+
+// Blocked panel step tile: rows [r0, r0+64) (and columns [c0, c0+64) for the
+// trailing update) of cblk c's panel.
+struct PanelItem {
+    int32_t c, r0, c0, pad;
 };
 
 }  // namespace pxa

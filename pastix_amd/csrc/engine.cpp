@@ -92,15 +92,25 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     lv_fwd_ptr_ = P.lv_fwd_ptr;
     lv_bwd_ptr_ = P.lv_bwd_ptr;
     d_lv_cblks_ = upload(*this, P.lv_cblks);
-    d_fwd_itemptr_ = upload(*this, P.fwd_itemptr);
+    lv_step_ptr_ = P.lv_step_ptr;
+    step_cb_ptr_ = P.step_cb_ptr;
+    step_trsm_ptr_ = P.step_trsm_ptr;
+    step_upd_ptr_ = P.step_upd_ptr;
+    d_step_cblks_ = upload(*this, P.step_cblks);
+    d_ptrsm_ = upload(*this, P.ptrsm);
+    d_pupd_ = upload(*this, P.pupd);
+    d_wide_ = upload(*this, P.wide_cblks);
+    n_wide_ = (int)P.wide_cblks.size();
     d_bwd_itemptr_ = upload(*this, P.bwd_itemptr);
+    d_yrow_ptr_ = upload(*this, P.yrow_ptr);
+    d_yrow_pos_ = upload(*this, P.yrow_pos);
+    nyrows_ = P.nyrows;
     d_shift_ = upload(*this, P.shift);
     d_trsm_ = upload(*this, P.trsm);
     d_upd_ = upload(*this, P.upd);
     d_ents_ = upload(*this, P.ents);
     d_fwd_ = upload(*this, P.fwd);
     d_bwd_ = upload(*this, P.bwd);
-    fwd_part_ = P.fwd_part;
     bwd_part_ = P.bwd_part;
 
     const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
@@ -117,9 +127,13 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
         d_W2_ = d_W1_;
     }
     d_stat_ = (int *)alloc(4 * sizeof(int));
-    // nrhs chunk bounded by the dynamic LDS of the solve kernels (48 KiB)
-    nrhs_chunk_ = (int)std::max<int64_t>(1, std::min<int64_t>(64, (48 * 1024) / (std::max(maxwidth_, 256) * (int64_t)fsz)));
-    d_part_ = alloc((size_t)std::max<int64_t>(std::max(fwd_part_, bwd_part_), 1) * nrhs_chunk_ * fsz);
+    d_WB_ = alloc(winvnbr_ * fsz);
+    // nrhs chunk: at most 8 (register accumulators of the solve kernels) and
+    // bounded by their dynamic LDS (48 KiB)
+    nrhs_chunk_ = (int)std::max<int64_t>(1, std::min<int64_t>(8, (48 * 1024) / (std::max(maxwidth_, 256) * (int64_t)fsz)));
+    d_part_ = alloc((size_t)std::max<int64_t>(bwd_part_, 1) * nrhs_chunk_ * fsz);
+    d_ybuf_ = alloc((size_t)std::max<int64_t>(nyrows_, 1) * nrhs_chunk_ * fsz);
+    if (!d_WB_ || !d_ybuf_) return -1;
     d_red_ = (double *)alloc((size_t)(512 * 32 + 64) * sizeof(double));
     if (!d_part_ || !d_red_ || !d_stat_) return -1;
     HIPCHK(hipStreamSynchronize(stream_));
@@ -159,20 +173,30 @@ template <typename T> int Engine::factorize_t(double thr) {
     for (size_t lv = 0; lv < nl; lv++) {
         const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
         mark(6 * lv + 0);
-        if (facto_ == 2)
-            launch_getrf_diag<T>(S_, d_lv_cblks_ + c0, nc, L, U, W1, W2, thr, d_stat_, stream_);
-        else
-            launch_potrf_diag<T>(S_, d_lv_cblks_ + c0, nc, L, W1, thr, d_stat_, stream_);
-        mark(6 * lv + 1);
-        const int t0 = lv_trsm_ptr_[lv], nt = lv_trsm_ptr_[lv + 1] - t0;
-        mark(6 * lv + 2);
         if (facto_ == 2) {
+            launch_getrf_diag<T>(S_, d_lv_cblks_ + c0, nc, L, U, W1, W2, thr, d_stat_, stream_);
+            launch_winv_transpose<T>(S_, d_lv_cblks_ + c0, nc, W2, (T *)d_WB_, stream_);
+            mark(6 * lv + 1);
+            const int t0 = lv_trsm_ptr_[lv], nt = lv_trsm_ptr_[lv + 1] - t0;
+            mark(6 * lv + 2);
             launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, L, W2, stream_);
             launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, U, W1, stream_);
+            mark(6 * lv + 3);
         } else {
-            launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, L, W1, stream_);
+            // blocked panel: 64-column steps, each over every cblk of the level
+            for (int s = lv_step_ptr_[lv]; s < lv_step_ptr_[lv + 1]; s++) {
+                const int kb = s - lv_step_ptr_[lv];
+                const int b0 = step_cb_ptr_[s], nb = step_cb_ptr_[s + 1] - b0;
+                launch_potrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, W1, (T *)d_WB_, thr, d_stat_, stream_);
+                const int p0 = step_trsm_ptr_[s], np = step_trsm_ptr_[s + 1] - p0;
+                launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W1, stream_);
+                const int q0 = step_upd_ptr_[s], nq = step_upd_ptr_[s + 1] - q0;
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, stream_);
+            }
+            mark(6 * lv + 1);
+            mark(6 * lv + 2);
+            mark(6 * lv + 3);
         }
-        mark(6 * lv + 3);
         const int u0 = lv_upd_ptr_[lv], nu = lv_upd_ptr_[lv + 1] - u0;
         mark(6 * lv + 4);
         if (facto_ == 2) {
@@ -183,6 +207,8 @@ template <typename T> int Engine::factorize_t(double thr) {
         }
         mark(6 * lv + 5);
     }
+    // full inverses of the wide diagonal blocks, needed only by the solve
+    if (facto_ != 2) launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, L, W1, (T *)d_WB_, stream_);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -275,11 +301,17 @@ int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) 
     return 0;
 }
 
+// Forward sweep by increasing level: gather the update-buffer rows facing the
+// level's cblks and apply inv(L_cc), then write their off-diagonal products to
+// the update buffer.  Backward sweep by decreasing level: transposed products
+// of the off-diagonal panels on row chunks, then inv(L_cc)^T (inv(U_cc)).
 template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
     const T *L = (const T *)d_L_;
     const T *U = (facto_ == 2) ? (const T *)d_U_ : L;
-    const T *W1 = (const T *)d_W1_, *W2 = (const T *)d_W2_;
+    const T *W1 = (const T *)d_W1_, *WB = (const T *)d_WB_;
     T *part = (T *)d_part_;
+    T *ybuf = (T *)d_ybuf_;
+    const int64_t ldy = std::max<int64_t>(nyrows_, 1);
     const int nl = (int)lv_cblk_ptr_.size() - 1;
     for (int r0 = 0; r0 < nrhs; r0 += nrhs_chunk_) {
         const int nr = std::min(nrhs_chunk_, nrhs - r0);
@@ -287,16 +319,15 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
         for (int lv = 0; lv < nl; lv++) {
             const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
             const int f0 = lv_fwd_ptr_[lv], nf = lv_fwd_ptr_[lv + 1] - f0;
-            launch_fwd_gather<T>(S_, d_fwd_ + f0, nf, L, xx, ldx, nr, part, maxwidth_, stream_);
-            // item slots are level-relative; the diag kernel reads items through absolute indices
-            launch_fwd_diag<T>(S_, d_lv_cblks_ + c0, nc, d_fwd_itemptr_ + c0 + lv, d_fwd_, xx, ldx, nr, part, W1,
+            launch_fwd_diag<T>(S_, d_lv_cblks_ + c0, nc, d_yrow_ptr_, d_yrow_pos_, ybuf, ldy, xx, ldx, nr, W1,
                                maxwidth_, stream_);
+            launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, xx, ldx, nr, ybuf, ldy, maxwidth_, stream_);
         }
         for (int lv = nl - 1; lv >= 0; lv--) {
             const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
             const int b0 = lv_bwd_ptr_[lv], nb = lv_bwd_ptr_[lv + 1] - b0;
-            launch_bwd_gather<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, stream_);
-            launch_bwd_diag<T>(S_, d_lv_cblks_ + c0, nc, d_bwd_itemptr_ + c0 + lv, d_bwd_, xx, ldx, nr, part, W2,
+            launch_bwd_prod<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, stream_);
+            launch_bwd_diag<T>(S_, d_lv_cblks_ + c0, nc, d_bwd_itemptr_ + c0 + lv, d_bwd_, xx, ldx, nr, part, WB,
                                maxwidth_, stream_);
         }
     }

@@ -87,8 +87,16 @@ class Engine {
     // host copies of the level pointers
     std::vector<int32_t> lv_cblk_ptr_, lv_trsm_ptr_, lv_upd_ptr_, lv_fwd_ptr_, lv_bwd_ptr_;
     // device plan
-    int32_t *d_lv_cblks_ = nullptr, *d_fwd_itemptr_ = nullptr, *d_bwd_itemptr_ = nullptr, *d_shift_ = nullptr;
+    int32_t *d_lv_cblks_ = nullptr, *d_bwd_itemptr_ = nullptr, *d_shift_ = nullptr;
+    int64_t *d_yrow_ptr_ = nullptr;
+    int32_t *d_yrow_pos_ = nullptr;
+    int64_t nyrows_ = 0;
+    void *d_ybuf_ = nullptr, *d_WB_ = nullptr;
     TrsmItem *d_trsm_ = nullptr;
+    std::vector<int32_t> lv_step_ptr_, step_cb_ptr_, step_trsm_ptr_, step_upd_ptr_;
+    int32_t *d_step_cblks_ = nullptr, *d_wide_ = nullptr;
+    int n_wide_ = 0;
+    PanelItem *d_ptrsm_ = nullptr, *d_pupd_ = nullptr;
     UpdItem *d_upd_ = nullptr;
     UpdEntry *d_ents_ = nullptr;
     FwdItem *d_fwd_ = nullptr;
@@ -96,7 +104,7 @@ class Engine {
     int32_t *d_perm_ = nullptr;  // permtab as int32 for device permutations
     void *d_L0_ = nullptr, *d_U0_ = nullptr;  // snapshot of the filled coefficients
     std::vector<hipEvent_t> pev_;
-    int64_t fwd_part_ = 0, bwd_part_ = 0;
+    int64_t bwd_part_ = 0;
     // numerics
     void *d_L_ = nullptr, *d_U_ = nullptr, *d_W1_ = nullptr, *d_W2_ = nullptr;
     int *d_stat_ = nullptr;

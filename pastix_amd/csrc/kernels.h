@@ -18,19 +18,36 @@ template <typename T>
 void launch_update(const DevSymbol &S, int mode, const UpdItem *items, int n, const UpdEntry *ents,
                    const int32_t *shift, const T *A, const T *B, T *C, hipStream_t st);
 
+// Blocked panel factorisation (64-column steps).
+template <typename T>
+void launch_potrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *coef, T *winv, T *winvt, double thr,
+                    int *stat, hipStream_t st);
+template <typename T>
+void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, const T *winv,
+                       hipStream_t st);
+template <typename T>
+void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, hipStream_t st);
+template <typename T>
+void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool transposed_factor, const T *coef,
+                         T *winv, T *winvt, hipStream_t st);
+
 // Supernodal triangular solves (reference kernels/solve_ztrsmsp.c:332-557).
+// nrhs <= 8 per launch.
 template <typename T>
-void launch_fwd_gather(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
-                       int nrhs, T *part, int maxwidth, hipStream_t st);
+void launch_fwd_diag(const DevSymbol &S, const int32_t *cblks, int n, const int64_t *yrow_ptr,
+                     const int32_t *yrow_pos, const T *ybuf, int64_t ldy, T *x, int64_t ldx, int nrhs, const T *w1,
+                     int maxwidth, hipStream_t st);
 template <typename T>
-void launch_fwd_diag(const DevSymbol &S, const int32_t *cblks, int n, const int32_t *itemptr, const FwdItem *items,
-                     T *x, int64_t ldx, int nrhs, const T *part, const T *w1, int maxwidth, hipStream_t st);
+void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
+                     int nrhs, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st);
 template <typename T>
-void launch_bwd_gather(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
-                       int nrhs, T *part, hipStream_t st);
+void launch_bwd_prod(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
+                     int nrhs, T *part, hipStream_t st);
 template <typename T>
 void launch_bwd_diag(const DevSymbol &S, const int32_t *cblks, int n, const int32_t *itemptr, const BwdItem *items,
-                     T *x, int64_t ldx, int nrhs, const T *part, const T *w2, int maxwidth, hipStream_t st);
+                     T *x, int64_t ldx, int nrhs, const T *part, const T *wb, int maxwidth, hipStream_t st);
+template <typename T>
+void launch_winv_transpose(const DevSymbol &S, const int32_t *cblks, int n, const T *w, T *wt, hipStream_t st);
 
 // Sparse matrix-vector product on the permuted block CSC
 // (reference bcsc/bcsc_zspmv.c:55-202): y = alpha*A*x + beta*y.

@@ -1,0 +1,278 @@
+// Blocked right-looking panel factorisation for CDNA4 (gfx950).
+//
+// A cblk's panel (stride x K, column-major) is factored by 64-column steps,
+// each step being three launches over all cblks of the level:
+//   k_potrf64   : Cholesky of the 64x64 diagonal sub-block with static
+//                 pivoting (reference kernels/core_zpotrfsp.c:66-108, one LDS
+//                 barrier per column), plus its inverse;
+//   k_panel_trsm: every panel row below the sub-block times inv(L_kk)^T
+//                 (core_zpotrfsp.c:166-176 and core_ztrsmsp.c:147-181);
+//   k_panel_upd : trailing update of the panel's remaining columns
+//                 (core_zpotrfsp.c:178-185), lower 64x64 tiles only.
+// k_winv_offdiag completes the inverse of the whole diagonal block for the
+// solve once the panel is factored.  Inverses are stored twice: W (lower,
+// column-major) for the forward sweep and WT = W^T for the backward sweep.
+#include "dev_common.h"
+#include "dev_types.h"
+#include "kernels.h"
+
+namespace pxa {
+
+#define QFOR(...)                                                        \
+    for (int m_ = 0; m_ < 2; m_++)                                       \
+        for (int n_ = 0; n_ < 2; n_++)                                   \
+            for (int r_ = 0; r_ < 4; r_++) {                             \
+                const int qrow = m_ * 16 + Mfma<T>::drow(lane, r_);       \
+                const int qcol = n_ * 16 + (lane & 15);                   \
+                __VA_ARGS__                                              \
+            }
+
+template <typename T> __device__ __forceinline__ T pabs(T v) { return v < (T)0 ? -v : v; }
+
+// Cholesky of the nb x nb lower block in Sm.  Step k reads the final column
+// k, updates the columns on its right with a_ik a_jk / d_k and scales column
+// k-1, so a single barrier per column suffices.
+template <typename T>
+__device__ void lds_potf2(T (*Sm)[65], int nb, T crit, int tid, int &npiv, int &neg) {
+    T sprev = 1;
+    for (int k = 0; k < nb; k++) {
+        T d = Sm[k][k];
+        const bool piv = pabs(d) < crit;
+        if (piv) d = crit;
+        const bool ng = d < (T)0;
+        if (ng) d = -d;
+        const T invd = (T)1 / d;
+        if (k > 0) {
+            const int kk = k - 1;
+            const T is = (T)1 / sprev;
+            for (int e = tid; e < nb - kk; e += 256) {
+                if (e == 0) Sm[kk][kk] = sprev;
+                else Sm[kk + e][kk] *= is;
+            }
+        }
+        // rank-1 update of the trailing lower triangle: thread (row, column
+        // phase) = (tid % 64, tid / 64), columns k+1+ph, k+5+ph, ... <= row
+        {
+            const int i = tid & 63, ph = tid >> 6;
+            if (i > k && i < nb) {
+                const T aik = Sm[i][k] * invd;
+                for (int j = k + 1 + ph; j <= i; j += 4) Sm[i][j] -= aik * Sm[j][k];
+            }
+        }
+        if (tid == 0) {
+            npiv += piv;
+            neg |= ng;
+        }
+        sprev = sqrt(d);
+        __syncthreads();
+    }
+    if (tid == 0 && nb > 0) Sm[nb - 1][nb - 1] = sprev;
+    __syncthreads();
+}
+
+// X = inv(L) for the nb x nb lower block in Sm (L stays in the lower part):
+// X[i][j] (j < i) goes to Sm[j][i], X[i][i] to dinv[i].  Column j of X is a
+// forward substitution owned by thread j (reads of L are broadcasts, the
+// thread's own column lives in row j of the upper triangle): no barrier
+// inside the substitution.
+template <typename T>
+__device__ void lds_trtri_cols(T (*Sm)[65], T *dinv, int nb, int tid) {
+    if (tid < nb) dinv[tid] = (T)1 / Sm[tid][tid];
+    __syncthreads();
+    if (tid < nb) {
+        const int j = tid;
+        for (int i = j + 1; i < nb; i++) {
+            T s = Sm[i][j] * dinv[j];
+            for (int k = j + 1; k < i; k++) s += Sm[i][k] * Sm[j][k];
+            Sm[j][i] = -dinv[i] * s;
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
+                                                 T *__restrict__ coef, T *__restrict__ winv, T *__restrict__ winvt,
+                                                 double thr, int *stat) {
+    __shared__ T Sm[64][65];
+    __shared__ T dinv[64];
+    const int tid = threadIdx.x;
+    const int c = cblks[blockIdx.x];
+    const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+    const int k0 = kb * 64;
+    const int nb = min(64, K - k0);
+    const int64_t ld = S.cblk_stride[c];
+    T *P = coef + S.cblk_coef[c];
+    T *W = winv + S.cblk_winv[c];
+    T *WT = winvt + S.cblk_winv[c];
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int i = e & 63, j = e >> 6;
+        Sm[i][j] = (i < nb && j < nb && i >= j) ? P[(k0 + i) + (int64_t)(k0 + j) * ld] : (T)0;
+    }
+    __syncthreads();
+    int npiv = 0, neg = 0;
+    lds_potf2<T>(Sm, nb, (T)thr, tid, npiv, neg);
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int i = e & 63, j = e >> 6;
+        if (i < nb && j < nb && i >= j) P[(k0 + i) + (int64_t)(k0 + j) * ld] = Sm[i][j];
+    }
+    lds_trtri_cols<T>(Sm, dinv, nb, tid);
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int i = e & 63, j = e >> 6;
+        if (i < nb && j < nb && i >= j) {
+            const T v = (i == j) ? dinv[i] : Sm[j][i];
+            W[(k0 + i) + (int64_t)(k0 + j) * K] = v;
+            WT[(k0 + j) + (int64_t)(k0 + i) * K] = v;
+        }
+    }
+    if (tid == 0) {
+        if (npiv) atomicAdd(&stat[0], npiv);
+        if (neg) atomicOr(&stat[1], 1);
+    }
+}
+
+// rows [r0, r0+64) below the diagonal sub-block kb: X = A * W_kk^T
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
+                                                    int kb, T *__restrict__ coef, const T *__restrict__ winv) {
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const PanelItem it = items[id];
+    const int c = it.c, r0 = it.r0;
+    const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+    const int stride = S.cblk_stride[c];
+    const int64_t ld = stride;
+    const int k0 = kb * 64, nb = min(64, K - k0);
+    T *P = coef + S.cblk_coef[c];
+    const T *W = winv + S.cblk_winv[c];
+    const int mr = min(64, stride - r0);
+    Quad<T> q;
+    q.zero();
+    if (wr < mr && wc < nb) {
+        View<T> A{P + (r0 + wr) + (int64_t)k0 * ld, 1, ld};
+        View<T> B{W + (k0 + wc) + (int64_t)k0 * K, 1, K};
+        quad_gemm_nt(q, A, mr - wr, B, nb - wc, nb, lane);
+    }
+    __syncthreads();
+    if (wr < mr && wc < nb) {
+        QFOR(if (qrow < mr - wr && qcol < nb - wc) P[(r0 + wr + qrow) + (int64_t)(k0 + wc + qcol) * ld] = q.c[m_][n_][r_];)
+    }
+}
+
+// trailing panel update: P[r0+i, c0+j] -= X[r0+i, kb] . X[c0+j, kb]
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
+                                                   int kb, T *__restrict__ coef) {
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const PanelItem it = items[id];
+    const int c = it.c, r0 = it.r0, c0 = it.c0;
+    const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+    const int stride = S.cblk_stride[c];
+    const int64_t ld = stride;
+    const int k0 = kb * 64, nb = min(64, K - k0);
+    T *P = coef + S.cblk_coef[c];
+    const int mr = min(64, stride - r0), mc = min(64, K - c0);
+    if (wr < mr && wc < mc) {
+        Quad<T> q;
+        q.zero();
+        View<T> A{P + (r0 + wr) + (int64_t)k0 * ld, 1, ld};
+        View<T> B{P + (c0 + wc) + (int64_t)k0 * ld, 1, ld};
+        quad_gemm_nt(q, A, mr - wr, B, mc - wc, nb, lane);
+        QFOR(if (qrow < mr - wr && qcol < mc - wc) P[(r0 + wr + qrow) + (int64_t)(c0 + wc + qcol) * ld] -= q.c[m_][n_][r_];)
+    }
+}
+
+// Off-diagonal 64-blocks of inv(L_cc) (and of its transpose) for the solve:
+//   W[ib,jb] = -W[ib,ib] * sum_{kb=jb}^{ib-1} L[ib,kb] W[kb,jb]
+template <typename T>
+__global__ __launch_bounds__(256) void k_winv_offdiag(DevSymbol S, const int32_t *__restrict__ cblks, int lstride,
+                                                      int lrs, int lcs_is_ld, const T *__restrict__ coef,
+                                                      T *__restrict__ winv, T *__restrict__ winvt) {
+    __shared__ T Sm[64][65];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int c = cblks[blockIdx.x];
+    const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+    const int64_t ld = S.cblk_stride[c];
+    const T *P = coef + S.cblk_coef[c];
+    // L(r, k) = P[r + k*ld] (lower factor) or P[k + r*ld] (transposed storage)
+    const int64_t rs = lcs_is_ld ? ld : 1, cs = lcs_is_ld ? 1 : ld;
+    (void)lstride;
+    (void)lrs;
+    T *W = winv + S.cblk_winv[c];
+    T *WT = winvt + S.cblk_winv[c];
+    for (int jb0 = 0; jb0 < K; jb0 += 64) {
+        const int nj = min(64, K - jb0);
+        for (int ib0 = jb0 + 64; ib0 < K; ib0 += 64) {
+            const int ni = min(64, K - ib0);
+            Quad<T> q;
+            q.zero();
+            if (wr < ni && wc < nj) {
+                View<T> A{P + (int64_t)(ib0 + wr) * rs + (int64_t)jb0 * cs, rs, cs};
+                View<T> B{W + jb0 + (int64_t)(jb0 + wc) * K, K, 1};
+                quad_gemm_nt(q, A, ni - wr, B, nj - wc, ib0 - jb0, lane);
+            }
+            QFOR(Sm[wr + qrow][wc + qcol] = q.c[m_][n_][r_];)
+            __syncthreads();
+            Quad<T> q2;
+            q2.zero();
+            if (wr < ni && wc < nj) {
+                View<T> A{W + (ib0 + wr) + (int64_t)ib0 * K, 1, K};
+                View<T> B{&Sm[0][wc], 1, 65};
+                quad_gemm_nt(q2, A, ni - wr, B, nj - wc, ni, lane);
+                QFOR(if (qrow < ni - wr && qcol < nj - wc) {
+                    const int gi = ib0 + wr + qrow, gj = jb0 + wc + qcol;
+                    const T v = -q2.c[m_][n_][r_];
+                    W[gi + (int64_t)gj * K] = v;
+                    WT[gj + (int64_t)gi * K] = v;
+                })
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ------------------------------- launchers ---------------------------------
+static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
+
+template <typename T>
+void launch_potrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *coef, T *winv, T *winvt, double thr,
+                    int *stat, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_potrf64<T>, dim3(n), dim3(256), 0, st, S, cblks, kb, coef, winv, winvt, thr, stat);
+}
+template <typename T>
+void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, const T *winv,
+                       hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_panel_trsm<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, coef, winv);
+}
+template <typename T>
+void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, coef);
+}
+template <typename T>
+void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool transposed_factor, const T *coef,
+                         T *winv, T *winvt, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_winv_offdiag<T>, dim3(n), dim3(256), 0, st, S, cblks, 0, 0, transposed_factor ? 1 : 0, coef,
+                       winv, winvt);
+}
+
+#define INST(T)                                                                                                    \
+    template void launch_potrf64<T>(const DevSymbol &, const int32_t *, int, int, T *, T *, T *, double, int *,    \
+                                    hipStream_t);                                                                  \
+    template void launch_panel_trsm<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, hipStream_t); \
+    template void launch_panel_upd<T>(const DevSymbol &, const PanelItem *, int, int, T *, hipStream_t);          \
+    template void launch_winv_offdiag<T>(const DevSymbol &, const int32_t *, int, bool, const T *, T *, T *,       \
+                                         hipStream_t);
+INST(double)
+INST(float)
+
+}  // namespace pxa

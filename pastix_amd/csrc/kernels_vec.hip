@@ -1,12 +1,6 @@
-// Solve, SpMV and vector kernels for CDNA4 (gfx950).
-//
-// Forward sweep (reference kernels/solve_ztrsmsp.c:332-410): per level,
-// k_fwd_gather sums the contributions L[rows facing c] * x[source] of the
-// blocks in browtab(c) into per-item partial vectors, then k_fwd_diag
-// subtracts them in item order and applies inv(L_cc).  Backward sweep
-// (:459-557): k_bwd_gather computes L[off-diagonal rows of c]^T * x[rows] on
-// row chunks, k_bwd_diag subtracts them and applies inv(L_cc)^T (inv(U_cc) for
-// LU).  Every output is written by one workgroup with a fixed summation order.
+// SpMV, vector (BLAS-1) and permutation kernels for CDNA4 (gfx950).
+// Reductions use a fixed grid and a fixed combination order, so results are
+// bitwise reproducible from run to run.
 #include "dev_common.h"
 #include "dev_types.h"
 #include "kernels.h"
@@ -14,133 +8,6 @@
 namespace pxa {
 
 static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_gather(DevSymbol S, const FwdItem *__restrict__ items, int nitems,
-                                                    const T *__restrict__ coef, const T *__restrict__ x, int64_t ldx,
-                                                    int nrhs, T *__restrict__ part) {
-    extern __shared__ char smem_raw[];
-    T *y = reinterpret_cast<T *>(smem_raw);
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    if (id >= nitems) return;
-    const int tid = threadIdx.x;
-    const FwdItem it = items[id];
-    const int c = it.c;
-    const int fc = S.cblk_fcol[c];
-    const int Kc = S.cblk_fcol[c + 1] - fc;
-    for (int e = tid; e < Kc * nrhs; e += 256) y[e] = (T)0;
-    __syncthreads();
-    for (int e = it.e0; e < it.e1; e++) {
-        const int b = S.browtab[e];
-        const int s = S.blok_lcblk[b];
-        const int nr = S.blok_lrow[b] - S.blok_frow[b] + 1;
-        const int off = S.blok_frow[b] - fc;
-        const int fs = S.cblk_fcol[s];
-        const int Ks = S.cblk_fcol[s + 1] - fs;
-        const int64_t lds = S.cblk_stride[s];
-        const T *A = coef + S.cblk_coef[s] + S.blok_roff[b];
-        for (int t = tid; t < nr * nrhs; t += 256) {
-            const int row = t % nr, rhs = t / nr;
-            const T *xs = x + fs + rhs * ldx;
-            T acc = 0;
-            for (int k = 0; k < Ks; k++) acc += A[row + k * lds] * xs[k];
-            y[off + row + rhs * Kc] += acc;
-        }
-        __syncthreads();
-    }
-    T *out = part + (int64_t)it.slot * nrhs;
-    for (int e = tid; e < Kc * nrhs; e += 256) out[e] = y[e];
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_diag(DevSymbol S, const int32_t *__restrict__ cblks,
-                                                  const int32_t *__restrict__ itemptr,
-                                                  const FwdItem *__restrict__ items, T *__restrict__ x, int64_t ldx,
-                                                  int nrhs, const T *__restrict__ part, const T *__restrict__ w1) {
-    extern __shared__ char smem_raw[];
-    T *y = reinterpret_cast<T *>(smem_raw);
-    const int tid = threadIdx.x;
-    const int c = cblks[blockIdx.x];
-    const int fc = S.cblk_fcol[c];
-    const int Kc = S.cblk_fcol[c + 1] - fc;
-    const int i0 = itemptr[blockIdx.x], i1 = itemptr[blockIdx.x + 1];
-    for (int t = tid; t < Kc * nrhs; t += 256) {
-        const int i = t % Kc, rhs = t / Kc;
-        T v = x[fc + i + rhs * ldx];
-        for (int k = i0; k < i1; k++) v -= part[(int64_t)items[k].slot * nrhs + t];
-        y[t] = v;
-    }
-    __syncthreads();
-    const T *W = w1 + S.cblk_winv[c];
-    for (int t = tid; t < Kc * nrhs; t += 256) {
-        const int i = t % Kc, rhs = t / Kc;
-        const T *yy = y + rhs * Kc;
-        T s = 0;
-        for (int k = 0; k <= i; k++) s += W[i + (int64_t)k * Kc] * yy[k];
-        x[fc + i + rhs * ldx] = s;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_gather(DevSymbol S, const BwdItem *__restrict__ items, int nitems,
-                                                    const T *__restrict__ coef, const T *__restrict__ x, int64_t ldx,
-                                                    int nrhs, T *__restrict__ part) {
-    extern __shared__ char smem_raw[];
-    T *xr = reinterpret_cast<T *>(smem_raw);
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    if (id >= nitems) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const BwdItem it = items[id];
-    const int c = it.c;
-    const int Kc = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
-    const int64_t ld = S.cblk_stride[c];
-    const int64_t rb = S.cblk_rowbase[c];
-    const int nr = it.r1 - it.r0;
-    const T *P = coef + S.cblk_coef[c] + it.r0;
-    for (int t = tid; t < nr * nrhs; t += 256) {
-        const int row = t % nr, rhs = t / nr;
-        xr[t] = x[S.panel_grow[rb + it.r0 + row] + rhs * ldx];
-    }
-    __syncthreads();
-    for (int j = wave; j < Kc; j += 4) {
-        for (int rhs = 0; rhs < nrhs; rhs++) {
-            T acc = 0;
-            for (int t = lane; t < nr; t += 64) acc += P[t + (int64_t)j * ld] * xr[t + rhs * nr];
-            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-            if (lane == 0) part[(int64_t)it.slot * nrhs + j + rhs * Kc] = acc;
-        }
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_diag(DevSymbol S, const int32_t *__restrict__ cblks,
-                                                  const int32_t *__restrict__ itemptr,
-                                                  const BwdItem *__restrict__ items, T *__restrict__ x, int64_t ldx,
-                                                  int nrhs, const T *__restrict__ part, const T *__restrict__ w2) {
-    extern __shared__ char smem_raw[];
-    T *y = reinterpret_cast<T *>(smem_raw);
-    const int tid = threadIdx.x;
-    const int c = cblks[blockIdx.x];
-    const int fc = S.cblk_fcol[c];
-    const int Kc = S.cblk_fcol[c + 1] - fc;
-    const int i0 = itemptr[blockIdx.x], i1 = itemptr[blockIdx.x + 1];
-    for (int t = tid; t < Kc * nrhs; t += 256) {
-        const int i = t % Kc, rhs = t / Kc;
-        T v = x[fc + i + rhs * ldx];
-        for (int k = i0; k < i1; k++) v -= part[(int64_t)items[k].slot * nrhs + t];
-        y[t] = v;
-    }
-    __syncthreads();
-    const T *W = w2 + S.cblk_winv[c];
-    for (int t = tid; t < Kc * nrhs; t += 256) {
-        const int i = t % Kc, rhs = t / Kc;
-        const T *yy = y + rhs * Kc;
-        const T *Wc = W + (int64_t)i * Kc;
-        T s = 0;
-        for (int k = i; k < Kc; k++) s += Wc[k] * yy[k];
-        x[fc + i + rhs * ldx] = s;
-    }
-}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restrict__ colptr,
@@ -293,34 +160,6 @@ __global__ __launch_bounds__(256) void k_permute(int64_t m, int nrhs, const int3
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 template <typename T>
-void launch_fwd_gather(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
-                       int nrhs, T *part, int maxwidth, hipStream_t st) {
-    if (n <= 0) return;
-    size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
-    hipLaunchKernelGGL(k_fwd_gather<T>, dim3(grid8(n)), dim3(256), sh, st, S, items, n, coef, x, ldx, nrhs, part);
-}
-template <typename T>
-void launch_fwd_diag(const DevSymbol &S, const int32_t *cblks, int n, const int32_t *itemptr, const FwdItem *items,
-                     T *x, int64_t ldx, int nrhs, const T *part, const T *w1, int maxwidth, hipStream_t st) {
-    if (n <= 0) return;
-    size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
-    hipLaunchKernelGGL(k_fwd_diag<T>, dim3(n), dim3(256), sh, st, S, cblks, itemptr, items, x, ldx, nrhs, part, w1);
-}
-template <typename T>
-void launch_bwd_gather(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
-                       int nrhs, T *part, hipStream_t st) {
-    if (n <= 0) return;
-    size_t sh = (size_t)256 * nrhs * sizeof(T);
-    hipLaunchKernelGGL(k_bwd_gather<T>, dim3(grid8(n)), dim3(256), sh, st, S, items, n, coef, x, ldx, nrhs, part);
-}
-template <typename T>
-void launch_bwd_diag(const DevSymbol &S, const int32_t *cblks, int n, const int32_t *itemptr, const BwdItem *items,
-                     T *x, int64_t ldx, int nrhs, const T *part, const T *w2, int maxwidth, hipStream_t st) {
-    if (n <= 0) return;
-    size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
-    hipLaunchKernelGGL(k_bwd_diag<T>, dim3(n), dim3(256), sh, st, S, cblks, itemptr, items, x, ldx, nrhs, part, w2);
-}
-template <typename T>
 void launch_spmv(int64_t n, const int64_t *colptr, const int32_t *rows, const T *vals, T alpha, const T *x, T beta,
                  T *y, hipStream_t st) {
     if (n <= 0) return;
@@ -374,14 +213,6 @@ void launch_berr_parts(int64_t n, const int64_t *colptr, const int32_t *rows, co
 }
 
 #define INST(T)                                                                                               \
-    template void launch_fwd_gather<T>(const DevSymbol &, const FwdItem *, int, const T *, const T *, int64_t, \
-                                       int, T *, int, hipStream_t);                                           \
-    template void launch_fwd_diag<T>(const DevSymbol &, const int32_t *, int, const int32_t *, const FwdItem *, \
-                                     T *, int64_t, int, const T *, const T *, int, hipStream_t);              \
-    template void launch_bwd_gather<T>(const DevSymbol &, const BwdItem *, int, const T *, const T *, int64_t, \
-                                       int, T *, hipStream_t);                                                \
-    template void launch_bwd_diag<T>(const DevSymbol &, const int32_t *, int, const int32_t *, const BwdItem *, \
-                                     T *, int64_t, int, const T *, const T *, int, hipStream_t);              \
     template void launch_spmv<T>(int64_t, const int64_t *, const int32_t *, const T *, T, const T *, T, T *,   \
                                  hipStream_t);                                                                \
     template void launch_permute<T>(int64_t, int, const int32_t *, bool, const T *, int64_t, T *, int64_t,     \

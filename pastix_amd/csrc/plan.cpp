@@ -78,15 +78,58 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     P.lv_upd_ptr.assign(nl + 1, 0);
     P.lv_fwd_ptr.assign(nl + 1, 0);
     P.lv_bwd_ptr.assign(nl + 1, 0);
-    P.fwd_itemptr.assign(cb + nl, 0);
     P.bwd_itemptr.assign(cb + nl, 0);
+    // update buffer: one row per off-diagonal panel row; CSR by permuted row
+    std::vector<int64_t> yoff(cb + 1, 0);
+    for (int64_t c = 0; c < cb; c++) yoff[c + 1] = yoff[c] + (L.cblk_stride[c] - width(c));
+    P.nyrows = yoff[cb];
+    P.yrow_ptr.assign(s.n + 1, 0);
+    for (int64_t c = 0; c < cb; c++)
+        for (int64_t r = width(c); r < L.cblk_stride[c]; r++) P.yrow_ptr[P.panel_grow[P.rowbase[c] + r] + 1]++;
+    for (int64_t g = 0; g < s.n; g++) P.yrow_ptr[g + 1] += P.yrow_ptr[g];
+    P.yrow_pos.assign(P.nyrows, 0);
+    {
+        std::vector<int64_t> pos(P.yrow_ptr.begin(), P.yrow_ptr.end() - 1);
+        for (int64_t c = 0; c < cb; c++)
+            for (int64_t r = width(c); r < L.cblk_stride[c]; r++)
+                P.yrow_pos[pos[P.panel_grow[P.rowbase[c] + r]]++] = (int32_t)(yoff[c] + r - width(c));
+    }
     std::vector<KeyEnt> ke;
     std::vector<Range> rr, cr;
-    const int64_t FWD_LIMIT = 1 << 16;  // factor entries read by one forward item
-    const int32_t BWD_ROWS = 256;       // panel rows read by one backward item
+    const int32_t FWD_ROWS = 64;  // panel rows of one forward product tile
+    const int32_t BWD_ROWS = 64;  // panel rows of one backward product tile
+
+    for (int64_t c = 0; c < cb; c++)
+        if (width(c) > 64) P.wide_cblks.push_back((int32_t)c);
+    P.lv_step_ptr.assign(nl + 1, 0);
+    P.step_cb_ptr.push_back(0);
+    P.step_trsm_ptr.push_back(0);
+    P.step_upd_ptr.push_back(0);
 
     for (int32_t lv = 0; lv < nl; lv++) {
         const int32_t l0 = P.lv_cblk_ptr[lv], l1 = P.lv_cblk_ptr[lv + 1];
+        // blocked panel steps of this level
+        int64_t maxk = 0;
+        for (int32_t k = l0; k < l1; k++) maxk = std::max<int64_t>(maxk, width(P.lv_cblks[k]));
+        const int64_t nsteps = (maxk + 63) / 64;
+        for (int64_t kb = 0; kb < nsteps; kb++) {
+            const int64_t k0 = kb * 64;
+            for (int32_t k = l0; k < l1; k++) {
+                const int32_t c = P.lv_cblks[k];
+                const int64_t K = width(c), st = L.cblk_stride[c];
+                if (K <= k0) continue;
+                const int64_t nb = std::min<int64_t>(64, K - k0);
+                P.step_cblks.push_back(c);
+                for (int64_t r0 = k0 + nb; r0 < st; r0 += 64) P.ptrsm.push_back(PanelItem{c, (int32_t)r0, 0, 0});
+                for (int64_t c0 = k0 + nb; c0 < K; c0 += 64)
+                    for (int64_t r0 = c0; r0 < st; r0 += 64)
+                        P.pupd.push_back(PanelItem{c, (int32_t)r0, (int32_t)c0, 0});
+            }
+            P.step_cb_ptr.push_back((int32_t)P.step_cblks.size());
+            P.step_trsm_ptr.push_back((int32_t)P.ptrsm.size());
+            P.step_upd_ptr.push_back((int32_t)P.pupd.size());
+        }
+        P.lv_step_ptr[lv + 1] = P.lv_step_ptr[lv] + (int32_t)nsteps;
         // TRSM tiles
         for (int32_t k = l0; k < l1; k++) {
             int32_t c = P.lv_cblks[k];
@@ -167,30 +210,16 @@ Plan build_plan(const Symbol &s, const Layout &L) {
         }
         P.lv_upd_ptr[lv + 1] = (int32_t)P.upd.size();
 
-        // forward gather items for the targets of this level
-        {
-            int64_t slot = 0;
-            const int32_t seg = l0 + lv;
-            for (int32_t k = l0; k < l1; k++) {
-                const int64_t c = P.lv_cblks[k];
-                P.fwd_itemptr[seg + (k - l0)] = (int32_t)P.fwd.size();
-                int64_t e = L.cblk_brow[c];
-                const int64_t ee = L.cblk_brow[c + 1];
-                while (e < ee) {
-                    int64_t e0 = e, cost = 0;
-                    while (e < ee && (e == e0 || cost < FWD_LIMIT)) {
-                        int64_t bb = L.browtab[e];
-                        cost += rows(bb) * width(L.blok_lcblk[bb]);
-                        e++;
-                    }
-                    P.fwd.push_back(FwdItem{(int32_t)c, (int32_t)e0, (int32_t)e, (int32_t)slot});
-                    slot += width(c);
-                }
+        // forward products of the sources of this level
+        for (int32_t k = l0; k < l1; k++) {
+            const int64_t c = P.lv_cblks[k];
+            const int64_t K = width(c), st = L.cblk_stride[c];
+            for (int64_t r0 = K; r0 < st; r0 += FWD_ROWS) {
+                int64_t r1 = std::min<int64_t>(st, r0 + FWD_ROWS);
+                P.fwd.push_back(FwdItem{(int32_t)c, (int32_t)r0, (int32_t)r1, (int32_t)(yoff[c] + r0 - K)});
             }
-            P.fwd_itemptr[seg + (l1 - l0)] = (int32_t)P.fwd.size();
-            P.fwd_part = std::max(P.fwd_part, slot);
-            P.lv_fwd_ptr[lv + 1] = (int32_t)P.fwd.size();
         }
+        P.lv_fwd_ptr[lv + 1] = (int32_t)P.fwd.size();
         // backward gather items for the cblks of this level
         {
             int64_t slot = 0;

@@ -16,20 +16,30 @@ struct Plan {
     // panel TRSM tiles
     std::vector<int32_t> lv_trsm_ptr;
     std::vector<TrsmItem> trsm;
+    // blocked panel steps: level L owns steps [lv_step_ptr[L], lv_step_ptr[L+1]),
+    // step s is column block kb = s - lv_step_ptr[L] of the level's cblks
+    std::vector<int32_t> lv_step_ptr;
+    std::vector<int32_t> step_cb_ptr, step_cblks;
+    std::vector<int32_t> step_trsm_ptr, step_upd_ptr;
+    std::vector<PanelItem> ptrsm, pupd;
+    std::vector<int32_t> wide_cblks;  // cblks wider than 64 columns
     // update tiles: items of level L come from the sources factored at level L
     std::vector<int32_t> lv_upd_ptr;
     std::vector<UpdItem> upd;
     std::vector<UpdEntry> ents;
     std::vector<int32_t> shift;
-    // forward gather items (targets of level L) and per-cblk item ranges
+    // forward products (sources of level L) into the update buffer, and the
+    // CSR from every permuted row to its update-buffer entries
     std::vector<int32_t> lv_fwd_ptr;
     std::vector<FwdItem> fwd;
-    std::vector<int32_t> fwd_itemptr;  // segment of level L starts at lv_cblk_ptr[L] + L
+    int64_t nyrows = 0;                  // update-buffer rows = all off-diagonal panel rows
+    std::vector<int64_t> yrow_ptr;       // n+1
+    std::vector<int32_t> yrow_pos;       // nyrows, sorted by source cblk per row
     // backward gather items
     std::vector<int32_t> lv_bwd_ptr;
     std::vector<BwdItem> bwd;
     std::vector<int32_t> bwd_itemptr;
-    int64_t fwd_part = 0, bwd_part = 0;  // partial buffer sizes per right-hand side
+    int64_t bwd_part = 0;  // partial buffer size per right-hand side
     // panel row maps and inverse offsets
     std::vector<int32_t> panel_grow, panel_blk;
     std::vector<int64_t> rowbase, winv_off;
