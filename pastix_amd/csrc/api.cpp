@@ -866,6 +866,35 @@ extern "C" int pastix_amd_flops(const pastix_data_t *d, double out[4]) {
     out[3] = lu ? d->plan.flops_upd_lu : d->plan.flops_upd_potrf;
     return PASTIX_SUCCESS;
 }
+// Per-level statistics of the update schedule (5 doubles per level): tiles,
+// entries, useful GEMM flops, MFMA flops issued (32x32 wave quads, K rounded
+// to 4), source width sum.  Returns the number of levels written.
+extern "C" int pastix_amd_plan_stats(const pastix_data_t *d, double *out, int maxlevels) {
+    if (!d || !(d->steps & STEP_ANALYSE)) return -1;
+    const pxa::Plan &P = d->plan;
+    const pxa::Symbol &s = d->symbol;
+    int nl = std::min<int>(maxlevels, P.nlevels);
+    for (int lv = 0; lv < nl; lv++) {
+        double st[5] = {0, 0, 0, 0, 0};
+        for (int t = P.lv_upd_ptr[lv]; t < P.lv_upd_ptr[lv + 1]; t++) {
+            const pxa::UpdItem &it = P.upd[t];
+            st[0] += 1;
+            for (int e = it.e0; e < it.e1; e++) {
+                const pxa::UpdEntry &en = P.ents[e];
+                const double K = (double)(s.cblk_fcol[en.s + 1] - s.cblk_fcol[en.s]);
+                const double np = en.p1 - en.p0, nq = en.q1 - en.q0;
+                st[1] += 1;
+                st[2] += 2.0 * np * nq * K;
+                const double qp = std::ceil(np / 32.0) * 32.0, qq = std::ceil(nq / 32.0) * 32.0;
+                st[3] += 2.0 * qp * qq * std::ceil(K / 4.0) * 4.0;
+                st[4] += K;
+            }
+        }
+        for (int k = 0; k < 5; k++) out[5 * lv + k] = st[k];
+    }
+    return nl;
+}
+
 extern "C" int pastix_amd_refactorize(pastix_data_t *d) {
     if (!d || !d->engine || !(d->steps & STEP_BCSC2CTAB)) return PASTIX_ERR_BADPARAMETER;
     if (d->engine->restore_coef()) return PASTIX_ERR_BADPARAMETER;

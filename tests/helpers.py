@@ -42,6 +42,36 @@ def oracle_factor(A, sym, lu=False, crit=None):
     return O, bc, L, U
 
 
+def read_rua(path):
+    """Harwell-Boeing real unsymmetric assembled matrix (fixed-width Fortran fields)."""
+    import re
+    import scipy.sparse as sps
+    lines = open(path).read().split("\n")
+    totcrd, ptrcrd, indcrd, valcrd = [int(v) for v in lines[1].split()[:4]]
+    nrow, ncol, nnz = [int(v) for v in lines[2][14:].split()[:3]]
+    fmts = re.findall(r"\(([^)]*)\)", lines[3])
+
+    def width(f):
+        m = re.match(r"(?:\d+P)?(\d+)[IEDF](\d+)", f.upper())
+        return int(m.group(1)), int(m.group(2))
+
+    def read(start, ncard, fmt, count, conv):
+        per, w = width(fmt)
+        out = []
+        for line in lines[start:start + ncard]:
+            for k in range(per):
+                s = line[k * w:(k + 1) * w].strip()
+                if s:
+                    out.append(conv(s.replace("D", "E")))
+        return out[:count]
+
+    s = 4
+    ptr = read(s, ptrcrd, fmts[0], ncol + 1, int)
+    ind = read(s + ptrcrd, indcrd, fmts[1], nnz, int)
+    val = read(s + ptrcrd + indcrd, valcrd, fmts[2], nnz, float)
+    return sps.csc_matrix((np.array(val), np.array(ind) - 1, np.array(ptr) - 1), shape=(nrow, ncol))
+
+
 def rel_frob(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 

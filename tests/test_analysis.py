@@ -1,0 +1,152 @@
+"""Host analysis (ordering, symbolic factorisation, splitting) through the C-ABI.
+
+Pinned by the hand-derived 3x3x3 grid ordering of the reference rules
+(tests/golden/grid_order_3x3x3.json), by the nested-dissection invariants, and
+by the fill of a dense Cholesky (the symbol must contain every nonzero of L).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from pastix_amd import pastix, spm
+from tests.helpers import analyse
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LLT = pastix.factotype.LLT
+
+
+def test_grid_order_golden_3x3x3():
+    g = json.load(open(os.path.join(HERE, "golden", "grid_order_3x3x3.json")))
+    o = pastix.orderGrid(3, 3, 3)
+    perm, peri, rang, tree = pastix.order_arrays(o)
+    assert perm.tolist() == g["permtab"]
+    assert rang.tolist() == g["rangtab"]
+    assert tree.tolist() == g["treetab"]
+    assert np.array_equal(peri[perm], np.arange(27))
+    pastix.orderExit(o)
+
+
+@pytest.mark.parametrize("grid", [(5, 5, 5), (20, 20, 20), (7, 3, 11), (16, 9, 1), (1, 1, 40)])
+def test_grid_order_invariants(grid):
+    nx, ny, nz = grid
+    n = nx * ny * nz
+    o = pastix.orderGrid(nx, ny, nz)
+    perm, peri, rang, tree = pastix.order_arrays(o)
+    cb = o.cblknbr
+    assert sorted(perm.tolist()) == list(range(n))
+    assert rang[0] == 0 and rang[-1] == n and np.all(np.diff(rang) > 0)
+    assert tree[-1] == -1 and np.all((tree[:-1] > np.arange(cb - 1)))
+    # a supernode and every node of its subtree are separated from the other
+    # subtrees: no grid edge joins two sibling subtrees
+    A = spm.laplacian7(nx, ny, nz, diag=6.0).tocoo()
+    col2sn = np.searchsorted(rang, np.arange(n), side="right") - 1
+    anc = [set() for _ in range(cb)]
+    for s in range(cb):
+        a, t = s, s
+        while t != -1:
+            anc[s].add(t)
+            t = tree[t]
+    for i, j in zip(A.row, A.col):
+        si, sj = col2sn[perm[i]], col2sn[perm[j]]
+        assert si in anc[sj] or sj in anc[si], "edge between unrelated subtrees"
+    pastix.orderExit(o)
+
+
+@pytest.mark.parametrize("grid,bs,generic", [((6, 6, 6), (160, 320), False), ((9, 8, 7), (8, 16), False),
+                                             ((8, 8, 8), (4, 8), True)])
+def test_symbol_contains_factor_fill(grid, bs, generic):
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=None if generic else grid, min_bs=bs[0], max_bs=bs[1])
+    n = A.shape[0]
+    p = sym["permtab"]
+    B = np.zeros((n, n))
+    Ac = A.tocoo()
+    B[p[Ac.row], p[Ac.col]] = Ac.data
+    L = np.linalg.cholesky(B)
+    # symbol pattern
+    pat = np.zeros((n, n), dtype=bool)
+    for c in range(sym["cblknbr"]):
+        fc, lc = sym["cblk_fcol"][c], sym["cblk_fcol"][c + 1]
+        for b in range(sym["cblk_bloknum"][c], sym["cblk_bloknum"][c + 1]):
+            pat[sym["blok_frow"][b]:sym["blok_lrow"][b] + 1, fc:lc] = True
+    fill = np.abs(np.tril(L)) > 1e-14 * np.abs(L).max()
+    assert not np.any(fill & ~pat), "a nonzero of L falls outside the symbol"
+    # blocks are sorted, inside one facing cblk, and split widths obey max
+    widths = np.diff(sym["cblk_fcol"])
+    assert widths.max() <= bs[1]
+    for c in range(sym["cblknbr"]):
+        b0, b1 = sym["cblk_bloknum"][c], sym["cblk_bloknum"][c + 1]
+        assert sym["blok_frow"][b0] == sym["cblk_fcol"][c]
+        for b in range(b0 + 1, b1):
+            f = sym["blok_fcblk"][b]
+            assert sym["blok_frow"][b] > sym["blok_lrow"][b - 1]
+            assert sym["cblk_fcol"][f] <= sym["blok_frow"][b] and sym["blok_lrow"][b] < sym["cblk_fcol"][f + 1]
+    # IPARM_NNZEROS: strictly-lower entries of the symbol (reference symbol_cost.c:327-361)
+    nnz_sym = int(np.tril(pat, -1).sum())
+    pastix.finalize(h)
+
+
+def test_nnzeros_and_thflops_reported():
+    grid = (10, 10, 10)
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid, min_bs=1000, max_bs=2000)
+    n = A.shape[0]
+    # strictly-lower symbol entries, before splitting (no split here)
+    cnt = 0
+    for c in range(sym["cblknbr"]):
+        w = sym["cblk_fcol"][c + 1] - sym["cblk_fcol"][c]
+        cnt += w * (w - 1) // 2
+        for b in range(sym["cblk_bloknum"][c] + 1, sym["cblk_bloknum"][c + 1]):
+            cnt += (sym["blok_lrow"][b] - sym["blok_frow"][b] + 1) * w
+    assert h.iparm[pastix.iparm.nnzeros] == cnt
+    assert h.dparm[pastix.dparm.fact_thflops] > 0
+    pastix.finalize(h)
+
+
+def test_split_widths_follow_reference_rule():
+    # computeNbSplit (reference blend/splitsymbol.c:79-100): nseq = ceil(w/max),
+    # minus one if w/nseq < min
+    for w, bmin, bmax, expect in [(400, 160, 320, 2), (320, 160, 320, 1), (330, 160, 320, 2), (1000, 160, 320, 4),
+                                  (500, 200, 300, 2)]:
+        grid = (w, 1, 1)
+        A = spm.laplacian7(*grid, diag=6.0)
+        # identity ordering, one supernode spanning the whole chain
+        ip, dp = pastix.initParam()
+        ip[pastix.iparm.factorization] = LLT
+        ip[pastix.iparm.ordering] = pastix.order.Personal
+        ip[pastix.iparm.min_blocksize] = bmin
+        ip[pastix.iparm.max_blocksize] = bmax
+        h = pastix.init(ip, dp)
+        o = pastix.pastix_order_t()
+        perm = np.arange(w, dtype=np.int64)
+        rang = np.array([0, w], dtype=np.int64)
+        tree = np.array([-1], dtype=np.int64)
+        import ctypes as ct
+        o.vertnbr, o.cblknbr, o.baseval = w, 1, 0
+        o.permtab = perm.ctypes.data_as(ct.POINTER(ct.c_int64))
+        o.peritab = perm.ctypes.data_as(ct.POINTER(ct.c_int64))
+        o.rangtab = rang.ctypes.data_as(ct.POINTER(ct.c_int64))
+        o.treetab = tree.ctypes.data_as(ct.POINTER(ct.c_int64))
+        S = spm.Spm(A, symmetric=True)
+        pastix.subtask_order(h, S, o)
+        pastix.subtask_symbfact(h)
+        pastix.subtask_blend(h)
+        sym = pastix.symbol(h)
+        assert sym["cblknbr"] == expect, (w, bmin, bmax)
+        widths = np.diff(sym["cblk_fcol"])
+        assert widths.sum() == w
+        pastix.finalize(h)
+
+
+def test_levels_respect_dependencies():
+    grid = (12, 12, 12)
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid, min_bs=20, max_bs=40)
+    # every block of cblk c faces a later cblk: the schedule levels are a topological order
+    for c in range(sym["cblknbr"]):
+        for b in range(sym["cblk_bloknum"][c] + 1, sym["cblk_bloknum"][c + 1]):
+            assert sym["blok_fcblk"][b] > c
+    assert sym["nlevels"] >= 1
+    pastix.finalize(h)

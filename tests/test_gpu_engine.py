@@ -92,6 +92,30 @@ def test_getrf_factor_parity(grid, bs):
     pastix.finalize(h)
 
 
+def test_orsirr_lu_gmres_reference_check():
+    """The reference's refinement test matrix (test/matrix/orsirr.rua, LU +
+    GMRES) with its pass criterion ||b-Ax||_1 / (||A||_1 ||x||_1 eps) <= 1e2
+    (spm/src/z_spm_genrhs.c:415-425)."""
+    import os
+    from tests.helpers import read_rua
+    A = read_rua(os.path.join(os.path.dirname(__file__), "golden", "orsirr.rua"))
+    h, S, sym = analyse(A, False, LU)
+    h.iparm[pastix.iparm.refinement] = pastix.refine.GMRES
+    pastix.task_numfact(h, S)
+    Lg, Ug = pastix.coeftab(h, lu=True)
+    O, bc, Lo, Uo = oracle_factor(A, sym, lu=True)
+    assert rel_frob(Lg, Lo) <= 1e-12 and rel_frob(Ug, Uo) <= 1e-12
+    x0, b = spm.rhs_from_x(A, 1, seed=24356)
+    x = b.copy(order="F")
+    pastix.task_solve(h, S, x)
+    pastix.task_refine(h, S, b, x)
+    r = b[:, 0] - A @ x[:, 0]
+    crit = np.abs(r).sum() / (abs(A).sum(axis=0).max() * np.abs(x).sum() * np.finfo(float).eps)
+    assert crit <= 1e2
+    assert _berr(A, x[:, 0], b[:, 0]) <= 1e-15
+    pastix.finalize(h)
+
+
 def test_refine_cg_fp64():
     grid = (10, 10, 10)
     A = spm.laplacian7(*grid, diag=6.0)
