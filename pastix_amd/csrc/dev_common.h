@@ -90,6 +90,69 @@ __device__ __forceinline__ void quad_gemm_nt(Quad<T> &q, View<T> A, int ma, View
     }
 }
 
+// Workgroup-level D(64x64) = X(64xK) * Y(64xK)^T with operands staged through
+// LDS in chunks of 16 columns: the 256 threads fetch each chunk once with
+// coalesced column segments (rows contiguous in memory), the 4 waves read
+// their MFMA fragments from LDS.  Global loads of chunk c+1 are in flight
+// while chunk c is multiplied.  Rows >= mx of X / >= my of Y read as zero;
+// X(r, k) = X[r + k*ldx].  Wave w owns the 32x32 quadrant (w>>1, w&1).
+// Must be called by all 256 threads (contains barriers).
+template <typename T> struct StageBuf {
+    T xs[16][68];
+    T ys[16][68];
+};
+
+template <typename T>
+__device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int64_t ldx, const T *Y, int my,
+                                            int64_t ldy, int K, StageBuf<T> &sb, int tid) {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int i = lane & 15, kq = lane >> 4;
+    const int row = tid & 63, kk = tid >> 6;
+    const bool rx = row < mx, ry = row < my;
+    T gx[4], gy[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int k = k0 + kk + 4 * u;
+            gx[u] = (rx && k < K) ? X[row + (int64_t)k * ldx] : (T)0;
+            gy[u] = (ry && k < K) ? Y[row + (int64_t)k * ldy] : (T)0;
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            sb.xs[kk + 4 * u][row] = gx[u];
+            sb.ys[kk + 4 * u][row] = gy[u];
+        }
+    };
+    const bool active = (wr < mx) && (wc < my);
+    gload(0);
+    sstore();
+    __syncthreads();
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const bool more = k0 + 16 < K;
+        if (more) gload(k0 + 16);
+        if (active) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                if (k0 + 4 * s < K) {
+                    const int kr = 4 * s + kq;
+                    const T a0 = sb.xs[kr][wr + i], a1 = sb.xs[kr][wr + 16 + i];
+                    const T b0 = sb.ys[kr][wc + i], b1 = sb.ys[kr][wc + 16 + i];
+                    q.c[0][0] = Mfma<T>::mma(a0, b0, q.c[0][0]);
+                    q.c[0][1] = Mfma<T>::mma(a0, b1, q.c[0][1]);
+                    q.c[1][0] = Mfma<T>::mma(a1, b0, q.c[1][0]);
+                    q.c[1][1] = Mfma<T>::mma(a1, b1, q.c[1][1]);
+                }
+            }
+        }
+        __syncthreads();
+        if (more) sstore();
+        __syncthreads();
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of work items so neighbouring items share its L2.  The grid
 // size must be a multiple of 8 (the map is then a bijection of [0, grid));

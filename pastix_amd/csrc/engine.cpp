@@ -89,6 +89,7 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     lv_cblk_ptr_ = P.lv_cblk_ptr;
     lv_trsm_ptr_ = P.lv_trsm_ptr;
     lv_upd_ptr_ = P.lv_upd_ptr;
+    lv_upd_split_ = P.lv_upd_split;
     lv_fwd_ptr_ = P.lv_fwd_ptr;
     lv_bwd_ptr_ = P.lv_bwd_ptr;
     d_lv_cblks_ = upload(*this, P.lv_cblks);
@@ -197,13 +198,16 @@ template <typename T> int Engine::factorize_t(double thr) {
             mark(6 * lv + 2);
             mark(6 * lv + 3);
         }
-        const int u0 = lv_upd_ptr_[lv], nu = lv_upd_ptr_[lv + 1] - u0;
+        const int u0 = lv_upd_ptr_[lv], us = lv_upd_split_[lv], u1 = lv_upd_ptr_[lv + 1];
         mark(6 * lv + 4);
         if (facto_ == 2) {
-            launch_update<T>(S_, 0, d_upd_ + u0, nu, d_ents_, d_shift_, L, U, L, stream_);
-            launch_update<T>(S_, 1, d_upd_ + u0, nu, d_ents_, d_shift_, U, L, U, stream_);
+            launch_update<T>(S_, 0, true, d_upd_ + u0, us - u0, d_ents_, d_shift_, L, U, L, stream_);
+            launch_update<T>(S_, 0, false, d_upd_ + us, u1 - us, d_ents_, d_shift_, L, U, L, stream_);
+            launch_update<T>(S_, 1, true, d_upd_ + u0, us - u0, d_ents_, d_shift_, U, L, U, stream_);
+            launch_update<T>(S_, 1, false, d_upd_ + us, u1 - us, d_ents_, d_shift_, U, L, U, stream_);
         } else {
-            launch_update<T>(S_, 0, d_upd_ + u0, nu, d_ents_, d_shift_, L, L, L, stream_);
+            launch_update<T>(S_, 0, true, d_upd_ + u0, us - u0, d_ents_, d_shift_, L, L, L, stream_);
+            launch_update<T>(S_, 0, false, d_upd_ + us, u1 - us, d_ents_, d_shift_, L, L, L, stream_);
         }
         mark(6 * lv + 5);
     }

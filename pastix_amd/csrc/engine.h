@@ -85,7 +85,7 @@ class Engine {
     int maxwidth_ = 0;
     DevSymbol S_{};
     // host copies of the level pointers
-    std::vector<int32_t> lv_cblk_ptr_, lv_trsm_ptr_, lv_upd_ptr_, lv_fwd_ptr_, lv_bwd_ptr_;
+    std::vector<int32_t> lv_cblk_ptr_, lv_trsm_ptr_, lv_upd_ptr_, lv_upd_split_, lv_fwd_ptr_, lv_bwd_ptr_;
     // device plan
     int32_t *d_lv_cblks_ = nullptr, *d_bwd_itemptr_ = nullptr, *d_shift_ = nullptr;
     int64_t *d_yrow_ptr_ = nullptr;

@@ -14,8 +14,9 @@ void launch_getrf_diag(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef
                        int *stat, hipStream_t st);
 template <typename T>
 void launch_trsm_panel(const DevSymbol &S, const TrsmItem *items, int n, T *coef, const T *winv, hipStream_t st);
+// direct: every item has exactly one entry (no LDS accumulation).
 template <typename T>
-void launch_update(const DevSymbol &S, int mode, const UpdItem *items, int n, const UpdEntry *ents,
+void launch_update(const DevSymbol &S, int mode, bool direct, const UpdItem *items, int n, const UpdEntry *ents,
                    const int32_t *shift, const T *A, const T *B, T *C, hipStream_t st);
 
 // Blocked panel factorisation (64-column steps).

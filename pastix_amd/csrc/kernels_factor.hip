@@ -9,11 +9,8 @@
 //                  (reference kernels/core_zgetrfsp.c:63-259, core_zgetmo.c).
 //  k_trsm_panel  : off-diagonal panel solve as a GEMM against the inverted
 //                  diagonal block (reference kernels/core_ztrsmsp.c:147-181).
-//  k_update      : Schur-complement update with scatter into the facing
-//                  cblk (reference kernels/core_zgemmsp.c:396-465); one
-//                  workgroup owns each 64x64 target tile and sums its sources
-//                  in a fixed order, so no atomics and bitwise-reproducible
-//                  results.
+// The Schur-complement update lives in kernels_update.hip and the blocked
+// panel factorisation used for LL^t in kernels_panel.hip.
 #include "dev_common.h"
 #include "dev_types.h"
 #include "kernels.h"
@@ -379,77 +376,6 @@ __global__ __launch_bounds__(256) void k_trsm_panel(DevSymbol S, const TrsmItem 
 }
 
 // ---------------------------------------------------------------------------
-// Schur-complement update, owner-computes.  MODE 0: L side (Cholesky or LU),
-// rows kept when block(p) >= block(q); MODE 1: U side of LU, block(p) > block(q).
-// C[f] -= A[s](p rows) * B[s](q rows)^T, scattered through the row/column maps.
-// ---------------------------------------------------------------------------
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_update(DevSymbol S, const UpdItem *__restrict__ items, int nitems,
-                                                const UpdEntry *__restrict__ ents, const int32_t *__restrict__ shift,
-                                                const T *__restrict__ Acoef, const T *__restrict__ Bcoef,
-                                                T *__restrict__ Ccoef) {
-    __shared__ T acc[64][65];
-    __shared__ int rmap[64], cmap[64], rblk[64], qblk[64];
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    if (id >= nitems) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    const UpdItem it = items[id];
-    const int f = it.f, r0 = it.r0, c0 = it.c0;
-    const int fcol = S.cblk_fcol[f];
-    for (int e = tid; e < 64 * 65; e += 256) (&acc[0][0])[e] = (T)0;
-    for (int e = it.e0; e < it.e1; e++) {
-        const UpdEntry en = ents[e];
-        const int s = en.s;
-        const int64_t rb = S.cblk_rowbase[s];
-        if (tid < 64) {
-            const int p = en.p0 + tid;
-            if (p < en.p1) {
-                const int bk = S.panel_blk[rb + p];
-                rmap[tid] = S.panel_grow[rb + p] + shift[en.sb + bk - en.ib0] - r0;
-                rblk[tid] = bk;
-            }
-        } else if (tid < 128) {
-            const int q = en.q0 + tid - 64;
-            if (q < en.q1) {
-                cmap[tid - 64] = S.panel_grow[rb + q] - fcol - c0;
-                qblk[tid - 64] = S.panel_blk[rb + q];
-            }
-        }
-        __syncthreads();
-        const int np = en.p1 - en.p0, nq = en.q1 - en.q0;
-        const int K = S.cblk_fcol[s + 1] - S.cblk_fcol[s];
-        const int64_t ld = S.cblk_stride[s];
-        if (wr < np && wc < nq) {
-            Quad<T> q;
-            q.zero();
-            View<T> A{Acoef + S.cblk_coef[s] + en.p0 + wr, 1, ld};
-            View<T> B{Bcoef + S.cblk_coef[s] + en.q0 + wc, 1, ld};
-            quad_gemm_nt(q, A, np - wr, B, nq - wc, K, lane);
-            QUAD_FOR({
-                const int pi = wr + qrow, qj = wc + qcol;
-                if (pi < np && qj < nq) {
-                    const bool keep = (MODE == 0) ? (rblk[pi] >= qblk[qj]) : (rblk[pi] > qblk[qj]);
-                    if (keep) acc[rmap[pi]][cmap[qj]] += q.c[m_][n_][r_];
-                }
-            })
-        }
-        __syncthreads();
-    }
-    const int Kf = S.cblk_fcol[f + 1] - fcol;
-    const int stf = S.cblk_stride[f];
-    T *C = Ccoef + S.cblk_coef[f];
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int i = e & 63, j = e >> 6;
-        const int r = r0 + i, cc = c0 + j;
-        if (r < stf && cc < Kf) {
-            const T v = acc[i][j];
-            if (v != (T)0) C[r + (int64_t)cc * stf] -= v;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
@@ -471,24 +397,12 @@ void launch_trsm_panel(const DevSymbol &S, const TrsmItem *items, int n, T *coef
     if (n <= 0) return;
     hipLaunchKernelGGL(k_trsm_panel<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, coef, winv);
 }
-template <typename T>
-void launch_update(const DevSymbol &S, int mode, const UpdItem *items, int n, const UpdEntry *ents,
-                   const int32_t *shift, const T *A, const T *B, T *C, hipStream_t st) {
-    if (n <= 0) return;
-    if (mode == 0)
-        hipLaunchKernelGGL((k_update<T, 0>), dim3(grid8(n)), dim3(256), 0, st, S, items, n, ents, shift, A, B, C);
-    else
-        hipLaunchKernelGGL((k_update<T, 1>), dim3(grid8(n)), dim3(256), 0, st, S, items, n, ents, shift, A, B, C);
-}
-
 #define INST(T)                                                                                              \
     template void launch_potrf_diag<T>(const DevSymbol &, const int32_t *, int, T *, T *, double, int *,      \
                                        hipStream_t);                                                         \
     template void launch_getrf_diag<T>(const DevSymbol &, const int32_t *, int, T *, T *, T *, T *, double,   \
                                        int *, hipStream_t);                                                  \
-    template void launch_trsm_panel<T>(const DevSymbol &, const TrsmItem *, int, T *, const T *, hipStream_t); \
-    template void launch_update<T>(const DevSymbol &, int, const UpdItem *, int, const UpdEntry *,            \
-                                   const int32_t *, const T *, const T *, T *, hipStream_t);
+    template void launch_trsm_panel<T>(const DevSymbol &, const TrsmItem *, int, T *, const T *, hipStream_t);
 INST(double)
 INST(float)
 

@@ -76,6 +76,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
 
     P.lv_trsm_ptr.assign(nl + 1, 0);
     P.lv_upd_ptr.assign(nl + 1, 0);
+    P.lv_upd_split.assign(nl, 0);
     P.lv_fwd_ptr.assign(nl + 1, 0);
     P.lv_bwd_ptr.assign(nl + 1, 0);
     P.bwd_itemptr.assign(cb + nl, 0);
@@ -208,6 +209,12 @@ Plan build_plan(const Symbol &s, const Layout &L) {
             P.upd.push_back(it);
             k = e;
         }
+        // single-entry tiles first (register-direct kernel), then the others
+        std::stable_partition(P.upd.begin() + P.lv_upd_ptr[lv], P.upd.end(),
+                              [](const UpdItem &it) { return it.e1 - it.e0 == 1; });
+        int32_t split = P.lv_upd_ptr[lv];
+        while (split < (int32_t)P.upd.size() && P.upd[split].e1 - P.upd[split].e0 == 1) split++;
+        P.lv_upd_split[lv] = split;
         P.lv_upd_ptr[lv + 1] = (int32_t)P.upd.size();
 
         // forward products of the sources of this level

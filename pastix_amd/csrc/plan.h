@@ -23,8 +23,9 @@ struct Plan {
     std::vector<int32_t> step_trsm_ptr, step_upd_ptr;
     std::vector<PanelItem> ptrsm, pupd;
     std::vector<int32_t> wide_cblks;  // cblks wider than 64 columns
-    // update tiles: items of level L come from the sources factored at level L
-    std::vector<int32_t> lv_upd_ptr;
+    // update tiles: items of level L come from the sources factored at level L;
+    // [lv_upd_ptr[L], lv_upd_split[L]) have a single entry, the rest several
+    std::vector<int32_t> lv_upd_ptr, lv_upd_split;
     std::vector<UpdItem> upd;
     std::vector<UpdEntry> ents;
     std::vector<int32_t> shift;
