@@ -78,6 +78,12 @@ int pastix_amd_fact_times(const pastix_data_t *pastix_data, double out[8]);
  * as the reference's per-block GEMMs (kernels/core_zgemmsp.c:1352-1395). */
 int pastix_amd_flops(const pastix_data_t *pastix_data, double out[4]);
 
+/* Update-schedule statistics per dependency level (up to maxlevels levels),
+ * 5 doubles each: update tiles, update entries, useful update flops, flops
+ * issued on 32-row quadrants, sum of source widths.  Returns the number of
+ * levels written or -1 before analysis. */
+int pastix_amd_plan_stats(const pastix_data_t *pastix_data, double *out, int maxlevels);
+
 /* Rerun the numerical factorisation from the device copy of the filled
  * coefficients (needs PXA_IPARM_RESIDENT_INPUT before pastix_task_numfact);
  * updates DPARM_FACT_TIME / DPARM_FACT_FLOPS like pastix_subtask_sopalin. */
