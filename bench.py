@@ -10,7 +10,7 @@ them).  value = DPARM_FACT_THFLOPS / median device factorisation time.
 
 Also reported on the same matrix: solve p50/p99 over device-resident
 right-hand sides, the FP32-factor + FP64-CG form to berr <= 1e-12, the
-roofline of the dominant kernel (k_update, FP64 MFMA) and the oracle's CPU
+roofline of the dominant kernel (a Schur-update kernel, FP64 MFMA) and the oracle's CPU
 factorisation timed on a bounded sample on this host.
 """
 import argparse
@@ -122,16 +122,33 @@ def main():
         print(json.dumps({"numfact_ms": tmed * 1e3, "gflops": gflops}))
         return
 
-    # per-family split of one more factorisation (events around every launch)
+    # per-kernel split of one more factorisation (HIP events around every
+    # update launch on the engine's stream)
     pastix.set_amd_iparm(h, pastix.amd_iparm.profile, 1)
     pastix.refactorize(h)
     ft = pastix.fact_times(h)
+    kt = pastix.kernel_times(h)
     pastix.set_amd_iparm(h, pastix.amd_iparm.profile, 0)
     pastix.refactorize(h)
+    uf = pastix.update_flops(h)
+    kern = {}
+    for cls, name, fl_ in (("update_direct", "k_update_direct<double,0>", uf["direct"]),
+                           ("update_wgemm", "k_update_wgemm<double>", uf["multi"]),
+                           ("update_acc", "k_update_acc<double,0>", uf["multi"])):
+        t_, n_ = kt[cls]
+        if n_ > 0:
+            kern[name] = {"ms": t_ * 1e3, "launches": n_, "avg_launch_ms": t_ / n_ * 1e3,
+                          "flops": fl_, "tflops": fl_ / t_ / 1e12}
+    for cls, name in (("panel", "panel steps (k_potrf64 + k_panel_trsm + k_panel_upd)"),
+                      ("update_gather", "k_update_gather<double,0>")):
+        t_, n_ = kt[cls]
+        if n_ > 0:
+            kern[name] = {"ms": t_ * 1e3, "spans": n_}
+    dom = max((k for k in kern if "flops" in kern[k]), key=lambda k: kern[k]["ms"])
     t_upd, n_upd = float(ft[3]), int(ft[6])
     upd_tf = float(fl[3]) / t_upd / 1e12
-    log(f"split: diag {ft[1]*1e3:.2f} ms trsm {ft[2]*1e3:.2f} ms update {ft[3]*1e3:.2f} ms ({n_upd} launches) "
-        f"update {upd_tf:.2f} TFLOP/s")
+    log(f"split: panel {ft[1]*1e3:.2f} ms update {ft[3]*1e3:.2f} ms ({n_upd} launches) "
+        f"update {upd_tf:.2f} TFLOP/s; " + "; ".join(f"{k} {v['ms']:.2f} ms" for k, v in kern.items()))
 
     # solves on device-resident right-hand sides
     x0, b = spm.rhs_from_x(A, 1, seed=2024)
@@ -195,11 +212,14 @@ def main():
         "wall_ms_per_step": wall / args.steps * 1e3,
         "solve_p50_ms": p50, "solve_p99_ms": p99, "solve_gbs_p50": solve_gbs, "solve_berr": berr64,
         "fp32_ir": fp32,
-        "roofline": {"bound": "mfma", "kernel": "k_update<double,0>", "achieved": upd_tf,
-                     "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": upd_tf / FP64_MFMA_PEAK_TFLOPS,
-                     "traffic": None, "launches": n_upd, "avg_launch_ms": t_upd / max(n_upd, 1) * 1e3,
-                     "flops_per_factorization": float(fl[3])},
-        "numfact_split_ms": {"diag": ft[1] * 1e3, "trsm": ft[2] * 1e3, "update": ft[3] * 1e3},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": kern[dom]["tflops"],
+                     "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": kern[dom]["tflops"] / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+                     "launches": kern[dom]["launches"], "avg_launch_ms": kern[dom]["avg_launch_ms"],
+                     "flops_per_factorization": kern[dom]["flops"],
+                     "update_family_tflops": upd_tf},
+        "numfact_split_ms": {"panel": ft[1] * 1e3, "update": ft[3] * 1e3},
+        "kernels": kern,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

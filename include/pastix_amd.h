@@ -73,15 +73,33 @@ int pastix_amd_backward_error(pastix_data_t *pastix_data, const double *b,
  * when PXA_IPARM_PROFILE is set. */
 int pastix_amd_fact_times(const pastix_data_t *pastix_data, double out[8]);
 
+/* Device time of the last numerical factorisation per kernel class, filled
+ * when PXA_IPARM_PROFILE is set: out[2k] seconds and out[2k+1] timed spans
+ * for k = 0 panel steps (diagonal blocks, panel TRSM and trailing panel
+ * update of one level per span), 1 LU panel TRSM, 2 single-source update
+ * tiles (k_update_direct), 3 multi-source product tiles (k_update_wgemm),
+ * 4 multi-source ordered sums (k_update_gather), 5 single-pass multi-source
+ * update (k_update_acc, used when a level's products exceed the workspace).
+ * Every span of classes 2-5 is exactly one kernel launch. */
+int pastix_amd_kernel_times(const pastix_data_t *pastix_data, double out[12]);
+
+/* Schur-update flops by kernel, counting only the (p, q) pairs the
+ * reference's per-block GEMMs compute: out[0] single-source tiles and out[1]
+ * multi-source tiles of the L side (block(p) >= block(q)), out[2], out[3]
+ * the same for the U side of LU (block(p) > block(q)), out[4] the flops of
+ * the multi-source product tiles as computed (one side), out[5] = 0. */
+int pastix_amd_update_flops(const pastix_data_t *pastix_data, double out[6]);
+
 /* Algorithmic flops of the analysed factorisation: out[0]=DPARM_FACT_THFLOPS,
  * [1]=diagonal factorisations, [2]=panel TRSMs, [3]=Schur updates counted
  * as the reference's per-block GEMMs (kernels/core_zgemmsp.c:1352-1395). */
 int pastix_amd_flops(const pastix_data_t *pastix_data, double out[4]);
 
 /* Update-schedule statistics per dependency level (up to maxlevels levels),
- * 5 doubles each: update tiles, update entries, useful update flops, flops
- * issued on 32-row quadrants, sum of source widths.  Returns the number of
- * levels written or -1 before analysis. */
+ * 8 doubles each: single-source tiles, their flops, multi-source tiles,
+ * their entries, their flops, flops issued on 32-row quadrants (all tiles),
+ * sum of source widths over entries, 0.  Returns the number of levels
+ * written or -1 before analysis. */
 int pastix_amd_plan_stats(const pastix_data_t *pastix_data, double *out, int maxlevels);
 
 /* Rerun the numerical factorisation from the device copy of the filled

@@ -626,8 +626,9 @@ static int refine_common(pastix_data_t *d, pastix_int_t nrhs, const void *b, pas
             hb[p] = isf ? ((const float *)b)[i + r * ldb] : ((const double *)b)[i + r * ldb];
             hx[p] = isf ? ((const float *)x)[i + r * ldx] : ((const double *)x)[i + r * ldx];
         }
-        hipMemcpy(db, hb.data(), n * sizeof(double), hipMemcpyHostToDevice);
-        hipMemcpy(dx, hx.data(), n * sizeof(double), hipMemcpyHostToDevice);
+        if (hipMemcpy(db, hb.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dx, hx.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return PASTIX_ERR_INTERNAL;
         int it = 0;
         double res = 0;
         int rc;
@@ -635,7 +636,8 @@ static int refine_common(pastix_data_t *d, pastix_int_t nrhs, const void *b, pas
         if (method == PastixRefineCG) rc = E.refine_cg(dx, db, itermax, eps, &it, &res);
         else rc = E.refine_gmres(dx, db, itermax, im, eps, &it, &res);
         if (rc) return PASTIX_ERR_INTERNAL;
-        hipMemcpy(hx.data(), dx, n * sizeof(double), hipMemcpyDeviceToHost);
+        if (hipMemcpy(hx.data(), dx, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            return PASTIX_ERR_INTERNAL;
         for (int64_t i = 0; i < n; i++) {
             int64_t p = permute ? d->order.permtab[i] : i;
             if (isf) ((float *)x)[i + r * ldx] = (float)hx[p];
@@ -875,22 +877,27 @@ extern "C" int pastix_amd_plan_stats(const pastix_data_t *d, double *out, int ma
     const pxa::Symbol &s = d->symbol;
     int nl = std::min<int>(maxlevels, P.nlevels);
     for (int lv = 0; lv < nl; lv++) {
-        double st[5] = {0, 0, 0, 0, 0};
+        double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int t = P.lv_upd_ptr[lv]; t < P.lv_upd_ptr[lv + 1]; t++) {
             const pxa::UpdItem &it = P.upd[t];
-            st[0] += 1;
+            const bool direct = t < P.lv_upd_split[lv];
+            st[direct ? 0 : 2] += 1;
             for (int e = it.e0; e < it.e1; e++) {
                 const pxa::UpdEntry &en = P.ents[e];
                 const double K = (double)(s.cblk_fcol[en.s + 1] - s.cblk_fcol[en.s]);
                 const double np = en.p1 - en.p0, nq = en.q1 - en.q0;
-                st[1] += 1;
-                st[2] += 2.0 * np * nq * K;
+                const double fl = 2.0 * np * nq * K;
+                if (direct) st[1] += fl;
+                else {
+                    st[3] += 1;
+                    st[4] += fl;
+                }
                 const double qp = std::ceil(np / 32.0) * 32.0, qq = std::ceil(nq / 32.0) * 32.0;
-                st[3] += 2.0 * qp * qq * std::ceil(K / 4.0) * 4.0;
-                st[4] += K;
+                st[5] += 2.0 * qp * qq * std::ceil(K / 4.0) * 4.0;
+                st[6] += K;
             }
         }
-        for (int k = 0; k < 5; k++) out[5 * lv + k] = st[k];
+        for (int k = 0; k < 8; k++) out[8 * lv + k] = st[k];
     }
     return nl;
 }
@@ -912,14 +919,36 @@ extern "C" int pastix_amd_backward_error(pastix_data_t *d, const double *b, cons
         hb[d->order.permtab[i]] = b[i];
         hx[d->order.permtab[i]] = x[i];
     }
-    hipMemcpy(db, hb.data(), n * sizeof(double), hipMemcpyHostToDevice);
-    hipMemcpy(db + n, hx.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    if (hipMemcpy(db, hb.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(db + n, hx.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return PASTIX_ERR_INTERNAL;
     double o[3];
     if (E.berr(db + n, db, o)) return PASTIX_ERR_INTERNAL;
     double den = d->bcsc.normInf * o[1] + o[2];
     *berr = den > 0 ? o[0] / den : o[0];
     return PASTIX_SUCCESS;
 }
+extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[12]) {
+    if (!d || !d->engine) return PASTIX_ERR_BADPARAMETER;
+    for (int c = 0; c < pxa::Engine::KC_N; c++) {
+        out[2 * c] = d->engine->kt[c];
+        out[2 * c + 1] = d->engine->kn[c];
+    }
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_update_flops(const pastix_data_t *d, double out[6]) {
+    if (!d || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
+    const pxa::Plan &P = d->plan;
+    out[0] = P.flops_kind[0][0];
+    out[1] = P.flops_kind[0][1];
+    out[2] = P.flops_kind[1][0];
+    out[3] = P.flops_kind[1][1];
+    out[4] = P.flops_wtile;
+    out[5] = 0;
+    return PASTIX_SUCCESS;
+}
+
 extern "C" int pastix_amd_fact_times(const pastix_data_t *d, double out[8]) {
     if (!d) return PASTIX_ERR_BADPARAMETER;
     for (int i = 0; i < 8; i++) out[i] = d->fact_times[i];

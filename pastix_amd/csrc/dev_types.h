@@ -34,9 +34,26 @@ struct UpdItem {
 
 // One source contribution to a tile: rows [p0,p1) and columns [q0,q1) are
 // row ranges of source cblk s's panel; row p lands on target row
-// panel_grow[p] + shift[sb + panel_blk[p] - ib0].
+// panel_grow[p] + shift[sb + panel_blk[p] - ib0].  grp: product group of
+// the entry (multi-source tiles only, -1 otherwise).
 struct UpdEntry {
-    int32_t s, p0, p1, q0, q1, sb, ib0, pad;
+    int32_t s, p0, p1, q0, q1, sb, ib0, grp;
+};
+
+// Product group of a source s and the run of its blocks facing one target
+// (first block ib0): the dense product P(p, q) = A(p, :) . B(q, :)^T over the
+// source panel rows p in [roff(ib0), stride) and q in [roff(ib0), roff(ib0)+nq),
+// stored column-major at wofs with leading dimension ldw = stride - roff(ib0).
+struct WGroup {
+    int64_t wofs;
+    int32_t ldw, nq;
+};
+
+// One 64x64 tile of a product group: source panel rows [p0, p0+mp) x
+// [q0, q0+mq), written at W[wofs + i + j*ldw].
+struct WTile {
+    int32_t s, p0, q0, mp, mq, ldw;
+    int64_t wofs;
 };
 
 // Forward-solve product: off-diagonal panel rows [r0,r1) of cblk c times x_c

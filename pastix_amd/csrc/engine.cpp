@@ -47,7 +47,8 @@ void *Engine::alloc(size_t bytes) {
 
 template <typename T> static T *upload(Engine &E, const std::vector<T> &v) {
     T *d = (T *)E.alloc(v.size() * sizeof(T));
-    if (d && !v.empty()) hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (d && !v.empty() && hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return nullptr;
     return d;
 }
 template <typename T, typename S> static T *upload_as(Engine &E, const std::vector<S> &v) {
@@ -113,8 +114,25 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     d_fwd_ = upload(*this, P.fwd);
     d_bwd_ = upload(*this, P.bwd);
     bwd_part_ = P.bwd_part;
+    d_wgrp_ = upload(*this, P.wgrp);
+    d_wtiles_ = upload(*this, P.wtiles);
+    lv_wt_ptr_ = P.lv_wt_ptr;
+    lv_wsize_ = P.lv_wsize;
 
     const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    // level workspace of the two-phase update: the largest level's products,
+    // capped at a quarter of the free device memory (levels above the cap
+    // keep the single-pass LDS-accumulating kernel)
+    {
+        size_t freeb = 0, totb = 0;
+        HIPCHK(hipMemGetInfo(&freeb, &totb));
+        const int64_t cap = (int64_t)(freeb / 4 / fsz);
+        wcap_ = std::min<int64_t>(P.wmax, cap);
+        if (wcap_ > 0) {
+            d_wspace_ = alloc((size_t)wcap_ * fsz);
+            if (!d_wspace_) return -1;
+        }
+    }
     d_L_ = alloc(coefnbr_ * fsz);
     d_W1_ = alloc(winvnbr_ * fsz);
     if (!d_L_ || !d_W1_) return -1;
@@ -160,29 +178,41 @@ int Engine::download_coef(void *hL, void *hU) {
 
 // Level L: factor the diagonal blocks of its cblks, solve their panels, then
 // apply every update these cblks send to later levels.
+int Engine::prof_begin() {
+    if (!profile) return -1;
+    while ((int)pev_.size() < nev_ + 2) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        pev_.push_back(e);
+    }
+    if (hipEventRecord(pev_[nev_], stream_) != hipSuccess) return -1;
+    return nev_++;
+}
+
+void Engine::prof_end(int cls, int k) {
+    if (k < 0) return;
+    if (hipEventRecord(pev_[nev_], stream_) != hipSuccess) return;
+    spans_.push_back(Span{cls, k, nev_});
+    nev_++;
+}
+
 template <typename T> int Engine::factorize_t(double thr) {
     T *L = (T *)d_L_, *U = (T *)d_U_, *W1 = (T *)d_W1_, *W2 = (T *)d_W2_;
     const size_t nl = lv_cblk_ptr_.size() - 1;
-    if (profile && pev_.size() < 6 * nl) {
-        for (hipEvent_t e : pev_) hipEventDestroy(e);
-        pev_.assign(6 * nl, nullptr);
-        for (auto &e : pev_) HIPCHK(hipEventCreate(&e));
-    }
-    auto mark = [&](size_t k) {
-        if (profile) (void)hipEventRecord(pev_[k], stream_);
-    };
+    nev_ = 0;
+    spans_.clear();
     for (size_t lv = 0; lv < nl; lv++) {
         const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
-        mark(6 * lv + 0);
+        int pk = prof_begin();
         if (facto_ == 2) {
             launch_getrf_diag<T>(S_, d_lv_cblks_ + c0, nc, L, U, W1, W2, thr, d_stat_, stream_);
             launch_winv_transpose<T>(S_, d_lv_cblks_ + c0, nc, W2, (T *)d_WB_, stream_);
-            mark(6 * lv + 1);
+            prof_end(KC_PANEL, pk);
             const int t0 = lv_trsm_ptr_[lv], nt = lv_trsm_ptr_[lv + 1] - t0;
-            mark(6 * lv + 2);
+            pk = prof_begin();
             launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, L, W2, stream_);
             launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, U, W1, stream_);
-            mark(6 * lv + 3);
+            prof_end(KC_TRSM, pk);
         } else {
             // blocked panel: 64-column steps, each over every cblk of the level
             for (int s = lv_step_ptr_[lv]; s < lv_step_ptr_[lv + 1]; s++) {
@@ -194,22 +224,41 @@ template <typename T> int Engine::factorize_t(double thr) {
                 const int q0 = step_upd_ptr_[s], nq = step_upd_ptr_[s + 1] - q0;
                 launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, stream_);
             }
-            mark(6 * lv + 1);
-            mark(6 * lv + 2);
-            mark(6 * lv + 3);
+            prof_end(KC_PANEL, pk);
         }
         const int u0 = lv_upd_ptr_[lv], us = lv_upd_split_[lv], u1 = lv_upd_ptr_[lv + 1];
-        mark(6 * lv + 4);
+        const int w0 = lv_wt_ptr_[lv], nw = lv_wt_ptr_[lv + 1] - w0;
+        const bool two_phase = u1 > us && lv_wsize_[lv] <= wcap_;
+        T *Wk = (T *)d_wspace_;
+        // one side of the level's update: (A, B) are the operand panels of
+        // the products, C the panels they are subtracted from
+        auto side = [&](int mode, const T *A, const T *B, T *C) {
+            int k;
+            if (us > u0) {
+                k = prof_begin();
+                launch_update<T>(S_, mode, true, d_upd_ + u0, us - u0, d_ents_, d_shift_, A, B, C, stream_);
+                prof_end(KC_DIRECT, k);
+            }
+            if (u1 == us) return;
+            if (two_phase) {
+                k = prof_begin();
+                launch_update_wgemm<T>(S_, d_wtiles_ + w0, nw, A, B, Wk, stream_);
+                prof_end(KC_WGEMM, k);
+                k = prof_begin();
+                launch_update_gather<T>(S_, mode, d_upd_ + us, u1 - us, d_ents_, d_shift_, d_wgrp_, Wk, C, stream_);
+                prof_end(KC_GATHER, k);
+            } else {
+                k = prof_begin();
+                launch_update<T>(S_, mode, false, d_upd_ + us, u1 - us, d_ents_, d_shift_, A, B, C, stream_);
+                prof_end(KC_ACC, k);
+            }
+        };
         if (facto_ == 2) {
-            launch_update<T>(S_, 0, true, d_upd_ + u0, us - u0, d_ents_, d_shift_, L, U, L, stream_);
-            launch_update<T>(S_, 0, false, d_upd_ + us, u1 - us, d_ents_, d_shift_, L, U, L, stream_);
-            launch_update<T>(S_, 1, true, d_upd_ + u0, us - u0, d_ents_, d_shift_, U, L, U, stream_);
-            launch_update<T>(S_, 1, false, d_upd_ + us, u1 - us, d_ents_, d_shift_, U, L, U, stream_);
+            side(0, L, U, L);
+            side(1, U, L, U);
         } else {
-            launch_update<T>(S_, 0, true, d_upd_ + u0, us - u0, d_ents_, d_shift_, L, L, L, stream_);
-            launch_update<T>(S_, 0, false, d_upd_ + us, u1 - us, d_ents_, d_shift_, L, L, L, stream_);
+            side(0, L, L, L);
         }
-        mark(6 * lv + 5);
     }
     // full inverses of the wide diagonal blocks, needed only by the solve
     if (facto_ != 2) launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, L, W1, (T *)d_WB_, stream_);
@@ -282,20 +331,22 @@ int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) 
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
     if (profile) {
-        t_diag = t_trsm = t_upd = 0;
-        n_diag = n_trsm = n_upd = 0;
-        for (size_t lv = 0; lv + 1 < lv_cblk_ptr_.size(); lv++) {
-            float a = 0, b = 0, c = 0;
-            HIPCHK(hipEventElapsedTime(&a, pev_[6 * lv + 0], pev_[6 * lv + 1]));
-            HIPCHK(hipEventElapsedTime(&b, pev_[6 * lv + 2], pev_[6 * lv + 3]));
-            HIPCHK(hipEventElapsedTime(&c, pev_[6 * lv + 4], pev_[6 * lv + 5]));
-            t_diag += a * 1e-3;
-            t_trsm += b * 1e-3;
-            t_upd += c * 1e-3;
-            n_diag += (lv_cblk_ptr_[lv + 1] > lv_cblk_ptr_[lv]);
-            n_trsm += (lv_trsm_ptr_[lv + 1] > lv_trsm_ptr_[lv]) * (facto_ == 2 ? 2 : 1);
-            n_upd += (lv_upd_ptr_[lv + 1] > lv_upd_ptr_[lv]) * (facto_ == 2 ? 2 : 1);
+        for (int c = 0; c < KC_N; c++) {
+            kt[c] = 0;
+            kn[c] = 0;
         }
+        for (const Span &sp : spans_) {
+            float a = 0;
+            HIPCHK(hipEventElapsedTime(&a, pev_[sp.e0], pev_[sp.e1]));
+            kt[sp.cls] += a * 1e-3;
+            kn[sp.cls]++;
+        }
+        t_diag = kt[KC_PANEL];
+        n_diag = kn[KC_PANEL];
+        t_trsm = kt[KC_TRSM];
+        n_trsm = kn[KC_TRSM];
+        t_upd = kt[KC_DIRECT] + kt[KC_WGEMM] + kt[KC_GATHER] + kt[KC_ACC];
+        n_upd = kn[KC_DIRECT] + kn[KC_WGEMM] + kn[KC_GATHER] + kn[KC_ACC];
     }
     int st[4];
     HIPCHK(hipMemcpy(st, d_stat_, sizeof(st), hipMemcpyDeviceToHost));
@@ -413,8 +464,9 @@ int Engine::spmv(double alpha, const double *x, double beta, double *y) {
 double Engine::dot(const double *x, const double *y) {
     launch_dot(n_, x, y, d_red_, d_red_ + 512 * 32, stream_);
     double v = 0;
-    hipMemcpyAsync(&v, d_red_ + 512 * 32, sizeof(double), hipMemcpyDeviceToHost, stream_);
-    hipStreamSynchronize(stream_);
+    if (hipMemcpyAsync(&v, d_red_ + 512 * 32, sizeof(double), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+        return std::nan("");
     return v;
 }
 

@@ -63,9 +63,15 @@ class Engine {
     int solve_user_device(void *db, int64_t ldb, int nrhs, double *seconds);
     int upload_perm(const std::vector<int64_t> &permtab);
 
-    // Per-family device time of the last factorize (seconds) when profiling
-    // is on (PASTIX_AMD_PROFILE=1): diag, trsm, update; launch counts.
+    // Device time of the last factorize (seconds) per kernel class when
+    // profiling is on, with the number of timed spans of each class.  The
+    // update classes time one launch per span; the panel class times the
+    // panel steps of one level (diagonal blocks, panel TRSM, trailing panel
+    // update) as one span.
+    enum KClass { KC_PANEL = 0, KC_TRSM, KC_DIRECT, KC_WGEMM, KC_GATHER, KC_ACC, KC_N };
     bool profile = false;
+    double kt[KC_N] = {0, 0, 0, 0, 0, 0};
+    int kn[KC_N] = {0, 0, 0, 0, 0, 0};
     double t_diag = 0, t_trsm = 0, t_upd = 0;
     int n_diag = 0, n_trsm = 0, n_upd = 0;
 
@@ -99,11 +105,25 @@ class Engine {
     PanelItem *d_ptrsm_ = nullptr, *d_pupd_ = nullptr;
     UpdItem *d_upd_ = nullptr;
     UpdEntry *d_ents_ = nullptr;
+    // two-phase multi-source update: product groups, tiles, level workspace
+    WGroup *d_wgrp_ = nullptr;
+    WTile *d_wtiles_ = nullptr;
+    std::vector<int32_t> lv_wt_ptr_;
+    std::vector<int64_t> lv_wsize_;
+    void *d_wspace_ = nullptr;
+    int64_t wcap_ = 0;  // workspace entries
     FwdItem *d_fwd_ = nullptr;
     BwdItem *d_bwd_ = nullptr;
     int32_t *d_perm_ = nullptr;  // permtab as int32 for device permutations
     void *d_L0_ = nullptr, *d_U0_ = nullptr;  // snapshot of the filled coefficients
     std::vector<hipEvent_t> pev_;
+    struct Span {
+        int cls, e0, e1;
+    };
+    std::vector<Span> spans_;
+    int nev_ = 0;
+    int prof_begin();
+    void prof_end(int cls, int k);
     int64_t bwd_part_ = 0;
     // numerics
     void *d_L_ = nullptr, *d_U_ = nullptr, *d_W1_ = nullptr, *d_W2_ = nullptr;

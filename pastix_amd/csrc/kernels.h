@@ -18,6 +18,13 @@ void launch_trsm_panel(const DevSymbol &S, const TrsmItem *items, int n, T *coef
 template <typename T>
 void launch_update(const DevSymbol &S, int mode, bool direct, const UpdItem *items, int n, const UpdEntry *ents,
                    const int32_t *shift, const T *A, const T *B, T *C, hipStream_t st);
+// Two-phase update of the multi-source tiles: dense product tiles into the
+// level workspace W, then the per-target-tile ordered sum.
+template <typename T>
+void launch_update_wgemm(const DevSymbol &S, const WTile *items, int n, const T *A, const T *B, T *W, hipStream_t st);
+template <typename T>
+void launch_update_gather(const DevSymbol &S, int mode, const UpdItem *items, int n, const UpdEntry *ents,
+                          const int32_t *shift, const WGroup *grp, const T *W, T *C, hipStream_t st);
 
 // Blocked panel factorisation (64-column steps).
 template <typename T>

@@ -138,6 +138,113 @@ __global__ __launch_bounds__(256) void k_update_acc(DevSymbol S, const UpdItem *
     }
 }
 
+// Phase 1 of the multi-source tiles: dense 64x64 tile of a product group in
+// the source's own row space, P(p, q) = A(p, :) . B(q, :)^T, stored to the
+// level workspace.  Computed transposed (quadrant rows index q) so that each
+// lane's stores walk down a workspace column.
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_wgemm(DevSymbol S, const WTile *__restrict__ items, int nitems,
+                                                      const T *__restrict__ Acoef, const T *__restrict__ Bcoef,
+                                                      T *__restrict__ W) {
+    __shared__ StageBuf<T> sb;
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const WTile w = items[id];
+    const int s = w.s;
+    const int K = S.cblk_fcol[s + 1] - S.cblk_fcol[s];
+    const int64_t ld = S.cblk_stride[s];
+    Quad<T> q;
+    q.zero();
+    wg_gemm_lds<T>(q, Bcoef + S.cblk_coef[s] + w.q0, w.mq, ld, Acoef + S.cblk_coef[s] + w.p0, w.mp, ld, K, sb, tid);
+    T *Wt = W + w.wofs;
+    if (wr < w.mq && wc < w.mp) {
+        UQFOR({
+            const int qj = wr + qrow, pi = wc + qcol;
+            if (qj < w.mq && pi < w.mp) Wt[pi + (int64_t)qj * w.ldw] = q.c[m_][n_][r_];
+        })
+    }
+}
+
+// Phase 2: every multi-source target tile sums, in entry order, the products
+// of its entries read from the workspace and subtracts the sum once.
+// Thread (r, ph) owns target row r and columns ph, ph+4, ..., ph+60 of the
+// tile.  Per entry, the source rows landing on each target row / column are
+// published in LDS tables tagged with the entry index; the tables alternate
+// between two buffers so one barrier per entry suffices.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_update_gather(DevSymbol S, const UpdItem *__restrict__ items, int nitems,
+                                                       const UpdEntry *__restrict__ ents,
+                                                       const int32_t *__restrict__ shift,
+                                                       const WGroup *__restrict__ grp, const T *__restrict__ W,
+                                                       T *__restrict__ Ccoef) {
+    __shared__ int rp[2][64], rs[2][64], rbk[2][64];
+    __shared__ int cq[2][64], cs[2][64], cbk[2][64];
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const int tid = threadIdx.x, r = tid & 63, ph = tid >> 6;
+    const UpdItem it = items[id];
+    const int f = it.f, r0 = it.r0, c0 = it.c0;
+    const int fcol = S.cblk_fcol[f];
+    if (tid < 128) {
+        (&rs[0][0])[tid] = -1;
+        (&cs[0][0])[tid] = -1;
+    }
+    __syncthreads();
+    T acc[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) acc[u] = (T)0;
+    for (int e = it.e0; e < it.e1; e++) {
+        const UpdEntry en = ents[e];
+        const int b = (e - it.e0) & 1;
+        const int64_t rb = S.cblk_rowbase[en.s];
+        if (tid < 64) {
+            const int p = en.p0 + tid;
+            if (p < en.p1) {
+                const int bk = S.panel_blk[rb + p];
+                const int tr = S.panel_grow[rb + p] + shift[en.sb + bk - en.ib0] - r0;
+                rp[b][tr] = p;
+                rbk[b][tr] = bk;
+                rs[b][tr] = e;
+            }
+        } else if (tid < 128) {
+            const int qq = en.q0 + tid - 64;
+            if (qq < en.q1) {
+                const int tc = S.panel_grow[rb + qq] - fcol - c0;
+                cq[b][tc] = qq;
+                cbk[b][tc] = S.panel_blk[rb + qq];
+                cs[b][tc] = e;
+            }
+        }
+        __syncthreads();
+        if (rs[b][r] == e) {
+            const WGroup G = grp[en.grp];
+            const int base = S.blok_roff[en.ib0];
+            const int bp = rbk[b][r];
+            const T *Wr = W + G.wofs + (rp[b][r] - base);
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int c = ph + 4 * u;
+                if (cs[b][c] == e) {
+                    const bool keep = (MODE == 0) ? (bp >= cbk[b][c]) : (bp > cbk[b][c]);
+                    if (keep) acc[u] += Wr[(int64_t)(cq[b][c] - base) * G.ldw];
+                }
+            }
+        }
+    }
+    const int Kf = S.cblk_fcol[f + 1] - fcol;
+    const int64_t stf = S.cblk_stride[f];
+    T *C = Ccoef + S.cblk_coef[f];
+    if (r0 + r < stf) {
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int cc = c0 + ph + 4 * u;
+            if (cc < Kf && acc[u] != (T)0) C[r0 + r + (int64_t)cc * stf] -= acc[u];
+        }
+    }
+}
+
 static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
 
 template <typename T>
@@ -154,9 +261,29 @@ void launch_update(const DevSymbol &S, int mode, bool direct, const UpdItem *ite
     }
 }
 
-template void launch_update<double>(const DevSymbol &, int, bool, const UpdItem *, int, const UpdEntry *,
-                                    const int32_t *, const double *, const double *, double *, hipStream_t);
-template void launch_update<float>(const DevSymbol &, int, bool, const UpdItem *, int, const UpdEntry *,
-                                   const int32_t *, const float *, const float *, float *, hipStream_t);
+template <typename T>
+void launch_update_wgemm(const DevSymbol &S, const WTile *items, int n, const T *A, const T *B, T *W, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_update_wgemm<T>), dim3(grid8(n)), dim3(256), 0, st, S, items, n, A, B, W);
+}
+
+template <typename T>
+void launch_update_gather(const DevSymbol &S, int mode, const UpdItem *items, int n, const UpdEntry *ents,
+                          const int32_t *shift, const WGroup *grp, const T *W, T *C, hipStream_t st) {
+    if (n <= 0) return;
+    const dim3 g(grid8(n)), b(256);
+    if (mode == 0) hipLaunchKernelGGL((k_update_gather<T, 0>), g, b, 0, st, S, items, n, ents, shift, grp, W, C);
+    else hipLaunchKernelGGL((k_update_gather<T, 1>), g, b, 0, st, S, items, n, ents, shift, grp, W, C);
+}
+
+#define PXA_UPD_INST(T)                                                                                           \
+    template void launch_update<T>(const DevSymbol &, int, bool, const UpdItem *, int, const UpdEntry *,          \
+                                   const int32_t *, const T *, const T *, T *, hipStream_t);                      \
+    template void launch_update_wgemm<T>(const DevSymbol &, const WTile *, int, const T *, const T *, T *,        \
+                                         hipStream_t);                                                            \
+    template void launch_update_gather<T>(const DevSymbol &, int, const UpdItem *, int, const UpdEntry *,         \
+                                          const int32_t *, const WGroup *, const T *, T *, hipStream_t);
+PXA_UPD_INST(double)
+PXA_UPD_INST(float)
 
 }  // namespace pxa

@@ -7,7 +7,6 @@
 
 namespace pxa {
 
-static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restrict__ colptr,

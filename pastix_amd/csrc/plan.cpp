@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
 
 namespace pxa {
 
@@ -23,7 +24,7 @@ struct KeyEnt {
 }  // namespace
 
 Plan build_plan(const Symbol &s, const Layout &L) {
-    const int64_t cb = s.cblknbr(), nb = s.bloknbr();
+    const int64_t cb = s.cblknbr();
     Plan P;
     // panel row maps
     P.rowbase.assign(cb + 1, 0);
@@ -79,6 +80,8 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     P.lv_upd_split.assign(nl, 0);
     P.lv_fwd_ptr.assign(nl + 1, 0);
     P.lv_bwd_ptr.assign(nl + 1, 0);
+    P.lv_wt_ptr.assign(nl + 1, 0);
+    P.lv_wsize.assign(nl, 0);
     P.bwd_itemptr.assign(cb + nl, 0);
     // update buffer: one row per off-diagonal panel row; CSR by permuted row
     std::vector<int64_t> yoff(cb + 1, 0);
@@ -190,7 +193,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                         // keep only tiles with at least one pair block(p) >= block(q)
                         if (P.panel_blk[rb + r.b - 1] < P.panel_blk[rb + cc.a]) continue;
                         uint64_t key = ((uint64_t)f << 40) | ((uint64_t)r.t << 16) | (uint64_t)cc.t;
-                        ke.push_back(KeyEnt{key, UpdEntry{(int32_t)src, r.a, r.b, cc.a, cc.b, sb, (int32_t)i0, 0}});
+                        ke.push_back(KeyEnt{key, UpdEntry{(int32_t)src, r.a, r.b, cc.a, cc.b, sb, (int32_t)i0, -1}});
                     }
             }
         }
@@ -216,6 +219,83 @@ Plan build_plan(const Symbol &s, const Layout &L) {
         while (split < (int32_t)P.upd.size() && P.upd[split].e1 - P.upd[split].e0 == 1) split++;
         P.lv_upd_split[lv] = split;
         P.lv_upd_ptr[lv + 1] = (int32_t)P.upd.size();
+        // exact flops of the kept (p, q) pairs of every entry
+        for (int32_t t = P.lv_upd_ptr[lv]; t < P.lv_upd_ptr[lv + 1]; t++) {
+            const int kind = (t < split) ? 0 : 1;
+            for (int32_t e = P.upd[t].e0; e < P.upd[t].e1; e++) {
+                const UpdEntry &en = P.ents[e];
+                const int64_t rb = P.rowbase[en.s];
+                const double K2 = 2.0 * (double)width(en.s);
+                int64_t qa = en.q0, qb = en.q0;  // first q with blk(q) >= / > blk(p)
+                double c0 = 0, c1 = 0;
+                for (int64_t p = en.p0; p < en.p1; p++) {
+                    const int32_t bp = P.panel_blk[rb + p];
+                    while (qa < en.q1 && P.panel_blk[rb + qa] < bp) qa++;
+                    while (qb < en.q1 && P.panel_blk[rb + qb] <= bp) qb++;
+                    c0 += (double)(qb - en.q0);
+                    c1 += (double)(qa - en.q0);
+                }
+                P.flops_kind[0][kind] += K2 * c0;
+                P.flops_kind[1][kind] += K2 * c1;
+            }
+        }
+
+        // product groups and their 64x64 tiles for the multi-source tiles
+        {
+            std::map<std::pair<int32_t, int32_t>, int32_t> gid;
+            std::vector<std::pair<int32_t, int64_t>> tl;  // (group, qt << 32 | pt)
+            int64_t wsz = 0;
+            for (int32_t t = split; t < (int32_t)P.upd.size(); t++)
+                for (int32_t e = P.upd[t].e0; e < P.upd[t].e1; e++) {
+                    UpdEntry &en = P.ents[e];
+                    const auto key = std::make_pair(en.s, en.ib0);
+                    auto gi = gid.find(key);
+                    int32_t g;
+                    if (gi == gid.end()) {
+                        const int64_t f = s.blok_fcblk[en.ib0], lastb = s.cblk_bloknum[en.s + 1];
+                        int64_t i1 = en.ib0;
+                        while (i1 < lastb && s.blok_fcblk[i1] == f) i1++;
+                        const int64_t base = L.blok_roff[en.ib0];
+                        const int64_t np = L.cblk_stride[en.s] - base;
+                        const int64_t nq = (i1 < lastb ? L.blok_roff[i1] : L.cblk_stride[en.s]) - base;
+                        g = (int32_t)P.wgrp.size();
+                        P.wgrp.push_back(WGroup{wsz, (int32_t)np, (int32_t)nq});
+                        wsz += (np * nq + 31) / 32 * 32;
+                        gid.emplace(key, g);
+                    } else {
+                        g = gi->second;
+                    }
+                    en.grp = g;
+                    const int64_t base = L.blok_roff[en.ib0];
+                    for (int64_t pt = (en.p0 - base) / 64; pt <= (en.p1 - 1 - base) / 64; pt++)
+                        for (int64_t qt = (en.q0 - base) / 64; qt <= (en.q1 - 1 - base) / 64; qt++)
+                            tl.push_back({g, (qt << 32) | pt});
+                }
+            std::sort(tl.begin(), tl.end());
+            tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
+            P.lv_wsize[lv] = wsz;
+            P.wmax = std::max(P.wmax, wsz);
+            const int32_t g0 = (int32_t)(P.wgrp.size() - gid.size());
+            std::vector<std::pair<int32_t, int32_t>> gsrc(gid.size());
+            for (const auto &kv : gid) gsrc[kv.second - g0] = kv.first;
+            for (const auto &t : tl) {
+                const WGroup &G = P.wgrp[t.first];
+                const auto &sk = gsrc[t.first - g0];
+                const int64_t pt = t.second & 0xFFFFFFFF, qt = t.second >> 32;
+                const int64_t base = L.blok_roff[sk.second];
+                WTile w;
+                w.s = sk.first;
+                w.p0 = (int32_t)(base + pt * 64);
+                w.q0 = (int32_t)(base + qt * 64);
+                w.mp = (int32_t)std::min<int64_t>(64, G.ldw - pt * 64);
+                w.mq = (int32_t)std::min<int64_t>(64, G.nq - qt * 64);
+                w.ldw = G.ldw;
+                w.wofs = G.wofs + pt * 64 + qt * 64 * (int64_t)G.ldw;
+                P.wtiles.push_back(w);
+                P.flops_wtile += 2.0 * w.mp * w.mq * (double)width(w.s);
+            }
+            P.lv_wt_ptr[lv + 1] = (int32_t)P.wtiles.size();
+        }
 
         // forward products of the sources of this level
         for (int32_t k = l0; k < l1; k++) {

@@ -29,6 +29,15 @@ struct Plan {
     std::vector<UpdItem> upd;
     std::vector<UpdEntry> ents;
     std::vector<int32_t> shift;
+    // multi-source tiles are updated in two phases: the products of their
+    // entries are computed as dense tiles in the sources' own row space
+    // (wtiles of level L: [lv_wt_ptr[L], lv_wt_ptr[L+1])) into a level
+    // workspace of lv_wsize[L] entries, then summed per target tile
+    std::vector<WGroup> wgrp;
+    std::vector<WTile> wtiles;
+    std::vector<int32_t> lv_wt_ptr;
+    std::vector<int64_t> lv_wsize;
+    int64_t wmax = 0;
     // forward products (sources of level L) into the update buffer, and the
     // CSR from every permuted row to its update-buffer entries
     std::vector<int32_t> lv_fwd_ptr;
@@ -51,6 +60,12 @@ struct Plan {
     // of kernels/core_zgemmsp.c:1352-1357 (x2 sides for LU)
     double flops_diag = 0, flops_trsm = 0, flops_upd = 0;
     double flops_upd_potrf = 0, flops_upd_lu = 0;
+    // the same update flops split by kernel: [side][0] single-source tiles,
+    // [side][1] multi-source tiles (side 0: L, block(p) >= block(q); side 1:
+    // U of LU, block(p) > block(q)); flops_wtile: product tiles as computed
+    // (64x64 tiles of the multi-source products, one side)
+    double flops_kind[2][2] = {{0, 0}, {0, 0}};
+    double flops_wtile = 0;
 };
 
 Plan build_plan(const Symbol &s, const Layout &L);

@@ -323,6 +323,10 @@ lib.pastix_amd_fact_times.argtypes = [_P, _dp]
 lib.pastix_amd_fact_times.restype = ct.c_int
 lib.pastix_amd_flops.argtypes = [_P, _dp]
 lib.pastix_amd_flops.restype = ct.c_int
+lib.pastix_amd_kernel_times.argtypes = [_P, _dp]
+lib.pastix_amd_kernel_times.restype = ct.c_int
+lib.pastix_amd_update_flops.argtypes = [_P, _dp]
+lib.pastix_amd_update_flops.restype = ct.c_int
 lib.pastix_amd_refactorize.argtypes = [_P]
 lib.pastix_amd_refactorize.restype = ct.c_int
 lib.pastix_amd_solve_device.argtypes = [_P, _I, ct.c_void_p, _I]
@@ -334,6 +338,25 @@ def fact_times(pastix_data):
     out = np.zeros(8)
     _check("pastix_amd_fact_times", lib.pastix_amd_fact_times(pastix_data.ptr, out))
     return out
+
+
+KERNEL_CLASSES = ("panel", "lu_trsm", "update_direct", "update_wgemm", "update_gather", "update_acc")
+
+
+def kernel_times(pastix_data):
+    """{class: (seconds, timed spans)} of the last numfact (profiling on)."""
+    out = np.zeros(12)
+    _check("pastix_amd_kernel_times", lib.pastix_amd_kernel_times(pastix_data.ptr, out))
+    return {k: (float(out[2 * i]), int(out[2 * i + 1])) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+def update_flops(pastix_data):
+    """Schur-update flops per kernel: L-side single/multi-source tiles, U-side
+    single/multi-source tiles, multi-source product tiles as computed."""
+    out = np.zeros(6)
+    _check("pastix_amd_update_flops", lib.pastix_amd_update_flops(pastix_data.ptr, out))
+    return {"direct": float(out[0]), "multi": float(out[1]), "direct_u": float(out[2]),
+            "multi_u": float(out[3]), "wtile": float(out[4])}
 
 
 def flops(pastix_data):
