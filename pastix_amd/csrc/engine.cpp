@@ -147,6 +147,7 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     }
     d_stat_ = (int *)alloc(4 * sizeof(int));
     d_WB_ = alloc(winvnbr_ * fsz);
+    if (d_WB_) HIPCHK(hipMemsetAsync(d_WB_, 0, winvnbr_ * fsz, stream_));
     // nrhs chunk: at most 8 (register accumulators of the solve kernels) and
     // bounded by their dynamic LDS (48 KiB)
     nrhs_chunk_ = (int)std::max<int64_t>(1, std::min<int64_t>(8, (48 * 1024) / (std::max(maxwidth_, 256) * (int64_t)fsz)));

@@ -3,8 +3,8 @@
 // A cblk's panel (stride x K, column-major) is factored by 64-column steps,
 // each step being three launches over all cblks of the level:
 //   k_potrf64   : Cholesky of the 64x64 diagonal sub-block with static
-//                 pivoting (reference kernels/core_zpotrfsp.c:66-108, one LDS
-//                 barrier per column), plus its inverse;
+//                 pivoting (reference kernels/core_zpotrfsp.c:66-108), the
+//                 block in registers, plus its inverse;
 //   k_panel_trsm: every panel row below the sub-block times inv(L_kk)^T
 //                 (core_zpotrfsp.c:166-176 and core_ztrsmsp.c:147-181);
 //   k_panel_upd : trailing update of the panel's remaining columns
@@ -29,105 +29,111 @@ namespace pxa {
 
 template <typename T> __device__ __forceinline__ T pabs(T v) { return v < (T)0 ? -v : v; }
 
-// Cholesky of the nb x nb lower block in Sm.  Step k reads the final column
-// k, updates the columns on its right with a_ik a_jk / d_k and scales column
-// k-1, so a single barrier per column suffices.
-template <typename T>
-__device__ void lds_potf2(T (*Sm)[65], int nb, T crit, int tid, int &npiv, int &neg) {
-    T sprev = 1;
-    for (int k = 0; k < nb; k++) {
-        T d = Sm[k][k];
-        const bool piv = pabs(d) < crit;
-        if (piv) d = crit;
-        const bool ng = d < (T)0;
-        if (ng) d = -d;
-        const T invd = (T)1 / d;
-        if (k > 0) {
-            const int kk = k - 1;
-            const T is = (T)1 / sprev;
-            for (int e = tid; e < nb - kk; e += 256) {
-                if (e == 0) Sm[kk][kk] = sprev;
-                else Sm[kk + e][kk] *= is;
-            }
-        }
-        // rank-1 update of the trailing lower triangle: thread (row, column
-        // phase) = (tid % 64, tid / 64), columns k+1+ph, k+5+ph, ... <= row
-        {
-            const int i = tid & 63, ph = tid >> 6;
-            if (i > k && i < nb) {
-                const T aik = Sm[i][k] * invd;
-                for (int j = k + 1 + ph; j <= i; j += 4) Sm[i][j] -= aik * Sm[j][k];
-            }
-        }
-        if (tid == 0) {
-            npiv += piv;
-            neg |= ng;
-        }
-        sprev = sqrt(d);
-        __syncthreads();
-    }
-    if (tid == 0 && nb > 0) Sm[nb - 1][nb - 1] = sprev;
-    __syncthreads();
-}
-
-// X = inv(L) for the nb x nb lower block in Sm (L stays in the lower part):
-// X[i][j] (j < i) goes to Sm[j][i], X[i][i] to dinv[i].  Column j of X is a
-// forward substitution owned by thread j (reads of L are broadcasts, the
-// thread's own column lives in row j of the upper triangle): no barrier
-// inside the substitution.
-template <typename T>
-__device__ void lds_trtri_cols(T (*Sm)[65], T *dinv, int nb, int tid) {
-    if (tid < nb) dinv[tid] = (T)1 / Sm[tid][tid];
-    __syncthreads();
-    if (tid < nb) {
-        const int j = tid;
-        for (int i = j + 1; i < nb; i++) {
-            T s = Sm[i][j] * dinv[j];
-            for (int k = j + 1; k < i; k++) s += Sm[i][k] * Sm[j][k];
-            Sm[j][i] = -dinv[i] * s;
-        }
-    }
-    __syncthreads();
-}
-
+// Cholesky and inverse of one 64-column diagonal sub-block.
+//
+// Thread (i, w) = (tid % 64, tid / 64) keeps the entries (i, j), j = w mod 4,
+// of row i in registers.  Step k publishes column k through LDS (two buffers
+// by parity, so one barrier per step orders every write after the reads of
+// the step before), replaces a pivot below the threshold (reference
+// kernels/core_zpotrfsp.c:78-81) and applies the rank-1 update in registers.
+// Rows beyond the block width are padded with the identity, which the
+// updates leave untouched.
+//
+// The inverse X = inv(L) is formed by columns: four lanes of one wave share
+// column j and split the forward substitution L x = e_j by k mod 4; their
+// partial sums are combined with lane shuffles (no barrier).  W receives X
+// (lower, column-major, ld = K) and WT its transpose, both with explicit
+// zeros in the other triangle of the sub-block.
 template <typename T>
 __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
                                                  T *__restrict__ coef, T *__restrict__ winv, T *__restrict__ winvt,
                                                  double thr, int *stat) {
-    __shared__ T Sm[64][65];
+    __shared__ T col[2][64];
+    __shared__ T Ls[64][65];
     __shared__ T dinv[64];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, i = tid & 63, w = tid >> 6;
     const int c = cblks[blockIdx.x];
     const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
     const int k0 = kb * 64;
     const int nb = min(64, K - k0);
     const int64_t ld = S.cblk_stride[c];
-    T *P = coef + S.cblk_coef[c];
-    T *W = winv + S.cblk_winv[c];
-    T *WT = winvt + S.cblk_winv[c];
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int i = e & 63, j = e >> 6;
-        Sm[i][j] = (i < nb && j < nb && i >= j) ? P[(k0 + i) + (int64_t)(k0 + j) * ld] : (T)0;
+    T *P = coef + S.cblk_coef[c] + k0 + (int64_t)k0 * ld;
+    T a[16];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+        const int j = 4 * jj + w;
+        T v = (i == j) ? (T)1 : (T)0;
+        if (i < nb && j < nb && j <= i) v = P[i + (int64_t)j * ld];
+        a[jj] = v;
     }
-    __syncthreads();
+    const T crit = (T)thr;
     int npiv = 0, neg = 0;
-    lds_potf2<T>(Sm, nb, (T)thr, tid, npiv, neg);
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int i = e & 63, j = e >> 6;
-        if (i < nb && j < nb && i >= j) P[(k0 + i) + (int64_t)(k0 + j) * ld] = Sm[i][j];
-    }
-    lds_trtri_cols<T>(Sm, dinv, nb, tid);
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int i = e & 63, j = e >> 6;
-        if (i < nb && j < nb && i >= j) {
-            const T v = (i == j) ? dinv[i] : Sm[j][i];
-            W[(k0 + i) + (int64_t)(k0 + j) * K] = v;
-            WT[(k0 + j) + (int64_t)(k0 + i) * K] = v;
+#pragma unroll
+    for (int k = 0; k < 64; k++) {
+        T *cb = col[k & 1];
+        if (w == (k & 3)) cb[i] = a[k >> 2];
+        __syncthreads();
+        T d = cb[k];
+        if (k < nb) {
+            if (pabs(d) < crit) {
+                d = crit;
+                npiv++;
+            }
+            if (d < (T)0) {
+                neg = 1;
+                d = -d;
+            }
         }
+        const T sd = sqrt(d);
+        const T m = cb[i] / d;
+#pragma unroll
+        for (int jj = k >> 2; jj < 16; jj++) {
+            const int j = 4 * jj + w;
+            if (j > k) a[jj] -= m * cb[j];
+        }
+        if (w == (k & 3)) a[k >> 2] = (i == k) ? sd : a[k >> 2] / sd;
+    }
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+        const int j = 4 * jj + w;
+        if (j <= i) Ls[i][j] = a[jj];
+        if (i < nb && j < nb && j <= i) P[i + (int64_t)j * ld] = a[jj];
     }
     if (tid == 0) {
         if (npiv) atomicAdd(&stat[0], npiv);
         if (neg) atomicOr(&stat[1], 1);
+    }
+    __syncthreads();
+    if (tid < 64) dinv[tid] = (T)1 / Ls[tid][tid];
+    __syncthreads();
+    // column j of inv(L): lanes 4*jl + q of wave w, x[kk] = X(4*kk + q, j)
+    const int q = tid & 3, j = 16 * w + ((tid >> 2) & 15);
+    T x[16];
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) x[kk] = (T)0;
+#pragma unroll
+    for (int r = 0; r < 64; r++) {
+        T s = (T)0;
+#pragma unroll
+        for (int kk = 0; 4 * kk < r; kk++)
+            if (4 * kk + q < r) s += Ls[r][4 * kk + q] * x[kk];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        const T xr = (r < j) ? (T)0 : ((r == j) ? dinv[r] : -s * dinv[r]);
+        if (q == (r & 3)) x[r >> 2] = xr;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) Ls[4 * kk + q][j] = x[kk];
+    __syncthreads();
+    T *W = winv + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
+    T *WT = winvt + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int lo = e & 63, hi = e >> 6;
+        if (lo < nb && hi < nb) {
+            W[lo + (int64_t)hi * K] = Ls[lo][hi];
+            WT[lo + (int64_t)hi * K] = Ls[hi][lo];
+        }
     }
 }
 

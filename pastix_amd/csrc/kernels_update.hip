@@ -170,65 +170,82 @@ __global__ __launch_bounds__(256) void k_update_wgemm(DevSymbol S, const WTile *
 // Phase 2: every multi-source target tile sums, in entry order, the products
 // of its entries read from the workspace and subtracts the sum once.
 // Thread (r, ph) owns target row r and columns ph, ph+4, ..., ph+60 of the
-// tile.  Per entry, the source rows landing on each target row / column are
-// published in LDS tables tagged with the entry index; the tables alternate
-// between two buffers so one barrier per entry suffices.
+// tile.  Entries are taken in batches of 8: first the source rows landing on
+// each target row / column of every entry of the batch are published in LDS
+// tables tagged with the entry index (64 threads per entry, independent
+// loads for the whole batch), then the products are accumulated in entry
+// order.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_update_gather(DevSymbol S, const UpdItem *__restrict__ items, int nitems,
                                                        const UpdEntry *__restrict__ ents,
                                                        const int32_t *__restrict__ shift,
                                                        const WGroup *__restrict__ grp, const T *__restrict__ W,
                                                        T *__restrict__ Ccoef) {
-    __shared__ int rp[2][64], rs[2][64], rbk[2][64];
-    __shared__ int cq[2][64], cs[2][64], cbk[2][64];
+    constexpr int NB = 8;
+    __shared__ int rp[NB][64], rs[NB][64], rbk[NB][64];
+    __shared__ int cq[NB][64], cs[NB][64], cbk[NB][64];
+    __shared__ int64_t wb[NB];
+    __shared__ int wl[NB];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     const int tid = threadIdx.x, r = tid & 63, ph = tid >> 6;
     const UpdItem it = items[id];
     const int f = it.f, r0 = it.r0, c0 = it.c0;
     const int fcol = S.cblk_fcol[f];
-    if (tid < 128) {
-        (&rs[0][0])[tid] = -1;
-        (&cs[0][0])[tid] = -1;
+    for (int e = tid; e < NB * 64; e += 256) {
+        (&rs[0][0])[e] = -1;
+        (&cs[0][0])[e] = -1;
     }
-    __syncthreads();
     T acc[16];
 #pragma unroll
     for (int u = 0; u < 16; u++) acc[u] = (T)0;
-    for (int e = it.e0; e < it.e1; e++) {
-        const UpdEntry en = ents[e];
-        const int b = (e - it.e0) & 1;
-        const int64_t rb = S.cblk_rowbase[en.s];
-        if (tid < 64) {
-            const int p = en.p0 + tid;
-            if (p < en.p1) {
-                const int bk = S.panel_blk[rb + p];
-                const int tr = S.panel_grow[rb + p] + shift[en.sb + bk - en.ib0] - r0;
-                rp[b][tr] = p;
-                rbk[b][tr] = bk;
-                rs[b][tr] = e;
-            }
-        } else if (tid < 128) {
-            const int qq = en.q0 + tid - 64;
-            if (qq < en.q1) {
-                const int tc = S.panel_grow[rb + qq] - fcol - c0;
-                cq[b][tc] = qq;
-                cbk[b][tc] = S.panel_blk[rb + qq];
-                cs[b][tc] = e;
+    for (int eb = it.e0; eb < it.e1; eb += NB) {
+        const int ne = min(NB, it.e1 - eb);
+        __syncthreads();  // previous batch consumed (and the tables initialised)
+#pragma unroll
+        for (int pass = 0; pass < NB / 4; pass++) {
+            const int slot = pass * 4 + ph;
+            if (slot < ne) {
+                const int e = eb + slot;
+                const UpdEntry en = ents[e];
+                const int64_t rb = S.cblk_rowbase[en.s];
+                const int p = en.p0 + r;
+                if (p < en.p1) {
+                    const int bk = S.panel_blk[rb + p];
+                    const int tr = S.panel_grow[rb + p] + shift[en.sb + bk - en.ib0] - r0;
+                    rp[slot][tr] = p;
+                    rbk[slot][tr] = bk;
+                    rs[slot][tr] = e;
+                }
+                const int qq = en.q0 + r;
+                if (qq < en.q1) {
+                    const int tc = S.panel_grow[rb + qq] - fcol - c0;
+                    cq[slot][tc] = qq;
+                    cbk[slot][tc] = S.panel_blk[rb + qq];
+                    cs[slot][tc] = e;
+                }
+                if (r == 0) {
+                    const WGroup G = grp[en.grp];
+                    const int base = S.blok_roff[en.ib0];
+                    wb[slot] = G.wofs - base - (int64_t)base * G.ldw;
+                    wl[slot] = G.ldw;
+                }
             }
         }
         __syncthreads();
-        if (rs[b][r] == e) {
-            const WGroup G = grp[en.grp];
-            const int base = S.blok_roff[en.ib0];
-            const int bp = rbk[b][r];
-            const T *Wr = W + G.wofs + (rp[b][r] - base);
+        for (int slot = 0; slot < ne; slot++) {
+            const int e = eb + slot;
+            if (rs[slot][r] == e) {
+                const int bp = rbk[slot][r];
+                const int64_t ldw = wl[slot];
+                const T *Wr = W + wb[slot] + rp[slot][r];
 #pragma unroll
-            for (int u = 0; u < 16; u++) {
-                const int c = ph + 4 * u;
-                if (cs[b][c] == e) {
-                    const bool keep = (MODE == 0) ? (bp >= cbk[b][c]) : (bp > cbk[b][c]);
-                    if (keep) acc[u] += Wr[(int64_t)(cq[b][c] - base) * G.ldw];
+                for (int u = 0; u < 16; u++) {
+                    const int c = ph + 4 * u;
+                    if (cs[slot][c] == e) {
+                        const bool keep = (MODE == 0) ? (bp >= cbk[slot][c]) : (bp > cbk[slot][c]);
+                        if (keep) acc[u] += Wr[(int64_t)cq[slot][c] * ldw];
+                    }
                 }
             }
         }
