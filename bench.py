@@ -85,12 +85,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="only the timed factorisations (for rocprofv3)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only three device-to-device copies of 256 MiB (reference dispatch for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU)", file=sys.stderr)
         sys.exit(2)
 
     from pastix_amd import pastix, spm
+
+    if args.calibrate_copy:
+        # a plain copy of known size: each dispatch reads and writes 256 MiB
+        n = 256 * 1024 * 1024 // 8
+        src = pastix.DeviceArray(np.ones(n))
+        dst = pastix.DeviceArray(np.zeros(n))
+        for _ in range(3):
+            dst.copy_from(src)
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "dispatches": 3}))
+        return
 
     grid = tuple(args.grid)
     log(f"building the {grid} 7-point Laplacian")

@@ -67,6 +67,12 @@ struct BwdItem {
     int32_t c, r0, r1, slot;
 };
 
+// Diagonal step of the solve: row block [64*ib, 64*ib+64) of cblk c, whose
+// backward tiles are items [b0, b1).
+struct DiagItem {
+    int32_t c, ib, b0, b1;
+};
+
 // Panel TRSM tile: off-diagonal rows [r0, r0+64) of cblk c (absolute panel rows).
 struct TrsmItem {
     int32_t c, r0;

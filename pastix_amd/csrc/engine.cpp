@@ -112,6 +112,8 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     d_upd_ = upload(*this, P.upd);
     d_ents_ = upload(*this, P.ents);
     d_fwd_ = upload(*this, P.fwd);
+    d_diag_ = upload(*this, P.diag);
+    lv_diag_ptr_ = P.lv_diag_ptr;
     d_bwd_ = upload(*this, P.bwd);
     bwd_part_ = P.bwd_part;
     d_wgrp_ = upload(*this, P.wgrp);
@@ -149,10 +151,12 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     d_WB_ = alloc(winvnbr_ * fsz);
     if (d_WB_) HIPCHK(hipMemsetAsync(d_WB_, 0, winvnbr_ * fsz, stream_));
     // nrhs chunk: at most 8 (register accumulators of the solve kernels) and
-    // bounded by their dynamic LDS (48 KiB)
-    nrhs_chunk_ = (int)std::max<int64_t>(1, std::min<int64_t>(8, (48 * 1024) / (std::max(maxwidth_, 256) * (int64_t)fsz)));
+    // bounded by their dynamic LDS (24 KiB)
+    nrhs_chunk_ = (int)std::max<int64_t>(1, std::min<int64_t>(8, (24 * 1024) / (std::max(maxwidth_, 256) * (int64_t)fsz)));
     d_part_ = alloc((size_t)std::max<int64_t>(bwd_part_, 1) * nrhs_chunk_ * fsz);
     d_ybuf_ = alloc((size_t)std::max<int64_t>(nyrows_, 1) * nrhs_chunk_ * fsz);
+    d_z_ = alloc((size_t)std::max<int64_t>(n_, 1) * nrhs_chunk_ * fsz);
+    if (!d_z_) return -1;
     if (!d_WB_ || !d_ybuf_) return -1;
     d_red_ = (double *)alloc((size_t)(512 * 32 + 64) * sizeof(double));
     if (!d_part_ || !d_red_ || !d_stat_) return -1;
@@ -367,24 +371,24 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
     const T *W1 = (const T *)d_W1_, *WB = (const T *)d_WB_;
     T *part = (T *)d_part_;
     T *ybuf = (T *)d_ybuf_;
-    const int64_t ldy = std::max<int64_t>(nyrows_, 1);
+    T *z = (T *)d_z_;
+    const int64_t ldy = std::max<int64_t>(nyrows_, 1), ldz = std::max<int64_t>(n_, 1);
     const int nl = (int)lv_cblk_ptr_.size() - 1;
     for (int r0 = 0; r0 < nrhs; r0 += nrhs_chunk_) {
         const int nr = std::min(nrhs_chunk_, nrhs - r0);
         T *xx = x + (int64_t)r0 * ldx;
         for (int lv = 0; lv < nl; lv++) {
-            const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
+            const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
             const int f0 = lv_fwd_ptr_[lv], nf = lv_fwd_ptr_[lv + 1] - f0;
-            launch_fwd_diag<T>(S_, d_lv_cblks_ + c0, nc, d_yrow_ptr_, d_yrow_pos_, ybuf, ldy, xx, ldx, nr, W1,
+            launch_fwd_diag<T>(S_, d_diag_ + d0, nd, d_yrow_ptr_, d_yrow_pos_, ybuf, ldy, xx, ldx, z, ldz, nr, W1,
                                maxwidth_, stream_);
-            launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, xx, ldx, nr, ybuf, ldy, maxwidth_, stream_);
+            launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, ybuf, ldy, maxwidth_, stream_);
         }
         for (int lv = nl - 1; lv >= 0; lv--) {
-            const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
+            const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
             const int b0 = lv_bwd_ptr_[lv], nb = lv_bwd_ptr_[lv + 1] - b0;
-            launch_bwd_prod<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, stream_);
-            launch_bwd_diag<T>(S_, d_lv_cblks_ + c0, nc, d_bwd_itemptr_ + c0 + lv, d_bwd_, xx, ldx, nr, part, WB,
-                               maxwidth_, stream_);
+            launch_bwd_prod<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, maxwidth_, stream_);
+            launch_bwd_diag<T>(S_, d_diag_ + d0, nd, d_bwd_, z, ldz, xx, ldx, nr, part, WB, maxwidth_, stream_);
         }
     }
     HIPCHK(hipGetLastError());

@@ -113,6 +113,9 @@ class Engine {
     void *d_wspace_ = nullptr;
     int64_t wcap_ = 0;  // workspace entries
     FwdItem *d_fwd_ = nullptr;
+    DiagItem *d_diag_ = nullptr;
+    std::vector<int32_t> lv_diag_ptr_;
+    void *d_z_ = nullptr;  // forward-sweep result, n x nrhs_chunk
     BwdItem *d_bwd_ = nullptr;
     int32_t *d_perm_ = nullptr;  // permtab as int32 for device permutations
     void *d_L0_ = nullptr, *d_U0_ = nullptr;  // snapshot of the filled coefficients

@@ -1,20 +1,25 @@
 // Supernodal forward / backward solve kernels for CDNA4 (gfx950)
 // (reference kernels/solve_ztrsmsp.c:67-557, sopalin/sequential_ztrsm.c:34-103).
 //
+// The sweeps read b from x and leave the solution in x; the forward result
+// z = inv(L) b lives in a separate buffer so that every workgroup of a cblk
+// can read the cblk's input while others write their rows of the output.
 // Forward sweep, level by level:
-//   k_fwd_diag : for every cblk c of the level, x_c -= the update-buffer rows
-//                that face c (a per-row CSR built at analysis, sources in
-//                increasing order), then x_c = inv(L_cc) x_c;
-//   k_fwd_prod : for the same cblks, ybuf[off-diagonal rows] = L_off * x_c on
-//                64-row tiles, the column range split over the 4 waves.
+//   k_fwd_diag : workgroup (c, ib) gathers, for rows [0, 64*ib+64) of cblk c,
+//                b minus the update-buffer rows that face them (a per-row CSR
+//                built at analysis, sources in increasing order), then writes
+//                rows [64*ib, 64*ib+64) of z_c = inv(L_cc) y;
+//   k_fwd_prod : ybuf[off-diagonal rows] = L_off z_c on 64-row tiles.
 // Backward sweep, levels from the root down:
-//   k_bwd_prod : partial[j] = sum over a 64-row tile r of L[r, j] x[row(r)],
-//                the tile staged through LDS so panel reads coalesce;
-//   k_bwd_diag : x_c = inv(L_cc)^T (x_c - sum of the tiles' partials), with the
-//                transposed inverse stored column-major.
-// Every output element is produced in a fixed order by one workgroup: no
-// atomics, bitwise-reproducible solves.  NR is the compile-time bound on the
-// right-hand sides of one launch (1 or 8).
+//   k_bwd_prod : partial[j] = sum over a 64x64 tile of L[r, j] x[row(r)];
+//   k_bwd_diag : workgroup (c, ib) forms y = z_c - the tiles' partials for
+//                columns [64*ib, K) and writes rows [64*ib, 64*ib+64) of
+//                x_c = inv(L_cc)^T y.
+// The diagonal inverses are stored whole (zeros in the other triangle), so
+// the triangular products run over full 64-column blocks.  Every output
+// element is produced in a fixed order by one workgroup: no atomics,
+// bitwise-reproducible solves.  NR is the compile-time bound on the
+// right-hand sides of one launch (1 or 8); NW the waves of a workgroup.
 #include "dev_common.h"
 #include "dev_types.h"
 #include "kernels.h"
@@ -23,80 +28,84 @@ namespace pxa {
 
 static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
 
-// out[i] = sum_k M[i, k] y[k] over the triangle of an n x n column-major M:
-// k <= i (LOWER) or k >= i (upper); rows in blocks of 64, k split in 4.
-template <typename T, int NR, bool LOWER>
-__device__ void tri_apply(const T *__restrict__ M, int n, const T *y, int nrhs, T *red, T *out, int64_t ldo) {
-    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
-    for (int ib = 0; ib < n; ib += 64) {
-        const int i = ib + r;
-        T acc[NR];
+// red holds NW x 64 x NR partial sums; the sum over waves is taken in wave
+// order by the threads of wave 0.
+template <typename T, int NR, int NW>
+__device__ __forceinline__ void wave_sum_store(T (&acc)[NR], T *red, int r, int q, bool ok, int nrhs, T *out,
+                                               int64_t ldo, const T *sub, int64_t lds) {
 #pragma unroll
-        for (int u = 0; u < NR; u++) acc[u] = 0;
-        if (i < n) {
-            const int klo = LOWER ? 0 : i, khi = LOWER ? i + 1 : n;
-            const int len = khi - klo, part = (len + 3) / 4;
-            const int ka = klo + q * part, kz = min(khi, ka + part);
-#pragma unroll 4
-            for (int k = ka; k < kz; k++) {
-                const T m = M[i + (int64_t)k * n];
+    for (int u = 0; u < NR; u++) red[(q * 64 + r) * NR + u] = acc[u];
+    __syncthreads();
+    if (q == 0 && ok) {
 #pragma unroll
-                for (int u = 0; u < NR; u++)
-                    if (u < nrhs) acc[u] += m * y[k + u * n];
+        for (int u = 0; u < NR; u++) {
+            if (u < nrhs) {
+                T v = red[r * NR + u];
+#pragma unroll
+                for (int w = 1; w < NW; w++) v += red[(w * 64 + r) * NR + u];
+                out[u * ldo] = sub ? sub[u * lds] - v : v;
             }
         }
-#pragma unroll
-        for (int u = 0; u < NR; u++) red[(q * 64 + r) * NR + u] = acc[u];
-        __syncthreads();
-        if (q == 0 && i < n) {
-#pragma unroll
-            for (int u = 0; u < NR; u++)
-                if (u < nrhs)
-                    out[i + u * ldo] = red[(0 * 64 + r) * NR + u] + red[(1 * 64 + r) * NR + u] +
-                                       red[(2 * 64 + r) * NR + u] + red[(3 * 64 + r) * NR + u];
-        }
-        __syncthreads();
     }
 }
 
-template <typename T, int NR>
-__global__ __launch_bounds__(256) void k_fwd_diag(DevSymbol S, const int32_t *__restrict__ cblks,
-                                                  const int64_t *__restrict__ yrow_ptr,
-                                                  const int32_t *__restrict__ yrow_pos, const T *__restrict__ ybuf,
-                                                  int64_t ldy, T *__restrict__ x, int64_t ldx, int nrhs,
-                                                  const T *__restrict__ w1) {
+template <typename T, int NR, int NW>
+__global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
+                                                      const int64_t *__restrict__ yrow_ptr,
+                                                      const int32_t *__restrict__ yrow_pos,
+                                                      const T *__restrict__ ybuf, int64_t ldy,
+                                                      const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
+                                                      int64_t ldz, int nrhs, const T *__restrict__ w1) {
     extern __shared__ char smem_raw[];
     T *y = reinterpret_cast<T *>(smem_raw);  // maxwidth * NR
-    __shared__ T red[256 * NR];
-    const int tid = threadIdx.x;
-    const int c = cblks[blockIdx.x];
+    __shared__ T red[NW * 64 * NR];
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+    const DiagItem it = items[id];
+    const int c = it.c, r0 = it.ib * 64;
     const int fc = S.cblk_fcol[c];
     const int Kc = S.cblk_fcol[c + 1] - fc;
-    for (int t = tid; t < Kc * nrhs; t += 256) {
-        const int i = t % Kc, rhs = t / Kc;
+    const int nrow = min(Kc, r0 + 64);
+    for (int t = tid; t < nrow * nrhs; t += NW * 64) {
+        const int i = t % nrow, u = t / nrow;
         const int g = fc + i;
-        const T *yb = ybuf + rhs * ldy;
-        T v = x[g + rhs * ldx];
+        const T *yb = ybuf + u * ldy;
+        T v = b[g + u * ldb];
         int64_t p = yrow_ptr[g];
         const int64_t pe = yrow_ptr[g + 1];
         for (; p + 4 <= pe; p += 4) {
-            const int a = yrow_pos[p], b = yrow_pos[p + 1], cc = yrow_pos[p + 2], d = yrow_pos[p + 3];
-            v -= ((yb[a] + yb[b]) + (yb[cc] + yb[d]));
+            const int a0 = yrow_pos[p], a1 = yrow_pos[p + 1], a2 = yrow_pos[p + 2], a3 = yrow_pos[p + 3];
+            v -= ((yb[a0] + yb[a1]) + (yb[a2] + yb[a3]));
         }
         for (; p < pe; p++) v -= yb[yrow_pos[p]];
-        y[t] = v;
+        y[i + u * Kc] = v;
     }
     __syncthreads();
-    tri_apply<T, NR, true>(w1 + S.cblk_winv[c], Kc, y, nrhs, red, x + fc, ldx);
+    const int i = r0 + r;
+    T acc[NR];
+#pragma unroll
+    for (int u = 0; u < NR; u++) acc[u] = 0;
+    if (i < Kc) {
+        const T *M = w1 + S.cblk_winv[c] + i;
+#pragma unroll 4
+        for (int k = q; k < nrow; k += NW) {
+            const T m = M[(int64_t)k * Kc];
+#pragma unroll
+            for (int u = 0; u < NR; u++)
+                if (u < nrhs) acc[u] += m * y[k + u * Kc];
+        }
+    }
+    wave_sum_store<T, NR, NW>(acc, red, r, q, i < Kc, nrhs, z + fc + i, ldz, (const T *)nullptr, 0);
 }
 
-template <typename T, int NR>
-__global__ __launch_bounds__(256) void k_fwd_prod(DevSymbol S, const FwdItem *__restrict__ items, int nitems,
-                                                  const T *__restrict__ coef, const T *__restrict__ x, int64_t ldx,
-                                                  int nrhs, T *__restrict__ ybuf, int64_t ldy) {
+template <typename T, int NR, int NW>
+__global__ __launch_bounds__(NW * 64) void k_fwd_prod(DevSymbol S, const FwdItem *__restrict__ items, int nitems,
+                                                      const T *__restrict__ coef, const T *__restrict__ z,
+                                                      int64_t ldz, int nrhs, T *__restrict__ ybuf, int64_t ldy) {
     extern __shared__ char smem_raw[];
     T *xs = reinterpret_cast<T *>(smem_raw);  // Kc * NR
-    __shared__ T red[256 * NR];
+    __shared__ T red[NW * 64 * NR];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
@@ -105,7 +114,7 @@ __global__ __launch_bounds__(256) void k_fwd_prod(DevSymbol S, const FwdItem *__
     const int fc = S.cblk_fcol[c];
     const int Kc = S.cblk_fcol[c + 1] - fc;
     const int64_t ld = S.cblk_stride[c];
-    for (int t = tid; t < Kc * nrhs; t += 256) xs[t] = x[fc + (t % Kc) + (int64_t)(t / Kc) * ldx];
+    for (int t = tid; t < Kc * nrhs; t += NW * 64) xs[t] = z[fc + (t % Kc) + (int64_t)(t / Kc) * ldz];
     __syncthreads();
     const int nr = it.r1 - it.r0;
     T acc[NR];
@@ -113,28 +122,18 @@ __global__ __launch_bounds__(256) void k_fwd_prod(DevSymbol S, const FwdItem *__
     for (int u = 0; u < NR; u++) acc[u] = 0;
     if (r < nr) {
         const T *row = coef + S.cblk_coef[c] + it.r0 + r;
-        const int part = (Kc + 3) / 4;
-        const int ka = q * part, kz = min(Kc, ka + part);
 #pragma unroll 4
-        for (int k = ka; k < kz; k++) {
+        for (int k = q; k < Kc; k += NW) {
             const T a = row[(int64_t)k * ld];
 #pragma unroll
             for (int u = 0; u < NR; u++)
                 if (u < nrhs) acc[u] += a * xs[k + u * Kc];
         }
     }
-#pragma unroll
-    for (int u = 0; u < NR; u++) red[(q * 64 + r) * NR + u] = acc[u];
-    __syncthreads();
-    if (q == 0 && r < nr) {
-#pragma unroll
-        for (int u = 0; u < NR; u++)
-            if (u < nrhs)
-                ybuf[it.slot + r + u * ldy] = red[(0 * 64 + r) * NR + u] + red[(1 * 64 + r) * NR + u] +
-                                              red[(2 * 64 + r) * NR + u] + red[(3 * 64 + r) * NR + u];
-    }
+    wave_sum_store<T, NR, NW>(acc, red, r, q, r < nr, nrhs, ybuf + it.slot + r, ldy, (const T *)nullptr, 0);
 }
 
+// One 64x64 tile (item, column block blockIdx.y) of the backward products.
 template <typename T, int NR>
 __global__ __launch_bounds__(256) void k_bwd_prod(DevSymbol S, const BwdItem *__restrict__ items, int nitems,
                                                   const T *__restrict__ coef, const T *__restrict__ x, int64_t ldx,
@@ -148,6 +147,8 @@ __global__ __launch_bounds__(256) void k_bwd_prod(DevSymbol S, const BwdItem *__
     const BwdItem it = items[id];
     const int c = it.c;
     const int Kc = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+    const int k0 = blockIdx.y * 64;
+    if (k0 >= Kc) return;
     const int64_t ld = S.cblk_stride[c];
     const int64_t rb = S.cblk_rowbase[c];
     const int nr = it.r1 - it.r0;
@@ -156,56 +157,72 @@ __global__ __launch_bounds__(256) void k_bwd_prod(DevSymbol S, const BwdItem *__
         xr[t] = (row < nr && u < nrhs) ? x[S.panel_grow[rb + it.r0 + row] + u * ldx] : (T)0;
     }
     const T *P = coef + S.cblk_coef[c] + it.r0;
-    T *out = part + (int64_t)it.slot * nrhs;
-    for (int k0 = 0; k0 < Kc; k0 += 64) {
-        __syncthreads();
-        for (int cc = q; cc < 64; cc += 4)
-            tile[lane][cc] = (lane < nr && k0 + cc < Kc) ? P[lane + (int64_t)(k0 + cc) * ld] : (T)0;
-        __syncthreads();
+    for (int cc = q; cc < 64; cc += 4)
+        tile[lane][cc] = (lane < nr && k0 + cc < Kc) ? P[lane + (int64_t)(k0 + cc) * ld] : (T)0;
+    __syncthreads();
+    T acc[NR];
+#pragma unroll
+    for (int u = 0; u < NR; u++) acc[u] = 0;
+#pragma unroll 4
+    for (int row = q * 16; row < q * 16 + 16; row++) {
+        const T a = tile[row][lane];
+#pragma unroll
+        for (int u = 0; u < NR; u++) acc[u] += a * xr[row + u * 64];
+    }
+    T *out = part + (int64_t)it.slot * nrhs + k0 + lane;
+    wave_sum_store<T, NR, 4>(acc, red, lane, q, k0 + lane < Kc, nrhs, out, Kc, (const T *)nullptr, 0);
+}
+
+template <typename T, int NR, int NW>
+__global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
+                                                      const BwdItem *__restrict__ bitems, const T *__restrict__ z,
+                                                      int64_t ldz, T *__restrict__ x, int64_t ldx, int nrhs,
+                                                      const T *__restrict__ part, const T *__restrict__ wb) {
+    extern __shared__ char smem_raw[];
+    T *y = reinterpret_cast<T *>(smem_raw);  // maxwidth * NR
+    __shared__ T red[NW * 64 * NR];
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+    const DiagItem it = items[id];
+    const int c = it.c, r0 = it.ib * 64;
+    const int fc = S.cblk_fcol[c];
+    const int Kc = S.cblk_fcol[c + 1] - fc;
+    // the tiles of one cblk own consecutive partial slots, Kc apart
+    const int64_t s0 = (it.b1 > it.b0) ? (int64_t)bitems[it.b0].slot : 0;
+    for (int kc = r0; kc < Kc; kc += 64) {
+        const int k = kc + r;
         T acc[NR];
 #pragma unroll
         for (int u = 0; u < NR; u++) acc[u] = 0;
+        if (k < Kc) {
 #pragma unroll 4
-        for (int row = q * 16; row < q * 16 + 16; row++) {
-            const T a = tile[row][lane];
+            for (int j = it.b0 + q; j < it.b1; j += NW) {
+                const T *pp = part + (s0 + (int64_t)(j - it.b0) * Kc) * nrhs + k;
 #pragma unroll
-            for (int u = 0; u < NR; u++) acc[u] += a * xr[row + u * 64];
+                for (int u = 0; u < NR; u++)
+                    if (u < nrhs) acc[u] += pp[u * Kc];
+            }
         }
-#pragma unroll
-        for (int u = 0; u < NR; u++) red[(q * 64 + lane) * NR + u] = acc[u];
+        wave_sum_store<T, NR, NW>(acc, red, r, q, k < Kc, nrhs, y + k, Kc, z + fc + k, ldz);
         __syncthreads();
-        if (q == 0 && k0 + lane < Kc) {
+    }
+    const int i = r0 + r;
+    T acc[NR];
+#pragma unroll
+    for (int u = 0; u < NR; u++) acc[u] = 0;
+    if (i < Kc) {
+        // wb holds inv(L_cc)^T (or inv(U_cc)) column-major: upper triangular
+        const T *M = wb + S.cblk_winv[c] + i;
+#pragma unroll 4
+        for (int k = r0 + q; k < Kc; k += NW) {
+            const T m = M[(int64_t)k * Kc];
 #pragma unroll
             for (int u = 0; u < NR; u++)
-                if (u < nrhs)
-                    out[k0 + lane + u * Kc] = red[(0 * 64 + lane) * NR + u] + red[(1 * 64 + lane) * NR + u] +
-                                              red[(2 * 64 + lane) * NR + u] + red[(3 * 64 + lane) * NR + u];
+                if (u < nrhs) acc[u] += m * y[k + u * Kc];
         }
     }
-}
-
-template <typename T, int NR>
-__global__ __launch_bounds__(256) void k_bwd_diag(DevSymbol S, const int32_t *__restrict__ cblks,
-                                                  const int32_t *__restrict__ itemptr,
-                                                  const BwdItem *__restrict__ items, T *__restrict__ x, int64_t ldx,
-                                                  int nrhs, const T *__restrict__ part, const T *__restrict__ wb) {
-    extern __shared__ char smem_raw[];
-    T *y = reinterpret_cast<T *>(smem_raw);
-    __shared__ T red[256 * NR];
-    const int tid = threadIdx.x;
-    const int c = cblks[blockIdx.x];
-    const int fc = S.cblk_fcol[c];
-    const int Kc = S.cblk_fcol[c + 1] - fc;
-    const int i0 = itemptr[blockIdx.x], i1 = itemptr[blockIdx.x + 1];
-    for (int t = tid; t < Kc * nrhs; t += 256) {
-        const int i = t % Kc, rhs = t / Kc;
-        T v = x[fc + i + rhs * ldx];
-        for (int k = i0; k < i1; k++) v -= part[(int64_t)items[k].slot * nrhs + t];
-        y[t] = v;
-    }
-    __syncthreads();
-    // wb holds inv(L_cc)^T (or inv(U_cc)) column-major: upper triangular
-    tri_apply<T, NR, false>(wb + S.cblk_winv[c], Kc, y, nrhs, red, x + fc, ldx);
+    wave_sum_store<T, NR, NW>(acc, red, r, q, i < Kc, nrhs, x + fc + i, ldx, (const T *)nullptr, 0);
 }
 
 // Transposed copy of the diagonal inverses of a level: WT[c] = W[c]^T.
@@ -235,50 +252,52 @@ __global__ __launch_bounds__(256) void k_winv_transpose(DevSymbol S, const int32
 
 // ------------------------------- launchers ---------------------------------
 template <typename T>
-void launch_fwd_diag(const DevSymbol &S, const int32_t *cblks, int n, const int64_t *yrow_ptr,
-                     const int32_t *yrow_pos, const T *ybuf, int64_t ldy, T *x, int64_t ldx, int nrhs, const T *w1,
-                     int maxwidth, hipStream_t st) {
+void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int64_t *yrow_ptr,
+                     const int32_t *yrow_pos, const T *ybuf, int64_t ldy, const T *b, int64_t ldb, T *z,
+                     int64_t ldz, int nrhs, const T *w1, int maxwidth, hipStream_t st) {
     if (n <= 0) return;
     size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
     if (nrhs == 1)
-        hipLaunchKernelGGL((k_fwd_diag<T, 1>), dim3(n), dim3(256), sh, st, S, cblks, yrow_ptr, yrow_pos, ybuf, ldy, x,
-                           ldx, nrhs, w1);
+        hipLaunchKernelGGL((k_fwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), sh, st, S, items, n, yrow_ptr,
+                           yrow_pos, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
     else
-        hipLaunchKernelGGL((k_fwd_diag<T, 8>), dim3(n), dim3(256), sh, st, S, cblks, yrow_ptr, yrow_pos, ybuf, ldy, x,
-                           ldx, nrhs, w1);
+        hipLaunchKernelGGL((k_fwd_diag<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, yrow_ptr, yrow_pos,
+                           ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
 }
 template <typename T>
-void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
+void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *z, int64_t ldz,
                      int nrhs, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st) {
     if (n <= 0) return;
     size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
     if (nrhs == 1)
-        hipLaunchKernelGGL((k_fwd_prod<T, 1>), dim3(grid8(n)), dim3(256), sh, st, S, items, n, coef, x, ldx, nrhs, ybuf,
-                           ldy);
+        hipLaunchKernelGGL((k_fwd_prod<T, 1, 16>), dim3(grid8(n)), dim3(1024), sh, st, S, items, n, coef, z, ldz,
+                           nrhs, ybuf, ldy);
     else
-        hipLaunchKernelGGL((k_fwd_prod<T, 8>), dim3(grid8(n)), dim3(256), sh, st, S, items, n, coef, x, ldx, nrhs, ybuf,
-                           ldy);
+        hipLaunchKernelGGL((k_fwd_prod<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, coef, z, ldz, nrhs,
+                           ybuf, ldy);
 }
 template <typename T>
 void launch_bwd_prod(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
-                     int nrhs, T *part, hipStream_t st) {
+                     int nrhs, T *part, int maxwidth, hipStream_t st) {
     if (n <= 0) return;
+    const dim3 g(grid8(n), (maxwidth + 63) / 64);
     if (nrhs == 1)
-        hipLaunchKernelGGL((k_bwd_prod<T, 1>), dim3(grid8(n)), dim3(256), 0, st, S, items, n, coef, x, ldx, nrhs, part);
+        hipLaunchKernelGGL((k_bwd_prod<T, 1>), g, dim3(256), 0, st, S, items, n, coef, x, ldx, nrhs, part);
     else
-        hipLaunchKernelGGL((k_bwd_prod<T, 8>), dim3(grid8(n)), dim3(256), 0, st, S, items, n, coef, x, ldx, nrhs, part);
+        hipLaunchKernelGGL((k_bwd_prod<T, 8>), g, dim3(256), 0, st, S, items, n, coef, x, ldx, nrhs, part);
 }
 template <typename T>
-void launch_bwd_diag(const DevSymbol &S, const int32_t *cblks, int n, const int32_t *itemptr, const BwdItem *items,
-                     T *x, int64_t ldx, int nrhs, const T *part, const T *wb, int maxwidth, hipStream_t st) {
+void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const BwdItem *bitems, const T *z,
+                     int64_t ldz, T *x, int64_t ldx, int nrhs, const T *part, const T *wb, int maxwidth,
+                     hipStream_t st) {
     if (n <= 0) return;
     size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
     if (nrhs == 1)
-        hipLaunchKernelGGL((k_bwd_diag<T, 1>), dim3(n), dim3(256), sh, st, S, cblks, itemptr, items, x, ldx, nrhs, part,
-                           wb);
+        hipLaunchKernelGGL((k_bwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), sh, st, S, items, n, bitems, z, ldz, x,
+                           ldx, nrhs, part, wb);
     else
-        hipLaunchKernelGGL((k_bwd_diag<T, 8>), dim3(n), dim3(256), sh, st, S, cblks, itemptr, items, x, ldx, nrhs, part,
-                           wb);
+        hipLaunchKernelGGL((k_bwd_diag<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, bitems, z, ldz, x,
+                           ldx, nrhs, part, wb);
 }
 template <typename T>
 void launch_winv_transpose(const DevSymbol &S, const int32_t *cblks, int n, const T *w, T *wt, hipStream_t st) {
@@ -287,14 +306,15 @@ void launch_winv_transpose(const DevSymbol &S, const int32_t *cblks, int n, cons
 }
 
 #define INST(T)                                                                                                   \
-    template void launch_fwd_diag<T>(const DevSymbol &, const int32_t *, int, const int64_t *, const int32_t *,  \
-                                     const T *, int64_t, T *, int64_t, int, const T *, int, hipStream_t);        \
+    template void launch_fwd_diag<T>(const DevSymbol &, const DiagItem *, int, const int64_t *, const int32_t *, \
+                                     const T *, int64_t, const T *, int64_t, T *, int64_t, int, const T *, int,  \
+                                     hipStream_t);                                                               \
     template void launch_fwd_prod<T>(const DevSymbol &, const FwdItem *, int, const T *, const T *, int64_t, int, \
                                      T *, int64_t, int, hipStream_t);                                            \
     template void launch_bwd_prod<T>(const DevSymbol &, const BwdItem *, int, const T *, const T *, int64_t, int, \
-                                     T *, hipStream_t);                                                          \
-    template void launch_bwd_diag<T>(const DevSymbol &, const int32_t *, int, const int32_t *, const BwdItem *,   \
-                                     T *, int64_t, int, const T *, const T *, int, hipStream_t);                 \
+                                     T *, int, hipStream_t);                                                     \
+    template void launch_bwd_diag<T>(const DevSymbol &, const DiagItem *, int, const BwdItem *, const T *,        \
+                                     int64_t, T *, int64_t, int, const T *, const T *, int, hipStream_t);        \
     template void launch_winv_transpose<T>(const DevSymbol &, const int32_t *, int, const T *, T *, hipStream_t);
 INST(double)
 INST(float)

@@ -81,6 +81,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     P.lv_fwd_ptr.assign(nl + 1, 0);
     P.lv_bwd_ptr.assign(nl + 1, 0);
     P.lv_wt_ptr.assign(nl + 1, 0);
+    P.lv_diag_ptr.assign(nl + 1, 0);
     P.lv_wsize.assign(nl, 0);
     P.bwd_itemptr.assign(cb + nl, 0);
     // update buffer: one row per off-diagonal panel row; CSR by permuted row
@@ -240,10 +241,16 @@ Plan build_plan(const Symbol &s, const Layout &L) {
             }
         }
 
-        // product groups and their 64x64 tiles for the multi-source tiles
+        // product groups of the multi-source tiles and their 64x64 tiles
         {
+            struct TileKey {
+                int32_t g;
+                int64_t qp;  // qt << 32 | pt
+                bool operator<(const TileKey &o) const { return g != o.g ? g < o.g : qp < o.qp; }
+                bool operator==(const TileKey &o) const { return g == o.g && qp == o.qp; }
+            };
             std::map<std::pair<int32_t, int32_t>, int32_t> gid;
-            std::vector<std::pair<int32_t, int64_t>> tl;  // (group, qt << 32 | pt)
+            std::vector<TileKey> tl;
             int64_t wsz = 0;
             for (int32_t t = split; t < (int32_t)P.upd.size(); t++)
                 for (int32_t e = P.upd[t].e0; e < P.upd[t].e1; e++) {
@@ -269,7 +276,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                     const int64_t base = L.blok_roff[en.ib0];
                     for (int64_t pt = (en.p0 - base) / 64; pt <= (en.p1 - 1 - base) / 64; pt++)
                         for (int64_t qt = (en.q0 - base) / 64; qt <= (en.q1 - 1 - base) / 64; qt++)
-                            tl.push_back({g, (qt << 32) | pt});
+                            tl.push_back(TileKey{g, (qt << 32) | pt});
                 }
             std::sort(tl.begin(), tl.end());
             tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
@@ -279,18 +286,19 @@ Plan build_plan(const Symbol &s, const Layout &L) {
             std::vector<std::pair<int32_t, int32_t>> gsrc(gid.size());
             for (const auto &kv : gid) gsrc[kv.second - g0] = kv.first;
             for (const auto &t : tl) {
-                const WGroup &G = P.wgrp[t.first];
-                const auto &sk = gsrc[t.first - g0];
-                const int64_t pt = t.second & 0xFFFFFFFF, qt = t.second >> 32;
+                const WGroup &G = P.wgrp[t.g];
+                const auto &sk = gsrc[t.g - g0];
+                const int64_t ts = 64;
+                const int64_t pt = t.qp & 0xFFFFFFFF, qt = t.qp >> 32;
                 const int64_t base = L.blok_roff[sk.second];
                 WTile w;
                 w.s = sk.first;
-                w.p0 = (int32_t)(base + pt * 64);
-                w.q0 = (int32_t)(base + qt * 64);
-                w.mp = (int32_t)std::min<int64_t>(64, G.ldw - pt * 64);
-                w.mq = (int32_t)std::min<int64_t>(64, G.nq - qt * 64);
+                w.p0 = (int32_t)(base + pt * ts);
+                w.q0 = (int32_t)(base + qt * ts);
+                w.mp = (int32_t)std::min<int64_t>(ts, G.ldw - pt * ts);
+                w.mq = (int32_t)std::min<int64_t>(ts, G.nq - qt * ts);
                 w.ldw = G.ldw;
-                w.wofs = G.wofs + pt * 64 + qt * 64 * (int64_t)G.ldw;
+                w.wofs = G.wofs + pt * ts + qt * ts * (int64_t)G.ldw;
                 P.wtiles.push_back(w);
                 P.flops_wtile += 2.0 * w.mp * w.mq * (double)width(w.s);
             }
@@ -313,14 +321,18 @@ Plan build_plan(const Symbol &s, const Layout &L) {
             const int32_t seg = l0 + lv;
             for (int32_t k = l0; k < l1; k++) {
                 const int64_t c = P.lv_cblks[k];
-                P.bwd_itemptr[seg + (k - l0)] = (int32_t)P.bwd.size();
+                const int32_t b0 = (int32_t)P.bwd.size();
+                P.bwd_itemptr[seg + (k - l0)] = b0;
                 const int64_t K = width(c), st = L.cblk_stride[c];
                 for (int64_t r0 = K; r0 < st; r0 += BWD_ROWS) {
                     int64_t r1 = std::min<int64_t>(st, r0 + BWD_ROWS);
                     P.bwd.push_back(BwdItem{(int32_t)c, (int32_t)r0, (int32_t)r1, (int32_t)slot});
                     slot += K;
                 }
+                for (int64_t ib = 0; ib * 64 < K; ib++)
+                    P.diag.push_back(DiagItem{(int32_t)c, (int32_t)ib, b0, (int32_t)P.bwd.size()});
             }
+            P.lv_diag_ptr[lv + 1] = (int32_t)P.diag.size();
             P.bwd_itemptr[seg + (l1 - l0)] = (int32_t)P.bwd.size();
             P.bwd_part = std::max(P.bwd_part, slot);
             P.lv_bwd_ptr[lv + 1] = (int32_t)P.bwd.size();

@@ -49,6 +49,9 @@ struct Plan {
     std::vector<int32_t> lv_bwd_ptr;
     std::vector<BwdItem> bwd;
     std::vector<int32_t> bwd_itemptr;
+    // diagonal solve steps: every 64-row block of the level's cblks
+    std::vector<int32_t> lv_diag_ptr;
+    std::vector<DiagItem> diag;
     int64_t bwd_part = 0;  // partial buffer size per right-hand side
     // panel row maps and inverse offsets
     std::vector<int32_t> panel_grow, panel_blk;

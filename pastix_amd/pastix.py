@@ -406,6 +406,17 @@ class DeviceArray:
         rc = self.hip().hipMemcpy(self.ptr, h.ctypes.data, self.nbytes, 1)
         assert rc == 0
 
+    def copy_from(self, other):
+        """Device-to-device copy of another buffer of the same size."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("copy_from: sizes differ")
+        rc = self.hip().hipMemcpy(self.ptr, other.ptr, self.nbytes, 3)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy device-to-device failed ({rc})")
+        rc = self.hip().hipDeviceSynchronize()
+        if rc != 0:
+            raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
     def download(self):
         out = np.empty(self.shape, dtype=self.dtype, order="F")
         rc = self.hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2)
