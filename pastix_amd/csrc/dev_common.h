@@ -91,29 +91,32 @@ __device__ __forceinline__ void quad_gemm_nt(Quad<T> &q, View<T> A, int ma, View
 }
 
 // Workgroup-level D(64x64) = X(64xK) * Y(64xK)^T with operands staged through
-// LDS in chunks of 16 columns: the 256 threads fetch each chunk once with
-// coalesced column segments (rows contiguous in memory), the 4 waves read
-// their MFMA fragments from LDS.  Global loads of chunk c+1 are in flight
-// while chunk c is multiplied.  Rows >= mx of X / >= my of Y read as zero;
-// X(r, k) = X[r + k*ldx].  Wave w owns the 32x32 quadrant (w>>1, w&1).
-// Must be called by all 256 threads (contains barriers).
-template <typename T> struct StageBuf {
-    T xs[16][68];
-    T ys[16][68];
+// LDS in chunks of CH columns (16 or 32): the 256 threads fetch each chunk
+// once with coalesced column segments (rows contiguous in memory), the 4
+// waves read their MFMA fragments from LDS.  Global loads of chunk c+1 are in
+// flight while chunk c is multiplied, so a chunk's MFMA time (CH/4 * 4
+// MFMAs per wave) is the latency the prefetch can hide.  Rows >= mx of X /
+// >= my of Y read as zero; X(r, k) = X[r + k*ldx].  Wave w owns the 32x32
+// quadrant (w>>1, w&1).  Must be called by all 256 threads (contains
+// barriers).
+template <typename T, int CH = 16> struct StageBuf {
+    T xs[CH][68];
+    T ys[CH][68];
 };
 
-template <typename T>
+template <typename T, int CH>
 __device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int64_t ldx, const T *Y, int my,
-                                            int64_t ldy, int K, StageBuf<T> &sb, int tid) {
+                                            int64_t ldy, int K, StageBuf<T, CH> &sb, int tid) {
+    constexpr int NU = CH / 4;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
     const int i = lane & 15, kq = lane >> 4;
     const int row = tid & 63, kk = tid >> 6;
     const bool rx = row < mx, ry = row < my;
-    T gx[4], gy[4];
+    T gx[NU], gy[NU];
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < NU; u++) {
             const int k = k0 + kk + 4 * u;
             gx[u] = (rx && k < K) ? X[row + (int64_t)k * ldx] : (T)0;
             gy[u] = (ry && k < K) ? Y[row + (int64_t)k * ldy] : (T)0;
@@ -121,7 +124,7 @@ __device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int6
     };
     auto sstore = [&]() {
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < NU; u++) {
             sb.xs[kk + 4 * u][row] = gx[u];
             sb.ys[kk + 4 * u][row] = gy[u];
         }
@@ -130,12 +133,12 @@ __device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int6
     gload(0);
     sstore();
     __syncthreads();
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const bool more = k0 + 16 < K;
-        if (more) gload(k0 + 16);
+    for (int k0 = 0; k0 < K; k0 += CH) {
+        const bool more = k0 + CH < K;
+        if (more) gload(k0 + CH);
         if (active) {
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
+            for (int s = 0; s < CH / 4; s++) {
                 if (k0 + 4 * s < K) {
                     const int kr = 4 * s + kq;
                     const T a0 = sb.xs[kr][wr + i], a1 = sb.xs[kr][wr + 16 + i];

@@ -105,7 +105,7 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     n_wide_ = (int)P.wide_cblks.size();
     d_bwd_itemptr_ = upload(*this, P.bwd_itemptr);
     d_yrow_ptr_ = upload(*this, P.yrow_ptr);
-    d_yrow_pos_ = upload(*this, P.yrow_pos);
+    d_ydst_ = upload(*this, P.ydst);
     nyrows_ = P.nyrows;
     d_shift_ = upload(*this, P.shift);
     d_trsm_ = upload(*this, P.trsm);
@@ -380,9 +380,9 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
         for (int lv = 0; lv < nl; lv++) {
             const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
             const int f0 = lv_fwd_ptr_[lv], nf = lv_fwd_ptr_[lv + 1] - f0;
-            launch_fwd_diag<T>(S_, d_diag_ + d0, nd, d_yrow_ptr_, d_yrow_pos_, ybuf, ldy, xx, ldx, z, ldz, nr, W1,
+            launch_fwd_diag<T>(S_, d_diag_ + d0, nd, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, nr, W1,
                                maxwidth_, stream_);
-            launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, ybuf, ldy, maxwidth_, stream_);
+            launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, d_ydst_, ybuf, ldy, maxwidth_, stream_);
         }
         for (int lv = nl - 1; lv >= 0; lv--) {
             const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;

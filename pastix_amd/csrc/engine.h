@@ -95,7 +95,7 @@ class Engine {
     // device plan
     int32_t *d_lv_cblks_ = nullptr, *d_bwd_itemptr_ = nullptr, *d_shift_ = nullptr;
     int64_t *d_yrow_ptr_ = nullptr;
-    int32_t *d_yrow_pos_ = nullptr;
+    int32_t *d_ydst_ = nullptr;
     int64_t nyrows_ = 0;
     void *d_ybuf_ = nullptr, *d_WB_ = nullptr;
     TrsmItem *d_trsm_ = nullptr;

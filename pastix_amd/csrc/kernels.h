@@ -42,12 +42,12 @@ void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool t
 // Supernodal triangular solves (reference kernels/solve_ztrsmsp.c:332-557).
 // nrhs <= 8 per launch; the forward sweep writes z, the backward sweep x.
 template <typename T>
-void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int64_t *yrow_ptr,
-                     const int32_t *yrow_pos, const T *ybuf, int64_t ldy, const T *b, int64_t ldb, T *z,
-                     int64_t ldz, int nrhs, const T *w1, int maxwidth, hipStream_t st);
+void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int64_t *yrow_ptr, const T *ybuf,
+                     int64_t ldy, const T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
+                     hipStream_t st);
 template <typename T>
 void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *z, int64_t ldz,
-                     int nrhs, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st);
+                     int nrhs, const int32_t *ydst, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st);
 template <typename T>
 void launch_bwd_prod(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
                      int nrhs, T *part, int maxwidth, hipStream_t st);

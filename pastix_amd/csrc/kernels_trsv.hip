@@ -6,10 +6,11 @@
 // can read the cblk's input while others write their rows of the output.
 // Forward sweep, level by level:
 //   k_fwd_diag : workgroup (c, ib) gathers, for rows [0, 64*ib+64) of cblk c,
-//                b minus the update-buffer rows that face them (a per-row CSR
-//                built at analysis, sources in increasing order), then writes
-//                rows [64*ib, 64*ib+64) of z_c = inv(L_cc) y;
-//   k_fwd_prod : ybuf[off-diagonal rows] = L_off z_c on 64-row tiles.
+//                b minus the update-buffer entries of these rows (stored
+//                contiguously per row, sources in increasing order), then
+//                writes rows [64*ib, 64*ib+64) of z_c = inv(L_cc) y;
+//   k_fwd_prod : L_off z_c on 64-row tiles, each product row stored at its
+//                entry of the update buffer.
 // Backward sweep, levels from the root down:
 //   k_bwd_prod : partial[j] = sum over a 64x64 tile of L[r, j] x[row(r)];
 //   k_bwd_diag : workgroup (c, ib) forms y = z_c - the tiles' partials for
@@ -52,7 +53,6 @@ __device__ __forceinline__ void wave_sum_store(T (&acc)[NR], T *red, int r, int 
 template <typename T, int NR, int NW>
 __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
                                                       const int64_t *__restrict__ yrow_ptr,
-                                                      const int32_t *__restrict__ yrow_pos,
                                                       const T *__restrict__ ybuf, int64_t ldy,
                                                       const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
                                                       int64_t ldz, int nrhs, const T *__restrict__ w1) {
@@ -74,11 +74,9 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
         T v = b[g + u * ldb];
         int64_t p = yrow_ptr[g];
         const int64_t pe = yrow_ptr[g + 1];
-        for (; p + 4 <= pe; p += 4) {
-            const int a0 = yrow_pos[p], a1 = yrow_pos[p + 1], a2 = yrow_pos[p + 2], a3 = yrow_pos[p + 3];
-            v -= ((yb[a0] + yb[a1]) + (yb[a2] + yb[a3]));
-        }
-        for (; p < pe; p++) v -= yb[yrow_pos[p]];
+#pragma unroll 4
+        for (; p + 4 <= pe; p += 4) v -= ((yb[p] + yb[p + 1]) + (yb[p + 2] + yb[p + 3]));
+        for (; p < pe; p++) v -= yb[p];
         y[i + u * Kc] = v;
     }
     __syncthreads();
@@ -88,7 +86,7 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
     for (int u = 0; u < NR; u++) acc[u] = 0;
     if (i < Kc) {
         const T *M = w1 + S.cblk_winv[c] + i;
-#pragma unroll 4
+#pragma unroll 8
         for (int k = q; k < nrow; k += NW) {
             const T m = M[(int64_t)k * Kc];
 #pragma unroll
@@ -102,7 +100,8 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
 template <typename T, int NR, int NW>
 __global__ __launch_bounds__(NW * 64) void k_fwd_prod(DevSymbol S, const FwdItem *__restrict__ items, int nitems,
                                                       const T *__restrict__ coef, const T *__restrict__ z,
-                                                      int64_t ldz, int nrhs, T *__restrict__ ybuf, int64_t ldy) {
+                                                      int64_t ldz, int nrhs, const int32_t *__restrict__ ydst,
+                                                      T *__restrict__ ybuf, int64_t ldy) {
     extern __shared__ char smem_raw[];
     T *xs = reinterpret_cast<T *>(smem_raw);  // Kc * NR
     __shared__ T red[NW * 64 * NR];
@@ -130,7 +129,8 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_prod(DevSymbol S, const FwdItem
                 if (u < nrhs) acc[u] += a * xs[k + u * Kc];
         }
     }
-    wave_sum_store<T, NR, NW>(acc, red, r, q, r < nr, nrhs, ybuf + it.slot + r, ldy, (const T *)nullptr, 0);
+    const int dst = (r < nr) ? ydst[it.slot + r] : 0;
+    wave_sum_store<T, NR, NW>(acc, red, r, q, r < nr, nrhs, ybuf + dst, ldy, (const T *)nullptr, 0);
 }
 
 // One 64x64 tile (item, column block blockIdx.y) of the backward products.
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagIte
     if (i < Kc) {
         // wb holds inv(L_cc)^T (or inv(U_cc)) column-major: upper triangular
         const T *M = wb + S.cblk_winv[c] + i;
-#pragma unroll 4
+#pragma unroll 8
         for (int k = r0 + q; k < Kc; k += NW) {
             const T m = M[(int64_t)k * Kc];
 #pragma unroll
@@ -252,29 +252,29 @@ __global__ __launch_bounds__(256) void k_winv_transpose(DevSymbol S, const int32
 
 // ------------------------------- launchers ---------------------------------
 template <typename T>
-void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int64_t *yrow_ptr,
-                     const int32_t *yrow_pos, const T *ybuf, int64_t ldy, const T *b, int64_t ldb, T *z,
-                     int64_t ldz, int nrhs, const T *w1, int maxwidth, hipStream_t st) {
+void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int64_t *yrow_ptr, const T *ybuf,
+                     int64_t ldy, const T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
+                     hipStream_t st) {
     if (n <= 0) return;
     size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
     if (nrhs == 1)
-        hipLaunchKernelGGL((k_fwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), sh, st, S, items, n, yrow_ptr,
-                           yrow_pos, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+        hipLaunchKernelGGL((k_fwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), sh, st, S, items, n, yrow_ptr, ybuf,
+                           ldy, b, ldb, z, ldz, nrhs, w1);
     else
-        hipLaunchKernelGGL((k_fwd_diag<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, yrow_ptr, yrow_pos,
-                           ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+        hipLaunchKernelGGL((k_fwd_diag<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, yrow_ptr, ybuf, ldy,
+                           b, ldb, z, ldz, nrhs, w1);
 }
 template <typename T>
 void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *z, int64_t ldz,
-                     int nrhs, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st) {
+                     int nrhs, const int32_t *ydst, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st) {
     if (n <= 0) return;
     size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
     if (nrhs == 1)
         hipLaunchKernelGGL((k_fwd_prod<T, 1, 16>), dim3(grid8(n)), dim3(1024), sh, st, S, items, n, coef, z, ldz,
-                           nrhs, ybuf, ldy);
+                           nrhs, ydst, ybuf, ldy);
     else
         hipLaunchKernelGGL((k_fwd_prod<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, coef, z, ldz, nrhs,
-                           ybuf, ldy);
+                           ydst, ybuf, ldy);
 }
 template <typename T>
 void launch_bwd_prod(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
@@ -306,11 +306,11 @@ void launch_winv_transpose(const DevSymbol &S, const int32_t *cblks, int n, cons
 }
 
 #define INST(T)                                                                                                   \
-    template void launch_fwd_diag<T>(const DevSymbol &, const DiagItem *, int, const int64_t *, const int32_t *, \
-                                     const T *, int64_t, const T *, int64_t, T *, int64_t, int, const T *, int,  \
+    template void launch_fwd_diag<T>(const DevSymbol &, const DiagItem *, int, const int64_t *, const T *,       \
+                                     int64_t, const T *, int64_t, T *, int64_t, int, const T *, int,             \
                                      hipStream_t);                                                               \
     template void launch_fwd_prod<T>(const DevSymbol &, const FwdItem *, int, const T *, const T *, int64_t, int, \
-                                     T *, int64_t, int, hipStream_t);                                            \
+                                     const int32_t *, T *, int64_t, int, hipStream_t);                           \
     template void launch_bwd_prod<T>(const DevSymbol &, const BwdItem *, int, const T *, const T *, int64_t, int, \
                                      T *, int, hipStream_t);                                                     \
     template void launch_bwd_diag<T>(const DevSymbol &, const DiagItem *, int, const BwdItem *, const T *,        \

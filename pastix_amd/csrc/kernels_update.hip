@@ -19,6 +19,11 @@
 
 namespace pxa {
 
+// K columns staged per LDS chunk in the single-tile GEMMs (16 measured
+// faster than 32 on gfx950: the smaller LDS footprint keeps 6 workgroups per
+// CU resident).
+constexpr int UPD_CH = 16;
+
 #define UQFOR(...)                                                       \
     for (int m_ = 0; m_ < 2; m_++)                                       \
         for (int n_ = 0; n_ < 2; n_++)                                   \
@@ -53,7 +58,7 @@ __global__ __launch_bounds__(256) void k_update_direct(DevSymbol S, const UpdIte
                                                        const UpdEntry *__restrict__ ents,
                                                        const int32_t *__restrict__ shift, const T *__restrict__ Acoef,
                                                        const T *__restrict__ Bcoef, T *__restrict__ Ccoef) {
-    __shared__ StageBuf<T> sb;
+    __shared__ StageBuf<T, UPD_CH> sb;
     __shared__ int rmap[64], cmap[64], rblk[64], qblk[64];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
@@ -71,7 +76,7 @@ __global__ __launch_bounds__(256) void k_update_direct(DevSymbol S, const UpdIte
     Quad<T> q;
     q.zero();
     // D(j, i) = B(q_j) . A(p_i): quadrant rows index columns of the target
-    wg_gemm_lds<T>(q, Bcoef + S.cblk_coef[s] + en.q0, nq, ld, Acoef + S.cblk_coef[s] + en.p0, np, ld, K, sb, tid);
+    wg_gemm_lds<T, UPD_CH>(q, Bcoef + S.cblk_coef[s] + en.q0, nq, ld, Acoef + S.cblk_coef[s] + en.p0, np, ld, K, sb, tid);
     const int64_t stf = S.cblk_stride[f];
     T *C = Ccoef + S.cblk_coef[f] + r0 + (int64_t)c0 * stf;
     if (wr < nq && wc < np) {
@@ -91,7 +96,7 @@ __global__ __launch_bounds__(256) void k_update_acc(DevSymbol S, const UpdItem *
                                                     const T *__restrict__ Acoef, const T *__restrict__ Bcoef,
                                                     T *__restrict__ Ccoef) {
     __shared__ T acc[64][65];
-    __shared__ StageBuf<T> sb;
+    __shared__ StageBuf<T, 16> sb;
     __shared__ int rmap[64], cmap[64], rblk[64], qblk[64];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
@@ -112,7 +117,7 @@ __global__ __launch_bounds__(256) void k_update_acc(DevSymbol S, const UpdItem *
         q.zero();
         // wg_gemm_lds starts with a barrier-protected LDS fill: the maps are
         // visible to every thread once it returns
-        wg_gemm_lds<T>(q, Acoef + S.cblk_coef[s] + en.p0, np, ld, Bcoef + S.cblk_coef[s] + en.q0, nq, ld, K, sb,
+        wg_gemm_lds<T, 16>(q, Acoef + S.cblk_coef[s] + en.p0, np, ld, Bcoef + S.cblk_coef[s] + en.q0, nq, ld, K, sb,
                        tid);
         if (wr < np && wc < nq) {
             UQFOR({
@@ -146,7 +151,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_update_wgemm(DevSymbol S, const WTile *__restrict__ items, int nitems,
                                                       const T *__restrict__ Acoef, const T *__restrict__ Bcoef,
                                                       T *__restrict__ W) {
-    __shared__ StageBuf<T> sb;
+    __shared__ StageBuf<T, UPD_CH> sb;
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -157,7 +162,7 @@ __global__ __launch_bounds__(256) void k_update_wgemm(DevSymbol S, const WTile *
     const int64_t ld = S.cblk_stride[s];
     Quad<T> q;
     q.zero();
-    wg_gemm_lds<T>(q, Bcoef + S.cblk_coef[s] + w.q0, w.mq, ld, Acoef + S.cblk_coef[s] + w.p0, w.mp, ld, K, sb, tid);
+    wg_gemm_lds<T, UPD_CH>(q, Bcoef + S.cblk_coef[s] + w.q0, w.mq, ld, Acoef + S.cblk_coef[s] + w.p0, w.mp, ld, K, sb, tid);
     T *Wt = W + w.wofs;
     if (wr < w.mq && wc < w.mp) {
         UQFOR({

@@ -92,12 +92,12 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     for (int64_t c = 0; c < cb; c++)
         for (int64_t r = width(c); r < L.cblk_stride[c]; r++) P.yrow_ptr[P.panel_grow[P.rowbase[c] + r] + 1]++;
     for (int64_t g = 0; g < s.n; g++) P.yrow_ptr[g + 1] += P.yrow_ptr[g];
-    P.yrow_pos.assign(P.nyrows, 0);
+    P.ydst.assign(P.nyrows, 0);
     {
         std::vector<int64_t> pos(P.yrow_ptr.begin(), P.yrow_ptr.end() - 1);
         for (int64_t c = 0; c < cb; c++)
             for (int64_t r = width(c); r < L.cblk_stride[c]; r++)
-                P.yrow_pos[pos[P.panel_grow[P.rowbase[c] + r]]++] = (int32_t)(yoff[c] + r - width(c));
+                P.ydst[yoff[c] + r - width(c)] = (int32_t)pos[P.panel_grow[P.rowbase[c] + r]]++;
     }
     std::vector<KeyEnt> ke;
     std::vector<Range> rr, cr;
@@ -245,7 +245,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
         {
             struct TileKey {
                 int32_t g;
-                int64_t qp;  // qt << 32 | pt
+                int64_t qp;  // pt << 32 | qt: consecutive tiles share their p rows
                 bool operator<(const TileKey &o) const { return g != o.g ? g < o.g : qp < o.qp; }
                 bool operator==(const TileKey &o) const { return g == o.g && qp == o.qp; }
             };
@@ -276,7 +276,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                     const int64_t base = L.blok_roff[en.ib0];
                     for (int64_t pt = (en.p0 - base) / 64; pt <= (en.p1 - 1 - base) / 64; pt++)
                         for (int64_t qt = (en.q0 - base) / 64; qt <= (en.q1 - 1 - base) / 64; qt++)
-                            tl.push_back(TileKey{g, (qt << 32) | pt});
+                            tl.push_back(TileKey{g, (pt << 32) | qt});
                 }
             std::sort(tl.begin(), tl.end());
             tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
@@ -289,7 +289,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                 const WGroup &G = P.wgrp[t.g];
                 const auto &sk = gsrc[t.g - g0];
                 const int64_t ts = 64;
-                const int64_t pt = t.qp & 0xFFFFFFFF, qt = t.qp >> 32;
+                const int64_t qt = t.qp & 0xFFFFFFFF, pt = t.qp >> 32;
                 const int64_t base = L.blok_roff[sk.second];
                 WTile w;
                 w.s = sk.first;

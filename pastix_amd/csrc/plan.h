@@ -38,13 +38,15 @@ struct Plan {
     std::vector<int32_t> lv_wt_ptr;
     std::vector<int64_t> lv_wsize;
     int64_t wmax = 0;
-    // forward products (sources of level L) into the update buffer, and the
-    // CSR from every permuted row to its update-buffer entries
+    // forward products (sources of level L) into the update buffer, whose
+    // entries are grouped by permuted row: row g owns [yrow_ptr[g], yrow_ptr[g+1])
     std::vector<int32_t> lv_fwd_ptr;
     std::vector<FwdItem> fwd;
     int64_t nyrows = 0;                  // update-buffer rows = all off-diagonal panel rows
     std::vector<int64_t> yrow_ptr;       // n+1
-    std::vector<int32_t> yrow_pos;       // nyrows, sorted by source cblk per row
+    std::vector<int32_t> ydst;           // nyrows: update-buffer entry of every off-diagonal
+                                         // panel row (entries grouped by permuted row, sources
+                                         // in increasing order within a row)
     // backward gather items
     std::vector<int32_t> lv_bwd_ptr;
     std::vector<BwdItem> bwd;
