@@ -28,6 +28,12 @@ typedef enum pastix_amd_iparm_e {
     PXA_IPARM_RESIDENT_INPUT   = 3,
     /* Record per-kernel-family device times of numfact (default 0).        */
     PXA_IPARM_PROFILE          = 4,
+    /* Capture numfact and device solves in HIP graphs on first use and
+     * replay them (default 0).  A graph is re-captured when the static-pivot
+     * threshold changes; up to 16 solve graphs are kept, keyed by the
+     * right-hand-side pointer, leading dimension and nrhs.  Profiling
+     * (PXA_IPARM_PROFILE) runs numfact without the graph.                   */
+    PXA_IPARM_GRAPH            = 5,
     PXA_IPARM_SIZE             = 8
 } pastix_amd_iparm_t;
 
@@ -101,6 +107,36 @@ int pastix_amd_flops(const pastix_data_t *pastix_data, double out[4]);
  * sum of source widths over entries, 0.  Returns the number of levels
  * written or -1 before analysis. */
 int pastix_amd_plan_stats(const pastix_data_t *pastix_data, double *out, int maxlevels);
+
+/* ---- Batches of independent systems (no counterpart in the reference) ----
+ * nsys systems are analysed, factored and solved as their block-diagonal
+ * union: every kernel launch (or, with PXA_IPARM_GRAPH, every graph launch)
+ * covers the supernodes of all systems at one dependency level.  System s
+ * takes rows [off[s], off[s+1]) of the union.  Orders may be NULL (or hold
+ * NULL entries): the system is then ordered by nested dissection of its own
+ * graph.  All systems share mtxtype; values are held in FP64.  The factor
+ * precision, graph mode etc. are the instance's. */
+int pastix_amd_batch_analyze(pastix_data_t *pastix_data, pastix_int_t nsys, const spmatrix_t *const *spms,
+                             const pastix_order_t *const *orders);
+/* Numerical factorisation of the batch (same patterns as analysed). */
+int pastix_amd_batch_numfact(pastix_data_t *pastix_data, const spmatrix_t *const *spms);
+/* Solves in place: system s has nrhs[s] FP64 right-hand sides at b[s]
+ * (column-major, leading dimension ldb[s]); all go through one multi-RHS
+ * solve of max(nrhs) columns. */
+int pastix_amd_batch_solve(pastix_data_t *pastix_data, const pastix_int_t *nrhs, void *const *b,
+                           const pastix_int_t *ldb);
+/* Batched preconditioned CG (IPARM_REFINEMENT must be PastixRefineCG): every
+ * (system, right-hand side) pair runs the reference's CG recurrence
+ * (refinement/z_refine_grad.c) with its own scalars and stopping test
+ * ||r||/||b|| <= DPARM_EPSILON_REFINEMENT; M^{-1} is one multi-RHS solve of
+ * the union per iteration.  x[s] holds the initial guesses on entry.
+ * IPARM_NBITER receives the largest iteration count, DPARM_RELATIVE_ERROR the
+ * largest final ||r||/||b||. */
+int pastix_amd_batch_refine(pastix_data_t *pastix_data, const pastix_int_t *nrhs, void *const *b,
+                            const pastix_int_t *ldb, void *const *x, const pastix_int_t *ldx);
+/* Offsets of the systems in the union (nsys+1 entries, may be NULL);
+ * returns nsys or PASTIX_ERR_BADPARAMETER. */
+int pastix_amd_batch_offsets(const pastix_data_t *pastix_data, int64_t *off);
 
 /* Rerun the numerical factorisation from the device copy of the filled
  * coefficients (needs PXA_IPARM_RESIDENT_INPUT before pastix_task_numfact);

@@ -4,66 +4,19 @@
 // its prerequisite checks (common/pastixdata.h:25-33,
 // sopalin/pastix_task_sopalin.c:90-101,206-217,367-388), iparm/dparm inputs and
 // outputs, in-place right-hand sides, and PASTIX_ERR_* return codes.
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <memory>
-#include <string>
-#include <vector>
+#include "pastix_data.h"
 
-#include "analysis.h"
-#include "engine.h"
-#include "plan.h"
-
-extern "C" {
-#include "pastix.h"
-#include "pastix_amd.h"
-}
-
-#define STEP_INIT (1 << 0)
-#define STEP_ORDERING (1 << 1)
-#define STEP_SYMBFACT (1 << 2)
-#define STEP_ANALYSE (1 << 3)
-#define STEP_CSC2BCSC (1 << 4)
-#define STEP_BCSC2CTAB (1 << 5)
-#define STEP_NUMFACT (1 << 6)
-#define STEP_SOLVE (1 << 7)
-#define STEP_REFINE (1 << 8)
-
-struct pastix_data_s {
-    pastix_int_t *iparm = nullptr;
-    double *dparm = nullptr;
-    pastix_int_t steps = 0;
-    pastix_int_t ext[PXA_IPARM_SIZE] = {0};
-    // analysis
-    pxa::Graph graph;
-    pxa::Order order;
-    pxa::Symbol symbol;
-    pxa::Layout layout;
-    pxa::Plan plan;
-    int64_t n = 0;
-    int mtxtype = SpmGeneral;
-    int flttype = SpmDouble;
-    const spmatrix_t *csc = nullptr;
-    // numerics
-    pxa::HostBcsc bcsc;
-    std::unique_ptr<pxa::Engine> engine;
-    std::vector<char> hostL, hostU;
-    double fact_times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double *dvec = nullptr;  // 2n FP64 device vectors for refinement / error checks
-    pxa::Engine *dvec_owner = nullptr;
-};
-
-static double *device_pair(pastix_data_t *d) {
+namespace pxa_api {
+double *device_pair(pastix_data_t *d) {
     if (d->dvec && d->dvec_owner == d->engine.get()) return d->dvec;
     d->dvec = (double *)d->engine->alloc(2 * d->n * sizeof(double));
     d->dvec_owner = d->engine.get();
     return d->dvec;
 }
 
-static void err_print(const char *fn, const char *msg) { fprintf(stderr, "%s: %s\n", fn, msg); }
+void err_print(const char *fn, const char *msg) { fprintf(stderr, "%s: %s\n", fn, msg); }
+}  // namespace pxa_api
+using namespace pxa_api;
 
 static double wall() {
     using namespace std::chrono;
@@ -163,14 +116,11 @@ extern "C" void pastixFinalize(pastix_data_t **pastix_data) {
 // --------------------------------------------------------------------------
 // spm helpers
 // --------------------------------------------------------------------------
-struct Triplets {
-    std::vector<int64_t> i, j;
-    std::vector<double> v;
-};
+
 
 // Expand the user matrix to 0-based triplets of the full matrix (a symmetric
 // matrix stores one triangle, as the SPM library does).
-static int spm_triplets(const spmatrix_t *spm, Triplets &t, bool values) {
+int pxa_api::spm_triplets(const spmatrix_t *spm, Triplets &t, bool values) {
     const int64_t n = spm->n, nnz = spm->nnz;
     if (spm->dof != 1) return PASTIX_ERR_NOTIMPLEMENTED;
     if (spm->flttype != SpmDouble && spm->flttype != SpmFloat && spm->flttype != SpmPattern)
@@ -214,7 +164,7 @@ static int spm_triplets(const spmatrix_t *spm, Triplets &t, bool values) {
 
 // CSC (sorted, duplicates summed) of the n x n matrix given by triplets,
 // optionally permuted: B(perm[i], perm[j]) = A(i, j).
-static void triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
+void pxa_api::triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
                             std::vector<int32_t> &rows, std::vector<double> &vals, bool transpose) {
     const size_t m = t.i.size();
     colptr.assign(n + 1, 0);
@@ -488,6 +438,7 @@ extern "C" int pastix_subtask_bcsc2ctab(pastix_data_t *d) {
     if (d->engine->upload_perm(d->order.permtab)) return PASTIX_ERR_INTERNAL;
     if (d->ext[PXA_IPARM_RESIDENT_INPUT] && d->engine->snapshot_coef()) return PASTIX_ERR_INTERNAL;
     d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
+    d->engine->use_graph = d->ext[PXA_IPARM_GRAPH] != 0;
     std::vector<char>().swap(d->hostL);
     std::vector<char>().swap(d->hostU);
     d->steps |= STEP_BCSC2CTAB;
@@ -805,6 +756,10 @@ extern "C" int pastix_amd_set_iparm(pastix_data_t *d, int key, pastix_int_t valu
     if (key == PXA_IPARM_FACTOR_PRECISION && value != -1 && value != SpmFloat && value != SpmDouble)
         return PASTIX_ERR_BADPARAMETER;
     d->ext[key] = value;
+    if (d->engine) {
+        d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
+        d->engine->use_graph = d->ext[PXA_IPARM_GRAPH] != 0;
+    }
     return PASTIX_SUCCESS;
 }
 extern "C" int pastix_amd_get_iparm(const pastix_data_t *d, int key, pastix_int_t *value) {

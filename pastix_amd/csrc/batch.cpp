@@ -1,0 +1,286 @@
+// Batches of independent systems.
+//
+// The reference solves one matrix per pastix_data_t.  For many small systems
+// this engine analyses their block-diagonal union once: every dependency
+// level of the union holds the supernodes of all systems at that depth, so
+// one launch per kernel and level factors (and solves) the whole batch, and
+// the captured-graph mode turns numfact and each solve into a single graph
+// launch.  System s owns rows and columns [off[s], off[s+1]) of the union in
+// the user numbering, and the same range in the permuted numbering since the
+// union ordering is the concatenation of the systems' orderings.
+#include "pastix_data.h"
+
+using namespace pxa_api;
+
+namespace {
+
+// Union CSC of the systems (lower triangle when symmetric, FP64 values).
+int build_union(pastix_data_t *d, pastix_int_t nsys, const spmatrix_t *const *spms, bool values,
+                std::vector<int64_t> &off) {
+    off.assign(nsys + 1, 0);
+    const int mtx = spms[0]->mtxtype;
+    Triplets u;
+    for (pastix_int_t s = 0; s < nsys; s++) {
+        const spmatrix_t *A = spms[s];
+        if (!A || A->mtxtype != mtx || A->n <= 0 || A->dof != 1) return PASTIX_ERR_BADPARAMETER;
+        if (values && A->flttype != SpmDouble && A->flttype != SpmFloat) return PASTIX_ERR_BADPARAMETER;
+        Triplets t;
+        const int rc = spm_triplets(A, t, values);
+        if (rc) return rc;
+        for (size_t k = 0; k < t.i.size(); k++) {
+            if (mtx != SpmGeneral && t.i[k] < t.j[k]) continue;
+            u.i.push_back(t.i[k] + off[s]);
+            u.j.push_back(t.j[k] + off[s]);
+            if (values) u.v.push_back(t.v[k]);
+        }
+        off[s + 1] = off[s] + A->n;
+    }
+    const int64_t N = off[nsys];
+    std::vector<int64_t> cp;
+    std::vector<int32_t> rows;
+    std::vector<double> vals;
+    triplets_to_csc(N, u, nullptr, cp, rows, vals, false);
+    if (!values) vals.assign(rows.size(), 1.0);
+    d->u_colptr.assign(cp.begin(), cp.end());
+    d->u_rows.assign(rows.begin(), rows.end());
+    d->u_vals.swap(vals);
+    spmatrix_t &S = d->u_spm;
+    memset(&S, 0, sizeof(S));
+    S.mtxtype = (spm_mtxtype_t)mtx;
+    S.flttype = SpmDouble;
+    S.fmttype = SpmCSC;
+    S.gN = S.n = S.gNexp = S.nexp = N;
+    S.gnnz = S.nnz = S.gnnzexp = S.nnzexp = (spm_int_t)d->u_rows.size();
+    S.dof = 1;
+    S.layout = SpmColMajor;
+    S.colptr = d->u_colptr.data();
+    S.rowptr = d->u_rows.data();
+    S.values = d->u_vals.data();
+    return PASTIX_SUCCESS;
+}
+
+// Ordering of one system: the caller's, or nested dissection of its graph.
+int system_order(const spmatrix_t *A, const pastix_order_t *o, pxa::Order &out) {
+    const int64_t n = A->n;
+    out = pxa::Order();
+    out.n = n;
+    if (o && (o->permtab || o->peritab)) {
+        const int64_t base = o->baseval;
+        out.permtab.resize(n);
+        out.peritab.resize(n);
+        if (o->permtab) {
+            for (int64_t i = 0; i < n; i++) out.permtab[i] = o->permtab[i] - base;
+            for (int64_t i = 0; i < n; i++) out.peritab[out.permtab[i]] = i;
+        } else {
+            for (int64_t i = 0; i < n; i++) out.peritab[i] = o->peritab[i] - base;
+            for (int64_t i = 0; i < n; i++) out.permtab[out.peritab[i]] = i;
+        }
+        if (o->rangtab && o->treetab && o->cblknbr > 0) {
+            out.cblknbr = o->cblknbr;
+            out.rangtab.resize(out.cblknbr + 1);
+            out.treetab.resize(out.cblknbr);
+            for (int64_t c = 0; c <= out.cblknbr; c++) out.rangtab[c] = o->rangtab[c] - base;
+            for (int64_t c = 0; c < out.cblknbr; c++) out.treetab[c] = o->treetab[c] < 0 ? -1 : o->treetab[c] - base;
+        }
+    } else {
+        Triplets t;
+        const int rc = spm_triplets(A, t, false);
+        if (rc) return rc;
+        std::vector<int64_t> cp;
+        std::vector<int32_t> rw;
+        std::vector<double> vv;
+        triplets_to_csc(n, t, nullptr, cp, rw, vv, false);
+        std::vector<int64_t> rows64(rw.begin(), rw.end());
+        const pxa::Graph g = pxa::graph_from_csc(n, cp.data(), rows64.data(), 0);
+        out = pxa::order_nested_dissection(g);
+    }
+    std::vector<char> seen(n, 0);
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t p = out.permtab[i];
+        if (p < 0 || p >= n || seen[p]) return PASTIX_ERR_BADPARAMETER;
+        seen[p] = 1;
+    }
+    std::string err;
+    if (out.cblknbr && !pxa::check_order(out, err)) return PASTIX_ERR_BADPARAMETER;
+    // without a supernode partition, supernodes are found on the union graph
+    // (they never span two systems: the union's elimination forest has one
+    // tree per system at least)
+    return PASTIX_SUCCESS;
+}
+
+// Column-major n x R block of the union from per-system blocks (FP64).
+void pack(const std::vector<int64_t> &off, pastix_int_t R, const pastix_int_t *nrhs, const void *const *b,
+          const pastix_int_t *ldb, std::vector<double> &B) {
+    const int64_t N = off.back();
+    B.assign((size_t)N * R, 0.0);
+    for (size_t s = 0; s + 1 < off.size(); s++) {
+        const double *bs = (const double *)b[s];
+        const int64_t ns = off[s + 1] - off[s];
+        for (pastix_int_t c = 0; c < nrhs[s]; c++)
+            for (int64_t i = 0; i < ns; i++) B[off[s] + i + (size_t)c * N] = bs[i + c * ldb[s]];
+    }
+}
+
+void unpack(const std::vector<int64_t> &off, const pastix_int_t *nrhs, const std::vector<double> &B,
+            void *const *b, const pastix_int_t *ldb) {
+    const int64_t N = off.back();
+    for (size_t s = 0; s + 1 < off.size(); s++) {
+        double *bs = (double *)b[s];
+        const int64_t ns = off[s + 1] - off[s];
+        for (pastix_int_t c = 0; c < nrhs[s]; c++)
+            for (int64_t i = 0; i < ns; i++) bs[i + c * ldb[s]] = B[off[s] + i + (size_t)c * N];
+    }
+}
+
+bool rhs_ok(const pastix_data_t *d, const pastix_int_t *nrhs, const void *const *b, const pastix_int_t *ldb,
+            pastix_int_t &R) {
+    R = 0;
+    for (size_t s = 0; s + 1 < d->batch_off.size(); s++) {
+        if (nrhs[s] < 0 || (nrhs[s] > 0 && (!b[s] || ldb[s] < d->batch_off[s + 1] - d->batch_off[s]))) return false;
+        R = std::max(R, nrhs[s]);
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" int pastix_amd_batch_analyze(pastix_data_t *d, pastix_int_t nsys, const spmatrix_t *const *spms,
+                                        const pastix_order_t *const *orders) {
+    if (!d || nsys < 1 || !spms) return PASTIX_ERR_BADPARAMETER;
+    std::vector<int64_t> off;
+    int rc = build_union(d, nsys, spms, true, off);
+    if (rc) return rc;
+    // concatenated ordering
+    const int64_t N = off[nsys];
+    std::vector<pastix_int_t> perm(N), rang, tree;
+    bool with_tree = true;
+    std::vector<pxa::Order> os(nsys);
+    for (pastix_int_t s = 0; s < nsys; s++) {
+        rc = system_order(spms[s], orders ? orders[s] : nullptr, os[s]);
+        if (rc) return rc;
+        with_tree = with_tree && os[s].cblknbr > 0;
+    }
+    rang.push_back(0);
+    for (pastix_int_t s = 0; s < nsys; s++) {
+        for (int64_t i = 0; i < spms[s]->n; i++) perm[off[s] + i] = off[s] + os[s].permtab[i];
+        if (with_tree) {
+            const pastix_int_t c0 = (pastix_int_t)tree.size();
+            for (int64_t c = 0; c < os[s].cblknbr; c++) {
+                rang.push_back(off[s] + os[s].rangtab[c + 1]);
+                tree.push_back(os[s].treetab[c] < 0 ? -1 : c0 + os[s].treetab[c]);
+            }
+        }
+    }
+    pastix_order_t uo;
+    memset(&uo, 0, sizeof(uo));
+    uo.baseval = 0;
+    uo.vertnbr = N;
+    uo.permtab = perm.data();
+    if (with_tree) {
+        uo.cblknbr = (pastix_int_t)tree.size();
+        uo.rangtab = rang.data();
+        uo.treetab = tree.data();
+    }
+    const pastix_int_t saved = d->iparm[IPARM_ORDERING];
+    d->iparm[IPARM_ORDERING] = PastixOrderPersonal;
+    rc = pastix_subtask_order(d, &d->u_spm, &uo);
+    d->iparm[IPARM_ORDERING] = saved;
+    if (rc) return rc;
+    if ((rc = pastix_subtask_symbfact(d)) || (rc = pastix_subtask_reordering(d)) || (rc = pastix_subtask_blend(d)))
+        return rc;
+    // the union ordering keeps every system in its own range
+    for (pastix_int_t s = 0; s < nsys; s++)
+        for (int64_t i = off[s]; i < off[s + 1]; i++)
+            if (d->order.permtab[i] < off[s] || d->order.permtab[i] >= off[s + 1]) {
+                err_print("pastix_amd_batch_analyze", "ordering mixes systems");
+                return PASTIX_ERR_INTERNAL;
+            }
+    d->batch_off = off;
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_batch_numfact(pastix_data_t *d, const spmatrix_t *const *spms) {
+    if (!d || !spms || d->batch_off.size() < 2) return PASTIX_ERR_BADPARAMETER;
+    const pastix_int_t nsys = (pastix_int_t)d->batch_off.size() - 1;
+    const size_t nnz = d->u_rows.size();
+    std::vector<int64_t> off;
+    int rc = build_union(d, nsys, spms, true, off);
+    if (rc) return rc;
+    if (off != d->batch_off || d->u_rows.size() != nnz) {
+        err_print("pastix_amd_batch_numfact", "the systems differ from the analysed batch");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    return pastix_task_numfact(d, &d->u_spm);
+}
+
+extern "C" int pastix_amd_batch_solve(pastix_data_t *d, const pastix_int_t *nrhs, void *const *b,
+                                      const pastix_int_t *ldb) {
+    if (!d || !nrhs || !b || !ldb || d->batch_off.size() < 2) return PASTIX_ERR_BADPARAMETER;
+    pastix_int_t R = 0;
+    if (!rhs_ok(d, nrhs, b, ldb, R)) return PASTIX_ERR_BADPARAMETER;
+    if (R == 0) return PASTIX_SUCCESS;
+    std::vector<double> B;
+    pack(d->batch_off, R, nrhs, b, ldb, B);
+    const int rc = pastix_task_solve(d, R, B.data(), d->batch_off.back());
+    if (rc) return rc;
+    unpack(d->batch_off, nrhs, B, b, ldb);
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_batch_refine(pastix_data_t *d, const pastix_int_t *nrhs, void *const *b,
+                                       const pastix_int_t *ldb, void *const *x, const pastix_int_t *ldx) {
+    if (!d || !nrhs || !b || !ldb || !x || !ldx || d->batch_off.size() < 2) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_NUMFACT) || !d->engine) return PASTIX_ERR_BADPARAMETER;
+    if (d->iparm[IPARM_REFINEMENT] != PastixRefineCG) {
+        err_print("pastix_amd_batch_refine", "batched refinement is conjugate gradient (IPARM_REFINEMENT = PastixRefineCG)");
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    }
+    pastix_int_t R = 0, Rx = 0;
+    if (!rhs_ok(d, nrhs, b, ldb, R) || !rhs_ok(d, nrhs, x, ldx, Rx)) return PASTIX_ERR_BADPARAMETER;
+    if (R == 0) return PASTIX_SUCCESS;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t N = d->batch_off.back();
+    std::vector<double> B, X, Bp((size_t)N * R), Xp((size_t)N * R);
+    pack(d->batch_off, R, nrhs, b, ldb, B);
+    pack(d->batch_off, R, nrhs, x, ldx, X);
+    const std::vector<int64_t> &perm = d->order.permtab;
+    for (pastix_int_t c = 0; c < R; c++)
+        for (int64_t i = 0; i < N; i++) {
+            Bp[perm[i] + (size_t)c * N] = B[i + (size_t)c * N];
+            Xp[perm[i] + (size_t)c * N] = X[i + (size_t)c * N];
+        }
+    pxa::Engine &E = *d->engine;
+    if (d->batch_dev_owner != &E || d->batch_dev_cap < 2 * N * R) {
+        d->batch_dev = (double *)E.alloc((size_t)2 * N * R * sizeof(double));
+        d->batch_dev_cap = 2 * N * R;
+        d->batch_dev_owner = &E;
+        if (!d->batch_dev) return PASTIX_ERR_OUTOFMEMORY;
+    }
+    double *dB = d->batch_dev;
+    double *dX = dB + (size_t)N * R;
+    if (hipMemcpy(dB, Bp.data(), Bp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dX, Xp.data(), Xp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return PASTIX_ERR_INTERNAL;
+    double eps = d->dparm[DPARM_EPSILON_REFINEMENT];
+    if (eps < 0.) eps = 1e-12;
+    int it = 0;
+    double res = 0;
+    if (E.refine_cg_batch(dX, dB, (int)R, d->batch_off, (int)d->iparm[IPARM_ITERMAX], eps, &it, &res))
+        return PASTIX_ERR_INTERNAL;
+    if (hipMemcpy(Xp.data(), dX, Xp.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return PASTIX_ERR_INTERNAL;
+    for (pastix_int_t c = 0; c < R; c++)
+        for (int64_t i = 0; i < N; i++) X[i + (size_t)c * N] = Xp[perm[i] + (size_t)c * N];
+    unpack(d->batch_off, nrhs, X, x, ldx);
+    d->iparm[IPARM_NBITER] = it;
+    d->dparm[DPARM_RELATIVE_ERROR] = res;
+    d->dparm[DPARM_REFINE_TIME] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_batch_offsets(const pastix_data_t *d, int64_t *off) {
+    if (!d || d->batch_off.size() < 2) return PASTIX_ERR_BADPARAMETER;
+    if (off)
+        for (size_t s = 0; s < d->batch_off.size(); s++) off[s] = d->batch_off[s];
+    return (int)d->batch_off.size() - 1;
+}

@@ -21,6 +21,8 @@ namespace pxa {
 
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (fact_exec_) (void)hipGraphExecDestroy(fact_exec_);
+    for (SolveGraph &g : solve_graphs_) (void)hipGraphExecDestroy(g.exec);
     for (void *p : owned_) (void)hipFree(p);
     for (hipEvent_t e : pev_) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -307,6 +309,47 @@ int Engine::upload_perm(const std::vector<int64_t> &permtab) {
 // Device right-hand sides in the user ordering: permute into the work block,
 // solve, permute back (reference sopalin/pastix_task_solve.c:550-577 with
 // bvec_zlapmr replaced by a gather/scatter kernel).
+int Engine::enqueue_solve(void *dx, int64_t ldx, int nrhs, bool user) {
+    if (!user) return (prec_ == 2) ? solve_t<float>((float *)dx, ldx, nrhs) : solve_t<double>((double *)dx, ldx, nrhs);
+    if (prec_ == 2) launch_permute<float>(n_, nrhs, d_perm_, true, (const float *)dx, ldx, (float *)d_rhs_, n_, stream_);
+    else launch_permute<double>(n_, nrhs, d_perm_, true, (const double *)dx, ldx, (double *)d_rhs_, n_, stream_);
+    const int rc = (prec_ == 2) ? solve_t<float>((float *)d_rhs_, n_, nrhs) : solve_t<double>((double *)d_rhs_, n_, nrhs);
+    if (rc) return rc;
+    if (prec_ == 2) launch_permute<float>(n_, nrhs, d_perm_, false, (const float *)d_rhs_, n_, (float *)dx, ldx, stream_);
+    else launch_permute<double>(n_, nrhs, d_perm_, false, (const double *)d_rhs_, n_, (double *)dx, ldx, stream_);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+hipGraphExec_t Engine::solve_graph(void *dx, int64_t ldx, int nrhs, bool user) {
+    for (const SolveGraph &g : solve_graphs_)
+        if (g.ptr == dx && g.ld == ldx && g.nrhs == nrhs && g.user == user) return g.exec;
+    if (solve_graphs_.size() >= 16) {
+        (void)hipGraphExecDestroy(solve_graphs_.front().exec);
+        solve_graphs_.erase(solve_graphs_.begin());
+    }
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    const int rc = enqueue_solve(dx, ldx, nrhs, user);
+    const hipError_t e = hipStreamEndCapture(stream_, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        fail("solve graph capture failed");
+        return nullptr;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        fail(std::string("solve graph instantiation: ") + hipGetErrorString(ei));
+        return nullptr;
+    }
+    solve_graphs_.push_back(SolveGraph{dx, ldx, nrhs, user, ex});
+    return ex;
+}
+
+// Device solve in the user ordering: permute into the work block, solve,
+// permute back (the timed region includes both permutations).
 int Engine::solve_user_device(void *db, int64_t ldb, int nrhs, double *seconds) {
     if (!d_perm_) return fail("permutation not uploaded");
     const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
@@ -315,22 +358,59 @@ int Engine::solve_user_device(void *db, int64_t ldb, int nrhs, double *seconds) 
         d_rhs_ = alloc(rhs_cap_ * fsz);
         if (!d_rhs_) return -1;
     }
-    if (prec_ == 2) launch_permute<float>(n_, nrhs, d_perm_, true, (const float *)db, ldb, (float *)d_rhs_, n_, stream_);
-    else launch_permute<double>(n_, nrhs, d_perm_, true, (const double *)db, ldb, (double *)d_rhs_, n_, stream_);
-    int rc = solve_device(d_rhs_, n_, nrhs, seconds);
-    if (rc) return rc;
-    if (prec_ == 2) launch_permute<float>(n_, nrhs, d_perm_, false, (const float *)d_rhs_, n_, (float *)db, ldb, stream_);
-    else launch_permute<double>(n_, nrhs, d_perm_, false, (const double *)d_rhs_, n_, (double *)db, ldb, stream_);
-    HIPCHK(hipStreamSynchronize(stream_));
+    HIPCHK(hipEventRecord(ev0_, stream_));
+    if (use_graph) {
+        hipGraphExec_t ex = solve_graph(db, ldb, nrhs, true);
+        if (!ex) return -1;
+        HIPCHK(hipGraphLaunch(ex, stream_));
+    } else {
+        const int rc = enqueue_solve(db, ldb, nrhs, true);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(ev1_, stream_));
+    HIPCHK(hipEventSynchronize(ev1_));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    if (seconds) *seconds = ms * 1e-3;
     return 0;
+}
+
+int Engine::enqueue_factorize(double thr) {
+    launch_zero_int(4, d_stat_, stream_);
+    return (prec_ == 2) ? factorize_t<float>(thr) : factorize_t<double>(thr);
 }
 
 int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) {
     if (!inited_) return fail("engine not initialised");
-    HIPCHK(hipMemsetAsync(d_stat_, 0, 4 * sizeof(int), stream_));
+    const bool graph = use_graph && !profile;
+    if (graph && (!fact_exec_ || thr != fact_thr_)) {
+        if (fact_exec_) {
+            HIPCHK(hipGraphExecDestroy(fact_exec_));
+            fact_exec_ = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_factorize(thr);
+        const hipError_t e = hipStreamEndCapture(stream_, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(std::string("numfact graph capture: ") + hipGetErrorString(e));
+        }
+        const hipError_t ei = hipGraphInstantiate(&fact_exec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            fact_exec_ = nullptr;
+            return fail(std::string("numfact graph instantiation: ") + hipGetErrorString(ei));
+        }
+        fact_thr_ = thr;
+    }
     HIPCHK(hipEventRecord(ev0_, stream_));
-    int rc = (prec_ == 2) ? factorize_t<float>(thr) : factorize_t<double>(thr);
-    if (rc) return rc;
+    if (graph) {
+        HIPCHK(hipGraphLaunch(fact_exec_, stream_));
+    } else {
+        const int rc = enqueue_factorize(thr);
+        if (rc) return rc;
+    }
     HIPCHK(hipEventRecord(ev1_, stream_));
     HIPCHK(hipEventSynchronize(ev1_));
     float ms = 0;
@@ -398,8 +478,14 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
 int Engine::solve_device(void *dx, int64_t ldx, int nrhs, double *seconds) {
     if (!inited_) return fail("engine not initialised");
     HIPCHK(hipEventRecord(ev0_, stream_));
-    int rc = (prec_ == 2) ? solve_t<float>((float *)dx, ldx, nrhs) : solve_t<double>((double *)dx, ldx, nrhs);
-    if (rc) return rc;
+    if (use_graph) {
+        hipGraphExec_t ex = solve_graph(dx, ldx, nrhs, false);
+        if (!ex) return -1;
+        HIPCHK(hipGraphLaunch(ex, stream_));
+    } else {
+        const int rc = enqueue_solve(dx, ldx, nrhs, false);
+        if (rc) return rc;
+    }
     HIPCHK(hipEventRecord(ev1_, stream_));
     HIPCHK(hipEventSynchronize(ev1_));
     float ms = 0;
@@ -481,6 +567,103 @@ int Engine::precond(double *dv) {
     int rc = solve_t<float>(d_svec_, n_, 1);
     if (rc) return rc;
     launch_s2d(n_, d_svec_, dv, stream_);
+    return 0;
+}
+
+int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector<int64_t> &segoff, int itermax,
+                            double eps, int *niter, double *resid) {
+    const int nseg = (int)segoff.size() - 1;
+    if (nseg < 1 || segoff.front() != 0 || segoff.back() != n_ || R < 1) return fail("bad batch segments");
+    const int64_t nr = n_ * R, ns = (int64_t)nseg * R;
+    if (bvec_cap_ < 4 * nr) {
+        bvec_cap_ = 4 * nr;
+        d_bvec_ = (double *)alloc(bvec_cap_ * sizeof(double));
+        d_bsvec_ = (float *)alloc(nr * sizeof(float));
+        if (!d_bvec_ || !d_bsvec_) return -1;
+    }
+    if (bscal_cap_ < 8 * ns) {
+        bscal_cap_ = 8 * ns;
+        d_bscal_ = (double *)alloc(bscal_cap_ * sizeof(double));
+        if (!d_bscal_) return -1;
+    }
+    if (segoff != segoff_host_) {
+        if (!d_seg_) {
+            d_seg_ = (int32_t *)alloc(n_ * sizeof(int32_t));
+            if (!d_seg_) return -1;
+        }
+        std::vector<int32_t> seg(n_);
+        for (int s = 0; s < nseg; s++)
+            for (int64_t i = segoff[s]; i < segoff[s + 1]; i++) seg[i] = s;
+        HIPCHK(hipMemcpy(d_seg_, seg.data(), n_ * sizeof(int32_t), hipMemcpyHostToDevice));
+        d_segoff_ = (int64_t *)alloc(segoff.size() * sizeof(int64_t));
+        if (!d_segoff_) return -1;
+        HIPCHK(hipMemcpy(d_segoff_, segoff.data(), segoff.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        segoff_host_ = segoff;
+    }
+    double *Rv = d_bvec_, *P = d_bvec_ + nr, *Z = d_bvec_ + 2 * nr, *AP = d_bvec_ + 3 * nr;
+    double *dots = d_bscal_, *coef = d_bscal_ + 4 * ns;
+    std::vector<double> h(4 * ns), rb(ns), normb(ns), rz(ns), a(ns);
+    auto segdot = [&](const double *x, const double *y, double *out_h) -> int {
+        launch_seg_dot(d_segoff_, nseg, R, n_, x, y, dots, stream_);
+        HIPCHK(hipMemcpyAsync(out_h, dots, ns * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        return 0;
+    };
+    auto precond_block = [&](double *zb) -> int {
+        if (prec_ == 3) return solve_t<double>(zb, n_, R);
+        launch_d2s(nr, zb, d_bsvec_, stream_);
+        const int rc = solve_t<float>(d_bsvec_, n_, R);
+        if (rc) return rc;
+        launch_s2d(nr, d_bsvec_, zb, stream_);
+        return 0;
+    };
+    auto upload = [&](const std::vector<double> &v) -> int {
+        HIPCHK(hipMemcpyAsync(coef, v.data(), ns * sizeof(double), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        return 0;
+    };
+    if (segdot(B, B, normb.data())) return -1;
+    for (int64_t k = 0; k < ns; k++) normb[k] = std::sqrt(normb[k]);
+    for (int c = 0; c < R; c++) {
+        launch_copy(n_, B + c * n_, Rv + c * n_, stream_);
+        spmv(-1.0, X + c * n_, 1.0, Rv + c * n_);
+    }
+    auto residuals = [&]() -> double {
+        double worst = 0;
+        for (int64_t k = 0; k < ns; k++) {
+            const double nr_ = std::sqrt(h[k]);
+            rb[k] = normb[k] > 0 ? nr_ / normb[k] : 0.0;
+            worst = std::max(worst, rb[k]);
+        }
+        return worst;
+    };
+    if (segdot(Rv, Rv, h.data())) return -1;
+    double worst = residuals();
+    launch_copy(nr, Rv, Z, stream_);
+    if (precond_block(Z)) return -1;
+    launch_copy(nr, Z, P, stream_);
+    int it = 0;
+    std::vector<double> pap(ns), rz2(ns);
+    while (worst > eps && it < itermax) {
+        it++;
+        for (int c = 0; c < R; c++) spmv(1.0, P + c * n_, 0.0, AP + c * n_);
+        if (segdot(Rv, Z, rz.data()) || segdot(AP, P, pap.data())) return -1;
+        for (int64_t k = 0; k < ns; k++) a[k] = (rb[k] > eps && pap[k] != 0.0) ? rz[k] / pap[k] : 0.0;
+        if (upload(a)) return -1;
+        launch_seg_axpy(n_, R, n_, d_seg_, nseg, coef, 1.0, P, X, stream_);
+        launch_seg_axpy(n_, R, n_, d_seg_, nseg, coef, -1.0, AP, Rv, stream_);
+        launch_copy(nr, Rv, Z, stream_);
+        if (precond_block(Z)) return -1;
+        if (segdot(Rv, Z, rz2.data())) return -1;
+        for (int64_t k = 0; k < ns; k++) a[k] = (rz[k] != 0.0) ? rz2[k] / rz[k] : 0.0;
+        if (upload(a)) return -1;
+        launch_seg_xpby(n_, R, n_, d_seg_, nseg, coef, Z, P, stream_);
+        if (segdot(Rv, Rv, h.data())) return -1;
+        worst = residuals();
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (niter) *niter = it;
+    if (resid) *resid = worst;
     return 0;
 }
 

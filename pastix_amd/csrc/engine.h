@@ -41,6 +41,13 @@ class Engine {
     // Device refinement drivers (vectors in the permuted ordering, FP64).
     int refine_cg(double *dx, const double *db, int itermax, double eps, int *niter, double *resid);
     int refine_gmres(double *dx, const double *db, int itermax, int im, double eps, int *niter, double *resid);
+    // Batched CG over independent systems (consecutive row segments segoff of
+    // the permuted order) and R right-hand-side columns (n x R blocks, ld = n):
+    // every (system, column) pair runs its own CG recurrence and stops at its
+    // own tolerance.  niter: largest iteration count, resid: largest
+    // ||r||/||b|| at exit.
+    int refine_cg_batch(double *dX, const double *dB, int R, const std::vector<int64_t> &segoff, int itermax,
+                        double eps, int *niter, double *resid);
     // Normwise backward error pieces: [max|b-Ax|, max|x|, max|b|].
     int berr(const double *dx, const double *db, double out[3]);
 
@@ -70,6 +77,9 @@ class Engine {
     // update) as one span.
     enum KClass { KC_PANEL = 0, KC_TRSM, KC_DIRECT, KC_WGEMM, KC_GATHER, KC_ACC, KC_N };
     bool profile = false;
+    // Replay numfact and device solves from captured HIP graphs (one graph
+    // launch per factorisation / solve instead of one launch per kernel).
+    bool use_graph = false;
     double kt[KC_N] = {0, 0, 0, 0, 0, 0};
     int kn[KC_N] = {0, 0, 0, 0, 0, 0};
     double t_diag = 0, t_trsm = 0, t_upd = 0;
@@ -120,6 +130,22 @@ class Engine {
     int32_t *d_perm_ = nullptr;  // permtab as int32 for device permutations
     void *d_L0_ = nullptr, *d_U0_ = nullptr;  // snapshot of the filled coefficients
     std::vector<hipEvent_t> pev_;
+    // captured graphs: numfact (re-captured when the pivot threshold changes)
+    // and solves keyed by (right-hand-side pointer, leading dimension, nrhs,
+    // user ordering)
+    hipGraphExec_t fact_exec_ = nullptr;
+    double fact_thr_ = 0;
+    struct SolveGraph {
+        const void *ptr;
+        int64_t ld;
+        int nrhs;
+        bool user;
+        hipGraphExec_t exec;
+    };
+    std::vector<SolveGraph> solve_graphs_;
+    int enqueue_factorize(double thr);
+    int enqueue_solve(void *dx, int64_t ldx, int nrhs, bool user);
+    hipGraphExec_t solve_graph(void *dx, int64_t ldx, int nrhs, bool user);
     struct Span {
         int cls, e0, e1;
     };
@@ -145,6 +171,13 @@ class Engine {
     double *d_vec_ = nullptr;
     int64_t vec_cap_ = 0;
     float *d_svec_ = nullptr;
+    // batched refinement work space
+    double *d_bvec_ = nullptr, *d_bscal_ = nullptr;
+    float *d_bsvec_ = nullptr;
+    int64_t bvec_cap_ = 0, bscal_cap_ = 0;
+    int32_t *d_seg_ = nullptr;
+    int64_t *d_segoff_ = nullptr;
+    std::vector<int64_t> segoff_host_;
 };
 
 // Host-side permuted matrix (reference bcsc/bcsc.h:28-47 keeps the same data

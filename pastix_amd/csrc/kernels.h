@@ -71,6 +71,17 @@ void launch_axpy(int64_t n, double alpha, const double *x, double *y, hipStream_
 void launch_axpy_dev(int64_t n, const double *alpha, double sign, const double *x, double *y, hipStream_t st);
 void launch_scal(int64_t n, double alpha, double *x, hipStream_t st);
 void launch_copy(int64_t n, const double *x, double *y, hipStream_t st);
+// Segmented BLAS-1 over an n x R block whose rows are split into nseg
+// consecutive segments (off: nseg+1 offsets, seg: segment of every row);
+// per-(segment, column) scalars are stored at [s + c*nseg].
+void launch_seg_dot(const int64_t *off, int nseg, int R, int64_t ld, const double *x, const double *y, double *out,
+                    hipStream_t st);
+void launch_seg_axpy(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg, const double *a, double sign,
+                     const double *x, double *y, hipStream_t st);
+void launch_seg_xpby(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg, const double *b, const double *z,
+                     double *p, hipStream_t st);
+// Zeroes n ints with a kernel (usable inside a captured graph).
+void launch_zero_int(int n, int *p, hipStream_t st);
 void launch_d2s(int64_t n, const double *x, float *y, hipStream_t st);
 void launch_s2d(int64_t n, const float *x, double *y, hipStream_t st);
 void launch_gemv_t(int64_t n, int m, const double *V, int64_t ldv, const double *x, double *partials, double *out,

@@ -67,6 +67,11 @@ __global__ __launch_bounds__(256) void k_scal(int64_t n, double alpha, double *_
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) x[i] *= alpha;
 }
+__global__ __launch_bounds__(256) void k_zero_int(int n, int *__restrict__ p) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void k_copy(int64_t n, const double *__restrict__ x, double *__restrict__ y) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) y[i] = x[i];
@@ -187,6 +192,70 @@ void launch_axpy_dev(int64_t n, const double *alpha, double sign, const double *
 void launch_scal(int64_t n, double alpha, double *x, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_scal, dim3(nblk(n)), dim3(256), 0, st, n, alpha, x);
 }
+// ---- segmented vector kernels for batches of independent systems ----------
+// Segment s owns rows [off[s], off[s+1]); column c of an n x R block is at
+// x + c*ld.  out[s + c*nseg] = sum over the segment of x*y, one workgroup per
+// (segment, column), fixed-order tree reduction.
+__global__ __launch_bounds__(256) void k_seg_dot(const int64_t *__restrict__ off, int nseg, int64_t ld,
+                                                 const double *__restrict__ x, const double *__restrict__ y,
+                                                 double *__restrict__ out) {
+    __shared__ double red[256];
+    const int s = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const double *xc = x + c * ld, *yc = y + c * ld;
+    double v = 0;
+    for (int64_t i = off[s] + tid; i < off[s + 1]; i += 256) v += xc[i] * yc[i];
+    red[tid] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) out[s + (int64_t)c * nseg] = red[0];
+}
+
+// y[:, c] += sign * a[seg(i) + c*nseg] * x[:, c]
+__global__ __launch_bounds__(256) void k_seg_axpy(int64_t n, int R, int64_t ld, const int32_t *__restrict__ seg,
+                                                  int nseg, const double *__restrict__ a, double sign,
+                                                  const double *__restrict__ x, double *__restrict__ y) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n * R) return;
+    const int64_t i = t % n, c = t / n;
+    y[i + c * ld] += sign * a[seg[i] + c * nseg] * x[i + c * ld];
+}
+
+// p[:, c] = z[:, c] + b[seg(i) + c*nseg] * p[:, c]
+__global__ __launch_bounds__(256) void k_seg_xpby(int64_t n, int R, int64_t ld, const int32_t *__restrict__ seg,
+                                                  int nseg, const double *__restrict__ b,
+                                                  const double *__restrict__ z, double *__restrict__ p) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n * R) return;
+    const int64_t i = t % n, c = t / n;
+    p[i + c * ld] = z[i + c * ld] + b[seg[i] + c * nseg] * p[i + c * ld];
+}
+
+void launch_seg_dot(const int64_t *off, int nseg, int R, int64_t ld, const double *x, const double *y, double *out,
+                    hipStream_t st) {
+    if (nseg > 0 && R > 0) hipLaunchKernelGGL(k_seg_dot, dim3(nseg, R), dim3(256), 0, st, off, nseg, ld, x, y, out);
+}
+void launch_seg_axpy(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg, const double *a, double sign,
+                     const double *x, double *y, hipStream_t st) {
+    const int64_t m = n * R;
+    if (m > 0)
+        hipLaunchKernelGGL(k_seg_axpy, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, n, R, ld, seg, nseg, a,
+                           sign, x, y);
+}
+void launch_seg_xpby(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg, const double *b, const double *z,
+                     double *p, hipStream_t st) {
+    const int64_t m = n * R;
+    if (m > 0)
+        hipLaunchKernelGGL(k_seg_xpby, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, n, R, ld, seg, nseg, b,
+                           z, p);
+}
+
+void launch_zero_int(int n, int *p, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, n, p);
+}
+
 void launch_copy(int64_t n, const double *x, double *y, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(256), 0, st, n, x, y);
 }

@@ -1,0 +1,80 @@
+// Internal state of one pastix instance (the opaque pastix_data_t of the
+// C API) and helpers shared by the API translation units.
+#pragma once
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "analysis.h"
+#include "engine.h"
+#include "plan.h"
+
+extern "C" {
+#include "pastix.h"
+#include "pastix_amd.h"
+}
+
+#define STEP_INIT (1 << 0)
+#define STEP_ORDERING (1 << 1)
+#define STEP_SYMBFACT (1 << 2)
+#define STEP_ANALYSE (1 << 3)
+#define STEP_CSC2BCSC (1 << 4)
+#define STEP_BCSC2CTAB (1 << 5)
+#define STEP_NUMFACT (1 << 6)
+#define STEP_SOLVE (1 << 7)
+#define STEP_REFINE (1 << 8)
+
+struct pastix_data_s {
+    pastix_int_t *iparm = nullptr;
+    double *dparm = nullptr;
+    pastix_int_t steps = 0;
+    pastix_int_t ext[PXA_IPARM_SIZE] = {0};
+    // analysis
+    pxa::Graph graph;
+    pxa::Order order;
+    pxa::Symbol symbol;
+    pxa::Layout layout;
+    pxa::Plan plan;
+    int64_t n = 0;
+    int mtxtype = SpmGeneral;
+    int flttype = SpmDouble;
+    const spmatrix_t *csc = nullptr;
+    // numerics
+    pxa::HostBcsc bcsc;
+    std::unique_ptr<pxa::Engine> engine;
+    std::vector<char> hostL, hostU;
+    double fact_times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double *dvec = nullptr;  // 2n FP64 device vectors for refinement / error checks
+    pxa::Engine *dvec_owner = nullptr;
+    // batch of independent systems analysed as one block-diagonal matrix:
+    // system s owns rows and columns [batch_off[s], batch_off[s+1]) of the
+    // union, in the user and in the permuted numbering alike
+    std::vector<int64_t> batch_off;
+    std::vector<spm_int_t> u_colptr, u_rows;
+    std::vector<double> u_vals;
+    spmatrix_t u_spm{};
+    double *batch_dev = nullptr;  // 2 x n x R FP64 blocks of the batched refinement
+    int64_t batch_dev_cap = 0;
+    pxa::Engine *batch_dev_owner = nullptr;
+};
+
+// Triplets (0-based, symmetric input expanded to both triangles).
+struct Triplets {
+    std::vector<int64_t> i, j;
+    std::vector<double> v;
+};
+
+namespace pxa_api {
+double *device_pair(pastix_data_t *d);
+void err_print(const char *fn, const char *msg);
+int spm_triplets(const spmatrix_t *spm, Triplets &t, bool values);
+void triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
+                     std::vector<int32_t> &rows, std::vector<double> &vals, bool transpose);
+}  // namespace pxa_api
+
+

@@ -78,7 +78,7 @@ class error:
 
 
 class amd_iparm:
-    factor_precision, device, level_order, resident_input, profile = 0, 1, 2, 3, 4
+    factor_precision, device, level_order, resident_input, profile, graph = 0, 1, 2, 3, 4, 5
 
 
 class PastixError(RuntimeError):
@@ -323,6 +323,18 @@ lib.pastix_amd_fact_times.argtypes = [_P, _dp]
 lib.pastix_amd_fact_times.restype = ct.c_int
 lib.pastix_amd_flops.argtypes = [_P, _dp]
 lib.pastix_amd_flops.restype = ct.c_int
+_vp = ct.POINTER(ct.c_void_p)
+_i64p = ct.POINTER(ct.c_int64)
+lib.pastix_amd_batch_analyze.argtypes = [_P, ct.c_int64, _vp, _vp]
+lib.pastix_amd_batch_analyze.restype = ct.c_int
+lib.pastix_amd_batch_numfact.argtypes = [_P, _vp]
+lib.pastix_amd_batch_numfact.restype = ct.c_int
+lib.pastix_amd_batch_solve.argtypes = [_P, _i64p, _vp, _i64p]
+lib.pastix_amd_batch_solve.restype = ct.c_int
+lib.pastix_amd_batch_refine.argtypes = [_P, _i64p, _vp, _i64p, _vp, _i64p]
+lib.pastix_amd_batch_refine.restype = ct.c_int
+lib.pastix_amd_batch_offsets.argtypes = [_P, _i64p]
+lib.pastix_amd_batch_offsets.restype = ct.c_int
 lib.pastix_amd_kernel_times.argtypes = [_P, _dp]
 lib.pastix_amd_kernel_times.restype = ct.c_int
 lib.pastix_amd_update_flops.argtypes = [_P, _dp]
@@ -357,6 +369,65 @@ def update_flops(pastix_data):
     _check("pastix_amd_update_flops", lib.pastix_amd_update_flops(pastix_data.ptr, out))
     return {"direct": float(out[0]), "multi": float(out[1]), "direct_u": float(out[2]),
             "multi_u": float(out[3]), "wtile": float(out[4])}
+
+
+def _ptr_array(objs):
+    arr = (ct.c_void_p * len(objs))()
+    for k, o in enumerate(objs):
+        arr[k] = ct.cast(o, ct.c_void_p) if o is not None else None
+    return arr
+
+
+def batch_analyze(pastix_data, spms, orders=None):
+    """Analyse independent systems as one block-diagonal batch; orders is a
+    list of pastix_order_t (or None entries / None for nested dissection)."""
+    sp = _ptr_array([ct.addressof(S.s) for S in spms])
+    od = _ptr_array([ct.addressof(o) if o is not None else None for o in orders]) if orders else None
+    _check("pastix_amd_batch_analyze", lib.pastix_amd_batch_analyze(pastix_data.ptr, len(spms), sp, od))
+    pastix_data.batch = (spms, orders)
+
+
+def batch_numfact(pastix_data, spms):
+    sp = _ptr_array([ct.addressof(S.s) for S in spms])
+    _check("pastix_amd_batch_numfact", lib.pastix_amd_batch_numfact(pastix_data.ptr, sp))
+
+
+def _rhs_lists(xs):
+    xs = [np.asfortranarray(np.asarray(x, dtype=np.float64)) for x in xs]
+    xs = [x.reshape(x.shape[0], 1, order="F") if x.ndim == 1 else x for x in xs]
+    nr = np.array([x.shape[1] for x in xs], dtype=np.int64)
+    ld = np.array([x.shape[0] for x in xs], dtype=np.int64)
+    return xs, nr, ld
+
+
+def batch_solve(pastix_data, bs):
+    """Solves every system for its right-hand sides (list of n_s x nrhs_s
+    arrays); returns the solutions."""
+    xs, nr, ld = _rhs_lists([b.copy() for b in bs])
+    ptrs = _ptr_array([x.ctypes.data for x in xs])
+    _check("pastix_amd_batch_solve", lib.pastix_amd_batch_solve(
+        pastix_data.ptr, nr.ctypes.data_as(_i64p), ptrs, ld.ctypes.data_as(_i64p)))
+    return xs
+
+
+def batch_refine(pastix_data, bs, xs):
+    """Batched CG refinement of the solutions xs (updated copies returned)."""
+    bl, nr, ldb = _rhs_lists(bs)
+    xl, _, ldx = _rhs_lists([x.copy() for x in xs])
+    pb = _ptr_array([b.ctypes.data for b in bl])
+    px = _ptr_array([x.ctypes.data for x in xl])
+    _check("pastix_amd_batch_refine", lib.pastix_amd_batch_refine(
+        pastix_data.ptr, nr.ctypes.data_as(_i64p), pb, ldb.ctypes.data_as(_i64p), px, ldx.ctypes.data_as(_i64p)))
+    return xl
+
+
+def batch_offsets(pastix_data):
+    n = lib.pastix_amd_batch_offsets(pastix_data.ptr, None)
+    if n < 0:
+        raise PastixError("pastix_amd_batch_offsets", n)
+    off = np.zeros(n + 1, dtype=np.int64)
+    lib.pastix_amd_batch_offsets(pastix_data.ptr, off.ctypes.data_as(_i64p))
+    return off
 
 
 def flops(pastix_data):

@@ -192,3 +192,41 @@ def test_factor_bitwise_reproducible():
     L2, _ = pastix.coeftab(h)
     assert np.array_equal(L1, L2)
     pastix.finalize(h)
+
+
+@pytest.mark.parametrize("facto", [LLT, LU])
+def test_graph_replays_match_eager(facto):
+    """Numfact and device solves replayed from captured HIP graphs give the
+    eager results bit for bit, on every replay, with eager work and host
+    synchronisation between the replays."""
+    grid = (10, 9, 8)
+    if facto == LLT:
+        A = spm.laplacian7(*grid, diag=6.0)
+    else:
+        A = spm.convection_diffusion27(*grid, seed=3)
+    h, S, sym = analyse(A, facto == LLT, facto, grid=grid, min_bs=16, max_bs=40)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.resident_input, 1)
+    pastix.task_numfact(h, S)
+    L0, U0 = pastix.coeftab(h, lu=facto == LU)
+    x0, b = spm.rhs_from_x(A, 3, seed=11)
+    db = pastix.DeviceArray(b)
+    pastix.solve_device(h, db, A.shape[0], 3)
+    xe = db.download()
+    pastix.set_amd_iparm(h, pastix.amd_iparm.graph, 1)
+    for rep in range(3):
+        pastix.refactorize(h)
+        L1, U1 = pastix.coeftab(h, lu=facto == LU)
+        assert np.array_equal(L0, L1), f"numfact replay {rep} differs"
+        if facto == LU:
+            assert np.array_equal(U0, U1), f"numfact replay {rep} differs (U)"
+        db.upload(b)
+        pastix.solve_device(h, db, A.shape[0], 3)
+        assert np.array_equal(db.download(), xe), f"solve replay {rep} differs"
+        # eager work in between
+        pastix.set_amd_iparm(h, pastix.amd_iparm.graph, 0)
+        db.upload(b)
+        pastix.solve_device(h, db, A.shape[0], 3)
+        assert np.array_equal(db.download(), xe)
+        pastix.set_amd_iparm(h, pastix.amd_iparm.graph, 1)
+    db.free()
+    pastix.finalize(h)
