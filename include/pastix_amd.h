@@ -143,6 +143,50 @@ int pastix_amd_batch_offsets(const pastix_data_t *pastix_data, int64_t *off);
  * updates DPARM_FACT_TIME / DPARM_FACT_FLOPS like pastix_subtask_sopalin. */
 int pastix_amd_refactorize(pastix_data_t *pastix_data);
 
+/* Refactorise A + delta*I (same pattern, no counterpart in the reference):
+ * the filled coefficients are restored from their device copy (needs
+ * PXA_IPARM_RESIDENT_INPUT), delta is added to their diagonal and to the
+ * diagonal of the device matrix used by the refinement, DPARM_A_NORM becomes
+ * ||A + delta I||_F (static-pivot threshold), and the factorisation reruns.
+ * delta is relative to the analysed matrix, not cumulative; every diagonal
+ * entry must be stored.  pastix_amd_refactorize() returns to delta = 0. */
+int pastix_amd_refactorize_shift(pastix_data_t *pastix_data, double delta);
+
+/* Device memory held by the instance's engine, in bytes (factor, inverses,
+ * plan, work space, input copy). */
+int pastix_amd_device_bytes(const pastix_data_t *pastix_data, int64_t *bytes);
+
+/* ---- Serving loop (no counterpart in the reference) ----------------------
+ * A server drives nslots instances analysed and factored (LL^t) on the same
+ * pattern with PXA_IPARM_RESIDENT_INPUT set; nslots is what the HBM budget
+ * admits (pastix_amd_device_bytes gives one instance's share).  Slot 0
+ * serves the matrix it was factored on (version 0).  Requests:
+ *   solve: nrhs FP64 right-hand sides at b (user ordering, ldb >= n),
+ *          overwritten with the solution of the newest matrix version
+ *          submitted before the request; queued solves of one version are
+ *          merged in arrival order, up to max_rhs columns, into one solve in
+ *          the factor precision followed by batched FP64 CG
+ *          (DPARM_EPSILON_REFINEMENT, IPARM_ITERMAX of the slot);
+ *   refactor: the next version, A + shift*I (A the analysed matrix); it
+ *          runs on a slot whose version has no solve left while other slots
+ *          keep solving, each slot on its own HIP stream.
+ * One worker thread per slot.  b must stay valid until the request is
+ * waited for. */
+typedef struct pastix_amd_server_s pastix_amd_server_t;
+int pastix_amd_server_create(pastix_amd_server_t **srv, pastix_data_t **slots, int nslots, pastix_int_t max_rhs);
+int pastix_amd_server_submit_solve(pastix_amd_server_t *srv, pastix_int_t nrhs, double *b, pastix_int_t ldb,
+                                   int64_t *ticket);
+int pastix_amd_server_submit_refactor(pastix_amd_server_t *srv, double shift, int64_t *ticket);
+/* Blocks until the request is done; latency = completion - submission
+ * (seconds), version = matrix version it used or created.  Returns the
+ * request's status. */
+int pastix_amd_server_wait(pastix_amd_server_t *srv, int64_t ticket, double *latency, int64_t *version);
+/* out[0] solve batches run, [1] right-hand sides they carried, [2] newest
+ * version, [3] requests not finished. */
+int pastix_amd_server_stats(pastix_amd_server_t *srv, int64_t out[4]);
+/* Finishes every admitted request, stops the workers, frees the server. */
+int pastix_amd_server_destroy(pastix_amd_server_t **srv);
+
 #ifdef __cplusplus
 }
 #endif

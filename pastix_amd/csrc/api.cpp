@@ -857,8 +857,20 @@ extern "C" int pastix_amd_plan_stats(const pastix_data_t *d, double *out, int ma
     return nl;
 }
 
+extern "C" int pastix_amd_refactorize_shift(pastix_data_t *d, double delta) {
+    if (!d || !d->engine || !(d->steps & STEP_BCSC2CTAB)) return PASTIX_ERR_BADPARAMETER;
+    pxa::Engine &E = *d->engine;
+    if (E.shift_diagonal(delta)) return PASTIX_ERR_BADPARAMETER;
+    // ||A + delta I||_F for the static-pivot threshold
+    const double nf = d->bcsc.normF;
+    d->dparm[DPARM_A_NORM] = std::sqrt(std::max(0.0, nf * nf + 2.0 * delta * E.diag_sum() + (double)d->n * delta * delta));
+    E.profile = d->ext[PXA_IPARM_PROFILE] != 0;
+    return pastix_subtask_sopalin(d);
+}
+
 extern "C" int pastix_amd_refactorize(pastix_data_t *d) {
     if (!d || !d->engine || !(d->steps & STEP_BCSC2CTAB)) return PASTIX_ERR_BADPARAMETER;
+    if (d->engine->shift() != 0.0) return pastix_amd_refactorize_shift(d, 0.0);
     if (d->engine->restore_coef()) return PASTIX_ERR_BADPARAMETER;
     d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
     return pastix_subtask_sopalin(d);

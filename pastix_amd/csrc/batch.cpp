@@ -144,6 +144,44 @@ bool rhs_ok(const pastix_data_t *d, const pastix_int_t *nrhs, const void *const 
 
 }  // namespace
 
+// Batched CG on n x R blocks in the user numbering (column-major, ld = n):
+// permutes into the factor's numbering, refines on the device with one
+// recurrence per (segment, column), permutes back.
+int pxa_api::refine_block(pastix_data_t *d, int R, const double *B, double *X, const std::vector<int64_t> &segoff) {
+    if (!(d->steps & STEP_NUMFACT) || !d->engine) return PASTIX_ERR_BADPARAMETER;
+    const int64_t N = d->n;
+    std::vector<double> Bp((size_t)N * R), Xp((size_t)N * R);
+    const std::vector<int64_t> &perm = d->order.permtab;
+    for (int c = 0; c < R; c++)
+        for (int64_t i = 0; i < N; i++) {
+            Bp[perm[i] + (size_t)c * N] = B[i + (size_t)c * N];
+            Xp[perm[i] + (size_t)c * N] = X[i + (size_t)c * N];
+        }
+    pxa::Engine &E = *d->engine;
+    if (d->batch_dev_owner != &E || d->batch_dev_cap < 2 * N * R) {
+        d->batch_dev = (double *)E.alloc((size_t)2 * N * R * sizeof(double));
+        d->batch_dev_cap = 2 * N * R;
+        d->batch_dev_owner = &E;
+        if (!d->batch_dev) return PASTIX_ERR_OUTOFMEMORY;
+    }
+    double *dB = d->batch_dev, *dX = dB + (size_t)N * R;
+    if (hipMemcpy(dB, Bp.data(), Bp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dX, Xp.data(), Xp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return PASTIX_ERR_INTERNAL;
+    double eps = d->dparm[DPARM_EPSILON_REFINEMENT];
+    if (eps < 0.) eps = 1e-12;
+    int it = 0;
+    double res = 0;
+    if (E.refine_cg_batch(dX, dB, R, segoff, (int)d->iparm[IPARM_ITERMAX], eps, &it, &res)) return PASTIX_ERR_INTERNAL;
+    if (hipMemcpy(Xp.data(), dX, Xp.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return PASTIX_ERR_INTERNAL;
+    for (int c = 0; c < R; c++)
+        for (int64_t i = 0; i < N; i++) X[i + (size_t)c * N] = Xp[perm[i] + (size_t)c * N];
+    d->iparm[IPARM_NBITER] = it;
+    d->dparm[DPARM_RELATIVE_ERROR] = res;
+    return PASTIX_SUCCESS;
+}
+
 extern "C" int pastix_amd_batch_analyze(pastix_data_t *d, pastix_int_t nsys, const spmatrix_t *const *spms,
                                         const pastix_order_t *const *orders) {
     if (!d || nsys < 1 || !spms) return PASTIX_ERR_BADPARAMETER;
@@ -239,41 +277,12 @@ extern "C" int pastix_amd_batch_refine(pastix_data_t *d, const pastix_int_t *nrh
     if (!rhs_ok(d, nrhs, b, ldb, R) || !rhs_ok(d, nrhs, x, ldx, Rx)) return PASTIX_ERR_BADPARAMETER;
     if (R == 0) return PASTIX_SUCCESS;
     const auto t0 = std::chrono::steady_clock::now();
-    const int64_t N = d->batch_off.back();
-    std::vector<double> B, X, Bp((size_t)N * R), Xp((size_t)N * R);
+    std::vector<double> B, X;
     pack(d->batch_off, R, nrhs, b, ldb, B);
     pack(d->batch_off, R, nrhs, x, ldx, X);
-    const std::vector<int64_t> &perm = d->order.permtab;
-    for (pastix_int_t c = 0; c < R; c++)
-        for (int64_t i = 0; i < N; i++) {
-            Bp[perm[i] + (size_t)c * N] = B[i + (size_t)c * N];
-            Xp[perm[i] + (size_t)c * N] = X[i + (size_t)c * N];
-        }
-    pxa::Engine &E = *d->engine;
-    if (d->batch_dev_owner != &E || d->batch_dev_cap < 2 * N * R) {
-        d->batch_dev = (double *)E.alloc((size_t)2 * N * R * sizeof(double));
-        d->batch_dev_cap = 2 * N * R;
-        d->batch_dev_owner = &E;
-        if (!d->batch_dev) return PASTIX_ERR_OUTOFMEMORY;
-    }
-    double *dB = d->batch_dev;
-    double *dX = dB + (size_t)N * R;
-    if (hipMemcpy(dB, Bp.data(), Bp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dX, Xp.data(), Xp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-        return PASTIX_ERR_INTERNAL;
-    double eps = d->dparm[DPARM_EPSILON_REFINEMENT];
-    if (eps < 0.) eps = 1e-12;
-    int it = 0;
-    double res = 0;
-    if (E.refine_cg_batch(dX, dB, (int)R, d->batch_off, (int)d->iparm[IPARM_ITERMAX], eps, &it, &res))
-        return PASTIX_ERR_INTERNAL;
-    if (hipMemcpy(Xp.data(), dX, Xp.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-        return PASTIX_ERR_INTERNAL;
-    for (pastix_int_t c = 0; c < R; c++)
-        for (int64_t i = 0; i < N; i++) X[i + (size_t)c * N] = Xp[perm[i] + (size_t)c * N];
+    const int rc = refine_block(d, (int)R, B.data(), X.data(), d->batch_off);
+    if (rc) return rc;
     unpack(d->batch_off, nrhs, X, x, ldx);
-    d->iparm[IPARM_NBITER] = it;
-    d->dparm[DPARM_RELATIVE_ERROR] = res;
     d->dparm[DPARM_REFINE_TIME] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return PASTIX_SUCCESS;
 }

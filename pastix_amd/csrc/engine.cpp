@@ -44,6 +44,7 @@ void *Engine::alloc(size_t bytes) {
         return nullptr;
     }
     owned_.push_back(p);
+    bytes_ += (int64_t)bytes;
     return p;
 }
 
@@ -118,6 +119,14 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     lv_diag_ptr_ = P.lv_diag_ptr;
     d_bwd_ = upload(*this, P.bwd);
     bwd_part_ = P.bwd_part;
+    {
+        std::vector<int64_t> dp(n_);
+        for (int64_t c = 0; c < cblknbr_; c++) {
+            const int64_t f = s.cblk_fcol[c], K = s.cblk_fcol[c + 1] - f, st = L.cblk_stride[c];
+            for (int64_t l = 0; l < K; l++) dp[f + l] = L.cblk_coef[c] + l + l * st;
+        }
+        d_diagpos_ = upload(*this, dp);
+    }
     d_wgrp_ = upload(*this, P.wgrp);
     d_wtiles_ = upload(*this, P.wtiles);
     lv_wt_ptr_ = P.lv_wt_ptr;
@@ -539,10 +548,39 @@ int Engine::upload_matrix(const HostBcsc &bc) {
         d_cols_ = upload(*this, bc.cols);
         d_vals_ = (double *)alloc(bc.tvals.size() * sizeof(double));
         if (!d_rowptr_ || !d_cols_ || !d_vals_) return -1;
+        std::vector<int64_t> cd(n_, -1);
+        for (int64_t i = 0; i < n_; i++)
+            for (int64_t k = bc.rowptr[i]; k < bc.rowptr[i + 1]; k++)
+                if (bc.cols[k] == i) cd[i] = k;
+        diag_complete_ = std::find(cd.begin(), cd.end(), (int64_t)-1) == cd.end();
+        d_csrdiag_ = upload(*this, cd);
+        d_diag0_ = (double *)alloc(n_ * sizeof(double));
+        if (!d_csrdiag_ || !d_diag0_) return -1;
         have_matrix_ = true;
+    }
+    {
+        std::vector<double> dg(n_, 0.0);
+        diag_sum_ = 0;
+        for (int64_t i = 0; i < n_; i++)
+            for (int64_t k = bc.rowptr[i]; k < bc.rowptr[i + 1]; k++)
+                if (bc.cols[k] == i) dg[i] = bc.tvals[k];
+        for (int64_t i = 0; i < n_; i++) diag_sum_ += dg[i];
+        HIPCHK(hipMemcpy(d_diag0_, dg.data(), n_ * sizeof(double), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpyAsync(d_vals_, bc.tvals.data(), bc.tvals.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Engine::shift_diagonal(double delta) {
+    if (!have_matrix_) return fail("matrix not uploaded");
+    if (!diag_complete_) return fail("diagonal shift needs every diagonal entry stored");
+    if (restore_coef()) return -1;
+    if (prec_ == 2) launch_add_diag<float>(n_, d_diagpos_, delta, (float *)d_L_, stream_);
+    else launch_add_diag<double>(n_, d_diagpos_, delta, (double *)d_L_, stream_);
+    launch_set_csr_diag(n_, d_csrdiag_, d_diag0_, delta, d_vals_, stream_);
+    shift_ = delta;
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

@@ -54,10 +54,19 @@ class Engine {
     hipStream_t stream() const { return stream_; }
     int64_t n() const { return n_; }
     void *alloc(size_t bytes);
+    int64_t device_bytes() const { return bytes_; }
     const std::string &last_error() const { return err_; }
     int device() const { return dev_; }
     // applies M^{-1} (one solve in factor precision) to an FP64 device vector
     int precond(double *dv);
+    // Replace the matrix by A + delta*I for the next factorisation and for the
+    // refinement products: restores the filled coefficients from the device
+    // snapshot and adds delta to their diagonal and to the diagonal of the
+    // FP64 matrix.  Needs snapshot_coef() and upload_matrix() first; every
+    // diagonal entry must be stored.
+    int shift_diagonal(double delta);
+    double diag_sum() const { return diag_sum_; }
+    double shift() const { return shift_; }
     // y = alpha*A*x + beta*y on FP64 device vectors
     int spmv(double alpha, const double *x, double beta, double *y);
     double dot(const double *x, const double *y);
@@ -95,6 +104,7 @@ class Engine {
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<void *> owned_;
+    int64_t bytes_ = 0;
     std::string err_;
 
     int64_t n_ = 0, cblknbr_ = 0, coefnbr_ = 0, winvnbr_ = 0;
@@ -167,6 +177,13 @@ class Engine {
     int32_t *d_cols_ = nullptr;
     double *d_vals_ = nullptr;
     bool have_matrix_ = false;
+    // diagonal positions: in the coefficient table (per permuted column) and
+    // in the CSR values, with the original FP64 diagonal
+    int64_t *d_diagpos_ = nullptr, *d_csrdiag_ = nullptr;
+    double *d_diag0_ = nullptr;
+    double diag_sum_ = 0;
+    bool diag_complete_ = false;
+    double shift_ = 0;  // delta of the last shift_diagonal
     // refinement vectors
     double *d_vec_ = nullptr;
     int64_t vec_cap_ = 0;

@@ -80,6 +80,11 @@ void launch_seg_axpy(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg,
                      const double *x, double *y, hipStream_t st);
 void launch_seg_xpby(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg, const double *b, const double *z,
                      double *p, hipStream_t st);
+// coef[pos[i]] += delta (diagonal of the coefficient table);
+// vals[pos[i]] = d0[i] + delta where pos[i] >= 0 (diagonal of the CSR).
+template <typename T>
+void launch_add_diag(int64_t n, const int64_t *pos, double delta, T *coef, hipStream_t st);
+void launch_set_csr_diag(int64_t n, const int64_t *pos, const double *d0, double delta, double *vals, hipStream_t st);
 // Zeroes n ints with a kernel (usable inside a captured graph).
 void launch_zero_int(int n, int *p, hipStream_t st);
 void launch_d2s(int64_t n, const double *x, float *y, hipStream_t st);

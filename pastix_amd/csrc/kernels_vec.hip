@@ -252,6 +252,28 @@ void launch_seg_xpby(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg,
                            z, p);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_add_diag(int64_t n, const int64_t *__restrict__ pos, double delta,
+                                                  T *__restrict__ coef) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) coef[pos[i]] += (T)delta;
+}
+__global__ __launch_bounds__(256) void k_set_csr_diag(int64_t n, const int64_t *__restrict__ pos,
+                                                      const double *__restrict__ d0, double delta,
+                                                      double *__restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n && pos[i] >= 0) vals[pos[i]] = d0[i] + delta;
+}
+template <typename T>
+void launch_add_diag(int64_t n, const int64_t *pos, double delta, T *coef, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_add_diag<T>, dim3(nblk(n)), dim3(256), 0, st, n, pos, delta, coef);
+}
+template void launch_add_diag<double>(int64_t, const int64_t *, double, double *, hipStream_t);
+template void launch_add_diag<float>(int64_t, const int64_t *, double, float *, hipStream_t);
+void launch_set_csr_diag(int64_t n, const int64_t *pos, const double *d0, double delta, double *vals, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_set_csr_diag, dim3(nblk(n)), dim3(256), 0, st, n, pos, d0, delta, vals);
+}
+
 void launch_zero_int(int n, int *p, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, n, p);
 }

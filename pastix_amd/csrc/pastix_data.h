@@ -75,6 +75,8 @@ void err_print(const char *fn, const char *msg);
 int spm_triplets(const spmatrix_t *spm, Triplets &t, bool values);
 void triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
                      std::vector<int32_t> &rows, std::vector<double> &vals, bool transpose);
+// Batched CG refinement of n x R user-ordered FP64 blocks (see batch.cpp).
+int refine_block(pastix_data_t *d, int R, const double *B, double *X, const std::vector<int64_t> &segoff);
 }  // namespace pxa_api
 
 

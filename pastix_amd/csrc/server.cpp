@@ -1,0 +1,270 @@
+// Serving loop: device-resident factors, continuous batching of solves,
+// refactorisations of the same pattern overlapped with solves.
+//
+// The server drives K slots, each a pastix instance analysed and factored on
+// the same pattern (so K factors fit the HBM budget the caller sized them
+// for).  Requests are taken in arrival order:
+//   - a refactorisation (A + shift I) creates matrix version v+1; it runs on
+//     a slot whose current version has no queued or running solve left, so
+//     older solves keep their factor while the new one is built on another
+//     slot's stream;
+//   - a solve binds, at submission, to the newest version submitted; queued
+//     solves of the version a slot holds are merged, in arrival order and up
+//     to max_rhs right-hand sides, into one multi-RHS solve + batched CG
+//     refinement (one CG recurrence per column, so every column's result does
+//     not depend on its batch mates beyond rounding).
+// One worker thread per slot issues that slot's work on its own HIP stream:
+// slots overlap on the device.  No request waits for a later one, and a slot
+// holding version v is freed as soon as v's solves drain, so the loop always
+// makes progress.
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "pastix_data.h"
+
+using namespace pxa_api;
+
+namespace {
+using Clock = std::chrono::steady_clock;
+
+struct Req {
+    int64_t id = 0;
+    int kind = 0;  // 0 solve, 1 refactorisation
+    int64_t version = 0;
+    pastix_int_t nrhs = 0;
+    double *b = nullptr;
+    pastix_int_t ldb = 0;
+    double shift = 0;
+    int state = 0;  // 0 queued, 1 running, 2 done
+    int rc = 0;
+    int slot = -1;
+    Clock::time_point t_submit, t_done;
+};
+}  // namespace
+
+struct pastix_amd_server_s {
+    std::vector<pastix_data_t *> slots;
+    pastix_int_t max_rhs = 64;
+    int64_t n = 0;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Req *> queue;
+    std::unordered_map<int64_t, std::unique_ptr<Req>> reqs;
+    std::map<int64_t, int64_t> live_solves;  // version -> queued or running solves
+    int64_t next_id = 1, latest = 0;
+    std::vector<int64_t> held;
+    std::vector<char> ready;
+    std::vector<std::thread> workers;
+    bool stop = false;
+    // statistics
+    int64_t batches = 0, batched_rhs = 0;
+};
+
+namespace {
+
+void run_solves(pastix_amd_server_s *S, int k, std::vector<Req *> &batch) {
+    pastix_data_t *d = S->slots[k];
+    const int64_t n = S->n;
+    pastix_int_t R = 0;
+    for (Req *r : batch) R += r->nrhs;
+    std::vector<double> B((size_t)n * R), X;
+    pastix_int_t c0 = 0;
+    for (Req *r : batch) {
+        for (pastix_int_t c = 0; c < r->nrhs; c++)
+            for (int64_t i = 0; i < n; i++) B[i + (size_t)(c0 + c) * n] = r->b[i + (size_t)c * r->ldb];
+        c0 += r->nrhs;
+    }
+    X = B;
+    int rc = pastix_task_solve(d, R, X.data(), n);
+    if (!rc) rc = refine_block(d, (int)R, B.data(), X.data(), std::vector<int64_t>{0, n});
+    c0 = 0;
+    for (Req *r : batch) {
+        if (!rc)
+            for (pastix_int_t c = 0; c < r->nrhs; c++)
+                for (int64_t i = 0; i < n; i++) r->b[i + (size_t)c * r->ldb] = X[i + (size_t)(c0 + c) * n];
+        c0 += r->nrhs;
+        r->rc = rc;
+    }
+}
+
+void worker(pastix_amd_server_s *S, int k) {
+    (void)hipSetDevice((int)S->slots[k]->ext[PXA_IPARM_DEVICE]);
+    std::unique_lock<std::mutex> lk(S->mu);
+    while (!S->stop) {
+        // 1. the oldest queued refactorisation, if this slot's version is drained
+        Req *ref = nullptr;
+        for (Req *r : S->queue)
+            if (r->kind == 1 && r->state == 0) {
+                ref = r;
+                break;
+            }
+        const bool free_slot = S->held[k] < 0 || S->live_solves[S->held[k]] == 0;
+        if (ref && free_slot) {
+            ref->state = 1;
+            ref->slot = k;
+            S->held[k] = ref->version;
+            S->ready[k] = 0;
+            lk.unlock();
+            const int rc = pastix_amd_refactorize_shift(S->slots[k], ref->shift);
+            lk.lock();
+            ref->rc = rc;
+            ref->state = 2;
+            ref->t_done = Clock::now();
+            S->ready[k] = rc == 0;
+            for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
+                if (*it == ref) {
+                    S->queue.erase(it);
+                    break;
+                }
+            S->cv.notify_all();
+            continue;
+        }
+        // 2. a batch of queued solves of the version this slot holds
+        std::vector<Req *> batch;
+        if (S->held[k] >= 0 && S->ready[k]) {
+            pastix_int_t R = 0;
+            for (Req *r : S->queue) {
+                if (r->kind != 0 || r->state != 0 || r->version != S->held[k]) continue;
+                if (!batch.empty() && R + r->nrhs > S->max_rhs) break;
+                batch.push_back(r);
+                R += r->nrhs;
+            }
+        }
+        if (!batch.empty()) {
+            for (Req *r : batch) {
+                r->state = 1;
+                r->slot = k;
+            }
+            lk.unlock();
+            run_solves(S, k, batch);
+            lk.lock();
+            const auto now = Clock::now();
+            for (Req *r : batch) {
+                r->state = 2;
+                r->t_done = now;
+                S->live_solves[r->version]--;
+                for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
+                    if (*it == r) {
+                        S->queue.erase(it);
+                        break;
+                    }
+                S->batched_rhs += r->nrhs;
+            }
+            S->batches++;
+            S->cv.notify_all();
+            continue;
+        }
+        S->cv.wait(lk);
+    }
+}
+
+}  // namespace
+
+extern "C" int pastix_amd_server_create(pastix_amd_server_t **srv, pastix_data_t **slots, int nslots,
+                                        pastix_int_t max_rhs) {
+    if (!srv || !slots || nslots < 1 || max_rhs < 1) return PASTIX_ERR_BADPARAMETER;
+    for (int k = 0; k < nslots; k++) {
+        pastix_data_t *d = slots[k];
+        if (!d || !d->engine || !(d->steps & STEP_NUMFACT) || !d->ext[PXA_IPARM_RESIDENT_INPUT] ||
+            d->n != slots[0]->n || d->symbol.cblknbr() != slots[0]->symbol.cblknbr() ||
+            d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF)
+            return PASTIX_ERR_BADPARAMETER;
+    }
+    auto *S = new pastix_amd_server_s();
+    S->slots.assign(slots, slots + nslots);
+    S->max_rhs = max_rhs;
+    S->n = slots[0]->n;
+    S->held.assign(nslots, -1);
+    S->ready.assign(nslots, 0);
+    S->held[0] = 0;  // slot 0 serves the matrix it was factored on
+    S->ready[0] = 1;
+    for (int k = 0; k < nslots; k++) S->workers.emplace_back(worker, S, k);
+    *srv = S;
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_server_submit_solve(pastix_amd_server_t *S, pastix_int_t nrhs, double *b, pastix_int_t ldb,
+                                              int64_t *ticket) {
+    if (!S || !b || nrhs < 1 || ldb < S->n || !ticket) return PASTIX_ERR_BADPARAMETER;
+    auto r = std::make_unique<Req>();
+    r->kind = 0;
+    r->nrhs = nrhs;
+    r->b = b;
+    r->ldb = ldb;
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> g(S->mu);
+    r->id = S->next_id++;
+    r->version = S->latest;
+    S->live_solves[r->version]++;
+    *ticket = r->id;
+    S->queue.push_back(r.get());
+    S->reqs.emplace(r->id, std::move(r));
+    S->cv.notify_all();
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_server_submit_refactor(pastix_amd_server_t *S, double shift, int64_t *ticket) {
+    if (!S || !ticket) return PASTIX_ERR_BADPARAMETER;
+    auto r = std::make_unique<Req>();
+    r->kind = 1;
+    r->shift = shift;
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> g(S->mu);
+    r->id = S->next_id++;
+    r->version = ++S->latest;
+    *ticket = r->id;
+    S->queue.push_back(r.get());
+    S->reqs.emplace(r->id, std::move(r));
+    S->cv.notify_all();
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_server_wait(pastix_amd_server_t *S, int64_t ticket, double *latency, int64_t *version) {
+    if (!S) return PASTIX_ERR_BADPARAMETER;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) return PASTIX_ERR_BADPARAMETER;
+    Req *r = it->second.get();
+    S->cv.wait(lk, [&] { return r->state == 2; });
+    if (latency) *latency = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+    if (version) *version = r->version;
+    const int rc = r->rc;
+    S->reqs.erase(it);
+    return rc;
+}
+
+extern "C" int pastix_amd_server_stats(pastix_amd_server_t *S, int64_t out[4]) {
+    if (!S || !out) return PASTIX_ERR_BADPARAMETER;
+    std::lock_guard<std::mutex> g(S->mu);
+    out[0] = S->batches;
+    out[1] = S->batched_rhs;
+    out[2] = S->latest;
+    out[3] = (int64_t)S->queue.size();
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_server_destroy(pastix_amd_server_t **srv) {
+    if (!srv || !*srv) return PASTIX_ERR_BADPARAMETER;
+    pastix_amd_server_s *S = *srv;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        // finish what was admitted
+        S->cv.wait(lk, [&] { return S->queue.empty(); });
+        S->stop = true;
+        S->cv.notify_all();
+    }
+    for (std::thread &t : S->workers) t.join();
+    delete S;
+    *srv = nullptr;
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_device_bytes(const pastix_data_t *d, int64_t *bytes) {
+    if (!d || !d->engine || !bytes) return PASTIX_ERR_BADPARAMETER;
+    *bytes = d->engine->device_bytes();
+    return PASTIX_SUCCESS;
+}

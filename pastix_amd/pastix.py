@@ -437,6 +437,86 @@ def flops(pastix_data):
     return out
 
 
+lib.pastix_amd_refactorize_shift.argtypes = [_P, ct.c_double]
+lib.pastix_amd_refactorize_shift.restype = ct.c_int
+
+
+def refactorize_shift(pastix_data, delta):
+    """Refactorise A + delta*I from the HBM-resident filled coefficients."""
+    _check("pastix_amd_refactorize_shift", lib.pastix_amd_refactorize_shift(pastix_data.ptr, float(delta)))
+
+
+lib.pastix_amd_device_bytes.argtypes = [_P, _i64p]
+lib.pastix_amd_device_bytes.restype = ct.c_int
+lib.pastix_amd_server_create.argtypes = [ct.POINTER(ct.c_void_p), ct.POINTER(_P), ct.c_int, ct.c_int64]
+lib.pastix_amd_server_create.restype = ct.c_int
+lib.pastix_amd_server_submit_solve.argtypes = [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64, _i64p]
+lib.pastix_amd_server_submit_solve.restype = ct.c_int
+lib.pastix_amd_server_submit_refactor.argtypes = [ct.c_void_p, ct.c_double, _i64p]
+lib.pastix_amd_server_submit_refactor.restype = ct.c_int
+lib.pastix_amd_server_wait.argtypes = [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double), _i64p]
+lib.pastix_amd_server_wait.restype = ct.c_int
+lib.pastix_amd_server_stats.argtypes = [ct.c_void_p, _i64p]
+lib.pastix_amd_server_stats.restype = ct.c_int
+lib.pastix_amd_server_destroy.argtypes = [ct.POINTER(ct.c_void_p)]
+lib.pastix_amd_server_destroy.restype = ct.c_int
+
+
+def device_bytes(pastix_data):
+    v = ct.c_int64()
+    _check("pastix_amd_device_bytes", lib.pastix_amd_device_bytes(pastix_data.ptr, ct.byref(v)))
+    return int(v.value)
+
+
+class Server:
+    """Serving loop over slots (instances factored on the same pattern with
+    resident input): continuous batching of solves, refactorisations of
+    A + shift*I overlapped with solves on other slots."""
+
+    def __init__(self, slots, max_rhs=64):
+        arr = (_P * len(slots))(*[h.ptr for h in slots])
+        self._slots = slots
+        self.ptr = ct.c_void_p()
+        _check("pastix_amd_server_create", lib.pastix_amd_server_create(ct.byref(self.ptr), arr, len(slots), max_rhs))
+        self._bufs = {}
+
+    def submit_solve(self, b):
+        """b: n x nrhs FP64, overwritten with the solution; returns a ticket."""
+        b = np.asfortranarray(b, dtype=np.float64)
+        if b.ndim == 1:
+            b = b.reshape(-1, 1, order="F")
+        t = ct.c_int64()
+        _check("pastix_amd_server_submit_solve",
+               lib.pastix_amd_server_submit_solve(self.ptr, b.shape[1], b.ctypes.data, b.shape[0], ct.byref(t)))
+        self._bufs[t.value] = b
+        return t.value
+
+    def submit_refactor(self, shift):
+        t = ct.c_int64()
+        _check("pastix_amd_server_submit_refactor",
+               lib.pastix_amd_server_submit_refactor(self.ptr, float(shift), ct.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        """Returns (result array or None, latency seconds, matrix version)."""
+        lat = ct.c_double()
+        ver = ct.c_int64()
+        rc = lib.pastix_amd_server_wait(self.ptr, ticket, ct.byref(lat), ct.byref(ver))
+        if rc != 0:
+            raise PastixError("pastix_amd_server_wait", rc)
+        return self._bufs.pop(ticket, None), float(lat.value), int(ver.value)
+
+    def stats(self):
+        out = np.zeros(4, dtype=np.int64)
+        _check("pastix_amd_server_stats", lib.pastix_amd_server_stats(self.ptr, out.ctypes.data_as(_i64p)))
+        return {"batches": int(out[0]), "batched_rhs": int(out[1]), "version": int(out[2]), "pending": int(out[3])}
+
+    def close(self):
+        if self.ptr:
+            _check("pastix_amd_server_destroy", lib.pastix_amd_server_destroy(ct.byref(self.ptr)))
+            self.ptr = ct.c_void_p()
+
+
 def refactorize(pastix_data):
     _check("pastix_amd_refactorize", lib.pastix_amd_refactorize(pastix_data.ptr))
 

@@ -1,0 +1,85 @@
+"""Serving loop on the GPU: continuous batching of solves and refactorised
+versions A + shift*I overlapped on two slots.
+
+Bars: every solve reaches berr = max|b-Ax| / (||A||_inf max|x| + max|b|)
+<= 1e-12 against the matrix version it was bound to at submission, and
+equals the same request solved alone (one request, one version, plain
+instance) within cond(A) * 1e-12, taken as 1e-8 relative in max norm for
+these anisotropic Laplacians (cond(A) < 1e4).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sps
+
+from pastix_amd import pastix, spm
+
+pytestmark = pytest.mark.gpu
+
+GRID = (12, 11, 10)
+
+
+def _berr(A, x, b):
+    r = b - A @ x
+    return float(np.abs(r).max() / (abs(A).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max()))
+
+
+def _slot(A, prec):
+    S = spm.Spm(A, symmetric=True)
+    ip, dp = pastix.initParam()
+    ip[pastix.iparm.factorization] = pastix.factotype.LLT
+    ip[pastix.iparm.ordering] = pastix.order.Personal
+    ip[pastix.iparm.refinement] = pastix.refine.CG
+    h = pastix.init(ip, dp)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.factor_precision, prec)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.resident_input, 1)
+    o = pastix.orderGrid(*GRID)
+    h.order = o
+    pastix.subtask_order(h, S, o)
+    pastix.subtask_symbfact(h)
+    pastix.subtask_reordering(h)
+    pastix.subtask_blend(h)
+    pastix.task_numfact(h, S)
+    return h, S
+
+
+def test_server_mix_equals_alone():
+    A = spm.laplacian7(*GRID, coef=(1.0, 1.0, 0.01))
+    n = A.shape[0]
+    slots = [_slot(A, spm.SpmFloat) for _ in range(2)]
+    srv = pastix.Server([h for h, _ in slots], max_rhs=16)
+    rng = np.random.default_rng(21)
+    shifts = {0: 0.0}
+    reqs = []
+    for k in range(14):
+        if k in (4, 9):
+            shift = float(rng.uniform(0.0, 0.1))
+            t = srv.submit_refactor(shift)
+            reqs.append(("refactor", t, shift, None))
+        else:
+            nrhs = int(rng.choice([1, 4, 16]))
+            b = rng.uniform(-1.0, 1.0, size=(n, nrhs))
+            t = srv.submit_solve(b.copy(order="F"))
+            reqs.append(("solve", t, None, b))
+    results = []
+    for kind, t, shift, b in reqs:
+        x, lat, ver = srv.wait(t)
+        if kind == "refactor":
+            shifts[ver] = shift
+        else:
+            results.append((b, x, ver, lat))
+    st = srv.stats()
+    srv.close()
+    assert st["pending"] == 0 and st["version"] == 2
+    # alone: one request, its own version, a plain instance
+    h1, _ = slots[0]
+    for b, x, ver, lat in results:
+        Av = A + shifts[ver] * sps.eye(n)
+        for c in range(b.shape[1]):
+            assert _berr(Av, x[:, c], b[:, c]) <= 1e-12
+        pastix.refactorize_shift(h1, shifts[ver])
+        xa = b.copy(order="F")
+        pastix.task_solve(h1, slots[0][1], xa)
+        pastix.task_refine(h1, slots[0][1], b.copy(order="F"), xa)
+        assert np.abs(x - xa).max() <= 1e-8 * np.abs(xa).max()
+    for h, _ in slots:
+        pastix.finalize(h)
