@@ -74,6 +74,198 @@ def cpu_baseline(pastix, spm, grid):
     return th / t / 1e9, t
 
 
+def leg_getrf(pastix, spm, grid, seed=5):
+    """Nonsymmetric 27-point convection-diffusion (symmetric pattern), GETRF
+    with static pivoting at eps*||A||: FP64 factor vs FP32 factor + FP64
+    GMRES to berr <= 1e-12."""
+    A = spm.convection_diffusion27(*grid, seed=seed)
+    x0, b = spm.rhs_from_x(A, 1, seed=seed + 1)
+    out = {"workload": f"convdiff27_{grid[0]}x{grid[1]}x{grid[2]}_getrf", "n": int(A.shape[0])}
+    for label, prec in (("fp64", None), ("fp32_gmres", spm.SpmFloat)):
+        S = spm.Spm(A, symmetric=False)
+        ip, dp = pastix.initParam()
+        ip[pastix.iparm.factorization] = pastix.factotype.LU
+        ip[pastix.iparm.ordering] = pastix.order.Personal
+        ip[pastix.iparm.refinement] = pastix.refine.GMRES
+        dp[pastix.dparm.epsilon_magn_ctrl] = np.ldexp(1.0, -52)
+        h = pastix.init(ip, dp)
+        if prec is not None:
+            pastix.set_amd_iparm(h, pastix.amd_iparm.factor_precision, prec)
+        pastix.set_amd_iparm(h, pastix.amd_iparm.resident_input, 1)
+        o = pastix.orderGrid(*grid)
+        h.order = o
+        pastix.subtask_order(h, S, o)
+        pastix.subtask_symbfact(h)
+        pastix.subtask_reordering(h)
+        pastix.subtask_blend(h)
+        pastix.task_numfact(h, S)
+        times = []
+        for _ in range(3):
+            pastix.refactorize(h)
+            times.append(float(h.dparm[pastix.dparm.fact_time]))
+        th = float(pastix.flops(h)[0])
+        x = b.copy(order="F")
+        r0 = time.perf_counter()
+        pastix.task_solve(h, S, x)
+        if prec is not None:
+            pastix.task_refine(h, S, b, x)
+        t_solve = time.perf_counter() - r0
+        tm = float(np.median(times))
+        out[label] = {"numfact_ms": tm * 1e3, "numfact_gflops": th / tm / 1e9,
+                      "solve_refine_ms_wall": t_solve * 1e3,
+                      "refine_iters": int(h.iparm[pastix.iparm.nbiter]) if prec is not None else 0,
+                      "berr": berr_inf(A, x[:, 0], b[:, 0]),
+                      "static_pivots": int(h.iparm[pastix.iparm.static_pivoting])}
+        pastix.finalize(h)
+    out["speedup_numfact_fp32_vs_fp64"] = out["fp64"]["numfact_ms"] / out["fp32_gmres"]["numfact_ms"]
+    return out
+
+
+def leg_batch(pastix, spm, nsys, seed=7, sample_alone=16):
+    """nsys SPD systems, 2D 5-point Laplacians on k x k grids (k uniform in
+    [16, 128]) plus a uniform(0, 1) diagonal shift, 1-8 RHS each: one
+    block-diagonal FP32 factor (graph replay) + batched FP64 CG, against plain
+    FP64 instances solved one after the other (timed on a sample)."""
+    import scipy.sparse as sps
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(16, 129, size=nsys)
+    mats = [spm.laplacian5_2d(int(k)) + rng.uniform(0.0, 1.0) * sps.eye(int(k) * int(k)) for k in ks]
+    nrhs = rng.integers(1, 9, size=nsys)
+    bs = [rng.uniform(-1.0, 1.0, size=(int(k) * int(k), int(r))) for k, r in zip(ks, nrhs)]
+    S = [spm.Spm(A, symmetric=True) for A in mats]
+    ip, dp = pastix.initParam()
+    ip[pastix.iparm.factorization] = pastix.factotype.LLT
+    ip[pastix.iparm.refinement] = pastix.refine.CG
+    h = pastix.init(ip, dp)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.factor_precision, spm.SpmFloat)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.resident_input, 1)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.graph, 1)
+    t0 = time.perf_counter()
+    pastix.batch_analyze(h, S, [pastix.orderGrid(int(k), int(k), 1) for k in ks])
+    t_an = time.perf_counter() - t0
+    pastix.batch_numfact(h, S)
+    ft = []
+    for _ in range(3):
+        pastix.refactorize(h)
+        ft.append(float(h.dparm[pastix.dparm.fact_time]))
+    r0 = time.perf_counter()
+    x0 = pastix.batch_solve(h, bs)
+    xs = pastix.batch_refine(h, bs, x0)
+    t_sr = time.perf_counter() - r0
+    worst = max(berr_inf(A, x[:, c], b[:, c]) for A, b, x in zip(mats, bs, xs) for c in range(b.shape[1]))
+    iters = int(h.iparm[pastix.iparm.nbiter])
+    pastix.finalize(h)
+    # plain FP64, one system at a time, on a sample
+    samp = list(range(min(sample_alone, nsys)))
+    a0 = time.perf_counter()
+    for i in samp:
+        ip1, dp1 = pastix.initParam()
+        ip1[pastix.iparm.factorization] = pastix.factotype.LLT
+        ip1[pastix.iparm.ordering] = pastix.order.Personal
+        h1 = pastix.init(ip1, dp1)
+        o = pastix.orderGrid(int(ks[i]), int(ks[i]), 1)
+        h1.order = o
+        pastix.subtask_order(h1, S[i], o)
+        pastix.subtask_symbfact(h1)
+        pastix.subtask_reordering(h1)
+        pastix.subtask_blend(h1)
+        pastix.task_numfact(h1, S[i])
+        xp = bs[i].copy(order="F")
+        pastix.task_solve(h1, S[i], xp)
+        pastix.finalize(h1)
+    t_alone = (time.perf_counter() - a0) / len(samp) * nsys
+    return {"workload": f"batch_{nsys}_laplace2d_k16-128", "systems": nsys, "n_total": int(sum(k * k for k in ks)),
+            "rhs_total": int(nrhs.sum()), "analysis_s": t_an, "numfact_ms": float(np.median(ft)) * 1e3,
+            "solve_refine_ms_wall": t_sr * 1e3, "refine_iters_max": iters, "berr_max": worst,
+            "berr_target_met": bool(worst <= 1e-12),
+            "plain_fp64_loop_s_wall_est": t_alone,
+            "plain_fp64_loop_sample": f"{len(samp)} systems analysed, factored and solved one at a time, scaled to {nsys}",
+            "speedup_vs_plain_loop": t_alone / ((float(np.median(ft)) + t_sr) + t_an)}
+
+
+def leg_serving(pastix, spm, grid, nreq, seed=11, rate=40.0, slots=2, max_rhs=64):
+    """Anisotropic 7-point Laplacian (1, 1, 0.01): Poisson request stream,
+    90 % solves (nrhs in {1, 4, 16, 64}), 10 % refactorisations A + shift*I
+    (shift uniform(0, 0.1)); FP32 slots + batched FP64 CG; latency p50/p99
+    and solves/s, next to the plain FP64 path solving the same requests one
+    at a time."""
+    A = spm.laplacian7(*grid, coef=(1.0, 1.0, 0.01))
+    n = A.shape[0]
+
+    def make(prec):
+        S = spm.Spm(A, symmetric=True)
+        ip, dp = pastix.initParam()
+        ip[pastix.iparm.factorization] = pastix.factotype.LLT
+        ip[pastix.iparm.ordering] = pastix.order.Personal
+        ip[pastix.iparm.refinement] = pastix.refine.CG
+        h = pastix.init(ip, dp)
+        if prec is not None:
+            pastix.set_amd_iparm(h, pastix.amd_iparm.factor_precision, prec)
+        pastix.set_amd_iparm(h, pastix.amd_iparm.resident_input, 1)
+        o = pastix.orderGrid(*grid)
+        h.order = o
+        pastix.subtask_order(h, S, o)
+        pastix.subtask_symbfact(h)
+        pastix.subtask_reordering(h)
+        pastix.subtask_blend(h)
+        pastix.task_numfact(h, S)
+        return h, S
+
+    hs = [make(spm.SpmFloat) for _ in range(slots)]
+    per_slot = pastix.device_bytes(hs[0][0])
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=nreq)
+    kinds = rng.uniform(size=nreq) < 0.9
+    nrhs = rng.choice([1, 4, 16, 64], size=nreq)
+    srv = pastix.Server([h for h, _ in hs], max_rhs=max_rhs)
+    tickets = []
+    w0 = time.perf_counter()
+    t_next = w0
+    for i in range(nreq):
+        t_next += gaps[i]
+        dt = t_next - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        if kinds[i]:
+            b = rng.uniform(-1.0, 1.0, size=(n, int(nrhs[i])))
+            tickets.append(("solve", srv.submit_solve(b), int(nrhs[i])))
+        else:
+            tickets.append(("refactor", srv.submit_refactor(float(rng.uniform(0.0, 0.1))), 0))
+    lat_s, lat_r, rhs_done = [], [], 0
+    for kind, t, r in tickets:
+        _, lat, _ = srv.wait(t)
+        if kind == "solve":
+            lat_s.append(lat)
+            rhs_done += r
+        else:
+            lat_r.append(lat)
+    wall = time.perf_counter() - w0
+    st = srv.stats()
+    srv.close()
+    # plain FP64 path: the same solves one at a time on one FP64 instance
+    hp, Sp = make(None)
+    nsamp = min(8, int(kinds.sum()))
+    p0 = time.perf_counter()
+    cnt = 0
+    for i in range(nreq):
+        if not kinds[i] or cnt >= nsamp:
+            continue
+        xb = rng.uniform(-1.0, 1.0, size=(n, int(nrhs[i]))).copy(order="F")
+        pastix.task_solve(hp, Sp, xb)
+        cnt += 1
+    t_plain = (time.perf_counter() - p0) / max(cnt, 1)
+    for h, _ in hs + [(hp, Sp)]:
+        pastix.finalize(h)
+    return {"workload": f"serving_laplacian7_aniso_{grid[0]}x{grid[1]}x{grid[2]}", "n": int(n), "requests": nreq,
+            "poisson_rate_per_s": rate, "slots": slots, "slot_device_gib": per_slot / 2 ** 30,
+            "max_rhs_per_batch": max_rhs, "solves": len(lat_s), "refactorizations": len(lat_r),
+            "solve_latency_p50_ms": float(np.percentile(lat_s, 50)) * 1e3 if lat_s else None,
+            "solve_latency_p99_ms": float(np.percentile(lat_s, 99)) * 1e3 if lat_s else None,
+            "refactor_latency_p50_ms": float(np.percentile(lat_r, 50)) * 1e3 if lat_r else None,
+            "solves_per_s": len(lat_s) / wall, "rhs_per_s": rhs_done / wall, "batches": st["batches"],
+            "plain_fp64_solve_ms_per_request_wall": t_plain * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +277,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="only the timed factorisations (for rocprofv3)")
+    ap.add_argument("--legs", default="getrf,batch,serving",
+                    help="extra workloads reported beside the headline: getrf, batch, serving (comma list or none)")
+    ap.add_argument("--getrf-grid", type=int, nargs=3, default=[64, 64, 64])
+    ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--serving-grid", type=int, nargs=3, default=[96, 96, 96])
+    ap.add_argument("--serving-requests", type=int, default=60)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only three device-to-device copies of 256 MiB (reference dispatch for HBM counters)")
     args = ap.parse_args()
@@ -212,6 +410,17 @@ def main():
                          f"(same stencil, smaller grid), {cpu_t:.1f} s of CPU work"}
         log(f"cpu baseline {cpu_gf:.2f} GFLOP/s in {cpu_t:.1f}s")
 
+    legs = {}
+    for leg in [x for x in args.legs.split(",") if x and x != "none"]:
+        t0 = time.perf_counter()
+        if leg == "getrf":
+            legs["getrf_fp32_gmres"] = leg_getrf(pastix, spm, tuple(args.getrf_grid))
+        elif leg == "batch":
+            legs["batch"] = leg_batch(pastix, spm, args.batch_systems)
+        elif leg == "serving":
+            legs["serving"] = leg_serving(pastix, spm, tuple(args.serving_grid), args.serving_requests)
+        log(f"leg {leg}: {time.perf_counter() - t0:.1f}s")
+
     out = {
         "metric": METRIC, "value": gflops, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": tmed * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -234,6 +443,7 @@ def main():
         "kernels": kern,
         "cpu_baseline": cpu,
     }
+    out.update(legs)
     print(json.dumps(out))
 
 

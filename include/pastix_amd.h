@@ -74,20 +74,22 @@ int pastix_amd_backward_error(pastix_data_t *pastix_data, const double *b,
                               const double *x, double *berr);
 
 /* Device time of the last numerical factorisation split by kernel family:
- * out[0]=total, [1]=diagonal, [2]=panel TRSM, [3]=update (seconds), and the
- * number of launches of each family out[4..6]; the split is filled only
- * when PXA_IPARM_PROFILE is set. */
+ * out[0]=total, [1]=panel steps (diagonal blocks, panel TRSM, trailing
+ * panel updates), [2]=0, [3]=Schur update (seconds), and the number of
+ * timed spans of each family out[4..6]; the split is filled only when
+ * PXA_IPARM_PROFILE is set. */
 int pastix_amd_fact_times(const pastix_data_t *pastix_data, double out[8]);
 
 /* Device time of the last numerical factorisation per kernel class, filled
  * when PXA_IPARM_PROFILE is set: out[2k] seconds and out[2k+1] timed spans
  * for k = 0 panel steps (diagonal blocks, panel TRSM and trailing panel
- * update of one level per span), 1 LU panel TRSM, 2 single-source update
- * tiles (k_update_direct), 3 multi-source product tiles (k_update_wgemm),
- * 4 multi-source ordered sums (k_update_gather), 5 single-pass multi-source
+ * update of one level per span), 1 single-source update tiles
+ * (k_update_direct), 2 multi-source product tiles (k_update_wgemm), 3
+ * multi-source ordered sums (k_update_gather), 4 single-pass multi-source
  * update (k_update_acc, used when a level's products exceed the workspace).
- * Every span of classes 2-5 is exactly one kernel launch. */
-int pastix_amd_kernel_times(const pastix_data_t *pastix_data, double out[12]);
+ * Every span of classes 1-4 is exactly one kernel launch (LU runs the L
+ * and U sides as separate launches). */
+int pastix_amd_kernel_times(const pastix_data_t *pastix_data, double out[10]);
 
 /* Schur-update flops by kernel, counting only the (p, q) pairs the
  * reference's per-block GEMMs compute: out[0] single-source tiles and out[1]

@@ -467,10 +467,10 @@ extern "C" int pastix_subtask_sopalin(pastix_data_t *d) {
     d->iparm[IPARM_STATIC_PIVOTING] = npiv;
     d->fact_times[0] = secs;
     d->fact_times[1] = d->engine->t_diag;
-    d->fact_times[2] = d->engine->t_trsm;
+    d->fact_times[2] = 0;
     d->fact_times[3] = d->engine->t_upd;
     d->fact_times[4] = d->engine->n_diag;
-    d->fact_times[5] = d->engine->n_trsm;
+    d->fact_times[5] = 0;
     d->fact_times[6] = d->engine->n_upd;
     if (neg && d->iparm[IPARM_FACTORIZATION] != PastixFactGETRF) {
         // The reference aborts on a negative pivot (kernels/core_zpotrfsp.c:84-89);
@@ -895,7 +895,7 @@ extern "C" int pastix_amd_backward_error(pastix_data_t *d, const double *b, cons
     *berr = den > 0 ? o[0] / den : o[0];
     return PASTIX_SUCCESS;
 }
-extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[12]) {
+extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[10]) {
     if (!d || !d->engine) return PASTIX_ERR_BADPARAMETER;
     for (int c = 0; c < pxa::Engine::KC_N; c++) {
         out[2 * c] = d->engine->kt[c];

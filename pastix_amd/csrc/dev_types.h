@@ -73,11 +73,6 @@ struct DiagItem {
     int32_t c, ib, b0, b1;
 };
 
-// Panel TRSM tile: off-diagonal rows [r0, r0+64) of cblk c (absolute panel rows).
-struct TrsmItem {
-    int32_t c, r0;
-};
-
 // Blocked panel step tile: rows [r0, r0+64) (and columns [c0, c0+64) for the
 // trailing update) of cblk c's panel.
 struct PanelItem {

@@ -91,7 +91,6 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     S_.browtab = upload_as<int32_t>(*this, L.browtab);
 
     lv_cblk_ptr_ = P.lv_cblk_ptr;
-    lv_trsm_ptr_ = P.lv_trsm_ptr;
     lv_upd_ptr_ = P.lv_upd_ptr;
     lv_upd_split_ = P.lv_upd_split;
     lv_fwd_ptr_ = P.lv_fwd_ptr;
@@ -111,7 +110,6 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     d_ydst_ = upload(*this, P.ydst);
     nyrows_ = P.nyrows;
     d_shift_ = upload(*this, P.shift);
-    d_trsm_ = upload(*this, P.trsm);
     d_upd_ = upload(*this, P.upd);
     d_ents_ = upload(*this, P.ents);
     d_fwd_ = upload(*this, P.fwd);
@@ -218,30 +216,29 @@ template <typename T> int Engine::factorize_t(double thr) {
     nev_ = 0;
     spans_.clear();
     for (size_t lv = 0; lv < nl; lv++) {
-        const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
         int pk = prof_begin();
-        if (facto_ == 2) {
-            launch_getrf_diag<T>(S_, d_lv_cblks_ + c0, nc, L, U, W1, W2, thr, d_stat_, stream_);
-            launch_winv_transpose<T>(S_, d_lv_cblks_ + c0, nc, W2, (T *)d_WB_, stream_);
-            prof_end(KC_PANEL, pk);
-            const int t0 = lv_trsm_ptr_[lv], nt = lv_trsm_ptr_[lv + 1] - t0;
-            pk = prof_begin();
-            launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, L, W2, stream_);
-            launch_trsm_panel<T>(S_, d_trsm_ + t0, nt, U, W1, stream_);
-            prof_end(KC_TRSM, pk);
-        } else {
-            // blocked panel: 64-column steps, each over every cblk of the level
-            for (int s = lv_step_ptr_[lv]; s < lv_step_ptr_[lv + 1]; s++) {
-                const int kb = s - lv_step_ptr_[lv];
-                const int b0 = step_cb_ptr_[s], nb = step_cb_ptr_[s + 1] - b0;
+        // blocked panel: 64-column steps, each over every cblk of the level
+        for (int s = lv_step_ptr_[lv]; s < lv_step_ptr_[lv + 1]; s++) {
+            const int kb = s - lv_step_ptr_[lv];
+            const int b0 = step_cb_ptr_[s], nb = step_cb_ptr_[s + 1] - b0;
+            const int p0 = step_trsm_ptr_[s], np = step_trsm_ptr_[s + 1] - p0;
+            const int q0 = step_upd_ptr_[s], nq = step_upd_ptr_[s + 1] - q0;
+            if (facto_ == 2) {
+                // rows below the sub-block: L side times inv(U_kk), U side
+                // (stored transposed) times inv(L_kk)^T; trailing updates
+                // L -= L U^T and U^T -= U^T L^T over the same tiles
+                launch_getrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, U, W1, W2, (T *)d_WB_, thr, d_stat_, stream_);
+                launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W2, stream_);
+                launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, U, W1, stream_);
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, U, stream_);
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, U, U, L, stream_);
+            } else {
                 launch_potrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, W1, (T *)d_WB_, thr, d_stat_, stream_);
-                const int p0 = step_trsm_ptr_[s], np = step_trsm_ptr_[s + 1] - p0;
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W1, stream_);
-                const int q0 = step_upd_ptr_[s], nq = step_upd_ptr_[s + 1] - q0;
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, stream_);
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, L, stream_);
             }
-            prof_end(KC_PANEL, pk);
         }
+        prof_end(KC_PANEL, pk);
         const int u0 = lv_upd_ptr_[lv], us = lv_upd_split_[lv], u1 = lv_upd_ptr_[lv + 1];
         const int w0 = lv_wt_ptr_[lv], nw = lv_wt_ptr_[lv + 1] - w0;
         const bool two_phase = u1 > us && lv_wsize_[lv] <= wcap_;
@@ -277,7 +274,15 @@ template <typename T> int Engine::factorize_t(double thr) {
         }
     }
     // full inverses of the wide diagonal blocks, needed only by the solve
-    if (facto_ != 2) launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, L, W1, (T *)d_WB_, stream_);
+    if (facto_ != 2) {
+        launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, L, W1, (T *)d_WB_, stream_);
+    } else {
+        // inv(L) from the unit lower L, inv(U^T) (and its transpose inv(U))
+        // from U^T kept in the U panel's square, then the reference layout
+        launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, L, W1, (T *)nullptr, stream_);
+        launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, U, W2, (T *)d_WB_, stream_);
+        launch_lu_square_layout<T>(S_, d_lv_cblks_, (int)cblknbr_, L, U, stream_);
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -437,8 +442,6 @@ int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) 
         }
         t_diag = kt[KC_PANEL];
         n_diag = kn[KC_PANEL];
-        t_trsm = kt[KC_TRSM];
-        n_trsm = kn[KC_TRSM];
         t_upd = kt[KC_DIRECT] + kt[KC_WGEMM] + kt[KC_GATHER] + kt[KC_ACC];
         n_upd = kn[KC_DIRECT] + kn[KC_WGEMM] + kn[KC_GATHER] + kn[KC_ACC];
     }

@@ -84,15 +84,15 @@ class Engine {
     // update classes time one launch per span; the panel class times the
     // panel steps of one level (diagonal blocks, panel TRSM, trailing panel
     // update) as one span.
-    enum KClass { KC_PANEL = 0, KC_TRSM, KC_DIRECT, KC_WGEMM, KC_GATHER, KC_ACC, KC_N };
+    enum KClass { KC_PANEL = 0, KC_DIRECT, KC_WGEMM, KC_GATHER, KC_ACC, KC_N };
     bool profile = false;
     // Replay numfact and device solves from captured HIP graphs (one graph
     // launch per factorisation / solve instead of one launch per kernel).
     bool use_graph = false;
-    double kt[KC_N] = {0, 0, 0, 0, 0, 0};
-    int kn[KC_N] = {0, 0, 0, 0, 0, 0};
-    double t_diag = 0, t_trsm = 0, t_upd = 0;
-    int n_diag = 0, n_trsm = 0, n_upd = 0;
+    double kt[KC_N] = {0, 0, 0, 0, 0};
+    int kn[KC_N] = {0, 0, 0, 0, 0};
+    double t_diag = 0, t_upd = 0;
+    int n_diag = 0, n_upd = 0;
 
   private:
     template <typename T> int factorize_t(double thr);
@@ -111,14 +111,13 @@ class Engine {
     int maxwidth_ = 0;
     DevSymbol S_{};
     // host copies of the level pointers
-    std::vector<int32_t> lv_cblk_ptr_, lv_trsm_ptr_, lv_upd_ptr_, lv_upd_split_, lv_fwd_ptr_, lv_bwd_ptr_;
+    std::vector<int32_t> lv_cblk_ptr_, lv_upd_ptr_, lv_upd_split_, lv_fwd_ptr_, lv_bwd_ptr_;
     // device plan
     int32_t *d_lv_cblks_ = nullptr, *d_bwd_itemptr_ = nullptr, *d_shift_ = nullptr;
     int64_t *d_yrow_ptr_ = nullptr;
     int32_t *d_ydst_ = nullptr;
     int64_t nyrows_ = 0;
     void *d_ybuf_ = nullptr, *d_WB_ = nullptr;
-    TrsmItem *d_trsm_ = nullptr;
     std::vector<int32_t> lv_step_ptr_, step_cb_ptr_, step_trsm_ptr_, step_upd_ptr_;
     int32_t *d_step_cblks_ = nullptr, *d_wide_ = nullptr;
     int n_wide_ = 0;

@@ -6,14 +6,6 @@
 
 namespace pxa {
 
-template <typename T>
-void launch_potrf_diag(const DevSymbol &S, const int32_t *cblks, int n, T *coef, T *winv, double thr, int *stat,
-                       hipStream_t st);
-template <typename T>
-void launch_getrf_diag(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, T *w1, T *w2, double thr,
-                       int *stat, hipStream_t st);
-template <typename T>
-void launch_trsm_panel(const DevSymbol &S, const TrsmItem *items, int n, T *coef, const T *winv, hipStream_t st);
 // direct: every item has exactly one entry (no LDS accumulation).
 template <typename T>
 void launch_update(const DevSymbol &S, int mode, bool direct, const UpdItem *items, int n, const UpdEntry *ents,
@@ -34,7 +26,15 @@ template <typename T>
 void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, const T *winv,
                        hipStream_t st);
 template <typename T>
-void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, hipStream_t st);
+void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *C, const T *X, const T *Y,
+                      hipStream_t st);
+// LU: diagonal 64-block with inverses; final layout of the square parts.
+template <typename T>
+void launch_getrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *lcoef, T *ucoef, T *w1, T *w2, T *wb,
+                    double thr, int *stat, hipStream_t st);
+template <typename T>
+void launch_lu_square_layout(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, hipStream_t st);
+// winvt may be null (no transposed copy).
 template <typename T>
 void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool transposed_factor, const T *coef,
                          T *winv, T *winvt, hipStream_t st);
@@ -55,8 +55,6 @@ template <typename T>
 void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const BwdItem *bitems, const T *z,
                      int64_t ldz, T *x, int64_t ldx, int nrhs, const T *part, const T *wb, int maxwidth,
                      hipStream_t st);
-template <typename T>
-void launch_winv_transpose(const DevSymbol &S, const int32_t *cblks, int n, const T *w, T *wt, hipStream_t st);
 
 // Sparse matrix-vector product on the permuted block CSC
 // (reference bcsc/bcsc_zspmv.c:55-202): y = alpha*A*x + beta*y.

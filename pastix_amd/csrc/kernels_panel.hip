@@ -29,6 +29,28 @@ namespace pxa {
 
 template <typename T> __device__ __forceinline__ T pabs(T v) { return v < (T)0 ? -v : v; }
 
+// Shared by the inverses: column j of inv(M) for the lower-triangular 64x64
+// M held in Ls (M(r, k) = Ls[r][k] when TRANS is false, Ls[k][r] when true;
+// diagonal 1 when UNIT), by the four lanes 4*jl+q of a wave, x[kk] =
+// inv(M)(4*kk + q, j).
+template <typename T, bool UNIT, bool TRANS>
+__device__ __forceinline__ void quad_lower_inverse(T (*Ls)[65], const T *dinv, int j, int q, T (&x)[16]) {
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) x[kk] = (T)0;
+#pragma unroll
+    for (int r = 0; r < 64; r++) {
+        T s = (T)0;
+#pragma unroll
+        for (int kk = 0; 4 * kk < r; kk++)
+            if (4 * kk + q < r) s += (TRANS ? Ls[4 * kk + q][r] : Ls[r][4 * kk + q]) * x[kk];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        const T dr = UNIT ? (T)1 : dinv[r];
+        const T xr = (r < j) ? (T)0 : ((r == j) ? dr : -s * dr);
+        if (q == (r & 3)) x[r >> 2] = xr;
+    }
+}
+
 // Cholesky and inverse of one 64-column diagonal sub-block.
 //
 // Thread (i, w) = (tid % 64, tid / 64) keeps the entries (i, j), j = w mod 4,
@@ -109,19 +131,7 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
     // column j of inv(L): lanes 4*jl + q of wave w, x[kk] = X(4*kk + q, j)
     const int q = tid & 3, j = 16 * w + ((tid >> 2) & 15);
     T x[16];
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) x[kk] = (T)0;
-#pragma unroll
-    for (int r = 0; r < 64; r++) {
-        T s = (T)0;
-#pragma unroll
-        for (int kk = 0; 4 * kk < r; kk++)
-            if (4 * kk + q < r) s += Ls[r][4 * kk + q] * x[kk];
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        const T xr = (r < j) ? (T)0 : ((r == j) ? dinv[r] : -s * dinv[r]);
-        if (q == (r & 3)) x[r >> 2] = xr;
-    }
+    quad_lower_inverse<T, false, false>(Ls, dinv, j, q, x);
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) Ls[4 * kk + q][j] = x[kk];
@@ -167,10 +177,13 @@ __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem
     }
 }
 
-// trailing panel update: P[r0+i, c0+j] -= X[r0+i, kb] . X[c0+j, kb]
+// trailing panel update: C[r0+i, c0+j] -= X[r0+i, kb] . Y[c0+j, kb] (C, X, Y
+// panels of the same cblk: all L for LL^t; L, L, U and U, U, L for the two
+// sides of LU)
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
-                                                   int kb, T *__restrict__ coef) {
+                                                   int kb, T *__restrict__ ccoef, const T *__restrict__ xcoef,
+                                                   const T *__restrict__ ycoef) {
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -181,13 +194,14 @@ __global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem 
     const int stride = S.cblk_stride[c];
     const int64_t ld = stride;
     const int k0 = kb * 64, nb = min(64, K - k0);
-    T *P = coef + S.cblk_coef[c];
+    T *P = ccoef + S.cblk_coef[c];
+    const T *X = xcoef + S.cblk_coef[c], *Y = ycoef + S.cblk_coef[c];
     const int mr = min(64, stride - r0), mc = min(64, K - c0);
     if (wr < mr && wc < mc) {
         Quad<T> q;
         q.zero();
-        View<T> A{P + (r0 + wr) + (int64_t)k0 * ld, 1, ld};
-        View<T> B{P + (c0 + wc) + (int64_t)k0 * ld, 1, ld};
+        View<T> A{X + (r0 + wr) + (int64_t)k0 * ld, 1, ld};
+        View<T> B{Y + (c0 + wc) + (int64_t)k0 * ld, 1, ld};
         quad_gemm_nt(q, A, mr - wr, B, mc - wc, nb, lane);
         QFOR(if (qrow < mr - wr && qcol < mc - wc) P[(r0 + wr + qrow) + (int64_t)(c0 + wc + qcol) * ld] -= q.c[m_][n_][r_];)
     }
@@ -211,7 +225,7 @@ __global__ __launch_bounds__(256) void k_winv_offdiag(DevSymbol S, const int32_t
     (void)lstride;
     (void)lrs;
     T *W = winv + S.cblk_winv[c];
-    T *WT = winvt + S.cblk_winv[c];
+    T *WT = winvt ? winvt + S.cblk_winv[c] : nullptr;
     for (int jb0 = 0; jb0 < K; jb0 += 64) {
         const int nj = min(64, K - jb0);
         for (int ib0 = jb0 + 64; ib0 < K; ib0 += 64) {
@@ -235,11 +249,134 @@ __global__ __launch_bounds__(256) void k_winv_offdiag(DevSymbol S, const int32_t
                     const int gi = ib0 + wr + qrow, gj = jb0 + wc + qcol;
                     const T v = -q2.c[m_][n_][r_];
                     W[gi + (int64_t)gj * K] = v;
-                    WT[gj + (int64_t)gi * K] = v;
+                    if (winvt) WT[gj + (int64_t)gi * K] = v;
                 })
             }
             __syncthreads();
         }
+    }
+}
+
+// LU without pivoting of one 64-column diagonal sub-block (reference
+// kernels/core_zgetrfsp.c:63-106, getf2 with static pivoting +-thr).  During
+// the factorisation the square part of a cblk keeps its lower triangle
+// (diagonal included) in the L panel and its strict upper triangle,
+// transposed, in the U panel.  Thread (i, w) keeps the entries (i, j),
+// j = w mod 4, of row i of the full sub-block; step k publishes column k and
+// row k through LDS.  Outputs: L (unit, strict lower) and the diagonal of U
+// in the L panel, the strict upper part of U transposed in the U panel,
+// W1 = inv(L) (lower, unit diagonal), W2 = inv(U)^T (lower) and WB = inv(U)
+// (upper), with explicit zeros in the other triangles of the sub-block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
+                                                 T *__restrict__ lcoef, T *__restrict__ ucoef, T *__restrict__ w1,
+                                                 T *__restrict__ w2, T *__restrict__ wb, double thr, int *stat) {
+    __shared__ T col[2][64], rowb[2][64];
+    __shared__ T Ls[64][65];
+    __shared__ T dinv[64];
+    const int tid = threadIdx.x, i = tid & 63, w = tid >> 6;
+    const int c = cblks[blockIdx.x];
+    const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+    const int k0 = kb * 64;
+    const int nb = min(64, K - k0);
+    const int64_t ld = S.cblk_stride[c];
+    T *PL = lcoef + S.cblk_coef[c] + k0 + (int64_t)k0 * ld;
+    T *PU = ucoef + S.cblk_coef[c] + k0 + (int64_t)k0 * ld;
+    T a[16];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+        const int j = 4 * jj + w;
+        T v = (i == j) ? (T)1 : (T)0;
+        if (i < nb && j < nb) v = (j <= i) ? PL[i + (int64_t)j * ld] : PU[j + (int64_t)i * ld];
+        a[jj] = v;
+    }
+    const T crit = (T)thr;
+    int npiv = 0;
+#pragma unroll
+    for (int k = 0; k < 64; k++) {
+        T *cb = col[k & 1], *rb = rowb[k & 1];
+        if (w == (k & 3)) cb[i] = a[k >> 2];
+        if (i == k) {
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) rb[4 * jj + w] = a[jj];
+        }
+        __syncthreads();
+        T d = cb[k];
+        if (k < nb && pabs(d) < crit) {
+            d = (d < (T)0) ? -crit : crit;
+            npiv++;
+        }
+        const T m = cb[i] / d;
+        if (i > k) {
+#pragma unroll
+            for (int jj = k >> 2; jj < 16; jj++) {
+                const int j = 4 * jj + w;
+                if (j > k) a[jj] -= m * rb[j];
+            }
+        }
+        if (w == (k & 3)) a[k >> 2] = (i == k) ? d : ((i > k) ? m : a[k >> 2]);
+    }
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+        const int j = 4 * jj + w;
+        Ls[i][j] = a[jj];
+        if (i < nb && j < nb) {
+            if (j <= i) PL[i + (int64_t)j * ld] = a[jj];
+            else PU[j + (int64_t)i * ld] = a[jj];
+        }
+    }
+    if (tid == 0 && npiv) atomicAdd(&stat[0], npiv);
+    __syncthreads();
+    if (tid < 64) dinv[tid] = (T)1 / Ls[tid][tid];
+    __syncthreads();
+    const int q = tid & 3, j = 16 * w + ((tid >> 2) & 15);
+    T x1[16], x2[16];
+    quad_lower_inverse<T, true, false>(Ls, dinv, j, q, x1);   // inv(L), L unit lower
+    quad_lower_inverse<T, false, true>(Ls, dinv, j, q, x2);   // inv(U^T)
+    T *W1 = w1 + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
+    T *W2 = w2 + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
+    T *WB = wb + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) Ls[4 * kk + q][j] = x1[kk];
+    __syncthreads();
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int lo = e & 63, hi = e >> 6;
+        if (lo < nb && hi < nb) W1[lo + (int64_t)hi * K] = Ls[lo][hi];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) Ls[4 * kk + q][j] = x2[kk];
+    __syncthreads();
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int lo = e & 63, hi = e >> 6;
+        if (lo < nb && hi < nb) {
+            W2[lo + (int64_t)hi * K] = Ls[lo][hi];
+            WB[lo + (int64_t)hi * K] = Ls[hi][lo];
+        }
+    }
+}
+
+// Final layout of the LU diagonal blocks (reference core_zgetrfsp.c:240-251):
+// the L panel's square receives the whole LU (strict upper part from the U
+// panel, where it was kept transposed) and the U panel's square its
+// transpose (core_zgetmo).  One workgroup per cblk.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_square_layout(DevSymbol S, const int32_t *__restrict__ cblks,
+                                                          T *__restrict__ lcoef, T *__restrict__ ucoef) {
+    const int c = cblks[blockIdx.x];
+    const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+    const int64_t ld = S.cblk_stride[c];
+    T *PL = lcoef + S.cblk_coef[c];
+    T *PU = ucoef + S.cblk_coef[c];
+    for (int64_t e = threadIdx.x; e < (int64_t)K * K; e += 256) {
+        const int r = (int)(e % K), cc = (int)(e / K);
+        if (r < cc) PL[r + (int64_t)cc * ld] = PU[cc + (int64_t)r * ld];
+    }
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < (int64_t)K * K; e += 256) {
+        const int r = (int)(e % K), cc = (int)(e / K);
+        PU[r + (int64_t)cc * ld] = PL[cc + (int64_t)r * ld];
     }
 }
 
@@ -253,15 +390,27 @@ void launch_potrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *
     hipLaunchKernelGGL(k_potrf64<T>, dim3(n), dim3(256), 0, st, S, cblks, kb, coef, winv, winvt, thr, stat);
 }
 template <typename T>
+void launch_getrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *lcoef, T *ucoef, T *w1, T *w2, T *wb,
+                    double thr, int *stat, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_getrf64<T>, dim3(n), dim3(256), 0, st, S, cblks, kb, lcoef, ucoef, w1, w2, wb, thr, stat);
+}
+template <typename T>
+void launch_lu_square_layout(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_lu_square_layout<T>, dim3(n), dim3(256), 0, st, S, cblks, lcoef, ucoef);
+}
+template <typename T>
 void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, const T *winv,
                        hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_panel_trsm<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, coef, winv);
 }
 template <typename T>
-void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, hipStream_t st) {
+void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *C, const T *X, const T *Y,
+                      hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, coef);
+    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, C, X, Y);
 }
 template <typename T>
 void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool transposed_factor, const T *coef,
@@ -272,10 +421,14 @@ void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool t
 }
 
 #define INST(T)                                                                                                    \
+    template void launch_getrf64<T>(const DevSymbol &, const int32_t *, int, int, T *, T *, T *, T *, T *, double, \
+                                    int *, hipStream_t);                                                           \
+    template void launch_lu_square_layout<T>(const DevSymbol &, const int32_t *, int, T *, T *, hipStream_t);      \
     template void launch_potrf64<T>(const DevSymbol &, const int32_t *, int, int, T *, T *, T *, double, int *,    \
                                     hipStream_t);                                                                  \
     template void launch_panel_trsm<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, hipStream_t); \
-    template void launch_panel_upd<T>(const DevSymbol &, const PanelItem *, int, int, T *, hipStream_t);          \
+    template void launch_panel_upd<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, const T *,  \
+                                      hipStream_t);                                                              \
     template void launch_winv_offdiag<T>(const DevSymbol &, const int32_t *, int, bool, const T *, T *, T *,       \
                                          hipStream_t);
 INST(double)

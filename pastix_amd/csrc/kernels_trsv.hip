@@ -225,31 +225,6 @@ __global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagIte
     wave_sum_store<T, NR, NW>(acc, red, r, q, i < Kc, nrhs, x + fc + i, ldx, (const T *)nullptr, 0);
 }
 
-// Transposed copy of the diagonal inverses of a level: WT[c] = W[c]^T.
-template <typename T>
-__global__ __launch_bounds__(256) void k_winv_transpose(DevSymbol S, const int32_t *__restrict__ cblks,
-                                                        const T *__restrict__ w, T *__restrict__ wt) {
-    __shared__ T tile[32][33];
-    const int c = cblks[blockIdx.x];
-    const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
-    const T *W = w + S.cblk_winv[c];
-    T *WT = wt + S.cblk_winv[c];
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    for (int bi = 0; bi < K; bi += 32)
-        for (int bj = 0; bj < K; bj += 32) {
-            for (int r = ty; r < 32; r += 8) {
-                const int i = bi + tx, j = bj + r;
-                tile[r][tx] = (i < K && j < K) ? W[i + (int64_t)j * K] : (T)0;
-            }
-            __syncthreads();
-            for (int r = ty; r < 32; r += 8) {
-                const int i = bj + tx, j = bi + r;  // WT(i, j) = W(j, i)
-                if (i < K && j < K) WT[i + (int64_t)j * K] = tile[tx][r];
-            }
-            __syncthreads();
-        }
-}
-
 // ------------------------------- launchers ---------------------------------
 template <typename T>
 void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int64_t *yrow_ptr, const T *ybuf,
@@ -299,11 +274,6 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
         hipLaunchKernelGGL((k_bwd_diag<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, bitems, z, ldz, x,
                            ldx, nrhs, part, wb);
 }
-template <typename T>
-void launch_winv_transpose(const DevSymbol &S, const int32_t *cblks, int n, const T *w, T *wt, hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_winv_transpose<T>, dim3(n), dim3(256), 0, st, S, cblks, w, wt);
-}
 
 #define INST(T)                                                                                                   \
     template void launch_fwd_diag<T>(const DevSymbol &, const DiagItem *, int, const int64_t *, const T *,       \
@@ -314,8 +284,7 @@ void launch_winv_transpose(const DevSymbol &S, const int32_t *cblks, int n, cons
     template void launch_bwd_prod<T>(const DevSymbol &, const BwdItem *, int, const T *, const T *, int64_t, int, \
                                      T *, int, hipStream_t);                                                     \
     template void launch_bwd_diag<T>(const DevSymbol &, const DiagItem *, int, const BwdItem *, const T *,        \
-                                     int64_t, T *, int64_t, int, const T *, const T *, int, hipStream_t);        \
-    template void launch_winv_transpose<T>(const DevSymbol &, const int32_t *, int, const T *, T *, hipStream_t);
+                                     int64_t, T *, int64_t, int, const T *, const T *, int, hipStream_t);
 INST(double)
 INST(float)
 

@@ -75,7 +75,6 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     auto width = [&](int64_t c) { return s.cblk_fcol[c + 1] - s.cblk_fcol[c]; };
     auto rows = [&](int64_t b) { return s.blok_lrow[b] - s.blok_frow[b] + 1; };
 
-    P.lv_trsm_ptr.assign(nl + 1, 0);
     P.lv_upd_ptr.assign(nl + 1, 0);
     P.lv_upd_split.assign(nl, 0);
     P.lv_fwd_ptr.assign(nl + 1, 0);
@@ -135,13 +134,6 @@ Plan build_plan(const Symbol &s, const Layout &L) {
             P.step_upd_ptr.push_back((int32_t)P.pupd.size());
         }
         P.lv_step_ptr[lv + 1] = P.lv_step_ptr[lv] + (int32_t)nsteps;
-        // TRSM tiles
-        for (int32_t k = l0; k < l1; k++) {
-            int32_t c = P.lv_cblks[k];
-            int64_t K = width(c), st = L.cblk_stride[c];
-            for (int64_t r0 = K; r0 < st; r0 += 64) P.trsm.push_back(TrsmItem{c, (int32_t)r0});
-        }
-        P.lv_trsm_ptr[lv + 1] = (int32_t)P.trsm.size();
 
         // update tiles from the sources of this level
         ke.clear();

@@ -13,9 +13,6 @@ struct Plan {
     int32_t nlevels = 0;
     // cblks of each level (increasing cblk index inside a level)
     std::vector<int32_t> lv_cblk_ptr, lv_cblks;
-    // panel TRSM tiles
-    std::vector<int32_t> lv_trsm_ptr;
-    std::vector<TrsmItem> trsm;
     // blocked panel steps: level L owns steps [lv_step_ptr[L], lv_step_ptr[L+1]),
     // step s is column block kb = s - lv_step_ptr[L] of the level's cblks
     std::vector<int32_t> lv_step_ptr;

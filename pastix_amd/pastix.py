@@ -352,12 +352,12 @@ def fact_times(pastix_data):
     return out
 
 
-KERNEL_CLASSES = ("panel", "lu_trsm", "update_direct", "update_wgemm", "update_gather", "update_acc")
+KERNEL_CLASSES = ("panel", "update_direct", "update_wgemm", "update_gather", "update_acc")
 
 
 def kernel_times(pastix_data):
     """{class: (seconds, timed spans)} of the last numfact (profiling on)."""
-    out = np.zeros(12)
+    out = np.zeros(10)
     _check("pastix_amd_kernel_times", lib.pastix_amd_kernel_times(pastix_data.ptr, out))
     return {k: (float(out[2 * i]), int(out[2 * i + 1])) for i, k in enumerate(KERNEL_CLASSES)}
 
