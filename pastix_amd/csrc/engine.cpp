@@ -217,6 +217,10 @@ template <typename T> int Engine::factorize_t(double thr) {
     spans_.clear();
     for (size_t lv = 0; lv < nl; lv++) {
         int pk = prof_begin();
+        if (facto_ == 2) {
+            const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
+            launch_lu_square_merge<T>(S_, d_lv_cblks_ + c0, nc, L, U, stream_);
+        }
         // blocked panel: 64-column steps, each over every cblk of the level
         for (int s = lv_step_ptr_[lv]; s < lv_step_ptr_[lv + 1]; s++) {
             const int kb = s - lv_step_ptr_[lv];
@@ -230,12 +234,12 @@ template <typename T> int Engine::factorize_t(double thr) {
                 launch_getrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, U, W1, W2, (T *)d_WB_, thr, d_stat_, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W2, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, U, W1, stream_);
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, U, stream_);
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, U, U, L, stream_);
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, U, false, stream_);
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, U, U, L, true, stream_);
             } else {
                 launch_potrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, W1, (T *)d_WB_, thr, d_stat_, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W1, stream_);
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, L, stream_);
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, L, false, stream_);
             }
         }
         prof_end(KC_PANEL, pk);

@@ -27,11 +27,13 @@ void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb
                        hipStream_t st);
 template <typename T>
 void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *C, const T *X, const T *Y,
-                      hipStream_t st);
+                      bool strict, hipStream_t st);
 // LU: diagonal 64-block with inverses; final layout of the square parts.
 template <typename T>
 void launch_getrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *lcoef, T *ucoef, T *w1, T *w2, T *wb,
                     double thr, int *stat, hipStream_t st);
+template <typename T>
+void launch_lu_square_merge(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, hipStream_t st);
 template <typename T>
 void launch_lu_square_layout(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, hipStream_t st);
 // winvt may be null (no transposed copy).

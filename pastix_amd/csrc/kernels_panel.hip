@@ -180,10 +180,13 @@ __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem
 // trailing panel update: C[r0+i, c0+j] -= X[r0+i, kb] . Y[c0+j, kb] (C, X, Y
 // panels of the same cblk: all L for LL^t; L, L, U and U, U, L for the two
 // sides of LU)
+// On the tiles that hold the diagonal (r0 == c0) only r >= c (strict = 0)
+// or r > c (strict = 1) is written: the L side of LU keeps the lower part and
+// the diagonal, the U side (transposed) the strict upper part.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
                                                    int kb, T *__restrict__ ccoef, const T *__restrict__ xcoef,
-                                                   const T *__restrict__ ycoef) {
+                                                   const T *__restrict__ ycoef, int strict) {
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -203,7 +206,9 @@ __global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem 
         View<T> A{X + (r0 + wr) + (int64_t)k0 * ld, 1, ld};
         View<T> B{Y + (c0 + wc) + (int64_t)k0 * ld, 1, ld};
         quad_gemm_nt(q, A, mr - wr, B, mc - wc, nb, lane);
-        QFOR(if (qrow < mr - wr && qcol < mc - wc) P[(r0 + wr + qrow) + (int64_t)(c0 + wc + qcol) * ld] -= q.c[m_][n_][r_];)
+        const bool diag = r0 == c0;
+        QFOR(if (qrow < mr - wr && qcol < mc - wc && (!diag || wr + qrow > wc + qcol - (strict ? 0 : 1)))
+                 P[(r0 + wr + qrow) + (int64_t)(c0 + wc + qcol) * ld] -= q.c[m_][n_][r_];)
     }
 }
 
@@ -357,6 +362,28 @@ __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__r
     }
 }
 
+// Before the LU panel of a cblk: the square part's upper triangle arrives
+// split between the L panel (same-block Schur contributions, written above
+// the diagonal as the reference does) and the U panel (transposed): the
+// reference adds them (core_zgeadd, kernels/core_zgetrfsp.c:240-242); here
+// the sum goes to the U panel and the L panel keeps the lower triangle only.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_square_merge(DevSymbol S, const int32_t *__restrict__ cblks,
+                                                         T *__restrict__ lcoef, T *__restrict__ ucoef) {
+    const int c = cblks[blockIdx.x];
+    const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+    const int64_t ld = S.cblk_stride[c];
+    T *PL = lcoef + S.cblk_coef[c];
+    T *PU = ucoef + S.cblk_coef[c];
+    for (int64_t e = threadIdx.x; e < (int64_t)K * K; e += 256) {
+        const int r = (int)(e % K), cc = (int)(e / K);
+        if (r < cc) {
+            PU[cc + (int64_t)r * ld] += PL[r + (int64_t)cc * ld];
+            PL[r + (int64_t)cc * ld] = (T)0;
+        }
+    }
+}
+
 // Final layout of the LU diagonal blocks (reference core_zgetrfsp.c:240-251):
 // the L panel's square receives the whole LU (strict upper part from the U
 // panel, where it was kept transposed) and the U panel's square its
@@ -396,6 +423,11 @@ void launch_getrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *
     hipLaunchKernelGGL(k_getrf64<T>, dim3(n), dim3(256), 0, st, S, cblks, kb, lcoef, ucoef, w1, w2, wb, thr, stat);
 }
 template <typename T>
+void launch_lu_square_merge(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_lu_square_merge<T>, dim3(n), dim3(256), 0, st, S, cblks, lcoef, ucoef);
+}
+template <typename T>
 void launch_lu_square_layout(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_lu_square_layout<T>, dim3(n), dim3(256), 0, st, S, cblks, lcoef, ucoef);
@@ -408,9 +440,9 @@ void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb
 }
 template <typename T>
 void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *C, const T *X, const T *Y,
-                      hipStream_t st) {
+                      bool strict, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, C, X, Y);
+    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, C, X, Y, strict ? 1 : 0);
 }
 template <typename T>
 void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool transposed_factor, const T *coef,
@@ -424,11 +456,12 @@ void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool t
     template void launch_getrf64<T>(const DevSymbol &, const int32_t *, int, int, T *, T *, T *, T *, T *, double, \
                                     int *, hipStream_t);                                                           \
     template void launch_lu_square_layout<T>(const DevSymbol &, const int32_t *, int, T *, T *, hipStream_t);      \
+    template void launch_lu_square_merge<T>(const DevSymbol &, const int32_t *, int, T *, T *, hipStream_t);       \
     template void launch_potrf64<T>(const DevSymbol &, const int32_t *, int, int, T *, T *, T *, double, int *,    \
                                     hipStream_t);                                                                  \
     template void launch_panel_trsm<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, hipStream_t); \
     template void launch_panel_upd<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, const T *,  \
-                                      hipStream_t);                                                              \
+                                      bool, hipStream_t);                                                        \
     template void launch_winv_offdiag<T>(const DevSymbol &, const int32_t *, int, bool, const T *, T *, T *,       \
                                          hipStream_t);
 INST(double)
