@@ -51,15 +51,51 @@ __device__ __forceinline__ void quad_lower_inverse(T (*Ls)[65], const T *dinv, i
     }
 }
 
+// The same inverse by blocks of 4 rows: per block, the four lanes of a
+// column form the partial dot products of the 4 rows with the known part of
+// x, combine them with lane shuffles, and solve the 4x4 diagonal block
+// redundantly; 16 rounds instead of 64.
+template <typename T, bool UNIT, bool TRANS>
+__device__ __forceinline__ void quad_lower_inverse4(T (*Ls)[65], const T *dinv, int j, int q, T (&x)[16]) {
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) x[kk] = (T)0;
+    auto M = [&](int r, int k) -> T { return TRANS ? Ls[k][r] : Ls[r][k]; };
+#pragma unroll
+    for (int rb = 0; rb < 16; rb++) {
+        T s[4] = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+        for (int kk = 0; kk < rb; kk++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) s[t] += M(4 * rb + t, 4 * kk + q) * x[kk];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            s[t] += __shfl_xor(s[t], 1);
+            s[t] += __shfl_xor(s[t], 2);
+        }
+        T y[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int r = 4 * rb + t;
+            T acc = s[t];
+#pragma unroll
+            for (int u = 0; u < t; u++) acc += M(r, 4 * rb + u) * y[u];
+            const T dr = UNIT ? (T)1 : dinv[r];
+            y[t] = (r < j) ? (T)0 : ((r == j) ? dr : -acc * dr);
+        }
+        x[rb] = (q == 0) ? y[0] : (q == 1) ? y[1] : (q == 2) ? y[2] : y[3];
+    }
+}
+
 // Cholesky and inverse of one 64-column diagonal sub-block.
 //
 // Thread (i, w) = (tid % 64, tid / 64) keeps the entries (i, j), j = w mod 4,
-// of row i in registers.  Step k publishes column k through LDS (two buffers
-// by parity, so one barrier per step orders every write after the reads of
-// the step before), replaces a pivot below the threshold (reference
-// kernels/core_zpotrfsp.c:78-81) and applies the rank-1 update in registers.
-// Rows beyond the block width are padded with the identity, which the
-// updates leave untouched.
+// of row i in registers.  Columns are taken 4 at a time: each wave publishes
+// one of them through LDS (two buffers by parity), every thread factors the
+// 4x4 diagonal block, replacing pivots below the threshold (reference
+// kernels/core_zpotrfsp.c:78-81), and forms its row's 4 multipliers, which
+// are published for the rank-4 update of the remaining columns.  Rows beyond
+// the block width are padded with the identity, which the updates leave
+// untouched.
 //
 // The inverse X = inv(L) is formed by columns: four lanes of one wave share
 // column j and split the forward substitution L x = e_j by k mod 4; their
@@ -70,7 +106,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
                                                  T *__restrict__ coef, T *__restrict__ winv, T *__restrict__ winvt,
                                                  double thr, int *stat) {
-    __shared__ T col[2][64];
+    __shared__ T colb[2][4][64];
+    __shared__ T lb[4][64];
     __shared__ T Ls[64][65];
     __shared__ T dinv[64];
     const int tid = threadIdx.x, i = tid & 63, w = tid >> 6;
@@ -90,13 +127,12 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
     }
     const T crit = (T)thr;
     int npiv = 0, neg = 0;
-#pragma unroll
-    for (int k = 0; k < 64; k++) {
-        T *cb = col[k & 1];
-        if (w == (k & 3)) cb[i] = a[k >> 2];
-        __syncthreads();
-        T d = cb[k];
-        if (k < nb) {
+    // blocks of 4 columns k..k+3, one column per wave: publish, factor the
+    // 4x4 diagonal block redundantly, form the row's multipliers, publish
+    // them, rank-4 update of the row's remaining columns (two barriers per
+    // block)
+    auto piv = [&](T d, int col) -> T {
+        if (col < nb) {
             if (pabs(d) < crit) {
                 d = crit;
                 npiv++;
@@ -106,14 +142,53 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
                 d = -d;
             }
         }
-        const T sd = sqrt(d);
-        const T m = cb[i] / d;
+        return d;
+    };
 #pragma unroll
-        for (int jj = k >> 2; jj < 16; jj++) {
-            const int j = 4 * jj + w;
-            if (j > k) a[jj] -= m * cb[j];
+    for (int b4 = 0; b4 < 16; b4++) {
+        const int k = 4 * b4;
+        T(*cb)[64] = colb[b4 & 1];
+        cb[w][i] = a[b4];
+        __syncthreads();
+        // 4x4 Cholesky of the diagonal block (same in every thread)
+        const T d0 = piv(cb[0][k], k);
+        const T L00 = sqrt(d0), i00 = (T)1 / L00;
+        const T L10 = cb[0][k + 1] * i00, L20 = cb[0][k + 2] * i00, L30 = cb[0][k + 3] * i00;
+        const T d1 = piv(cb[1][k + 1] - L10 * L10, k + 1);
+        const T L11 = sqrt(d1), i11 = (T)1 / L11;
+        const T L21 = (cb[1][k + 2] - L20 * L10) * i11, L31 = (cb[1][k + 3] - L30 * L10) * i11;
+        const T d2 = piv(cb[2][k + 2] - L20 * L20 - L21 * L21, k + 2);
+        const T L22 = sqrt(d2), i22 = (T)1 / L22;
+        const T L32 = (cb[2][k + 3] - L30 * L20 - L31 * L21) * i22;
+        const T d3 = piv(cb[3][k + 3] - L30 * L30 - L31 * L31 - L32 * L32, k + 3);
+        const T L33 = sqrt(d3), i33 = (T)1 / L33;
+        // multipliers of row i
+        T l0, l1, l2, l3;
+        if (i > k + 3) {
+            l0 = cb[0][i] * i00;
+            l1 = (cb[1][i] - l0 * L10) * i11;
+            l2 = (cb[2][i] - l0 * L20 - l1 * L21) * i22;
+            l3 = (cb[3][i] - l0 * L30 - l1 * L31 - l2 * L32) * i33;
+        } else {
+            const int t = i - k;  // rows of the block (t < 0: rows above)
+            l0 = (t == 0) ? L00 : (t == 1) ? L10 : (t == 2) ? L20 : L30;
+            l1 = (t == 1) ? L11 : (t == 2) ? L21 : L31;
+            l2 = (t == 2) ? L22 : L32;
+            l3 = L33;
         }
-        if (w == (k & 3)) a[k >> 2] = (i == k) ? sd : a[k >> 2] / sd;
+        lb[0][i] = l0;
+        lb[1][i] = l1;
+        lb[2][i] = l2;
+        lb[3][i] = l3;
+        if (i >= k + w) a[b4] = (w == 0) ? l0 : (w == 1) ? l1 : (w == 2) ? l2 : l3;
+        __syncthreads();
+        if (i > k + 3) {
+#pragma unroll
+            for (int jj = b4 + 1; jj < 16; jj++) {
+                const int j = 4 * jj + w;
+                a[jj] -= l0 * lb[0][j] + l1 * lb[1][j] + l2 * lb[2][j] + l3 * lb[3][j];
+            }
+        }
     }
 #pragma unroll
     for (int jj = 0; jj < 16; jj++) {
@@ -131,7 +206,7 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
     // column j of inv(L): lanes 4*jl + q of wave w, x[kk] = X(4*kk + q, j)
     const int q = tid & 3, j = 16 * w + ((tid >> 2) & 15);
     T x[16];
-    quad_lower_inverse<T, false, false>(Ls, dinv, j, q, x);
+    quad_lower_inverse4<T, false, false>(Ls, dinv, j, q, x);
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) Ls[4 * kk + q][j] = x[kk];
