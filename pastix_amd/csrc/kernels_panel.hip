@@ -342,8 +342,8 @@ __global__ __launch_bounds__(256) void k_winv_offdiag(DevSymbol S, const int32_t
 // the factorisation the square part of a cblk keeps its lower triangle
 // (diagonal included) in the L panel and its strict upper triangle,
 // transposed, in the U panel.  Thread (i, w) keeps the entries (i, j),
-// j = w mod 4, of row i of the full sub-block; step k publishes column k and
-// row k through LDS.  Outputs: L (unit, strict lower) and the diagonal of U
+// j = w mod 4, of row i of the full sub-block; the elimination runs by
+// blocks of four columns with one barrier each.  Outputs: L (unit, strict lower) and the diagonal of U
 // in the L panel, the strict upper part of U transposed in the U panel,
 // W1 = inv(L) (lower, unit diagonal), W2 = inv(U)^T (lower) and WB = inv(U)
 // (upper), with explicit zeros in the other triangles of the sub-block.
@@ -351,7 +351,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
                                                  T *__restrict__ lcoef, T *__restrict__ ucoef, T *__restrict__ w1,
                                                  T *__restrict__ w2, T *__restrict__ wb, double thr, int *stat) {
-    __shared__ T col[2][64], rowb[2][64];
+    __shared__ T colb[2][4][64], rowb[2][4][64];
     __shared__ T Ls[64][65];
     __shared__ T dinv[64];
     const int tid = threadIdx.x, i = tid & 63, w = tid >> 6;
@@ -372,29 +372,73 @@ __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__r
     }
     const T crit = (T)thr;
     int npiv = 0;
-#pragma unroll
-    for (int k = 0; k < 64; k++) {
-        T *cb = col[k & 1], *rb = rowb[k & 1];
-        if (w == (k & 3)) cb[i] = a[k >> 2];
-        if (i == k) {
-#pragma unroll
-            for (int jj = 0; jj < 16; jj++) rb[4 * jj + w] = a[jj];
-        }
-        __syncthreads();
-        T d = cb[k];
-        if (k < nb && pabs(d) < crit) {
+    auto piv = [&](T d, int col) -> T {
+        if (col < nb && pabs(d) < crit) {
             d = (d < (T)0) ? -crit : crit;
             npiv++;
         }
-        const T m = cb[i] / d;
-        if (i > k) {
+        return d;
+    };
+    // blocks of 4 columns k..k+3: publish the block's 4 columns and 4 rows,
+    // eliminate the 4x4 diagonal block redundantly in every thread, then row
+    // i forms its 4 multipliers and column j its 4 rows of U from the
+    // published values, and the rank-4 update needs no second barrier
 #pragma unroll
-            for (int jj = k >> 2; jj < 16; jj++) {
+    for (int b4 = 0; b4 < 16; b4++) {
+        const int k = 4 * b4;
+        T(*cb)[64] = colb[b4 & 1];
+        T(*rb)[64] = rowb[b4 & 1];
+        cb[w][i] = a[b4];
+        if (i >= k && i < k + 4) {
+#pragma unroll
+            for (int jj = b4 + 1; jj < 16; jj++) rb[i - k][4 * jj + w] = a[jj];
+        }
+        __syncthreads();
+        // D(t, u) = cb[u][k + t]
+        const T u00 = piv(cb[0][k], k), r0 = (T)1 / u00;
+        const T l10 = cb[0][k + 1] * r0, l20 = cb[0][k + 2] * r0, l30 = cb[0][k + 3] * r0;
+        const T u01 = cb[1][k], u02 = cb[2][k], u03 = cb[3][k];
+        const T u11 = piv(cb[1][k + 1] - l10 * u01, k + 1), r1 = (T)1 / u11;
+        const T u12 = cb[2][k + 1] - l10 * u02, u13 = cb[3][k + 1] - l10 * u03;
+        const T l21 = (cb[1][k + 2] - l20 * u01) * r1, l31 = (cb[1][k + 3] - l30 * u01) * r1;
+        const T u22 = piv(cb[2][k + 2] - l20 * u02 - l21 * u12, k + 2), r2 = (T)1 / u22;
+        const T u23 = cb[3][k + 2] - l20 * u03 - l21 * u13;
+        const T l32 = (cb[2][k + 3] - l30 * u02 - l31 * u12) * r2;
+        const T u33 = piv(cb[3][k + 3] - l30 * u03 - l31 * u13 - l32 * u23, k + 3), r3 = (T)1 / u33;
+        if (i > k + 3) {
+            const T m0 = cb[0][i] * r0;
+            const T m1 = (cb[1][i] - m0 * u01) * r1;
+            const T m2 = (cb[2][i] - m0 * u02 - m1 * u12) * r2;
+            const T m3 = (cb[3][i] - m0 * u03 - m1 * u13 - m2 * u23) * r3;
+            a[b4] = (w == 0) ? m0 : (w == 1) ? m1 : (w == 2) ? m2 : m3;
+#pragma unroll
+            for (int jj = b4 + 1; jj < 16; jj++) {
                 const int j = 4 * jj + w;
-                if (j > k) a[jj] -= m * rb[j];
+                const T v0 = rb[0][j];
+                const T v1 = rb[1][j] - l10 * v0;
+                const T v2 = rb[2][j] - l20 * v0 - l21 * v1;
+                const T v3 = rb[3][j] - l30 * v0 - l31 * v1 - l32 * v2;
+                a[jj] -= m0 * v0 + m1 * v1 + m2 * v2 + m3 * v3;
+            }
+        } else if (i >= k) {
+            const int t = i - k;
+            // row t of the block: L strictly left of the diagonal, U from it
+            T e;
+            if (w == 0) e = (t == 0) ? u00 : (t == 1) ? l10 : (t == 2) ? l20 : l30;
+            else if (w == 1) e = (t == 0) ? u01 : (t == 1) ? u11 : (t == 2) ? l21 : l31;
+            else if (w == 2) e = (t == 0) ? u02 : (t == 1) ? u12 : (t == 2) ? u22 : l32;
+            else e = (t == 0) ? u03 : (t == 1) ? u13 : (t == 2) ? u23 : u33;
+            a[b4] = e;
+#pragma unroll
+            for (int jj = b4 + 1; jj < 16; jj++) {
+                const int j = 4 * jj + w;
+                const T v0 = rb[0][j];
+                const T v1 = rb[1][j] - l10 * v0;
+                const T v2 = rb[2][j] - l20 * v0 - l21 * v1;
+                const T v3 = rb[3][j] - l30 * v0 - l31 * v1 - l32 * v2;
+                a[jj] = (t == 0) ? v0 : (t == 1) ? v1 : (t == 2) ? v2 : v3;
             }
         }
-        if (w == (k & 3)) a[k >> 2] = (i == k) ? d : ((i > k) ? m : a[k >> 2]);
     }
 #pragma unroll
     for (int jj = 0; jj < 16; jj++) {
