@@ -34,6 +34,10 @@ typedef enum pastix_amd_iparm_e {
      * right-hand-side pointer, leading dimension and nrhs.  Profiling
      * (PXA_IPARM_PROFILE) runs numfact without the graph.                   */
     PXA_IPARM_GRAPH            = 5,
+    /* Run each level's updates into cblks beyond the next level on a second
+     * HIP stream, beside the next level's panel steps (default 1; 0: one
+     * stream, level after level).  Results are identical either way.       */
+    PXA_IPARM_OVERLAP          = 6,
     PXA_IPARM_SIZE             = 8
 } pastix_amd_iparm_t;
 

@@ -99,6 +99,7 @@ extern "C" void pastixInitWithAffinity(pastix_data_t **pastix_data, PASTIX_Comm,
     d->ext[PXA_IPARM_FACTOR_PRECISION] = -1;
     d->ext[PXA_IPARM_DEVICE] = 0;
     d->ext[PXA_IPARM_LEVEL_ORDER] = 0;
+    d->ext[PXA_IPARM_OVERLAP] = 1;
     d->steps = STEP_INIT;
     *pastix_data = d;
 }
@@ -406,6 +407,13 @@ template <typename T> static void fill_coeftab(pastix_data_t *d, int facto) {
     }
 }
 
+// engine switches held in the extension parameters
+static void engine_flags(pastix_data_t *d) {
+    d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
+    d->engine->use_graph = d->ext[PXA_IPARM_GRAPH] != 0;
+    d->engine->overlap = d->ext[PXA_IPARM_OVERLAP] != 0;
+}
+
 static int factor_prec(pastix_data_t *d) {
     pastix_int_t p = d->ext[PXA_IPARM_FACTOR_PRECISION];
     if (p == SpmFloat || p == SpmDouble) return (int)p;
@@ -437,8 +445,7 @@ extern "C" int pastix_subtask_bcsc2ctab(pastix_data_t *d) {
     if (d->engine->upload_matrix(d->bcsc)) return PASTIX_ERR_INTERNAL;
     if (d->engine->upload_perm(d->order.permtab)) return PASTIX_ERR_INTERNAL;
     if (d->ext[PXA_IPARM_RESIDENT_INPUT] && d->engine->snapshot_coef()) return PASTIX_ERR_INTERNAL;
-    d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
-    d->engine->use_graph = d->ext[PXA_IPARM_GRAPH] != 0;
+    engine_flags(d);
     std::vector<char>().swap(d->hostL);
     std::vector<char>().swap(d->hostU);
     d->steps |= STEP_BCSC2CTAB;
@@ -756,10 +763,7 @@ extern "C" int pastix_amd_set_iparm(pastix_data_t *d, int key, pastix_int_t valu
     if (key == PXA_IPARM_FACTOR_PRECISION && value != -1 && value != SpmFloat && value != SpmDouble)
         return PASTIX_ERR_BADPARAMETER;
     d->ext[key] = value;
-    if (d->engine) {
-        d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
-        d->engine->use_graph = d->ext[PXA_IPARM_GRAPH] != 0;
-    }
+    if (d->engine) engine_flags(d);
     return PASTIX_SUCCESS;
 }
 extern "C" int pastix_amd_get_iparm(const pastix_data_t *d, int key, pastix_int_t *value) {
@@ -864,7 +868,7 @@ extern "C" int pastix_amd_refactorize_shift(pastix_data_t *d, double delta) {
     // ||A + delta I||_F for the static-pivot threshold
     const double nf = d->bcsc.normF;
     d->dparm[DPARM_A_NORM] = std::sqrt(std::max(0.0, nf * nf + 2.0 * delta * E.diag_sum() + (double)d->n * delta * delta));
-    E.profile = d->ext[PXA_IPARM_PROFILE] != 0;
+    engine_flags(d);
     return pastix_subtask_sopalin(d);
 }
 
@@ -872,7 +876,7 @@ extern "C" int pastix_amd_refactorize(pastix_data_t *d) {
     if (!d || !d->engine || !(d->steps & STEP_BCSC2CTAB)) return PASTIX_ERR_BADPARAMETER;
     if (d->engine->shift() != 0.0) return pastix_amd_refactorize_shift(d, 0.0);
     if (d->engine->restore_coef()) return PASTIX_ERR_BADPARAMETER;
-    d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
+    engine_flags(d);
     return pastix_subtask_sopalin(d);
 }
 extern "C" int pastix_amd_backward_error(pastix_data_t *d, const double *b, const double *x, double *berr) {

@@ -21,12 +21,16 @@ namespace pxa {
 
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (side_) (void)hipStreamSynchronize(side_);
     if (fact_exec_) (void)hipGraphExecDestroy(fact_exec_);
     for (SolveGraph &g : solve_graphs_) (void)hipGraphExecDestroy(g.exec);
     for (void *p : owned_) (void)hipFree(p);
     for (hipEvent_t e : pev_) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    if (ev_panel_) (void)hipEventDestroy(ev_panel_);
+    if (ev_side_) (void)hipEventDestroy(ev_side_);
+    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -65,6 +69,9 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0_));
     HIPCHK(hipEventCreate(&ev1_));
+    HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_panel_, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
     facto_ = factotype;
     prec_ = prec;
     n_ = s.n;
@@ -93,6 +100,9 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     lv_cblk_ptr_ = P.lv_cblk_ptr;
     lv_upd_ptr_ = P.lv_upd_ptr;
     lv_upd_split_ = P.lv_upd_split;
+    lv_upd_dnext_ = P.lv_upd_dnext;
+    lv_upd_mnext_ = P.lv_upd_mnext;
+    lv_wt_next_ = P.lv_wt_next;
     lv_fwd_ptr_ = P.lv_fwd_ptr;
     lv_bwd_ptr_ = P.lv_bwd_ptr;
     d_lv_cblks_ = upload(*this, P.lv_cblks);
@@ -192,20 +202,20 @@ int Engine::download_coef(void *hL, void *hU) {
 
 // Level L: factor the diagonal blocks of its cblks, solve their panels, then
 // apply every update these cblks send to later levels.
-int Engine::prof_begin() {
+int Engine::prof_begin(hipStream_t st) {
     if (!profile) return -1;
     while ((int)pev_.size() < nev_ + 2) {
         hipEvent_t e = nullptr;
         if (hipEventCreate(&e) != hipSuccess) return -1;
         pev_.push_back(e);
     }
-    if (hipEventRecord(pev_[nev_], stream_) != hipSuccess) return -1;
+    if (hipEventRecord(pev_[nev_], st) != hipSuccess) return -1;
     return nev_++;
 }
 
-void Engine::prof_end(int cls, int k) {
+void Engine::prof_end(int cls, int k, hipStream_t st) {
     if (k < 0) return;
-    if (hipEventRecord(pev_[nev_], stream_) != hipSuccess) return;
+    if (hipEventRecord(pev_[nev_], st) != hipSuccess) return;
     spans_.push_back(Span{cls, k, nev_});
     nev_++;
 }
@@ -215,8 +225,15 @@ template <typename T> int Engine::factorize_t(double thr) {
     const size_t nl = lv_cblk_ptr_.size() - 1;
     nev_ = 0;
     spans_.clear();
+    // Updates of level L into cblks of level L+1 go on the main stream, right
+    // after L's panels; the others go on the side stream, where they run
+    // beside the panels of level L+1.  The main stream waits for them before
+    // issuing level L+1's updates: targets and workspace regions of the two
+    // streams are then disjoint at every moment, and every target tile still
+    // receives its contributions level by level, as with one stream.
+    bool side_open = false;
     for (size_t lv = 0; lv < nl; lv++) {
-        int pk = prof_begin();
+        int pk = prof_begin(stream_);
         if (facto_ == 2) {
             const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
             launch_lu_square_merge<T>(S_, d_lv_cblks_ + c0, nc, L, U, stream_);
@@ -242,41 +259,60 @@ template <typename T> int Engine::factorize_t(double thr) {
                 launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, L, false, stream_);
             }
         }
-        prof_end(KC_PANEL, pk);
+        prof_end(KC_PANEL, pk, stream_);
         const int u0 = lv_upd_ptr_[lv], us = lv_upd_split_[lv], u1 = lv_upd_ptr_[lv + 1];
-        const int w0 = lv_wt_ptr_[lv], nw = lv_wt_ptr_[lv + 1] - w0;
+        const int w0 = lv_wt_ptr_[lv], w1 = lv_wt_ptr_[lv + 1];
         const bool two_phase = u1 > us && lv_wsize_[lv] <= wcap_;
         T *Wk = (T *)d_wspace_;
-        // one side of the level's update: (A, B) are the operand panels of
-        // the products, C the panels they are subtracted from
-        auto side = [&](int mode, const T *A, const T *B, T *C) {
+        // one side of an update range: direct tiles [da, db), multi-source
+        // tiles [ma, mb) with their product tiles [wa, wb); (A, B) are the
+        // operand panels of the products, C the panels they are subtracted from
+        auto part = [&](hipStream_t st, int mode, const T *A, const T *B, T *C, int da, int db, int ma, int mb,
+                        int wa, int wb) {
             int k;
-            if (us > u0) {
-                k = prof_begin();
-                launch_update<T>(S_, mode, true, d_upd_ + u0, us - u0, d_ents_, d_shift_, A, B, C, stream_);
-                prof_end(KC_DIRECT, k);
+            if (db > da) {
+                k = prof_begin(st);
+                launch_update<T>(S_, mode, true, d_upd_ + da, db - da, d_ents_, d_shift_, A, B, C, st);
+                prof_end(KC_DIRECT, k, st);
             }
-            if (u1 == us) return;
+            if (mb == ma) return;
             if (two_phase) {
-                k = prof_begin();
-                launch_update_wgemm<T>(S_, d_wtiles_ + w0, nw, A, B, Wk, stream_);
-                prof_end(KC_WGEMM, k);
-                k = prof_begin();
-                launch_update_gather<T>(S_, mode, d_upd_ + us, u1 - us, d_ents_, d_shift_, d_wgrp_, Wk, C, stream_);
-                prof_end(KC_GATHER, k);
+                if (wb > wa) {
+                    k = prof_begin(st);
+                    launch_update_wgemm<T>(S_, d_wtiles_ + wa, wb - wa, A, B, Wk, st);
+                    prof_end(KC_WGEMM, k, st);
+                }
+                k = prof_begin(st);
+                launch_update_gather<T>(S_, mode, d_upd_ + ma, mb - ma, d_ents_, d_shift_, d_wgrp_, Wk, C, st);
+                prof_end(KC_GATHER, k, st);
             } else {
-                k = prof_begin();
-                launch_update<T>(S_, mode, false, d_upd_ + us, u1 - us, d_ents_, d_shift_, A, B, C, stream_);
-                prof_end(KC_ACC, k);
+                k = prof_begin(st);
+                launch_update<T>(S_, mode, false, d_upd_ + ma, mb - ma, d_ents_, d_shift_, A, B, C, st);
+                prof_end(KC_ACC, k, st);
             }
         };
-        if (facto_ == 2) {
-            side(0, L, U, L);
-            side(1, U, L, U);
-        } else {
-            side(0, L, L, L);
+        auto sides = [&](hipStream_t st, int da, int db, int ma, int mb, int wa, int wb) {
+            if (facto_ == 2) {
+                part(st, 0, L, U, L, da, db, ma, mb, wa, wb);
+                part(st, 1, U, L, U, da, db, ma, mb, wa, wb);
+            } else {
+                part(st, 0, L, L, L, da, db, ma, mb, wa, wb);
+            }
+        };
+        if (!overlap) {
+            sides(stream_, u0, us, us, u1, w0, w1);
+            continue;
         }
+        const int dn = lv_upd_dnext_[lv], mn = lv_upd_mnext_[lv], wn = lv_wt_next_[lv];
+        HIPCHK(hipEventRecord(ev_panel_, stream_));
+        if (side_open) HIPCHK(hipStreamWaitEvent(stream_, ev_side_, 0));
+        sides(stream_, u0, dn, us, mn, w0, wn);
+        HIPCHK(hipStreamWaitEvent(side_, ev_panel_, 0));
+        sides(side_, dn, us, mn, u1, wn, w1);
+        HIPCHK(hipEventRecord(ev_side_, side_));
+        side_open = true;
     }
+    if (side_open) HIPCHK(hipStreamWaitEvent(stream_, ev_side_, 0));
     // full inverses of the wide diagonal blocks, needed only by the solve
     if (facto_ != 2) {
         launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, L, W1, (T *)d_WB_, stream_);
@@ -401,7 +437,7 @@ int Engine::enqueue_factorize(double thr) {
 int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) {
     if (!inited_) return fail("engine not initialised");
     const bool graph = use_graph && !profile;
-    if (graph && (!fact_exec_ || thr != fact_thr_)) {
+    if (graph && (!fact_exec_ || thr != fact_thr_ || overlap != fact_overlap_)) {
         if (fact_exec_) {
             HIPCHK(hipGraphExecDestroy(fact_exec_));
             fact_exec_ = nullptr;
@@ -421,6 +457,7 @@ int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) 
             return fail(std::string("numfact graph instantiation: ") + hipGetErrorString(ei));
         }
         fact_thr_ = thr;
+        fact_overlap_ = overlap;
     }
     HIPCHK(hipEventRecord(ev0_, stream_));
     if (graph) {

@@ -89,6 +89,9 @@ class Engine {
     // Replay numfact and device solves from captured HIP graphs (one graph
     // launch per factorisation / solve instead of one launch per kernel).
     bool use_graph = false;
+    // Overlap each level's updates into later levels with the next level's
+    // panel steps (second stream); off: one stream, level by level.
+    bool overlap = true;
     double kt[KC_N] = {0, 0, 0, 0, 0};
     int kn[KC_N] = {0, 0, 0, 0, 0};
     double t_diag = 0, t_upd = 0;
@@ -102,7 +105,10 @@ class Engine {
     bool inited_ = false;
     int dev_ = 0, facto_ = 0, prec_ = 3;
     hipStream_t stream_ = nullptr;
-    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    // numfact issues the updates that the next level's panels do not wait
+    // for on a second stream (joined back before the level after)
+    hipStream_t side_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev_panel_ = nullptr, ev_side_ = nullptr;
     std::vector<void *> owned_;
     int64_t bytes_ = 0;
     std::string err_;
@@ -112,6 +118,7 @@ class Engine {
     DevSymbol S_{};
     // host copies of the level pointers
     std::vector<int32_t> lv_cblk_ptr_, lv_upd_ptr_, lv_upd_split_, lv_fwd_ptr_, lv_bwd_ptr_;
+    std::vector<int32_t> lv_upd_dnext_, lv_upd_mnext_, lv_wt_next_;
     // device plan
     int32_t *d_lv_cblks_ = nullptr, *d_bwd_itemptr_ = nullptr, *d_shift_ = nullptr;
     int64_t *d_yrow_ptr_ = nullptr;
@@ -144,6 +151,7 @@ class Engine {
     // user ordering)
     hipGraphExec_t fact_exec_ = nullptr;
     double fact_thr_ = 0;
+    bool fact_overlap_ = true;
     struct SolveGraph {
         const void *ptr;
         int64_t ld;
@@ -160,8 +168,8 @@ class Engine {
     };
     std::vector<Span> spans_;
     int nev_ = 0;
-    int prof_begin();
-    void prof_end(int cls, int k);
+    int prof_begin(hipStream_t st);
+    void prof_end(int cls, int k, hipStream_t st);
     int64_t bwd_part_ = 0;
     // numerics
     void *d_L_ = nullptr, *d_U_ = nullptr, *d_W1_ = nullptr, *d_W2_ = nullptr;

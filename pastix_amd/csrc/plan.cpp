@@ -77,6 +77,9 @@ Plan build_plan(const Symbol &s, const Layout &L) {
 
     P.lv_upd_ptr.assign(nl + 1, 0);
     P.lv_upd_split.assign(nl, 0);
+    P.lv_upd_dnext.assign(nl, 0);
+    P.lv_upd_mnext.assign(nl, 0);
+    P.lv_wt_next.assign(nl, 0);
     P.lv_fwd_ptr.assign(nl + 1, 0);
     P.lv_bwd_ptr.assign(nl + 1, 0);
     P.lv_wt_ptr.assign(nl + 1, 0);
@@ -212,6 +215,12 @@ Plan build_plan(const Symbol &s, const Layout &L) {
         while (split < (int32_t)P.upd.size() && P.upd[split].e1 - P.upd[split].e0 == 1) split++;
         P.lv_upd_split[lv] = split;
         P.lv_upd_ptr[lv + 1] = (int32_t)P.upd.size();
+        // tiles of the next level's cblks first within each range
+        auto next_level = [&](const UpdItem &it) { return L.cblk_level[it.f] == lv + 1; };
+        P.lv_upd_dnext[lv] = (int32_t)(std::stable_partition(P.upd.begin() + P.lv_upd_ptr[lv],
+                                                             P.upd.begin() + split, next_level) - P.upd.begin());
+        P.lv_upd_mnext[lv] =
+            (int32_t)(std::stable_partition(P.upd.begin() + split, P.upd.end(), next_level) - P.upd.begin());
         // exact flops of the kept (p, q) pairs of every entry
         for (int32_t t = P.lv_upd_ptr[lv]; t < P.lv_upd_ptr[lv + 1]; t++) {
             const int kind = (t < split) ? 0 : 1;
@@ -277,6 +286,13 @@ Plan build_plan(const Symbol &s, const Layout &L) {
             const int32_t g0 = (int32_t)(P.wgrp.size() - gid.size());
             std::vector<std::pair<int32_t, int32_t>> gsrc(gid.size());
             for (const auto &kv : gid) gsrc[kv.second - g0] = kv.first;
+            // tiles of groups facing cblks of level lv+1 first
+            std::stable_partition(tl.begin(), tl.end(), [&](const TileKey &t) {
+                return L.cblk_level[s.blok_fcblk[gsrc[t.g - g0].second]] == lv + 1;
+            });
+            int64_t nnext = 0;
+            for (const auto &t : tl) nnext += L.cblk_level[s.blok_fcblk[gsrc[t.g - g0].second]] == lv + 1;
+            P.lv_wt_next[lv] = P.lv_wt_ptr[lv] + (int32_t)nnext;
             for (const auto &t : tl) {
                 const WGroup &G = P.wgrp[t.g];
                 const auto &sk = gsrc[t.g - g0];

@@ -23,6 +23,11 @@ struct Plan {
     // update tiles: items of level L come from the sources factored at level L;
     // [lv_upd_ptr[L], lv_upd_split[L]) have a single entry, the rest several
     std::vector<int32_t> lv_upd_ptr, lv_upd_split;
+    // within each of the two ranges, the tiles of cblks of level L+1 come
+    // first: [lv_upd_ptr[L], lv_upd_dnext[L]) and [lv_upd_split[L],
+    // lv_upd_mnext[L]), so that the updates the next panels wait for can be
+    // issued apart from the others
+    std::vector<int32_t> lv_upd_dnext, lv_upd_mnext;
     std::vector<UpdItem> upd;
     std::vector<UpdEntry> ents;
     std::vector<int32_t> shift;
@@ -33,6 +38,7 @@ struct Plan {
     std::vector<WGroup> wgrp;
     std::vector<WTile> wtiles;
     std::vector<int32_t> lv_wt_ptr;
+    std::vector<int32_t> lv_wt_next;  // [lv_wt_ptr[L], lv_wt_next[L]): groups facing level L+1
     std::vector<int64_t> lv_wsize;
     int64_t wmax = 0;
     // forward products (sources of level L) into the update buffer, whose

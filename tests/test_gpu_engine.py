@@ -230,3 +230,28 @@ def test_graph_replays_match_eager(facto):
         pastix.set_amd_iparm(h, pastix.amd_iparm.graph, 1)
     db.free()
     pastix.finalize(h)
+
+
+@pytest.mark.parametrize("facto", [LLT, LU])
+def test_overlap_matches_single_stream(facto):
+    """Updates issued on a second stream beside the next level's panels give
+    the single-stream factor bit for bit, eagerly and from a captured graph."""
+    grid = (14, 13, 12)
+    if facto == LLT:
+        A = spm.laplacian7(*grid, diag=6.0)
+    else:
+        A = spm.convection_diffusion27(*grid, seed=5)
+    h, S, sym = analyse(A, facto == LLT, facto, grid=grid, min_bs=16, max_bs=48)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.resident_input, 1)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.overlap, 0)
+    pastix.task_numfact(h, S)
+    L0, U0 = pastix.coeftab(h, lu=facto == LU)
+    for graph in (0, 1, 1):
+        pastix.set_amd_iparm(h, pastix.amd_iparm.graph, graph)
+        pastix.set_amd_iparm(h, pastix.amd_iparm.overlap, 1)
+        pastix.refactorize(h)
+        L1, U1 = pastix.coeftab(h, lu=facto == LU)
+        assert np.array_equal(L0, L1), f"graph={graph}"
+        if facto == LU:
+            assert np.array_equal(U0, U1), f"graph={graph} (U)"
+    pastix.finalize(h)
