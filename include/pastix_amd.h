@@ -72,6 +72,22 @@ int pastix_amd_coeftab_export(const pastix_data_t *pastix_data, void *L, void *U
 int pastix_amd_solve_device(pastix_data_t *pastix_data, pastix_int_t nrhs,
                             void *d_b, pastix_int_t ldb);
 
+/* Refinement on FP64 device vectors in the user ordering (ldb, ldx >= n):
+ * the reference's pastix_task_refine (refinement/pastix_task_refine.c:115-288)
+ * without the host copies.  x holds the initial guesses.  Every right-hand
+ * side runs its own recurrence (CG: all columns advance together with
+ * per-column scalars; GMRES: one column after the other).  IPARM_NBITER,
+ * DPARM_RELATIVE_ERROR and DPARM_REFINE_TIME (wall clock) as in the
+ * reference. */
+int pastix_amd_refine_device(pastix_data_t *pastix_data, pastix_int_t nrhs,
+                             const double *d_b, pastix_int_t ldb,
+                             double *d_x, pastix_int_t ldx);
+/* x = M^{-1} b (one solve in the factor precision) followed by the same
+ * refinement: pastix_task_solve + pastix_task_refine on device vectors. */
+int pastix_amd_solve_refine_device(pastix_data_t *pastix_data, pastix_int_t nrhs,
+                                   const double *d_b, pastix_int_t ldb,
+                                   double *d_x, pastix_int_t ldx);
+
 /* Normwise backward error max|b-Ax| / (||A||_inf max|x| + max|b|) for host
  * FP64 vectors in the user ordering, computed on the device. */
 int pastix_amd_backward_error(pastix_data_t *pastix_data, const double *b,

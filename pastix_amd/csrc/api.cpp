@@ -565,47 +565,37 @@ extern "C" int pastix_subtask_diag(pastix_data_t *, pastix_coeftype_t, pastix_in
 // --------------------------------------------------------------------------
 static int refine_common(pastix_data_t *d, pastix_int_t nrhs, const void *b, pastix_int_t ldb, void *x,
                          pastix_int_t ldx, bool permute) {
-    pxa::Engine &E = *d->engine;
     const int64_t n = d->n;
-    double eps = d->dparm[DPARM_EPSILON_REFINEMENT];
-    if (eps < 0.) eps = (d->flttype == SpmFloat) ? 1e-6 : 1e-12;
-    const int itermax = (int)d->iparm[IPARM_ITERMAX];
-    const int im = (int)std::max<pastix_int_t>(1, d->iparm[IPARM_GMRES_IM]);
-    double *db = device_pair(d);
-    if (!db) return PASTIX_ERR_OUTOFMEMORY;
-    double *dx = db + n;
-    std::vector<double> hb(n), hx(n);
+    if (nrhs <= 0) return PASTIX_SUCCESS;
     const bool isf = d->flttype == SpmFloat;
-    int total = 0;
-    double worst = 0;
-    for (pastix_int_t r = 0; r < nrhs; r++) {
+    // FP64 copies in the caller's ordering; the permutation runs on the device
+    std::vector<double> hb((size_t)n * nrhs), hx((size_t)n * nrhs);
+    for (pastix_int_t r = 0; r < nrhs; r++)
         for (int64_t i = 0; i < n; i++) {
-            int64_t p = permute ? d->order.permtab[i] : i;
-            hb[p] = isf ? ((const float *)b)[i + r * ldb] : ((const double *)b)[i + r * ldb];
-            hx[p] = isf ? ((const float *)x)[i + r * ldx] : ((const double *)x)[i + r * ldx];
+            hb[i + r * n] = isf ? ((const float *)b)[i + r * ldb] : ((const double *)b)[i + r * ldb];
+            hx[i + r * n] = isf ? ((const float *)x)[i + r * ldx] : ((const double *)x)[i + r * ldx];
         }
-        if (hipMemcpy(db, hb.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(dx, hx.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-            return PASTIX_ERR_INTERNAL;
-        int it = 0;
-        double res = 0;
-        int rc;
-        pastix_int_t method = d->iparm[IPARM_REFINEMENT];
-        if (method == PastixRefineCG) rc = E.refine_cg(dx, db, itermax, eps, &it, &res);
-        else rc = E.refine_gmres(dx, db, itermax, im, eps, &it, &res);
-        if (rc) return PASTIX_ERR_INTERNAL;
-        if (hipMemcpy(hx.data(), dx, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-            return PASTIX_ERR_INTERNAL;
+    double *dB = device_block(d, 2 * n * nrhs);
+    if (!dB) return PASTIX_ERR_OUTOFMEMORY;
+    double *dX = dB + (size_t)n * nrhs;
+    const size_t bytes = hb.size() * sizeof(double);
+    if (hipMemcpy(dB, hb.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dX, hx.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        return PASTIX_ERR_INTERNAL;
+    int it = 0;
+    double res = 0;
+    const int im = (int)std::max<pastix_int_t>(1, d->iparm[IPARM_GMRES_IM]);
+    if (d->engine->refine_user(dB, n, dX, n, (int)nrhs, std::vector<int64_t>{0, n}, refine_method(d), false, permute,
+                               (int)d->iparm[IPARM_ITERMAX], im, refine_eps(d), &it, &res))
+        return PASTIX_ERR_INTERNAL;
+    if (hipMemcpy(hx.data(), dX, bytes, hipMemcpyDeviceToHost) != hipSuccess) return PASTIX_ERR_INTERNAL;
+    for (pastix_int_t r = 0; r < nrhs; r++)
         for (int64_t i = 0; i < n; i++) {
-            int64_t p = permute ? d->order.permtab[i] : i;
-            if (isf) ((float *)x)[i + r * ldx] = (float)hx[p];
-            else ((double *)x)[i + r * ldx] = hx[p];
+            if (isf) ((float *)x)[i + r * ldx] = (float)hx[i + r * n];
+            else ((double *)x)[i + r * ldx] = hx[i + r * n];
         }
-        total = std::max(total, it);
-        worst = std::max(worst, res);
-    }
-    d->iparm[IPARM_NBITER] = total;
-    d->dparm[DPARM_RELATIVE_ERROR] = worst;
+    d->iparm[IPARM_NBITER] = it;
+    d->dparm[DPARM_RELATIVE_ERROR] = res;
     return PASTIX_SUCCESS;
 }
 
@@ -817,6 +807,32 @@ extern "C" int pastix_amd_solve_device(pastix_data_t *d, pastix_int_t nrhs, void
     if (d->engine->solve_user_device(d_b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
     d->dparm[DPARM_SOLV_TIME] = secs;
     return PASTIX_SUCCESS;
+}
+static int refine_device(pastix_data_t *d, pastix_int_t nrhs, const double *d_b, pastix_int_t ldb, double *d_x,
+                         pastix_int_t ldx, bool initial_solve) {
+    if (!d || !d_b || !d_x || nrhs < 0 || ldb < d->n || ldx < d->n) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_NUMFACT) || !d->engine) return PASTIX_ERR_BADPARAMETER;
+    if (nrhs == 0) return PASTIX_SUCCESS;
+    const double t0 = wall();
+    int it = 0;
+    double res = 0;
+    const int im = (int)std::max<pastix_int_t>(1, d->iparm[IPARM_GMRES_IM]);
+    if (d->engine->refine_user(d_b, ldb, d_x, ldx, (int)nrhs, std::vector<int64_t>{0, d->n}, refine_method(d),
+                               initial_solve, true, (int)d->iparm[IPARM_ITERMAX], im, refine_eps(d), &it, &res))
+        return PASTIX_ERR_INTERNAL;
+    d->iparm[IPARM_NBITER] = it;
+    d->dparm[DPARM_RELATIVE_ERROR] = res;
+    d->dparm[DPARM_REFINE_TIME] = wall() - t0;
+    d->steps |= STEP_REFINE;
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_refine_device(pastix_data_t *d, pastix_int_t nrhs, const double *d_b, pastix_int_t ldb,
+                                        double *d_x, pastix_int_t ldx) {
+    return refine_device(d, nrhs, d_b, ldb, d_x, ldx, false);
+}
+extern "C" int pastix_amd_solve_refine_device(pastix_data_t *d, pastix_int_t nrhs, const double *d_b,
+                                              pastix_int_t ldb, double *d_x, pastix_int_t ldx) {
+    return refine_device(d, nrhs, d_b, ldb, d_x, ldx, true);
 }
 extern "C" int pastix_amd_flops(const pastix_data_t *d, double out[4]) {
     if (!d || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;

@@ -147,38 +147,43 @@ bool rhs_ok(const pastix_data_t *d, const pastix_int_t *nrhs, const void *const 
 // Batched CG on n x R blocks in the user numbering (column-major, ld = n):
 // permutes into the factor's numbering, refines on the device with one
 // recurrence per (segment, column), permutes back.
-int pxa_api::refine_block(pastix_data_t *d, int R, const double *B, double *X, const std::vector<int64_t> &segoff) {
+double *pxa_api::device_block(pastix_data_t *d, int64_t count) {
+    pxa::Engine &E = *d->engine;
+    if (d->batch_dev_owner != &E || d->batch_dev_cap < count) {
+        d->batch_dev = (double *)E.alloc((size_t)count * sizeof(double));
+        d->batch_dev_cap = d->batch_dev ? count : 0;
+        d->batch_dev_owner = &E;
+    }
+    return d->batch_dev;
+}
+
+int pxa_api::refine_method(const pastix_data_t *d) { return d->iparm[IPARM_REFINEMENT] == PastixRefineCG ? 1 : 0; }
+
+double pxa_api::refine_eps(const pastix_data_t *d) {
+    const double eps = d->dparm[DPARM_EPSILON_REFINEMENT];
+    return eps < 0. ? ((d->flttype == SpmFloat) ? 1e-6 : 1e-12) : eps;
+}
+
+int pxa_api::refine_block(pastix_data_t *d, int R, const double *B, double *X, const std::vector<int64_t> &segoff,
+                          bool initial_solve, int method) {
     if (!(d->steps & STEP_NUMFACT) || !d->engine) return PASTIX_ERR_BADPARAMETER;
     const int64_t N = d->n;
-    std::vector<double> Bp((size_t)N * R), Xp((size_t)N * R);
-    const std::vector<int64_t> &perm = d->order.permtab;
-    for (int c = 0; c < R; c++)
-        for (int64_t i = 0; i < N; i++) {
-            Bp[perm[i] + (size_t)c * N] = B[i + (size_t)c * N];
-            Xp[perm[i] + (size_t)c * N] = X[i + (size_t)c * N];
-        }
-    pxa::Engine &E = *d->engine;
-    if (d->batch_dev_owner != &E || d->batch_dev_cap < 2 * N * R) {
-        d->batch_dev = (double *)E.alloc((size_t)2 * N * R * sizeof(double));
-        d->batch_dev_cap = 2 * N * R;
-        d->batch_dev_owner = &E;
-        if (!d->batch_dev) return PASTIX_ERR_OUTOFMEMORY;
-    }
-    double *dB = d->batch_dev, *dX = dB + (size_t)N * R;
-    if (hipMemcpy(dB, Bp.data(), Bp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dX, Xp.data(), Xp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-        return PASTIX_ERR_INTERNAL;
-    double eps = d->dparm[DPARM_EPSILON_REFINEMENT];
-    if (eps < 0.) eps = 1e-12;
+    const size_t bytes = (size_t)N * R * sizeof(double);
+    double *dB = device_block(d, 2 * N * R);
+    if (!dB) return PASTIX_ERR_OUTOFMEMORY;
+    double *dX = dB + (size_t)N * R;
+    if (hipMemcpy(dB, B, bytes, hipMemcpyHostToDevice) != hipSuccess) return PASTIX_ERR_INTERNAL;
+    if (!initial_solve && hipMemcpy(dX, X, bytes, hipMemcpyHostToDevice) != hipSuccess) return PASTIX_ERR_INTERNAL;
     int it = 0;
     double res = 0;
-    if (E.refine_cg_batch(dX, dB, R, segoff, (int)d->iparm[IPARM_ITERMAX], eps, &it, &res)) return PASTIX_ERR_INTERNAL;
-    if (hipMemcpy(Xp.data(), dX, Xp.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    if (d->engine->refine_user(dB, N, dX, N, R, segoff, method, initial_solve, true, (int)d->iparm[IPARM_ITERMAX], 1,
+                               refine_eps(d), &it, &res))
         return PASTIX_ERR_INTERNAL;
-    for (int c = 0; c < R; c++)
-        for (int64_t i = 0; i < N; i++) X[i + (size_t)c * N] = Xp[perm[i] + (size_t)c * N];
-    d->iparm[IPARM_NBITER] = it;
-    d->dparm[DPARM_RELATIVE_ERROR] = res;
+    if (hipMemcpy(X, dX, bytes, hipMemcpyDeviceToHost) != hipSuccess) return PASTIX_ERR_INTERNAL;
+    if (method >= 0) {
+        d->iparm[IPARM_NBITER] = it;
+        d->dparm[DPARM_RELATIVE_ERROR] = res;
+    }
     return PASTIX_SUCCESS;
 }
 
@@ -257,11 +262,14 @@ extern "C" int pastix_amd_batch_solve(pastix_data_t *d, const pastix_int_t *nrhs
     pastix_int_t R = 0;
     if (!rhs_ok(d, nrhs, b, ldb, R)) return PASTIX_ERR_BADPARAMETER;
     if (R == 0) return PASTIX_SUCCESS;
-    std::vector<double> B;
+    if (!(d->steps & STEP_NUMFACT) || !d->engine) return PASTIX_ERR_BADPARAMETER;
+    std::vector<double> B, X(d->batch_off.back() * (size_t)R);
     pack(d->batch_off, R, nrhs, b, ldb, B);
-    const int rc = pastix_task_solve(d, R, B.data(), d->batch_off.back());
+    // one solve in the factor precision, permutations and conversions on the device
+    const int rc = refine_block(d, (int)R, B.data(), X.data(), d->batch_off, true, -1);
     if (rc) return rc;
-    unpack(d->batch_off, nrhs, B, b, ldb);
+    unpack(d->batch_off, nrhs, X, b, ldb);
+    d->steps |= STEP_SOLVE;
     return PASTIX_SUCCESS;
 }
 
@@ -280,7 +288,7 @@ extern "C" int pastix_amd_batch_refine(pastix_data_t *d, const pastix_int_t *nrh
     std::vector<double> B, X;
     pack(d->batch_off, R, nrhs, b, ldb, B);
     pack(d->batch_off, R, nrhs, x, ldx, X);
-    const int rc = refine_block(d, (int)R, B.data(), X.data(), d->batch_off);
+    const int rc = refine_block(d, (int)R, B.data(), X.data(), d->batch_off, false, 1);
     if (rc) return rc;
     unpack(d->batch_off, nrhs, X, x, ldx);
     d->dparm[DPARM_REFINE_TIME] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -652,6 +652,21 @@ int Engine::precond(double *dv) {
     return 0;
 }
 
+int Engine::precond_block(double *zb, int R) {
+    if (prec_ == 3) return solve_t<double>(zb, n_, R);
+    const int64_t nr = n_ * (int64_t)R;
+    if (bsvec_cap_ < nr) {
+        bsvec_cap_ = nr;
+        d_bsvec_ = (float *)alloc(nr * sizeof(float));
+        if (!d_bsvec_) return -1;
+    }
+    launch_d2s(nr, zb, d_bsvec_, stream_);
+    const int rc = solve_t<float>(d_bsvec_, n_, R);
+    if (rc) return rc;
+    launch_s2d(nr, d_bsvec_, zb, stream_);
+    return 0;
+}
+
 int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector<int64_t> &segoff, int itermax,
                             double eps, int *niter, double *resid) {
     const int nseg = (int)segoff.size() - 1;
@@ -660,8 +675,7 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
     if (bvec_cap_ < 4 * nr) {
         bvec_cap_ = 4 * nr;
         d_bvec_ = (double *)alloc(bvec_cap_ * sizeof(double));
-        d_bsvec_ = (float *)alloc(nr * sizeof(float));
-        if (!d_bvec_ || !d_bsvec_) return -1;
+        if (!d_bvec_) return -1;
     }
     if (bscal_cap_ < 8 * ns) {
         bscal_cap_ = 8 * ns;
@@ -677,8 +691,11 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
         for (int s = 0; s < nseg; s++)
             for (int64_t i = segoff[s]; i < segoff[s + 1]; i++) seg[i] = s;
         HIPCHK(hipMemcpy(d_seg_, seg.data(), n_ * sizeof(int32_t), hipMemcpyHostToDevice));
-        d_segoff_ = (int64_t *)alloc(segoff.size() * sizeof(int64_t));
-        if (!d_segoff_) return -1;
+        if (segoff_dcap_ < (int64_t)segoff.size()) {
+            segoff_dcap_ = (int64_t)segoff.size();
+            d_segoff_ = (int64_t *)alloc(segoff.size() * sizeof(int64_t));
+            if (!d_segoff_) return -1;
+        }
         HIPCHK(hipMemcpy(d_segoff_, segoff.data(), segoff.size() * sizeof(int64_t), hipMemcpyHostToDevice));
         segoff_host_ = segoff;
     }
@@ -689,14 +706,6 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
         launch_seg_dot(d_segoff_, nseg, R, n_, x, y, dots, stream_);
         HIPCHK(hipMemcpyAsync(out_h, dots, ns * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
-        return 0;
-    };
-    auto precond_block = [&](double *zb) -> int {
-        if (prec_ == 3) return solve_t<double>(zb, n_, R);
-        launch_d2s(nr, zb, d_bsvec_, stream_);
-        const int rc = solve_t<float>(d_bsvec_, n_, R);
-        if (rc) return rc;
-        launch_s2d(nr, d_bsvec_, zb, stream_);
         return 0;
     };
     auto upload = [&](const std::vector<double> &v) -> int {
@@ -721,10 +730,12 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
     };
     if (segdot(Rv, Rv, h.data())) return -1;
     double worst = residuals();
-    launch_copy(nr, Rv, Z, stream_);
-    if (precond_block(Z)) return -1;
-    launch_copy(nr, Z, P, stream_);
     int it = 0;
+    if (worst > eps && it < itermax) {
+        launch_copy(nr, Rv, Z, stream_);
+        if (precond_block(Z, R)) return -1;
+        launch_copy(nr, Z, P, stream_);
+    }
     std::vector<double> pap(ns), rz2(ns);
     while (worst > eps && it < itermax) {
         it++;
@@ -734,18 +745,73 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
         if (upload(a)) return -1;
         launch_seg_axpy(n_, R, n_, d_seg_, nseg, coef, 1.0, P, X, stream_);
         launch_seg_axpy(n_, R, n_, d_seg_, nseg, coef, -1.0, AP, Rv, stream_);
+        // stopping test before M^{-1} r (its result is unused once every
+        // pair has converged)
+        if (segdot(Rv, Rv, h.data())) return -1;
+        worst = residuals();
+        if (!(worst > eps && it < itermax)) break;
         launch_copy(nr, Rv, Z, stream_);
-        if (precond_block(Z)) return -1;
+        if (precond_block(Z, R)) return -1;
         if (segdot(Rv, Z, rz2.data())) return -1;
         for (int64_t k = 0; k < ns; k++) a[k] = (rz[k] != 0.0) ? rz2[k] / rz[k] : 0.0;
         if (upload(a)) return -1;
         launch_seg_xpby(n_, R, n_, d_seg_, nseg, coef, Z, P, stream_);
-        if (segdot(Rv, Rv, h.data())) return -1;
-        worst = residuals();
     }
     HIPCHK(hipStreamSynchronize(stream_));
     if (niter) *niter = it;
     if (resid) *resid = worst;
+    return 0;
+}
+
+// Right-hand sides and solutions in FP64 device memory, user ordering: rows
+// are permuted into work blocks (reference pastix_task_refine.c:252-284 does
+// it in place on the host), an initial solve x = M^{-1} b is applied when
+// asked, then the refinement, and the result is permuted back.
+int Engine::refine_user(const double *Bu, int64_t ldb, double *Xu, int64_t ldx, int R,
+                        const std::vector<int64_t> &segoff, int method, bool initial_solve, bool user_order,
+                        int itermax, int im, double eps, int *niter, double *resid) {
+    if (R < 1) return 0;
+    if (user_order && !d_perm_) return fail("permutation not uploaded");
+    const int64_t nr = n_ * (int64_t)R;
+    if (sr_cap_ < 2 * nr) {
+        sr_cap_ = 2 * nr;
+        d_sr_ = (double *)alloc(sr_cap_ * sizeof(double));
+        if (!d_sr_) return -1;
+    }
+    double *PB = d_sr_, *PX = d_sr_ + nr;
+    auto in = [&](const double *src, int64_t ld, double *dst) {
+        if (user_order) launch_permute<double>(n_, R, d_perm_, true, src, ld, dst, n_, stream_);
+        else
+            for (int c = 0; c < R; c++) launch_copy(n_, src + c * ld, dst + c * n_, stream_);
+    };
+    in(Bu, ldb, PB);
+    if (initial_solve) {
+        launch_copy(nr, PB, PX, stream_);
+        if (precond_block(PX, R)) return -1;
+    } else {
+        in(Xu, ldx, PX);
+    }
+    int it = 0;
+    double res = 0;
+    if (method == 1) {
+        if (refine_cg_batch(PX, PB, R, segoff, itermax, eps, &it, &res)) return -1;
+    } else if (method == 0) {
+        if (segoff.size() != 2) return fail("GMRES refinement runs on one system");
+        for (int c = 0; c < R; c++) {
+            int i2 = 0;
+            double r2 = 0;
+            if (refine_gmres(PX + c * n_, PB + c * n_, itermax, im, eps, &i2, &r2)) return -1;
+            it = std::max(it, i2);
+            res = std::max(res, r2);
+        }
+    }
+    if (user_order) launch_permute<double>(n_, R, d_perm_, false, PX, n_, Xu, ldx, stream_);
+    else
+        for (int c = 0; c < R; c++) launch_copy(n_, PX + c * n_, Xu + c * ldx, stream_);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (niter) *niter = it;
+    if (resid) *resid = res;
     return 0;
 }
 
@@ -773,10 +839,12 @@ int Engine::refine_cg(double *x, const double *b, int itermax, double eps, int *
     if (normx > 0.) spmv(-1.0, x, 1.0, r);
     double normr = std::sqrt(dot(r, r));
     double rb = (normb > 0) ? normr / normb : normr;
-    launch_copy(n_, r, z, stream_);
-    if (precond(z)) return -1;
-    launch_copy(n_, z, p, stream_);
     int it = 0;
+    if (rb > eps && it < itermax) {
+        launch_copy(n_, r, z, stream_);
+        if (precond(z)) return -1;
+        launch_copy(n_, z, p, stream_);
+    }
     while (rb > eps && it < itermax) {
         it++;
         spmv(1.0, p, 0.0, Ap);
@@ -785,14 +853,17 @@ int Engine::refine_cg(double *x, const double *b, int itermax, double eps, int *
         double alpha = rz / pAp;
         launch_axpy(n_, alpha, p, x, stream_);
         launch_axpy(n_, -alpha, Ap, r, stream_);
+        // the stopping test reads r only: taken before M^{-1} r, whose
+        // result would be discarded when the loop ends here
+        normr = std::sqrt(dot(r, r));
+        rb = (normb > 0) ? normr / normb : normr;
+        if (!(rb > eps && it < itermax)) break;
         launch_copy(n_, r, z, stream_);
         if (precond(z)) return -1;
         double rz2 = dot(r, z);
         double beta = rz2 / rz;
         launch_scal(n_, beta, p, stream_);
         launch_axpy(n_, 1.0, z, p, stream_);
-        normr = std::sqrt(dot(r, r));
-        rb = (normb > 0) ? normr / normb : normr;
     }
     HIPCHK(hipStreamSynchronize(stream_));
     if (niter) *niter = it;

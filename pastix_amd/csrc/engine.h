@@ -48,6 +48,13 @@ class Engine {
     // ||r||/||b|| at exit.
     int refine_cg_batch(double *dX, const double *dB, int R, const std::vector<int64_t> &segoff, int itermax,
                         double eps, int *niter, double *resid);
+    // FP64 device right-hand sides B and solutions X (ld >= n), in the user
+    // ordering when user_order: X = M^{-1} B first if initial_solve (else X
+    // holds the initial guesses), then refinement method 1 = batched CG over
+    // the row segments segoff, 0 = GMRES per column (one segment), -1 = none.
+    int refine_user(const double *B, int64_t ldb, double *X, int64_t ldx, int R, const std::vector<int64_t> &segoff,
+                    int method, bool initial_solve, bool user_order, int itermax, int im, double eps, int *niter,
+                    double *resid);
     // Normwise backward error pieces: [max|b-Ax|, max|x|, max|b|].
     int berr(const double *dx, const double *db, double out[3]);
 
@@ -198,7 +205,10 @@ class Engine {
     // batched refinement work space
     double *d_bvec_ = nullptr, *d_bscal_ = nullptr;
     float *d_bsvec_ = nullptr;
-    int64_t bvec_cap_ = 0, bscal_cap_ = 0;
+    int64_t bvec_cap_ = 0, bscal_cap_ = 0, bsvec_cap_ = 0, segoff_dcap_ = 0;
+    double *d_sr_ = nullptr;  // permuted right-hand sides and solutions of refine_user
+    int64_t sr_cap_ = 0;
+    int precond_block(double *zb, int R);
     int32_t *d_seg_ = nullptr;
     int64_t *d_segoff_ = nullptr;
     std::vector<int64_t> segoff_host_;

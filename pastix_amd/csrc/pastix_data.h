@@ -76,7 +76,17 @@ int spm_triplets(const spmatrix_t *spm, Triplets &t, bool values);
 void triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
                      std::vector<int32_t> &rows, std::vector<double> &vals, bool transpose);
 // Batched CG refinement of n x R user-ordered FP64 blocks (see batch.cpp).
-int refine_block(pastix_data_t *d, int R, const double *B, double *X, const std::vector<int64_t> &segoff);
+// Device FP64 block of at least count entries, kept by the instance between
+// calls (grown on demand).
+double *device_block(pastix_data_t *d, int64_t count);
+// Refinement method code of the instance for Engine::refine_user: 1 CG, 0 GMRES.
+int refine_method(const pastix_data_t *d);
+double refine_eps(const pastix_data_t *d);
+// Host FP64 blocks (n x R, ld n, user ordering of the instance): X = M^{-1} B
+// when initial_solve (else X holds the guesses), then batched CG over the
+// row segments segoff (method 1) or nothing (method -1).
+int refine_block(pastix_data_t *d, int R, const double *B, double *X, const std::vector<int64_t> &segoff,
+                 bool initial_solve, int method);
 }  // namespace pxa_api
 
 

@@ -66,29 +66,41 @@ struct pastix_amd_server_s {
 
 namespace {
 
+// One merged solve: the requests' columns are copied side by side into a
+// device block, solved and refined there (permutations on the device), and
+// copied back into each request's b.
 void run_solves(pastix_amd_server_s *S, int k, std::vector<Req *> &batch) {
     pastix_data_t *d = S->slots[k];
     const int64_t n = S->n;
     pastix_int_t R = 0;
     for (Req *r : batch) R += r->nrhs;
-    std::vector<double> B((size_t)n * R), X;
+    int rc = PASTIX_SUCCESS;
+    double *dB = device_block(d, 2 * n * R);
+    if (!dB) rc = PASTIX_ERR_OUTOFMEMORY;
+    double *dX = dB ? dB + (size_t)n * R : nullptr;
+    const hipStream_t st = d->engine->stream();
+    const size_t col = (size_t)n * sizeof(double);
     pastix_int_t c0 = 0;
     for (Req *r : batch) {
-        for (pastix_int_t c = 0; c < r->nrhs; c++)
-            for (int64_t i = 0; i < n; i++) B[i + (size_t)(c0 + c) * n] = r->b[i + (size_t)c * r->ldb];
+        if (!rc && hipMemcpy2DAsync(dB + (size_t)c0 * n, col, r->b, (size_t)r->ldb * sizeof(double), col, r->nrhs,
+                                    hipMemcpyHostToDevice, st) != hipSuccess)
+            rc = PASTIX_ERR_INTERNAL;
         c0 += r->nrhs;
     }
-    X = B;
-    int rc = pastix_task_solve(d, R, X.data(), n);
-    if (!rc) rc = refine_block(d, (int)R, B.data(), X.data(), std::vector<int64_t>{0, n});
+    int it = 0;
+    double res = 0;
+    if (!rc && d->engine->refine_user(dB, n, dX, n, (int)R, std::vector<int64_t>{0, n}, 1, true, true,
+                                      (int)d->iparm[IPARM_ITERMAX], 1, refine_eps(d), &it, &res))
+        rc = PASTIX_ERR_INTERNAL;
     c0 = 0;
     for (Req *r : batch) {
-        if (!rc)
-            for (pastix_int_t c = 0; c < r->nrhs; c++)
-                for (int64_t i = 0; i < n; i++) r->b[i + (size_t)c * r->ldb] = X[i + (size_t)(c0 + c) * n];
+        if (!rc && hipMemcpy2DAsync(r->b, (size_t)r->ldb * sizeof(double), dX + (size_t)c0 * n, col, col, r->nrhs,
+                                    hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = PASTIX_ERR_INTERNAL;
         c0 += r->nrhs;
-        r->rc = rc;
     }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = PASTIX_ERR_INTERNAL;
+    for (Req *r : batch) r->rc = rc;
 }
 
 void worker(pastix_amd_server_s *S, int k) {

@@ -526,6 +526,25 @@ def solve_device(pastix_data, dbuf, n, nrhs):
     _check("pastix_amd_solve_device", lib.pastix_amd_solve_device(pastix_data.ptr, nrhs, dbuf.ptr, n))
 
 
+lib.pastix_amd_refine_device.argtypes = [_P, _I, ct.c_void_p, _I, ct.c_void_p, _I]
+lib.pastix_amd_refine_device.restype = ct.c_int
+lib.pastix_amd_solve_refine_device.argtypes = [_P, _I, ct.c_void_p, _I, ct.c_void_p, _I]
+lib.pastix_amd_solve_refine_device.restype = ct.c_int
+
+
+def refine_device(pastix_data, db, dx, n, nrhs):
+    """Refine the FP64 device block dx (initial guesses) against db, both
+    n x nrhs in the user ordering."""
+    _check("pastix_amd_refine_device", lib.pastix_amd_refine_device(pastix_data.ptr, nrhs, db.ptr, n, dx.ptr, n))
+
+
+def solve_refine_device(pastix_data, db, dx, n, nrhs):
+    """dx = refined solution of A x = db (FP64 device blocks, user ordering):
+    one solve in the factor precision, then the instance's refinement."""
+    _check("pastix_amd_solve_refine_device",
+           lib.pastix_amd_solve_refine_device(pastix_data.ptr, nrhs, db.ptr, n, dx.ptr, n))
+
+
 class DeviceArray:
     """Minimal HIP device buffer (hipMalloc/hipMemcpy through ctypes) used to
     hand device-resident right-hand sides to pastix_amd_solve_device."""

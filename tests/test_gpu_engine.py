@@ -161,9 +161,65 @@ def test_fp32_factor_fp64_refinement(facto):
     e0 = _berr(A, x[:, 0], b[:, 0])
     pastix.task_refine(h, S, b, x)
     e1 = _berr(A, x[:, 0], b[:, 0])
-    print("facto", facto, "berr fp32 solve", e0, "refined", e1, "iters", ip[pastix.iparm.nbiter])
+    it_host = int(ip[pastix.iparm.nbiter])
+    print("facto", facto, "berr fp32 solve", e0, "refined", e1, "iters", it_host)
     assert e0 < 1e-4
     assert e1 <= 1e-12
+    # the same solve + refinement on device vectors: same recurrence, same
+    # values in the same order, so the same iterate
+    db = pastix.DeviceArray(b)
+    dx = pastix.DeviceArray(np.zeros_like(b))
+    pastix.solve_refine_device(h, db, dx, A.shape[0], 1)
+    xd = dx.download()
+    assert int(ip[pastix.iparm.nbiter]) == it_host
+    assert np.allclose(xd, x, rtol=1e-13, atol=0)
+    # refinement from the device copy of the host solve's guess
+    x2 = b.copy(order="F")
+    pastix.task_solve(h, S, x2)
+    dx.upload(x2)
+    pastix.refine_device(h, db, dx, A.shape[0], 1)
+    assert np.allclose(dx.download(), x, rtol=1e-13, atol=0)
+    db.free()
+    dx.free()
+    pastix.finalize(h)
+
+
+def test_batched_cg_columns_independent():
+    """Several right-hand sides refined together (one CG recurrence per
+    column, FP32 factor) give each column's result alone, within rounding."""
+    grid = (11, 10, 9)
+    A = spm.laplacian7(*grid, coef=(1.0, 1.0, 0.01))
+    S = spm.Spm(A, symmetric=True)
+    ip, dp = pastix.initParam()
+    ip[pastix.iparm.factorization] = LLT
+    ip[pastix.iparm.ordering] = pastix.order.Personal
+    ip[pastix.iparm.refinement] = pastix.refine.CG
+    h = pastix.init(ip, dp)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.factor_precision, spm.SpmFloat)
+    o = pastix.orderGrid(*grid)
+    h.order = o
+    pastix.subtask_order(h, S, o)
+    pastix.subtask_symbfact(h)
+    pastix.subtask_blend(h)
+    pastix.task_numfact(h, S)
+    x0, b = spm.rhs_from_x(A, 5, seed=21)
+    b[:, 3] *= 1e-3  # columns of different scales converge at different steps
+    db = pastix.DeviceArray(b)
+    dx = pastix.DeviceArray(np.zeros_like(b))
+    pastix.solve_refine_device(h, db, dx, A.shape[0], 5)
+    X = dx.download()
+    db.free()
+    dx.free()
+    for c in range(5):
+        bc = np.asfortranarray(b[:, c:c + 1])
+        d1 = pastix.DeviceArray(bc)
+        d2 = pastix.DeviceArray(np.zeros_like(bc))
+        pastix.solve_refine_device(h, d1, d2, A.shape[0], 1)
+        xc = d2.download()[:, 0]
+        d1.free()
+        d2.free()
+        assert _berr(A, X[:, c], b[:, c]) <= 1e-12
+        assert np.linalg.norm(X[:, c] - xc) <= 1e-12 * np.linalg.norm(xc), c
     pastix.finalize(h)
 
 
