@@ -37,6 +37,9 @@ def berr_inf(A, x, b):
     return float(np.abs(r).max() / (abs(A).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max()))
 
 
+OVERLAP = 1  # second stream for the updates beside the next panels (--overlap)
+
+
 def setup(pastix, spm, A, grid, prec=None, resident=True):
     S = spm.Spm(A, symmetric=True)
     ip, dp = pastix.initParam()
@@ -48,6 +51,7 @@ def setup(pastix, spm, A, grid, prec=None, resident=True):
         pastix.set_amd_iparm(h, pastix.amd_iparm.factor_precision, prec)
     if resident:
         pastix.set_amd_iparm(h, pastix.amd_iparm.resident_input, 1)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.overlap, OVERLAP)
     o = pastix.orderGrid(*grid)
     h.order = o
     pastix.subtask_order(h, S, o)
@@ -55,6 +59,23 @@ def setup(pastix, spm, A, grid, prec=None, resident=True):
     pastix.subtask_reordering(h)
     pastix.subtask_blend(h)
     return h, S
+
+
+def traffic_per_launch(kernel, grid):
+    """HBM bytes per launch of a kernel on this workload, from the committed
+    rocprofv3 counter summary (lines 'traffic_bytes_per_dispatch <kernel>
+    <bytes>'), or None when that summary does not cover it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"laplacian7_{grid[0]}cube_potrf_fp64_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    key = kernel.replace(" ", "")
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_dispatch" and parts[1] == key:
+                return float(parts[2])
+    return None
 
 
 def cpu_baseline(pastix, spm, grid):
@@ -298,9 +319,13 @@ def main():
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--serving-grid", type=int, nargs=3, default=[96, 96, 96])
     ap.add_argument("--serving-requests", type=int, default=60)
+    ap.add_argument("--overlap", type=int, default=1,
+                    help="1: updates on a second stream beside the next level's panels; 0: one stream")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only three device-to-device copies of 256 MiB (reference dispatch for HBM counters)")
     args = ap.parse_args()
+    global OVERLAP
+    OVERLAP = args.overlap
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU)", file=sys.stderr)
         sys.exit(2)
@@ -348,12 +373,15 @@ def main():
         return
 
     # per-kernel split of one more factorisation (HIP events around every
-    # update launch on the engine's stream)
+    # update launch), on one stream so that no kernel shares the device
     pastix.set_amd_iparm(h, pastix.amd_iparm.profile, 1)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.overlap, 0)
     pastix.refactorize(h)
     ft = pastix.fact_times(h)
     kt = pastix.kernel_times(h)
+    t_single = float(h.dparm[pastix.dparm.fact_time])
     pastix.set_amd_iparm(h, pastix.amd_iparm.profile, 0)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.overlap, OVERLAP)
     pastix.refactorize(h)
     uf = pastix.update_flops(h)
     kern = {}
@@ -467,11 +495,16 @@ def main():
         "fp32_ir": fp32,
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": kern[dom]["tflops"],
                      "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": kern[dom]["tflops"] / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+                     "frac": kern[dom]["tflops"] / FP64_MFMA_PEAK_TFLOPS,
+                     "traffic": traffic_per_launch(dom, grid),
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
+                                     "copy-calibrated; profiles/)",
                      "launches": kern[dom]["launches"], "avg_launch_ms": kern[dom]["avg_launch_ms"],
                      "flops_per_factorization": kern[dom]["flops"],
                      "update_family_tflops": upd_tf},
-        "numfact_split_ms": {"panel": ft[1] * 1e3, "update": ft[3] * 1e3},
+        "numfact_split_ms": {"panel": ft[1] * 1e3, "update": ft[3] * 1e3,
+                             "note": "one-stream profiled run (overlap off)"},
+        "overlap": bool(OVERLAP),
         "kernels": kern,
         "cpu_baseline": cpu,
     }
