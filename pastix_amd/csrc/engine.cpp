@@ -697,13 +697,35 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
             if (!d_segoff_) return -1;
         }
         HIPCHK(hipMemcpy(d_segoff_, segoff.data(), segoff.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        // chunks of the segmented dot products
+        const int64_t ch = seg_dot_chunk();
+        std::vector<int32_t> coff(nseg + 1, 0), cseg;
+        for (int s = 0; s < nseg; s++) {
+            const int64_t nc = std::max<int64_t>(1, (segoff[s + 1] - segoff[s] + ch - 1) / ch);
+            coff[s + 1] = coff[s] + (int32_t)nc;
+            for (int64_t k = 0; k < nc; k++) cseg.push_back(s);
+        }
+        nchunk_ = coff[nseg];
+        if (chunk_cap_ < nchunk_ + nseg + 1) {
+            chunk_cap_ = nchunk_ + nseg + 1;
+            d_coff_ = (int32_t *)alloc(chunk_cap_ * sizeof(int32_t));
+            if (!d_coff_) return -1;
+        }
+        d_cseg_ = d_coff_ + nseg + 1;
+        HIPCHK(hipMemcpy(d_coff_, coff.data(), coff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_cseg_, cseg.data(), cseg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         segoff_host_ = segoff;
+    }
+    if (cpart_cap_ < (int64_t)nchunk_ * R) {
+        cpart_cap_ = (int64_t)nchunk_ * R;
+        d_cpart_ = (double *)alloc(cpart_cap_ * sizeof(double));
+        if (!d_cpart_) return -1;
     }
     double *Rv = d_bvec_, *P = d_bvec_ + nr, *Z = d_bvec_ + 2 * nr, *AP = d_bvec_ + 3 * nr;
     double *dots = d_bscal_, *coef = d_bscal_ + 4 * ns;
     std::vector<double> h(4 * ns), rb(ns), normb(ns), rz(ns), a(ns);
     auto segdot = [&](const double *x, const double *y, double *out_h) -> int {
-        launch_seg_dot(d_segoff_, nseg, R, n_, x, y, dots, stream_);
+        launch_seg_dot(d_segoff_, d_coff_, d_cseg_, nseg, nchunk_, R, n_, x, y, d_cpart_, dots, stream_);
         HIPCHK(hipMemcpyAsync(out_h, dots, ns * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
         return 0;

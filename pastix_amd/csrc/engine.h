@@ -211,6 +211,12 @@ class Engine {
     int precond_block(double *zb, int R);
     int32_t *d_seg_ = nullptr;
     int64_t *d_segoff_ = nullptr;
+    // chunks of the segmented dot products: offsets per segment, then the
+    // segment of every chunk; first-pass sums
+    int32_t *d_coff_ = nullptr, *d_cseg_ = nullptr;
+    int nchunk_ = 0;
+    int64_t chunk_cap_ = 0, cpart_cap_ = 0;
+    double *d_cpart_ = nullptr;
     std::vector<int64_t> segoff_host_;
 };
 

@@ -74,8 +74,13 @@ void launch_copy(int64_t n, const double *x, double *y, hipStream_t st);
 // Segmented BLAS-1 over an n x R block whose rows are split into nseg
 // consecutive segments (off: nseg+1 offsets, seg: segment of every row);
 // per-(segment, column) scalars are stored at [s + c*nseg].
-void launch_seg_dot(const int64_t *off, int nseg, int R, int64_t ld, const double *x, const double *y, double *out,
-                    hipStream_t st);
+// chunk length of the segmented dot products (entries per first-pass block)
+int seg_dot_chunk();
+// out[s + c*nseg] = x[:,c] . y[:,c] over rows [off[s], off[s+1]); the chunks
+// of segment s are [coff[s], coff[s+1]) (chunk k in segment cseg[k], nchunk
+// chunks in all, part: nchunk x R scratch)
+void launch_seg_dot(const int64_t *off, const int32_t *coff, const int32_t *cseg, int nseg, int nchunk, int R,
+                    int64_t ld, const double *x, const double *y, double *part, double *out, hipStream_t st);
 void launch_seg_axpy(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg, const double *a, double sign,
                      const double *x, double *y, hipStream_t st);
 void launch_seg_xpby(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg, const double *b, const double *z,

@@ -43,14 +43,15 @@ __global__ __launch_bounds__(256) void k_dot_part(int64_t n, const double *__res
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(256) void k_sum_final(int nparts, int m, const double *__restrict__ partials,
-                                                   double *__restrict__ out) {
-    // out[j] = sum_b partials[b*m + j], fixed order
-    const int j = threadIdx.x;
-    if (j >= m) return;
+// out[j] = sum_b partials[b*m + j] by one wave per j (block j): lane l adds
+// b = l, l+64, ..., then a fixed butterfly over the lanes.
+__global__ __launch_bounds__(64) void k_sum_final(int nparts, int m, const double *__restrict__ partials,
+                                                  double *__restrict__ out) {
+    const int j = blockIdx.x, lane = threadIdx.x;
     double t = 0;
-    for (int b = 0; b < nparts; b++) t += partials[(int64_t)b * m + j];
-    out[j] = t;
+    for (int b = lane; b < nparts; b += 64) t += partials[(int64_t)b * m + j];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (lane == 0) out[j] = t;
 }
 
 __global__ __launch_bounds__(256) void k_axpy(int64_t n, double alpha, const double *__restrict__ x,
@@ -196,17 +197,43 @@ void launch_scal(int64_t n, double alpha, double *x, hipStream_t st) {
 // Segment s owns rows [off[s], off[s+1]); column c of an n x R block is at
 // x + c*ld.  out[s + c*nseg] = sum over the segment of x*y, one workgroup per
 // (segment, column), fixed-order tree reduction.
-__global__ __launch_bounds__(256) void k_seg_dot(const int64_t *__restrict__ off, int nseg, int64_t ld,
-                                                 const double *__restrict__ x, const double *__restrict__ y,
-                                                 double *__restrict__ out) {
+// Segmented dot products in two passes of fixed order: pass 1 reduces
+// chunks of SEG_CHUNK entries (chunk k lies in segment cseg[k]; the chunks of
+// segment s are [coff[s], coff[s+1])), pass 2 adds the chunk sums of each
+// segment in chunk order.
+static constexpr int SEG_CHUNK = 4096;
+__global__ __launch_bounds__(256) void k_seg_dot_part(const int64_t *__restrict__ off,
+                                                      const int32_t *__restrict__ coff,
+                                                      const int32_t *__restrict__ cseg, int nchunk, int64_t ld,
+                                                      const double *__restrict__ x, const double *__restrict__ y,
+                                                      double *__restrict__ part) {
     __shared__ double red[256];
-    const int s = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const int k = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const int s = cseg[k];
+    const int64_t i0 = off[s] + (int64_t)(k - coff[s]) * SEG_CHUNK;
+    const int64_t i1 = min(i0 + SEG_CHUNK, off[s + 1]);
     const double *xc = x + c * ld, *yc = y + c * ld;
     double v = 0;
-    for (int64_t i = off[s] + tid; i < off[s + 1]; i += 256) v += xc[i] * yc[i];
+    for (int64_t i = i0 + tid; i < i1; i += 256) v += xc[i] * yc[i];
     red[tid] = v;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) part[k + (int64_t)c * nchunk] = red[0];
+}
+
+__global__ __launch_bounds__(64) void k_seg_dot_final(const int32_t *__restrict__ coff, int nseg, int nchunk,
+                                                      const double *__restrict__ part, double *__restrict__ out) {
+    __shared__ double red[64];
+    const int s = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const double *pc = part + (int64_t)c * nchunk;
+    double v = 0;
+    for (int k = coff[s] + tid; k < coff[s + 1]; k += 64) v += pc[k];
+    red[tid] = v;
+    __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) {
         if (tid < o) red[tid] += red[tid + o];
         __syncthreads();
     }
@@ -233,9 +260,12 @@ __global__ __launch_bounds__(256) void k_seg_xpby(int64_t n, int R, int64_t ld, 
     p[i + c * ld] = z[i + c * ld] + b[seg[i] + c * nseg] * p[i + c * ld];
 }
 
-void launch_seg_dot(const int64_t *off, int nseg, int R, int64_t ld, const double *x, const double *y, double *out,
-                    hipStream_t st) {
-    if (nseg > 0 && R > 0) hipLaunchKernelGGL(k_seg_dot, dim3(nseg, R), dim3(256), 0, st, off, nseg, ld, x, y, out);
+int seg_dot_chunk() { return SEG_CHUNK; }
+void launch_seg_dot(const int64_t *off, const int32_t *coff, const int32_t *cseg, int nseg, int nchunk, int R,
+                    int64_t ld, const double *x, const double *y, double *part, double *out, hipStream_t st) {
+    if (nseg <= 0 || nchunk <= 0 || R <= 0) return;
+    hipLaunchKernelGGL(k_seg_dot_part, dim3(nchunk, R), dim3(256), 0, st, off, coff, cseg, nchunk, ld, x, y, part);
+    hipLaunchKernelGGL(k_seg_dot_final, dim3(nseg, R), dim3(64), 0, st, coff, nseg, nchunk, part, out);
 }
 void launch_seg_axpy(int64_t n, int R, int64_t ld, const int32_t *seg, int nseg, const double *a, double sign,
                      const double *x, double *y, hipStream_t st) {
@@ -290,7 +320,7 @@ void launch_s2d(int64_t n, const float *x, double *y, hipStream_t st) {
 void launch_gemv_t(int64_t n, int m, const double *V, int64_t ldv, const double *x, double *partials, double *out,
                    hipStream_t st) {
     hipLaunchKernelGGL(k_gemv_t_part, dim3(RED_BLOCKS), dim3(256), 0, st, n, m, V, ldv, x, partials);
-    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, RED_BLOCKS, m, partials, out);
+    hipLaunchKernelGGL(k_sum_final, dim3(m), dim3(64), 0, st, RED_BLOCKS, m, partials, out);
 }
 void launch_gemv_n(int64_t n, int m, const double *V, int64_t ldv, const double *y, double alpha, double *x,
                    hipStream_t st) {
