@@ -38,6 +38,12 @@ typedef enum pastix_amd_iparm_e {
      * HIP stream, beside the next level's panel steps (default 1; 0: one
      * stream, level after level).  Results are identical either way.       */
     PXA_IPARM_OVERLAP          = 6,
+    /* Solve with one launch of workgroups that take the diagonal blocks and
+     * product tiles of both sweeps from a queue in dependency order, waiting
+     * on device counters (1); default 0: one kernel launch per level and
+     * sweep step, which is faster on the 3D stencil matrices measured so
+     * far.  Both give the same solution to rounding.                        */
+    PXA_IPARM_TASK_SOLVE       = 7,
     PXA_IPARM_SIZE             = 8
 } pastix_amd_iparm_t;
 

@@ -100,6 +100,7 @@ extern "C" void pastixInitWithAffinity(pastix_data_t **pastix_data, PASTIX_Comm,
     d->ext[PXA_IPARM_DEVICE] = 0;
     d->ext[PXA_IPARM_LEVEL_ORDER] = 0;
     d->ext[PXA_IPARM_OVERLAP] = 1;
+    d->ext[PXA_IPARM_TASK_SOLVE] = 0;
     d->steps = STEP_INIT;
     *pastix_data = d;
 }
@@ -412,6 +413,7 @@ static void engine_flags(pastix_data_t *d) {
     d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
     d->engine->use_graph = d->ext[PXA_IPARM_GRAPH] != 0;
     d->engine->overlap = d->ext[PXA_IPARM_OVERLAP] != 0;
+    d->engine->task_solve = d->ext[PXA_IPARM_TASK_SOLVE] != 0;
 }
 
 static int factor_prec(pastix_data_t *d) {

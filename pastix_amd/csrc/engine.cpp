@@ -135,6 +135,17 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
         }
         d_diagpos_ = upload(*this, dp);
     }
+    d_stasks_ = upload(*this, P.stasks);
+    n_stasks_ = (int)P.stasks.size();
+    d_schunk_ = upload(*this, P.schunk);
+    n_schunk_ = (int)P.schunk.size() - 1;
+    d_itgt_ptr_ = upload(*this, P.itgt_ptr);
+    d_itgt_ = upload(*this, P.itgt);
+    d_sneed_ = upload(*this, P.sneed);
+    d_scnt_ = (int32_t *)alloc((size_t)(4 * cblknbr_ + 2) * sizeof(int32_t));
+    if (!d_stasks_ || !d_itgt_ptr_ || !d_sneed_ || !d_scnt_) return -1;
+    HIPCHK(hipMemsetAsync(d_scnt_, 0, (size_t)(4 * cblknbr_ + 2) * sizeof(int32_t), stream_));
+    HIPCHK(hipDeviceGetAttribute(&n_cu_, hipDeviceAttributeMultiprocessorCount, dev_));
     d_wgrp_ = upload(*this, P.wgrp);
     d_wtiles_ = upload(*this, P.wtiles);
     lv_wt_ptr_ = P.lv_wt_ptr;
@@ -377,7 +388,7 @@ int Engine::enqueue_solve(void *dx, int64_t ldx, int nrhs, bool user) {
 
 hipGraphExec_t Engine::solve_graph(void *dx, int64_t ldx, int nrhs, bool user) {
     for (const SolveGraph &g : solve_graphs_)
-        if (g.ptr == dx && g.ld == ldx && g.nrhs == nrhs && g.user == user) return g.exec;
+        if (g.ptr == dx && g.ld == ldx && g.nrhs == nrhs && g.user == user && g.tasks == task_solve) return g.exec;
     if (solve_graphs_.size() >= 16) {
         (void)hipGraphExecDestroy(solve_graphs_.front().exec);
         solve_graphs_.erase(solve_graphs_.begin());
@@ -398,7 +409,7 @@ hipGraphExec_t Engine::solve_graph(void *dx, int64_t ldx, int nrhs, bool user) {
         fail(std::string("solve graph instantiation: ") + hipGetErrorString(ei));
         return nullptr;
     }
-    solve_graphs_.push_back(SolveGraph{dx, ldx, nrhs, user, ex});
+    solve_graphs_.push_back(SolveGraph{dx, ldx, nrhs, user, task_solve, ex});
     return ex;
 }
 
@@ -423,6 +434,7 @@ int Engine::solve_user_device(void *db, int64_t ldb, int nrhs, double *seconds) 
     }
     HIPCHK(hipEventRecord(ev1_, stream_));
     HIPCHK(hipEventSynchronize(ev1_));
+    if (check_solve()) return -1;
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
     if (seconds) *seconds = ms * 1e-3;
@@ -510,6 +522,15 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
     for (int r0 = 0; r0 < nrhs; r0 += nrhs_chunk_) {
         const int nr = std::min(nrhs_chunk_, nrhs - r0);
         T *xx = x + (int64_t)r0 * ldx;
+        if (task_solve && n_stasks_ > 0) {
+            // counters and queue head to zero (a kernel, so that graph
+            // replays clear them too); the error flag stays until read
+            launch_zero_int((int)(4 * cblknbr_ + 1), d_scnt_, stream_);
+            launch_solve_tasks<T>(S_, d_stasks_, d_schunk_, n_schunk_, d_diag_, d_fwd_, d_bwd_, d_itgt_ptr_, d_itgt_, d_sneed_,
+                                  d_scnt_, (int)cblknbr_, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, nr, L, U, W1, WB,
+                                  d_ydst_, part, maxwidth_, 8 * n_cu_, stream_);
+            continue;
+        }
         for (int lv = 0; lv < nl; lv++) {
             const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
             const int f0 = lv_fwd_ptr_[lv], nf = lv_fwd_ptr_[lv + 1] - f0;
@@ -528,6 +549,16 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
     return 0;
 }
 
+int Engine::check_solve() {
+    if (!d_scnt_) return 0;
+    int32_t e = 0;
+    int32_t *flag = d_scnt_ + 4 * cblknbr_ + 1;
+    HIPCHK(hipMemcpy(&e, flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (e == 0) return 0;
+    HIPCHK(hipMemset(flag, 0, sizeof(int32_t)));
+    return fail("solve: a dependency wait of the task queue timed out");
+}
+
 int Engine::solve_device(void *dx, int64_t ldx, int nrhs, double *seconds) {
     if (!inited_) return fail("engine not initialised");
     HIPCHK(hipEventRecord(ev0_, stream_));
@@ -541,6 +572,7 @@ int Engine::solve_device(void *dx, int64_t ldx, int nrhs, double *seconds) {
     }
     HIPCHK(hipEventRecord(ev1_, stream_));
     HIPCHK(hipEventSynchronize(ev1_));
+    if (check_solve()) return -1;
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
     if (seconds) *seconds = ms * 1e-3;
@@ -832,6 +864,7 @@ int Engine::refine_user(const double *Bu, int64_t ldb, double *Xu, int64_t ldx, 
         for (int c = 0; c < R; c++) launch_copy(n_, PX + c * n_, Xu + c * ldx, stream_);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
+    if (check_solve()) return -1;
     if (niter) *niter = it;
     if (resid) *resid = res;
     return 0;

@@ -99,6 +99,10 @@ class Engine {
     // Overlap each level's updates into later levels with the next level's
     // panel steps (second stream); off: one stream, level by level.
     bool overlap = true;
+    // Run both solve sweeps as one launch of workgroups that take tasks in
+    // dependency order and wait on device counters; off: one kernel launch
+    // per level and sweep step (the default: faster on the 3D stencils).
+    bool task_solve = false;
     double kt[KC_N] = {0, 0, 0, 0, 0};
     int kn[KC_N] = {0, 0, 0, 0, 0};
     double t_diag = 0, t_upd = 0;
@@ -163,7 +167,7 @@ class Engine {
         const void *ptr;
         int64_t ld;
         int nrhs;
-        bool user;
+        bool user, tasks;
         hipGraphExec_t exec;
     };
     std::vector<SolveGraph> solve_graphs_;
@@ -209,6 +213,15 @@ class Engine {
     double *d_sr_ = nullptr;  // permuted right-hand sides and solutions of refine_user
     int64_t sr_cap_ = 0;
     int precond_block(double *zb, int R);
+    // dependency-driven solve: tasks, product-tile targets, expected counts,
+    // counters (4*cblknbr + 2: the last one is a sticky error flag)
+    SolveTask *d_stasks_ = nullptr;
+    int n_stasks_ = 0, n_cu_ = 256;
+    int32_t *d_schunk_ = nullptr;
+    int n_schunk_ = 0;
+    int32_t *d_itgt_ptr_ = nullptr, *d_itgt_ = nullptr, *d_sneed_ = nullptr, *d_scnt_ = nullptr;
+    // reads and clears the error flag of the dependency-driven solve
+    int check_solve();
     int32_t *d_seg_ = nullptr;
     int64_t *d_segoff_ = nullptr;
     // chunks of the segmented dot products: offsets per segment, then the

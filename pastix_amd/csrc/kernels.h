@@ -91,6 +91,17 @@ template <typename T>
 void launch_add_diag(int64_t n, const int64_t *pos, double delta, T *coef, hipStream_t st);
 void launch_set_csr_diag(int64_t n, const int64_t *pos, const double *d0, double delta, double *vals, hipStream_t st);
 // Zeroes n ints with a kernel (usable inside a captured graph).
+// Both sweeps of the solve in one launch of nwg workgroups taking the tasks
+// in list order, a chunk [chunks[k], chunks[k+1]) at a time and waiting on device counters (cnt: 4*cb+2 ints, zeroed
+// before the launch; cnt[4*cb+1] != 0 afterwards reports a wait that timed
+// out).  need, tgt_ptr/tgt: expected counts and target cblks of the plan.
+template <typename T>
+void launch_solve_tasks(const DevSymbol &S, const SolveTask *tasks, const int32_t *chunks, int nchunks,
+                        const DiagItem *diag, const FwdItem *fwd, const BwdItem *bwd, const int32_t *tgt_ptr,
+                        const int32_t *tgt, const int32_t *need, int32_t *cnt, int cb, const int64_t *yrow_ptr,
+                        T *ybuf, int64_t ldy, T *x, int64_t ldx, T *z, int64_t ldz, int nrhs, const T *L, const T *U,
+                        const T *w1, const T *wb, const int32_t *ydst, T *part, int maxwidth, int nwg,
+                        hipStream_t st);
 void launch_zero_int(int n, int *p, hipStream_t st);
 void launch_d2s(int64_t n, const double *x, float *y, hipStream_t st);
 void launch_s2d(int64_t n, const float *x, double *y, hipStream_t st);

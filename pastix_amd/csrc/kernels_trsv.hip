@@ -21,6 +21,8 @@
 // element is produced in a fixed order by one workgroup: no atomics,
 // bitwise-reproducible solves.  NR is the compile-time bound on the
 // right-hand sides of one launch (1 or 8); NW the waves of a workgroup.
+#include <algorithm>
+
 #include "dev_common.h"
 #include "dev_types.h"
 #include "kernels.h"
@@ -29,9 +31,22 @@ namespace pxa {
 
 static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
 
+// Loads and stores of the data one solve task hands to another (update
+// buffer, forward result, solution rows, backward partials).  CO: device-
+// coherent accesses for the single-launch solve, where the producer and the
+// consumer may sit on different dies; plain accesses between level kernels.
+template <bool CO, typename T> __device__ __forceinline__ T ld_co(const T *p) {
+    if constexpr (CO) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+template <bool CO, typename T> __device__ __forceinline__ void st_co(T *p, T v) {
+    if constexpr (CO) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
 // red holds NW x 64 x NR partial sums; the sum over waves is taken in wave
 // order by the threads of wave 0.
-template <typename T, int NR, int NW>
+template <bool CO, typename T, int NR, int NW>
 __device__ __forceinline__ void wave_sum_store(T (&acc)[NR], T *red, int r, int q, bool ok, int nrhs, T *out,
                                                int64_t ldo, const T *sub, int64_t lds) {
 #pragma unroll
@@ -44,25 +59,22 @@ __device__ __forceinline__ void wave_sum_store(T (&acc)[NR], T *red, int r, int 
                 T v = red[r * NR + u];
 #pragma unroll
                 for (int w = 1; w < NW; w++) v += red[(w * 64 + r) * NR + u];
-                out[u * ldo] = sub ? sub[u * lds] - v : v;
+                st_co<CO>(out + u * ldo, sub ? ld_co<CO>(sub + u * lds) - v : v);
             }
         }
     }
 }
 
-template <typename T, int NR, int NW>
-__global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
-                                                      const int64_t *__restrict__ yrow_ptr,
-                                                      const T *__restrict__ ybuf, int64_t ldy,
-                                                      const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
-                                                      int64_t ldz, int nrhs, const T *__restrict__ w1) {
-    extern __shared__ char smem_raw[];
-    T *y = reinterpret_cast<T *>(smem_raw);  // maxwidth * NR
-    __shared__ T red[NW * 64 * NR];
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    if (id >= nitems) return;
+// ---------------------------------------------------------------------------
+// Task bodies, each run by one whole workgroup of NW waves (the level
+// kernels below and the dependency-driven solve share them).
+// ---------------------------------------------------------------------------
+template <bool CO, typename T, int NR, int NW>
+__device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem it,
+                                              const int64_t *__restrict__ yrow_ptr, const T *__restrict__ ybuf,
+                                              int64_t ldy, const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
+                                              int64_t ldz, int nrhs, const T *__restrict__ w1, T *y, T *red) {
     const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
-    const DiagItem it = items[id];
     const int c = it.c, r0 = it.ib * 64;
     const int fc = S.cblk_fcol[c];
     const int Kc = S.cblk_fcol[c + 1] - fc;
@@ -75,8 +87,9 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
         int64_t p = yrow_ptr[g];
         const int64_t pe = yrow_ptr[g + 1];
 #pragma unroll 4
-        for (; p + 4 <= pe; p += 4) v -= ((yb[p] + yb[p + 1]) + (yb[p + 2] + yb[p + 3]));
-        for (; p < pe; p++) v -= yb[p];
+        for (; p + 4 <= pe; p += 4)
+            v -= ((ld_co<CO>(yb + p) + ld_co<CO>(yb + p + 1)) + (ld_co<CO>(yb + p + 2) + ld_co<CO>(yb + p + 3)));
+        for (; p < pe; p++) v -= ld_co<CO>(yb + p);
         y[i + u * Kc] = v;
     }
     __syncthreads();
@@ -94,26 +107,20 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
                 if (u < nrhs) acc[u] += m * y[k + u * Kc];
         }
     }
-    wave_sum_store<T, NR, NW>(acc, red, r, q, i < Kc, nrhs, z + fc + i, ldz, (const T *)nullptr, 0);
+    wave_sum_store<CO, T, NR, NW>(acc, red, r, q, i < Kc, nrhs, z + fc + i, ldz, (const T *)nullptr, 0);
 }
 
-template <typename T, int NR, int NW>
-__global__ __launch_bounds__(NW * 64) void k_fwd_prod(DevSymbol S, const FwdItem *__restrict__ items, int nitems,
-                                                      const T *__restrict__ coef, const T *__restrict__ z,
-                                                      int64_t ldz, int nrhs, const int32_t *__restrict__ ydst,
-                                                      T *__restrict__ ybuf, int64_t ldy) {
-    extern __shared__ char smem_raw[];
-    T *xs = reinterpret_cast<T *>(smem_raw);  // Kc * NR
-    __shared__ T red[NW * 64 * NR];
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    if (id >= nitems) return;
+template <bool CO, typename T, int NR, int NW>
+__device__ __forceinline__ void fwd_prod_body(const DevSymbol &S, const FwdItem it, const T *__restrict__ coef,
+                                              const T *__restrict__ z, int64_t ldz, int nrhs,
+                                              const int32_t *__restrict__ ydst, T *__restrict__ ybuf, int64_t ldy,
+                                              T *xs, T *red) {
     const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
-    const FwdItem it = items[id];
     const int c = it.c;
     const int fc = S.cblk_fcol[c];
     const int Kc = S.cblk_fcol[c + 1] - fc;
     const int64_t ld = S.cblk_stride[c];
-    for (int t = tid; t < Kc * nrhs; t += NW * 64) xs[t] = z[fc + (t % Kc) + (int64_t)(t / Kc) * ldz];
+    for (int t = tid; t < Kc * nrhs; t += NW * 64) xs[t] = ld_co<CO>(z + fc + (t % Kc) + (int64_t)(t / Kc) * ldz);
     __syncthreads();
     const int nr = it.r1 - it.r0;
     T acc[NR];
@@ -130,31 +137,23 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_prod(DevSymbol S, const FwdItem
         }
     }
     const int dst = (r < nr) ? ydst[it.slot + r] : 0;
-    wave_sum_store<T, NR, NW>(acc, red, r, q, r < nr, nrhs, ybuf + dst, ldy, (const T *)nullptr, 0);
+    wave_sum_store<CO, T, NR, NW>(acc, red, r, q, r < nr, nrhs, ybuf + dst, ldy, (const T *)nullptr, 0);
 }
 
-// One 64x64 tile (item, column block blockIdx.y) of the backward products.
-template <typename T, int NR>
-__global__ __launch_bounds__(256) void k_bwd_prod(DevSymbol S, const BwdItem *__restrict__ items, int nitems,
-                                                  const T *__restrict__ coef, const T *__restrict__ x, int64_t ldx,
-                                                  int nrhs, T *__restrict__ part) {
-    __shared__ T tile[64][65];
-    __shared__ T xr[64 * NR];
-    __shared__ T red[256 * NR];
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    if (id >= nitems) return;
+// one 64x64 tile (column block k0/64) of the backward products, 4 waves
+template <bool CO, typename T, int NR>
+__device__ __forceinline__ void bwd_prod_body(const DevSymbol &S, const BwdItem it, int k0,
+                                              const T *__restrict__ coef, const T *__restrict__ x, int64_t ldx,
+                                              int nrhs, T *__restrict__ part, T (*tile)[65], T *xr, T *red) {
     const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
-    const BwdItem it = items[id];
     const int c = it.c;
     const int Kc = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
-    const int k0 = blockIdx.y * 64;
-    if (k0 >= Kc) return;
     const int64_t ld = S.cblk_stride[c];
     const int64_t rb = S.cblk_rowbase[c];
     const int nr = it.r1 - it.r0;
     for (int t = tid; t < 64 * NR; t += 256) {
         const int row = t & 63, u = t >> 6;
-        xr[t] = (row < nr && u < nrhs) ? x[S.panel_grow[rb + it.r0 + row] + u * ldx] : (T)0;
+        xr[t] = (row < nr && u < nrhs) ? ld_co<CO>(x + S.panel_grow[rb + it.r0 + row] + u * ldx) : (T)0;
     }
     const T *P = coef + S.cblk_coef[c] + it.r0;
     for (int cc = q; cc < 64; cc += 4)
@@ -170,21 +169,15 @@ __global__ __launch_bounds__(256) void k_bwd_prod(DevSymbol S, const BwdItem *__
         for (int u = 0; u < NR; u++) acc[u] += a * xr[row + u * 64];
     }
     T *out = part + (int64_t)it.slot * nrhs + k0 + lane;
-    wave_sum_store<T, NR, 4>(acc, red, lane, q, k0 + lane < Kc, nrhs, out, Kc, (const T *)nullptr, 0);
+    wave_sum_store<CO, T, NR, 4>(acc, red, lane, q, k0 + lane < Kc, nrhs, out, Kc, (const T *)nullptr, 0);
 }
 
-template <typename T, int NR, int NW>
-__global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
-                                                      const BwdItem *__restrict__ bitems, const T *__restrict__ z,
-                                                      int64_t ldz, T *__restrict__ x, int64_t ldx, int nrhs,
-                                                      const T *__restrict__ part, const T *__restrict__ wb) {
-    extern __shared__ char smem_raw[];
-    T *y = reinterpret_cast<T *>(smem_raw);  // maxwidth * NR
-    __shared__ T red[NW * 64 * NR];
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    if (id >= nitems) return;
+template <bool CO, typename T, int NR, int NW>
+__device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem it,
+                                              const BwdItem *__restrict__ bitems, const T *__restrict__ z,
+                                              int64_t ldz, T *__restrict__ x, int64_t ldx, int nrhs,
+                                              const T *__restrict__ part, const T *__restrict__ wb, T *y, T *red) {
     const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
-    const DiagItem it = items[id];
     const int c = it.c, r0 = it.ib * 64;
     const int fc = S.cblk_fcol[c];
     const int Kc = S.cblk_fcol[c + 1] - fc;
@@ -201,10 +194,11 @@ __global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagIte
                 const T *pp = part + (s0 + (int64_t)(j - it.b0) * Kc) * nrhs + k;
 #pragma unroll
                 for (int u = 0; u < NR; u++)
-                    if (u < nrhs) acc[u] += pp[u * Kc];
+                    if (u < nrhs) acc[u] += ld_co<CO>(pp + u * Kc);
             }
         }
-        wave_sum_store<T, NR, NW>(acc, red, r, q, k < Kc, nrhs, y + k, Kc, z + fc + k, ldz);
+        // y lives in LDS: a plain store; z comes from other tasks
+        wave_sum_store<CO, T, NR, NW>(acc, red, r, q, k < Kc, nrhs, y + k, Kc, z + fc + k, ldz);
         __syncthreads();
     }
     const int i = r0 + r;
@@ -222,7 +216,194 @@ __global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagIte
                 if (u < nrhs) acc[u] += m * y[k + u * Kc];
         }
     }
-    wave_sum_store<T, NR, NW>(acc, red, r, q, i < Kc, nrhs, x + fc + i, ldx, (const T *)nullptr, 0);
+    wave_sum_store<CO, T, NR, NW>(acc, red, r, q, i < Kc, nrhs, x + fc + i, ldx, (const T *)nullptr, 0);
+}
+
+// ------------------------------ level kernels -------------------------------
+template <typename T, int NR, int NW>
+__global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
+                                                      const int64_t *__restrict__ yrow_ptr,
+                                                      const T *__restrict__ ybuf, int64_t ldy,
+                                                      const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
+                                                      int64_t ldz, int nrhs, const T *__restrict__ w1) {
+    extern __shared__ char smem_raw[];
+    __shared__ T red[NW * 64 * NR];
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    fwd_diag_body<false, T, NR, NW>(S, items[id], yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1,
+                             reinterpret_cast<T *>(smem_raw), red);
+}
+
+template <typename T, int NR, int NW>
+__global__ __launch_bounds__(NW * 64) void k_fwd_prod(DevSymbol S, const FwdItem *__restrict__ items, int nitems,
+                                                      const T *__restrict__ coef, const T *__restrict__ z,
+                                                      int64_t ldz, int nrhs, const int32_t *__restrict__ ydst,
+                                                      T *__restrict__ ybuf, int64_t ldy) {
+    extern __shared__ char smem_raw[];
+    __shared__ T red[NW * 64 * NR];
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    fwd_prod_body<false, T, NR, NW>(S, items[id], coef, z, ldz, nrhs, ydst, ybuf, ldy, reinterpret_cast<T *>(smem_raw), red);
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void k_bwd_prod(DevSymbol S, const BwdItem *__restrict__ items, int nitems,
+                                                  const T *__restrict__ coef, const T *__restrict__ x, int64_t ldx,
+                                                  int nrhs, T *__restrict__ part) {
+    __shared__ T tile[64][65];
+    __shared__ T xr[64 * NR];
+    __shared__ T red[256 * NR];
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const BwdItem it = items[id];
+    const int k0 = blockIdx.y * 64;
+    if (k0 >= S.cblk_fcol[it.c + 1] - S.cblk_fcol[it.c]) return;
+    bwd_prod_body<false, T, NR>(S, it, k0, coef, x, ldx, nrhs, part, tile, xr, red);
+}
+
+template <typename T, int NR, int NW>
+__global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
+                                                      const BwdItem *__restrict__ bitems, const T *__restrict__ z,
+                                                      int64_t ldz, T *__restrict__ x, int64_t ldx, int nrhs,
+                                                      const T *__restrict__ part, const T *__restrict__ wb) {
+    extern __shared__ char smem_raw[];
+    __shared__ T red[NW * 64 * NR];
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    bwd_diag_body<false, T, NR, NW>(S, items[id], bitems, z, ldz, x, ldx, nrhs, part, wb, reinterpret_cast<T *>(smem_raw),
+                             red);
+}
+
+// --------------------- dependency-driven solve (one launch) -----------------
+// Workgroups take tasks from a queue in the order of the task list (forward
+// levels upward, then backward levels downward) and wait, spinning on
+// device counters, for the tasks theirs depends on:
+//   0 forward diagonal block (c, ib): every forward product tile facing c;
+//   1 forward product tile of c: every forward diagonal block of c;
+//   2 backward product tile (item, column block): every backward diagonal
+//     block of the cblks its rows belong to;
+//   3 backward diagonal block of c: every backward product tile of c and
+//     every forward diagonal block of c.
+// A task only waits for tasks placed before it in the list, which running
+// workgroups have already taken, so the queue always drains.  A wait longer
+// than SOLVE_SPIN_LIMIT polls raises the error flag, and every workgroup then
+// leaves: the launch always ends.  The data handed from task to task is
+// written and read with device-coherent accesses, and a task signals only
+// after its stores are complete, so a consumer that sees the counter reads
+// the producer's values.  Counters: cnt[0..cb) forward products
+// received, [cb..2cb) forward diagonal blocks done, [2cb..3cb) backward
+// products done, [3cb..4cb) backward diagonal blocks done, cnt[4cb] queue
+// head, cnt[4cb+1] error flag; need[0..cb) products facing c, [cb..2cb)
+// diagonal blocks of c, [2cb..3cb) backward product tiles of c.
+constexpr int SOLVE_SPIN_LIMIT = 1 << 20;
+
+__device__ __forceinline__ bool solve_wait(const int *c, int need, int *err) {
+    if (need <= 0) return true;
+    int spins = 0;
+    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+        if (++spins > SOLVE_SPIN_LIMIT) {
+            __hip_atomic_fetch_add(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void k_solve_tasks(DevSymbol S, const SolveTask *__restrict__ tasks,
+                                                     const int32_t *__restrict__ chunks, int nchunks,
+                                                     const DiagItem *__restrict__ diag,
+                                                     const FwdItem *__restrict__ fwd, const BwdItem *__restrict__ bwd,
+                                                     const int32_t *__restrict__ tgt_ptr,
+                                                     const int32_t *__restrict__ tgt, const int32_t *__restrict__ need,
+                                                     int32_t *cnt, int cb, const int64_t *__restrict__ yrow_ptr,
+                                                     T *ybuf, int64_t ldy, T *x, int64_t ldx, T *z, int64_t ldz,
+                                                     int nrhs, const T *__restrict__ L, const T *__restrict__ U,
+                                                     const T *__restrict__ w1, const T *__restrict__ wb,
+                                                     const int32_t *__restrict__ ydst, T *part) {
+    // dynamic LDS: the y / z staging of the diagonal and forward tasks or the
+    // 64x65 tile of a backward product task
+    extern __shared__ char smem_raw[];
+    T *ysh = reinterpret_cast<T *>(smem_raw);
+    T(*tile)[65] = reinterpret_cast<T(*)[65]>(smem_raw);
+    __shared__ T red[4 * 64 * NR];
+    __shared__ T xr[64 * NR];
+    __shared__ int sh_t, sh_ok, sh_c;
+    int *fwd_in = cnt, *fdiag = cnt + cb, *bprod = cnt + 2 * cb, *bdiag = cnt + 3 * cb;
+    int *head = cnt + 4 * cb, *err = head + 1;
+    const int tid = threadIdx.x;
+    int t = 0, tend = 0;  // tasks [t, tend) of the chunk in hand
+    for (;;) {
+        if (t == tend) {
+            // next chunk of the queue
+            if (tid == 0) sh_c = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            const int ck = sh_c;
+            if (ck >= nchunks) return;
+            t = chunks[ck];
+            tend = chunks[ck + 1];
+        }
+        if (tid == 0) {
+            bool ok = true;
+            {
+                const SolveTask tk = tasks[t];
+                const int type = tk.code & 3;
+                if (type == 0) {
+                    const int c = diag[tk.idx].c;
+                    ok = solve_wait(fwd_in + c, need[c], err);
+                } else if (type == 1) {
+                    const int c = fwd[tk.idx].c;
+                    ok = solve_wait(fdiag + c, need[cb + c], err);
+                } else if (type == 2) {
+                    for (int k = tgt_ptr[tk.idx]; ok && k < tgt_ptr[tk.idx + 1]; k++) {
+                        const int f = tgt[k];
+                        ok = solve_wait(bdiag + f, need[cb + f], err);
+                    }
+                } else {
+                    const int c = diag[tk.idx].c;
+                    ok = solve_wait(fdiag + c, need[cb + c], err) && solve_wait(bprod + c, need[2 * cb + c], err);
+                }
+            }
+            sh_ok = ok ? 1 : 0;
+        }
+        __syncthreads();
+        if (!sh_ok) return;
+        const SolveTask tk = tasks[t];
+        const int type = tk.code & 3;
+        int sig = -1;
+        if (type == 0) {
+            const DiagItem it = diag[tk.idx];
+            fwd_diag_body<true, T, NR, 4>(S, it, yrow_ptr, ybuf, ldy, x, ldx, z, ldz, nrhs, w1, ysh, red);
+            sig = cb + it.c;
+        } else if (type == 1) {
+            const FwdItem it = fwd[tk.idx];
+            fwd_prod_body<true, T, NR, 4>(S, it, L, z, ldz, nrhs, ydst, ybuf, ldy, ysh, red);
+        } else if (type == 2) {
+            const BwdItem it = bwd[tk.idx];
+            bwd_prod_body<true, T, NR>(S, it, (tk.code >> 2) * 64, U, x, ldx, nrhs, part, tile, xr, red);
+            sig = 2 * cb + it.c;
+        } else {
+            const DiagItem it = diag[tk.idx];
+            bwd_diag_body<true, T, NR, 4>(S, it, bwd, z, ldz, x, ldx, nrhs, part, wb, ysh, red);
+            sig = 3 * cb + it.c;
+        }
+        // the task's hand-over data went out with device-coherent stores:
+        // once every wave has them complete, one thread signals
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) {
+            if (type == 1) {
+                for (int k = tgt_ptr[tk.idx]; k < tgt_ptr[tk.idx + 1]; k++)
+                    __hip_atomic_fetch_add(fwd_in + tgt[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_fetch_add(cnt + sig, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        t++;
+    }
 }
 
 // ------------------------------- launchers ---------------------------------
@@ -275,6 +456,26 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
                            ldx, nrhs, part, wb);
 }
 
+template <typename T>
+void launch_solve_tasks(const DevSymbol &S, const SolveTask *tasks, const int32_t *chunks, int nchunks,
+                        const DiagItem *diag, const FwdItem *fwd, const BwdItem *bwd, const int32_t *tgt_ptr,
+                        const int32_t *tgt, const int32_t *need, int32_t *cnt, int cb, const int64_t *yrow_ptr,
+                        T *ybuf, int64_t ldy, T *x, int64_t ldx, T *z, int64_t ldz, int nrhs, const T *L, const T *U,
+                        const T *w1, const T *wb, const int32_t *ydst, T *part, int maxwidth, int nwg,
+                        hipStream_t st) {
+    if (nchunks <= 0) return;
+    const int g = std::max(1, std::min(nwg, nchunks));
+    // the right-hand sides of one launch times the widest cblk stay within
+    // the level kernels' 24 KiB bound, so this stays below 33 KiB
+    const size_t sh = (size_t)std::max(maxwidth * nrhs, 64 * 65) * sizeof(T);
+    if (nrhs == 1)
+        hipLaunchKernelGGL((k_solve_tasks<T, 1>), dim3(g), dim3(256), sh, st, S, tasks, chunks, nchunks, diag, fwd, bwd, tgt_ptr,
+                           tgt, need, cnt, cb, yrow_ptr, ybuf, ldy, x, ldx, z, ldz, nrhs, L, U, w1, wb, ydst, part);
+    else
+        hipLaunchKernelGGL((k_solve_tasks<T, 8>), dim3(g), dim3(256), sh, st, S, tasks, chunks, nchunks, diag, fwd, bwd, tgt_ptr,
+                           tgt, need, cnt, cb, yrow_ptr, ybuf, ldy, x, ldx, z, ldz, nrhs, L, U, w1, wb, ydst, part);
+}
+
 #define INST(T)                                                                                                   \
     template void launch_fwd_diag<T>(const DevSymbol &, const DiagItem *, int, const int64_t *, const T *,       \
                                      int64_t, const T *, int64_t, T *, int64_t, int, const T *, int,             \
@@ -287,5 +488,15 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
                                      int64_t, T *, int64_t, int, const T *, const T *, int, hipStream_t);
 INST(double)
 INST(float)
+
+#define INST_TASKS(T)                                                                                             \
+    template void launch_solve_tasks<T>(const DevSymbol &, const SolveTask *, const int32_t *, int,              \
+                                        const DiagItem *,                                                        \
+                                        const FwdItem *, const BwdItem *, const int32_t *, const int32_t *,      \
+                                        const int32_t *, int32_t *, int, const int64_t *, T *, int64_t, T *,     \
+                                        int64_t, T *, int64_t, int, const T *, const T *, const T *, const T *,  \
+                                        const int32_t *, T *, int, int, hipStream_t);
+INST_TASKS(double)
+INST_TASKS(float)
 
 }  // namespace pxa

@@ -78,7 +78,7 @@ class error:
 
 
 class amd_iparm:
-    factor_precision, device, level_order, resident_input, profile, graph, overlap = 0, 1, 2, 3, 4, 5, 6
+    factor_precision, device, level_order, resident_input, profile, graph, overlap, task_solve = 0, 1, 2, 3, 4, 5, 6, 7
 
 
 class PastixError(RuntimeError):
