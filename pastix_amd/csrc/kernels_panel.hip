@@ -29,6 +29,38 @@ namespace pxa {
 
 template <typename T> __device__ __forceinline__ T pabs(T v) { return v < (T)0 ? -v : v; }
 
+// Square root and its reciprocal of a positive pivot, and reciprocals, from
+// the hardware estimates refined by Newton steps (y <- y + y(1/2 - d y^2/2)
+// and y <- y + y(1 - d y)), each step doubling the correct bits, until about
+// one ulp: fewer dependent instructions than the correctly rounded sqrt and
+// division on the diagonal block's critical path.
+__device__ __forceinline__ void sqrt_rsqrt(double d, double &L, double &r) {
+    double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const double h = 0.5 * d * y;
+        y = __builtin_fma(y, __builtin_fma(-h, y, 0.5), y);
+    }
+    r = y;
+    L = d * y;
+}
+__device__ __forceinline__ void sqrt_rsqrt(float d, float &L, float &r) {
+    float y = __builtin_amdgcn_rsqf(d);
+    const float h = 0.5f * d * y;
+    y = __builtin_fmaf(y, __builtin_fmaf(-h, y, 0.5f), y);
+    r = y;
+    L = d * y;
+}
+__device__ __forceinline__ double recip(double d) {
+    double y = __builtin_amdgcn_rcp(d);
+    y = __builtin_fma(y, __builtin_fma(-d, y, 1.0), y);
+    return __builtin_fma(y, __builtin_fma(-d, y, 1.0), y);
+}
+__device__ __forceinline__ float recip(float d) {
+    const float y = __builtin_amdgcn_rcpf(d);
+    return __builtin_fmaf(y, __builtin_fmaf(-d, y, 1.0f), y);
+}
+
 // Shared by the inverses: column j of inv(M) for the lower-triangular 64x64
 // M held in Ls (M(r, k) = Ls[r][k] when TRANS is false, Ls[k][r] when true;
 // diagonal 1 when UNIT), by the four lanes 4*jl+q of a wave, x[kk] =
@@ -152,16 +184,20 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
         __syncthreads();
         // 4x4 Cholesky of the diagonal block (same in every thread)
         const T d0 = piv(cb[0][k], k);
-        const T L00 = sqrt(d0), i00 = (T)1 / L00;
+        T L00, i00;
+        sqrt_rsqrt(d0, L00, i00);
         const T L10 = cb[0][k + 1] * i00, L20 = cb[0][k + 2] * i00, L30 = cb[0][k + 3] * i00;
         const T d1 = piv(cb[1][k + 1] - L10 * L10, k + 1);
-        const T L11 = sqrt(d1), i11 = (T)1 / L11;
+        T L11, i11;
+        sqrt_rsqrt(d1, L11, i11);
         const T L21 = (cb[1][k + 2] - L20 * L10) * i11, L31 = (cb[1][k + 3] - L30 * L10) * i11;
         const T d2 = piv(cb[2][k + 2] - L20 * L20 - L21 * L21, k + 2);
-        const T L22 = sqrt(d2), i22 = (T)1 / L22;
+        T L22, i22;
+        sqrt_rsqrt(d2, L22, i22);
         const T L32 = (cb[2][k + 3] - L30 * L20 - L31 * L21) * i22;
         const T d3 = piv(cb[3][k + 3] - L30 * L30 - L31 * L31 - L32 * L32, k + 3);
-        const T L33 = sqrt(d3), i33 = (T)1 / L33;
+        T L33, i33;
+        sqrt_rsqrt(d3, L33, i33);
         // multipliers of row i
         T l0, l1, l2, l3;
         if (i > k + 3) {
@@ -201,7 +237,7 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
         if (neg) atomicOr(&stat[1], 1);
     }
     __syncthreads();
-    if (tid < 64) dinv[tid] = (T)1 / Ls[tid][tid];
+    if (tid < 64) dinv[tid] = recip(Ls[tid][tid]);
     __syncthreads();
     // column j of inv(L): lanes 4*jl + q of wave w, x[kk] = X(4*kk + q, j)
     const int q = tid & 3, j = 16 * w + ((tid >> 2) & 15);
@@ -395,16 +431,16 @@ __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__r
         }
         __syncthreads();
         // D(t, u) = cb[u][k + t]
-        const T u00 = piv(cb[0][k], k), r0 = (T)1 / u00;
+        const T u00 = piv(cb[0][k], k), r0 = recip(u00);
         const T l10 = cb[0][k + 1] * r0, l20 = cb[0][k + 2] * r0, l30 = cb[0][k + 3] * r0;
         const T u01 = cb[1][k], u02 = cb[2][k], u03 = cb[3][k];
-        const T u11 = piv(cb[1][k + 1] - l10 * u01, k + 1), r1 = (T)1 / u11;
+        const T u11 = piv(cb[1][k + 1] - l10 * u01, k + 1), r1 = recip(u11);
         const T u12 = cb[2][k + 1] - l10 * u02, u13 = cb[3][k + 1] - l10 * u03;
         const T l21 = (cb[1][k + 2] - l20 * u01) * r1, l31 = (cb[1][k + 3] - l30 * u01) * r1;
-        const T u22 = piv(cb[2][k + 2] - l20 * u02 - l21 * u12, k + 2), r2 = (T)1 / u22;
+        const T u22 = piv(cb[2][k + 2] - l20 * u02 - l21 * u12, k + 2), r2 = recip(u22);
         const T u23 = cb[3][k + 2] - l20 * u03 - l21 * u13;
         const T l32 = (cb[2][k + 3] - l30 * u02 - l31 * u12) * r2;
-        const T u33 = piv(cb[3][k + 3] - l30 * u03 - l31 * u13 - l32 * u23, k + 3), r3 = (T)1 / u33;
+        const T u33 = piv(cb[3][k + 3] - l30 * u03 - l31 * u13 - l32 * u23, k + 3), r3 = recip(u33);
         if (i > k + 3) {
             const T m0 = cb[0][i] * r0;
             const T m1 = (cb[1][i] - m0 * u01) * r1;
@@ -451,7 +487,7 @@ __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__r
     }
     if (tid == 0 && npiv) atomicAdd(&stat[0], npiv);
     __syncthreads();
-    if (tid < 64) dinv[tid] = (T)1 / Ls[tid][tid];
+    if (tid < 64) dinv[tid] = recip(Ls[tid][tid]);
     __syncthreads();
     const int q = tid & 3, j = 16 * w + ((tid >> 2) & 15);
     T x1[16], x2[16];
