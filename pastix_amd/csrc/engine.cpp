@@ -255,19 +255,22 @@ template <typename T> int Engine::factorize_t(double thr) {
             const int b0 = step_cb_ptr_[s], nb = step_cb_ptr_[s + 1] - b0;
             const int p0 = step_trsm_ptr_[s], np = step_trsm_ptr_[s + 1] - p0;
             const int q0 = step_upd_ptr_[s], nq = step_upd_ptr_[s + 1] - q0;
+            // the step's column block first takes the products of the blocks
+            // before it (left-looking), then its diagonal sub-block is
+            // factored and the rows below are solved against it
             if (facto_ == 2) {
-                // rows below the sub-block: L side times inv(U_kk), U side
-                // (stored transposed) times inv(L_kk)^T; trailing updates
-                // L -= L U^T and U^T -= U^T L^T over the same tiles
+                // L -= L U^T and U^T -= U^T L^T on the block's tiles; rows
+                // below the sub-block: L side times inv(U_kk), U side (stored
+                // transposed) times inv(L_kk)^T
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, U, false, stream_);
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, U, U, L, true, stream_);
                 launch_getrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, U, W1, W2, (T *)d_WB_, thr, d_stat_, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W2, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, U, W1, stream_);
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, U, false, stream_);
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, U, U, L, true, stream_);
             } else {
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, L, false, stream_);
                 launch_potrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, W1, (T *)d_WB_, thr, d_stat_, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W1, stream_);
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, L, false, stream_);
             }
         }
         prof_end(KC_PANEL, pk, stream_);

@@ -1,14 +1,15 @@
 // Blocked right-looking panel factorisation for CDNA4 (gfx950).
 //
 // A cblk's panel (stride x K, column-major) is factored by 64-column steps,
-// each step being three launches over all cblks of the level:
+// each step being (at most) three launches over all cblks of the level:
 //   k_potrf64   : Cholesky of the 64x64 diagonal sub-block with static
 //                 pivoting (reference kernels/core_zpotrfsp.c:66-108), the
 //                 block in registers, plus its inverse;
 //   k_panel_trsm: every panel row below the sub-block times inv(L_kk)^T
 //                 (core_zpotrfsp.c:166-176 and core_ztrsmsp.c:147-181);
-//   k_panel_upd : trailing update of the panel's remaining columns
-//                 (core_zpotrfsp.c:178-185), lower 64x64 tiles only.
+//   k_panel_upd : before the step, left-looking update of the step's
+//                 column block by the blocks before it (core_zpotrfsp.c:178-185
+//                 applied column block by column block).
 // k_winv_offdiag completes the inverse of the whole diagonal block for the
 // solve once the panel is factored.  Inverses are stored twice: W (lower,
 // column-major) for the forward sweep and WT = W^T for the backward sweep.
@@ -138,6 +139,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
                                                  T *__restrict__ coef, T *__restrict__ winv, T *__restrict__ winvt,
                                                  double thr, int *stat) {
+    __builtin_amdgcn_s_setprio(3);  // panel steps lead the critical path
     __shared__ T colb[2][4][64];
     __shared__ T lb[4][64];
     __shared__ T Ls[64][65];
@@ -262,6 +264,7 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
                                                     int kb, T *__restrict__ coef, const T *__restrict__ winv) {
+    __builtin_amdgcn_s_setprio(3);
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -288,38 +291,42 @@ __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem
     }
 }
 
-// trailing panel update: C[r0+i, c0+j] -= X[r0+i, kb] . Y[c0+j, kb] (C, X, Y
-// panels of the same cblk: all L for LL^t; L, L, U and U, U, L for the two
-// sides of LU)
-// On the tiles that hold the diagonal (r0 == c0) only r >= c (strict = 0)
-// or r > c (strict = 1) is written: the L side of LU keeps the lower part and
-// the diagonal, the U side (transposed) the strict upper part.
+// Left-looking update of the panel's column block kb (columns [k0, k0+64),
+// k0 = 64*kb) by all the blocks before it, before that block is factored:
+//   C[r, k0+j] -= sum_{k < k0} X[r, k] . Y[k0+j, k]   for the 64-row tile r0
+// (C, X, Y panels of the same cblk: all L for LL^t; L, L, U and U, U, L for
+// the two sides of LU; the reference's blocked potrf/getrf applies the same
+// products step by step, kernels/core_zpotrfsp.c:178-185).  Every element of
+// the block is read and written once, with K = k0 staged through LDS.  On the
+// diagonal tile (r0 == k0) only r >= c (strict = 0) or r > c (strict = 1) is
+// written: the L side of LU keeps the lower part and the diagonal, the U side
+// (transposed) the strict upper part.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
                                                    int kb, T *__restrict__ ccoef, const T *__restrict__ xcoef,
                                                    const T *__restrict__ ycoef, int strict) {
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ StageBuf<T, 16> sb;
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
     const PanelItem it = items[id];
-    const int c = it.c, r0 = it.r0, c0 = it.c0;
+    const int c = it.c, r0 = it.r0;
     const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
     const int stride = S.cblk_stride[c];
     const int64_t ld = stride;
-    const int k0 = kb * 64, nb = min(64, K - k0);
+    const int k0 = kb * 64;
     T *P = ccoef + S.cblk_coef[c];
     const T *X = xcoef + S.cblk_coef[c], *Y = ycoef + S.cblk_coef[c];
-    const int mr = min(64, stride - r0), mc = min(64, K - c0);
+    const int mr = min(64, stride - r0), mc = min(64, K - k0);
+    Quad<T> q;
+    q.zero();
+    wg_gemm_lds<T, 16>(q, X + r0, mr, ld, Y + k0, mc, ld, k0, sb, tid);
     if (wr < mr && wc < mc) {
-        Quad<T> q;
-        q.zero();
-        View<T> A{X + (r0 + wr) + (int64_t)k0 * ld, 1, ld};
-        View<T> B{Y + (c0 + wc) + (int64_t)k0 * ld, 1, ld};
-        quad_gemm_nt(q, A, mr - wr, B, mc - wc, nb, lane);
-        const bool diag = r0 == c0;
+        const bool diag = r0 == k0;
         QFOR(if (qrow < mr - wr && qcol < mc - wc && (!diag || wr + qrow > wc + qcol - (strict ? 0 : 1)))
-                 P[(r0 + wr + qrow) + (int64_t)(c0 + wc + qcol) * ld] -= q.c[m_][n_][r_];)
+                 P[(r0 + wr + qrow) + (int64_t)(k0 + wc + qcol) * ld] -= q.c[m_][n_][r_];)
     }
 }
 
@@ -387,6 +394,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
                                                  T *__restrict__ lcoef, T *__restrict__ ucoef, T *__restrict__ w1,
                                                  T *__restrict__ w2, T *__restrict__ wb, double thr, int *stat) {
+    __builtin_amdgcn_s_setprio(3);
     __shared__ T colb[2][4][64], rowb[2][4][64];
     __shared__ T Ls[64][65];
     __shared__ T dinv[64];

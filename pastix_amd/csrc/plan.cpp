@@ -233,9 +233,10 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                 const int64_t nb = std::min<int64_t>(64, K - k0);
                 P.step_cblks.push_back(c);
                 for (int64_t r0 = k0 + nb; r0 < st; r0 += 64) P.ptrsm.push_back(PanelItem{c, (int32_t)r0, 0, 0});
-                for (int64_t c0 = k0 + nb; c0 < K; c0 += 64)
-                    for (int64_t r0 = c0; r0 < st; r0 += 64)
-                        P.pupd.push_back(PanelItem{c, (int32_t)r0, (int32_t)c0, 0});
+                // left-looking update of this column block by the blocks
+                // before it, from the diagonal tile down
+                if (kb > 0)
+                    for (int64_t r0 = k0; r0 < st; r0 += 64) P.pupd.push_back(PanelItem{c, (int32_t)r0, (int32_t)k0, 0});
             }
             P.step_cb_ptr.push_back((int32_t)P.step_cblks.size());
             P.step_trsm_ptr.push_back((int32_t)P.ptrsm.size());

@@ -17,6 +17,9 @@ struct Plan {
     // step s is column block kb = s - lv_step_ptr[L] of the level's cblks
     std::vector<int32_t> lv_step_ptr;
     std::vector<int32_t> step_cb_ptr, step_cblks;
+    // per step: panel-TRSM row tiles below the sub-block, and (from the
+    // second step on) the row tiles, from the diagonal tile down, of the
+    // left-looking update of the step's column block by the blocks before it
     std::vector<int32_t> step_trsm_ptr, step_upd_ptr;
     std::vector<PanelItem> ptrsm, pupd;
     std::vector<int32_t> wide_cblks;  // cblks wider than 64 columns
