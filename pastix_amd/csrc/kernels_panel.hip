@@ -278,14 +278,11 @@ __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem
     T *P = coef + S.cblk_coef[c];
     const T *W = winv + S.cblk_winv[c];
     const int mr = min(64, stride - r0);
+    __shared__ StageBuf<T, 16> sb;
     Quad<T> q;
     q.zero();
-    if (wr < mr && wc < nb) {
-        View<T> A{P + (r0 + wr) + (int64_t)k0 * ld, 1, ld};
-        View<T> B{W + (k0 + wc) + (int64_t)k0 * K, 1, K};
-        quad_gemm_nt(q, A, mr - wr, B, nb - wc, nb, lane);
-    }
-    __syncthreads();
+    // the row tile is read completely (staged through LDS) before it is overwritten
+    wg_gemm_lds<T, 16>(q, P + r0 + (int64_t)k0 * ld, mr, ld, W + k0 + (int64_t)k0 * K, nb, K, nb, sb, tid);
     if (wr < mr && wc < nb) {
         QFOR(if (qrow < mr - wr && qcol < nb - wc) P[(r0 + wr + qrow) + (int64_t)(k0 + wc + qcol) * ld] = q.c[m_][n_][r_];)
     }
