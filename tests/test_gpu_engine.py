@@ -311,3 +311,29 @@ def test_overlap_matches_single_stream(facto):
         if facto == LU:
             assert np.array_equal(U0, U1), f"graph={graph} (U)"
     pastix.finalize(h)
+
+
+def test_static_pivot_on_device():
+    """A zero LU pivot (the first eliminated unknown's diagonal) is replaced by
+    the threshold and counted on the device exactly as the oracle does
+    (kernels/core_zgetf2sp.c:80-88); the rest of the factor follows from the
+    replaced value."""
+    facto = LU
+    grid = (6, 6, 5)
+    A = spm.convection_diffusion27(*grid, seed=1)
+    h, S, sym = analyse(A, False, facto, grid=grid, min_bs=8, max_bs=16)
+    v = sym["peritab"][0]
+    A2 = A.copy().tocsc()
+    A2.sort_indices()
+    k = A2.indptr[v] + np.searchsorted(A2.indices[A2.indptr[v]:A2.indptr[v + 1]], v)
+    A2.data[k] = 0.0
+    S2 = spm.Spm(A2, symmetric=False)
+    thr = 1e-3
+    h.dparm[pastix.dparm.epsilon_magn_ctrl] = -thr  # negative: absolute threshold
+    pastix.task_numfact(h, S2)
+    assert h.iparm[pastix.iparm.static_pivoting] == 1
+    Lg, Ug = pastix.coeftab(h, lu=True)
+    O, bc, Lo, Uo = oracle_factor(A2, sym, lu=True, crit=thr)
+    assert rel_frob(Lg, Lo) <= 1e-12 and rel_frob(Ug, Uo) <= 1e-12
+    assert Lg[0] == pytest.approx(thr, rel=1e-15)
+    pastix.finalize(h)
