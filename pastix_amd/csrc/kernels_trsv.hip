@@ -172,11 +172,14 @@ __device__ __forceinline__ void bwd_prod_body(const DevSymbol &S, const BwdItem 
     wave_sum_store<CO, T, NR, 4>(acc, red, lane, q, k0 + lane < Kc, nrhs, out, Kc, (const T *)nullptr, 0);
 }
 
+// nrhs right-hand sides starting at column u0 of the partials, which are laid
+// out for nrhs_tot columns (z and x point at column u0 already)
 template <bool CO, typename T, int NR, int NW>
 __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem it,
                                               const BwdItem *__restrict__ bitems, const T *__restrict__ z,
-                                              int64_t ldz, T *__restrict__ x, int64_t ldx, int nrhs,
-                                              const T *__restrict__ part, const T *__restrict__ wb, T *y, T *red) {
+                                              int64_t ldz, T *__restrict__ x, int64_t ldx, int nrhs, int nrhs_tot,
+                                              int u0, const T *__restrict__ part, const T *__restrict__ wb, T *y,
+                                              T *red) {
     const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
     const int c = it.c, r0 = it.ib * 64;
     const int fc = S.cblk_fcol[c];
@@ -191,7 +194,7 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
         if (k < Kc) {
 #pragma unroll 4
             for (int j = it.b0 + q; j < it.b1; j += NW) {
-                const T *pp = part + (s0 + (int64_t)(j - it.b0) * Kc) * nrhs + k;
+                const T *pp = part + (s0 + (int64_t)(j - it.b0) * Kc) * nrhs_tot + (int64_t)u0 * Kc + k;
 #pragma unroll
                 for (int u = 0; u < NR; u++)
                     if (u < nrhs) acc[u] += ld_co<CO>(pp + u * Kc);
@@ -230,8 +233,10 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
     __shared__ T red[NW * 64 * NR];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
-    fwd_diag_body<false, T, NR, NW>(S, items[id], yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1,
-                             reinterpret_cast<T *>(smem_raw), red);
+    // right-hand sides [NR*blockIdx.y, NR*blockIdx.y + NR)
+    const int u0 = NR * blockIdx.y;
+    fwd_diag_body<false, T, NR, NW>(S, items[id], yrow_ptr, ybuf + u0 * ldy, ldy, b + u0 * ldb, ldb, z + u0 * ldz,
+                                    ldz, min(NR, nrhs - u0), w1, reinterpret_cast<T *>(smem_raw), red);
 }
 
 template <typename T, int NR, int NW>
@@ -270,8 +275,10 @@ __global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagIte
     __shared__ T red[NW * 64 * NR];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
-    bwd_diag_body<false, T, NR, NW>(S, items[id], bitems, z, ldz, x, ldx, nrhs, part, wb, reinterpret_cast<T *>(smem_raw),
-                             red);
+    // right-hand sides [NR*blockIdx.y, NR*blockIdx.y + NR)
+    const int u0 = NR * blockIdx.y;
+    bwd_diag_body<false, T, NR, NW>(S, items[id], bitems, z + u0 * ldz, ldz, x + u0 * ldx, ldx, min(NR, nrhs - u0),
+                                    nrhs, u0, part, wb, reinterpret_cast<T *>(smem_raw), red);
 }
 
 // --------------------- dependency-driven solve (one launch) -----------------
@@ -386,7 +393,7 @@ __global__ __launch_bounds__(256) void k_solve_tasks(DevSymbol S, const SolveTas
             sig = 2 * cb + it.c;
         } else {
             const DiagItem it = diag[tk.idx];
-            bwd_diag_body<true, T, NR, 4>(S, it, bwd, z, ldz, x, ldx, nrhs, part, wb, ysh, red);
+            bwd_diag_body<true, T, NR, 4>(S, it, bwd, z, ldz, x, ldx, nrhs, nrhs, 0, part, wb, ysh, red);
             sig = 3 * cb + it.c;
         }
         // the task's hand-over data went out with device-coherent stores:
@@ -412,13 +419,15 @@ void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int
                      int64_t ldy, const T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
                      hipStream_t st) {
     if (n <= 0) return;
-    size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
+    // several right-hand sides: pairs of columns on the grid's y dimension,
+    // so that the few diagonal blocks of the top levels use more of the chip
     if (nrhs == 1)
-        hipLaunchKernelGGL((k_fwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), sh, st, S, items, n, yrow_ptr, ybuf,
-                           ldy, b, ldb, z, ldz, nrhs, w1);
+        hipLaunchKernelGGL((k_fwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st, S,
+                           items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
     else
-        hipLaunchKernelGGL((k_fwd_diag<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, yrow_ptr, ybuf, ldy,
-                           b, ldb, z, ldz, nrhs, w1);
+        hipLaunchKernelGGL((k_fwd_diag<T, 2, 16>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
+                           (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz,
+                           nrhs, w1);
 }
 template <typename T>
 void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *z, int64_t ldz,
@@ -447,13 +456,12 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
                      int64_t ldz, T *x, int64_t ldx, int nrhs, const T *part, const T *wb, int maxwidth,
                      hipStream_t st) {
     if (n <= 0) return;
-    size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
     if (nrhs == 1)
-        hipLaunchKernelGGL((k_bwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), sh, st, S, items, n, bitems, z, ldz, x,
-                           ldx, nrhs, part, wb);
+        hipLaunchKernelGGL((k_bwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st, S,
+                           items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
     else
-        hipLaunchKernelGGL((k_bwd_diag<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, bitems, z, ldz, x,
-                           ldx, nrhs, part, wb);
+        hipLaunchKernelGGL((k_bwd_diag<T, 2, 16>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
+                           (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
 }
 
 template <typename T>
