@@ -162,6 +162,11 @@ int pastix_amd_batch_solve(pastix_data_t *pastix_data, const pastix_int_t *nrhs,
  * largest final ||r||/||b||. */
 int pastix_amd_batch_refine(pastix_data_t *pastix_data, const pastix_int_t *nrhs, void *const *b,
                             const pastix_int_t *ldb, void *const *x, const pastix_int_t *ldx);
+/* Both in one: x[s] receives the solution of one multi-RHS solve in the
+ * factor precision refined by the batched CG above (b[s] is left as is);
+ * the right-hand sides cross to the device once. */
+int pastix_amd_batch_solve_refine(pastix_data_t *pastix_data, const pastix_int_t *nrhs, void *const *b,
+                                  const pastix_int_t *ldb, void *const *x, const pastix_int_t *ldx);
 /* Offsets of the systems in the union (nsys+1 entries, may be NULL);
  * returns nsys or PASTIX_ERR_BADPARAMETER. */
 int pastix_amd_batch_offsets(const pastix_data_t *pastix_data, int64_t *off);

@@ -295,6 +295,28 @@ extern "C" int pastix_amd_batch_refine(pastix_data_t *d, const pastix_int_t *nrh
     return PASTIX_SUCCESS;
 }
 
+extern "C" int pastix_amd_batch_solve_refine(pastix_data_t *d, const pastix_int_t *nrhs, void *const *b,
+                                             const pastix_int_t *ldb, void *const *x, const pastix_int_t *ldx) {
+    if (!d || !nrhs || !b || !ldb || !x || !ldx || d->batch_off.size() < 2) return PASTIX_ERR_BADPARAMETER;
+    if (!(d->steps & STEP_NUMFACT) || !d->engine) return PASTIX_ERR_BADPARAMETER;
+    if (d->iparm[IPARM_REFINEMENT] != PastixRefineCG) {
+        err_print("pastix_amd_batch_solve_refine", "batched refinement is conjugate gradient (IPARM_REFINEMENT = PastixRefineCG)");
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    }
+    pastix_int_t R = 0, Rx = 0;
+    if (!rhs_ok(d, nrhs, b, ldb, R) || !rhs_ok(d, nrhs, x, ldx, Rx)) return PASTIX_ERR_BADPARAMETER;
+    if (R == 0) return PASTIX_SUCCESS;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> B, X(d->batch_off.back() * (size_t)R);
+    pack(d->batch_off, R, nrhs, b, ldb, B);
+    const int rc = refine_block(d, (int)R, B.data(), X.data(), d->batch_off, true, 1);
+    if (rc) return rc;
+    unpack(d->batch_off, nrhs, X, x, ldx);
+    d->steps |= STEP_SOLVE;
+    d->dparm[DPARM_REFINE_TIME] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return PASTIX_SUCCESS;
+}
+
 extern "C" int pastix_amd_batch_offsets(const pastix_data_t *d, int64_t *off) {
     if (!d || d->batch_off.size() < 2) return PASTIX_ERR_BADPARAMETER;
     if (off)

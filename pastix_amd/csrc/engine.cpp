@@ -180,9 +180,10 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     d_stat_ = (int *)alloc(4 * sizeof(int));
     d_WB_ = alloc(winvnbr_ * fsz);
     if (d_WB_) HIPCHK(hipMemsetAsync(d_WB_, 0, winvnbr_ * fsz, stream_));
-    // nrhs chunk: at most 8 (register accumulators of the solve kernels) and
-    // bounded by their dynamic LDS (24 KiB)
-    nrhs_chunk_ = (int)std::max<int64_t>(1, std::min<int64_t>(8, (24 * 1024) / (std::max(maxwidth_, 256) * (int64_t)fsz)));
+    // right-hand sides of one sweep: up to 32, the kernels taking them in
+    // groups of 2 or 8 on the grid (the factor is read once per sweep); the
+    // dependency-driven solve runs 8 at a time
+    nrhs_chunk_ = 32;
     d_part_ = alloc((size_t)std::max<int64_t>(bwd_part_, 1) * nrhs_chunk_ * fsz);
     d_ybuf_ = alloc((size_t)std::max<int64_t>(nyrows_, 1) * nrhs_chunk_ * fsz);
     d_z_ = alloc((size_t)std::max<int64_t>(n_, 1) * nrhs_chunk_ * fsz);
@@ -522,8 +523,9 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
     T *z = (T *)d_z_;
     const int64_t ldy = std::max<int64_t>(nyrows_, 1), ldz = std::max<int64_t>(n_, 1);
     const int nl = (int)lv_cblk_ptr_.size() - 1;
-    for (int r0 = 0; r0 < nrhs; r0 += nrhs_chunk_) {
-        const int nr = std::min(nrhs_chunk_, nrhs - r0);
+    const int chunk = (task_solve && n_stasks_ > 0) ? std::min(nrhs_chunk_, 8) : nrhs_chunk_;
+    for (int r0 = 0; r0 < nrhs; r0 += chunk) {
+        const int nr = std::min(chunk, nrhs - r0);
         T *xx = x + (int64_t)r0 * ldx;
         if (task_solve && n_stasks_ > 0) {
             // counters and queue head to zero (a kernel, so that graph

@@ -141,10 +141,13 @@ __device__ __forceinline__ void fwd_prod_body(const DevSymbol &S, const FwdItem 
 }
 
 // one 64x64 tile (column block k0/64) of the backward products, 4 waves
+// nrhs right-hand sides starting at column u0 of the partials (laid out for
+// nrhs_tot columns); x points at column u0 already
 template <bool CO, typename T, int NR>
 __device__ __forceinline__ void bwd_prod_body(const DevSymbol &S, const BwdItem it, int k0,
                                               const T *__restrict__ coef, const T *__restrict__ x, int64_t ldx,
-                                              int nrhs, T *__restrict__ part, T (*tile)[65], T *xr, T *red) {
+                                              int nrhs, int nrhs_tot, int u0, T *__restrict__ part, T (*tile)[65],
+                                              T *xr, T *red) {
     const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
     const int c = it.c;
     const int Kc = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
@@ -168,7 +171,7 @@ __device__ __forceinline__ void bwd_prod_body(const DevSymbol &S, const BwdItem 
 #pragma unroll
         for (int u = 0; u < NR; u++) acc[u] += a * xr[row + u * 64];
     }
-    T *out = part + (int64_t)it.slot * nrhs + k0 + lane;
+    T *out = part + (int64_t)it.slot * nrhs_tot + (int64_t)u0 * Kc + k0 + lane;
     wave_sum_store<CO, T, NR, 4>(acc, red, lane, q, k0 + lane < Kc, nrhs, out, Kc, (const T *)nullptr, 0);
 }
 
@@ -248,7 +251,10 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_prod(DevSymbol S, const FwdItem
     __shared__ T red[NW * 64 * NR];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
-    fwd_prod_body<false, T, NR, NW>(S, items[id], coef, z, ldz, nrhs, ydst, ybuf, ldy, reinterpret_cast<T *>(smem_raw), red);
+    // right-hand sides [NR*blockIdx.y, NR*blockIdx.y + NR)
+    const int u0 = NR * blockIdx.y;
+    fwd_prod_body<false, T, NR, NW>(S, items[id], coef, z + u0 * ldz, ldz, min(NR, nrhs - u0), ydst, ybuf + u0 * ldy,
+                                    ldy, reinterpret_cast<T *>(smem_raw), red);
 }
 
 template <typename T, int NR>
@@ -263,7 +269,10 @@ __global__ __launch_bounds__(256) void k_bwd_prod(DevSymbol S, const BwdItem *__
     const BwdItem it = items[id];
     const int k0 = blockIdx.y * 64;
     if (k0 >= S.cblk_fcol[it.c + 1] - S.cblk_fcol[it.c]) return;
-    bwd_prod_body<false, T, NR>(S, it, k0, coef, x, ldx, nrhs, part, tile, xr, red);
+    // right-hand sides [NR*blockIdx.z, NR*blockIdx.z + NR)
+    const int u0 = NR * blockIdx.z;
+    bwd_prod_body<false, T, NR>(S, it, k0, coef, x + u0 * ldx, ldx, min(NR, nrhs - u0), nrhs, u0, part, tile, xr,
+                                red);
 }
 
 template <typename T, int NR, int NW>
@@ -389,7 +398,7 @@ __global__ __launch_bounds__(256) void k_solve_tasks(DevSymbol S, const SolveTas
             fwd_prod_body<true, T, NR, 4>(S, it, L, z, ldz, nrhs, ydst, ybuf, ldy, ysh, red);
         } else if (type == 2) {
             const BwdItem it = bwd[tk.idx];
-            bwd_prod_body<true, T, NR>(S, it, (tk.code >> 2) * 64, U, x, ldx, nrhs, part, tile, xr, red);
+            bwd_prod_body<true, T, NR>(S, it, (tk.code >> 2) * 64, U, x, ldx, nrhs, nrhs, 0, part, tile, xr, red);
             sig = 2 * cb + it.c;
         } else {
             const DiagItem it = diag[tk.idx];
@@ -433,19 +442,18 @@ template <typename T>
 void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *z, int64_t ldz,
                      int nrhs, const int32_t *ydst, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st) {
     if (n <= 0) return;
-    size_t sh = (size_t)maxwidth * nrhs * sizeof(T);
     if (nrhs == 1)
-        hipLaunchKernelGGL((k_fwd_prod<T, 1, 16>), dim3(grid8(n)), dim3(1024), sh, st, S, items, n, coef, z, ldz,
-                           nrhs, ydst, ybuf, ldy);
+        hipLaunchKernelGGL((k_fwd_prod<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st, S,
+                           items, n, coef, z, ldz, nrhs, ydst, ybuf, ldy);
     else
-        hipLaunchKernelGGL((k_fwd_prod<T, 8, 8>), dim3(grid8(n)), dim3(512), sh, st, S, items, n, coef, z, ldz, nrhs,
-                           ydst, ybuf, ldy);
+        hipLaunchKernelGGL((k_fwd_prod<T, 8, 8>), dim3(grid8(n), (nrhs + 7) / 8), dim3(512),
+                           (size_t)maxwidth * 8 * sizeof(T), st, S, items, n, coef, z, ldz, nrhs, ydst, ybuf, ldy);
 }
 template <typename T>
 void launch_bwd_prod(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
                      int nrhs, T *part, int maxwidth, hipStream_t st) {
     if (n <= 0) return;
-    const dim3 g(grid8(n), (maxwidth + 63) / 64);
+    const dim3 g(grid8(n), (maxwidth + 63) / 64, nrhs == 1 ? 1 : (nrhs + 7) / 8);
     if (nrhs == 1)
         hipLaunchKernelGGL((k_bwd_prod<T, 1>), g, dim3(256), 0, st, S, items, n, coef, x, ldx, nrhs, part);
     else

@@ -333,6 +333,8 @@ lib.pastix_amd_batch_solve.argtypes = [_P, _i64p, _vp, _i64p]
 lib.pastix_amd_batch_solve.restype = ct.c_int
 lib.pastix_amd_batch_refine.argtypes = [_P, _i64p, _vp, _i64p, _vp, _i64p]
 lib.pastix_amd_batch_refine.restype = ct.c_int
+lib.pastix_amd_batch_solve_refine.argtypes = [_P, _i64p, _vp, _i64p, _vp, _i64p]
+lib.pastix_amd_batch_solve_refine.restype = ct.c_int
 lib.pastix_amd_batch_offsets.argtypes = [_P, _i64p]
 lib.pastix_amd_batch_offsets.restype = ct.c_int
 lib.pastix_amd_kernel_times.argtypes = [_P, _dp]
@@ -417,6 +419,18 @@ def batch_refine(pastix_data, bs, xs):
     pb = _ptr_array([b.ctypes.data for b in bl])
     px = _ptr_array([x.ctypes.data for x in xl])
     _check("pastix_amd_batch_refine", lib.pastix_amd_batch_refine(
+        pastix_data.ptr, nr.ctypes.data_as(_i64p), pb, ldb.ctypes.data_as(_i64p), px, ldx.ctypes.data_as(_i64p)))
+    return xl
+
+
+def batch_solve_refine(pastix_data, bs):
+    """Solve + batched CG for every system in one call (right-hand sides cross
+    to the device once); returns the solutions."""
+    bl, nr, ldb = _rhs_lists(bs)
+    xl, _, ldx = _rhs_lists([np.zeros_like(b) for b in bs])
+    pb = _ptr_array([b.ctypes.data for b in bl])
+    px = _ptr_array([x.ctypes.data for x in xl])
+    _check("pastix_amd_batch_solve_refine", lib.pastix_amd_batch_solve_refine(
         pastix_data.ptr, nr.ctypes.data_as(_i64p), pb, ldb.ctypes.data_as(_i64p), px, ldx.ctypes.data_as(_i64p)))
     return xl
 

@@ -63,6 +63,10 @@ def test_batch_fp32_factor_fp64_cg(graph):
     x0 = pastix.batch_solve(h, bs)
     xs = pastix.batch_refine(h, bs, x0)
     assert h.iparm[pastix.iparm.nbiter] >= 1
+    # the combined call runs the same solve and recurrence
+    xc = pastix.batch_solve_refine(h, bs)
+    for x, y in zip(xs, xc):
+        assert np.abs(x - y).max() <= 1e-13 * np.abs(x).max()
     for k, A, b, x in zip(ks, mats, bs, xs):
         for c in range(b.shape[1]):
             assert _berr(A, x[:, c], b[:, c]) <= 1e-12
