@@ -156,6 +156,79 @@ __device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int6
     }
 }
 
+// The same product for a 64 x 128 tile: q0, q1 += X[0:64, 0:K] * Y[0:128,
+// 0:K]^T, wave w owning X rows (w>>1)*32 + [0, 32) against Y rows (w&1)*64 +
+// [0, 32) (q0) and [32, 64) (q1).  Each staged X fragment feeds twice as
+// many MFMAs as in the 64 x 64 form.  Must be called by all 256 threads.
+template <typename T, int CH = 16> struct StageBuf2 {
+    T xs[CH][68];
+    T ys[CH][132];
+};
+
+template <typename T, int CH>
+__device__ __forceinline__ void wg_gemm_lds_64x128(Quad<T> &q0, Quad<T> &q1, const T *X, int mx, int64_t ldx,
+                                                   const T *Y, int my, int64_t ldy, int K, StageBuf2<T, CH> &sb,
+                                                   int tid) {
+    constexpr int NUX = CH / 4, NUY = CH / 2;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
+    const int i = lane & 15, kq = lane >> 4;
+    const int rowx = tid & 63, kkx = tid >> 6;   // X: 64 rows, 4 columns per pass
+    const int rowy = tid & 127, kky = tid >> 7;  // Y: 128 rows, 2 columns per pass
+    const bool rx = rowx < mx, ry = rowy < my;
+    T gx[NUX], gy[NUY];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NUX; u++) {
+            const int k = k0 + kkx + 4 * u;
+            gx[u] = (rx && k < K) ? X[rowx + (int64_t)k * ldx] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < NUY; u++) {
+            const int k = k0 + kky + 2 * u;
+            gy[u] = (ry && k < K) ? Y[rowy + (int64_t)k * ldy] : (T)0;
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int u = 0; u < NUX; u++) sb.xs[kkx + 4 * u][rowx] = gx[u];
+#pragma unroll
+        for (int u = 0; u < NUY; u++) sb.ys[kky + 2 * u][rowy] = gy[u];
+    };
+    const bool act0 = (wr < mx) && (wc < my), act1 = act0 && (wc + 32 < my);
+    gload(0);
+    sstore();
+    __syncthreads();
+    for (int k0 = 0; k0 < K; k0 += CH) {
+        const bool more = k0 + CH < K;
+        if (more) gload(k0 + CH);
+        if (act0) {
+#pragma unroll
+            for (int s = 0; s < CH / 4; s++) {
+                if (k0 + 4 * s < K) {
+                    const int kr = 4 * s + kq;
+                    const T a0 = sb.xs[kr][wr + i], a1 = sb.xs[kr][wr + 16 + i];
+                    const T b0 = sb.ys[kr][wc + i], b1 = sb.ys[kr][wc + 16 + i];
+                    q0.c[0][0] = Mfma<T>::mma(a0, b0, q0.c[0][0]);
+                    q0.c[0][1] = Mfma<T>::mma(a0, b1, q0.c[0][1]);
+                    q0.c[1][0] = Mfma<T>::mma(a1, b0, q0.c[1][0]);
+                    q0.c[1][1] = Mfma<T>::mma(a1, b1, q0.c[1][1]);
+                    if (act1) {
+                        const T b2 = sb.ys[kr][wc + 32 + i], b3 = sb.ys[kr][wc + 48 + i];
+                        q1.c[0][0] = Mfma<T>::mma(a0, b2, q1.c[0][0]);
+                        q1.c[0][1] = Mfma<T>::mma(a0, b3, q1.c[0][1]);
+                        q1.c[1][0] = Mfma<T>::mma(a1, b2, q1.c[1][0]);
+                        q1.c[1][1] = Mfma<T>::mma(a1, b3, q1.c[1][1]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (more) sstore();
+        __syncthreads();
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of work items so neighbouring items share its L2.  The grid
 // size must be a multiple of 8 (the map is then a bijection of [0, grid));

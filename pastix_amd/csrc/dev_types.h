@@ -49,7 +49,7 @@ struct WGroup {
     int32_t ldw, nq;
 };
 
-// One 64x64 tile of a product group: source panel rows [p0, p0+mp) x
+// One tile of a product group (up to 128 p rows x 64 q rows): source panel rows [p0, p0+mp) x
 // [q0, q0+mq), written at W[wofs + i + j*ldw].
 struct WTile {
     int32_t s, p0, q0, mp, mq, ldw;

@@ -143,31 +143,41 @@ __global__ __launch_bounds__(256) void k_update_acc(DevSymbol S, const UpdItem *
     }
 }
 
-// Phase 1 of the multi-source tiles: dense 64x64 tile of a product group in
-// the source's own row space, P(p, q) = A(p, :) . B(q, :)^T, stored to the
+// Phase 1 of the multi-source tiles: dense 128 (p) x 64 (q) tile of a
+// product group in the source's own row space (register budget held to four
+// waves per SIMD: one more than the compiler picks unbounded, no spills), P(p, q) = A(p, :) . B(q, :)^T, stored to the
 // level workspace.  Computed transposed (quadrant rows index q) so that each
 // lane's stores walk down a workspace column.
 template <typename T>
-__global__ __launch_bounds__(256) void k_update_wgemm(DevSymbol S, const WTile *__restrict__ items, int nitems,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_update_wgemm(DevSymbol S, const WTile *__restrict__ items, int nitems,
                                                       const T *__restrict__ Acoef, const T *__restrict__ Bcoef,
                                                       T *__restrict__ W) {
-    __shared__ StageBuf<T, UPD_CH> sb;
+    __shared__ StageBuf2<T, UPD_CH> sb;
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
     const WTile w = items[id];
     const int s = w.s;
     const int K = S.cblk_fcol[s + 1] - S.cblk_fcol[s];
     const int64_t ld = S.cblk_stride[s];
-    Quad<T> q;
-    q.zero();
-    wg_gemm_lds<T, UPD_CH>(q, Bcoef + S.cblk_coef[s] + w.q0, w.mq, ld, Acoef + S.cblk_coef[s] + w.p0, w.mp, ld, K, sb, tid);
+    Quad<T> q0, q1;
+    q0.zero();
+    q1.zero();
+    // 64 q rows (B) x 128 p rows (A) of the group's product
+    wg_gemm_lds_64x128<T, UPD_CH>(q0, q1, Bcoef + S.cblk_coef[s] + w.q0, w.mq, ld, Acoef + S.cblk_coef[s] + w.p0,
+                                  w.mp, ld, K, sb, tid);
     T *Wt = W + w.wofs;
     if (wr < w.mq && wc < w.mp) {
         UQFOR({
             const int qj = wr + qrow, pi = wc + qcol;
-            if (qj < w.mq && pi < w.mp) Wt[pi + (int64_t)qj * w.ldw] = q.c[m_][n_][r_];
+            if (qj < w.mq && pi < w.mp) Wt[pi + (int64_t)qj * w.ldw] = q0.c[m_][n_][r_];
+        })
+    }
+    if (wr < w.mq && wc + 32 < w.mp) {
+        UQFOR({
+            const int qj = wr + qrow, pi = wc + 32 + qcol;
+            if (qj < w.mq && pi < w.mp) Wt[pi + (int64_t)qj * w.ldw] = q1.c[m_][n_][r_];
         })
     }
 }

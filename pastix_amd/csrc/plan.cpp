@@ -208,6 +208,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     std::vector<KeyEnt> ke;
     std::vector<Range> rr, cr;
     int64_t bslot = 0;            // backward partial slots handed out so far
+    const int64_t WT_P = 128;     // source rows (p) of one product tile; 64 columns (q)
     const int32_t FWD_ROWS = 64;  // panel rows of one forward product tile
     const int32_t BWD_ROWS = 64;  // panel rows of one backward product tile
 
@@ -381,7 +382,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                     }
                     en.grp = g;
                     const int64_t base = L.blok_roff[en.ib0];
-                    for (int64_t pt = (en.p0 - base) / 64; pt <= (en.p1 - 1 - base) / 64; pt++)
+                    for (int64_t pt = (en.p0 - base) / WT_P; pt <= (en.p1 - 1 - base) / WT_P; pt++)
                         for (int64_t qt = (en.q0 - base) / 64; qt <= (en.q1 - 1 - base) / 64; qt++)
                             tl.push_back(TileKey{g, (pt << 32) | qt});
                 }
@@ -407,12 +408,12 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                 const int64_t base = L.blok_roff[sk.second];
                 WTile w;
                 w.s = sk.first;
-                w.p0 = (int32_t)(base + pt * ts);
+                w.p0 = (int32_t)(base + pt * WT_P);
                 w.q0 = (int32_t)(base + qt * ts);
-                w.mp = (int32_t)std::min<int64_t>(ts, G.ldw - pt * ts);
+                w.mp = (int32_t)std::min<int64_t>(WT_P, G.ldw - pt * WT_P);
                 w.mq = (int32_t)std::min<int64_t>(ts, G.nq - qt * ts);
                 w.ldw = G.ldw;
-                w.wofs = G.wofs + pt * ts + qt * ts * (int64_t)G.ldw;
+                w.wofs = G.wofs + pt * WT_P + qt * ts * (int64_t)G.ldw;
                 P.wtiles.push_back(w);
                 P.flops_wtile += 2.0 * w.mp * w.mq * (double)width(w.s);
             }
