@@ -401,7 +401,7 @@ def main():
     kern = {}
     for cls, name, fl_ in (("update_direct", "k_update_direct<double,0>", uf["direct"]),
                            ("update_wgemm", "k_update_wgemm<double>", uf["multi"]),
-                           ("update_acc", "k_update_acc<double,0>", uf["multi"])):
+                           ("update_acc", "k_update_acc<double,0>", uf["single"])):
         t_, n_ = kt[cls]
         if n_ > 0:
             kern[name] = {"ms": t_ * 1e3, "launches": n_, "avg_launch_ms": t_ / n_ * 1e3,

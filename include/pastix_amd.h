@@ -119,10 +119,12 @@ int pastix_amd_kernel_times(const pastix_data_t *pastix_data, double out[10]);
 
 /* Schur-update flops by kernel, counting only the (p, q) pairs the
  * reference's per-block GEMMs compute: out[0] single-source tiles and out[1]
- * multi-source tiles of the L side (block(p) >= block(q)), out[2], out[3]
- * the same for the U side of LU (block(p) > block(q)), out[4] the flops of
- * the multi-source product tiles as computed (one side), out[5] = 0. */
-int pastix_amd_update_flops(const pastix_data_t *pastix_data, double out[6]);
+ * multi-source tiles of two-phase levels (product tiles, then gather) on the
+ * L side (block(p) >= block(q)), out[2], out[3] the same for the U side of
+ * LU (block(p) > block(q)), out[4] the flops of the multi-source product
+ * tiles as computed (one side), out[5], out[6] the multi-source tiles of
+ * single-pass levels (k_update_acc), L and U side. */
+int pastix_amd_update_flops(const pastix_data_t *pastix_data, double out[7]);
 
 /* Algorithmic flops of the analysed factorisation: out[0]=DPARM_FACT_THFLOPS,
  * [1]=diagonal factorisations, [2]=panel TRSMs, [3]=Schur updates counted

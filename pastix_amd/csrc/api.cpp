@@ -926,7 +926,7 @@ extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[10]) {
     return PASTIX_SUCCESS;
 }
 
-extern "C" int pastix_amd_update_flops(const pastix_data_t *d, double out[6]) {
+extern "C" int pastix_amd_update_flops(const pastix_data_t *d, double out[7]) {
     if (!d || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
     const pxa::Plan &P = d->plan;
     out[0] = P.flops_kind[0][0];
@@ -934,7 +934,8 @@ extern "C" int pastix_amd_update_flops(const pastix_data_t *d, double out[6]) {
     out[2] = P.flops_kind[1][0];
     out[3] = P.flops_kind[1][1];
     out[4] = P.flops_wtile;
-    out[5] = 0;
+    out[5] = P.flops_kind[0][2];
+    out[6] = P.flops_kind[1][2];
     return PASTIX_SUCCESS;
 }
 

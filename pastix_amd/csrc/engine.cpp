@@ -150,6 +150,7 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     d_wtiles_ = upload(*this, P.wtiles);
     lv_wt_ptr_ = P.lv_wt_ptr;
     lv_wsize_ = P.lv_wsize;
+    lv_single_pass_ = P.lv_single_pass;
 
     const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
     // level workspace of the two-phase update: the largest level's products,
@@ -277,7 +278,7 @@ template <typename T> int Engine::factorize_t(double thr) {
         prof_end(KC_PANEL, pk, stream_);
         const int u0 = lv_upd_ptr_[lv], us = lv_upd_split_[lv], u1 = lv_upd_ptr_[lv + 1];
         const int w0 = lv_wt_ptr_[lv], w1 = lv_wt_ptr_[lv + 1];
-        const bool two_phase = u1 > us && lv_wsize_[lv] <= wcap_;
+        const bool two_phase = u1 > us && !lv_single_pass_[lv] && lv_wsize_[lv] <= wcap_;
         T *Wk = (T *)d_wspace_;
         // one side of an update range: direct tiles [da, db), multi-source
         // tiles [ma, mb) with their product tiles [wa, wb); (A, B) are the

@@ -147,6 +147,7 @@ class Engine {
     WTile *d_wtiles_ = nullptr;
     std::vector<int32_t> lv_wt_ptr_;
     std::vector<int64_t> lv_wsize_;
+    std::vector<uint8_t> lv_single_pass_;
     void *d_wspace_ = nullptr;
     int64_t wcap_ = 0;  // workspace entries
     FwdItem *d_fwd_ = nullptr;

@@ -189,6 +189,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     P.lv_wt_ptr.assign(nl + 1, 0);
     P.lv_diag_ptr.assign(nl + 1, 0);
     P.lv_wsize.assign(nl, 0);
+    P.lv_single_pass.assign(nl, 0);
     P.bwd_itemptr.assign(cb + nl, 0);
     // update buffer: one row per off-diagonal panel row; CSR by permuted row
     std::vector<int64_t> yoff(cb + 1, 0);
@@ -328,9 +329,21 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                                                              P.upd.begin() + split, next_level) - P.upd.begin());
         P.lv_upd_mnext[lv] =
             (int32_t)(std::stable_partition(P.upd.begin() + split, P.upd.end(), next_level) - P.upd.begin());
+        // levels whose multi-source tiles hold about two entries on average
+        // are updated in one pass: the product workspace would write and
+        // re-read every product for little reuse of the target tile
+        {
+            int64_t nm = 0, nme = 0;
+            for (int32_t t = split; t < P.lv_upd_ptr[lv + 1]; t++) {
+                nm++;
+                nme += P.upd[t].e1 - P.upd[t].e0;
+            }
+            P.lv_single_pass[lv] = nm > 0 && (double)nme <= SINGLE_PASS_ENTS * (double)nm;
+        }
+        const bool single = P.lv_single_pass[lv] != 0;
         // exact flops of the kept (p, q) pairs of every entry
         for (int32_t t = P.lv_upd_ptr[lv]; t < P.lv_upd_ptr[lv + 1]; t++) {
-            const int kind = (t < split) ? 0 : 1;
+            const int kind = (t < split) ? 0 : (single ? 2 : 1);
             for (int32_t e = P.upd[t].e0; e < P.upd[t].e1; e++) {
                 const UpdEntry &en = P.ents[e];
                 const int64_t rb = P.rowbase[en.s];
@@ -350,7 +363,10 @@ Plan build_plan(const Symbol &s, const Layout &L) {
         }
 
         // product groups of the multi-source tiles and their 64x64 tiles
-        {
+        if (single) {
+            P.lv_wt_next[lv] = P.lv_wt_ptr[lv];
+            P.lv_wt_ptr[lv + 1] = P.lv_wt_ptr[lv];
+        } else {
             struct TileKey {
                 int32_t g;
                 int64_t qp;  // pt << 32 | qt: consecutive tiles share their p rows

@@ -9,6 +9,10 @@
 
 namespace pxa {
 
+// a level whose multi-source tiles average at most this many entries skips
+// the product workspace (the one-pass update kernel sums its entries)
+constexpr double SINGLE_PASS_ENTS = 2.05;
+
 struct Plan {
     int32_t nlevels = 0;
     // cblks of each level (increasing cblk index inside a level)
@@ -43,6 +47,9 @@ struct Plan {
     std::vector<int32_t> lv_wt_ptr;
     std::vector<int32_t> lv_wt_next;  // [lv_wt_ptr[L], lv_wt_next[L]): groups facing level L+1
     std::vector<int64_t> lv_wsize;
+    // levels whose multi-source tiles are summed in one pass (no product
+    // tiles, lv_wsize 0): those averaging at most SINGLE_PASS_ENTS entries
+    std::vector<uint8_t> lv_single_pass;
     int64_t wmax = 0;
     // forward products (sources of level L) into the update buffer, whose
     // entries are grouped by permuted row: row g owns [yrow_ptr[g], yrow_ptr[g+1])
@@ -79,10 +86,11 @@ struct Plan {
     double flops_diag = 0, flops_trsm = 0, flops_upd = 0;
     double flops_upd_potrf = 0, flops_upd_lu = 0;
     // the same update flops split by kernel: [side][0] single-source tiles,
-    // [side][1] multi-source tiles (side 0: L, block(p) >= block(q); side 1:
-    // U of LU, block(p) > block(q)); flops_wtile: product tiles as computed
-    // (64x64 tiles of the multi-source products, one side)
-    double flops_kind[2][2] = {{0, 0}, {0, 0}};
+    // [side][1] multi-source tiles of two-phase levels, [side][2]
+    // multi-source tiles of single-pass levels (side 0: L, block(p) >=
+    // block(q); side 1: U of LU, block(p) > block(q)); flops_wtile: product
+    // tiles as computed (128 x 64 tiles of the multi-source products, one side)
+    double flops_kind[2][3] = {{0, 0, 0}, {0, 0, 0}};
     double flops_wtile = 0;
 };
 

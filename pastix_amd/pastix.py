@@ -366,11 +366,13 @@ def kernel_times(pastix_data):
 
 def update_flops(pastix_data):
     """Schur-update flops per kernel: L-side single/multi-source tiles, U-side
-    single/multi-source tiles, multi-source product tiles as computed."""
-    out = np.zeros(6)
+    single/multi-source tiles, multi-source product tiles as computed, and
+    the multi-source tiles of single-pass levels (L, U side)."""
+    out = np.zeros(7)
     _check("pastix_amd_update_flops", lib.pastix_amd_update_flops(pastix_data.ptr, out))
     return {"direct": float(out[0]), "multi": float(out[1]), "direct_u": float(out[2]),
-            "multi_u": float(out[3]), "wtile": float(out[4])}
+            "multi_u": float(out[3]), "wtile": float(out[4]), "single": float(out[5]),
+            "single_u": float(out[6])}
 
 
 def _ptr_array(objs):

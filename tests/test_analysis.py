@@ -150,3 +150,21 @@ def test_levels_respect_dependencies():
             assert sym["blok_fcblk"][b] > c
     assert sym["nlevels"] >= 1
     pastix.finalize(h)
+
+
+@pytest.mark.parametrize("grid", [(12, 12, 12), (20, 20, 20)])
+def test_update_flops_partition_the_reference_count(grid):
+    # the per-kernel split of the Schur-update flops (single-source tiles,
+    # two-phase multi-source tiles, single-pass multi-source tiles) covers
+    # every (p, q) pair of the reference's per-block GEMMs exactly once
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid, min_bs=20, max_bs=40)
+    uf = pastix.update_flops(h)
+    fl = pastix.flops(h)
+    total = uf["direct"] + uf["multi"] + uf["single"]
+    assert total == pytest.approx(fl[3], rel=1e-12)
+    # the U side of LU drops the pairs of a source's diagonal-facing block
+    assert 0 < uf["direct_u"] + uf["multi_u"] + uf["single_u"] < total
+    # product tiles cover at least the two-phase pairs they serve (one side)
+    assert uf["wtile"] >= uf["multi"] * (1 - 1e-12)
+    pastix.finalize(h)
