@@ -31,17 +31,16 @@ namespace pxa {
 template <typename T> __device__ __forceinline__ T pabs(T v) { return v < (T)0 ? -v : v; }
 
 // Square root and its reciprocal of a positive pivot, and reciprocals, from
-// the hardware estimates refined by Newton steps (y <- y + y(1/2 - d y^2/2)
-// and y <- y + y(1 - d y)), each step doubling the correct bits, until about
-// one ulp: fewer dependent instructions than the correctly rounded sqrt and
-// division on the diagonal block's critical path.
+// the hardware estimates refined until about one ulp: for the FP64 inverse
+// square root one third-order step, y <- y + y e (1/2 + 3/8 e) with
+// e = 1 - d y^2, which triples the correct bits of the estimate; Newton steps
+// y <- y + y(1/2 - d y^2/2) and y <- y + y(1 - d y), each doubling them,
+// elsewhere.  Fewer dependent instructions than the correctly rounded sqrt
+// and division on the diagonal block's critical path.
 __device__ __forceinline__ void sqrt_rsqrt(double d, double &L, double &r) {
     double y = __builtin_amdgcn_rsq(d);
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-        const double h = 0.5 * d * y;
-        y = __builtin_fma(y, __builtin_fma(-h, y, 0.5), y);
-    }
+    const double e = __builtin_fma(-(d * y), y, 1.0);
+    y = __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
     r = y;
     L = d * y;
 }
@@ -62,89 +61,163 @@ __device__ __forceinline__ float recip(float d) {
     return __builtin_fmaf(y, __builtin_fmaf(-d, y, 1.0f), y);
 }
 
-// Shared by the inverses: column j of inv(M) for the lower-triangular 64x64
-// M held in Ls (M(r, k) = Ls[r][k] when TRANS is false, Ls[k][r] when true;
-// diagonal 1 when UNIT), by the four lanes 4*jl+q of a wave, x[kk] =
-// inv(M)(4*kk + q, j).
-template <typename T, bool UNIT, bool TRANS>
-__device__ __forceinline__ void quad_lower_inverse(T (*Ls)[65], const T *dinv, int j, int q, T (&x)[16]) {
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) x[kk] = (T)0;
-#pragma unroll
-    for (int r = 0; r < 64; r++) {
-        T s = (T)0;
-#pragma unroll
-        for (int kk = 0; 4 * kk < r; kk++)
-            if (4 * kk + q < r) s += (TRANS ? Ls[4 * kk + q][r] : Ls[r][4 * kk + q]) * x[kk];
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        const T dr = UNIT ? (T)1 : dinv[r];
-        const T xr = (r < j) ? (T)0 : ((r == j) ? dr : -s * dr);
-        if (q == (r & 3)) x[r >> 2] = xr;
-    }
+__device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmax_(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float fmax_(float a, float b) { return __builtin_fmaxf(a, b); }
+
+// Static pivoting of one Cholesky pivot (reference
+// kernels/core_zpotrfsp.c:78-81: |d| < crit is replaced by crit, a negative
+// pivot is an error): max(|d|, crit), one instruction on the dependency
+// chain; the counts are kept off the chain.  Identity padding columns (pivot
+// exactly 1) pass through too: they stay decoupled from the real columns, and
+// the caller removes them from the count when crit > 1.
+template <typename T> __device__ __forceinline__ T piv_max(T d, T crit, int &npiv, int &neg) {
+    npiv += (pabs(d) < crit) ? 1 : 0;
+    neg |= (d < (T)0 && d <= -crit) ? 1 : 0;
+    return fmax_(pabs(d), crit);
 }
 
-// The same inverse by blocks of 4 rows: per block, the four lanes of a
-// column form the partial dot products of the 4 rows with the known part of
-// x, combine them with lane shuffles, and solve the 4x4 diagonal block
-// redundantly; 16 rounds instead of 64.
-template <typename T, bool UNIT, bool TRANS>
-__device__ __forceinline__ void quad_lower_inverse4(T (*Ls)[65], const T *dinv, int j, int q, T (&x)[16]) {
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) x[kk] = (T)0;
+// Value of v in lane `lane` (wave-uniform) of the calling wave.
+__device__ __forceinline__ double rlane(double v, int lane) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float rlane(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// One joining level of the recursive-doubling inverse below, on the vector
+// units: for every pair of inverted diagonal blocks of size B (X11 at
+// [base, base+B), X22 at [base+B, base+2B)), X21 = -X22 (M21 X11).
+template <typename T, int B, bool TRANS>
+__device__ __forceinline__ void inv_join_valu(T (*Ls)[65], T (*Xs)[65], T *Ts, int tid) {
+    constexpr int NO = 32 * B;  // outputs of the level: 64/(2B) pairs of B x B
     auto M = [&](int r, int k) -> T { return TRANS ? Ls[k][r] : Ls[r][k]; };
+    const int o = tid;
+    const int p = o / (B * B), rem = o % (B * B), r = rem / B, cc = rem % B, base = 2 * B * p;
+    if (o < NO) {
+        T s = (T)0;
 #pragma unroll
-    for (int rb = 0; rb < 16; rb++) {
-        T s[4] = {(T)0, (T)0, (T)0, (T)0};
-#pragma unroll
-        for (int kk = 0; kk < rb; kk++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) s[t] += M(4 * rb + t, 4 * kk + q) * x[kk];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            s[t] += __shfl_xor(s[t], 1);
-            s[t] += __shfl_xor(s[t], 2);
-        }
-        T y[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int r = 4 * rb + t;
-            T acc = s[t];
-#pragma unroll
-            for (int u = 0; u < t; u++) acc += M(r, 4 * rb + u) * y[u];
-            const T dr = UNIT ? (T)1 : dinv[r];
-            y[t] = (r < j) ? (T)0 : ((r == j) ? dr : -acc * dr);
-        }
-        x[rb] = (q == 0) ? y[0] : (q == 1) ? y[1] : (q == 2) ? y[2] : y[3];
+        for (int m = 0; m < B; m++) s = fma_(M(base + B + r, base + m), Xs[base + m][base + cc], s);
+        Ts[o] = s;
     }
+    __syncthreads();
+    if (o < NO) {
+        T s = (T)0;
+#pragma unroll
+        for (int m = 0; m < B; m++) s = fma_(Xs[base + B + r][base + B + m], Ts[p * B * B + m * B + cc], s);
+        Xs[base + B + r][base + cc] = -s;
+    }
+    __syncthreads();
+}
+
+// The same on the matrix cores for B = 16, 32: one 16x16 tile per wave.
+template <typename T, int B, bool TRANS>
+__device__ __forceinline__ void inv_join_mfma(T (*Ls)[65], T (*Xs)[65], T *Ts, int lane, int w) {
+    constexpr int TPP = (B / 16) * (B / 16), NT = (64 / (2 * B)) * TPP;
+    auto M = [&](int r, int k) -> T { return TRANS ? Ls[k][r] : Ls[r][k]; };
+    const int i = lane & 15, kq = lane >> 4;
+    const int p = w / TPP, ti = (w % TPP) / (B / 16), tj = w % (B / 16), base = 2 * B * p;
+    if (w < NT) {
+        typename Mfma<T>::v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < B / 4; s++) {
+            const int k = 4 * s + kq;
+            acc = Mfma<T>::mma(M(base + B + ti * 16 + i, base + k), Xs[base + k][base + tj * 16 + i], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) Ts[p * B * B + (ti * 16 + Mfma<T>::drow(lane, r)) * B + tj * 16 + i] = acc[r];
+    }
+    __syncthreads();
+    if (w < NT) {
+        typename Mfma<T>::v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < B / 4; s++) {
+            const int k = 4 * s + kq;
+            acc = Mfma<T>::mma(Xs[base + B + ti * 16 + i][base + B + k], Ts[p * B * B + k * B + tj * 16 + i], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) Xs[base + B + ti * 16 + Mfma<T>::drow(lane, r)][base + tj * 16 + i] = -acc[r];
+    }
+    __syncthreads();
+}
+
+// X = inv(M) for the lower-triangular 64x64 M held in Ls (M(r, k) = Ls[r][k],
+// or Ls[k][r] when TRANS; unit diagonal when UNIT, else 1/M(r, r) = dinv[r])
+// by recursive doubling: the sixteen 4x4 diagonal blocks are inverted
+// directly, then pairs of inverted blocks of size 4, 8, 16 and 32 are joined
+// (inv([M11 0; M21 M22]) = [X11 0; -X22 M21 X11 X22]): 4 levels of small
+// products instead of a 64-step substitution.  Xs receives X with zeros above
+// the diagonal; Ts (1024 entries) is scratch.  With INPLACE (not TRANS), Xs
+// is Ls, whose upper triangle the caller has zeroed: every join overwrites
+// the block M21 it has consumed, and only M21 blocks are read from M after
+// the 4x4 blocks.  All 256 threads, barriers inside; Ls and dinv must be
+// visible to all threads on entry.
+template <typename T, bool UNIT, bool TRANS, bool INPLACE>
+__device__ __forceinline__ void lower_inverse(T (*Ls)[65], T (*Xs)[65], T *Ts, const T *dinv, int tid, int w) {
+    static_assert(!(INPLACE && TRANS), "in place only from the lower triangle");
+    auto M = [&](int r, int k) -> T { return TRANS ? Ls[k][r] : Ls[r][k]; };
+    if (!INPLACE) {
+        for (int e = tid; e < 64 * 64; e += 256) Xs[e >> 6][e & 63] = (T)0;
+        __syncthreads();
+    }
+    if (tid < 64) {
+        // one wave; the block's strict lower entries are read before any write
+        const int b0 = 4 * (tid >> 2), cc = tid & 3;
+        const T m10 = M(b0 + 1, b0), m20 = M(b0 + 2, b0), m21 = M(b0 + 2, b0 + 1);
+        const T m30 = M(b0 + 3, b0), m31 = M(b0 + 3, b0 + 1), m32 = M(b0 + 3, b0 + 2);
+        const T d0 = UNIT ? (T)1 : dinv[b0], d1 = UNIT ? (T)1 : dinv[b0 + 1];
+        const T d2 = UNIT ? (T)1 : dinv[b0 + 2], d3 = UNIT ? (T)1 : dinv[b0 + 3];
+        const T x0 = (cc == 0) ? d0 : (T)0;
+        const T x1 = (cc > 1) ? (T)0 : (cc == 1) ? d1 : -(m10 * x0) * d1;
+        const T x2 = (cc > 2) ? (T)0 : (cc == 2) ? d2 : -fma_(m21, x1, m20 * x0) * d2;
+        const T x3 = (cc == 3) ? d3 : -fma_(m32, x2, fma_(m31, x1, m30 * x0)) * d3;
+        Xs[b0][b0 + cc] = x0;
+        Xs[b0 + 1][b0 + cc] = x1;
+        Xs[b0 + 2][b0 + cc] = x2;
+        Xs[b0 + 3][b0 + cc] = x3;
+    }
+    __syncthreads();
+    inv_join_valu<T, 4, TRANS>(Ls, Xs, Ts, tid);
+    inv_join_valu<T, 8, TRANS>(Ls, Xs, Ts, tid);
+    inv_join_mfma<T, 16, TRANS>(Ls, Xs, Ts, tid & 63, w);
+    inv_join_mfma<T, 32, TRANS>(Ls, Xs, Ts, tid & 63, w);
 }
 
 // Cholesky and inverse of one 64-column diagonal sub-block.
 //
 // Thread (i, w) = (tid % 64, tid / 64) keeps the entries (i, j), j = w mod 4,
-// of row i in registers.  Columns are taken 4 at a time: each wave publishes
-// one of them through LDS (two buffers by parity), every thread factors the
-// 4x4 diagonal block, replacing pivots below the threshold (reference
-// kernels/core_zpotrfsp.c:78-81), and forms its row's 4 multipliers, which
-// are published for the rank-4 update of the remaining columns.  Rows beyond
-// the block width are padded with the identity, which the updates leave
-// untouched.
+// of row i in registers.  Columns are taken 4 at a time with one barrier per
+// block: the block's 4 columns are published through LDS (two buffers by
+// parity), every thread factors the 4x4 diagonal block, replacing pivots
+// below the threshold (reference kernels/core_zpotrfsp.c:78-81), and forms
+// its row's 4 multipliers.  The multipliers of the rows a wave's columns
+// belong to are held by lanes of the same wave (every wave forms all rows'
+// multipliers), so the rank-4 update reads them across lanes (readlane)
+// rather than through LDS.  Only the next block's columns are updated before
+// they are published; the update of the columns after them is deferred into
+// the next block, where it overlaps that block's 4x4 dependency chain.  Rows
+// beyond the block width are padded with the identity, which the updates
+// leave untouched.
 //
-// The inverse X = inv(L) is formed by columns: four lanes of one wave share
-// column j and split the forward substitution L x = e_j by k mod 4; their
-// partial sums are combined with lane shuffles (no barrier).  W receives X
-// (lower, column-major, ld = K) and WT its transpose, both with explicit
-// zeros in the other triangle of the sub-block.
+// The inverse X = inv(L) comes from lower_inverse (recursive doubling, in
+// place of L in LDS).  W receives X (lower, column-major, ld = K) and WT its
+// transpose, both with explicit zeros in the other triangle of the sub-block.
 template <typename T>
 __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
                                                  T *__restrict__ coef, T *__restrict__ winv, T *__restrict__ winvt,
                                                  double thr, int *stat) {
     __builtin_amdgcn_s_setprio(3);  // panel steps lead the critical path
-    __shared__ T colb[2][4][64];
-    __shared__ T lb[4][64];
+    __shared__ T fac[1024];
+    T(*colb)[4][64] = reinterpret_cast<T(*)[4][64]>(fac);  // [parity][w][row]
     __shared__ T Ls[64][65];
     __shared__ T dinv[64];
-    const int tid = threadIdx.x, i = tid & 63, w = tid >> 6;
+    T *Ts = fac;  // inverse scratch once the factor is done
+    const int tid = threadIdx.x, i = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int c = cblks[blockIdx.x];
     const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
     const int k0 = kb * 64;
@@ -161,79 +234,73 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
     }
     const T crit = (T)thr;
     int npiv = 0, neg = 0;
-    // blocks of 4 columns k..k+3, one column per wave: publish, factor the
-    // 4x4 diagonal block redundantly, form the row's multipliers, publish
-    // them, rank-4 update of the row's remaining columns (two barriers per
-    // block)
-    auto piv = [&](T d, int col) -> T {
-        if (col < nb) {
-            if (pabs(d) < crit) {
-                d = crit;
-                npiv++;
-            }
-            if (d < (T)0) {
-                neg = 1;
-                d = -d;
-            }
-        }
-        return d;
-    };
+    T pl0 = 0, pl1 = 0, pl2 = 0, pl3 = 0;  // this row's multipliers of the last block (0 above it)
+    colb[0][w][i] = a[0];
 #pragma unroll
     for (int b4 = 0; b4 < 16; b4++) {
         const int k = 4 * b4;
+        __syncthreads();
+        // the row index as an opaque value: keeps the compiler from hoisting
+        // the 16 blocks' row masks out of the loop (they would hold registers
+        // for the whole factorisation)
+        int ii = i;
+        asm volatile("" : "+v"(ii));
         T(*cb)[64] = colb[b4 & 1];
-        cb[w][i] = a[b4];
-        __syncthreads();
-        // 4x4 Cholesky of the diagonal block (same in every thread)
-        const T d0 = piv(cb[0][k], k);
-        T L00, i00;
-        sqrt_rsqrt(d0, L00, i00);
-        const T L10 = cb[0][k + 1] * i00, L20 = cb[0][k + 2] * i00, L30 = cb[0][k + 3] * i00;
-        const T d1 = piv(cb[1][k + 1] - L10 * L10, k + 1);
-        T L11, i11;
-        sqrt_rsqrt(d1, L11, i11);
-        const T L21 = (cb[1][k + 2] - L20 * L10) * i11, L31 = (cb[1][k + 3] - L30 * L10) * i11;
-        const T d2 = piv(cb[2][k + 2] - L20 * L20 - L21 * L21, k + 2);
-        T L22, i22;
-        sqrt_rsqrt(d2, L22, i22);
-        const T L32 = (cb[2][k + 3] - L30 * L20 - L31 * L21) * i22;
-        const T d3 = piv(cb[3][k + 3] - L30 * L30 - L31 * L31 - L32 * L32, k + 3);
-        T L33, i33;
-        sqrt_rsqrt(d3, L33, i33);
-        // multipliers of row i
-        T l0, l1, l2, l3;
-        if (i > k + 3) {
-            l0 = cb[0][i] * i00;
-            l1 = (cb[1][i] - l0 * L10) * i11;
-            l2 = (cb[2][i] - l0 * L20 - l1 * L21) * i22;
-            l3 = (cb[3][i] - l0 * L30 - l1 * L31 - l2 * L32) * i33;
-        } else {
-            const int t = i - k;  // rows of the block (t < 0: rows above)
-            l0 = (t == 0) ? L00 : (t == 1) ? L10 : (t == 2) ? L20 : L30;
-            l1 = (t == 1) ? L11 : (t == 2) ? L21 : L31;
-            l2 = (t == 2) ? L22 : L32;
-            l3 = L33;
-        }
-        lb[0][i] = l0;
-        lb[1][i] = l1;
-        lb[2][i] = l2;
-        lb[3][i] = l3;
-        if (i >= k + w) a[b4] = (w == 0) ? l0 : (w == 1) ? l1 : (w == 2) ? l2 : l3;
-        __syncthreads();
-        if (i > k + 3) {
+        const T c00 = cb[0][k], c10 = cb[0][k + 1], c20 = cb[0][k + 2], c30 = cb[0][k + 3];
+        const T c11 = cb[1][k + 1], c21 = cb[1][k + 2], c31 = cb[1][k + 3];
+        const T c22 = cb[2][k + 2], c32 = cb[2][k + 3], c33 = cb[3][k + 3];
+        const T r0 = cb[0][ii], r1 = cb[1][ii], r2 = cb[2][ii], r3 = cb[3][ii];
+        // deferred rank-4 update by the previous block of the columns after this block's
+        if (b4 >= 1) {
 #pragma unroll
             for (int jj = b4 + 1; jj < 16; jj++) {
-                const int j = 4 * jj + w;
-                a[jj] -= l0 * lb[0][j] + l1 * lb[1][j] + l2 * lb[2][j] + l3 * lb[3][j];
+                const int lj = 4 * jj + w;
+                a[jj] = fma_(-pl3, rlane(pl3, lj), fma_(-pl2, rlane(pl2, lj),
+                        fma_(-pl1, rlane(pl1, lj), fma_(-pl0, rlane(pl0, lj), a[jj]))));
             }
+        }
+        // 4x4 Cholesky of the diagonal block (same in every thread)
+        T L00, i00, L11, i11, L22, i22, L33, i33;
+        sqrt_rsqrt(piv_max(c00, crit, npiv, neg), L00, i00);
+        const T L10 = c10 * i00, L20 = c20 * i00, L30 = c30 * i00;
+        sqrt_rsqrt(piv_max(fma_(-L10, L10, c11), crit, npiv, neg), L11, i11);
+        const T L21 = fma_(-L20, L10, c21) * i11, L31 = fma_(-L30, L10, c31) * i11;
+        sqrt_rsqrt(piv_max(fma_(-L21, L21, fma_(-L20, L20, c22)), crit, npiv, neg), L22, i22);
+        const T L32 = fma_(-L31, L21, fma_(-L30, L20, c32)) * i22;
+        sqrt_rsqrt(piv_max(fma_(-L32, L32, fma_(-L31, L31, fma_(-L30, L30, c33))), crit, npiv, neg), L33, i33);
+        // take the counts now: left to the end, they would keep every pivot in registers
+        asm volatile("" : "+v"(npiv), "+v"(neg));
+        // multipliers of row i (rows of the block take the block's L)
+        const T m0 = r0 * i00;
+        const T m1 = (r1 - m0 * L10) * i11;
+        const T m2 = (r2 - m0 * L20 - m1 * L21) * i22;
+        const T m3 = (r3 - m0 * L30 - m1 * L31 - m2 * L32) * i33;
+        const int t = ii - k;
+        const bool below = t > 3;
+        const T l0 = below ? m0 : (t == 0) ? L00 : (t == 1) ? L10 : (t == 2) ? L20 : L30;
+        const T l1 = below ? m1 : (t == 1) ? L11 : (t == 2) ? L21 : L31;
+        const T l2 = below ? m2 : (t == 2) ? L22 : L32;
+        const T l3 = below ? m3 : L33;
+        const T lw = (w == 0) ? l0 : (w == 1) ? l1 : (w == 2) ? l2 : l3;
+        if (t >= w) a[b4] = lw;
+        pl0 = below ? l0 : (T)0;
+        pl1 = below ? l1 : (T)0;
+        pl2 = below ? l2 : (T)0;
+        pl3 = below ? l3 : (T)0;
+        if (b4 < 15) {
+            const int jn = b4 + 1, lj = 4 * jn + w;
+            a[jn] = fma_(-pl3, rlane(pl3, lj), fma_(-pl2, rlane(pl2, lj),
+                    fma_(-pl1, rlane(pl1, lj), fma_(-pl0, rlane(pl0, lj), a[jn]))));
+            colb[jn & 1][w][ii] = a[jn];
         }
     }
 #pragma unroll
     for (int jj = 0; jj < 16; jj++) {
         const int j = 4 * jj + w;
-        if (j <= i) Ls[i][j] = a[jj];
+        Ls[i][j] = (j <= i) ? a[jj] : (T)0;
         if (i < nb && j < nb && j <= i) P[i + (int64_t)j * ld] = a[jj];
     }
+    if (crit > (T)1) npiv -= 64 - nb;  // the identity padding's pivots
     if (tid == 0) {
         if (npiv) atomicAdd(&stat[0], npiv);
         if (neg) atomicOr(&stat[1], 1);
@@ -241,14 +308,7 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
     __syncthreads();
     if (tid < 64) dinv[tid] = recip(Ls[tid][tid]);
     __syncthreads();
-    // column j of inv(L): lanes 4*jl + q of wave w, x[kk] = X(4*kk + q, j)
-    const int q = tid & 3, j = 16 * w + ((tid >> 2) & 15);
-    T x[16];
-    quad_lower_inverse4<T, false, false>(Ls, dinv, j, q, x);
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) Ls[4 * kk + q][j] = x[kk];
-    __syncthreads();
+    lower_inverse<T, false, false, true>(Ls, Ls, Ts, dinv, tid, w);
     T *W = winv + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
     T *WT = winvt + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
     for (int e = tid; e < 64 * 64; e += 256) {
@@ -386,15 +446,20 @@ __global__ __launch_bounds__(256) void k_winv_offdiag(DevSymbol S, const int32_t
 // blocks of four columns with one barrier each.  Outputs: L (unit, strict lower) and the diagonal of U
 // in the L panel, the strict upper part of U transposed in the U panel,
 // W1 = inv(L) (lower, unit diagonal), W2 = inv(U)^T (lower) and WB = inv(U)
-// (upper), with explicit zeros in the other triangles of the sub-block.
+// (upper), with explicit zeros in the other triangles of the sub-block; the
+// inverses come from lower_inverse (recursive doubling, in place in LDS: U^T
+// first, then L rebuilt from the registers).
 template <typename T>
 __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
                                                  T *__restrict__ lcoef, T *__restrict__ ucoef, T *__restrict__ w1,
                                                  T *__restrict__ w2, T *__restrict__ wb, double thr, int *stat) {
     __builtin_amdgcn_s_setprio(3);
-    __shared__ T colb[2][4][64], rowb[2][4][64];
+    __shared__ T fac[1024];
+    T(*colb)[4][64] = reinterpret_cast<T(*)[4][64]>(fac);
+    T(*rowb)[4][64] = reinterpret_cast<T(*)[4][64]>(fac + 512);
     __shared__ T Ls[64][65];
     __shared__ T dinv[64];
+    T *Ts = fac;  // inverse scratch once the factor is done
     const int tid = threadIdx.x, i = tid & 63, w = tid >> 6;
     const int c = cblks[blockIdx.x];
     const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
@@ -481,10 +546,11 @@ __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__r
             }
         }
     }
+    // U^T (lower, diagonal of U) in LDS for the first in-place inverse
 #pragma unroll
     for (int jj = 0; jj < 16; jj++) {
         const int j = 4 * jj + w;
-        Ls[i][j] = a[jj];
+        Ls[j][i] = (j >= i) ? a[jj] : (T)0;
         if (i < nb && j < nb) {
             if (j <= i) PL[i + (int64_t)j * ld] = a[jj];
             else PU[j + (int64_t)i * ld] = a[jj];
@@ -494,31 +560,30 @@ __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__r
     __syncthreads();
     if (tid < 64) dinv[tid] = recip(Ls[tid][tid]);
     __syncthreads();
-    const int q = tid & 3, j = 16 * w + ((tid >> 2) & 15);
-    T x1[16], x2[16];
-    quad_lower_inverse<T, true, false>(Ls, dinv, j, q, x1);   // inv(L), L unit lower
-    quad_lower_inverse<T, false, true>(Ls, dinv, j, q, x2);   // inv(U^T)
     T *W1 = w1 + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
     T *W2 = w2 + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
     T *WB = wb + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) Ls[4 * kk + q][j] = x1[kk];
-    __syncthreads();
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int lo = e & 63, hi = e >> 6;
-        if (lo < nb && hi < nb) W1[lo + (int64_t)hi * K] = Ls[lo][hi];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) Ls[4 * kk + q][j] = x2[kk];
-    __syncthreads();
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    lower_inverse<T, false, false, true>(Ls, Ls, Ts, dinv, tid, wu);  // inv(U^T)
     for (int e = tid; e < 64 * 64; e += 256) {
         const int lo = e & 63, hi = e >> 6;
         if (lo < nb && hi < nb) {
             W2[lo + (int64_t)hi * K] = Ls[lo][hi];
             WB[lo + (int64_t)hi * K] = Ls[hi][lo];
         }
+    }
+    __syncthreads();
+    // then the unit lower L (its diagonal is not read)
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+        const int j = 4 * jj + w;
+        Ls[i][j] = (j < i) ? a[jj] : (T)0;
+    }
+    __syncthreads();
+    lower_inverse<T, true, false, true>(Ls, Ls, Ts, dinv, tid, wu);  // inv(L)
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int lo = e & 63, hi = e >> 6;
+        if (lo < nb && hi < nb) W1[lo + (int64_t)hi * K] = Ls[lo][hi];
     }
 }
 

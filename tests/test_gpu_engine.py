@@ -313,6 +313,24 @@ def test_overlap_matches_single_stream(facto):
     pastix.finalize(h)
 
 
+def test_llt_threshold_above_one_counts_no_padding():
+    """With an absolute threshold above 1 no real pivot of a well-scaled SPD
+    matrix is replaced; the identity padding of diagonal blocks narrower than
+    64 columns (pivots of 1) must not be counted either, and the factor
+    matches the oracle's with the same threshold."""
+    grid = (9, 8, 7)
+    A = (spm.laplacian7(*grid, diag=6.0) * 1e8).tocsr()
+    h, S, sym = analyse(A, True, LLT, grid=grid, min_bs=12, max_bs=40)
+    thr = 10.0
+    h.dparm[pastix.dparm.epsilon_magn_ctrl] = -thr  # negative: absolute threshold
+    pastix.task_numfact(h, S)
+    assert h.iparm[pastix.iparm.static_pivoting] == 0
+    Lg, _ = pastix.coeftab(h)
+    O, bc, Lo, _ = oracle_factor(A, sym, crit=thr)
+    assert rel_frob(lower_mask_panel(sym, Lg), lower_mask_panel(sym, Lo)) <= 1e-12
+    pastix.finalize(h)
+
+
 def test_static_pivot_on_device():
     """A zero LU pivot (the first eliminated unknown's diagonal) is replaced by
     the threshold and counted on the device exactly as the oracle does
