@@ -110,6 +110,7 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     step_cb_ptr_ = P.step_cb_ptr;
     step_trsm_ptr_ = P.step_trsm_ptr;
     step_upd_ptr_ = P.step_upd_ptr;
+    lv_panel_right_ = P.lv_panel_right;
     d_step_cblks_ = upload(*this, P.step_cblks);
     d_ptrsm_ = upload(*this, P.ptrsm);
     d_pupd_ = upload(*this, P.pupd);
@@ -257,22 +258,31 @@ template <typename T> int Engine::factorize_t(double thr) {
             const int b0 = step_cb_ptr_[s], nb = step_cb_ptr_[s + 1] - b0;
             const int p0 = step_trsm_ptr_[s], np = step_trsm_ptr_[s + 1] - p0;
             const int q0 = step_upd_ptr_[s], nq = step_upd_ptr_[s + 1] - q0;
-            // the step's column block first takes the products of the blocks
-            // before it (left-looking), then its diagonal sub-block is
-            // factored and the rows below are solved against it
+            // the step's diagonal sub-block is factored and the rows below
+            // are solved against it; the in-cblk products come before (the
+            // step's block takes those of the blocks before it) or, on
+            // right-looking levels, after (the later blocks take the step's)
+            const bool right = lv_panel_right_[lv] != 0;
             if (facto_ == 2) {
-                // L -= L U^T and U^T -= U^T L^T on the block's tiles; rows
-                // below the sub-block: L side times inv(U_kk), U side (stored
+                // L -= L U^T and U^T -= U^T L^T on the tiles; rows below the
+                // sub-block: L side times inv(U_kk), U side (stored
                 // transposed) times inv(L_kk)^T
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, U, false, stream_);
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, U, U, L, true, stream_);
+                if (!right) {
+                    launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, false, L, L, U, false, stream_);
+                    launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, false, U, U, L, true, stream_);
+                }
                 launch_getrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, U, W1, W2, (T *)d_WB_, thr, d_stat_, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W2, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, U, W1, stream_);
+                if (right) {
+                    launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, true, L, L, U, false, stream_);
+                    launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, true, U, U, L, true, stream_);
+                }
             } else {
-                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, L, L, L, false, stream_);
+                if (!right) launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, false, L, L, L, false, stream_);
                 launch_potrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, W1, (T *)d_WB_, thr, d_stat_, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W1, stream_);
+                if (right) launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, true, L, L, L, false, stream_);
             }
         }
         prof_end(KC_PANEL, pk, stream_);

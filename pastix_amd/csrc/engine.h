@@ -137,6 +137,7 @@ class Engine {
     int64_t nyrows_ = 0;
     void *d_ybuf_ = nullptr, *d_WB_ = nullptr;
     std::vector<int32_t> lv_step_ptr_, step_cb_ptr_, step_trsm_ptr_, step_upd_ptr_;
+    std::vector<uint8_t> lv_panel_right_;
     int32_t *d_step_cblks_ = nullptr, *d_wide_ = nullptr;
     int n_wide_ = 0;
     PanelItem *d_ptrsm_ = nullptr, *d_pupd_ = nullptr;

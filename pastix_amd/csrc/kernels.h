@@ -26,8 +26,8 @@ template <typename T>
 void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, const T *winv,
                        hipStream_t st);
 template <typename T>
-void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *C, const T *X, const T *Y,
-                      bool strict, hipStream_t st);
+void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, bool right, T *C, const T *X,
+                      const T *Y, bool strict, hipStream_t st);
 // LU: diagonal 64-block with inverses; final layout of the square parts.
 template <typename T>
 void launch_getrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *lcoef, T *ucoef, T *w1, T *w2, T *wb,

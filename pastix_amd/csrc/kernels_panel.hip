@@ -1,15 +1,17 @@
 // Blocked right-looking panel factorisation for CDNA4 (gfx950).
 //
 // A cblk's panel (stride x K, column-major) is factored by 64-column steps,
-// each step being (at most) three launches over all cblks of the level:
+// each step being three launches over all cblks of the level:
 //   k_potrf64   : Cholesky of the 64x64 diagonal sub-block with static
 //                 pivoting (reference kernels/core_zpotrfsp.c:66-108), the
 //                 block in registers, plus its inverse;
 //   k_panel_trsm: every panel row below the sub-block times inv(L_kk)^T
 //                 (core_zpotrfsp.c:166-176 and core_ztrsmsp.c:147-181);
-//   k_panel_upd : before the step, left-looking update of the step's
-//                 column block by the blocks before it (core_zpotrfsp.c:178-185
-//                 applied column block by column block).
+//   k_panel_upd : the in-cblk products (core_zpotrfsp.c:178-185 applied
+//                 column block by column block): before the step, of the
+//                 blocks before it into the step's block (left-looking), or,
+//                 on levels with few rows, after the step, of the step's block
+//                 into every later block (right-looking).
 // k_winv_offdiag completes the inverse of the whole diagonal block for the
 // solve once the panel is factored.  Inverses are stored twice: W (lower,
 // column-major) for the forward sweep and WT = W^T for the backward sweep.
@@ -154,10 +156,13 @@ __device__ __forceinline__ void inv_join_mfma(T (*Ls)[65], T (*Xs)[65], T *Ts, i
 // the diagonal; Ts (1024 entries) is scratch.  With INPLACE (not TRANS), Xs
 // is Ls, whose upper triangle the caller has zeroed: every join overwrites
 // the block M21 it has consumed, and only M21 blocks are read from M after
-// the 4x4 blocks.  All 256 threads, barriers inside; Ls and dinv must be
-// visible to all threads on entry.
+// the 4x4 blocks.  Only X[0:nb, 0:nb] is complete: levels that would join
+// blocks of the identity padding only are skipped.  All 256 threads (nb
+// uniform), barriers inside; Ls and dinv must be visible to all threads on
+// entry.
 template <typename T, bool UNIT, bool TRANS, bool INPLACE>
-__device__ __forceinline__ void lower_inverse(T (*Ls)[65], T (*Xs)[65], T *Ts, const T *dinv, int tid, int w) {
+__device__ __forceinline__ void lower_inverse(T (*Ls)[65], T (*Xs)[65], T *Ts, const T *dinv, int nb, int tid,
+                                              int w) {
     static_assert(!(INPLACE && TRANS), "in place only from the lower triangle");
     auto M = [&](int r, int k) -> T { return TRANS ? Ls[k][r] : Ls[r][k]; };
     if (!INPLACE) {
@@ -181,10 +186,11 @@ __device__ __forceinline__ void lower_inverse(T (*Ls)[65], T (*Xs)[65], T *Ts, c
         Xs[b0 + 3][b0 + cc] = x3;
     }
     __syncthreads();
-    inv_join_valu<T, 4, TRANS>(Ls, Xs, Ts, tid);
-    inv_join_valu<T, 8, TRANS>(Ls, Xs, Ts, tid);
-    inv_join_mfma<T, 16, TRANS>(Ls, Xs, Ts, tid & 63, w);
-    inv_join_mfma<T, 32, TRANS>(Ls, Xs, Ts, tid & 63, w);
+    // a level whose joined blocks start at or beyond nb only builds padding
+    if (nb > 4) inv_join_valu<T, 4, TRANS>(Ls, Xs, Ts, tid);
+    if (nb > 8) inv_join_valu<T, 8, TRANS>(Ls, Xs, Ts, tid);
+    if (nb > 16) inv_join_mfma<T, 16, TRANS>(Ls, Xs, Ts, tid & 63, w);
+    if (nb > 32) inv_join_mfma<T, 32, TRANS>(Ls, Xs, Ts, tid & 63, w);
 }
 
 // Cholesky and inverse of one 64-column diagonal sub-block.
@@ -239,6 +245,7 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
 #pragma unroll
     for (int b4 = 0; b4 < 16; b4++) {
         const int k = 4 * b4;
+        if (k >= nb) break;  // uniform: the padding needs no elimination
         __syncthreads();
         // the row index as an opaque value: keeps the compiler from hoisting
         // the 16 blocks' row masks out of the loop (they would hold registers
@@ -300,7 +307,7 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
         Ls[i][j] = (j <= i) ? a[jj] : (T)0;
         if (i < nb && j < nb && j <= i) P[i + (int64_t)j * ld] = a[jj];
     }
-    if (crit > (T)1) npiv -= 64 - nb;  // the identity padding's pivots
+    if (crit > (T)1) npiv -= ((nb + 3) & ~3) - nb;  // the identity padding's pivots
     if (tid == 0) {
         if (npiv) atomicAdd(&stat[0], npiv);
         if (neg) atomicOr(&stat[1], 1);
@@ -308,7 +315,7 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
     __syncthreads();
     if (tid < 64) dinv[tid] = recip(Ls[tid][tid]);
     __syncthreads();
-    lower_inverse<T, false, false, true>(Ls, Ls, Ts, dinv, tid, w);
+    lower_inverse<T, false, false, true>(Ls, Ls, Ts, dinv, nb, tid, w);
     T *W = winv + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
     T *WT = winvt + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
     for (int e = tid; e < 64 * 64; e += 256) {
@@ -348,20 +355,22 @@ __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem
     }
 }
 
-// Left-looking update of the panel's column block kb (columns [k0, k0+64),
-// k0 = 64*kb) by all the blocks before it, before that block is factored:
-//   C[r, k0+j] -= sum_{k < k0} X[r, k] . Y[k0+j, k]   for the 64-row tile r0
-// (C, X, Y panels of the same cblk: all L for LL^t; L, L, U and U, U, L for
-// the two sides of LU; the reference's blocked potrf/getrf applies the same
-// products step by step, kernels/core_zpotrfsp.c:178-185).  Every element of
-// the block is read and written once, with K = k0 staged through LDS.  On the
-// diagonal tile (r0 == k0) only r >= c (strict = 0) or r > c (strict = 1) is
-// written: the L side of LU keeps the lower part and the diagonal, the U side
-// (transposed) the strict upper part.
+// In-cblk update of column block c0 of the panel (item), 64-row tile r0:
+//   C[r, c0+j] -= sum_{k in Kr} X[r, k] . Y[c0+j, k]
+// left-looking (right = 0: c0 is the step's block kb, Kr = [0, c0), before
+// the step) or right-looking (right = 1: c0 is a later block, Kr = the
+// columns of block kb, after the step).  C, X, Y are panels of the same cblk:
+// all L for LL^t; L, L, U and U, U, L for the two sides of LU; the
+// reference's blocked potrf/getrf applies the same products,
+// kernels/core_zpotrfsp.c:178-185.  On a block's diagonal tile (r0 == c0)
+// only r >= c (strict = 0) or r > c (strict = 1) is written: the L side of LU
+// keeps the lower part and the diagonal, the U side (transposed) the strict
+// upper part.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
-                                                   int kb, T *__restrict__ ccoef, const T *__restrict__ xcoef,
-                                                   const T *__restrict__ ycoef, int strict) {
+                                                   int kb, int right, T *__restrict__ ccoef,
+                                                   const T *__restrict__ xcoef, const T *__restrict__ ycoef,
+                                                   int strict) {
     __builtin_amdgcn_s_setprio(3);
     __shared__ StageBuf<T, 16> sb;
     const int id = xcd_remap(blockIdx.x, gridDim.x);
@@ -369,21 +378,21 @@ __global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
     const PanelItem it = items[id];
-    const int c = it.c, r0 = it.r0;
+    const int c = it.c, r0 = it.r0, c0 = it.c0;
     const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
     const int stride = S.cblk_stride[c];
     const int64_t ld = stride;
-    const int k0 = kb * 64;
+    const int ks = right ? kb * 64 : 0, kn = right ? min(64, K - ks) : c0;
     T *P = ccoef + S.cblk_coef[c];
-    const T *X = xcoef + S.cblk_coef[c], *Y = ycoef + S.cblk_coef[c];
-    const int mr = min(64, stride - r0), mc = min(64, K - k0);
+    const T *X = xcoef + S.cblk_coef[c] + (int64_t)ks * ld, *Y = ycoef + S.cblk_coef[c] + (int64_t)ks * ld;
+    const int mr = min(64, stride - r0), mc = min(64, K - c0);
     Quad<T> q;
     q.zero();
-    wg_gemm_lds<T, 16>(q, X + r0, mr, ld, Y + k0, mc, ld, k0, sb, tid);
+    wg_gemm_lds<T, 16>(q, X + r0, mr, ld, Y + c0, mc, ld, kn, sb, tid);
     if (wr < mr && wc < mc) {
-        const bool diag = r0 == k0;
+        const bool diag = r0 == c0;
         QFOR(if (qrow < mr - wr && qcol < mc - wc && (!diag || wr + qrow > wc + qcol - (strict ? 0 : 1)))
-                 P[(r0 + wr + qrow) + (int64_t)(k0 + wc + qcol) * ld] -= q.c[m_][n_][r_];)
+                 P[(r0 + wr + qrow) + (int64_t)(c0 + wc + qcol) * ld] -= q.c[m_][n_][r_];)
     }
 }
 
@@ -492,6 +501,7 @@ __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__r
 #pragma unroll
     for (int b4 = 0; b4 < 16; b4++) {
         const int k = 4 * b4;
+        if (k >= nb) break;  // uniform: the padding needs no elimination
         T(*cb)[64] = colb[b4 & 1];
         T(*rb)[64] = rowb[b4 & 1];
         cb[w][i] = a[b4];
@@ -564,7 +574,7 @@ __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__r
     T *W2 = w2 + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
     T *WB = wb + S.cblk_winv[c] + k0 + (int64_t)k0 * K;
     const int wu = __builtin_amdgcn_readfirstlane(w);
-    lower_inverse<T, false, false, true>(Ls, Ls, Ts, dinv, tid, wu);  // inv(U^T)
+    lower_inverse<T, false, false, true>(Ls, Ls, Ts, dinv, nb, tid, wu);  // inv(U^T)
     for (int e = tid; e < 64 * 64; e += 256) {
         const int lo = e & 63, hi = e >> 6;
         if (lo < nb && hi < nb) {
@@ -580,7 +590,7 @@ __global__ __launch_bounds__(256) void k_getrf64(DevSymbol S, const int32_t *__r
         Ls[i][j] = (j < i) ? a[jj] : (T)0;
     }
     __syncthreads();
-    lower_inverse<T, true, false, true>(Ls, Ls, Ts, dinv, tid, wu);  // inv(L)
+    lower_inverse<T, true, false, true>(Ls, Ls, Ts, dinv, nb, tid, wu);  // inv(L)
     for (int e = tid; e < 64 * 64; e += 256) {
         const int lo = e & 63, hi = e >> 6;
         if (lo < nb && hi < nb) W1[lo + (int64_t)hi * K] = Ls[lo][hi];
@@ -664,10 +674,11 @@ void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb
     hipLaunchKernelGGL(k_panel_trsm<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, coef, winv);
 }
 template <typename T>
-void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, T *C, const T *X, const T *Y,
-                      bool strict, hipStream_t st) {
+void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, bool right, T *C, const T *X,
+                      const T *Y, bool strict, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, C, X, Y, strict ? 1 : 0);
+    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, right ? 1 : 0, C, X, Y,
+                       strict ? 1 : 0);
 }
 template <typename T>
 void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool transposed_factor, const T *coef,
@@ -685,8 +696,8 @@ void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool t
     template void launch_potrf64<T>(const DevSymbol &, const int32_t *, int, int, T *, T *, T *, double, int *,    \
                                     hipStream_t);                                                                  \
     template void launch_panel_trsm<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, hipStream_t); \
-    template void launch_panel_upd<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, const T *,  \
-                                      bool, hipStream_t);                                                        \
+    template void launch_panel_upd<T>(const DevSymbol &, const PanelItem *, int, int, bool, T *, const T *,       \
+                                      const T *, bool, hipStream_t);                                             \
     template void launch_winv_offdiag<T>(const DevSymbol &, const int32_t *, int, bool, const T *, T *, T *,       \
                                          hipStream_t);
 INST(double)

@@ -190,6 +190,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     P.lv_diag_ptr.assign(nl + 1, 0);
     P.lv_wsize.assign(nl, 0);
     P.lv_single_pass.assign(nl, 0);
+    P.lv_panel_right.assign(nl, 0);
     P.bwd_itemptr.assign(cb + nl, 0);
     // update buffer: one row per off-diagonal panel row; CSR by permuted row
     std::vector<int64_t> yoff(cb + 1, 0);
@@ -226,6 +227,15 @@ Plan build_plan(const Symbol &s, const Layout &L) {
         int64_t maxk = 0;
         for (int32_t k = l0; k < l1; k++) maxk = std::max<int64_t>(maxk, width(P.lv_cblks[k]));
         const int64_t nsteps = (maxk + 63) / 64;
+        // in-cblk updates: left-looking (the step's block takes all earlier
+        // blocks' products, K up to the cblk width, one tile per row tile)
+        // where the level has rows enough to fill the device; right-looking
+        // (every later block takes the step's products, 64 deep) near the top
+        // of the tree, where few row tiles would leave the device idle
+        int64_t rowtiles = 0;
+        for (int32_t k = l0; k < l1; k++) rowtiles += (L.cblk_stride[P.lv_cblks[k]] + 63) / 64;
+        const bool right = rowtiles < PANEL_RIGHT_ROWTILES;
+        P.lv_panel_right[lv] = right;
         for (int64_t kb = 0; kb < nsteps; kb++) {
             const int64_t k0 = kb * 64;
             for (int32_t k = l0; k < l1; k++) {
@@ -235,10 +245,17 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                 const int64_t nb = std::min<int64_t>(64, K - k0);
                 P.step_cblks.push_back(c);
                 for (int64_t r0 = k0 + nb; r0 < st; r0 += 64) P.ptrsm.push_back(PanelItem{c, (int32_t)r0, 0, 0});
-                // left-looking update of this column block by the blocks
-                // before it, from the diagonal tile down
-                if (kb > 0)
+                if (right) {
+                    // every later column block by this one, from each
+                    // block's diagonal tile down
+                    for (int64_t c0 = k0 + 64; c0 < K; c0 += 64)
+                        for (int64_t r0 = c0; r0 < st; r0 += 64)
+                            P.pupd.push_back(PanelItem{c, (int32_t)r0, (int32_t)c0, 0});
+                } else if (kb > 0) {
+                    // this column block by the blocks before it, from the
+                    // diagonal tile down
                     for (int64_t r0 = k0; r0 < st; r0 += 64) P.pupd.push_back(PanelItem{c, (int32_t)r0, (int32_t)k0, 0});
+                }
             }
             P.step_cb_ptr.push_back((int32_t)P.step_cblks.size());
             P.step_trsm_ptr.push_back((int32_t)P.ptrsm.size());

@@ -12,6 +12,9 @@ namespace pxa {
 // a level whose multi-source tiles average at most this many entries skips
 // the product workspace (the one-pass update kernel sums its entries)
 constexpr double SINGLE_PASS_ENTS = 2.05;
+// levels with fewer 64-row panel tiles than this update their panels
+// right-looking (64-deep products over all later blocks)
+constexpr int64_t PANEL_RIGHT_ROWTILES = 200;
 
 struct Plan {
     int32_t nlevels = 0;
@@ -21,11 +24,15 @@ struct Plan {
     // step s is column block kb = s - lv_step_ptr[L] of the level's cblks
     std::vector<int32_t> lv_step_ptr;
     std::vector<int32_t> step_cb_ptr, step_cblks;
-    // per step: panel-TRSM row tiles below the sub-block, and (from the
-    // second step on) the row tiles, from the diagonal tile down, of the
-    // left-looking update of the step's column block by the blocks before it
+    // per step: panel-TRSM row tiles below the sub-block, and the tiles of
+    // the in-cblk update: on right-looking levels (lv_panel_right), after the
+    // step, of the later column blocks by the step's block (for each later
+    // block, its row tiles from the diagonal tile down); on the others,
+    // before the step, of the step's column block by the blocks before it
+    // (from the diagonal tile down)
     std::vector<int32_t> step_trsm_ptr, step_upd_ptr;
     std::vector<PanelItem> ptrsm, pupd;
+    std::vector<uint8_t> lv_panel_right;
     std::vector<int32_t> wide_cblks;  // cblks wider than 64 columns
     // update tiles: items of level L come from the sources factored at level L;
     // [lv_upd_ptr[L], lv_upd_split[L]) have a single entry, the rest several
