@@ -6,7 +6,7 @@ package, as the checker and as the timed CPU baseline.  The product
 
 `Oracle` wraps liboracle.so (oracle.c: a plain-C restatement of the
 reference sequential supernodal factorisation, solve, SpMV, flop count and
-CG refinement) over the block structure exported by the engine's analysis.
+the CG, simple and BiCGSTAB refinement drivers) over the block structure exported by the engine's analysis.
 `bcsc()` restates the reference permuted block-CSC construction
 (bcsc/bcsc_zinit.c:485-541) with scipy: B = P A P^T with its full pattern.
 """
@@ -45,6 +45,9 @@ def _lib():
     lib.ora_thflops.restype = ct.c_double
     lib.ora_cg.argtypes = [P, P, ct.c_int64, P, P, P, P, P, ct.c_int, ct.c_double, P]
     lib.ora_cg.restype = ct.c_int
+    for f in (lib.ora_sr, lib.ora_bicgstab):
+        f.argtypes = [P, ct.c_int, P, P, ct.c_int64, P, P, P, P, P, ct.c_int, ct.c_double, P]
+        f.restype = ct.c_int
     return lib
 
 
@@ -122,6 +125,23 @@ class Oracle:
                              bc["trows"].ctypes.data, bc["tvals"].ctypes.data, b.ctypes.data, x.ctypes.data,
                              itermax, eps, ct.byref(res))
         return x, it, res.value
+
+    def _refine(self, fn, L, U, bc, b, x, itermax, eps):
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        res = ct.c_double()
+        it = fn(ct.byref(self.s), 1 if U is not None else 0, L.ctypes.data, U.ctypes.data if U is not None else None,
+                self.n, bc["tcolptr"].ctypes.data, bc["trows"].ctypes.data, bc["tvals"].ctypes.data, b.ctypes.data,
+                x.ctypes.data, itermax, eps, ct.byref(res))
+        return x, it, res.value
+
+    def sr(self, L, U, bc, b, x, itermax=250, eps=1e-12):
+        """Simple refinement (reference refinement/z_refine_pivot.c); returns (x, passes, ||r||/||b||)."""
+        return self._refine(self.lib.ora_sr, L, U, bc, b, x, itermax, eps)
+
+    def bicgstab(self, L, U, bc, b, x, itermax=250, eps=1e-12):
+        """Preconditioned BiCGSTAB (reference refinement/z_refine_bicgstab.c); returns (x, iterations, ||r||/||b||)."""
+        return self._refine(self.lib.ora_bicgstab, L, U, bc, b, x, itermax, eps)
 
     # ---- helpers on the 1D panel layout ----
     def to_dense(self, L, lower_only=True):

@@ -391,3 +391,82 @@ int ora_cg(const ora_symbol_t *s, const double *L, int64_t n, const int64_t *row
     if (resid) *resid = rb;
     return it;
 }
+
+/* Simple iterative refinement, reference refinement/z_refine_pivot.c:26-119.
+ * Each pass: r = b - A x, berr = ||r||_2 / ||b||_2; while iter < itermax,
+ * berr > eps and berr <= last_berr / 2 (the first pass takes last_berr =
+ * 3 berr), x += M^{-1} r with M^{-1} the supernodal solve.  Returns the number
+ * of passes, the stopping one included. */
+int ora_sr(const ora_symbol_t *s, int lu, const double *L, const double *U, int64_t n, const int64_t *rowptr,
+           const int64_t *cols, const double *vals, const double *b, double *x, int itermax, double eps,
+           double *resid) {
+    double *r = malloc(2 * n * sizeof(double));
+    if (!r) return -1;
+    double *dx = r + n;
+    const double normb = sqrt(dot(n, b, b));
+    double berr = 0.0, last = 0.0;
+    int it = 0;
+    for (;;) {
+        memcpy(r, b, n * sizeof(double));
+        ora_spmv(n, rowptr, cols, vals, -1.0, x, 1.0, r);
+        berr = sqrt(dot(n, r, r)) / normb;
+        if (it == 0) last = 3.0 * berr;
+        const int go = it < itermax && berr > eps && berr <= last / 2.0;
+        it++;
+        if (!go) break;
+        memcpy(dx, r, n * sizeof(double));
+        ora_solve(s, lu, L, U, dx, n, 1);
+        for (int64_t i = 0; i < n; i++) x[i] += dx[i];
+        last = berr;
+    }
+    free(r);
+    if (resid) *resid = berr;
+    return it;
+}
+
+/* Preconditioned BiCGSTAB, reference refinement/z_refine_bicgstab.c:25-185:
+ * y = M^{-1} p, v = A y, alpha = (r, r2) / (v, r2), s = r - alpha v,
+ * z = M^{-1} s, t = A z, w = (M^{-1} t, z) / (M^{-1} t, M^{-1} t),
+ * x += alpha y + w z, r = s - w t, beta = (r, r2) / (r_old, r2) * alpha / w,
+ * p = r + beta (p - w v); stops when ||r|| / ||b|| <= eps or after itermax
+ * iterations.  Returns the iterations. */
+int ora_bicgstab(const ora_symbol_t *s, int lu, const double *L, const double *U, int64_t n, const int64_t *rowptr,
+                 const int64_t *cols, const double *vals, const double *b, double *x, int itermax, double eps,
+                 double *resid) {
+    double *r = malloc(9 * n * sizeof(double));
+    if (!r) return -1;
+    double *r2 = r + n, *p = r2 + n, *y = p + n, *v = y + n, *sv = v + n, *z = sv + n, *t = z + n, *mt = t + n;
+    const double normb = sqrt(dot(n, b, b)), normx = sqrt(dot(n, x, x));
+    memcpy(r, b, n * sizeof(double));
+    if (normx > 0.0) ora_spmv(n, rowptr, cols, vals, -1.0, x, 1.0, r);
+    double rb = sqrt(dot(n, r, r)) / normb;
+    memcpy(r2, r, n * sizeof(double));
+    memcpy(p, r, n * sizeof(double));
+    int it = 0;
+    while (rb > eps && it < itermax) {
+        it++;
+        memcpy(y, p, n * sizeof(double));
+        ora_solve(s, lu, L, U, y, n, 1);
+        ora_spmv(n, rowptr, cols, vals, 1.0, y, 0.0, v);
+        const double rr2 = dot(n, r, r2);
+        const double alpha = rr2 / dot(n, v, r2);
+        for (int64_t i = 0; i < n; i++) sv[i] = r[i] - alpha * v[i];
+        memcpy(z, sv, n * sizeof(double));
+        ora_solve(s, lu, L, U, z, n, 1);
+        ora_spmv(n, rowptr, cols, vals, 1.0, z, 0.0, t);
+        memcpy(mt, t, n * sizeof(double));
+        ora_solve(s, lu, L, U, mt, n, 1);
+        const double w = dot(n, mt, z) / dot(n, mt, mt);
+        for (int64_t i = 0; i < n; i++) {
+            x[i] += alpha * y[i];
+            x[i] += w * z[i];
+            r[i] = sv[i] - w * t[i];
+        }
+        const double beta = dot(n, r, r2) / rr2 * (alpha / w);
+        for (int64_t i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - w * v[i]);
+        rb = sqrt(dot(n, r, r)) / normb;
+    }
+    free(r);
+    if (resid) *resid = rb;
+    return it;
+}

@@ -22,4 +22,10 @@ void   ora_spmv(int64_t n, const int64_t *rowptr, const int64_t *cols, const dou
 double ora_thflops(const ora_symbol_t *s, int lu);
 int    ora_cg(const ora_symbol_t *s, const double *L, int64_t n, const int64_t *rowptr, const int64_t *cols,
               const double *vals, const double *b, double *x, int itermax, double eps, double *resid);
+int    ora_sr(const ora_symbol_t *s, int lu, const double *L, const double *U, int64_t n, const int64_t *rowptr,
+              const int64_t *cols, const double *vals, const double *b, double *x, int itermax, double eps,
+              double *resid);
+int    ora_bicgstab(const ora_symbol_t *s, int lu, const double *L, const double *U, int64_t n,
+                    const int64_t *rowptr, const int64_t *cols, const double *vals, const double *b, double *x,
+                    int itermax, double eps, double *resid);
 #endif

@@ -157,7 +157,14 @@ double *pxa_api::device_block(pastix_data_t *d, int64_t count) {
     return d->batch_dev;
 }
 
-int pxa_api::refine_method(const pastix_data_t *d) { return d->iparm[IPARM_REFINEMENT] == PastixRefineCG ? 1 : 0; }
+int pxa_api::refine_method(const pastix_data_t *d) {
+    switch (d->iparm[IPARM_REFINEMENT]) {
+    case PastixRefineCG: return 1;
+    case PastixRefineSR: return 2;
+    case PastixRefineBiCGSTAB: return 3;
+    default: return 0;  // GMRES, as the reference's refinement table (pastix_task_refine.c:36-66)
+    }
+}
 
 double pxa_api::refine_eps(const pastix_data_t *d) {
     const double eps = d->dparm[DPARM_EPSILON_REFINEMENT];

@@ -865,12 +865,19 @@ int Engine::refine_user(const double *Bu, int64_t ldb, double *Xu, int64_t ldx, 
     double res = 0;
     if (method == 1) {
         if (refine_cg_batch(PX, PB, R, segoff, itermax, eps, &it, &res)) return -1;
-    } else if (method == 0) {
-        if (segoff.size() != 2) return fail("GMRES refinement runs on one system");
+    } else if (method == 0 || method == 2 || method == 3) {
+        if (segoff.size() != 2) return fail("GMRES, simple and BiCGSTAB refinement run on one system");
+        // one right-hand side at a time, as the reference
+        // (refinement/pastix_task_refine.c:157-161)
         for (int c = 0; c < R; c++) {
             int i2 = 0;
             double r2 = 0;
-            if (refine_gmres(PX + c * n_, PB + c * n_, itermax, im, eps, &i2, &r2)) return -1;
+            double *xc = PX + c * n_;
+            const double *bc = PB + c * n_;
+            const int rc = (method == 0)   ? refine_gmres(xc, bc, itermax, im, eps, &i2, &r2)
+                           : (method == 2) ? refine_sr(xc, bc, itermax, eps, &i2, &r2)
+                                           : refine_bicgstab(xc, bc, itermax, eps, &i2, &r2);
+            if (rc) return -1;
             it = std::max(it, i2);
             res = std::max(res, r2);
         }
@@ -935,6 +942,98 @@ int Engine::refine_cg(double *x, const double *b, int itermax, double eps, int *
         double beta = rz2 / rz;
         launch_scal(n_, beta, p, stream_);
         launch_axpy(n_, 1.0, z, p, stream_);
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (niter) *niter = it;
+    if (resid) *resid = rb;
+    return 0;
+}
+
+// Simple iterative refinement, reference refinement/z_refine_pivot.c:26-119:
+// r = b - A x and berr = ||r|| / ||b|| on every pass; while iter < itermax,
+// berr > eps and berr has at least halved since the last correction (the first
+// pass compares with 3 berr), x += M^{-1} r.  niter counts the passes, the
+// last, stopping one included, as the reference's return value does.
+int Engine::refine_sr(double *x, const double *b, int itermax, double eps, int *niter, double *resid) {
+    if (vec_cap_ < 2 * n_) {
+        vec_cap_ = 2 * n_;
+        d_vec_ = (double *)alloc(vec_cap_ * sizeof(double));
+        d_svec_ = (float *)alloc(n_ * sizeof(float));
+        if (!d_vec_ || !d_svec_) return -1;
+    }
+    double *r = d_vec_, *dx = d_vec_ + n_;
+    const double normb = std::sqrt(dot(b, b));
+    double berr = 0, last = 0;
+    int it = 0;
+    for (;;) {
+        launch_copy(n_, b, r, stream_);
+        spmv(-1.0, x, 1.0, r);
+        const double normr = std::sqrt(dot(r, r));
+        berr = (normb > 0) ? normr / normb : normr;
+        if (it == 0) last = 3 * berr;
+        const bool go = it < itermax && berr > eps && berr <= last / 2.;
+        it++;
+        if (!go) break;
+        launch_copy(n_, r, dx, stream_);
+        if (precond(dx)) return -1;
+        launch_axpy(n_, 1.0, dx, x, stream_);
+        last = berr;
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (niter) *niter = it;
+    if (resid) *resid = berr;
+    return 0;
+}
+
+// Preconditioned BiCGSTAB, reference refinement/z_refine_bicgstab.c:25-185,
+// step for step: y = M^{-1} p, v = A y, alpha = (r, r2) / (v, r2),
+// s = r - alpha v, z = M^{-1} s, t = A z, and the weight
+// w = (M^{-1} t, z) / (M^{-1} t, M^{-1} t) from preconditioned vectors (three
+// supernodal solves per iteration); x += alpha y + w z, r = s - w t,
+// beta = (r, r2) / (r_old, r2) * alpha / w, p = r + beta (p - w v).
+int Engine::refine_bicgstab(double *x, const double *b, int itermax, double eps, int *niter, double *resid) {
+    if (vec_cap_ < 9 * n_) {
+        vec_cap_ = 9 * n_;
+        d_vec_ = (double *)alloc(vec_cap_ * sizeof(double));
+        d_svec_ = (float *)alloc(n_ * sizeof(float));
+        if (!d_vec_ || !d_svec_) return -1;
+    }
+    double *r = d_vec_, *r2 = r + n_, *p = r2 + n_, *y = p + n_, *v = y + n_, *sv = v + n_, *z = sv + n_,
+           *t = z + n_, *mt = t + n_;
+    const double normb = std::sqrt(dot(b, b));
+    const double normx = std::sqrt(dot(x, x));
+    launch_copy(n_, b, r, stream_);
+    if (normx > 0.) spmv(-1.0, x, 1.0, r);
+    double rb = std::sqrt(dot(r, r));
+    rb = (normb > 0) ? rb / normb : rb;
+    launch_copy(n_, r, r2, stream_);
+    launch_copy(n_, r, p, stream_);
+    int it = 0;
+    while (rb > eps && it < itermax) {
+        it++;
+        launch_copy(n_, p, y, stream_);
+        if (precond(y)) return -1;
+        spmv(1.0, y, 0.0, v);
+        const double rr2 = dot(r, r2);
+        const double alpha = rr2 / dot(v, r2);
+        launch_copy(n_, r, sv, stream_);
+        launch_axpy(n_, -alpha, v, sv, stream_);
+        launch_copy(n_, sv, z, stream_);
+        if (precond(z)) return -1;
+        spmv(1.0, z, 0.0, t);
+        launch_copy(n_, t, mt, stream_);
+        if (precond(mt)) return -1;
+        const double w = dot(mt, z) / dot(mt, mt);
+        launch_axpy(n_, alpha, y, x, stream_);
+        launch_axpy(n_, w, z, x, stream_);
+        launch_copy(n_, sv, r, stream_);
+        launch_axpy(n_, -w, t, r, stream_);
+        const double beta = dot(r, r2) / rr2 * (alpha / w);
+        launch_axpy(n_, -w, v, p, stream_);
+        launch_scal(n_, beta, p, stream_);
+        launch_axpy(n_, 1.0, r, p, stream_);
+        const double nr = std::sqrt(dot(r, r));
+        rb = (normb > 0) ? nr / normb : nr;
     }
     HIPCHK(hipStreamSynchronize(stream_));
     if (niter) *niter = it;

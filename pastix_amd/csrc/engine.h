@@ -41,6 +41,8 @@ class Engine {
     // Device refinement drivers (vectors in the permuted ordering, FP64).
     int refine_cg(double *dx, const double *db, int itermax, double eps, int *niter, double *resid);
     int refine_gmres(double *dx, const double *db, int itermax, int im, double eps, int *niter, double *resid);
+    int refine_sr(double *dx, const double *db, int itermax, double eps, int *niter, double *resid);
+    int refine_bicgstab(double *dx, const double *db, int itermax, double eps, int *niter, double *resid);
     // Batched CG over independent systems (consecutive row segments segoff of
     // the permuted order) and R right-hand-side columns (n x R blocks, ld = n):
     // every (system, column) pair runs its own CG recurrence and stops at its
@@ -51,7 +53,8 @@ class Engine {
     // FP64 device right-hand sides B and solutions X (ld >= n), in the user
     // ordering when user_order: X = M^{-1} B first if initial_solve (else X
     // holds the initial guesses), then refinement method 1 = batched CG over
-    // the row segments segoff, 0 = GMRES per column (one segment), -1 = none.
+    // the row segments segoff, 0 = GMRES, 2 = simple refinement, 3 = BiCGSTAB
+    // (these three per column, one segment), -1 = none.
     int refine_user(const double *B, int64_t ldb, double *X, int64_t ldx, int R, const std::vector<int64_t> &segoff,
                     int method, bool initial_solve, bool user_order, int itermax, int im, double eps, int *niter,
                     double *resid);

@@ -9,6 +9,7 @@ import pytest
 
 from pastix_amd import pastix, spm
 from tests.helpers import analyse, oracle_factor, rel_frob, lower_mask_panel
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -354,4 +355,77 @@ def test_static_pivot_on_device():
     O, bc, Lo, Uo = oracle_factor(A2, sym, lu=True, crit=thr)
     assert rel_frob(Lg, Lo) <= 1e-12 and rel_frob(Ug, Uo) <= 1e-12
     assert Lg[0] == pytest.approx(thr, rel=1e-15)
+    pastix.finalize(h)
+
+
+@pytest.mark.parametrize("method", ["SR", "BiCGSTAB"])
+def test_refine_sr_bicgstab_fp64_factor_matches_oracle(method):
+    """Simple refinement and BiCGSTAB (reference refinement/z_refine_pivot.c,
+    z_refine_bicgstab.c) on the device with the FP64 factor: the oracle's
+    drivers on the oracle's factor take the same number of steps (the exact
+    preconditioner: 2 passes of simple refinement, 1 BiCGSTAB iteration), and
+    the iterates agree."""
+    grid = (10, 9, 8)
+    A = spm.laplacian7(*grid, coef=(1.0, 1.0, 0.01))
+    h, S, sym = analyse(A, True, LLT, grid=grid)
+    h.iparm[pastix.iparm.refinement] = getattr(pastix.refine, method)
+    pastix.task_numfact(h, S)
+    x0, b = spm.rhs_from_x(A, 1, seed=31)
+    x = np.zeros_like(b, order="F")
+    pastix.task_refine(h, S, b, x)
+    it = int(h.iparm[pastix.iparm.nbiter])
+    O, bc, Lo, _ = oracle_factor(A, sym)
+    p = sym["permtab"]
+    bp = np.zeros(A.shape[0])
+    bp[p] = b[:, 0]
+    fn = O.sr if method == "SR" else O.bicgstab
+    xo, ito, reso = fn(Lo, None, bc, bp, np.zeros(A.shape[0]), itermax=250, eps=1e-12)
+    assert it == ito == (2 if method == "SR" else 1)
+    assert np.abs(x[:, 0] - xo[p]).max() <= 1e-13 * np.abs(xo).max()
+    assert _berr(A, x[:, 0], b[:, 0]) <= 1e-15
+    pastix.finalize(h)
+
+
+@pytest.mark.parametrize("method", ["SR", "BiCGSTAB"])
+@pytest.mark.parametrize("facto", [LLT, LU])
+def test_refine_sr_bicgstab_fp32_factor(method, facto):
+    """FP32 factor + FP64 simple refinement / BiCGSTAB reach berr <= 1e-12;
+    the device-vector entry point runs the same recurrence (same iterate,
+    same count), and the oracle's driver preconditioned with the same FP32
+    factor values (applied in FP64) takes about as many steps."""
+    grid = (12, 11, 10)
+    if facto == LLT:
+        A = spm.laplacian7(*grid, coef=(1.0, 1.0, 0.01))
+    else:
+        A = spm.convection_diffusion27(*grid, seed=5)
+    h, S, sym = analyse(A, facto == LLT, facto, grid=grid)
+    h.iparm[pastix.iparm.refinement] = getattr(pastix.refine, method)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.factor_precision, spm.SpmFloat)
+    if facto == LU:
+        h.dparm[pastix.dparm.epsilon_magn_ctrl] = np.ldexp(1.0, -52)
+    pastix.task_numfact(h, S)
+    x0, b = spm.rhs_from_x(A, 1, seed=17)
+    x = b.copy(order="F")
+    pastix.task_solve(h, S, x)
+    pastix.task_refine(h, S, b, x)
+    it = int(h.iparm[pastix.iparm.nbiter])
+    assert _berr(A, x[:, 0], b[:, 0]) <= 1e-12
+    db = pastix.DeviceArray(b)
+    dx = pastix.DeviceArray(np.zeros_like(b))
+    pastix.solve_refine_device(h, db, dx, A.shape[0], 1)
+    assert int(h.iparm[pastix.iparm.nbiter]) == it
+    assert np.allclose(dx.download(), x, rtol=1e-13, atol=0)
+    Lg, Ug = pastix.coeftab(h, dtype=np.float32, lu=facto == LU)
+    O = oracle.Oracle(sym)
+    bc = oracle.bcsc(A, sym["permtab"])
+    p = sym["permtab"]
+    bp = np.zeros(A.shape[0])
+    bp[p] = b[:, 0]
+    xs = bp.copy()
+    O.solve(Lg.astype(np.float64), None if Ug is None else Ug.astype(np.float64), xs)
+    fn = O.sr if method == "SR" else O.bicgstab
+    xo, ito, reso = fn(Lg.astype(np.float64), None if Ug is None else Ug.astype(np.float64), bc, bp, xs, eps=1e-12)
+    print(method, facto, "device steps", it, "oracle steps", ito, "oracle res", reso)
+    assert reso <= 1e-12 and abs(it - ito) <= 1
+    assert np.abs(x[:, 0] - xo[p]).max() <= 1e-9 * np.abs(xo).max()
     pastix.finalize(h)

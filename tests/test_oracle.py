@@ -7,6 +7,12 @@
 - Solves: dense numpy.linalg.solve.  SpMV: scipy.sparse products.
 - Flop counts: closed forms checked against brute-force operation counts.
 - CG refinement: convergence to the stop criterion.
+- Simple refinement: on A = I with M = c I every pass contracts the error by
+  rho = 1 - 1/c exactly, so the pass count and the stagnation stop
+  (berr must halve) have closed forms.
+- BiCGSTAB: with the exact factor as preconditioner it finishes in one
+  iteration; with a perturbed one it converges and its recursive residual
+  stays the true one.
 A planted mistake (dropping an update, a transposed operand, a wrong pivot
 rule) fails at least one of these.
 """
@@ -193,4 +199,74 @@ def test_cg_refinement_converges():
     x, it, res = O.cg(Lp, bc, b, np.zeros(A.shape[0]), itermax=50, eps=1e-12)
     assert res <= 1e-12 and 1 <= it <= 10
     assert np.abs(bc["B"] @ x - b).max() <= 1e-10
+    pastix.finalize(h)
+
+
+def _identity_case(c):
+    """A = I on a 3D grid's symbol with the factor sqrt(c) I: M = c I."""
+    grid = (6, 5, 4)
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid)
+    import scipy.sparse as sps
+    I = sps.identity(A.shape[0], format="csc")
+    O = oracle.Oracle(sym)
+    bc = oracle.bcsc(I, sym["permtab"])
+    L, _, rc = O.fill(bc)
+    assert rc == 0
+    pastix.finalize(h)
+    return O, bc, L * np.sqrt(c)
+
+
+@pytest.mark.parametrize("rho,passes,stop_berr", [(0.25, 21, None), (0.6, 2, 0.6), (0.0, 2, None)])
+def test_sr_pass_counts_closed_form(rho, passes, stop_berr):
+    # x_{k+1} = x_k + (b - x_k) / c: berr_k = rho^k with rho = 1 - 1/c.  The
+    # reference loop (z_refine_pivot.c) corrects while berr > eps and berr <=
+    # last_berr / 2: rho = 0.25 needs 20 corrections to reach 1e-12 (21 passes
+    # with the stopping one); rho = 0.6 stops after one correction, berr 0.6
+    # not having halved; the exact preconditioner takes one correction.
+    O, bc, L = _identity_case(1.0 / (1.0 - rho))
+    b = np.random.default_rng(3).uniform(-1, 1, O.n)
+    x, it, res = O.sr(L, None, bc, b, np.zeros(O.n), itermax=100, eps=1e-12)
+    assert it == passes
+    if stop_berr is None:
+        assert res <= 1e-12
+        assert np.abs(x - b).max() <= 1e-11
+    else:
+        assert res == pytest.approx(stop_berr, rel=1e-12)
+        assert np.abs(x - b / (1.0 / (1.0 - rho))).max() <= 1e-14
+
+
+def test_sr_itermax_bound():
+    O, bc, L = _identity_case(1.0 / (1.0 - 0.25))
+    b = np.random.default_rng(4).uniform(-1, 1, O.n)
+    x, it, res = O.sr(L, None, bc, b, np.zeros(O.n), itermax=3, eps=1e-12)
+    # 3 corrections, then the pass that stops on itermax
+    assert it == 4 and res == pytest.approx(0.25 ** 3, rel=1e-10)
+
+
+@pytest.mark.parametrize("lu", [False, True])
+def test_bicgstab_refinement(lu):
+    grid = (7, 6, 5)
+    if lu:
+        A = spm.convection_diffusion27(*grid, seed=4)
+    else:
+        A = spm.laplacian7(*grid, coef=(1.0, 1.0, 0.01))
+    h, S, sym = analyse(A, not lu, LU if lu else LLT, grid=grid)
+    O, bc, L, U = oracle_factor(A, sym, lu=lu)
+    b = np.random.default_rng(6).uniform(-1, 1, A.shape[0])
+    # exact preconditioner: alpha = 1 and s is rounding noise, one iteration
+    x, it, res = O.bicgstab(L, U, bc, b, np.zeros(A.shape[0]), itermax=50, eps=1e-12)
+    assert it == 1 and res <= 1e-12
+    # perturbed preconditioner: converges, and the final iterate's true
+    # residual agrees with the recursive one
+    rng = np.random.default_rng(0)
+    Lp = L * (1.0 + 1e-5 * rng.standard_normal(L.shape))
+    Up = None if U is None else U * (1.0 + 1e-5 * rng.standard_normal(U.shape))
+    x, it, res = O.bicgstab(Lp, Up, bc, b, np.zeros(A.shape[0]), itermax=50, eps=1e-12)
+    assert 2 <= it <= 20 and res <= 1e-12
+    true = np.linalg.norm(bc["B"] @ x - b) / np.linalg.norm(b)
+    assert true <= 1e-11
+    # simple refinement on the same perturbed factor converges too
+    x2, it2, res2 = O.sr(Lp, Up, bc, b, np.zeros(A.shape[0]), itermax=50, eps=1e-12)
+    assert res2 <= 1e-12 and np.abs(x2 - x).max() <= 1e-10 * np.abs(x).max()
     pastix.finalize(h)
