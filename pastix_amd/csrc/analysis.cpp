@@ -51,6 +51,9 @@ double symbol_thflops(const Symbol &s, int factotype) {
         if (factotype == 2) {  // GETRF
             total += fl_getrf(N, N);
             if (M > 0) total += 2. * fl_trsm_right(M, N) + fl_gemm(M, M, N);
+        } else if (factotype == 1) {  // SYTRF / LDL^t (symbol_cost_flops.c:85-88,142-144,199-201)
+            total += N * ((N / 6. + 0.5) * N + 10. / 3.) + N * ((N / 6.) * N - 1. / 6.);
+            if (M > 0) total += fl_trsm_right(M, N) + N * M + fl_syrk(N, M) + M * M;
         } else {               // POTRF / LLT
             total += fl_potrf(N);
             if (M > 0) total += fl_trsm_right(M, N) + fl_syrk(N, M);

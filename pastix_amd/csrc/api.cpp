@@ -308,7 +308,7 @@ extern "C" int pastix_subtask_blend(pastix_data_t *d) {
     d->layout = pxa::build_layout(d->symbol);
     d->plan = pxa::build_plan(d->symbol, d->layout);
     int facto = (int)d->iparm[IPARM_FACTORIZATION];
-    int ft = (facto == PastixFactGETRF) ? 2 : 0;
+    int ft = (facto == PastixFactGETRF) ? 2 : is_ldlt(d) ? 1 : 0;
     d->dparm[DPARM_FACT_THFLOPS] = pxa::symbol_thflops(d->symbol, ft);
     if (facto == PastixFactLU) d->iparm[IPARM_NNZEROS] *= 2;
     d->dparm[DPARM_SOLV_FLOPS] = (double)d->iparm[IPARM_NNZEROS];
@@ -428,11 +428,11 @@ extern "C" int pastix_subtask_bcsc2ctab(pastix_data_t *d) {
         err_print("pastix_subtask_bcsc2ctab", "All steps from pastix_task_init() to pastix_stask_spm2bcsc() have to be called before calling this function");
         return PASTIX_ERR_BADPARAMETER;
     }
-    const int facto = (d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF) ? 2 : 0;
-    if (d->iparm[IPARM_FACTORIZATION] == PastixFactSYTRF || d->iparm[IPARM_FACTORIZATION] == PastixFactHETRF) {
-        err_print("pastix_subtask_bcsc2ctab", "LDL^t factorizations are not provided by this engine");
-        return PASTIX_ERR_NOTIMPLEMENTED;
-    }
+    // LDL^t runs on the LU engine: on a symmetric matrix, LU without pivoting
+    // gives L (unit) with D = diag(U) and U = D L^t, so the L panel ends with
+    // the reference's LDL^t layout (kernels/core_zsytrfsp.c:162-168: L below
+    // the diagonal, D on it, D L^t above) and the LU solve is the LDL^t solve
+    const int facto = (d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF || is_ldlt(d)) ? 2 : 0;
     const int prec = factor_prec(d);
     if (!d->engine || !d->engine->ready() || d->engine->prec() != prec || d->engine->factotype() != facto) {
         d->engine.reset(new pxa::Engine());
@@ -481,7 +481,7 @@ extern "C" int pastix_subtask_sopalin(pastix_data_t *d) {
     d->fact_times[4] = d->engine->n_diag;
     d->fact_times[5] = 0;
     d->fact_times[6] = d->engine->n_upd;
-    if (neg && d->iparm[IPARM_FACTORIZATION] != PastixFactGETRF) {
+    if (neg && d->iparm[IPARM_FACTORIZATION] != PastixFactGETRF && !is_ldlt(d)) {
         // The reference aborts on a negative pivot (kernels/core_zpotrfsp.c:84-89);
         // this engine reports the failure instead.
         err_print("pastix_subtask_sopalin", "Negative diagonal term");
@@ -838,7 +838,7 @@ extern "C" int pastix_amd_solve_refine_device(pastix_data_t *d, pastix_int_t nrh
 }
 extern "C" int pastix_amd_flops(const pastix_data_t *d, double out[4]) {
     if (!d || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
-    const bool lu = d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF;
+    const bool lu = d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF || is_ldlt(d);
     out[0] = d->dparm[DPARM_FACT_THFLOPS];
     out[1] = d->plan.flops_diag;
     out[2] = d->plan.flops_trsm * (lu ? 2.0 : 1.0);

@@ -81,6 +81,10 @@ void triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vec
 double *device_block(pastix_data_t *d, int64_t count);
 // Refinement method code of the instance for Engine::refine_user: 1 CG, 0 GMRES.
 int refine_method(const pastix_data_t *d);
+// LDL^t requested (PastixFactSYTRF / PastixFactHETRF, the same in real arithmetic)
+inline bool is_ldlt(const pastix_data_t *d) {
+    return d->iparm[IPARM_FACTORIZATION] == PastixFactSYTRF || d->iparm[IPARM_FACTORIZATION] == PastixFactHETRF;
+}
 double refine_eps(const pastix_data_t *d);
 // Host FP64 blocks (n x R, ld n, user ordering of the instance): X = M^{-1} B
 // when initial_solve (else X holds the guesses), then batched CG over the

@@ -183,7 +183,7 @@ extern "C" int pastix_amd_server_create(pastix_amd_server_t **srv, pastix_data_t
         pastix_data_t *d = slots[k];
         if (!d || !d->engine || !(d->steps & STEP_NUMFACT) || !d->ext[PXA_IPARM_RESIDENT_INPUT] ||
             d->n != slots[0]->n || d->symbol.cblknbr() != slots[0]->symbol.cblknbr() ||
-            d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF)
+            d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF || is_ldlt(d))
             return PASTIX_ERR_BADPARAMETER;
     }
     auto *S = new pastix_amd_server_s();

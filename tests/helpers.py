@@ -43,7 +43,8 @@ def oracle_factor(A, sym, lu=False, crit=None):
 
 
 def read_rua(path):
-    """Harwell-Boeing real unsymmetric assembled matrix (fixed-width Fortran fields)."""
+    """Harwell-Boeing real assembled matrix (fixed-width Fortran fields): RUA
+    as stored, RSA (lower triangle stored) expanded to both triangles."""
     import re
     import scipy.sparse as sps
     lines = open(path).read().split("\n")
@@ -69,7 +70,11 @@ def read_rua(path):
     ptr = read(s, ptrcrd, fmts[0], ncol + 1, int)
     ind = read(s + ptrcrd, indcrd, fmts[1], nnz, int)
     val = read(s + ptrcrd + indcrd, valcrd, fmts[2], nnz, float)
-    return sps.csc_matrix((np.array(val), np.array(ind) - 1, np.array(ptr) - 1), shape=(nrow, ncol))
+    A = sps.csc_matrix((np.array(val), np.array(ind) - 1, np.array(ptr) - 1), shape=(nrow, ncol))
+    if lines[2][:3].upper() == "RSA":
+        # symmetric assembled: the lower triangle is stored
+        A = (A + A.T - sps.diags(A.diagonal())).tocsc()
+    return A
 
 
 def rel_frob(a, b):

@@ -168,3 +168,24 @@ def test_update_flops_partition_the_reference_count(grid):
     # product tiles cover at least the two-phase pairs they serve (one side)
     assert uf["wtile"] >= uf["multi"] * (1 - 1e-12)
     pastix.finalize(h)
+
+
+def test_ldlt_thflops_follow_reference_formulas():
+    # DPARM_FACT_THFLOPS for LDL^t: sum over cblks of FLOPS_DSYTRF(N) +
+    # FLOPS_DTRSM(Right, M, N) + N M + FLOPS_DSYRK(N, M) + M^2
+    # (symbol/symbol_cost_flops.c:85-88,142-144,199-201; common/flops.h:101-102)
+    grid = (9, 8, 7)
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, pastix.factotype.LDLT, grid=grid, min_bs=12, max_bs=40)
+    tot = 0.0
+    for c in range(sym["cblknbr"]):
+        N = float(sym["cblk_fcol"][c + 1] - sym["cblk_fcol"][c])
+        M = float(sum(sym["blok_lrow"][b] - sym["blok_frow"][b] + 1
+                      for b in range(sym["cblk_bloknum"][c] + 1, sym["cblk_bloknum"][c + 1])))
+        fm = N * ((N / 6 + 0.5) * N + 10 / 3)
+        fa = N * ((N / 6) * N - 1 / 6)
+        tot += fm + fa
+        if M > 0:
+            tot += M * N * (N + 1) + N * M + N * M * (M + 1) + M * M
+    assert h.dparm[pastix.dparm.fact_thflops] == pytest.approx(tot, rel=1e-12)
+    pastix.finalize(h)
