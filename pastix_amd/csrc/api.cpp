@@ -551,15 +551,52 @@ extern "C" int pastix_task_solve(pastix_data_t *d, pastix_int_t nrhs, void *b, p
     return PASTIX_SUCCESS;
 }
 
-extern "C" int pastix_subtask_trsm(pastix_data_t *, pastix_coeftype_t, pastix_side_t, pastix_uplo_t, pastix_trans_t,
-                                   pastix_diag_t, pastix_int_t, void *, pastix_int_t) {
-    err_print("pastix_subtask_trsm", "single sweeps are not exposed by this engine; use pastix_subtask_solve");
-    return PASTIX_ERR_NOTIMPLEMENTED;
+// One sweep of the solve on b in the permuted ordering (reference
+// sopalin/pastix_task_solve.c:219-293).  The sweeps of each factorisation, as
+// the reference's solve chains them (:433-516): LL^t forward (Lower, NoTrans,
+// NonUnit) and backward (Lower, Trans); LU forward (Lower, NoTrans, Unit) and
+// backward (Upper, NoTrans, NonUnit); LDL^t forward (Lower, NoTrans, Unit),
+// pastix_subtask_diag, backward (Lower, Trans, Unit).  Only the left side.
+extern "C" int pastix_subtask_trsm(pastix_data_t *d, pastix_coeftype_t flttype, pastix_side_t side,
+                                   pastix_uplo_t uplo, pastix_trans_t trans, pastix_diag_t diag, pastix_int_t nrhs,
+                                   void *b, pastix_int_t ldb) {
+    if (!d) { err_print("pastix_subtask_trsm", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!b) { err_print("pastix_subtask_trsm", "wrong b parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_NUMFACT)) { err_print("pastix_subtask_trsm", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
+    if (nrhs < 0 || ldb < d->n || (flttype != SpmDouble && flttype != SpmFloat)) return PASTIX_ERR_BADPARAMETER;
+    if (trans == PastixConjTrans) trans = PastixTrans;  // real arithmetic
+    const bool lu = d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF, ldlt = is_ldlt(d);
+    int part = 0;
+    if (side == PastixLeft) {
+        if (uplo == PastixLower && trans == PastixNoTrans && diag == ((lu || ldlt) ? PastixUnit : PastixNonUnit))
+            part = 1;
+        else if (lu && uplo == PastixUpper && trans == PastixNoTrans && diag == PastixNonUnit)
+            part = 2;
+        else if (!lu && uplo == PastixLower && trans == PastixTrans && diag == (ldlt ? PastixUnit : PastixNonUnit))
+            part = ldlt ? 3 : 2;
+    }
+    if (!part) {
+        err_print("pastix_subtask_trsm", "this side/uplo/trans/diag is not a sweep of the factor");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if (nrhs == 0) return PASTIX_SUCCESS;
+    double secs = 0;
+    if (d->engine->solve_host_part(part, flttype, b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
+    return PASTIX_SUCCESS;
 }
 
-extern "C" int pastix_subtask_diag(pastix_data_t *, pastix_coeftype_t, pastix_int_t, void *, pastix_int_t) {
-    err_print("pastix_subtask_diag", "LDL^t diagonal solves are not provided by this engine");
-    return PASTIX_ERR_NOTIMPLEMENTED;
+// b_i /= the factor's diagonal entry (D of LDL^t), reference
+// sopalin/pastix_task_solve.c:330-371 (solve_cblk_zdiag), b permuted.
+extern "C" int pastix_subtask_diag(pastix_data_t *d, pastix_coeftype_t flttype, pastix_int_t nrhs, void *b,
+                                   pastix_int_t ldb) {
+    if (!d) { err_print("pastix_subtask_diag", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!b) { err_print("pastix_subtask_diag", "wrong b parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_NUMFACT)) { err_print("pastix_subtask_diag", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
+    if (nrhs < 0 || ldb < d->n || (flttype != SpmDouble && flttype != SpmFloat)) return PASTIX_ERR_BADPARAMETER;
+    if (nrhs == 0) return PASTIX_SUCCESS;
+    double secs = 0;
+    if (d->engine->solve_host_part(4, flttype, b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
+    return PASTIX_SUCCESS;
 }
 
 // --------------------------------------------------------------------------

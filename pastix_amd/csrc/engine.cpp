@@ -389,6 +389,19 @@ int Engine::upload_perm(const std::vector<int64_t> &permtab) {
 // Device right-hand sides in the user ordering: permute into the work block,
 // solve, permute back (reference sopalin/pastix_task_solve.c:550-577 with
 // bvec_zlapmr replaced by a gather/scatter kernel).
+int Engine::solve_host_part(int part, int flttype, void *b, int64_t ldb, int nrhs, double *seconds) {
+    // b is in the permuted ordering already; partial sweeps run eagerly
+    std::vector<int64_t> id(n_);
+    for (int64_t i = 0; i < n_; i++) id[i] = i;
+    const bool g = use_graph;
+    use_graph = false;
+    part_ = part;
+    const int rc = solve_host(id.data(), flttype, b, ldb, nrhs, seconds);
+    part_ = 0;
+    use_graph = g;
+    return rc;
+}
+
 int Engine::enqueue_solve(void *dx, int64_t ldx, int nrhs, bool user) {
     if (!user) return (prec_ == 2) ? solve_t<float>((float *)dx, ldx, nrhs) : solve_t<double>((double *)dx, ldx, nrhs);
     if (prec_ == 2) launch_permute<float>(n_, nrhs, d_perm_, true, (const float *)dx, ldx, (float *)d_rhs_, n_, stream_);
@@ -547,13 +560,28 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
                                   d_ydst_, part, maxwidth_, 8 * n_cu_, stream_);
             continue;
         }
-        for (int lv = 0; lv < nl; lv++) {
-            const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
-            const int f0 = lv_fwd_ptr_[lv], nf = lv_fwd_ptr_[lv + 1] - f0;
-            launch_fwd_diag<T>(S_, d_diag_ + d0, nd, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, nr, W1,
-                               maxwidth_, stream_);
-            launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, d_ydst_, ybuf, ldy, maxwidth_, stream_);
+        // partial sweeps (pastix_subtask_trsm / _diag): 1 forward only, 2
+        // backward only, 3 backward after x *= D (LDL^t: L^{-t} = U^{-1} D),
+        // 4 x /= diag(factor)
+        if (part_ == 4) {
+            launch_diag_scale<T>(n_, nr, d_diagpos_, L, true, xx, ldx, stream_);
+            continue;
         }
+        if (part_ == 0 || part_ == 1) {
+            for (int lv = 0; lv < nl; lv++) {
+                const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
+                const int f0 = lv_fwd_ptr_[lv], nf = lv_fwd_ptr_[lv + 1] - f0;
+                launch_fwd_diag<T>(S_, d_diag_ + d0, nd, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, nr, W1,
+                                   maxwidth_, stream_);
+                launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, d_ydst_, ybuf, ldy, maxwidth_, stream_);
+            }
+        }
+        if (part_ == 1) {
+            launch_copy2d<T>(n_, nr, z, ldz, xx, ldx, stream_);
+            continue;
+        }
+        if (part_ == 3) launch_diag_scale<T>(n_, nr, d_diagpos_, L, false, xx, ldx, stream_);
+        if (part_ == 2 || part_ == 3) launch_copy2d<T>(n_, nr, xx, ldx, z, ldz, stream_);
         for (int lv = nl - 1; lv >= 0; lv--) {
             const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
             const int b0 = lv_bwd_ptr_[lv], nb = lv_bwd_ptr_[lv + 1] - b0;

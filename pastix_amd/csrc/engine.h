@@ -35,6 +35,10 @@ class Engine {
     int solve_device(void *dx, int64_t ldx, int nrhs, double *seconds);
     // Host right-hand sides in the user's ordering (FP64 or factor precision).
     int solve_host(const int64_t *perm, int flttype, void *b, int64_t ldb, int nrhs, double *seconds);
+    // One part of the solve on host right-hand sides already in the permuted
+    // ordering: 1 forward sweep, 2 backward sweep, 3 backward sweep of LDL^t's
+    // L^t (x *= D first), 4 division by the factor's diagonal.
+    int solve_host_part(int part, int flttype, void *b, int64_t ldb, int nrhs, double *seconds);
 
     // Permuted matrix in CSR form for the residual SpMV (always FP64).
     int upload_matrix(const HostBcsc &bc);
@@ -222,6 +226,7 @@ class Engine {
     // counters (4*cblknbr + 2: the last one is a sticky error flag)
     SolveTask *d_stasks_ = nullptr;
     int n_stasks_ = 0, n_cu_ = 256;
+    int part_ = 0;  // solve part of the next solve_t (0: whole solve)
     int32_t *d_schunk_ = nullptr;
     int n_schunk_ = 0;
     int32_t *d_itgt_ptr_ = nullptr, *d_itgt_ = nullptr, *d_sneed_ = nullptr, *d_scnt_ = nullptr;

@@ -102,6 +102,10 @@ void launch_solve_tasks(const DevSymbol &S, const SolveTask *tasks, const int32_
                         T *ybuf, int64_t ldy, T *x, int64_t ldx, T *z, int64_t ldz, int nrhs, const T *L, const T *U,
                         const T *w1, const T *wb, const int32_t *ydst, T *part, int maxwidth, int nwg,
                         hipStream_t st);
+template <typename T> void launch_copy2d(int64_t n, int nr, const T *x, int64_t ldx, T *y, int64_t ldy, hipStream_t st);
+template <typename T>
+void launch_diag_scale(int64_t n, int nr, const int64_t *pos, const T *coef, bool divide, T *x, int64_t ldx,
+                       hipStream_t st);
 void launch_zero_int(int n, int *p, hipStream_t st);
 void launch_d2s(int64_t n, const double *x, float *y, hipStream_t st);
 void launch_s2d(int64_t n, const float *x, double *y, hipStream_t st);

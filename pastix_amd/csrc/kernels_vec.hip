@@ -304,6 +304,44 @@ void launch_set_csr_diag(int64_t n, const int64_t *pos, const double *d0, double
     if (n > 0) hipLaunchKernelGGL(k_set_csr_diag, dim3(nblk(n)), dim3(256), 0, st, n, pos, d0, delta, vals);
 }
 
+// columns [0, nr) of an n-row block: y = x (different leading dimensions)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy2d(int64_t n, const T *__restrict__ x, int64_t ldx,
+                                                T *__restrict__ y, int64_t ldy) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t u = blockIdx.y;
+    if (i < n) y[i + u * ldy] = x[i + u * ldx];
+}
+template <typename T> void launch_copy2d(int64_t n, int nr, const T *x, int64_t ldx, T *y, int64_t ldy, hipStream_t st) {
+    if (n > 0 && nr > 0) hipLaunchKernelGGL(k_copy2d<T>, dim3(nblk(n), nr), dim3(256), 0, st, n, x, ldx, y, ldy);
+}
+template void launch_copy2d<double>(int64_t, int, const double *, int64_t, double *, int64_t, hipStream_t);
+template void launch_copy2d<float>(int64_t, int, const float *, int64_t, float *, int64_t, hipStream_t);
+
+// x_i *= d_i (divide = 0) or x_i /= d_i (divide = 1), d_i = coef[pos[i]]: the
+// factor's diagonal (reference solve_cblk_zdiag, kernels/solve_ztrsmsp.c:584-633)
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_scale(int64_t n, const int64_t *__restrict__ pos,
+                                                    const T *__restrict__ coef, int divide, T *__restrict__ x,
+                                                    int64_t ldx) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t u = blockIdx.y;
+    if (i < n) {
+        const T d = coef[pos[i]];
+        x[i + u * ldx] = divide ? x[i + u * ldx] / d : x[i + u * ldx] * d;
+    }
+}
+template <typename T>
+void launch_diag_scale(int64_t n, int nr, const int64_t *pos, const T *coef, bool divide, T *x, int64_t ldx,
+                       hipStream_t st) {
+    if (n > 0 && nr > 0)
+        hipLaunchKernelGGL(k_diag_scale<T>, dim3(nblk(n), nr), dim3(256), 0, st, n, pos, coef, divide ? 1 : 0, x, ldx);
+}
+template void launch_diag_scale<double>(int64_t, int, const int64_t *, const double *, bool, double *, int64_t,
+                                        hipStream_t);
+template void launch_diag_scale<float>(int64_t, int, const int64_t *, const float *, bool, float *, int64_t,
+                                       hipStream_t);
+
 void launch_zero_int(int n, int *p, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, n, p);
 }

@@ -73,6 +73,22 @@ class dir:
     Forward, Backward = 391, 392
 
 
+class trans:
+    NoTrans, Trans, ConjTrans = 111, 112, 113
+
+
+class uplo:
+    Upper, Lower = 121, 122
+
+
+class diag:
+    NonUnit, Unit = 131, 132
+
+
+class side:
+    Left, Right = 141, 142
+
+
 class error:
     SUCCESS, UNKNOWN, ALLOC, NOTIMPLEMENTED, OUTOFMEMORY, THREAD, INTERNAL, BADPARAMETER = range(8)
 
@@ -127,6 +143,10 @@ lib.pastix_subtask_refine.argtypes = [_P, _I, _I, ct.c_void_p, _I, ct.c_void_p, 
 lib.pastix_subtask_refine.restype = ct.c_int
 lib.pastix_subtask_applyorder.argtypes = [_P, ct.c_int, ct.c_int, _I, _I, ct.c_void_p, _I]
 lib.pastix_subtask_applyorder.restype = ct.c_int
+lib.pastix_subtask_trsm.argtypes = [_P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _I, ct.c_void_p, _I]
+lib.pastix_subtask_trsm.restype = ct.c_int
+lib.pastix_subtask_diag.argtypes = [_P, ct.c_int, _I, ct.c_void_p, _I]
+lib.pastix_subtask_diag.restype = ct.c_int
 lib.pastixOrderGrid.argtypes = [ct.POINTER(ct.POINTER(pastix_order_t)), _I, _I, _I]
 lib.pastixOrderGrid.restype = ct.c_int
 lib.pastixOrderExit.argtypes = [ct.POINTER(pastix_order_t)]
@@ -247,6 +267,21 @@ def subtask_applyorder(pastix_data, direction, b):
     n = 1 if b.ndim == 1 else b.shape[1]
     _check("pastix_subtask_applyorder",
            lib.pastix_subtask_applyorder(pastix_data.ptr, flt, direction, m, n, b.ctypes.data, m))
+
+
+def subtask_trsm(pastix_data, side_, uplo_, trans_, diag_, b, nrhs=-1):
+    """One sweep of the solve on b in the permuted ordering, in place."""
+    nrhs = _getnrhs(nrhs, b)
+    flt = _spm.SpmDouble if b.dtype == np.float64 else _spm.SpmFloat
+    _check("pastix_subtask_trsm", lib.pastix_subtask_trsm(pastix_data.ptr, flt, side_, uplo_, trans_, diag_, nrhs,
+                                                          b.ctypes.data, b.shape[0]))
+
+
+def subtask_diag(pastix_data, b, nrhs=-1):
+    """b /= the factor's diagonal (D of LDL^t), b in the permuted ordering."""
+    nrhs = _getnrhs(nrhs, b)
+    flt = _spm.SpmDouble if b.dtype == np.float64 else _spm.SpmFloat
+    _check("pastix_subtask_diag", lib.pastix_subtask_diag(pastix_data.ptr, flt, nrhs, b.ctypes.data, b.shape[0]))
 
 
 def subtask_solve(pastix_data, b, nrhs=-1):
