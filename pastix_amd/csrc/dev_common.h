@@ -99,6 +99,11 @@ __device__ __forceinline__ void quad_gemm_nt(Quad<T> &q, View<T> A, int ma, View
 // >= my of Y read as zero; X(r, k) = X[r + k*ldx].  Wave w owns the 32x32
 // quadrant (w>>1, w&1).  Must be called by all 256 threads (contains
 // barriers).
+// two floats: F2u with the 4-byte alignment of a panel element (panels
+// start and lead at any element), F2a 8-byte aligned (LDS staging rows)
+typedef float F2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef float F2a __attribute__((ext_vector_type(2)));
+
 template <typename T, int CH = 16> struct StageBuf {
     T xs[CH][68];
     T ys[CH][68];
@@ -114,19 +119,60 @@ __device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int6
     const int row = tid & 63, kk = tid >> 6;
     const bool rx = row < mx, ry = row < my;
     T gx[NU], gy[NU];
+    // FP32: two rows per lane and two columns per wave instruction (8-byte
+    // loads and LDS stores), half the staging instructions
+    const int row2 = 2 * (tid & 31), kk2 = tid >> 5;
     auto gload = [&](int k0) {
+        if constexpr (sizeof(T) == 4) {
 #pragma unroll
-        for (int u = 0; u < NU; u++) {
-            const int k = k0 + kk + 4 * u;
-            gx[u] = (rx && k < K) ? X[row + (int64_t)k * ldx] : (T)0;
-            gy[u] = (ry && k < K) ? Y[row + (int64_t)k * ldy] : (T)0;
+            for (int u = 0; u < NU / 2; u++) {
+                const int k = k0 + kk2 + 8 * u;
+                gx[2 * u] = gx[2 * u + 1] = gy[2 * u] = gy[2 * u + 1] = (T)0;
+                if (k < K) {
+                    const T *px = X + row2 + (int64_t)k * ldx, *py = Y + row2 + (int64_t)k * ldy;
+                    if (row2 + 1 < mx) {
+                        const F2u v = *reinterpret_cast<const F2u *>(px);
+                        gx[2 * u] = v[0];
+                        gx[2 * u + 1] = v[1];
+                    } else if (row2 < mx) {
+                        gx[2 * u] = px[0];
+                    }
+                    if (row2 + 1 < my) {
+                        const F2u v = *reinterpret_cast<const F2u *>(py);
+                        gy[2 * u] = v[0];
+                        gy[2 * u + 1] = v[1];
+                    } else if (row2 < my) {
+                        gy[2 * u] = py[0];
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                const int k = k0 + kk + 4 * u;
+                gx[u] = (rx && k < K) ? X[row + (int64_t)k * ldx] : (T)0;
+                gy[u] = (ry && k < K) ? Y[row + (int64_t)k * ldy] : (T)0;
+            }
         }
     };
     auto sstore = [&]() {
+        if constexpr (sizeof(T) == 4) {
 #pragma unroll
-        for (int u = 0; u < NU; u++) {
-            sb.xs[kk + 4 * u][row] = gx[u];
-            sb.ys[kk + 4 * u][row] = gy[u];
+            for (int u = 0; u < NU / 2; u++) {
+                F2a vx, vy;
+                vx[0] = gx[2 * u];
+                vx[1] = gx[2 * u + 1];
+                vy[0] = gy[2 * u];
+                vy[1] = gy[2 * u + 1];
+                *reinterpret_cast<F2a *>(&sb.xs[kk2 + 8 * u][row2]) = vx;
+                *reinterpret_cast<F2a *>(&sb.ys[kk2 + 8 * u][row2]) = vy;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                sb.xs[kk + 4 * u][row] = gx[u];
+                sb.ys[kk + 4 * u][row] = gy[u];
+            }
         }
     };
     const bool active = (wr < mx) && (wc < my);
@@ -177,23 +223,70 @@ __device__ __forceinline__ void wg_gemm_lds_64x128(Quad<T> &q0, Quad<T> &q1, con
     const int rowy = tid & 127, kky = tid >> 7;  // Y: 128 rows, 2 columns per pass
     const bool rx = rowx < mx, ry = rowy < my;
     T gx[NUX], gy[NUY];
-    auto gload = [&](int k0) {
-#pragma unroll
-        for (int u = 0; u < NUX; u++) {
-            const int k = k0 + kkx + 4 * u;
-            gx[u] = (rx && k < K) ? X[rowx + (int64_t)k * ldx] : (T)0;
+    // FP32: two rows per lane (8-byte loads and LDS stores), as in wg_gemm_lds
+    const int r2x = 2 * (tid & 31), k2x = tid >> 5;  // X: 8 columns per pass
+    const int r2y = 2 * (tid & 63), k2y = tid >> 6;  // Y: 4 columns per pass
+    auto ld2 = [&](const T *p, int r, int m, T &a, T &b) {
+        if constexpr (sizeof(T) == 4) {
+            a = b = (T)0;
+            if (r + 1 < m) {
+                const F2u v = *reinterpret_cast<const F2u *>(p);
+                a = v[0];
+                b = v[1];
+            } else if (r < m) {
+                a = p[0];
+            }
         }
+    };
+    auto gload = [&](int k0) {
+        if constexpr (sizeof(T) == 4) {
 #pragma unroll
-        for (int u = 0; u < NUY; u++) {
-            const int k = k0 + kky + 2 * u;
-            gy[u] = (ry && k < K) ? Y[rowy + (int64_t)k * ldy] : (T)0;
+            for (int u = 0; u < NUX / 2; u++) {
+                const int k = k0 + k2x + 8 * u;
+                gx[2 * u] = gx[2 * u + 1] = (T)0;
+                if (k < K) ld2(X + r2x + (int64_t)k * ldx, r2x, mx, gx[2 * u], gx[2 * u + 1]);
+            }
+#pragma unroll
+            for (int u = 0; u < NUY / 2; u++) {
+                const int k = k0 + k2y + 4 * u;
+                gy[2 * u] = gy[2 * u + 1] = (T)0;
+                if (k < K) ld2(Y + r2y + (int64_t)k * ldy, r2y, my, gy[2 * u], gy[2 * u + 1]);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < NUX; u++) {
+                const int k = k0 + kkx + 4 * u;
+                gx[u] = (rx && k < K) ? X[rowx + (int64_t)k * ldx] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < NUY; u++) {
+                const int k = k0 + kky + 2 * u;
+                gy[u] = (ry && k < K) ? Y[rowy + (int64_t)k * ldy] : (T)0;
+            }
         }
     };
     auto sstore = [&]() {
+        if constexpr (sizeof(T) == 4) {
 #pragma unroll
-        for (int u = 0; u < NUX; u++) sb.xs[kkx + 4 * u][rowx] = gx[u];
+            for (int u = 0; u < NUX / 2; u++) {
+                F2a v;
+                v[0] = gx[2 * u];
+                v[1] = gx[2 * u + 1];
+                *reinterpret_cast<F2a *>(&sb.xs[k2x + 8 * u][r2x]) = v;
+            }
 #pragma unroll
-        for (int u = 0; u < NUY; u++) sb.ys[kky + 2 * u][rowy] = gy[u];
+            for (int u = 0; u < NUY / 2; u++) {
+                F2a v;
+                v[0] = gy[2 * u];
+                v[1] = gy[2 * u + 1];
+                *reinterpret_cast<F2a *>(&sb.ys[k2y + 4 * u][r2y]) = v;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < NUX; u++) sb.xs[kkx + 4 * u][rowx] = gx[u];
+#pragma unroll
+            for (int u = 0; u < NUY; u++) sb.ys[kky + 2 * u][rowy] = gy[u];
+        }
     };
     const bool act0 = (wr < mx) && (wc < my), act1 = act0 && (wc + 32 < my);
     gload(0);
