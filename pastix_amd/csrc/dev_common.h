@@ -99,10 +99,18 @@ __device__ __forceinline__ void quad_gemm_nt(Quad<T> &q, View<T> A, int ma, View
 // >= my of Y read as zero; X(r, k) = X[r + k*ldx].  Wave w owns the 32x32
 // quadrant (w>>1, w&1).  Must be called by all 256 threads (contains
 // barriers).
-// two floats: F2u with the 4-byte alignment of a panel element (panels
-// start and lead at any element), F2a 8-byte aligned (LDS staging rows)
-typedef float F2u __attribute__((ext_vector_type(2), aligned(4)));
-typedef float F2a __attribute__((ext_vector_type(2)));
+// two elements: V2<T>::u with the alignment of one panel element (panels
+// start and lead at any element), V2<T>::a fully aligned (LDS staging rows)
+constexpr bool PAIR_STAGING = true;
+template <typename T> struct V2;
+template <> struct V2<float> {
+    typedef float u __attribute__((ext_vector_type(2), aligned(4)));
+    typedef float a __attribute__((ext_vector_type(2)));
+};
+template <> struct V2<double> {
+    typedef double u __attribute__((ext_vector_type(2), aligned(8)));
+    typedef double a __attribute__((ext_vector_type(2)));
+};
 
 template <typename T, int CH = 16> struct StageBuf {
     T xs[CH][68];
@@ -119,11 +127,11 @@ __device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int6
     const int row = tid & 63, kk = tid >> 6;
     const bool rx = row < mx, ry = row < my;
     T gx[NU], gy[NU];
-    // FP32: two rows per lane and two columns per wave instruction (8-byte
-    // loads and LDS stores), half the staging instructions
+    // two rows per lane and two columns per wave instruction (8- or 16-byte
+    // loads and LDS stores): half the staging instructions
     const int row2 = 2 * (tid & 31), kk2 = tid >> 5;
     auto gload = [&](int k0) {
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (PAIR_STAGING) {
 #pragma unroll
             for (int u = 0; u < NU / 2; u++) {
                 const int k = k0 + kk2 + 8 * u;
@@ -131,14 +139,14 @@ __device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int6
                 if (k < K) {
                     const T *px = X + row2 + (int64_t)k * ldx, *py = Y + row2 + (int64_t)k * ldy;
                     if (row2 + 1 < mx) {
-                        const F2u v = *reinterpret_cast<const F2u *>(px);
+                        const typename V2<T>::u v = *reinterpret_cast<const typename V2<T>::u *>(px);
                         gx[2 * u] = v[0];
                         gx[2 * u + 1] = v[1];
                     } else if (row2 < mx) {
                         gx[2 * u] = px[0];
                     }
                     if (row2 + 1 < my) {
-                        const F2u v = *reinterpret_cast<const F2u *>(py);
+                        const typename V2<T>::u v = *reinterpret_cast<const typename V2<T>::u *>(py);
                         gy[2 * u] = v[0];
                         gy[2 * u + 1] = v[1];
                     } else if (row2 < my) {
@@ -156,16 +164,16 @@ __device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int6
         }
     };
     auto sstore = [&]() {
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (PAIR_STAGING) {
 #pragma unroll
             for (int u = 0; u < NU / 2; u++) {
-                F2a vx, vy;
+                typename V2<T>::a vx, vy;
                 vx[0] = gx[2 * u];
                 vx[1] = gx[2 * u + 1];
                 vy[0] = gy[2 * u];
                 vy[1] = gy[2 * u + 1];
-                *reinterpret_cast<F2a *>(&sb.xs[kk2 + 8 * u][row2]) = vx;
-                *reinterpret_cast<F2a *>(&sb.ys[kk2 + 8 * u][row2]) = vy;
+                *reinterpret_cast<typename V2<T>::a *>(&sb.xs[kk2 + 8 * u][row2]) = vx;
+                *reinterpret_cast<typename V2<T>::a *>(&sb.ys[kk2 + 8 * u][row2]) = vy;
             }
         } else {
 #pragma unroll
@@ -223,14 +231,14 @@ __device__ __forceinline__ void wg_gemm_lds_64x128(Quad<T> &q0, Quad<T> &q1, con
     const int rowy = tid & 127, kky = tid >> 7;  // Y: 128 rows, 2 columns per pass
     const bool rx = rowx < mx, ry = rowy < my;
     T gx[NUX], gy[NUY];
-    // FP32: two rows per lane (8-byte loads and LDS stores), as in wg_gemm_lds
+    // two rows per lane (8- or 16-byte loads and LDS stores), as in wg_gemm_lds
     const int r2x = 2 * (tid & 31), k2x = tid >> 5;  // X: 8 columns per pass
     const int r2y = 2 * (tid & 63), k2y = tid >> 6;  // Y: 4 columns per pass
     auto ld2 = [&](const T *p, int r, int m, T &a, T &b) {
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (PAIR_STAGING) {
             a = b = (T)0;
             if (r + 1 < m) {
-                const F2u v = *reinterpret_cast<const F2u *>(p);
+                const typename V2<T>::u v = *reinterpret_cast<const typename V2<T>::u *>(p);
                 a = v[0];
                 b = v[1];
             } else if (r < m) {
@@ -239,7 +247,7 @@ __device__ __forceinline__ void wg_gemm_lds_64x128(Quad<T> &q0, Quad<T> &q1, con
         }
     };
     auto gload = [&](int k0) {
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (PAIR_STAGING) {
 #pragma unroll
             for (int u = 0; u < NUX / 2; u++) {
                 const int k = k0 + k2x + 8 * u;
@@ -266,20 +274,20 @@ __device__ __forceinline__ void wg_gemm_lds_64x128(Quad<T> &q0, Quad<T> &q1, con
         }
     };
     auto sstore = [&]() {
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (PAIR_STAGING) {
 #pragma unroll
             for (int u = 0; u < NUX / 2; u++) {
-                F2a v;
+                typename V2<T>::a v;
                 v[0] = gx[2 * u];
                 v[1] = gx[2 * u + 1];
-                *reinterpret_cast<F2a *>(&sb.xs[k2x + 8 * u][r2x]) = v;
+                *reinterpret_cast<typename V2<T>::a *>(&sb.xs[k2x + 8 * u][r2x]) = v;
             }
 #pragma unroll
             for (int u = 0; u < NUY / 2; u++) {
-                F2a v;
+                typename V2<T>::a v;
                 v[0] = gy[2 * u];
                 v[1] = gy[2 * u + 1];
-                *reinterpret_cast<F2a *>(&sb.ys[k2y + 4 * u][r2y]) = v;
+                *reinterpret_cast<typename V2<T>::a *>(&sb.ys[k2y + 4 * u][r2y]) = v;
             }
         } else {
 #pragma unroll
