@@ -101,16 +101,39 @@ __device__ __forceinline__ void quad_gemm_nt(Quad<T> &q, View<T> A, int ma, View
 // barriers).
 // two elements: V2<T>::u with the alignment of one panel element (panels
 // start and lead at any element), V2<T>::a fully aligned (LDS staging rows)
-constexpr bool PAIR_STAGING = true;
-template <typename T> struct V2;
-template <> struct V2<float> {
-    typedef float u __attribute__((ext_vector_type(2), aligned(4)));
-    typedef float a __attribute__((ext_vector_type(2)));
+// Operand staging moves 16 bytes per lane: E = 16 / sizeof(T) consecutive
+// rows of one column (4 floats or 2 doubles).  Vec<T>::u has the alignment
+// of one panel element (panels start and lead at any element), Vec<T>::a is
+// fully aligned (LDS staging rows, whose pitch is a multiple of 16 bytes).
+template <typename T> struct Vec;
+template <> struct Vec<float> {
+    static constexpr int E = 4;
+    typedef float u __attribute__((ext_vector_type(4), aligned(4)));
+    typedef float a __attribute__((ext_vector_type(4)));
 };
-template <> struct V2<double> {
+template <> struct Vec<double> {
+    static constexpr int E = 2;
     typedef double u __attribute__((ext_vector_type(2), aligned(8)));
     typedef double a __attribute__((ext_vector_type(2)));
 };
+// E rows [r, r+E) of column p (rows >= m read as zero) into o
+template <typename T> __device__ __forceinline__ void ld_rows(const T *p, int r, int m, T *o) {
+    constexpr int E = Vec<T>::E;
+    if (r + E <= m) {
+        const typename Vec<T>::u v = *reinterpret_cast<const typename Vec<T>::u *>(p);
+#pragma unroll
+        for (int e = 0; e < E; e++) o[e] = v[e];
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) o[e] = (r + e < m) ? p[e] : (T)0;
+    }
+}
+template <typename T> __device__ __forceinline__ void st_rows(T *p, const T *o) {
+    typename Vec<T>::a v;
+#pragma unroll
+    for (int e = 0; e < Vec<T>::E; e++) v[e] = o[e];
+    *reinterpret_cast<typename Vec<T>::a *>(p) = v;
+}
 
 template <typename T, int CH = 16> struct StageBuf {
     T xs[CH][68];
@@ -120,67 +143,28 @@ template <typename T, int CH = 16> struct StageBuf {
 template <typename T, int CH>
 __device__ __forceinline__ void wg_gemm_lds(Quad<T> &q, const T *X, int mx, int64_t ldx, const T *Y, int my,
                                             int64_t ldy, int K, StageBuf<T, CH> &sb, int tid) {
-    constexpr int NU = CH / 4;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
     const int i = lane & 15, kq = lane >> 4;
-    const int row = tid & 63, kk = tid >> 6;
-    const bool rx = row < mx, ry = row < my;
-    T gx[NU], gy[NU];
-    // two rows per lane and two columns per wave instruction (8- or 16-byte
-    // loads and LDS stores): half the staging instructions
-    const int row2 = 2 * (tid & 31), kk2 = tid >> 5;
+    // 16 bytes per lane: E rows of one column, 4E columns per pass of the
+    // workgroup, CH / 4E passes per chunk
+    constexpr int E = Vec<T>::E, NP = CH / (4 * E);
+    static_assert(CH % (4 * E) == 0, "chunk of whole passes");
+    const int re = E * (tid % (64 / E)), ke = tid / (64 / E);
+    T gx[NP][E], gy[NP][E];
     auto gload = [&](int k0) {
-        if constexpr (PAIR_STAGING) {
 #pragma unroll
-            for (int u = 0; u < NU / 2; u++) {
-                const int k = k0 + kk2 + 8 * u;
-                gx[2 * u] = gx[2 * u + 1] = gy[2 * u] = gy[2 * u + 1] = (T)0;
-                if (k < K) {
-                    const T *px = X + row2 + (int64_t)k * ldx, *py = Y + row2 + (int64_t)k * ldy;
-                    if (row2 + 1 < mx) {
-                        const typename V2<T>::u v = *reinterpret_cast<const typename V2<T>::u *>(px);
-                        gx[2 * u] = v[0];
-                        gx[2 * u + 1] = v[1];
-                    } else if (row2 < mx) {
-                        gx[2 * u] = px[0];
-                    }
-                    if (row2 + 1 < my) {
-                        const typename V2<T>::u v = *reinterpret_cast<const typename V2<T>::u *>(py);
-                        gy[2 * u] = v[0];
-                        gy[2 * u + 1] = v[1];
-                    } else if (row2 < my) {
-                        gy[2 * u] = py[0];
-                    }
-                }
-            }
-        } else {
-#pragma unroll
-            for (int u = 0; u < NU; u++) {
-                const int k = k0 + kk + 4 * u;
-                gx[u] = (rx && k < K) ? X[row + (int64_t)k * ldx] : (T)0;
-                gy[u] = (ry && k < K) ? Y[row + (int64_t)k * ldy] : (T)0;
-            }
+        for (int u = 0; u < NP; u++) {
+            const int k = k0 + ke + 4 * E * u;
+            ld_rows(X + re + (int64_t)k * ldx, re, k < K ? mx : 0, gx[u]);
+            ld_rows(Y + re + (int64_t)k * ldy, re, k < K ? my : 0, gy[u]);
         }
     };
     auto sstore = [&]() {
-        if constexpr (PAIR_STAGING) {
 #pragma unroll
-            for (int u = 0; u < NU / 2; u++) {
-                typename V2<T>::a vx, vy;
-                vx[0] = gx[2 * u];
-                vx[1] = gx[2 * u + 1];
-                vy[0] = gy[2 * u];
-                vy[1] = gy[2 * u + 1];
-                *reinterpret_cast<typename V2<T>::a *>(&sb.xs[kk2 + 8 * u][row2]) = vx;
-                *reinterpret_cast<typename V2<T>::a *>(&sb.ys[kk2 + 8 * u][row2]) = vy;
-            }
-        } else {
-#pragma unroll
-            for (int u = 0; u < NU; u++) {
-                sb.xs[kk + 4 * u][row] = gx[u];
-                sb.ys[kk + 4 * u][row] = gy[u];
-            }
+        for (int u = 0; u < NP; u++) {
+            st_rows(&sb.xs[ke + 4 * E * u][re], gx[u]);
+            st_rows(&sb.ys[ke + 4 * E * u][re], gy[u]);
         }
     };
     const bool active = (wr < mx) && (wc < my);
@@ -223,78 +207,32 @@ template <typename T, int CH>
 __device__ __forceinline__ void wg_gemm_lds_64x128(Quad<T> &q0, Quad<T> &q1, const T *X, int mx, int64_t ldx,
                                                    const T *Y, int my, int64_t ldy, int K, StageBuf2<T, CH> &sb,
                                                    int tid) {
-    constexpr int NUX = CH / 4, NUY = CH / 2;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
     const int i = lane & 15, kq = lane >> 4;
-    const int rowx = tid & 63, kkx = tid >> 6;   // X: 64 rows, 4 columns per pass
-    const int rowy = tid & 127, kky = tid >> 7;  // Y: 128 rows, 2 columns per pass
-    const bool rx = rowx < mx, ry = rowy < my;
-    T gx[NUX], gy[NUY];
-    // two rows per lane (8- or 16-byte loads and LDS stores), as in wg_gemm_lds
-    const int r2x = 2 * (tid & 31), k2x = tid >> 5;  // X: 8 columns per pass
-    const int r2y = 2 * (tid & 63), k2y = tid >> 6;  // Y: 4 columns per pass
-    auto ld2 = [&](const T *p, int r, int m, T &a, T &b) {
-        if constexpr (PAIR_STAGING) {
-            a = b = (T)0;
-            if (r + 1 < m) {
-                const typename V2<T>::u v = *reinterpret_cast<const typename V2<T>::u *>(p);
-                a = v[0];
-                b = v[1];
-            } else if (r < m) {
-                a = p[0];
-            }
-        }
-    };
+    // 16 bytes per lane: X (64 rows) 4E columns per pass, Y (128 rows) 2E
+    constexpr int E = Vec<T>::E, NPX = CH / (4 * E), NPY = CH / (2 * E);
+    static_assert(CH % (4 * E) == 0, "chunk of whole passes");
+    const int rex = E * (tid % (64 / E)), kex = tid / (64 / E);
+    const int rey = E * (tid % (128 / E)), key = tid / (128 / E);
+    T gx[NPX][E], gy[NPY][E];
     auto gload = [&](int k0) {
-        if constexpr (PAIR_STAGING) {
 #pragma unroll
-            for (int u = 0; u < NUX / 2; u++) {
-                const int k = k0 + k2x + 8 * u;
-                gx[2 * u] = gx[2 * u + 1] = (T)0;
-                if (k < K) ld2(X + r2x + (int64_t)k * ldx, r2x, mx, gx[2 * u], gx[2 * u + 1]);
-            }
+        for (int u = 0; u < NPX; u++) {
+            const int k = k0 + kex + 4 * E * u;
+            ld_rows(X + rex + (int64_t)k * ldx, rex, k < K ? mx : 0, gx[u]);
+        }
 #pragma unroll
-            for (int u = 0; u < NUY / 2; u++) {
-                const int k = k0 + k2y + 4 * u;
-                gy[2 * u] = gy[2 * u + 1] = (T)0;
-                if (k < K) ld2(Y + r2y + (int64_t)k * ldy, r2y, my, gy[2 * u], gy[2 * u + 1]);
-            }
-        } else {
-#pragma unroll
-            for (int u = 0; u < NUX; u++) {
-                const int k = k0 + kkx + 4 * u;
-                gx[u] = (rx && k < K) ? X[rowx + (int64_t)k * ldx] : (T)0;
-            }
-#pragma unroll
-            for (int u = 0; u < NUY; u++) {
-                const int k = k0 + kky + 2 * u;
-                gy[u] = (ry && k < K) ? Y[rowy + (int64_t)k * ldy] : (T)0;
-            }
+        for (int u = 0; u < NPY; u++) {
+            const int k = k0 + key + 2 * E * u;
+            ld_rows(Y + rey + (int64_t)k * ldy, rey, k < K ? my : 0, gy[u]);
         }
     };
     auto sstore = [&]() {
-        if constexpr (PAIR_STAGING) {
 #pragma unroll
-            for (int u = 0; u < NUX / 2; u++) {
-                typename V2<T>::a v;
-                v[0] = gx[2 * u];
-                v[1] = gx[2 * u + 1];
-                *reinterpret_cast<typename V2<T>::a *>(&sb.xs[k2x + 8 * u][r2x]) = v;
-            }
+        for (int u = 0; u < NPX; u++) st_rows(&sb.xs[kex + 4 * E * u][rex], gx[u]);
 #pragma unroll
-            for (int u = 0; u < NUY / 2; u++) {
-                typename V2<T>::a v;
-                v[0] = gy[2 * u];
-                v[1] = gy[2 * u + 1];
-                *reinterpret_cast<typename V2<T>::a *>(&sb.ys[k2y + 4 * u][r2y]) = v;
-            }
-        } else {
-#pragma unroll
-            for (int u = 0; u < NUX; u++) sb.xs[kkx + 4 * u][rowx] = gx[u];
-#pragma unroll
-            for (int u = 0; u < NUY; u++) sb.ys[kky + 2 * u][rowy] = gy[u];
-        }
+        for (int u = 0; u < NPY; u++) st_rows(&sb.ys[key + 2 * E * u][rey], gy[u]);
     };
     const bool act0 = (wr < mx) && (wc < my), act1 = act0 && (wc + 32 < my);
     gload(0);

@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_solve_tasks(DevSymbol S, const SolveTas
     T(*tile)[65] = reinterpret_cast<T(*)[65]>(smem_raw);
     __shared__ T red[4 * 64 * NR];
     __shared__ T xr[64 * NR];
-    __shared__ int sh_t, sh_ok, sh_c;
+    __shared__ int sh_ok, sh_c;
     int *fwd_in = cnt, *fdiag = cnt + cb, *bprod = cnt + 2 * cb, *bdiag = cnt + 3 * cb;
     int *head = cnt + 4 * cb, *err = head + 1;
     const int tid = threadIdx.x;
