@@ -211,9 +211,16 @@ __global__ __launch_bounds__(256) void k_update_gather(DevSymbol S, const UpdIte
         (&rs[0][0])[e] = -1;
         (&cs[0][0])[e] = -1;
     }
-    T acc[16];
+    // products: thread (g, cph) owns the E consecutive target rows E*g.. of
+    // columns cph + CP*u; when the rows are one run of source rows of the
+    // entry, a column's contribution is one 16-byte load
+    constexpr int E = Vec<T>::E, RG = 64 / E, CP = 256 / RG, NC = 64 / CP;
+    const int rr = E * (tid % RG), cph = tid / RG;
+    T acc[NC][E];
 #pragma unroll
-    for (int u = 0; u < 16; u++) acc[u] = (T)0;
+    for (int u = 0; u < NC; u++)
+#pragma unroll
+        for (int i = 0; i < E; i++) acc[u][i] = (T)0;
     for (int eb = it.e0; eb < it.e1; eb += NB) {
         const int ne = min(NB, it.e1 - eb);
         __syncthreads();  // previous batch consumed (and the tables initialised)
@@ -250,17 +257,39 @@ __global__ __launch_bounds__(256) void k_update_gather(DevSymbol S, const UpdIte
         __syncthreads();
         for (int slot = 0; slot < ne; slot++) {
             const int e = eb + slot;
-            if (rs[slot][r] == e) {
-                const int bp = rbk[slot][r];
-                const int64_t ldw = wl[slot];
-                const T *Wr = W + wb[slot] + rp[slot][r];
+            bool cov[E], any = false, run = true;
+            int pr[E], bkr[E];
+            const int p0 = rp[slot][rr];
 #pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const int c = ph + 4 * u;
-                    if (cs[slot][c] == e) {
-                        const bool keep = (MODE == 0) ? (bp >= cbk[slot][c]) : (bp > cbk[slot][c]);
-                        if (keep) acc[u] += Wr[(int64_t)cq[slot][c] * ldw];
-                    }
+            for (int i = 0; i < E; i++) {
+                cov[i] = rs[slot][rr + i] == e;
+                pr[i] = rp[slot][rr + i];
+                bkr[i] = rbk[slot][rr + i];
+                any |= cov[i];
+                run &= cov[i] && pr[i] == p0 + i;
+            }
+            if (!any) continue;
+            const int64_t ldw = wl[slot];
+            const T *Wb = W + wb[slot];
+#pragma unroll
+            for (int u = 0; u < NC; u++) {
+                const int c = cph + CP * u;
+                if (cs[slot][c] != e) continue;
+                const T *Wc = Wb + (int64_t)cq[slot][c] * ldw;
+                const int cb = cbk[slot][c];
+                T v[E];
+                if (run) {
+                    const typename Vec<T>::u w = *reinterpret_cast<const typename Vec<T>::u *>(Wc + p0);
+#pragma unroll
+                    for (int i = 0; i < E; i++) v[i] = w[i];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < E; i++) v[i] = cov[i] ? Wc[pr[i]] : (T)0;
+                }
+#pragma unroll
+                for (int i = 0; i < E; i++) {
+                    const bool keep = cov[i] && ((MODE == 0) ? (bkr[i] >= cb) : (bkr[i] > cb));
+                    if (keep) acc[u][i] += v[i];
                 }
             }
         }
@@ -268,11 +297,24 @@ __global__ __launch_bounds__(256) void k_update_gather(DevSymbol S, const UpdIte
     const int Kf = S.cblk_fcol[f + 1] - fcol;
     const int64_t stf = S.cblk_stride[f];
     T *C = Ccoef + S.cblk_coef[f];
-    if (r0 + r < stf) {
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            const int cc = c0 + ph + 4 * u;
-            if (cc < Kf && acc[u] != (T)0) C[r0 + r + (int64_t)cc * stf] -= acc[u];
+    for (int u = 0; u < NC; u++) {
+        const int cc = c0 + cph + CP * u;
+        if (cc >= Kf) continue;
+        bool nz = false;
+#pragma unroll
+        for (int i = 0; i < E; i++) nz |= acc[u][i] != (T)0;
+        if (!nz) continue;
+        T *Cp = C + r0 + rr + (int64_t)cc * stf;
+        if (r0 + rr + E <= stf) {
+            typename Vec<T>::u w = *reinterpret_cast<const typename Vec<T>::u *>(Cp);
+#pragma unroll
+            for (int i = 0; i < E; i++) w[i] -= acc[u][i];
+            *reinterpret_cast<typename Vec<T>::u *>(Cp) = w;
+        } else {
+#pragma unroll
+            for (int i = 0; i < E; i++)
+                if (r0 + rr + i < stf && acc[u][i] != (T)0) Cp[i] -= acc[u][i];
         }
     }
 }
