@@ -65,6 +65,40 @@ __device__ __forceinline__ void wave_sum_store(T (&acc)[NR], T *red, int r, int 
     }
 }
 
+// The same for partial sums held as E consecutive rows per lane: lane l of
+// wave q holds rows E*(l % (64/E)) + i of columns l / (64/E); the E lanes of a
+// row group combine with lane shuffles, then the waves as above (row r =
+// the calling lane).
+template <bool CO, typename T, int NR, int NW, int E>
+__device__ __forceinline__ void wave_sum_store_rows(T (&acc)[E][NR], T *red, int r, int q, bool ok, int nrhs,
+                                                    T *out, int64_t ldo) {
+    constexpr int G = 64 / E;
+#pragma unroll
+    for (int off = G; off < 64; off *= 2)
+#pragma unroll
+        for (int i = 0; i < E; i++)
+#pragma unroll
+            for (int u = 0; u < NR; u++) acc[i][u] += __shfl_xor(acc[i][u], off);
+    if (r < G) {
+#pragma unroll
+        for (int i = 0; i < E; i++)
+#pragma unroll
+            for (int u = 0; u < NR; u++) red[(q * 64 + E * r + i) * NR + u] = acc[i][u];
+    }
+    __syncthreads();
+    if (q == 0 && ok) {
+#pragma unroll
+        for (int u = 0; u < NR; u++) {
+            if (u < nrhs) {
+                T v = red[r * NR + u];
+#pragma unroll
+                for (int w = 1; w < NW; w++) v += red[(w * 64 + r) * NR + u];
+                st_co<CO>(out + u * ldo, v);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Task bodies, each run by one whole workgroup of NW waves (the level
 // kernels below and the dependency-driven solve share them).
@@ -123,21 +157,33 @@ __device__ __forceinline__ void fwd_prod_body(const DevSymbol &S, const FwdItem 
     for (int t = tid; t < Kc * nrhs; t += NW * 64) xs[t] = ld_co<CO>(z + fc + (t % Kc) + (int64_t)(t / Kc) * ldz);
     __syncthreads();
     const int nr = it.r1 - it.r0;
-    T acc[NR];
+    // E consecutive rows per lane (one 16-byte load), E columns per wave
+    // instruction
+    constexpr int E = Vec<T>::E, G = 64 / E;
+    const int rr = E * (r % G), kc = r / G;
+    T acc[E][NR];
 #pragma unroll
-    for (int u = 0; u < NR; u++) acc[u] = 0;
-    if (r < nr) {
-        const T *row = coef + S.cblk_coef[c] + it.r0 + r;
-#pragma unroll 4
-        for (int k = q; k < Kc; k += NW) {
-            const T a = row[(int64_t)k * ld];
+    for (int i = 0; i < E; i++)
 #pragma unroll
-            for (int u = 0; u < NR; u++)
-                if (u < nrhs) acc[u] += a * xs[k + u * Kc];
+        for (int u = 0; u < NR; u++) acc[i][u] = 0;
+    if (rr < nr) {
+        const T *rows = coef + S.cblk_coef[c] + it.r0 + rr;
+#pragma unroll 2
+        for (int k = q * E + kc; k < Kc; k += NW * E) {
+            T a[E];
+            ld_rows(rows + (int64_t)k * ld, rr, nr, a);
+#pragma unroll
+            for (int u = 0; u < NR; u++) {
+                if (u < nrhs) {
+                    const T xv = xs[k + u * Kc];
+#pragma unroll
+                    for (int i = 0; i < E; i++) acc[i][u] += a[i] * xv;
+                }
+            }
         }
     }
     const int dst = (r < nr) ? ydst[it.slot + r] : 0;
-    wave_sum_store<CO, T, NR, NW>(acc, red, r, q, r < nr, nrhs, ybuf + dst, ldy, (const T *)nullptr, 0);
+    wave_sum_store_rows<CO, T, NR, NW, E>(acc, red, r, q, r < nr, nrhs, ybuf + dst, ldy);
 }
 
 // one 64x64 tile (column block k0/64) of the backward products, 4 waves
@@ -159,8 +205,18 @@ __device__ __forceinline__ void bwd_prod_body(const DevSymbol &S, const BwdItem 
         xr[t] = (row < nr && u < nrhs) ? ld_co<CO>(x + S.panel_grow[rb + it.r0 + row] + u * ldx) : (T)0;
     }
     const T *P = coef + S.cblk_coef[c] + it.r0;
-    for (int cc = q; cc < 64; cc += 4)
-        tile[lane][cc] = (lane < nr && k0 + cc < Kc) ? P[lane + (int64_t)(k0 + cc) * ld] : (T)0;
+    {
+        // E consecutive rows per lane (one 16-byte load), 4E columns per pass
+        constexpr int E = Vec<T>::E, G = 64 / E;
+        const int rr = E * (lane % G), kc = lane / G;
+#pragma unroll
+        for (int cc = q * E + kc; cc < 64; cc += 4 * E) {
+            T a[E];
+            ld_rows(P + rr + (int64_t)(k0 + cc) * ld, rr, (k0 + cc < Kc) ? nr : 0, a);
+#pragma unroll
+            for (int i = 0; i < E; i++) tile[rr + i][cc] = a[i];
+        }
+    }
     __syncthreads();
     T acc[NR];
 #pragma unroll
