@@ -128,20 +128,31 @@ __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem
     }
     __syncthreads();
     const int i = r0 + r;
-    T acc[NR];
+    // rows of the inverse: E consecutive per lane (16-byte loads)
+    constexpr int E = Vec<T>::E, G = 64 / E;
+    const int rr = E * (r % G), kc = r / G;
+    T acc[E][NR];
 #pragma unroll
-    for (int u = 0; u < NR; u++) acc[u] = 0;
-    if (i < Kc) {
-        const T *M = w1 + S.cblk_winv[c] + i;
-#pragma unroll 8
-        for (int k = q; k < nrow; k += NW) {
-            const T m = M[(int64_t)k * Kc];
+    for (int e = 0; e < E; e++)
 #pragma unroll
-            for (int u = 0; u < NR; u++)
-                if (u < nrhs) acc[u] += m * y[k + u * Kc];
+        for (int u = 0; u < NR; u++) acc[e][u] = 0;
+    if (r0 + rr < Kc) {
+        const T *M = w1 + S.cblk_winv[c] + r0 + rr;
+#pragma unroll 4
+        for (int k = q * E + kc; k < nrow; k += NW * E) {
+            T m[E];
+            ld_rows(M + (int64_t)k * Kc, rr, Kc - r0, m);
+#pragma unroll
+            for (int u = 0; u < NR; u++) {
+                if (u < nrhs) {
+                    const T yv = y[k + u * Kc];
+#pragma unroll
+                    for (int e = 0; e < E; e++) acc[e][u] += m[e] * yv;
+                }
+            }
         }
     }
-    wave_sum_store<CO, T, NR, NW>(acc, red, r, q, i < Kc, nrhs, z + fc + i, ldz, (const T *)nullptr, 0);
+    wave_sum_store_rows<CO, T, NR, NW, E>(acc, red, r, q, i < Kc, nrhs, z + fc + i, ldz);
 }
 
 template <bool CO, typename T, int NR, int NW>
@@ -264,21 +275,32 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
         __syncthreads();
     }
     const int i = r0 + r;
-    T acc[NR];
+    // rows of the inverse: E consecutive per lane (16-byte loads)
+    constexpr int E = Vec<T>::E, G = 64 / E;
+    const int rr = E * (r % G), kc = r / G;
+    T acc2[E][NR];
 #pragma unroll
-    for (int u = 0; u < NR; u++) acc[u] = 0;
-    if (i < Kc) {
+    for (int e = 0; e < E; e++)
+#pragma unroll
+        for (int u = 0; u < NR; u++) acc2[e][u] = 0;
+    if (r0 + rr < Kc) {
         // wb holds inv(L_cc)^T (or inv(U_cc)) column-major: upper triangular
-        const T *M = wb + S.cblk_winv[c] + i;
-#pragma unroll 8
-        for (int k = r0 + q; k < Kc; k += NW) {
-            const T m = M[(int64_t)k * Kc];
+        const T *M = wb + S.cblk_winv[c] + r0 + rr;
+#pragma unroll 4
+        for (int k = r0 + q * E + kc; k < Kc; k += NW * E) {
+            T m[E];
+            ld_rows(M + (int64_t)k * Kc, rr, Kc - r0, m);
 #pragma unroll
-            for (int u = 0; u < NR; u++)
-                if (u < nrhs) acc[u] += m * y[k + u * Kc];
+            for (int u = 0; u < NR; u++) {
+                if (u < nrhs) {
+                    const T yv = y[k + u * Kc];
+#pragma unroll
+                    for (int e = 0; e < E; e++) acc2[e][u] += m[e] * yv;
+                }
+            }
         }
     }
-    wave_sum_store<CO, T, NR, NW>(acc, red, r, q, i < Kc, nrhs, x + fc + i, ldx, (const T *)nullptr, 0);
+    wave_sum_store_rows<CO, T, NR, NW, E>(acc2, red, r, q, i < Kc, nrhs, x + fc + i, ldx);
 }
 
 // ------------------------------ level kernels -------------------------------
