@@ -278,11 +278,24 @@ template <typename T> int Engine::factorize_t(double thr) {
                     launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, true, L, L, U, false, stream_);
                     launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, true, U, U, L, true, stream_);
                 }
-            } else {
-                if (!right) launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, false, L, L, L, false, stream_);
+            } else if (!right) {
+                // (one launch for the update and the factorisation is slower
+                // here: it reserves the factorisation's LDS for every update
+                // workgroup, which the second stream's updates then lack)
+                launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, false, L, L, L, false, stream_);
                 launch_potrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, W1, (T *)d_WB_, thr, d_stat_, stream_);
                 launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W1, stream_);
-                if (right) launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, true, L, L, L, false, stream_);
+            } else {
+                // right-looking: the step's update and the next step's
+                // diagonal factorisations share one launch
+                if (kb == 0)
+                    launch_potrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, W1, (T *)d_WB_, thr, d_stat_, stream_);
+                launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W1, stream_);
+                if (s + 1 < lv_step_ptr_[lv + 1]) {
+                    const int b1 = step_cb_ptr_[s + 1], nb1 = step_cb_ptr_[s + 2] - b1;
+                    launch_panel_upd_potrf<T>(S_, d_step_cblks_ + b1, nb1, d_pupd_ + q0, nq, kb, true, L, W1,
+                                              (T *)d_WB_, thr, d_stat_, stream_);
+                }
             }
         }
         prof_end(KC_PANEL, pk, stream_);

@@ -28,6 +28,12 @@ void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb
 template <typename T>
 void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, bool right, T *C, const T *X,
                       const T *Y, bool strict, hipStream_t st);
+// An LL^t step's in-cblk update and the diagonal factorisation waiting on it
+// in one launch (right-looking: update of step kb, factorisation of block
+// kb+1; left-looking: update and factorisation of block kb).
+template <typename T>
+void launch_panel_upd_potrf(const DevSymbol &S, const int32_t *cblks, int npot, const PanelItem *items, int n, int kb,
+                            bool right, T *coef, T *winv, T *winvt, double thr, int *stat, hipStream_t st);
 // LU: diagonal 64-block with inverses; final layout of the square parts.
 template <typename T>
 void launch_getrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *lcoef, T *ucoef, T *w1, T *w2, T *wb,

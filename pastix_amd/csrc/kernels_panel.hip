@@ -213,10 +213,8 @@ __device__ __forceinline__ void lower_inverse(T (*Ls)[65], T (*Xs)[65], T *Ts, c
 // place of L in LDS).  W receives X (lower, column-major, ld = K) and WT its
 // transpose, both with explicit zeros in the other triangle of the sub-block.
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
-                                                 T *__restrict__ coef, T *__restrict__ winv, T *__restrict__ winvt,
-                                                 double thr, int *stat) {
-    __builtin_amdgcn_s_setprio(3);  // panel steps lead the critical path
+__device__ __forceinline__ void potrf64_block(const DevSymbol &S, const int c, int kb, T *__restrict__ coef,
+                                              T *__restrict__ winv, T *__restrict__ winvt, double thr, int *stat) {
     __shared__ T fac[1024];
     T(*colb)[4][64] = reinterpret_cast<T(*)[4][64]>(fac);  // [parity][w][row]
     __shared__ T Ls[64][65];
@@ -224,7 +222,6 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
     T *Ts = fac;  // inverse scratch once the factor is done
     const int tid = threadIdx.x, i = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int c = cblks[blockIdx.x];
     const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
     const int k0 = kb * 64;
     const int nb = min(64, K - k0);
@@ -327,6 +324,14 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__restrict__ cblks, int kb,
+                                                 T *__restrict__ coef, T *__restrict__ winv, T *__restrict__ winvt,
+                                                 double thr, int *stat) {
+    __builtin_amdgcn_s_setprio(3);  // panel steps lead the critical path
+    potrf64_block<T>(S, cblks[blockIdx.x], kb, coef, winv, winvt, thr, stat);
+}
+
 // rows [r0, r0+64) below the diagonal sub-block kb: X = A * W_kk^T
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
@@ -367,17 +372,12 @@ __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem
 // keeps the lower part and the diagonal, the U side (transposed) the strict
 // upper part.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
-                                                   int kb, int right, T *__restrict__ ccoef,
-                                                   const T *__restrict__ xcoef, const T *__restrict__ ycoef,
-                                                   int strict) {
-    __builtin_amdgcn_s_setprio(3);
+__device__ __forceinline__ void panel_upd_tile(const DevSymbol &S, const PanelItem it, int kb, int right,
+                                               T *__restrict__ ccoef, const T *__restrict__ xcoef,
+                                               const T *__restrict__ ycoef, int strict) {
     __shared__ StageBuf<T, 16> sb;
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    if (id >= nitems) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    const PanelItem it = items[id];
     const int c = it.c, r0 = it.r0, c0 = it.c0;
     const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
     const int stride = S.cblk_stride[c];
@@ -394,6 +394,48 @@ __global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem 
         QFOR(if (qrow < mr - wr && qcol < mc - wc && (!diag || wr + qrow > wc + qcol - (strict ? 0 : 1)))
                  P[(r0 + wr + qrow) + (int64_t)(c0 + wc + qcol) * ld] -= q.c[m_][n_][r_];)
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
+                                                   int kb, int right, T *__restrict__ ccoef,
+                                                   const T *__restrict__ xcoef, const T *__restrict__ ycoef,
+                                                   int strict) {
+    __builtin_amdgcn_s_setprio(3);
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    panel_upd_tile<T>(S, items[id], kb, right, ccoef, xcoef, ycoef, strict);
+}
+
+// One launch for an LL^t panel step's in-cblk update and the diagonal
+// factorisation that waits on it.  Right-looking (right = 1): step kb's
+// update of the later blocks, and the factorisation of block kb+1;
+// left-looking (right = 0): the update of block kb by the blocks before it,
+// and the factorisation of block kb.  Workgroups [0, npot) update the
+// diagonal tile of the block to factor for cblk cblks[i] and factor it at
+// once (potrf64_block); the others apply the rest of the update (items; the
+// diagonal tiles of that block are skipped, their workgroup above has them).
+// The factorisation no longer waits for the whole update, only for its tile.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_upd_potrf(DevSymbol S, const int32_t *__restrict__ cblks, int npot,
+                                                         const PanelItem *__restrict__ items, int nitems, int kb,
+                                                         int right, T *__restrict__ coef, T *__restrict__ winv,
+                                                         T *__restrict__ winvt, double thr, int *stat) {
+    __builtin_amdgcn_s_setprio(3);
+    const int id = blockIdx.x;
+    const int kf = right ? kb + 1 : kb;  // the block factored here
+    const int k1 = 64 * kf;
+    if (id < npot) {
+        const int c = cblks[id];
+        panel_upd_tile<T>(S, PanelItem{c, k1, k1, 0}, kb, right, coef, coef, coef, 0);
+        __syncthreads();  // the tile's stores reach the whole workgroup
+        potrf64_block<T>(S, c, kf, coef, winv, winvt, thr, stat);
+        return;
+    }
+    if (id - npot >= nitems) return;
+    const PanelItem it = items[id - npot];
+    if (it.r0 == k1 && it.c0 == k1) return;
+    panel_upd_tile<T>(S, it, kb, right, coef, coef, coef, 0);
 }
 
 // Off-diagonal 64-blocks of inv(L_cc) (and of its transpose) for the solve:
@@ -681,6 +723,13 @@ void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb,
                        strict ? 1 : 0);
 }
 template <typename T>
+void launch_panel_upd_potrf(const DevSymbol &S, const int32_t *cblks, int npot, const PanelItem *items, int n, int kb,
+                            bool right, T *coef, T *winv, T *winvt, double thr, int *stat, hipStream_t st) {
+    if (npot + n <= 0) return;
+    hipLaunchKernelGGL(k_panel_upd_potrf<T>, dim3(npot + n), dim3(256), 0, st, S, cblks, npot, items, n, kb,
+                       right ? 1 : 0, coef, winv, winvt, thr, stat);
+}
+template <typename T>
 void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool transposed_factor, const T *coef,
                          T *winv, T *winvt, hipStream_t st) {
     if (n <= 0) return;
@@ -698,6 +747,8 @@ void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool t
     template void launch_panel_trsm<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, hipStream_t); \
     template void launch_panel_upd<T>(const DevSymbol &, const PanelItem *, int, int, bool, T *, const T *,       \
                                       const T *, bool, hipStream_t);                                             \
+    template void launch_panel_upd_potrf<T>(const DevSymbol &, const int32_t *, int, const PanelItem *, int, int,  \
+                                            bool, T *, T *, T *, double, int *, hipStream_t);                    \
     template void launch_winv_offdiag<T>(const DevSymbol &, const int32_t *, int, bool, const T *, T *, T *,       \
                                          hipStream_t);
 INST(double)
