@@ -79,6 +79,14 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     coefnbr_ = L.coefnbr;
     winvnbr_ = P.winvnbr;
     maxwidth_ = (int)L.maxwidth;
+    // widest cblk of each level: sizes the solve kernels' workgroups and
+    // staging (the leaf levels hold thousands of narrow cblks)
+    lv_maxw_.assign(P.lv_cblk_ptr.size() - 1, 1);
+    for (size_t lv = 0; lv + 1 < P.lv_cblk_ptr.size(); lv++)
+        for (int32_t k = P.lv_cblk_ptr[lv]; k < P.lv_cblk_ptr[lv + 1]; k++) {
+            const int c = P.lv_cblks[k];
+            lv_maxw_[lv] = std::max<int>(lv_maxw_[lv], (int)(s.cblk_fcol[c + 1] - s.cblk_fcol[c]));
+        }
     if (n_ >= (int64_t)INT32_MAX) return fail("matrix order exceeds the 32-bit device index range");
 
     S_.cblk_fcol = upload_as<int32_t>(*this, s.cblk_fcol);
@@ -585,8 +593,8 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
                 const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
                 const int f0 = lv_fwd_ptr_[lv], nf = lv_fwd_ptr_[lv + 1] - f0;
                 launch_fwd_diag<T>(S_, d_diag_ + d0, nd, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, nr, W1,
-                                   maxwidth_, stream_);
-                launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, d_ydst_, ybuf, ldy, maxwidth_, stream_);
+                                   lv_maxw_[lv], stream_);
+                launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, d_ydst_, ybuf, ldy, lv_maxw_[lv], stream_);
             }
         }
         if (part_ == 1) {
@@ -598,8 +606,8 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
         for (int lv = nl - 1; lv >= 0; lv--) {
             const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
             const int b0 = lv_bwd_ptr_[lv], nb = lv_bwd_ptr_[lv + 1] - b0;
-            launch_bwd_prod<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, maxwidth_, stream_);
-            launch_bwd_diag<T>(S_, d_diag_ + d0, nd, d_bwd_, z, ldz, xx, ldx, nr, part, WB, maxwidth_, stream_);
+            launch_bwd_prod<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, lv_maxw_[lv], stream_);
+            launch_bwd_diag<T>(S_, d_diag_ + d0, nd, d_bwd_, z, ldz, xx, ldx, nr, part, WB, lv_maxw_[lv], stream_);
         }
     }
     HIPCHK(hipGetLastError());

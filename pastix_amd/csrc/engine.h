@@ -133,6 +133,7 @@ class Engine {
 
     int64_t n_ = 0, cblknbr_ = 0, coefnbr_ = 0, winvnbr_ = 0;
     int maxwidth_ = 0;
+    std::vector<int> lv_maxw_;  // widest cblk per level
     DevSymbol S_{};
     // host copies of the level pointers
     std::vector<int32_t> lv_cblk_ptr_, lv_upd_ptr_, lv_upd_split_, lv_fwd_ptr_, lv_bwd_ptr_;

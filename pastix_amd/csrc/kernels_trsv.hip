@@ -30,6 +30,9 @@
 namespace pxa {
 
 static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
+// levels whose widest cblk has at most this many columns run the solve
+// kernels on 4-wave workgroups
+constexpr int SOLVE_NARROW = 64;
 
 // Loads and stores of the data one solve task hands to another (update
 // buffer, forward result, solution rows, backward partials).  CO: device-
@@ -507,20 +510,36 @@ void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int
                      hipStream_t st) {
     if (n <= 0) return;
     // several right-hand sides: pairs of columns on the grid's y dimension,
-    // so that the few diagonal blocks of the top levels use more of the chip
-    if (nrhs == 1)
-        hipLaunchKernelGGL((k_fwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st, S,
-                           items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
-    else
-        hipLaunchKernelGGL((k_fwd_diag<T, 2, 16>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
-                           (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz,
-                           nrhs, w1);
+    // so that the few diagonal blocks of the top levels use more of the chip;
+    // levels of narrow cblks (maxwidth = the level's widest) take 4-wave
+    // workgroups
+    const bool narrow = maxwidth <= SOLVE_NARROW;
+    if (nrhs == 1) {
+        if (narrow)
+            hipLaunchKernelGGL((k_fwd_diag<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
+                               items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+        else
+            hipLaunchKernelGGL((k_fwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st,
+                               S, items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+    } else {
+        if (narrow)
+            hipLaunchKernelGGL((k_fwd_diag<T, 2, 4>), dim3(grid8(n), (nrhs + 1) / 2), dim3(256),
+                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz,
+                               nrhs, w1);
+        else
+            hipLaunchKernelGGL((k_fwd_diag<T, 2, 16>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
+                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz,
+                               nrhs, w1);
+    }
 }
 template <typename T>
 void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *z, int64_t ldz,
                      int nrhs, const int32_t *ydst, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st) {
     if (n <= 0) return;
-    if (nrhs == 1)
+    if (nrhs == 1 && maxwidth <= SOLVE_NARROW)
+        hipLaunchKernelGGL((k_fwd_prod<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
+                           items, n, coef, z, ldz, nrhs, ydst, ybuf, ldy);
+    else if (nrhs == 1)
         hipLaunchKernelGGL((k_fwd_prod<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st, S,
                            items, n, coef, z, ldz, nrhs, ydst, ybuf, ldy);
     else
@@ -542,12 +561,24 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
                      int64_t ldz, T *x, int64_t ldx, int nrhs, const T *part, const T *wb, int maxwidth,
                      hipStream_t st) {
     if (n <= 0) return;
-    if (nrhs == 1)
-        hipLaunchKernelGGL((k_bwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st, S,
-                           items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
-    else
-        hipLaunchKernelGGL((k_bwd_diag<T, 2, 16>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
-                           (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
+    const bool narrow = maxwidth <= SOLVE_NARROW;
+    if (nrhs == 1) {
+        if (narrow)
+            hipLaunchKernelGGL((k_bwd_diag<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
+                               items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
+        else
+            hipLaunchKernelGGL((k_bwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st,
+                               S, items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
+    } else {
+        if (narrow)
+            hipLaunchKernelGGL((k_bwd_diag<T, 2, 4>), dim3(grid8(n), (nrhs + 1) / 2), dim3(256),
+                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, bitems, z, ldz, x, ldx, nrhs, part,
+                               wb);
+        else
+            hipLaunchKernelGGL((k_bwd_diag<T, 2, 16>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
+                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, bitems, z, ldz, x, ldx, nrhs, part,
+                               wb);
+    }
 }
 
 template <typename T>
