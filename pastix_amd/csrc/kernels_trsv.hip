@@ -31,8 +31,10 @@ namespace pxa {
 
 static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
 // levels whose widest cblk has at most this many columns run the solve
-// kernels on 4-wave workgroups
+// kernels on 4-wave workgroups,
 constexpr int SOLVE_NARROW = 64;
+// and one-wave workgroups at most this many (the leaf levels)
+constexpr int SOLVE_TINY = 16;
 
 // Loads and stores of the data one solve task hands to another (update
 // buffer, forward result, solution rows, backward partials).  CO: device-
@@ -515,7 +517,10 @@ void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int
     // workgroups
     const bool narrow = maxwidth <= SOLVE_NARROW;
     if (nrhs == 1) {
-        if (narrow)
+        if (maxwidth <= SOLVE_TINY)
+            hipLaunchKernelGGL((k_fwd_diag<T, 1, 1>), dim3(grid8(n)), dim3(64), (size_t)maxwidth * sizeof(T), st, S,
+                               items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+        else if (narrow)
             hipLaunchKernelGGL((k_fwd_diag<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
                                items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
         else
@@ -536,7 +541,10 @@ template <typename T>
 void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *z, int64_t ldz,
                      int nrhs, const int32_t *ydst, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st) {
     if (n <= 0) return;
-    if (nrhs == 1 && maxwidth <= SOLVE_NARROW)
+    if (nrhs == 1 && maxwidth <= SOLVE_TINY)
+        hipLaunchKernelGGL((k_fwd_prod<T, 1, 1>), dim3(grid8(n)), dim3(64), (size_t)maxwidth * sizeof(T), st, S,
+                           items, n, coef, z, ldz, nrhs, ydst, ybuf, ldy);
+    else if (nrhs == 1 && maxwidth <= SOLVE_NARROW)
         hipLaunchKernelGGL((k_fwd_prod<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
                            items, n, coef, z, ldz, nrhs, ydst, ybuf, ldy);
     else if (nrhs == 1)
@@ -563,7 +571,10 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
     if (n <= 0) return;
     const bool narrow = maxwidth <= SOLVE_NARROW;
     if (nrhs == 1) {
-        if (narrow)
+        if (maxwidth <= SOLVE_TINY)
+            hipLaunchKernelGGL((k_bwd_diag<T, 1, 1>), dim3(grid8(n)), dim3(64), (size_t)maxwidth * sizeof(T), st, S,
+                               items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
+        else if (narrow)
             hipLaunchKernelGGL((k_bwd_diag<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
                                items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
         else
