@@ -369,7 +369,8 @@ def main():
         f"THFLOPS {thflops:.3e}")
     t0 = time.perf_counter()
     pastix.task_numfact(h, S)
-    log(f"first numfact (fill + upload + factor) {time.perf_counter() - t0:.2f}s, device {h.dparm[pastix.dparm.fact_time]*1e3:.1f} ms")
+    t_first = time.perf_counter() - t0
+    log(f"first numfact (fill + upload + factor) {t_first:.2f}s, device {h.dparm[pastix.dparm.fact_time]*1e3:.1f} ms")
 
     for _ in range(args.warmup):
         pastix.refactorize(h)
@@ -385,6 +386,21 @@ def main():
     if args.profile_only:
         print(json.dumps({"numfact_ms": tmed * 1e3, "gflops": gflops}))
         return
+
+    # pastix_task_numfact on new values of the analysed pattern: permuted
+    # values by one gather on the host, their upload, the device fill of the
+    # tables and the factorisation (wall clock); then the original values back
+    refill = {"first_numfact_wall_s": t_first}
+    walls = []
+    for k, dg in enumerate((6.5, 6.25, 6.0)):
+        S.set_values(spm.laplacian7(*grid, diag=dg))
+        t0 = time.perf_counter()
+        pastix.task_numfact(h, S)
+        walls.append(time.perf_counter() - t0)
+        log(f"numfact with new values (diag {dg}): wall {walls[-1]*1e3:.1f} ms, device "
+            f"{h.dparm[pastix.dparm.fact_time]*1e3:.1f} ms")
+    refill["numfact_new_values_wall_ms"] = float(np.median(walls)) * 1e3
+    refill["device_numfact_ms"] = float(h.dparm[pastix.dparm.fact_time]) * 1e3
 
     # per-kernel split of one more factorisation (HIP events around every
     # update launch), on one stream so that no kernel shares the device
@@ -507,6 +523,7 @@ def main():
         "wall_ms_per_step": wall / args.steps * 1e3,
         "solve_p50_ms": p50, "solve_p99_ms": p99, "solve_gbs_p50": solve_gbs, "solve_berr": berr64,
         "fp32_ir": fp32,
+        "refill": refill,
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": kern[dom]["tflops"],
                      "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": kern[dom]["tflops"] / FP64_MFMA_PEAK_TFLOPS,

@@ -130,34 +130,38 @@ int pxa_api::spm_triplets(const spmatrix_t *spm, Triplets &t, bool values) {
     const spm_int_t *cp = spm->colptr, *rp = spm->rowptr;
     t.i.reserve(2 * nnz);
     t.j.reserve(2 * nnz);
+    t.src.reserve(2 * nnz);
     if (values) t.v.reserve(2 * nnz);
     auto val = [&](int64_t k) -> double {
         if (!values || spm->flttype == SpmPattern || !spm->values) return 1.0;
         return spm->flttype == SpmDouble ? ((const double *)spm->values)[k] : ((const float *)spm->values)[k];
     };
-    auto add = [&](int64_t i, int64_t j, double v) {
+    auto add = [&](int64_t i, int64_t j, int64_t k) {
+        const double v = val(k);
         t.i.push_back(i);
         t.j.push_back(j);
+        t.src.push_back(k);
         if (values) t.v.push_back(v);
         if (spm->mtxtype != SpmGeneral && i != j) {
             t.i.push_back(j);
             t.j.push_back(i);
+            t.src.push_back(k);
             if (values) t.v.push_back(v);
         }
     };
     if (spm->fmttype == SpmCSC) {
         const int64_t base = cp[0];
         for (int64_t j = 0; j < n; j++)
-            for (int64_t k = cp[j] - base; k < cp[j + 1] - base; k++) add(rp[k] - base, j, val(k));
+            for (int64_t k = cp[j] - base; k < cp[j + 1] - base; k++) add(rp[k] - base, j, k);
     } else if (spm->fmttype == SpmCSR) {
         const int64_t base = rp[0];
         for (int64_t i = 0; i < n; i++)
-            for (int64_t k = rp[i] - base; k < rp[i + 1] - base; k++) add(i, cp[k] - base, val(k));
+            for (int64_t k = rp[i] - base; k < rp[i + 1] - base; k++) add(i, cp[k] - base, k);
     } else {
         int64_t base = INT64_MAX;
         for (int64_t k = 0; k < nnz; k++) base = std::min<int64_t>(base, std::min<int64_t>(cp[k], rp[k]));
         if (nnz == 0) base = 0;
-        for (int64_t k = 0; k < nnz; k++) add(rp[k] - base, cp[k] - base, val(k));
+        for (int64_t k = 0; k < nnz; k++) add(rp[k] - base, cp[k] - base, k);
     }
     for (size_t k = 0; k < t.i.size(); k++)
         if (t.i[k] < 0 || t.i[k] >= n || t.j[k] < 0 || t.j[k] >= n) return PASTIX_ERR_BADPARAMETER;
@@ -167,7 +171,8 @@ int pxa_api::spm_triplets(const spmatrix_t *spm, Triplets &t, bool values) {
 // CSC (sorted, duplicates summed) of the n x n matrix given by triplets,
 // optionally permuted: B(perm[i], perm[j]) = A(i, j).
 void pxa_api::triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
-                            std::vector<int32_t> &rows, std::vector<double> &vals, bool transpose) {
+                              std::vector<int32_t> &rows, std::vector<double> &vals, bool transpose,
+                              std::vector<int64_t> *eptr, std::vector<int64_t> *esrc) {
     const size_t m = t.i.size();
     colptr.assign(n + 1, 0);
     std::vector<int64_t> I(m), J(m);
@@ -180,34 +185,43 @@ void pxa_api::triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm,
         colptr[j + 1]++;
     }
     for (int64_t j = 0; j < n; j++) colptr[j + 1] += colptr[j];
+    // bucket by column, triplets in input order within a column
     std::vector<int64_t> pos(colptr.begin(), colptr.end() - 1);
-    std::vector<int64_t> tr(m);
-    std::vector<double> tv(m);
-    for (size_t k = 0; k < m; k++) {
-        int64_t p = pos[J[k]]++;
-        tr[p] = I[k];
-        tv[p] = t.v.empty() ? 1.0 : t.v[k];
-    }
-    // sort each column by row and sum duplicates
+    std::vector<int64_t> tk(m);
+    for (size_t k = 0; k < m; k++) tk[pos[J[k]]++] = (int64_t)k;
+    // sort each column by (row, triplet index): duplicates are summed in
+    // input order, so that the sum is a fixed function of the input
     rows.clear();
     vals.clear();
     rows.reserve(m);
     vals.reserve(m);
+    if (eptr) {
+        eptr->clear();
+        eptr->reserve(m + 1);
+    }
+    if (esrc) {
+        esrc->clear();
+        esrc->reserve(m);
+    }
     std::vector<int64_t> newptr(n + 1, 0);
-    std::vector<std::pair<int64_t, double>> col;
+    std::vector<std::pair<int64_t, int64_t>> col;
     for (int64_t j = 0; j < n; j++) {
         col.clear();
-        for (int64_t p = colptr[j]; p < colptr[j + 1]; p++) col.emplace_back(tr[p], tv[p]);
-        std::sort(col.begin(), col.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        for (int64_t p = colptr[j]; p < colptr[j + 1]; p++) col.emplace_back(I[tk[p]], tk[p]);
+        std::sort(col.begin(), col.end());
         for (size_t q = 0; q < col.size(); q++) {
-            if (q > 0 && col[q].first == col[q - 1].first) vals.back() += col[q].second;
+            const double v = t.v.empty() ? 1.0 : t.v[col[q].second];
+            if (q > 0 && col[q].first == col[q - 1].first) vals.back() += v;
             else {
+                if (eptr) eptr->push_back(esrc ? (int64_t)esrc->size() : 0);
                 rows.push_back((int32_t)col[q].first);
-                vals.push_back(col[q].second);
+                vals.push_back(v);
             }
+            if (esrc) esrc->push_back(col[q].second);
         }
         newptr[j + 1] = (int64_t)rows.size();
     }
+    if (eptr) eptr->push_back(esrc ? (int64_t)esrc->size() : 0);
     colptr = newptr;
 }
 
@@ -274,6 +288,7 @@ extern "C" int pastix_subtask_order(pastix_data_t *d, const spmatrix_t *spm, pas
     d->steps |= STEP_ORDERING;
     d->steps &= ~(STEP_SYMBFACT | STEP_ANALYSE | STEP_CSC2BCSC | STEP_BCSC2CTAB | STEP_NUMFACT | STEP_SOLVE | STEP_REFINE);
     d->engine.reset();
+    d->bcsc = pxa::HostBcsc();  // the permuted matrix follows the new ordering
     return PASTIX_SUCCESS;
 }
 
@@ -316,6 +331,9 @@ extern "C" int pastix_subtask_blend(pastix_data_t *d) {
     d->dparm[DPARM_ANALYZE_TIME] = wall() - t0;
     d->steps |= STEP_ANALYSE;
     d->engine.reset();
+    // table positions follow the new partition
+    d->bcsc.cmap.clear();
+    d->bcsc.cmap_facto = -1;
     return PASTIX_SUCCESS;
 }
 
@@ -335,6 +353,57 @@ extern "C" int pastix_task_analyze(pastix_data_t *d, spmatrix_t *spm) {
 // --------------------------------------------------------------------------
 // Numerical factorisation subtasks
 // --------------------------------------------------------------------------
+// Refill of an input with the analysed pattern (the reference rebuilds its
+// bcsc from the spm on every numfact, bcsc/bcsc_zinit.c:485-541): the input
+// pattern is kept as given, and a later input with the same format, type,
+// size and index arrays takes the permuted values by one gather.
+static void keep_pattern(const spmatrix_t *spm, pxa::HostBcsc &bc) {
+    const int64_t n = spm->n, nnz = spm->nnz;
+    const int64_t nc = spm->fmttype == SpmCSC ? n + 1 : nnz, nr = spm->fmttype == SpmCSR ? n + 1 : nnz;
+    bc.in_fmt = spm->fmttype;
+    bc.in_mtx = spm->mtxtype;
+    bc.in_n = n;
+    bc.in_nnz = nnz;
+    bc.in_colptr.assign(spm->colptr, spm->colptr + nc);
+    bc.in_rowptr.assign(spm->rowptr, spm->rowptr + nr);
+}
+
+static bool same_pattern(const spmatrix_t *spm, const pxa::HostBcsc &bc) {
+    if (bc.in_fmt < 0 || spm->fmttype != bc.in_fmt || spm->mtxtype != bc.in_mtx || spm->n != bc.in_n ||
+        spm->nnz != bc.in_nnz || spm->dof != 1)
+        return false;
+    if (spm->flttype != SpmDouble && spm->flttype != SpmFloat && spm->flttype != SpmPattern) return false;
+    for (size_t k = 0; k < bc.in_colptr.size(); k++)
+        if (spm->colptr[k] != bc.in_colptr[k]) return false;
+    for (size_t k = 0; k < bc.in_rowptr.size(); k++)
+        if (spm->rowptr[k] != bc.in_rowptr[k]) return false;
+    return true;
+}
+
+// The values the triplet path gives (spm_triplets + triplets_to_csc): the
+// same input values summed in the same order.
+static void refill_values(const spmatrix_t *spm, pxa::HostBcsc &bc) {
+    const int64_t m = (int64_t)bc.tvals.size();
+    if (spm->flttype == SpmPattern || !spm->values) {
+        for (int64_t e = 0; e < m; e++) {
+            double v = 1.0;
+            for (int64_t q = bc.sptr[e] + 1; q < bc.sptr[e + 1]; q++) v += 1.0;
+            bc.tvals[e] = v;
+        }
+        return;
+    }
+    auto gather = [&](auto *vals) {
+        for (int64_t e = 0; e < m; e++) {
+            int64_t q = bc.sptr[e];
+            double v = (double)vals[bc.src[q]];
+            for (q++; q < bc.sptr[e + 1]; q++) v += (double)vals[bc.src[q]];
+            bc.tvals[e] = v;
+        }
+    };
+    if (spm->flttype == SpmDouble) gather((const double *)spm->values);
+    else gather((const float *)spm->values);
+}
+
 extern "C" int pastix_subtask_spm2bcsc(pastix_data_t *d, spmatrix_t *spm) {
     if (!d) { err_print("pastix_subtask_spm2bcsc", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
     if (!spm) { err_print("pastix_subtask_spm2bcsc", "wrong spm parameter"); return PASTIX_ERR_BADPARAMETER; }
@@ -343,23 +412,36 @@ extern "C" int pastix_subtask_spm2bcsc(pastix_data_t *d, spmatrix_t *spm) {
         return PASTIX_ERR_BADPARAMETER;
     }
     if (spm->n != d->n) { err_print("pastix_subtask_spm2bcsc", "matrix size differs from the analysed one"); return PASTIX_ERR_BADPARAMETER; }
-    Triplets t;
-    int rc = spm_triplets(spm, t, true);
-    if (rc) return rc;
     pxa::HostBcsc &bc = d->bcsc;
-    bc.n = d->n;
-    bc.mtxtype = spm->mtxtype;
-    triplets_to_csc(d->n, t, d->order.permtab.data(), bc.colptr, bc.rows, bc.vals, false);
-    triplets_to_csc(d->n, t, d->order.permtab.data(), bc.rowptr, bc.cols, bc.tvals, true);
-    double f2 = 0;
-    for (double v : bc.vals) f2 += v * v;
-    bc.normF = std::sqrt(f2);
-    double ninf = 0;
-    for (int64_t i = 0; i < d->n; i++) {
-        double s = 0;
-        for (int64_t k = bc.rowptr[i]; k < bc.rowptr[i + 1]; k++) s += std::fabs(bc.tvals[k]);
-        ninf = std::max(ninf, s);
+    if (same_pattern(spm, bc)) {
+        // the analysed pattern again: the permuted values by one gather
+        refill_values(spm, bc);
+    } else {
+        Triplets t;
+        int rc = spm_triplets(spm, t, true);
+        if (rc) return rc;
+        bc.n = d->n;
+        bc.mtxtype = spm->mtxtype;
+        std::vector<int64_t> tsrc;
+        triplets_to_csc(d->n, t, d->order.permtab.data(), bc.rowptr, bc.cols, bc.tvals, true, &bc.sptr, &tsrc);
+        bc.src.resize(tsrc.size());
+        for (size_t q = 0; q < tsrc.size(); q++) bc.src[q] = t.src[tsrc[q]];
+        keep_pattern(spm, bc);
+        bc.cmap.clear();
+        bc.cmap_facto = -1;
+        if (d->engine) d->engine->drop_matrix();
     }
+    // norms of the permuted matrix (CSR order)
+    double f2 = 0, ninf = 0;
+    for (int64_t i = 0; i < d->n; i++) {
+        double r = 0;
+        for (int64_t k = bc.rowptr[i]; k < bc.rowptr[i + 1]; k++) {
+            f2 += bc.tvals[k] * bc.tvals[k];
+            r += std::fabs(bc.tvals[k]);
+        }
+        ninf = std::max(ninf, r);
+    }
+    bc.normF = std::sqrt(f2);
     bc.normInf = ninf;
     d->dparm[DPARM_A_NORM] = bc.normF;
     d->flttype = spm->flttype == SpmPattern ? SpmDouble : spm->flttype;
@@ -368,44 +450,47 @@ extern "C" int pastix_subtask_spm2bcsc(pastix_data_t *d, spmatrix_t *spm) {
     return PASTIX_SUCCESS;
 }
 
-template <typename T> static void fill_coeftab(pastix_data_t *d, int facto) {
+// Position in the coefficient table of every CSR entry of the permuted
+// matrix (reference kernels/cpucblk_zinit.c:229-309, cpucblk_zfillin_fr):
+// entry (i, j) with i >= j goes to L(i, j) in the cblk of column j; with
+// i < j, for LU, B(i, j) goes to U(j, i) in the cblk of column i (the U
+// panels hold U^t).  Entries outside the symbol are dropped (-1).
+static bool build_fillmap(pastix_data_t *d, int facto) {
     const pxa::Symbol &s = d->symbol;
     const pxa::Layout &L = d->layout;
-    const pxa::HostBcsc &bc = d->bcsc;
-    d->hostL.assign(L.coefnbr * sizeof(T), 0);
-    T *Lc = (T *)d->hostL.data();
-    T *Uc = nullptr;
-    if (facto == 2) {
-        d->hostU.assign(L.coefnbr * sizeof(T), 0);
-        Uc = (T *)d->hostU.data();
-    }
-    // reference kernels/cpucblk_zinit.c:229-309 (cpucblk_zfillin_fr)
-    for (int64_t c = 0; c < s.cblknbr(); c++) {
-        const int64_t fc = s.cblk_fcol[c], lc = s.cblk_fcol[c + 1];
-        const int64_t st = L.cblk_stride[c];
-        T *P = Lc + L.cblk_coef[c];
-        for (int64_t j = fc; j < lc; j++) {
-            int64_t b = s.cblk_bloknum[c];
-            for (int64_t k = bc.colptr[j]; k < bc.colptr[j + 1]; k++) {
-                const int64_t i = bc.rows[k];
-                if (i < j) continue;
-                while (b < s.cblk_bloknum[c + 1] && s.blok_lrow[b] < i) b++;
-                if (b >= s.cblk_bloknum[c + 1] || s.blok_frow[b] > i) continue;  // dropped (outside the symbol)
-                P[L.blok_roff[b] + (i - s.blok_frow[b]) + (j - fc) * st] = (T)bc.vals[k];
+    pxa::HostBcsc &bc = d->bcsc;
+    const int64_t n = d->n;
+    std::vector<int32_t> col2cblk(n);
+    for (int64_t c = 0; c < s.cblknbr(); c++)
+        for (int64_t j = s.cblk_fcol[c]; j < s.cblk_fcol[c + 1]; j++) col2cblk[j] = (int32_t)c;
+    auto position = [&](int64_t i, int64_t j) -> int64_t {
+        const int64_t c = col2cblk[j];
+        int64_t lo = s.cblk_bloknum[c], hi = s.cblk_bloknum[c + 1];
+        while (lo < hi) {  // first block whose last row is >= i
+            const int64_t mid = (lo + hi) / 2;
+            if (s.blok_lrow[mid] < i) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo >= s.cblk_bloknum[c + 1] || s.blok_frow[lo] > i) return -1;
+        return L.cblk_coef[c] + L.blok_roff[lo] + (i - s.blok_frow[lo]) + (j - s.cblk_fcol[c]) * L.cblk_stride[c];
+    };
+    bc.cmap.assign(bc.tvals.size(), -1);
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t e = bc.rowptr[i]; e < bc.rowptr[i + 1]; e++) {
+            const int64_t j = bc.cols[e];
+            int64_t p = -1;
+            if (i >= j) p = position(i, j);
+            else if (facto == 2) {
+                const int64_t q = position(j, i);
+                p = q >= 0 ? -2 - q : -1;
             }
-            if (Uc) {
-                T *Q = Uc + L.cblk_coef[c];
-                int64_t bu = s.cblk_bloknum[c];
-                for (int64_t k = bc.rowptr[j]; k < bc.rowptr[j + 1]; k++) {
-                    const int64_t i = bc.cols[k];
-                    if (i <= j) continue;
-                    while (bu < s.cblk_bloknum[c + 1] && s.blok_lrow[bu] < i) bu++;
-                    if (bu >= s.cblk_bloknum[c + 1] || s.blok_frow[bu] > i) continue;
-                    Q[L.blok_roff[bu] + (i - s.blok_frow[bu]) + (j - fc) * st] = (T)bc.tvals[k];
-                }
-            }
+            // every position lies inside the table
+            if (p >= L.coefnbr || -2 - p >= L.coefnbr) return false;
+            bc.cmap[e] = p;
         }
     }
+    bc.cmap_facto = facto;
+    return true;
 }
 
 // engine switches held in the extension parameters
@@ -441,15 +526,20 @@ extern "C" int pastix_subtask_bcsc2ctab(pastix_data_t *d) {
             return PASTIX_ERR_INTERNAL;
         }
     }
-    if (prec == SpmFloat) fill_coeftab<float>(d, facto);
-    else fill_coeftab<double>(d, facto);
-    if (d->engine->upload_coef(d->hostL.data(), facto == 2 ? d->hostU.data() : nullptr)) return PASTIX_ERR_INTERNAL;
+    // the permuted values go to the device, which fills the tables
     if (d->engine->upload_matrix(d->bcsc)) return PASTIX_ERR_INTERNAL;
+    if (d->bcsc.cmap_facto != facto) {
+        if (!build_fillmap(d, facto)) {
+            err_print("pastix_subtask_bcsc2ctab", "coefficient position outside the table");
+            return PASTIX_ERR_INTERNAL;
+        }
+        d->engine->drop_fillmap();
+    }
+    if (!d->engine->has_fillmap() && d->engine->upload_fillmap(d->bcsc.cmap)) return PASTIX_ERR_INTERNAL;
+    if (d->engine->fill_coef()) return PASTIX_ERR_INTERNAL;
     if (d->engine->upload_perm(d->order.permtab)) return PASTIX_ERR_INTERNAL;
     if (d->ext[PXA_IPARM_RESIDENT_INPUT] && d->engine->snapshot_coef()) return PASTIX_ERR_INTERNAL;
     engine_flags(d);
-    std::vector<char>().swap(d->hostL);
-    std::vector<char>().swap(d->hostU);
     d->steps |= STEP_BCSC2CTAB;
     return PASTIX_SUCCESS;
 }

@@ -206,14 +206,6 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     return 0;
 }
 
-int Engine::upload_coef(const void *hL, const void *hU) {
-    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
-    HIPCHK(hipMemcpyAsync(d_L_, hL, coefnbr_ * fsz, hipMemcpyHostToDevice, stream_));
-    if (facto_ == 2) HIPCHK(hipMemcpyAsync(d_U_, hU, coefnbr_ * fsz, hipMemcpyHostToDevice, stream_));
-    HIPCHK(hipStreamSynchronize(stream_));
-    return 0;
-}
-
 int Engine::download_coef(void *hL, void *hU) {
     const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
     if (hL) HIPCHK(hipMemcpyAsync(hL, d_L_, coefnbr_ * fsz, hipMemcpyDeviceToHost, stream_));
@@ -685,6 +677,8 @@ int Engine::solve_host(const int64_t *perm, int flttype, void *b, int64_t ldb, i
 
 int Engine::upload_matrix(const HostBcsc &bc) {
     if (!have_matrix_) {
+        have_fillmap_ = false;
+        csr_nnz_ = (int64_t)bc.tvals.size();
         d_rowptr_ = upload(*this, bc.rowptr);
         d_cols_ = upload(*this, bc.cols);
         d_vals_ = (double *)alloc(bc.tvals.size() * sizeof(double));
@@ -708,7 +702,32 @@ int Engine::upload_matrix(const HostBcsc &bc) {
         for (int64_t i = 0; i < n_; i++) diag_sum_ += dg[i];
         HIPCHK(hipMemcpy(d_diag0_, dg.data(), n_ * sizeof(double), hipMemcpyHostToDevice));
     }
+    if ((int64_t)bc.tvals.size() != csr_nnz_) return fail("matrix pattern changed without drop_matrix");
     HIPCHK(hipMemcpyAsync(d_vals_, bc.tvals.data(), bc.tvals.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Engine::upload_fillmap(const std::vector<int64_t> &cmap) {
+    if (!have_matrix_ || (int64_t)cmap.size() != csr_nnz_) return fail("fill map does not match the uploaded matrix");
+    d_cmap_ = upload(*this, cmap);
+    if (!d_cmap_) return -1;
+    have_fillmap_ = true;
+    return 0;
+}
+
+int Engine::fill_coef() {
+    if (!have_matrix_ || !have_fillmap_) return fail("coefficient fill needs the matrix and its fill map");
+    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    HIPCHK(hipMemsetAsync(d_L_, 0, coefnbr_ * fsz, stream_));
+    if (facto_ == 2) HIPCHK(hipMemsetAsync(d_U_, 0, coefnbr_ * fsz, stream_));
+    if (prec_ == 2)
+        launch_fill_coef<float>(csr_nnz_, d_cmap_, d_vals_, (float *)d_L_, facto_ == 2 ? (float *)d_U_ : nullptr,
+                                stream_);
+    else
+        launch_fill_coef<double>(csr_nnz_, d_cmap_, d_vals_, (double *)d_L_, facto_ == 2 ? (double *)d_U_ : nullptr,
+                                 stream_);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
     return 0;
 }

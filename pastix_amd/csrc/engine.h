@@ -26,8 +26,7 @@ class Engine {
     int prec() const { return prec_; }
     int factotype() const { return facto_; }
 
-    // Host coefficient tables (1D layout, factor precision) to the device.
-    int upload_coef(const void *hL, const void *hU);
+    // Coefficient tables (1D layout, factor precision) to the host.
     int download_coef(void *hL, void *hU);
     // Numerical factorisation over all levels; seconds = device time.
     int factorize(double thr, int64_t *npiv, int *negdiag, double *seconds);
@@ -42,6 +41,16 @@ class Engine {
 
     // Permuted matrix in CSR form for the residual SpMV (always FP64).
     int upload_matrix(const HostBcsc &bc);
+    // Forget the uploaded CSR pattern and fill map (the input's pattern
+    // changed); the next upload_matrix sends them again.
+    void drop_matrix() { have_matrix_ = false; have_fillmap_ = false; }
+    // Coefficient tables filled on the device from the uploaded CSR values:
+    // cmap gives per CSR entry its position in the L table (>= 0), in the U
+    // table (-2 - position) or -1 (outside the symbol).
+    bool has_fillmap() const { return have_fillmap_; }
+    void drop_fillmap() { have_fillmap_ = false; }
+    int upload_fillmap(const std::vector<int64_t> &cmap);
+    int fill_coef();
     // Device refinement drivers (vectors in the permuted ordering, FP64).
     int refine_cg(double *dx, const double *db, int itermax, double eps, int *niter, double *resid);
     int refine_gmres(double *dx, const double *db, int itermax, int im, double eps, int *niter, double *resid);
@@ -204,7 +213,10 @@ class Engine {
     int64_t *d_rowptr_ = nullptr;
     int32_t *d_cols_ = nullptr;
     double *d_vals_ = nullptr;
+    int64_t csr_nnz_ = 0;
     bool have_matrix_ = false;
+    int64_t *d_cmap_ = nullptr;
+    bool have_fillmap_ = false;
     // diagonal positions: in the coefficient table (per permuted column) and
     // in the CSR values, with the original FP64 diagonal
     int64_t *d_diagpos_ = nullptr, *d_csrdiag_ = nullptr;
@@ -245,15 +257,27 @@ class Engine {
 };
 
 // Host-side permuted matrix (reference bcsc/bcsc.h:28-47 keeps the same data
-// grouped by cblk).  colptr/rows/vals: CSC of B = P A P^T with the full
-// pattern; trowptr/tcols/tvals: CSR of B (= CSC of B^T).
+// grouped by cblk): rowptr/cols/tvals, the CSR of B = P A P^T with the full
+// pattern (duplicates summed in input order).
 struct HostBcsc {
     int64_t n = 0;
     int mtxtype = 0;
-    std::vector<int64_t> colptr, rowptr;
-    std::vector<int32_t> rows, cols;
-    std::vector<double> vals, tvals;
+    std::vector<int64_t> rowptr;
+    std::vector<int32_t> cols;
+    std::vector<double> tvals;
     double normF = 0, normInf = 0;
+    // A later input with the same pattern is refilled by one gather: the
+    // input's format, type, size and index arrays as given, and per CSR
+    // entry e the input value indices src[sptr[e] .. sptr[e+1]) summed into
+    // it, in input order.
+    int in_fmt = -1, in_mtx = -1;
+    int64_t in_n = -1, in_nnz = -1;
+    std::vector<int64_t> in_colptr, in_rowptr;
+    std::vector<int64_t> sptr, src;
+    // coefficient-table position of every CSR entry (Engine::upload_fillmap)
+    // for the factorisation kind cmap_facto; -1: not built
+    std::vector<int64_t> cmap;
+    int cmap_facto = -1;
 };
 
 }  // namespace pxa

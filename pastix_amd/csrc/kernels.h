@@ -108,6 +108,11 @@ void launch_solve_tasks(const DevSymbol &S, const SolveTask *tasks, const int32_
                         T *ybuf, int64_t ldy, T *x, int64_t ldx, T *z, int64_t ldz, int nrhs, const T *L, const T *U,
                         const T *w1, const T *wb, const int32_t *ydst, T *part, int maxwidth, int nwg,
                         hipStream_t st);
+// Coefficient tables (zeroed beforehand) from the FP64 CSR values of the
+// permuted matrix: L[cmap[e]] for cmap[e] >= 0, U[-2 - cmap[e]] for
+// cmap[e] <= -2 (when U is given), nothing at -1.
+template <typename T>
+void launch_fill_coef(int64_t nnz, const int64_t *cmap, const double *vals, T *L, T *U, hipStream_t st);
 template <typename T> void launch_copy2d(int64_t n, int nr, const T *x, int64_t ldx, T *y, int64_t ldy, hipStream_t st);
 template <typename T>
 void launch_diag_scale(int64_t n, int nr, const int64_t *pos, const T *coef, bool divide, T *x, int64_t ldx,

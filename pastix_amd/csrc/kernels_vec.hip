@@ -304,6 +304,31 @@ void launch_set_csr_diag(int64_t n, const int64_t *pos, const double *d0, double
     if (n > 0) hipLaunchKernelGGL(k_set_csr_diag, dim3(nblk(n)), dim3(256), 0, st, n, pos, d0, delta, vals);
 }
 
+// Coefficient-table fill (reference kernels/cpucblk_zinit.c:229-309) from
+// the FP64 CSR values of the permuted matrix: entry e goes to L[cmap[e]]
+// when cmap[e] >= 0, to U[-2 - cmap[e]] when cmap[e] <= -2 (U given), and
+// nowhere at -1 (outside the symbol).  The tables are zero beforehand; every
+// position receives at most one entry.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_coef(int64_t nnz, const int64_t *__restrict__ cmap,
+                                                   const double *__restrict__ vals, T *__restrict__ L,
+                                                   T *__restrict__ U) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nnz; e += stride) {
+        const int64_t p = cmap[e];
+        if (p >= 0) L[p] = (T)vals[e];
+        else if (p <= -2 && U) U[-2 - p] = (T)vals[e];
+    }
+}
+template <typename T>
+void launch_fill_coef(int64_t nnz, const int64_t *cmap, const double *vals, T *L, T *U, hipStream_t st) {
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_fill_coef<T>, dim3(nnz > 16384 * 256 ? 16384u : nblk(nnz)), dim3(256), 0, st, nnz,
+                           cmap, vals, L, U);
+}
+template void launch_fill_coef<double>(int64_t, const int64_t *, const double *, double *, double *, hipStream_t);
+template void launch_fill_coef<float>(int64_t, const int64_t *, const double *, float *, float *, hipStream_t);
+
 // columns [0, nr) of an n-row block: y = x (different leading dimensions)
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy2d(int64_t n, const T *__restrict__ x, int64_t ldx,

@@ -47,7 +47,6 @@ struct pastix_data_s {
     // numerics
     pxa::HostBcsc bcsc;
     std::unique_ptr<pxa::Engine> engine;
-    std::vector<char> hostL, hostU;
     double fact_times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double *dvec = nullptr;  // 2n FP64 device vectors for refinement / error checks
     pxa::Engine *dvec_owner = nullptr;
@@ -67,14 +66,18 @@ struct pastix_data_s {
 struct Triplets {
     std::vector<int64_t> i, j;
     std::vector<double> v;
+    std::vector<int64_t> src;  // index of each triplet's value in the input
 };
 
 namespace pxa_api {
 double *device_pair(pastix_data_t *d);
 void err_print(const char *fn, const char *msg);
 int spm_triplets(const spmatrix_t *spm, Triplets &t, bool values);
+// esrc (optional): for every output entry the triplets summed into it,
+// entry e owning (*esrc)[(*eptr)[e] .. (*eptr)[e+1]) in summation order
 void triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
-                     std::vector<int32_t> &rows, std::vector<double> &vals, bool transpose);
+                     std::vector<int32_t> &rows, std::vector<double> &vals, bool transpose,
+                     std::vector<int64_t> *eptr = nullptr, std::vector<int64_t> *esrc = nullptr);
 // Batched CG refinement of n x R user-ordered FP64 blocks (see batch.cpp).
 // Device FP64 block of at least count entries, kept by the instance between
 // calls (grown on demand).
