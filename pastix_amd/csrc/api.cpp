@@ -763,7 +763,26 @@ extern "C" void pastixSetSchurUnknownList(pastix_data_t *, pastix_int_t, const p
 }
 extern "C" int pastixGetSchur(const pastix_data_t *, void *, pastix_int_t) { return PASTIX_ERR_NOTIMPLEMENTED; }
 extern "C" void pastixExpand(const pastix_data_t *, spmatrix_t *) {}
-extern "C" int pastixGetDiag(const pastix_data_t *, void *, pastix_int_t) { return PASTIX_ERR_NOTIMPLEMENTED; }
+// Diagonal of the factor, permuted order, at stride incD, in the instance's
+// arithmetic (reference sopalin/diag.c:55-102, coeftab_zgetdiag
+// sopalin/coeftab_z.c:462-492): L's diagonal for LL^t, U's for LU, D for
+// LDL^t.
+extern "C" int pastixGetDiag(const pastix_data_t *d, void *D, pastix_int_t incD) {
+    if (!d) { err_print("pastix_getDiag", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!D) { err_print("pastix_getDiag", "D parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (incD <= 0) { err_print("pastix_getDiag", "incD parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_NUMFACT) || !d->engine) {
+        err_print("pastix_getDiag", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    std::vector<double> dg(d->n);
+    if (d->engine->download_diag(dg.data())) return PASTIX_ERR_INTERNAL;
+    if (d->flttype == SpmFloat)
+        for (int64_t i = 0; i < d->n; i++) ((float *)D)[i * incD] = (float)dg[i];
+    else
+        for (int64_t i = 0; i < d->n; i++) ((double *)D)[i * incD] = dg[i];
+    return PASTIX_SUCCESS;
+}
 extern "C" void pastixGetOptions(int, char **, pastix_int_t *, double *, int *check, spm_driver_t *driver,
                                  char **filename) {
     if (check) *check = 1;

@@ -206,6 +206,19 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     return 0;
 }
 
+int Engine::download_diag(double *h) {
+    if (!d_diagout_) {
+        d_diagout_ = (double *)alloc(n_ * sizeof(double));
+        if (!d_diagout_) return -1;
+    }
+    if (prec_ == 2) launch_gather_diag<float>(n_, d_diagpos_, (const float *)d_L_, d_diagout_, stream_);
+    else launch_gather_diag<double>(n_, d_diagpos_, (const double *)d_L_, d_diagout_, stream_);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h, d_diagout_, n_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
 int Engine::download_coef(void *hL, void *hU) {
     const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
     if (hL) HIPCHK(hipMemcpyAsync(hL, d_L_, coefnbr_ * fsz, hipMemcpyDeviceToHost, stream_));

@@ -28,6 +28,8 @@ class Engine {
 
     // Coefficient tables (1D layout, factor precision) to the host.
     int download_coef(void *hL, void *hU);
+    // Diagonal of the factor's L panels (permuted order) to the host, FP64.
+    int download_diag(double *h);
     // Numerical factorisation over all levels; seconds = device time.
     int factorize(double thr, int64_t *npiv, int *negdiag, double *seconds);
     // Solve in place on a device block (factor precision, permuted order).
@@ -216,6 +218,7 @@ class Engine {
     int64_t csr_nnz_ = 0;
     bool have_matrix_ = false;
     int64_t *d_cmap_ = nullptr;
+    double *d_diagout_ = nullptr;
     bool have_fillmap_ = false;
     // diagonal positions: in the coefficient table (per permuted column) and
     // in the CSR values, with the original FP64 diagonal

@@ -108,6 +108,8 @@ void launch_solve_tasks(const DevSymbol &S, const SolveTask *tasks, const int32_
                         T *ybuf, int64_t ldy, T *x, int64_t ldx, T *z, int64_t ldz, int nrhs, const T *L, const T *U,
                         const T *w1, const T *wb, const int32_t *ydst, T *part, int maxwidth, int nwg,
                         hipStream_t st);
+// out[i] = coef[pos[i]] for i < n (the factor's diagonal, as FP64)
+template <typename T> void launch_gather_diag(int64_t n, const int64_t *pos, const T *coef, double *out, hipStream_t st);
 // Coefficient tables (zeroed beforehand) from the FP64 CSR values of the
 // permuted matrix: L[cmap[e]] for cmap[e] >= 0, U[-2 - cmap[e]] for
 // cmap[e] <= -2 (when U is given), nothing at -1.

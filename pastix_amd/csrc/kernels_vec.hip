@@ -304,6 +304,19 @@ void launch_set_csr_diag(int64_t n, const int64_t *pos, const double *d0, double
     if (n > 0) hipLaunchKernelGGL(k_set_csr_diag, dim3(nblk(n)), dim3(256), 0, st, n, pos, d0, delta, vals);
 }
 
+// out[i] = coef[pos[i]] (the factor's diagonal, widened to FP64)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_diag(int64_t n, const int64_t *__restrict__ pos,
+                                                     const T *__restrict__ coef, double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = (double)coef[pos[i]];
+}
+template <typename T> void launch_gather_diag(int64_t n, const int64_t *pos, const T *coef, double *out, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_diag<T>, dim3(nblk(n)), dim3(256), 0, st, n, pos, coef, out);
+}
+template void launch_gather_diag<double>(int64_t, const int64_t *, const double *, double *, hipStream_t);
+template void launch_gather_diag<float>(int64_t, const int64_t *, const float *, double *, hipStream_t);
+
 // Coefficient-table fill (reference kernels/cpucblk_zinit.c:229-309) from
 // the FP64 CSR values of the permuted matrix: entry e goes to L[cmap[e]]
 // when cmap[e] >= 0, to U[-2 - cmap[e]] when cmap[e] <= -2 (U given), and

@@ -146,6 +146,8 @@ lib.pastix_subtask_applyorder.restype = ct.c_int
 lib.pastix_subtask_trsm.argtypes = [_P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _I, ct.c_void_p, _I]
 lib.pastix_subtask_trsm.restype = ct.c_int
 lib.pastix_subtask_diag.argtypes = [_P, ct.c_int, _I, ct.c_void_p, _I]
+lib.pastixGetDiag.argtypes = [_P, ct.c_void_p, _I]
+lib.pastixGetDiag.restype = ct.c_int
 lib.pastix_subtask_diag.restype = ct.c_int
 lib.pastixOrderGrid.argtypes = [ct.POINTER(ct.POINTER(pastix_order_t)), _I, _I, _I]
 lib.pastixOrderGrid.restype = ct.c_int
@@ -282,6 +284,14 @@ def subtask_diag(pastix_data, b, nrhs=-1):
     nrhs = _getnrhs(nrhs, b)
     flt = _spm.SpmDouble if b.dtype == np.float64 else _spm.SpmFloat
     _check("pastix_subtask_diag", lib.pastix_subtask_diag(pastix_data.ptr, flt, nrhs, b.ctypes.data, b.shape[0]))
+
+
+def getDiag(pastix_data, n, incD=1, dtype=np.float64):
+    """Diagonal of the factor in the permuted ordering (pastixGetDiag):
+    entries D[0::incD] of an n*incD array of the instance's arithmetic."""
+    D = np.zeros(n * incD, dtype=dtype)
+    _check("pastixGetDiag", lib.pastixGetDiag(pastix_data.ptr, D.ctypes.data, incD))
+    return D
 
 
 def subtask_solve(pastix_data, b, nrhs=-1):

@@ -95,3 +95,27 @@ def test_subtask_trsm_rejects_other_sweeps():
     with pytest.raises(Exception):
         pastix.subtask_trsm(h, pastix.side.Right, L_, N_, NU, x)
     pastix.finalize(h)
+
+
+@pytest.mark.parametrize("facto", [F.LLT, F.LU, F.LDLT])
+def test_get_diag_matches_factor(facto):
+    """pastixGetDiag returns the diagonal of the L panels (reference
+    sopalin/coeftab_z.c:462-492) in the permuted ordering, at stride incD."""
+    A, h, S, sym = _case(facto)
+    L, _ = pastix.coeftab(h, lu=facto != F.LLT)
+    want = np.empty(sym["n"])
+    for c in range(sym["cblknbr"]):
+        f0, f1 = sym["cblk_fcol"][c], sym["cblk_fcol"][c + 1]
+        j = np.arange(f1 - f0)
+        want[f0:f1] = L[sym["cblk_coef"][c] + j * sym["cblk_stride"][c] + j]
+    D = pastix.getDiag(h, sym["n"], incD=3)
+    assert np.array_equal(D[0::3], want)
+    assert not D[1::3].any() and not D[2::3].any()
+    # the diagonal fixes the determinant: det A = prod(L_ii)^2 for LL^t,
+    # prod(U_ii) for LU, prod(D_i) for LDL^t
+    sign, logdet = np.linalg.slogdet(A.toarray())
+    ld = np.log(np.abs(want)).sum() * (2 if facto == F.LLT else 1)
+    assert abs(ld - logdet) < 1e-9 * abs(logdet)
+    if facto != F.LLT:
+        assert np.prod(np.sign(want)) == sign
+    pastix.finalize(h)
