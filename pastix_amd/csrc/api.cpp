@@ -168,6 +168,39 @@ int pxa_api::spm_triplets(const spmatrix_t *spm, Triplets &t, bool values) {
     return PASTIX_SUCCESS;
 }
 
+// Adjacency graph of the input's pattern (symmetrised, no diagonal): CSC
+// and CSR arrays are read in place after a bounds check (the graph of A and
+// of A^t is the same), IJV through triplets.
+int pxa_api::spm_graph(const spmatrix_t *spm, pxa::Graph &g) {
+    const int64_t n = spm->n, nnz = spm->nnz;
+    if (spm->dof != 1) return PASTIX_ERR_NOTIMPLEMENTED;
+    if (spm->flttype != SpmDouble && spm->flttype != SpmFloat && spm->flttype != SpmPattern)
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    if (spm->fmttype == SpmCSC || spm->fmttype == SpmCSR) {
+        const spm_int_t *ptr = spm->fmttype == SpmCSC ? spm->colptr : spm->rowptr;
+        const spm_int_t *idx = spm->fmttype == SpmCSC ? spm->rowptr : spm->colptr;
+        const int64_t base = ptr[0];
+        if (ptr[n] - base != nnz) return PASTIX_ERR_BADPARAMETER;
+        for (int64_t j = 0; j < n; j++)
+            if (ptr[j + 1] < ptr[j]) return PASTIX_ERR_BADPARAMETER;
+        for (int64_t k = 0; k < nnz; k++)
+            if (idx[k] < base || idx[k] - base >= n) return PASTIX_ERR_BADPARAMETER;
+        static_assert(sizeof(spm_int_t) == sizeof(int64_t), "64-bit spm indices");
+        g = pxa::graph_from_csc(n, (const int64_t *)ptr, (const int64_t *)idx, base);
+        return PASTIX_SUCCESS;
+    }
+    Triplets t;
+    const int rc = spm_triplets(spm, t, false);
+    if (rc) return rc;
+    std::vector<int64_t> cp;
+    std::vector<int32_t> rw;
+    std::vector<double> vv;
+    triplets_to_csc(n, t, nullptr, cp, rw, vv, false);
+    std::vector<int64_t> rows64(rw.begin(), rw.end());
+    g = pxa::graph_from_csc(n, cp.data(), rows64.data(), 0);
+    return PASTIX_SUCCESS;
+}
+
 // CSC (sorted, duplicates summed) of the n x n matrix given by triplets,
 // optionally permuted: B(perm[i], perm[j]) = A(i, j).
 void pxa_api::triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
@@ -185,12 +218,20 @@ void pxa_api::triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm,
         colptr[j + 1]++;
     }
     for (int64_t j = 0; j < n; j++) colptr[j + 1] += colptr[j];
-    // bucket by column, triplets in input order within a column
+    // bucket by column (one scattering pass; the triplets of a column keep
+    // their input order), then sort each column by (row, triplet index):
+    // duplicates are summed in input order, so that the sum is a fixed
+    // function of the input
+    struct Ent {
+        int64_t row, k;
+        double v;
+        bool operator<(const Ent &o) const { return row < o.row || (row == o.row && k < o.k); }
+    };
     std::vector<int64_t> pos(colptr.begin(), colptr.end() - 1);
-    std::vector<int64_t> tk(m);
-    for (size_t k = 0; k < m; k++) tk[pos[J[k]]++] = (int64_t)k;
-    // sort each column by (row, triplet index): duplicates are summed in
-    // input order, so that the sum is a fixed function of the input
+    std::vector<Ent> ent(m);
+    for (size_t k = 0; k < m; k++) ent[pos[J[k]]++] = Ent{I[k], (int64_t)k, t.v.empty() ? 1.0 : t.v[k]};
+    std::vector<int64_t>().swap(I);
+    std::vector<int64_t>().swap(J);
     rows.clear();
     vals.clear();
     rows.reserve(m);
@@ -204,20 +245,17 @@ void pxa_api::triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm,
         esrc->reserve(m);
     }
     std::vector<int64_t> newptr(n + 1, 0);
-    std::vector<std::pair<int64_t, int64_t>> col;
     for (int64_t j = 0; j < n; j++) {
-        col.clear();
-        for (int64_t p = colptr[j]; p < colptr[j + 1]; p++) col.emplace_back(I[tk[p]], tk[p]);
-        std::sort(col.begin(), col.end());
-        for (size_t q = 0; q < col.size(); q++) {
-            const double v = t.v.empty() ? 1.0 : t.v[col[q].second];
-            if (q > 0 && col[q].first == col[q - 1].first) vals.back() += v;
+        Ent *c0 = ent.data() + colptr[j], *c1 = ent.data() + colptr[j + 1];
+        if (!std::is_sorted(c0, c1)) std::sort(c0, c1);
+        for (const Ent *e = c0; e < c1; e++) {
+            if (e > c0 && e->row == e[-1].row) vals.back() += e->v;
             else {
                 if (eptr) eptr->push_back(esrc ? (int64_t)esrc->size() : 0);
-                rows.push_back((int32_t)col[q].first);
-                vals.push_back(v);
+                rows.push_back((int32_t)e->row);
+                vals.push_back(e->v);
             }
-            if (esrc) esrc->push_back(col[q].second);
+            if (esrc) esrc->push_back(e->k);
         }
         newptr[j + 1] = (int64_t)rows.size();
     }
@@ -232,23 +270,12 @@ extern "C" int pastix_subtask_order(pastix_data_t *d, const spmatrix_t *spm, pas
     if (!d) { err_print("pastix_subtask_order", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
     if (!spm) { err_print("pastix_subtask_order", "wrong spm parameter"); return PASTIX_ERR_BADPARAMETER; }
     if (!(d->steps & STEP_INIT)) { err_print("pastix_subtask_order", "pastixInit() has to be called before"); return PASTIX_ERR_BADPARAMETER; }
-    Triplets t;
-    int rc = spm_triplets(spm, t, false);
+    int rc = spm_graph(spm, d->graph);
     if (rc) return rc;
     const int64_t n = spm->n;
     d->n = n;
     d->mtxtype = spm->mtxtype;
     d->flttype = spm->flttype;
-    {
-        std::vector<int64_t> cp;
-        std::vector<int32_t> rw;
-        std::vector<double> vv;
-        Triplets tt = t;
-        tt.v.clear();
-        triplets_to_csc(n, tt, nullptr, cp, rw, vv, false);
-        std::vector<int64_t> rows64(rw.begin(), rw.end());
-        d->graph = pxa::graph_from_csc(n, cp.data(), rows64.data(), 0);
-    }
     pxa::Order o;
     o.n = n;
     const pastix_int_t ord = d->iparm[IPARM_ORDERING];

@@ -14,11 +14,63 @@ using namespace pxa_api;
 
 namespace {
 
+int publish_union(pastix_data_t *d, int mtx, int64_t N, const std::vector<int64_t> &cp,
+                  const std::vector<int32_t> &rows, std::vector<double> &vals);
+
+// The union by concatenation, when every system is a CSC whose rows
+// increase strictly within each column (lower-triangle entries only when
+// symmetric): that is already the union's sorted, duplicate-free CSC.
+bool concat_union(pastix_int_t nsys, const spmatrix_t *const *spms, bool values, std::vector<int64_t> &off,
+                  std::vector<int64_t> &cp, std::vector<int32_t> &rows, std::vector<double> &vals) {
+    const int mtx = spms[0]->mtxtype;
+    int64_t nnz = 0;
+    for (pastix_int_t s = 0; s < nsys; s++) {
+        const spmatrix_t *A = spms[s];
+        if (A->fmttype != SpmCSC) return false;
+        const spm_int_t *c = A->colptr, *r = A->rowptr;
+        const int64_t base = c[0];
+        for (int64_t j = 0; j < A->n; j++)
+            for (int64_t k = c[j] - base; k < c[j + 1] - base; k++) {
+                const int64_t i = r[k] - base;
+                if (i < 0 || i >= A->n || (k > c[j] - base && i <= r[k - 1] - base) || (mtx != SpmGeneral && i < j))
+                    return false;
+            }
+        nnz += c[A->n] - base;
+        off[s + 1] = off[s] + A->n;
+    }
+    cp.assign(1, 0);
+    cp.reserve(off[nsys] + 1);
+    rows.clear();
+    rows.reserve(nnz);
+    vals.clear();
+    vals.reserve(nnz);
+    for (pastix_int_t s = 0; s < nsys; s++) {
+        const spmatrix_t *A = spms[s];
+        const spm_int_t *c = A->colptr, *r = A->rowptr;
+        const int64_t base = c[0], e = c[A->n] - base, r0 = (int64_t)rows.size();
+        for (int64_t j = 0; j < A->n; j++) cp.push_back(r0 + c[j + 1] - base);
+        for (int64_t k = 0; k < e; k++) rows.push_back((int32_t)(r[k] - base + off[s]));
+        if (!values || A->flttype == SpmPattern || !A->values) vals.insert(vals.end(), e, 1.0);
+        else if (A->flttype == SpmDouble) vals.insert(vals.end(), (const double *)A->values, (const double *)A->values + e);
+        else for (int64_t k = 0; k < e; k++) vals.push_back((double)((const float *)A->values)[k]);
+    }
+    return true;
+}
+
 // Union CSC of the systems (lower triangle when symmetric, FP64 values).
 int build_union(pastix_data_t *d, pastix_int_t nsys, const spmatrix_t *const *spms, bool values,
                 std::vector<int64_t> &off) {
     off.assign(nsys + 1, 0);
     const int mtx = spms[0]->mtxtype;
+    for (pastix_int_t s = 0; s < nsys; s++) {
+        const spmatrix_t *A = spms[s];
+        if (!A || A->mtxtype != mtx || A->n <= 0 || A->dof != 1) return PASTIX_ERR_BADPARAMETER;
+        if (values && A->flttype != SpmDouble && A->flttype != SpmFloat) return PASTIX_ERR_BADPARAMETER;
+    }
+    std::vector<int64_t> cp;
+    std::vector<int32_t> rows;
+    std::vector<double> vals;
+    if (concat_union(nsys, spms, values, off, cp, rows, vals)) return publish_union(d, mtx, off[nsys], cp, rows, vals);
     Triplets u;
     for (pastix_int_t s = 0; s < nsys; s++) {
         const spmatrix_t *A = spms[s];
@@ -36,11 +88,13 @@ int build_union(pastix_data_t *d, pastix_int_t nsys, const spmatrix_t *const *sp
         off[s + 1] = off[s] + A->n;
     }
     const int64_t N = off[nsys];
-    std::vector<int64_t> cp;
-    std::vector<int32_t> rows;
-    std::vector<double> vals;
     triplets_to_csc(N, u, nullptr, cp, rows, vals, false);
     if (!values) vals.assign(rows.size(), 1.0);
+    return publish_union(d, mtx, N, cp, rows, vals);
+}
+
+int publish_union(pastix_data_t *d, int mtx, int64_t N, const std::vector<int64_t> &cp,
+                  const std::vector<int32_t> &rows, std::vector<double> &vals) {
     d->u_colptr.assign(cp.begin(), cp.end());
     d->u_rows.assign(rows.begin(), rows.end());
     d->u_vals.swap(vals);
@@ -83,15 +137,9 @@ int system_order(const spmatrix_t *A, const pastix_order_t *o, pxa::Order &out) 
             for (int64_t c = 0; c < out.cblknbr; c++) out.treetab[c] = o->treetab[c] < 0 ? -1 : o->treetab[c] - base;
         }
     } else {
-        Triplets t;
-        const int rc = spm_triplets(A, t, false);
+        pxa::Graph g;
+        const int rc = spm_graph(A, g);
         if (rc) return rc;
-        std::vector<int64_t> cp;
-        std::vector<int32_t> rw;
-        std::vector<double> vv;
-        triplets_to_csc(n, t, nullptr, cp, rw, vv, false);
-        std::vector<int64_t> rows64(rw.begin(), rw.end());
-        const pxa::Graph g = pxa::graph_from_csc(n, cp.data(), rows64.data(), 0);
         out = pxa::order_nested_dissection(g);
     }
     std::vector<char> seen(n, 0);

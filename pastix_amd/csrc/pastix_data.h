@@ -73,6 +73,8 @@ namespace pxa_api {
 double *device_pair(pastix_data_t *d);
 void err_print(const char *fn, const char *msg);
 int spm_triplets(const spmatrix_t *spm, Triplets &t, bool values);
+// adjacency graph of the input's pattern (symmetrised, without the diagonal)
+int spm_graph(const spmatrix_t *spm, pxa::Graph &g);
 // esrc (optional): for every output entry the triplets summed into it,
 // entry e owning (*esrc)[(*eptr)[e] .. (*eptr)[e+1]) in summation order
 void triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,

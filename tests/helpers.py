@@ -6,6 +6,38 @@ from pastix_amd import pastix, spm
 import oracle
 
 
+class RawSpm:
+    """An spmatrix_t over given index arrays: fmt SpmCSC (colptr = column
+    pointers, rowptr = rows), SpmCSR (rowptr = row pointers, colptr =
+    columns) or SpmIJV (colptr = columns, rowptr = rows; entries may repeat
+    and add up)."""
+
+    def __init__(self, fmt, colptr, rowptr, vals, n, symmetric=False):
+        import ctypes as ct
+        self.colptr = np.ascontiguousarray(colptr, dtype=np.int64)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        self.values = np.ascontiguousarray(vals, dtype=np.float64)
+        self.dtype = self.values.dtype
+        s = spm.spmatrix_t()
+        s.mtxtype = spm.SpmSymmetric if symmetric else spm.SpmGeneral
+        s.flttype = spm.SpmDouble
+        s.fmttype = fmt
+        s.gN = s.n = s.gNexp = s.nexp = n
+        s.gnnz = s.nnz = s.gnnzexp = s.nnzexp = len(self.values)
+        s.dof = 1
+        s.layout = spm.SpmColMajor
+        s.colptr = self.colptr.ctypes.data_as(ct.POINTER(spm.spm_int))
+        s.rowptr = self.rowptr.ctypes.data_as(ct.POINTER(spm.spm_int))
+        s.values = self.values.ctypes.data_as(ct.c_void_p)
+        self.s = s
+        self.n = n
+
+    @property
+    def id_ptr(self):
+        import ctypes as ct
+        return ct.byref(self.s)
+
+
 def analyse(A, symmetric, facto, grid=None, min_bs=160, max_bs=320, level_order=False):
     """Init + analysis; returns (handle, Spm, symbol dict, grid order or None)."""
     S = spm.Spm(A, symmetric=symmetric)

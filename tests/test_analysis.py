@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from pastix_amd import pastix, spm
-from tests.helpers import analyse
+from tests.helpers import RawSpm, analyse
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LLT = pastix.factotype.LLT
@@ -189,3 +189,41 @@ def test_ldlt_thflops_follow_reference_formulas():
             tot += M * N * (N + 1) + N * M + N * M * (M + 1) + M * M
     assert h.dparm[pastix.dparm.fact_thflops] == pytest.approx(tot, rel=1e-12)
     pastix.finalize(h)
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_same_analysis_for_every_input_format(symmetric):
+    """The adjacency graph is read from CSC and CSR arrays in place and from
+    IJV through triplets: the nested-dissection ordering and the partition
+    built on it must not depend on the format (nor on IJV entry order and
+    repeats)."""
+    import scipy.sparse as sps
+    A = spm.laplacian7(7, 6, 5, diag=6.0) if symmetric else spm.convection_diffusion27(6, 5, 5, seed=2)
+    n = A.shape[0]
+    T = sps.tril(A, format="csc") if symmetric else sps.csc_matrix(A)
+    T.sort_indices()
+    R = sps.csr_matrix(T)
+    R.sort_indices()
+    C = T.tocoo()
+    rng = np.random.default_rng(3)
+    rep = rng.permutation(C.nnz)[: C.nnz // 4]
+    order = rng.permutation(C.nnz + len(rep))
+    rows = np.concatenate([C.row, C.row[rep]])[order]
+    cols = np.concatenate([C.col, C.col[rep]])[order]
+    vals = np.concatenate([C.data, C.data[rep]])[order]
+    inputs = [RawSpm(spm.SpmCSC, T.indptr, T.indices, T.data, n, symmetric),
+              RawSpm(spm.SpmCSR, R.indices, R.indptr, R.data, n, symmetric),
+              RawSpm(spm.SpmIJV, cols, rows, vals, n, symmetric)]
+    syms = []
+    for S in inputs:
+        ip, dp = pastix.initParam()
+        ip[pastix.iparm.factorization] = LLT if symmetric else pastix.factotype.LU
+        ip[pastix.iparm.min_blocksize] = 8
+        ip[pastix.iparm.max_blocksize] = 24
+        h = pastix.init(ip, dp)
+        pastix.task_analyze(h, S)
+        syms.append(pastix.symbol(h))
+        pastix.finalize(h)
+    for sym in syms[1:]:
+        for k in ("permtab", "cblk_fcol", "cblk_bloknum", "blok_frow", "blok_lrow", "blok_fcblk"):
+            assert np.array_equal(sym[k], syms[0][k]), k

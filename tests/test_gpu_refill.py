@@ -8,14 +8,12 @@ gather, and the device fills the tables.  The factor must be the same, bit
 for bit, as the one of a fresh instance that reads the same matrix through
 the whole triplet path; an input whose pattern differs must be detected and
 take the whole path."""
-import ctypes as ct
-
 import numpy as np
 import pytest
 import scipy.sparse as sps
 
 from pastix_amd import pastix, spm
-from tests.helpers import analyse
+from tests.helpers import RawSpm, analyse
 
 pytestmark = pytest.mark.gpu
 
@@ -23,31 +21,11 @@ F = pastix.factotype
 GRID = (9, 8, 7)
 
 
-class IJV:
+class IJV(RawSpm):
     """An spmatrix_t in IJV format whose entries may repeat (repeats add)."""
 
     def __init__(self, rows, cols, vals, n, symmetric=False):
-        self.rows = np.ascontiguousarray(rows, dtype=np.int64)
-        self.cols = np.ascontiguousarray(cols, dtype=np.int64)
-        self.values = np.ascontiguousarray(vals, dtype=np.float64)
-        self.dtype = self.values.dtype
-        s = spm.spmatrix_t()
-        s.mtxtype = spm.SpmSymmetric if symmetric else spm.SpmGeneral
-        s.flttype = spm.SpmDouble
-        s.fmttype = spm.SpmIJV
-        s.gN = s.n = s.gNexp = s.nexp = n
-        s.gnnz = s.nnz = s.gnnzexp = s.nnzexp = len(self.values)
-        s.dof = 1
-        s.layout = spm.SpmColMajor
-        s.colptr = self.cols.ctypes.data_as(ct.POINTER(spm.spm_int))
-        s.rowptr = self.rows.ctypes.data_as(ct.POINTER(spm.spm_int))
-        s.values = self.values.ctypes.data_as(ct.c_void_p)
-        self.s = s
-        self.n = n
-
-    @property
-    def id_ptr(self):
-        return ct.byref(self.s)
+        super().__init__(spm.SpmIJV, cols, rows, vals, n, symmetric)
 
 
 def _perturbed(A, seed, symmetric):
