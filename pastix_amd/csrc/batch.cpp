@@ -157,27 +157,34 @@ int system_order(const spmatrix_t *A, const pastix_order_t *o, pxa::Order &out) 
 }
 
 // Column-major n x R block of the union from per-system blocks (FP64).
+// B (N x R, ld N): the systems' columns, zeros past each system's nrhs
 void pack(const std::vector<int64_t> &off, pastix_int_t R, const pastix_int_t *nrhs, const void *const *b,
-          const pastix_int_t *ldb, std::vector<double> &B) {
+          const pastix_int_t *ldb, double *B) {
     const int64_t N = off.back();
-    B.assign((size_t)N * R, 0.0);
     for (size_t s = 0; s + 1 < off.size(); s++) {
         const double *bs = (const double *)b[s];
         const int64_t ns = off[s + 1] - off[s];
-        for (pastix_int_t c = 0; c < nrhs[s]; c++)
-            for (int64_t i = 0; i < ns; i++) B[off[s] + i + (size_t)c * N] = bs[i + c * ldb[s]];
+        for (pastix_int_t c = 0; c < R; c++) {
+            double *col = B + off[s] + (size_t)c * N;
+            if (c < nrhs[s]) memcpy(col, bs + c * ldb[s], ns * sizeof(double));
+            else memset(col, 0, ns * sizeof(double));
+        }
     }
 }
 
-void unpack(const std::vector<int64_t> &off, const pastix_int_t *nrhs, const std::vector<double> &B,
-            void *const *b, const pastix_int_t *ldb) {
+void unpack(const std::vector<int64_t> &off, const pastix_int_t *nrhs, const double *B, void *const *b,
+            const pastix_int_t *ldb) {
     const int64_t N = off.back();
     for (size_t s = 0; s + 1 < off.size(); s++) {
         double *bs = (double *)b[s];
         const int64_t ns = off[s + 1] - off[s];
-        for (pastix_int_t c = 0; c < nrhs[s]; c++)
-            for (int64_t i = 0; i < ns; i++) bs[i + c * ldb[s]] = B[off[s] + i + (size_t)c * N];
+        for (pastix_int_t c = 0; c < nrhs[s]; c++) memcpy(bs + c * ldb[s], B + off[s] + (size_t)c * N, ns * sizeof(double));
     }
+}
+
+// pinned host staging for the two N x R blocks of a batched call
+double *staging(pastix_data_t *d, pastix_int_t R) {
+    return d->engine ? d->engine->host_block(2 * d->batch_off.back() * (int64_t)R) : nullptr;
 }
 
 bool rhs_ok(const pastix_data_t *d, const pastix_int_t *nrhs, const void *const *b, const pastix_int_t *ldb,
@@ -318,10 +325,12 @@ extern "C" int pastix_amd_batch_solve(pastix_data_t *d, const pastix_int_t *nrhs
     if (!rhs_ok(d, nrhs, b, ldb, R)) return PASTIX_ERR_BADPARAMETER;
     if (R == 0) return PASTIX_SUCCESS;
     if (!(d->steps & STEP_NUMFACT) || !d->engine) return PASTIX_ERR_BADPARAMETER;
-    std::vector<double> B, X(d->batch_off.back() * (size_t)R);
+    double *B = staging(d, R);
+    if (!B) return PASTIX_ERR_OUTOFMEMORY;
+    double *X = B + d->batch_off.back() * (size_t)R;
     pack(d->batch_off, R, nrhs, b, ldb, B);
     // one solve in the factor precision, permutations and conversions on the device
-    const int rc = refine_block(d, (int)R, B.data(), X.data(), d->batch_off, true, -1);
+    const int rc = refine_block(d, (int)R, B, X, d->batch_off, true, -1);
     if (rc) return rc;
     unpack(d->batch_off, nrhs, X, b, ldb);
     d->steps |= STEP_SOLVE;
@@ -340,10 +349,12 @@ extern "C" int pastix_amd_batch_refine(pastix_data_t *d, const pastix_int_t *nrh
     if (!rhs_ok(d, nrhs, b, ldb, R) || !rhs_ok(d, nrhs, x, ldx, Rx)) return PASTIX_ERR_BADPARAMETER;
     if (R == 0) return PASTIX_SUCCESS;
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<double> B, X;
+    double *B = staging(d, R);
+    if (!B) return PASTIX_ERR_OUTOFMEMORY;
+    double *X = B + d->batch_off.back() * (size_t)R;
     pack(d->batch_off, R, nrhs, b, ldb, B);
     pack(d->batch_off, R, nrhs, x, ldx, X);
-    const int rc = refine_block(d, (int)R, B.data(), X.data(), d->batch_off, false, 1);
+    const int rc = refine_block(d, (int)R, B, X, d->batch_off, false, 1);
     if (rc) return rc;
     unpack(d->batch_off, nrhs, X, x, ldx);
     d->dparm[DPARM_REFINE_TIME] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -362,9 +373,11 @@ extern "C" int pastix_amd_batch_solve_refine(pastix_data_t *d, const pastix_int_
     if (!rhs_ok(d, nrhs, b, ldb, R) || !rhs_ok(d, nrhs, x, ldx, Rx)) return PASTIX_ERR_BADPARAMETER;
     if (R == 0) return PASTIX_SUCCESS;
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<double> B, X(d->batch_off.back() * (size_t)R);
+    double *B = staging(d, R);
+    if (!B) return PASTIX_ERR_OUTOFMEMORY;
+    double *X = B + d->batch_off.back() * (size_t)R;
     pack(d->batch_off, R, nrhs, b, ldb, B);
-    const int rc = refine_block(d, (int)R, B.data(), X.data(), d->batch_off, true, 1);
+    const int rc = refine_block(d, (int)R, B, X, d->batch_off, true, 1);
     if (rc) return rc;
     unpack(d->batch_off, nrhs, X, x, ldx);
     d->steps |= STEP_SOLVE;

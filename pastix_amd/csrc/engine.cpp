@@ -25,6 +25,7 @@ Engine::~Engine() {
     if (fact_exec_) (void)hipGraphExecDestroy(fact_exec_);
     for (SolveGraph &g : solve_graphs_) (void)hipGraphExecDestroy(g.exec);
     for (void *p : owned_) (void)hipFree(p);
+    if (h_pin_) (void)hipHostFree(h_pin_);
     for (hipEvent_t e : pev_) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -204,6 +205,21 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     HIPCHK(hipStreamSynchronize(stream_));
     inited_ = true;
     return 0;
+}
+
+double *Engine::host_block(int64_t count) {
+    if (h_pin_cap_ < count) {
+        if (h_pin_) (void)hipHostFree(h_pin_);
+        h_pin_ = nullptr;
+        h_pin_cap_ = 0;
+        if (hipHostMalloc((void **)&h_pin_, (size_t)count * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+            h_pin_ = nullptr;
+            fail("hipHostMalloc of " + std::to_string(count * sizeof(double)) + " bytes failed");
+            return nullptr;
+        }
+        h_pin_cap_ = count;
+    }
+    return h_pin_;
 }
 
 int Engine::download_diag(double *h) {

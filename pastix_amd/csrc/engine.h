@@ -30,6 +30,9 @@ class Engine {
     int download_coef(void *hL, void *hU);
     // Diagonal of the factor's L panels (permuted order) to the host, FP64.
     int download_diag(double *h);
+    // Pinned host staging block of at least count doubles, kept and grown
+    // between calls (freed with the engine); nullptr when hipHostMalloc fails.
+    double *host_block(int64_t count);
     // Numerical factorisation over all levels; seconds = device time.
     int factorize(double thr, int64_t *npiv, int *negdiag, double *seconds);
     // Solve in place on a device block (factor precision, permuted order).
@@ -219,6 +222,8 @@ class Engine {
     bool have_matrix_ = false;
     int64_t *d_cmap_ = nullptr;
     double *d_diagout_ = nullptr;
+    double *h_pin_ = nullptr;
+    int64_t h_pin_cap_ = 0;
     bool have_fillmap_ = false;
     // diagonal positions: in the coefficient table (per permuted column) and
     // in the CSR values, with the original FP64 diagonal
