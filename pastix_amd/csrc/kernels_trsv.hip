@@ -261,22 +261,65 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
     const int Kc = S.cblk_fcol[c + 1] - fc;
     // the tiles of one cblk own consecutive partial slots, Kc apart
     const int64_t s0 = (it.b1 > it.b0) ? (int64_t)bitems[it.b0].slot : 0;
-    for (int kc = r0; kc < Kc; kc += 64) {
-        const int k = kc + r;
+    // y = z_c - the tiles' partials for columns [r0, Kc).  Several
+    // right-hand sides: 64 columns at a time, the waves splitting the tiles.
+    if constexpr (NR > 1) {
+        for (int kc = r0; kc < Kc; kc += 64) {
+            const int k = kc + r;
+            T acc[NR];
+#pragma unroll
+            for (int u = 0; u < NR; u++) acc[u] = 0;
+            if (k < Kc) {
+#pragma unroll 4
+                for (int j = it.b0 + q; j < it.b1; j += NW) {
+                    const T *pp = part + (s0 + (int64_t)(j - it.b0) * Kc) * nrhs_tot + (int64_t)u0 * Kc + k;
+#pragma unroll
+                    for (int u = 0; u < NR; u++)
+                        if (u < nrhs) acc[u] += ld_co<CO>(pp + u * Kc);
+                }
+            }
+            // y lives in LDS: a plain store; z comes from other tasks
+            wave_sum_store<CO, T, NR, NW>(acc, red, r, q, k < Kc, nrhs, y + k, Kc, z + fc + k, ldz);
+            __syncthreads();
+        }
+    }
+    // One right-hand side: all columns at once (cpp per pass), each summed
+    // over every L-th tile by L threads, whose sums are then added in thread
+    // order; one barrier pair per pass instead of one per 64 columns.
+    const int ncol = (NR > 1) ? 0 : Kc - r0;
+    const int cpp = max(1, min(ncol, NW * 64)), L = (NW * 64) / cpp;
+    for (int c0 = 0; c0 < ncol; c0 += cpp) {
+        const int cc = tid % cpp, h = tid / cpp;
+        const int k = r0 + c0 + cc;
+        const bool live = h < L && c0 + cc < ncol;
         T acc[NR];
 #pragma unroll
         for (int u = 0; u < NR; u++) acc[u] = 0;
-        if (k < Kc) {
+        if (live) {
 #pragma unroll 4
-            for (int j = it.b0 + q; j < it.b1; j += NW) {
+            for (int j = it.b0 + h; j < it.b1; j += L) {
                 const T *pp = part + (s0 + (int64_t)(j - it.b0) * Kc) * nrhs_tot + (int64_t)u0 * Kc + k;
 #pragma unroll
                 for (int u = 0; u < NR; u++)
                     if (u < nrhs) acc[u] += ld_co<CO>(pp + u * Kc);
             }
         }
-        // y lives in LDS: a plain store; z comes from other tasks
-        wave_sum_store<CO, T, NR, NW>(acc, red, r, q, k < Kc, nrhs, y + k, Kc, z + fc + k, ldz);
+        if (h < L) {
+#pragma unroll
+            for (int u = 0; u < NR; u++) red[(h * cpp + cc) * NR + u] = acc[u];
+        }
+        __syncthreads();
+        if (h == 0 && c0 + cc < ncol) {
+#pragma unroll
+            for (int u = 0; u < NR; u++) {
+                if (u < nrhs) {
+                    T v = red[cc * NR + u];
+                    for (int w = 1; w < L; w++) v += red[(w * cpp + cc) * NR + u];
+                    // y lives in LDS: a plain store; z comes from other tasks
+                    y[k + u * Kc] = ld_co<CO>(z + fc + k + u * ldz) - v;
+                }
+            }
+        }
         __syncthreads();
     }
     const int i = r0 + r;
