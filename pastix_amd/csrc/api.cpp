@@ -5,6 +5,10 @@
 // sopalin/pastix_task_sopalin.c:90-101,206-217,367-388), iparm/dparm inputs and
 // outputs, in-place right-hand sides, and PASTIX_ERR_* return codes.
 #include "pastix_data.h"
+#include <algorithm>
+#include <atomic>
+#include <functional>
+#include <thread>
 
 namespace pxa_api {
 double *device_pair(pastix_data_t *d) {
@@ -395,16 +399,39 @@ static void keep_pattern(const spmatrix_t *spm, pxa::HostBcsc &bc) {
     bc.in_rowptr.assign(spm->rowptr, spm->rowptr + nr);
 }
 
+// f(a, b) over [0, n) in up to 16 host threads (one call when n is small);
+// callers keep every element independent, so the result does not depend on
+// the split
+static void parallel_ranges(int64_t n, const std::function<void(int64_t, int64_t)> &f) {
+    const int64_t T = std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (T == 1 || n < (1 << 18)) {
+        f(0, n);
+        return;
+    }
+    const int64_t chunk = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (int64_t a = 0; a < n; a += chunk) th.emplace_back(f, a, std::min(n, a + chunk));
+    for (std::thread &t : th) t.join();
+}
+
+static bool same_array(const spm_int_t *a, const std::vector<int64_t> &b) {
+    std::atomic<bool> same{true};
+    parallel_ranges((int64_t)b.size(), [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; k++)
+            if (a[k] != b[k]) {
+                same = false;
+                return;
+            }
+    });
+    return same;
+}
+
 static bool same_pattern(const spmatrix_t *spm, const pxa::HostBcsc &bc) {
     if (bc.in_fmt < 0 || spm->fmttype != bc.in_fmt || spm->mtxtype != bc.in_mtx || spm->n != bc.in_n ||
         spm->nnz != bc.in_nnz || spm->dof != 1)
         return false;
     if (spm->flttype != SpmDouble && spm->flttype != SpmFloat && spm->flttype != SpmPattern) return false;
-    for (size_t k = 0; k < bc.in_colptr.size(); k++)
-        if (spm->colptr[k] != bc.in_colptr[k]) return false;
-    for (size_t k = 0; k < bc.in_rowptr.size(); k++)
-        if (spm->rowptr[k] != bc.in_rowptr[k]) return false;
-    return true;
+    return same_array(spm->colptr, bc.in_colptr) && same_array(spm->rowptr, bc.in_rowptr);
 }
 
 // The values the triplet path gives (spm_triplets + triplets_to_csc): the
@@ -420,12 +447,14 @@ static void refill_values(const spmatrix_t *spm, pxa::HostBcsc &bc) {
         return;
     }
     auto gather = [&](auto *vals) {
-        for (int64_t e = 0; e < m; e++) {
-            int64_t q = bc.sptr[e];
-            double v = (double)vals[bc.src[q]];
-            for (q++; q < bc.sptr[e + 1]; q++) v += (double)vals[bc.src[q]];
-            bc.tvals[e] = v;
-        }
+        parallel_ranges(m, [&](int64_t lo, int64_t hi) {
+            for (int64_t e = lo; e < hi; e++) {
+                int64_t q = bc.sptr[e];
+                double v = (double)vals[bc.src[q]];
+                for (q++; q < bc.sptr[e + 1]; q++) v += (double)vals[bc.src[q]];
+                bc.tvals[e] = v;
+            }
+        });
     };
     if (spm->flttype == SpmDouble) gather((const double *)spm->values);
     else gather((const float *)spm->values);

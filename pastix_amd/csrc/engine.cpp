@@ -718,6 +718,7 @@ int Engine::upload_matrix(const HostBcsc &bc) {
                 if (bc.cols[k] == i) cd[i] = k;
         diag_complete_ = std::find(cd.begin(), cd.end(), (int64_t)-1) == cd.end();
         d_csrdiag_ = upload(*this, cd);
+        csrdiag_ = cd;
         d_diag0_ = (double *)alloc(n_ * sizeof(double));
         if (!d_csrdiag_ || !d_diag0_) return -1;
         have_matrix_ = true;
@@ -726,8 +727,7 @@ int Engine::upload_matrix(const HostBcsc &bc) {
         std::vector<double> dg(n_, 0.0);
         diag_sum_ = 0;
         for (int64_t i = 0; i < n_; i++)
-            for (int64_t k = bc.rowptr[i]; k < bc.rowptr[i + 1]; k++)
-                if (bc.cols[k] == i) dg[i] = bc.tvals[k];
+            if (csrdiag_[i] >= 0) dg[i] = bc.tvals[csrdiag_[i]];
         for (int64_t i = 0; i < n_; i++) diag_sum_ += dg[i];
         HIPCHK(hipMemcpy(d_diag0_, dg.data(), n_ * sizeof(double), hipMemcpyHostToDevice));
     }

@@ -228,6 +228,7 @@ class Engine {
     // diagonal positions: in the coefficient table (per permuted column) and
     // in the CSR values, with the original FP64 diagonal
     int64_t *d_diagpos_ = nullptr, *d_csrdiag_ = nullptr;
+    std::vector<int64_t> csrdiag_;  // host copy of the CSR diagonal positions (-1: absent)
     double *d_diag0_ = nullptr;
     double diag_sum_ = 0;
     bool diag_complete_ = false;
