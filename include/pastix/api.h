@@ -193,6 +193,13 @@ enum pastix_order_e {
     PastixOrderParMetis
 };
 
+/* Threaded communication mode (IPARM_THREAD_COMM_MODE); the reference
+ * defines it only in MPI builds (include/pastix/api.h:312-315), values kept. */
+typedef enum pastix_threadmode_e {
+    PastixThreadMultiple = 1,
+    PastixThreadFunneled = 2
+} pastix_threadmode_t;
+
 typedef enum pastix_error_e {
     PASTIX_SUCCESS            = 0,
     PASTIX_ERR_UNKNOWN        = 1,

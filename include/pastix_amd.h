@@ -38,13 +38,7 @@ typedef enum pastix_amd_iparm_e {
      * HIP stream, beside the next level's panel steps (default 1; 0: one
      * stream, level after level).  Results are identical either way.       */
     PXA_IPARM_OVERLAP          = 6,
-    /* Solve with one launch of workgroups that take the diagonal blocks and
-     * product tiles of both sweeps from a queue in dependency order, waiting
-     * on device counters (1); default 0: one kernel launch per level and
-     * sweep step, which is faster on the 3D stencil matrices measured so
-     * far.  Both give the same solution to rounding.                        */
-    PXA_IPARM_TASK_SOLVE       = 7,
-    PXA_IPARM_SIZE             = 8
+    PXA_IPARM_SIZE             = 7
 } pastix_amd_iparm_t;
 
 /* Set / get an engine parameter.  Returns PASTIX_SUCCESS or
@@ -98,6 +92,32 @@ int pastix_amd_solve_refine_device(pastix_data_t *pastix_data, pastix_int_t nrhs
  * FP64 vectors in the user ordering, computed on the device. */
 int pastix_amd_backward_error(pastix_data_t *pastix_data, const double *b,
                               const double *x, double *berr);
+
+/* ---- Residual SpMV and vector kernels of the refinement ------------------
+ * The operations the reference's refinement drivers call through their
+ * z_solver table (refinement/z_refine_functions.c:115-148), on FP64 device
+ * vectors and on the instance's HIP stream; every call returns after its
+ * work is done.  Reductions use a fixed grid and order: bitwise reproducible.
+ *
+ * y = alpha op(A) x + beta y with A the permuted matrix P A P^T of the last
+ * numfact (x, y of length n in the permuted ordering): the reference's
+ * bcsc_zspmv (bcsc/bcsc_zspmv.c:625-646).  op(A) = A for PastixNoTrans, A^T
+ * for PastixTrans and PastixConjTrans (real arithmetic). */
+int pastix_amd_bcsc_spmv(pastix_data_t *pastix_data, pastix_trans_t trans, double alpha, const double *d_x,
+                         double beta, double *d_y);
+/* bvec_zdotc / bvec_znrm2 (scaled sum of squares) / bvec_zaxpy / bvec_zscal /
+ * bvec_zcopy / bvec_zgemv (bcsc/bvec_zcompute.c:67-120,245-280,384-420,
+ * 547-600,1168-1210) on vectors of length n (gemv: A is m x n column-major,
+ * y = alpha A x + beta y, y not read when beta = 0).  The scalar results go
+ * to host memory. */
+int pastix_amd_bvec_dot(pastix_data_t *pastix_data, pastix_int_t n, const double *d_x, const double *d_y,
+                        double *result);
+int pastix_amd_bvec_nrm2(pastix_data_t *pastix_data, pastix_int_t n, const double *d_x, double *result);
+int pastix_amd_bvec_axpy(pastix_data_t *pastix_data, pastix_int_t n, double alpha, const double *d_x, double *d_y);
+int pastix_amd_bvec_scal(pastix_data_t *pastix_data, pastix_int_t n, double alpha, double *d_x);
+int pastix_amd_bvec_copy(pastix_data_t *pastix_data, pastix_int_t n, const double *d_x, double *d_y);
+int pastix_amd_bvec_gemv(pastix_data_t *pastix_data, pastix_int_t m, pastix_int_t n, double alpha, const double *d_A,
+                         pastix_int_t lda, const double *d_x, double beta, double *d_y);
 
 /* Device time of the last numerical factorisation split by kernel family:
  * out[0]=total, [1]=panel steps (diagonal blocks, panel TRSM, trailing

@@ -6,7 +6,7 @@ package, as the checker and as the timed CPU baseline.  The product
 
 `Oracle` wraps liboracle.so (oracle.c: a plain-C restatement of the
 reference sequential supernodal factorisation, solve, SpMV, flop count and
-the CG, simple and BiCGSTAB refinement drivers) over the block structure exported by the engine's analysis.
+the CG, GMRES, simple and BiCGSTAB refinement drivers) over the block structure exported by the engine's analysis.
 `bcsc()` restates the reference permuted block-CSC construction
 (bcsc/bcsc_zinit.c:485-541) with scipy: B = P A P^T with its full pattern.
 """
@@ -48,6 +48,8 @@ def _lib():
     for f in (lib.ora_sr, lib.ora_bicgstab):
         f.argtypes = [P, ct.c_int, P, P, ct.c_int64, P, P, P, P, P, ct.c_int, ct.c_double, P]
         f.restype = ct.c_int
+    lib.ora_gmres.argtypes = [P, ct.c_int, P, P, ct.c_int64, P, P, P, P, P, ct.c_int, ct.c_int, ct.c_double, P]
+    lib.ora_gmres.restype = ct.c_int
     return lib
 
 
@@ -142,6 +144,18 @@ class Oracle:
     def bicgstab(self, L, U, bc, b, x, itermax=250, eps=1e-12):
         """Preconditioned BiCGSTAB (reference refinement/z_refine_bicgstab.c); returns (x, iterations, ||r||/||b||)."""
         return self._refine(self.lib.ora_bicgstab, L, U, bc, b, x, itermax, eps)
+
+    def gmres(self, L, U, bc, b, x, itermax=250, im=25, eps=1e-12):
+        """Right-preconditioned restarted GMRES (reference refinement/z_refine_gmres.c);
+        returns (x, steps, last residual estimate / ||b||)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        res = ct.c_double()
+        it = self.lib.ora_gmres(ct.byref(self.s), 1 if U is not None else 0, L.ctypes.data,
+                                U.ctypes.data if U is not None else None, self.n, bc["tcolptr"].ctypes.data,
+                                bc["trows"].ctypes.data, bc["tvals"].ctypes.data, b.ctypes.data, x.ctypes.data,
+                                itermax, im, eps, ct.byref(res))
+        return x, it, res.value
 
     # ---- helpers on the 1D panel layout ----
     def to_dense(self, L, lower_only=True):

@@ -17,7 +17,9 @@
  *   solve         kernels/solve_ztrsmsp.c:67-100,332-557, sopalin/sequential_ztrsm.c:34-103
  *   spmv          bcsc/bcsc_zspmv.c:55-202
  *   thflops       symbol/symbol_cost.c:53-88, symbol_cost_flops.c, common/flops.h
- *   cg / gmres    refinement/z_refine_grad.c:46-156, z_refine_gmres.c:46-352
+ *   cg            refinement/z_refine_grad.c:46-156
+ *   sr, bicgstab  refinement/z_refine_pivot.c:26-119, z_refine_bicgstab.c:25-185
+ *   gmres         refinement/z_refine_gmres.c:46-352
  *
  * Parity status of each function is recorded in the repository design notes;
  * every function here is pinned by a test in tests/ (dense factorisations,
@@ -469,4 +471,94 @@ int ora_bicgstab(const ora_symbol_t *s, int lu, const double *L, const double *U
     free(r);
     if (resid) *resid = rb;
     return it;
+}
+
+/* Right-preconditioned restarted GMRES, reference refinement/z_refine_gmres.c:46-352
+ * (Saad, Iterative Methods, 2nd ed., Alg. 9.5):
+ *   cycle: v_0 = b - A x with the current x, resid =
+ *          ||v_0||; leave when resid / ||b|| <= eps (:190-205); g_0 = resid;
+ *   step i: w_i = M^{-1} v_i, v_{i+1} = A w_i, then for j = 0..i in turn
+ *          h_{j,i} = (v_{i+1}, v_j) and v_{i+1} -= h_{j,i} v_j (:240-249);
+ *          h_{i+1,i} = ||v_{i+1}||, v_{i+1} /= h_{i+1,i} only when it exceeds
+ *          1e-50 (:252-260); the earlier rotations on column i (:263-273); the
+ *          new rotation from t = sqrt(h_ii^2 + h_{i+1,i}^2), t replaced by eps
+ *          when t <= eps (:281-289); g_{i+1} = -s_i g_i, g_i = c_i g_i;
+ *          resid = |g_{i+1}|; the step count grows; the cycle ends at i+1 = im,
+ *          resid / ||b|| <= eps or the step limit (:300-309);
+ *   end of cycle: y = H^{-1} g (upper triangular, i+1 columns), x += W y
+ *          (:338-348); leave when the step count reached itermax (:357-360).
+ * Returns the step count; resid gets the last resid / ||b||. */
+int ora_gmres(const ora_symbol_t *s, int lu, const double *L, const double *U, int64_t n, const int64_t *rowptr,
+              const int64_t *cols, const double *vals, const double *b, double *x, int itermax, int im, double eps,
+              double *resid) {
+    const int im1 = im + 1;
+    double *V = malloc((size_t)n * im1 * sizeof(double));
+    double *W = malloc((size_t)n * im * sizeof(double));
+    double *H = calloc((size_t)im * im1, sizeof(double));
+    double *cs = malloc(im * sizeof(double)), *sn = malloc(im * sizeof(double)), *g = malloc(im1 * sizeof(double));
+    if (!V || !W || !H || !cs || !sn || !g) return -1;
+    const double normb = sqrt(dot(n, b, b));
+    double res = 0.0, rb = 0.0;
+    int iters = 0;
+    for (;;) {
+        /* the reference tests ||x|| once, before the first cycle (:145,185),
+         * and with a zero initial guess would take r = b on every restart;
+         * the product is taken with the current x on every cycle (the same
+         * vector on the first cycle, since b - A 0 = b exactly) */
+        memcpy(V, b, n * sizeof(double));
+        ora_spmv(n, rowptr, cols, vals, -1.0, x, 1.0, V);
+        res = sqrt(dot(n, V, V));
+        rb = res / normb;
+        if (rb <= eps) break;
+        for (int64_t k = 0; k < n; k++) V[k] *= 1.0 / res;
+        g[0] = res;
+        int i = -1;
+        for (;;) {
+            i++;
+            double *Hi = H + (size_t)i * im1;
+            double *vi = V + (size_t)i * n, *wi = W + (size_t)i * n, *vn = V + (size_t)(i + 1) * n;
+            memcpy(wi, vi, n * sizeof(double));
+            ora_solve(s, lu, L, U, wi, n, 1);
+            ora_spmv(n, rowptr, cols, vals, 1.0, wi, 0.0, vn);
+            for (int j = 0; j <= i; j++) {
+                const double *vj = V + (size_t)j * n;
+                Hi[j] = dot(n, vn, vj);
+                for (int64_t k = 0; k < n; k++) vn[k] += -1.0 * Hi[j] * vj[k];
+            }
+            const double nrm = sqrt(dot(n, vn, vn));
+            Hi[i + 1] = nrm;
+            if (nrm > 1e-50)
+                for (int64_t k = 0; k < n; k++) vn[k] *= 1.0 / nrm;
+            for (int j = 0; j < i; j++) {
+                const double t = Hi[j];
+                Hi[j] = cs[j] * t + sn[j] * Hi[j + 1];
+                Hi[j + 1] = cs[j] * Hi[j + 1] - sn[j] * t;
+            }
+            double t = sqrt(Hi[i] * Hi[i] + Hi[i + 1] * Hi[i + 1]);
+            if (fabs(t) <= eps) t = eps;
+            cs[i] = Hi[i] / t;
+            sn[i] = Hi[i + 1] / t;
+            g[i + 1] = -sn[i] * g[i];
+            g[i] = cs[i] * g[i];
+            Hi[i] = cs[i] * Hi[i] + sn[i] * Hi[i + 1];
+            res = fabs(g[i + 1]);
+            rb = res / normb;
+            iters++;
+            if (i + 1 >= im || rb <= eps || iters >= itermax) break;
+        }
+        /* y = H^{-1} g: upper triangular (i+1) x (i+1), ld im1 (dtrsv) */
+        for (int k = i; k >= 0; k--) {
+            double v = g[k];
+            for (int l = k + 1; l <= i; l++) v -= H[k + (size_t)l * im1] * g[l];
+            g[k] = v / H[k + (size_t)k * im1];
+        }
+        for (int l = 0; l <= i; l++) {
+            const double *wl = W + (size_t)l * n;
+            for (int64_t k = 0; k < n; k++) x[k] += g[l] * wl[k];
+        }
+        if (iters >= itermax) break;
+    }
+    free(V); free(W); free(H); free(cs); free(sn); free(g);
+    if (resid) *resid = rb;
+    return iters;
 }

@@ -28,4 +28,7 @@ int    ora_sr(const ora_symbol_t *s, int lu, const double *L, const double *U, i
 int    ora_bicgstab(const ora_symbol_t *s, int lu, const double *L, const double *U, int64_t n,
                     const int64_t *rowptr, const int64_t *cols, const double *vals, const double *b, double *x,
                     int itermax, double eps, double *resid);
+int    ora_gmres(const ora_symbol_t *s, int lu, const double *L, const double *U, int64_t n,
+                 const int64_t *rowptr, const int64_t *cols, const double *vals, const double *b, double *x,
+                 int itermax, int im, double eps, double *resid);
 #endif

@@ -104,7 +104,6 @@ extern "C" void pastixInitWithAffinity(pastix_data_t **pastix_data, PASTIX_Comm,
     d->ext[PXA_IPARM_DEVICE] = 0;
     d->ext[PXA_IPARM_LEVEL_ORDER] = 0;
     d->ext[PXA_IPARM_OVERLAP] = 1;
-    d->ext[PXA_IPARM_TASK_SOLVE] = 0;
     d->steps = STEP_INIT;
     *pastix_data = d;
 }
@@ -278,6 +277,7 @@ extern "C" int pastix_subtask_order(pastix_data_t *d, const spmatrix_t *spm, pas
     if (rc) return rc;
     const int64_t n = spm->n;
     d->n = n;
+    d->nnz_in = spm->gnnzexp > 0 ? spm->gnnzexp : spm->nnz;
     d->mtxtype = spm->mtxtype;
     d->flttype = spm->flttype;
     pxa::Order o;
@@ -358,6 +358,9 @@ extern "C" int pastix_subtask_blend(pastix_data_t *d) {
     d->dparm[DPARM_FACT_THFLOPS] = pxa::symbol_thflops(d->symbol, ft);
     if (facto == PastixFactLU) d->iparm[IPARM_NNZEROS] *= 2;
     d->dparm[DPARM_SOLV_FLOPS] = (double)d->iparm[IPARM_NNZEROS];
+    // reference blend/pastix_subtask_blend.c:545: factor entries per entry of
+    // the input (gnnzexp: the stored entries of the user's matrix)
+    if (d->nnz_in > 0) d->dparm[DPARM_FILL_IN] = (double)d->iparm[IPARM_NNZEROS] / (double)d->nnz_in;
     d->iparm[IPARM_NNZEROS_BLOCK_LOCAL] = d->layout.coefnbr;
     d->dparm[DPARM_ANALYZE_TIME] = wall() - t0;
     d->steps |= STEP_ANALYSE;
@@ -554,7 +557,6 @@ static void engine_flags(pastix_data_t *d) {
     d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
     d->engine->use_graph = d->ext[PXA_IPARM_GRAPH] != 0;
     d->engine->overlap = d->ext[PXA_IPARM_OVERLAP] != 0;
-    d->engine->task_solve = d->ext[PXA_IPARM_TASK_SOLVE] != 0;
 }
 
 static int factor_prec(pastix_data_t *d) {
@@ -1038,6 +1040,60 @@ extern "C" int pastix_amd_solve_refine_device(pastix_data_t *d, pastix_int_t nrh
                                               pastix_int_t ldb, double *d_x, pastix_int_t ldx) {
     return refine_device(d, nrhs, d_b, ldb, d_x, ldx, true);
 }
+// ---- residual SpMV and refinement vector kernels ----------------------------
+static bool engine_ready(const pastix_data_t *d, const char *fn) {
+    if (!d || !d->engine || !d->engine->ready() || !(d->steps & STEP_BCSC2CTAB)) {
+        err_print(fn, "pastix_task_numfact() has to be called before");
+        return false;
+    }
+    return true;
+}
+
+extern "C" int pastix_amd_bcsc_spmv(pastix_data_t *d, pastix_trans_t trans, double alpha, const double *x,
+                                    double beta, double *y) {
+    if (!engine_ready(d, "pastix_amd_bcsc_spmv")) return PASTIX_ERR_BADPARAMETER;
+    if (!x || !y) return PASTIX_ERR_BADPARAMETER;
+    // A^T = A for the symmetric storage types
+    if (trans == PastixNoTrans || d->mtxtype == SpmSymmetric || d->mtxtype == SpmHermitian)
+        return d->engine->spmv(alpha, x, beta, y) || hipStreamSynchronize(d->engine->stream()) != hipSuccess
+                   ? PASTIX_ERR_INTERNAL
+                   : PASTIX_SUCCESS;
+    if (trans != PastixTrans && trans != PastixConjTrans) return PASTIX_ERR_BADPARAMETER;
+    if (!d->engine->has_transpose() && d->engine->upload_transpose(d->bcsc)) return PASTIX_ERR_INTERNAL;
+    if (d->engine->spmv_t(alpha, x, beta, y) || hipStreamSynchronize(d->engine->stream()) != hipSuccess)
+        return PASTIX_ERR_INTERNAL;
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_bvec_dot(pastix_data_t *d, pastix_int_t n, const double *x, const double *y,
+                                   double *result) {
+    if (!engine_ready(d, "pastix_amd_bvec_dot") || n < 0 || (n > 0 && (!x || !y)) || !result)
+        return PASTIX_ERR_BADPARAMETER;
+    return d->engine->vec_dot(n, x, y, result) ? PASTIX_ERR_INTERNAL : PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_bvec_nrm2(pastix_data_t *d, pastix_int_t n, const double *x, double *result) {
+    if (!engine_ready(d, "pastix_amd_bvec_nrm2") || n < 0 || (n > 0 && !x) || !result) return PASTIX_ERR_BADPARAMETER;
+    return d->engine->vec_nrm2(n, x, result) ? PASTIX_ERR_INTERNAL : PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_bvec_axpy(pastix_data_t *d, pastix_int_t n, double alpha, const double *x, double *y) {
+    if (!engine_ready(d, "pastix_amd_bvec_axpy") || n < 0 || (n > 0 && (!x || !y))) return PASTIX_ERR_BADPARAMETER;
+    return d->engine->vec_axpy(n, alpha, x, y) ? PASTIX_ERR_INTERNAL : PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_bvec_scal(pastix_data_t *d, pastix_int_t n, double alpha, double *x) {
+    if (!engine_ready(d, "pastix_amd_bvec_scal") || n < 0 || (n > 0 && !x)) return PASTIX_ERR_BADPARAMETER;
+    return d->engine->vec_scal(n, alpha, x) ? PASTIX_ERR_INTERNAL : PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_bvec_copy(pastix_data_t *d, pastix_int_t n, const double *x, double *y) {
+    if (!engine_ready(d, "pastix_amd_bvec_copy") || n < 0 || (n > 0 && (!x || !y))) return PASTIX_ERR_BADPARAMETER;
+    return d->engine->vec_copy(n, x, y) ? PASTIX_ERR_INTERNAL : PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_bvec_gemv(pastix_data_t *d, pastix_int_t m, pastix_int_t n, double alpha, const double *A,
+                                    pastix_int_t lda, const double *x, double beta, double *y) {
+    if (!engine_ready(d, "pastix_amd_bvec_gemv") || m < 0 || n < 0 || lda < std::max<pastix_int_t>(1, m) ||
+        (m > 0 && (!y || (n > 0 && (!A || !x)))))
+        return PASTIX_ERR_BADPARAMETER;
+    return d->engine->vec_gemv(m, n, alpha, A, lda, x, beta, y) ? PASTIX_ERR_INTERNAL : PASTIX_SUCCESS;
+}
+
 extern "C" int pastix_amd_flops(const pastix_data_t *d, double out[4]) {
     if (!d || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
     const bool lu = d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF || is_ldlt(d);

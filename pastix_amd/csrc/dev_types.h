@@ -67,14 +67,6 @@ struct BwdItem {
     int32_t c, r0, r1, slot;
 };
 
-// One task of the dependency-driven solve: code & 3 = kind (0 forward
-// diagonal block, 1 forward product tile, 2 backward product tile, 3 backward
-// diagonal block), code >> 2 = column block of a backward product tile; idx
-// indexes the diagonal, forward or backward item list.
-struct SolveTask {
-    int32_t code, idx;
-};
-
 // Diagonal step of the solve: row block [64*ib, 64*ib+64) of cblk c, whose
 // backward tiles are items [b0, b1).
 struct DiagItem {

@@ -24,7 +24,7 @@ Engine::~Engine() {
     if (side_) (void)hipStreamSynchronize(side_);
     if (fact_exec_) (void)hipGraphExecDestroy(fact_exec_);
     for (SolveGraph &g : solve_graphs_) (void)hipGraphExecDestroy(g.exec);
-    for (void *p : owned_) (void)hipFree(p);
+    for (const Owned &o : owned_) (void)hipFree(o.p);
     if (h_pin_) (void)hipHostFree(h_pin_);
     for (hipEvent_t e : pev_) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -48,9 +48,21 @@ void *Engine::alloc(size_t bytes) {
         fail("hipMalloc of " + std::to_string(bytes) + " bytes failed");
         return nullptr;
     }
-    owned_.push_back(p);
+    owned_.push_back(Owned{p, bytes});
     bytes_ += (int64_t)bytes;
     return p;
+}
+
+void Engine::release_ptr(void *p) {
+    if (!p) return;
+    for (size_t k = 0; k < owned_.size(); k++)
+        if (owned_[k].p == p) {
+            (void)hipStreamSynchronize(stream_);
+            (void)hipFree(p);
+            bytes_ -= (int64_t)owned_[k].bytes;
+            owned_.erase(owned_.begin() + (long)k);
+            return;
+        }
 }
 
 template <typename T> static T *upload(Engine &E, const std::vector<T> &v) {
@@ -145,17 +157,6 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
         }
         d_diagpos_ = upload(*this, dp);
     }
-    d_stasks_ = upload(*this, P.stasks);
-    n_stasks_ = (int)P.stasks.size();
-    d_schunk_ = upload(*this, P.schunk);
-    n_schunk_ = (int)P.schunk.size() - 1;
-    d_itgt_ptr_ = upload(*this, P.itgt_ptr);
-    d_itgt_ = upload(*this, P.itgt);
-    d_sneed_ = upload(*this, P.sneed);
-    d_scnt_ = (int32_t *)alloc((size_t)(4 * cblknbr_ + 2) * sizeof(int32_t));
-    if (!d_stasks_ || !d_itgt_ptr_ || !d_sneed_ || !d_scnt_) return -1;
-    HIPCHK(hipMemsetAsync(d_scnt_, 0, (size_t)(4 * cblknbr_ + 2) * sizeof(int32_t), stream_));
-    HIPCHK(hipDeviceGetAttribute(&n_cu_, hipDeviceAttributeMultiprocessorCount, dev_));
     d_wgrp_ = upload(*this, P.wgrp);
     d_wtiles_ = upload(*this, P.wtiles);
     lv_wt_ptr_ = P.lv_wt_ptr;
@@ -458,7 +459,7 @@ int Engine::enqueue_solve(void *dx, int64_t ldx, int nrhs, bool user) {
 
 hipGraphExec_t Engine::solve_graph(void *dx, int64_t ldx, int nrhs, bool user) {
     for (const SolveGraph &g : solve_graphs_)
-        if (g.ptr == dx && g.ld == ldx && g.nrhs == nrhs && g.user == user && g.tasks == task_solve) return g.exec;
+        if (g.ptr == dx && g.ld == ldx && g.nrhs == nrhs && g.user == user) return g.exec;
     if (solve_graphs_.size() >= 16) {
         (void)hipGraphExecDestroy(solve_graphs_.front().exec);
         solve_graphs_.erase(solve_graphs_.begin());
@@ -479,7 +480,7 @@ hipGraphExec_t Engine::solve_graph(void *dx, int64_t ldx, int nrhs, bool user) {
         fail(std::string("solve graph instantiation: ") + hipGetErrorString(ei));
         return nullptr;
     }
-    solve_graphs_.push_back(SolveGraph{dx, ldx, nrhs, user, task_solve, ex});
+    solve_graphs_.push_back(SolveGraph{dx, ldx, nrhs, user, ex});
     return ex;
 }
 
@@ -504,7 +505,6 @@ int Engine::solve_user_device(void *db, int64_t ldb, int nrhs, double *seconds) 
     }
     HIPCHK(hipEventRecord(ev1_, stream_));
     HIPCHK(hipEventSynchronize(ev1_));
-    if (check_solve()) return -1;
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
     if (seconds) *seconds = ms * 1e-3;
@@ -589,19 +589,10 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
     T *z = (T *)d_z_;
     const int64_t ldy = std::max<int64_t>(nyrows_, 1), ldz = std::max<int64_t>(n_, 1);
     const int nl = (int)lv_cblk_ptr_.size() - 1;
-    const int chunk = (task_solve && n_stasks_ > 0) ? std::min(nrhs_chunk_, 8) : nrhs_chunk_;
+    const int chunk = nrhs_chunk_;
     for (int r0 = 0; r0 < nrhs; r0 += chunk) {
         const int nr = std::min(chunk, nrhs - r0);
         T *xx = x + (int64_t)r0 * ldx;
-        if (task_solve && n_stasks_ > 0) {
-            // counters and queue head to zero (a kernel, so that graph
-            // replays clear them too); the error flag stays until read
-            launch_zero_int((int)(4 * cblknbr_ + 1), d_scnt_, stream_);
-            launch_solve_tasks<T>(S_, d_stasks_, d_schunk_, n_schunk_, d_diag_, d_fwd_, d_bwd_, d_itgt_ptr_, d_itgt_, d_sneed_,
-                                  d_scnt_, (int)cblknbr_, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, nr, L, U, W1, WB,
-                                  d_ydst_, part, maxwidth_, 8 * n_cu_, stream_);
-            continue;
-        }
         // partial sweeps (pastix_subtask_trsm / _diag): 1 forward only, 2
         // backward only, 3 backward after x *= D (LDL^t: L^{-t} = U^{-1} D),
         // 4 x /= diag(factor)
@@ -635,16 +626,6 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
     return 0;
 }
 
-int Engine::check_solve() {
-    if (!d_scnt_) return 0;
-    int32_t e = 0;
-    int32_t *flag = d_scnt_ + 4 * cblknbr_ + 1;
-    HIPCHK(hipMemcpy(&e, flag, sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (e == 0) return 0;
-    HIPCHK(hipMemset(flag, 0, sizeof(int32_t)));
-    return fail("solve: a dependency wait of the task queue timed out");
-}
-
 int Engine::solve_device(void *dx, int64_t ldx, int nrhs, double *seconds) {
     if (!inited_) return fail("engine not initialised");
     HIPCHK(hipEventRecord(ev0_, stream_));
@@ -658,7 +639,6 @@ int Engine::solve_device(void *dx, int64_t ldx, int nrhs, double *seconds) {
     }
     HIPCHK(hipEventRecord(ev1_, stream_));
     HIPCHK(hipEventSynchronize(ev1_));
-    if (check_solve()) return -1;
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
     if (seconds) *seconds = ms * 1e-3;
@@ -706,7 +686,20 @@ int Engine::solve_host(const int64_t *perm, int flttype, void *b, int64_t ldb, i
 
 int Engine::upload_matrix(const HostBcsc &bc) {
     if (!have_matrix_) {
+        // a new pattern: the buffers of the previous one are freed first, so
+        // that an instance whose input alternates between patterns keeps a
+        // constant footprint
+        release(d_rowptr_);
+        release(d_cols_);
+        release(d_vals_);
+        release(d_csrdiag_);
+        release(d_diag0_);
+        release(d_cmap_);
+        release(d_trowptr_);
+        release(d_tcols_);
+        release(d_tpos_);
         have_fillmap_ = false;
+        have_transpose_ = false;
         csr_nnz_ = (int64_t)bc.tvals.size();
         d_rowptr_ = upload(*this, bc.rowptr);
         d_cols_ = upload(*this, bc.cols);
@@ -723,6 +716,8 @@ int Engine::upload_matrix(const HostBcsc &bc) {
         if (!d_csrdiag_ || !d_diag0_) return -1;
         have_matrix_ = true;
     }
+    // the cached diagonal positions hold only for the uploaded pattern
+    if ((int64_t)bc.tvals.size() != csr_nnz_) return fail("matrix pattern changed without drop_matrix");
     {
         std::vector<double> dg(n_, 0.0);
         diag_sum_ = 0;
@@ -731,14 +726,38 @@ int Engine::upload_matrix(const HostBcsc &bc) {
         for (int64_t i = 0; i < n_; i++) diag_sum_ += dg[i];
         HIPCHK(hipMemcpy(d_diag0_, dg.data(), n_ * sizeof(double), hipMemcpyHostToDevice));
     }
-    if ((int64_t)bc.tvals.size() != csr_nnz_) return fail("matrix pattern changed without drop_matrix");
     HIPCHK(hipMemcpyAsync(d_vals_, bc.tvals.data(), bc.tvals.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
     return 0;
 }
 
+int Engine::upload_transpose(const HostBcsc &bc) {
+    if (!have_matrix_ || (int64_t)bc.tvals.size() != csr_nnz_) return fail("transpose needs the uploaded matrix");
+    release(d_trowptr_);
+    release(d_tcols_);
+    release(d_tpos_);
+    std::vector<int64_t> tp(n_ + 1, 0), pos(csr_nnz_);
+    std::vector<int32_t> tc(csr_nnz_);
+    for (int64_t k = 0; k < csr_nnz_; k++) tp[bc.cols[k] + 1]++;
+    for (int64_t j = 0; j < n_; j++) tp[j + 1] += tp[j];
+    std::vector<int64_t> next(tp.begin(), tp.end() - 1);
+    for (int64_t i = 0; i < n_; i++)  // rows in increasing order: columns of A^T sorted
+        for (int64_t k = bc.rowptr[i]; k < bc.rowptr[i + 1]; k++) {
+            const int64_t q = next[bc.cols[k]]++;
+            tc[q] = (int32_t)i;
+            pos[q] = k;
+        }
+    d_trowptr_ = upload(*this, tp);
+    d_tcols_ = upload(*this, tc);
+    d_tpos_ = upload(*this, pos);
+    if (!d_trowptr_ || !d_tcols_ || !d_tpos_) return -1;
+    have_transpose_ = true;
+    return 0;
+}
+
 int Engine::upload_fillmap(const std::vector<int64_t> &cmap) {
     if (!have_matrix_ || (int64_t)cmap.size() != csr_nnz_) return fail("fill map does not match the uploaded matrix");
+    release(d_cmap_);
     d_cmap_ = upload(*this, cmap);
     if (!d_cmap_) return -1;
     have_fillmap_ = true;
@@ -776,6 +795,51 @@ int Engine::shift_diagonal(double delta) {
 int Engine::spmv(double alpha, const double *x, double beta, double *y) {
     if (!have_matrix_) return fail("matrix not uploaded");
     launch_spmv<double>(n_, d_rowptr_, d_cols_, d_vals_, alpha, x, beta, y, stream_);
+    return 0;
+}
+
+int Engine::spmv_t(double alpha, const double *x, double beta, double *y) {
+    if (!have_matrix_ || !have_transpose_) return fail("transposed matrix not uploaded");
+    launch_spmv_t(n_, d_trowptr_, d_tcols_, d_tpos_, d_vals_, alpha, x, beta, y, stream_);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int Engine::vec_dot(int64_t n, const double *x, const double *y, double *out) {
+    launch_dot(n, x, y, d_red_, d_red_ + 512 * 32, stream_);
+    HIPCHK(hipMemcpyAsync(out, d_red_ + 512 * 32, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+int Engine::vec_nrm2(int64_t n, const double *x, double *out) {
+    launch_nrm2(n, x, d_red_, d_red_ + 512 * 32, stream_);
+    HIPCHK(hipMemcpyAsync(out, d_red_ + 512 * 32, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+int Engine::vec_axpy(int64_t n, double alpha, const double *x, double *y) {
+    launch_axpy(n, alpha, x, y, stream_);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+int Engine::vec_scal(int64_t n, double alpha, double *x) {
+    launch_scal(n, alpha, x, stream_);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+int Engine::vec_copy(int64_t n, const double *x, double *y) {
+    launch_copy(n, x, y, stream_);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+int Engine::vec_gemv(int64_t m, int64_t n, double alpha, const double *A, int64_t lda, const double *x, double beta,
+                     double *y) {
+    launch_gemv(m, n, alpha, A, lda, x, beta, y, stream_);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
     return 0;
 }
 
@@ -984,7 +1048,6 @@ int Engine::refine_user(const double *Bu, int64_t ldb, double *Xu, int64_t ldx, 
         for (int c = 0; c < R; c++) launch_copy(n_, PX + c * n_, Xu + c * ldx, stream_);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
-    if (check_solve()) return -1;
     if (niter) *niter = it;
     if (resid) *resid = res;
     return 0;
@@ -1138,94 +1201,91 @@ int Engine::refine_bicgstab(double *x, const double *b, int itermax, double eps,
     return 0;
 }
 
-// Right-preconditioned restarted GMRES with classical Gram-Schmidt and Givens
-// rotations (reference refinement/z_refine_gmres.c:46-352).
+// Right-preconditioned restarted GMRES, reference
+// refinement/z_refine_gmres.c:46-352 step for step: each cycle starts from the
+// true residual and ends when it is small enough (:190-205); each step
+// orthogonalises A M^{-1} v_i against v_0..v_i one after the other (dot, then
+// axpy: :240-249), normalises only above 1e-50 (:252-260), applies the earlier
+// Givens rotations and a new one whose denominator is raised to eps when it
+// falls below (:263-289); the cycle ends at im steps, on the residual estimate
+// or at itermax steps (:300-309), then x += W H^{-1} g (:338-348).  The
+// projections and their coefficients stay on the device; the step's column of
+// H crosses to the host once per step.  The reference takes the product A x
+// of a cycle only when ||x_0|| > 0, tested once (:145,185); here it is taken
+// with the current x on every cycle (the same first cycle, since b - A 0 = b).
 int Engine::refine_gmres(double *x, const double *b, int itermax, int im, double eps, int *niter, double *resid) {
-    const int64_t need = (int64_t)(2 * im + 3) * n_;
+    const int64_t need = (int64_t)(2 * im + 1) * n_ + (im + 2);
     if (vec_cap_ < need) {
         vec_cap_ = need;
         d_vec_ = (double *)alloc(vec_cap_ * sizeof(double));
         d_svec_ = (float *)alloc(n_ * sizeof(float));
         if (!d_vec_ || !d_svec_) return -1;
     }
-    double *V = d_vec_;                    // n x (im+1)
-    double *Wz = d_vec_ + (int64_t)(im + 1) * n_;  // n x im: preconditioned directions
-    double *w = Wz + (int64_t)im * n_;     // n
-    double *hd = d_red_ + 512 * 32 + 8;    // device h column
-    std::vector<double> H((size_t)(im + 1) * im, 0.0), cs(im), sn(im), g(im + 1);
+    double *V = d_vec_;                            // n x (im+1)
+    double *W = d_vec_ + (int64_t)(im + 1) * n_;   // n x im: M^{-1} v_i
+    double *hd = W + (int64_t)im * n_;             // im+2: column i of H, then y
+    const int im1 = im + 1;
+    std::vector<double> H((size_t)im1 * im, 0.0), cs(im), sn(im), g(im1), h(im1 + 1);
     const double normb = std::sqrt(dot(b, b));
-    int it = 0;
-    double rb = 1.0;
-    bool done = false;
-    while (!done) {
-        // r = b - A x
+    auto relres = [&](double r) { return (normb > 0) ? r / normb : r; };
+    int iters = 0;
+    double rb = 0;
+    for (;;) {
         launch_copy(n_, b, V, stream_);
         spmv(-1.0, x, 1.0, V);
-        double beta = std::sqrt(dot(V, V));
-        rb = (normb > 0) ? beta / normb : beta;
-        if (rb <= eps || it >= itermax) break;
-        launch_scal(n_, 1.0 / beta, V, stream_);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        int j = 0;
-        for (; j < im && it < itermax; j++) {
-            it++;
-            double *vj = V + (int64_t)j * n_;
-            double *zj = Wz + (int64_t)j * n_;
-            launch_copy(n_, vj, zj, stream_);
-            if (precond(zj)) return -1;
-            spmv(1.0, zj, 0.0, w);
-            // classical Gram-Schmidt: h = V^T w ; w -= V h
-            launch_gemv_t(n_, j + 1, V, n_, w, d_red_, hd, stream_);
-            std::vector<double> h(j + 2);
-            HIPCHK(hipMemcpyAsync(h.data(), hd, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        double res = std::sqrt(dot(V, V));
+        rb = relres(res);
+        if (rb <= eps) break;
+        launch_scal(n_, 1.0 / res, V, stream_);
+        g[0] = res;
+        int i = -1;
+        for (;;) {
+            i++;
+            double *Hi = H.data() + (size_t)i * im1;
+            double *vi = V + (int64_t)i * n_, *wi = W + (int64_t)i * n_, *vn = V + (int64_t)(i + 1) * n_;
+            launch_copy(n_, vi, wi, stream_);
+            if (precond(wi)) return -1;
+            spmv(1.0, wi, 0.0, vn);
+            for (int j = 0; j <= i; j++) {
+                const double *vj = V + (int64_t)j * n_;
+                launch_dot(n_, vn, vj, d_red_, hd + j, stream_);
+                launch_axpy_dev(n_, hd + j, -1.0, vj, vn, stream_);
+            }
+            launch_dot(n_, vn, vn, d_red_, hd + i + 1, stream_);
+            HIPCHK(hipMemcpyAsync(h.data(), hd, (i + 2) * sizeof(double), hipMemcpyDeviceToHost, stream_));
             HIPCHK(hipStreamSynchronize(stream_));
-            launch_gemv_n(n_, j + 1, V, n_, hd, -1.0, w, stream_);
-            h[j + 1] = std::sqrt(dot(w, w));
-            if (h[j + 1] > 0) {
-                launch_copy(n_, w, V + (int64_t)(j + 1) * n_, stream_);
-                launch_scal(n_, 1.0 / h[j + 1], V + (int64_t)(j + 1) * n_, stream_);
+            for (int j = 0; j <= i; j++) Hi[j] = h[j];
+            const double nrm = std::sqrt(h[i + 1]);
+            Hi[i + 1] = nrm;
+            if (nrm > 1e-50) launch_scal(n_, 1.0 / nrm, vn, stream_);
+            for (int j = 0; j < i; j++) {
+                const double t = Hi[j];
+                Hi[j] = cs[j] * t + sn[j] * Hi[j + 1];
+                Hi[j + 1] = cs[j] * Hi[j + 1] - sn[j] * t;
             }
-            for (int k = 0; k < j; k++) {
-                double t = cs[k] * h[k] + sn[k] * h[k + 1];
-                h[k + 1] = -sn[k] * h[k] + cs[k] * h[k + 1];
-                h[k] = t;
-            }
-            double den = std::sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
-            cs[j] = (den > 0) ? h[j] / den : 1.0;
-            sn[j] = (den > 0) ? h[j + 1] / den : 0.0;
-            h[j] = cs[j] * h[j] + sn[j] * h[j + 1];
-            h[j + 1] = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            for (int k = 0; k <= j; k++) H[(size_t)k + (size_t)j * (im + 1)] = h[k];
-            rb = (normb > 0) ? std::fabs(g[j + 1]) / normb : std::fabs(g[j + 1]);
-            if (rb <= eps) {
-                j++;
-                break;
-            }
+            double t = std::sqrt(Hi[i] * Hi[i] + Hi[i + 1] * Hi[i + 1]);
+            if (std::fabs(t) <= eps) t = eps;
+            cs[i] = Hi[i] / t;
+            sn[i] = Hi[i + 1] / t;
+            g[i + 1] = -sn[i] * g[i];
+            g[i] = cs[i] * g[i];
+            Hi[i] = cs[i] * Hi[i] + sn[i] * Hi[i + 1];
+            rb = relres(std::fabs(g[i + 1]));
+            iters++;
+            if (i + 1 >= im || rb <= eps || iters >= itermax) break;
         }
-        // y = H^{-1} g (upper triangular j x j), x += Wz y
-        std::vector<double> y(j);
-        for (int k = j - 1; k >= 0; k--) {
-            double s = g[k];
-            for (int l = k + 1; l < j; l++) s -= H[(size_t)k + (size_t)l * (im + 1)] * y[l];
-            y[k] = s / H[(size_t)k + (size_t)k * (im + 1)];
+        // y = H^{-1} g (upper triangular, i+1 columns, ld im+1), x += W y
+        for (int k = i; k >= 0; k--) {
+            double v = g[k];
+            for (int l = k + 1; l <= i; l++) v -= H[(size_t)k + (size_t)l * im1] * g[l];
+            g[k] = v / H[(size_t)k + (size_t)k * im1];
         }
-        if (j > 0) {
-            HIPCHK(hipMemcpyAsync(hd, y.data(), j * sizeof(double), hipMemcpyHostToDevice, stream_));
-            launch_gemv_n(n_, j, Wz, n_, hd, 1.0, x, stream_);
-        }
-        if (rb <= eps || it >= itermax) {
-            launch_copy(n_, b, w, stream_);
-            spmv(-1.0, x, 1.0, w);
-            double nr = std::sqrt(dot(w, w));
-            rb = (normb > 0) ? nr / normb : nr;
-            done = true;
-        }
+        HIPCHK(hipMemcpyAsync(hd, g.data(), (i + 1) * sizeof(double), hipMemcpyHostToDevice, stream_));
+        launch_gemv_n(n_, i + 1, W, n_, hd, 1.0, x, stream_);
+        if (iters >= itermax) break;
     }
     HIPCHK(hipStreamSynchronize(stream_));
-    if (niter) *niter = it;
+    if (niter) *niter = iters;
     if (resid) *resid = rb;
     return 0;
 }

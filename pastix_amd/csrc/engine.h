@@ -48,7 +48,11 @@ class Engine {
     int upload_matrix(const HostBcsc &bc);
     // Forget the uploaded CSR pattern and fill map (the input's pattern
     // changed); the next upload_matrix sends them again.
-    void drop_matrix() { have_matrix_ = false; have_fillmap_ = false; }
+    void drop_matrix() {
+        have_matrix_ = false;
+        have_fillmap_ = false;
+        have_transpose_ = false;
+    }
     // Coefficient tables filled on the device from the uploaded CSR values:
     // cmap gives per CSR entry its position in the L table (>= 0), in the U
     // table (-2 - position) or -1 (outside the symbol).
@@ -97,7 +101,21 @@ class Engine {
     double shift() const { return shift_; }
     // y = alpha*A*x + beta*y on FP64 device vectors
     int spmv(double alpha, const double *x, double beta, double *y);
+    // y = alpha*A^T*x + beta*y; needs upload_transpose() for the current pattern
+    int spmv_t(double alpha, const double *x, double beta, double *y);
+    bool has_transpose() const { return have_transpose_; }
+    // transposed pattern of the uploaded CSR (reads the host copy once per pattern)
+    int upload_transpose(const HostBcsc &bc);
     double dot(const double *x, const double *y);
+    // BLAS-1 / BLAS-2 on FP64 device vectors of any length, on the engine's
+    // stream; results of the reductions to the host
+    int vec_dot(int64_t n, const double *x, const double *y, double *out);
+    int vec_nrm2(int64_t n, const double *x, double *out);
+    int vec_axpy(int64_t n, double alpha, const double *x, double *y);
+    int vec_scal(int64_t n, double alpha, double *x);
+    int vec_copy(int64_t n, const double *x, double *y);
+    int vec_gemv(int64_t m, int64_t n, double alpha, const double *A, int64_t lda, const double *x, double beta,
+                 double *y);
 
     // Keep a device copy of the freshly filled coefficients so that the
     // factorisation can be re-run from HBM-resident input (restore_coef).
@@ -120,10 +138,6 @@ class Engine {
     // Overlap each level's updates into later levels with the next level's
     // panel steps (second stream); off: one stream, level by level.
     bool overlap = true;
-    // Run both solve sweeps as one launch of workgroups that take tasks in
-    // dependency order and wait on device counters; off: one kernel launch
-    // per level and sweep step (the default: faster on the 3D stencils).
-    bool task_solve = false;
     double kt[KC_N] = {0, 0, 0, 0, 0};
     int kn[KC_N] = {0, 0, 0, 0, 0};
     double t_diag = 0, t_upd = 0;
@@ -141,8 +155,18 @@ class Engine {
     // for on a second stream (joined back before the level after)
     hipStream_t side_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev_panel_ = nullptr, ev_side_ = nullptr;
-    std::vector<void *> owned_;
+    struct Owned {
+        void *p;
+        size_t bytes;
+    };
+    std::vector<Owned> owned_;
     int64_t bytes_ = 0;
+    // frees a buffer from alloc() (no-op on nullptr) and sets it to nullptr
+    template <typename P> void release(P *&p) {
+        release_ptr((void *)p);
+        p = nullptr;
+    }
+    void release_ptr(void *p);
     std::string err_;
 
     int64_t n_ = 0, cblknbr_ = 0, coefnbr_ = 0, winvnbr_ = 0;
@@ -191,7 +215,7 @@ class Engine {
         const void *ptr;
         int64_t ld;
         int nrhs;
-        bool user, tasks;
+        bool user;
         hipGraphExec_t exec;
     };
     std::vector<SolveGraph> solve_graphs_;
@@ -220,6 +244,10 @@ class Engine {
     double *d_vals_ = nullptr;
     int64_t csr_nnz_ = 0;
     bool have_matrix_ = false;
+    // transposed pattern: row pointers, columns and CSR positions of A^T
+    int64_t *d_trowptr_ = nullptr, *d_tpos_ = nullptr;
+    int32_t *d_tcols_ = nullptr;
+    bool have_transpose_ = false;
     int64_t *d_cmap_ = nullptr;
     double *d_diagout_ = nullptr;
     double *h_pin_ = nullptr;
@@ -244,16 +272,7 @@ class Engine {
     double *d_sr_ = nullptr;  // permuted right-hand sides and solutions of refine_user
     int64_t sr_cap_ = 0;
     int precond_block(double *zb, int R);
-    // dependency-driven solve: tasks, product-tile targets, expected counts,
-    // counters (4*cblknbr + 2: the last one is a sticky error flag)
-    SolveTask *d_stasks_ = nullptr;
-    int n_stasks_ = 0, n_cu_ = 256;
     int part_ = 0;  // solve part of the next solve_t (0: whole solve)
-    int32_t *d_schunk_ = nullptr;
-    int n_schunk_ = 0;
-    int32_t *d_itgt_ptr_ = nullptr, *d_itgt_ = nullptr, *d_sneed_ = nullptr, *d_scnt_ = nullptr;
-    // reads and clears the error flag of the dependency-driven solve
-    int check_solve();
     int32_t *d_seg_ = nullptr;
     int64_t *d_segoff_ = nullptr;
     // chunks of the segmented dot products: offsets per segment, then the

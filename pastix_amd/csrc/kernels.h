@@ -77,6 +77,14 @@ void launch_axpy(int64_t n, double alpha, const double *x, double *y, hipStream_
 void launch_axpy_dev(int64_t n, const double *alpha, double sign, const double *x, double *y, hipStream_t st);
 void launch_scal(int64_t n, double alpha, double *x, hipStream_t st);
 void launch_copy(int64_t n, const double *x, double *y, hipStream_t st);
+// out[0] = ||x||_2 by scaled sums of squares (partials: 2 * 512 doubles)
+void launch_nrm2(int64_t n, const double *x, double *partials, double *out, hipStream_t st);
+// y = alpha A x + beta y, A m x n column-major
+void launch_gemv(int64_t m, int64_t n, double alpha, const double *A, int64_t lda, const double *x, double beta,
+                 double *y, hipStream_t st);
+// y = alpha A^T x + beta y from the transposed pattern of the CSR of A
+void launch_spmv_t(int64_t n, const int64_t *trowptr, const int32_t *tcols, const int64_t *tpos, const double *vals,
+                   double alpha, const double *x, double beta, double *y, hipStream_t st);
 // Segmented BLAS-1 over an n x R block whose rows are split into nseg
 // consecutive segments (off: nseg+1 offsets, seg: segment of every row);
 // per-(segment, column) scalars are stored at [s + c*nseg].
@@ -97,17 +105,6 @@ template <typename T>
 void launch_add_diag(int64_t n, const int64_t *pos, double delta, T *coef, hipStream_t st);
 void launch_set_csr_diag(int64_t n, const int64_t *pos, const double *d0, double delta, double *vals, hipStream_t st);
 // Zeroes n ints with a kernel (usable inside a captured graph).
-// Both sweeps of the solve in one launch of nwg workgroups taking the tasks
-// in list order, a chunk [chunks[k], chunks[k+1]) at a time and waiting on device counters (cnt: 4*cb+2 ints, zeroed
-// before the launch; cnt[4*cb+1] != 0 afterwards reports a wait that timed
-// out).  need, tgt_ptr/tgt: expected counts and target cblks of the plan.
-template <typename T>
-void launch_solve_tasks(const DevSymbol &S, const SolveTask *tasks, const int32_t *chunks, int nchunks,
-                        const DiagItem *diag, const FwdItem *fwd, const BwdItem *bwd, const int32_t *tgt_ptr,
-                        const int32_t *tgt, const int32_t *need, int32_t *cnt, int cb, const int64_t *yrow_ptr,
-                        T *ybuf, int64_t ldy, T *x, int64_t ldx, T *z, int64_t ldz, int nrhs, const T *L, const T *U,
-                        const T *w1, const T *wb, const int32_t *ydst, T *part, int maxwidth, int nwg,
-                        hipStream_t st);
 // out[i] = coef[pos[i]] for i < n (the factor's diagonal, as FP64)
 template <typename T> void launch_gather_diag(int64_t n, const int64_t *pos, const T *coef, double *out, hipStream_t st);
 // Coefficient tables (zeroed beforehand) from the FP64 CSR values of the

@@ -416,137 +416,6 @@ __global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagIte
                                     nrhs, u0, part, wb, reinterpret_cast<T *>(smem_raw), red);
 }
 
-// --------------------- dependency-driven solve (one launch) -----------------
-// Workgroups take tasks from a queue in the order of the task list (forward
-// levels upward, then backward levels downward) and wait, spinning on
-// device counters, for the tasks theirs depends on:
-//   0 forward diagonal block (c, ib): every forward product tile facing c;
-//   1 forward product tile of c: every forward diagonal block of c;
-//   2 backward product tile (item, column block): every backward diagonal
-//     block of the cblks its rows belong to;
-//   3 backward diagonal block of c: every backward product tile of c and
-//     every forward diagonal block of c.
-// A task only waits for tasks placed before it in the list, which running
-// workgroups have already taken, so the queue always drains.  A wait longer
-// than SOLVE_SPIN_LIMIT polls raises the error flag, and every workgroup then
-// leaves: the launch always ends.  The data handed from task to task is
-// written and read with device-coherent accesses, and a task signals only
-// after its stores are complete, so a consumer that sees the counter reads
-// the producer's values.  Counters: cnt[0..cb) forward products
-// received, [cb..2cb) forward diagonal blocks done, [2cb..3cb) backward
-// products done, [3cb..4cb) backward diagonal blocks done, cnt[4cb] queue
-// head, cnt[4cb+1] error flag; need[0..cb) products facing c, [cb..2cb)
-// diagonal blocks of c, [2cb..3cb) backward product tiles of c.
-constexpr int SOLVE_SPIN_LIMIT = 1 << 20;
-
-__device__ __forceinline__ bool solve_wait(const int *c, int need, int *err) {
-    if (need <= 0) return true;
-    int spins = 0;
-    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
-        if (++spins > SOLVE_SPIN_LIMIT) {
-            __hip_atomic_fetch_add(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
-    return true;
-}
-
-template <typename T, int NR>
-__global__ __launch_bounds__(256) void k_solve_tasks(DevSymbol S, const SolveTask *__restrict__ tasks,
-                                                     const int32_t *__restrict__ chunks, int nchunks,
-                                                     const DiagItem *__restrict__ diag,
-                                                     const FwdItem *__restrict__ fwd, const BwdItem *__restrict__ bwd,
-                                                     const int32_t *__restrict__ tgt_ptr,
-                                                     const int32_t *__restrict__ tgt, const int32_t *__restrict__ need,
-                                                     int32_t *cnt, int cb, const int64_t *__restrict__ yrow_ptr,
-                                                     T *ybuf, int64_t ldy, T *x, int64_t ldx, T *z, int64_t ldz,
-                                                     int nrhs, const T *__restrict__ L, const T *__restrict__ U,
-                                                     const T *__restrict__ w1, const T *__restrict__ wb,
-                                                     const int32_t *__restrict__ ydst, T *part) {
-    // dynamic LDS: the y / z staging of the diagonal and forward tasks or the
-    // 64x65 tile of a backward product task
-    extern __shared__ char smem_raw[];
-    T *ysh = reinterpret_cast<T *>(smem_raw);
-    T(*tile)[65] = reinterpret_cast<T(*)[65]>(smem_raw);
-    __shared__ T red[4 * 64 * NR];
-    __shared__ T xr[64 * NR];
-    __shared__ int sh_ok, sh_c;
-    int *fwd_in = cnt, *fdiag = cnt + cb, *bprod = cnt + 2 * cb, *bdiag = cnt + 3 * cb;
-    int *head = cnt + 4 * cb, *err = head + 1;
-    const int tid = threadIdx.x;
-    int t = 0, tend = 0;  // tasks [t, tend) of the chunk in hand
-    for (;;) {
-        if (t == tend) {
-            // next chunk of the queue
-            if (tid == 0) sh_c = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __syncthreads();
-            const int ck = sh_c;
-            if (ck >= nchunks) return;
-            t = chunks[ck];
-            tend = chunks[ck + 1];
-        }
-        if (tid == 0) {
-            bool ok = true;
-            {
-                const SolveTask tk = tasks[t];
-                const int type = tk.code & 3;
-                if (type == 0) {
-                    const int c = diag[tk.idx].c;
-                    ok = solve_wait(fwd_in + c, need[c], err);
-                } else if (type == 1) {
-                    const int c = fwd[tk.idx].c;
-                    ok = solve_wait(fdiag + c, need[cb + c], err);
-                } else if (type == 2) {
-                    for (int k = tgt_ptr[tk.idx]; ok && k < tgt_ptr[tk.idx + 1]; k++) {
-                        const int f = tgt[k];
-                        ok = solve_wait(bdiag + f, need[cb + f], err);
-                    }
-                } else {
-                    const int c = diag[tk.idx].c;
-                    ok = solve_wait(fdiag + c, need[cb + c], err) && solve_wait(bprod + c, need[2 * cb + c], err);
-                }
-            }
-            sh_ok = ok ? 1 : 0;
-        }
-        __syncthreads();
-        if (!sh_ok) return;
-        const SolveTask tk = tasks[t];
-        const int type = tk.code & 3;
-        int sig = -1;
-        if (type == 0) {
-            const DiagItem it = diag[tk.idx];
-            fwd_diag_body<true, T, NR, 4>(S, it, yrow_ptr, ybuf, ldy, x, ldx, z, ldz, nrhs, w1, ysh, red);
-            sig = cb + it.c;
-        } else if (type == 1) {
-            const FwdItem it = fwd[tk.idx];
-            fwd_prod_body<true, T, NR, 4>(S, it, L, z, ldz, nrhs, ydst, ybuf, ldy, ysh, red);
-        } else if (type == 2) {
-            const BwdItem it = bwd[tk.idx];
-            bwd_prod_body<true, T, NR>(S, it, (tk.code >> 2) * 64, U, x, ldx, nrhs, nrhs, 0, part, tile, xr, red);
-            sig = 2 * cb + it.c;
-        } else {
-            const DiagItem it = diag[tk.idx];
-            bwd_diag_body<true, T, NR, 4>(S, it, bwd, z, ldz, x, ldx, nrhs, nrhs, 0, part, wb, ysh, red);
-            sig = 3 * cb + it.c;
-        }
-        // the task's hand-over data went out with device-coherent stores:
-        // once every wave has them complete, one thread signals
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (tid == 0) {
-            if (type == 1) {
-                for (int k = tgt_ptr[tk.idx]; k < tgt_ptr[tk.idx + 1]; k++)
-                    __hip_atomic_fetch_add(fwd_in + tgt[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                __hip_atomic_fetch_add(cnt + sig, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        t++;
-    }
-}
 
 // ------------------------------- launchers ---------------------------------
 template <typename T>
@@ -635,25 +504,6 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
     }
 }
 
-template <typename T>
-void launch_solve_tasks(const DevSymbol &S, const SolveTask *tasks, const int32_t *chunks, int nchunks,
-                        const DiagItem *diag, const FwdItem *fwd, const BwdItem *bwd, const int32_t *tgt_ptr,
-                        const int32_t *tgt, const int32_t *need, int32_t *cnt, int cb, const int64_t *yrow_ptr,
-                        T *ybuf, int64_t ldy, T *x, int64_t ldx, T *z, int64_t ldz, int nrhs, const T *L, const T *U,
-                        const T *w1, const T *wb, const int32_t *ydst, T *part, int maxwidth, int nwg,
-                        hipStream_t st) {
-    if (nchunks <= 0) return;
-    const int g = std::max(1, std::min(nwg, nchunks));
-    // the right-hand sides of one launch times the widest cblk stay within
-    // the level kernels' 24 KiB bound, so this stays below 33 KiB
-    const size_t sh = (size_t)std::max(maxwidth * nrhs, 64 * 65) * sizeof(T);
-    if (nrhs == 1)
-        hipLaunchKernelGGL((k_solve_tasks<T, 1>), dim3(g), dim3(256), sh, st, S, tasks, chunks, nchunks, diag, fwd, bwd, tgt_ptr,
-                           tgt, need, cnt, cb, yrow_ptr, ybuf, ldy, x, ldx, z, ldz, nrhs, L, U, w1, wb, ydst, part);
-    else
-        hipLaunchKernelGGL((k_solve_tasks<T, 8>), dim3(g), dim3(256), sh, st, S, tasks, chunks, nchunks, diag, fwd, bwd, tgt_ptr,
-                           tgt, need, cnt, cb, yrow_ptr, ybuf, ldy, x, ldx, z, ldz, nrhs, L, U, w1, wb, ydst, part);
-}
 
 #define INST(T)                                                                                                   \
     template void launch_fwd_diag<T>(const DevSymbol &, const DiagItem *, int, const int64_t *, const T *,       \
@@ -668,14 +518,5 @@ void launch_solve_tasks(const DevSymbol &S, const SolveTask *tasks, const int32_
 INST(double)
 INST(float)
 
-#define INST_TASKS(T)                                                                                             \
-    template void launch_solve_tasks<T>(const DevSymbol &, const SolveTask *, const int32_t *, int,              \
-                                        const DiagItem *,                                                        \
-                                        const FwdItem *, const BwdItem *, const int32_t *, const int32_t *,      \
-                                        const int32_t *, int32_t *, int, const int64_t *, T *, int64_t, T *,     \
-                                        int64_t, T *, int64_t, int, const T *, const T *, const T *, const T *,  \
-                                        const int32_t *, T *, int, int, hipStream_t);
-INST_TASKS(double)
-INST_TASKS(float)
 
 }  // namespace pxa

@@ -161,6 +161,90 @@ __global__ __launch_bounds__(256) void k_permute(int64_t m, int nrhs, const int3
     }
 }
 
+// ---- reference-style BLAS-1 / BLAS-2 (bcsc/bvec_zcompute.c) -------------------
+// Euclidean norm by scaled sums of squares (frobenius_update, the LAPACK
+// dlassq rule used by bvec_znrm2_seq :67-120): a pair (scale, ssq) stands for
+// scale^2 * ssq; pairs merge by rescaling to the larger scale.  Each thread
+// walks its strided entries in order, lanes and waves merge in a fixed
+// butterfly and index order, then the block pairs in block order.
+__device__ __forceinline__ void ssq_add(double &scale, double &ssq, double v) {
+    const double a = fabs(v);
+    if (a == 0.0) return;
+    if (scale < a) {
+        const double r = scale / a;
+        ssq = 1.0 + ssq * r * r;
+        scale = a;
+    } else {
+        const double r = a / scale;
+        ssq += r * r;
+    }
+}
+__device__ __forceinline__ void ssq_merge(double &scale, double &ssq, double s2, double q2) {
+    if (s2 == 0.0) return;
+    if (scale < s2) {
+        const double r = scale / s2;
+        ssq = q2 + ssq * r * r;
+        scale = s2;
+    } else {
+        const double r = s2 / scale;
+        ssq += q2 * r * r;
+    }
+}
+__global__ __launch_bounds__(256) void k_nrm2_part(int64_t n, const double *__restrict__ x,
+                                                   double *__restrict__ partials) {
+    __shared__ double sh[2][4];
+    double scale = 0.0, ssq = 1.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        ssq_add(scale, ssq, x[i]);
+    for (int o = 32; o > 0; o >>= 1) {
+        const double s2 = __shfl_xor(scale, o, 64), q2 = __shfl_xor(ssq, o, 64);
+        ssq_merge(scale, ssq, s2, q2);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][wave] = scale;
+        sh[1][wave] = ssq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0, q = 1.0;
+        for (int w = 0; w < 4; w++) ssq_merge(s, q, sh[0][w], sh[1][w]);
+        partials[2 * blockIdx.x] = s;
+        partials[2 * blockIdx.x + 1] = q;
+    }
+}
+__global__ __launch_bounds__(64) void k_nrm2_final(int nparts, const double *__restrict__ partials,
+                                                   double *__restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0, q = 1.0;
+    for (int b = 0; b < nparts; b++) ssq_merge(s, q, partials[2 * b], partials[2 * b + 1]);
+    out[0] = s * sqrt(q);
+}
+// y = alpha A x + beta y, A m x n column-major (bvec_zgemv_seq :1168-1210);
+// y is not read when beta = 0
+__global__ __launch_bounds__(256) void k_gemv(int64_t m, int64_t n, double alpha, const double *__restrict__ A,
+                                              int64_t lda, const double *__restrict__ x, double beta,
+                                              double *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    double s = 0;
+    for (int64_t j = 0; j < n; j++) s += A[i + j * lda] * x[j];
+    y[i] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[i];
+}
+// y = alpha A^T x + beta y through the transposed pattern of the CSR: row j of
+// A^T holds the entries vals[tpos[k]], k in [trowptr[j], trowptr[j+1]), of
+// column j of A, at rows tcols[k] (increasing)
+__global__ __launch_bounds__(256) void k_spmv_t(int64_t n, const int64_t *__restrict__ trowptr,
+                                                const int32_t *__restrict__ tcols, const int64_t *__restrict__ tpos,
+                                                const double *__restrict__ vals, double alpha,
+                                                const double *__restrict__ x, double beta, double *__restrict__ y) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double s = 0;
+    for (int64_t k = trowptr[j]; k < trowptr[j + 1]; k++) s += vals[tpos[k]] * x[tcols[k]];
+    y[j] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[j];
+}
+
 // ------------------------------- launchers ---------------------------------
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
@@ -401,6 +485,20 @@ void launch_gemv_t(int64_t n, int m, const double *V, int64_t ldv, const double 
 void launch_gemv_n(int64_t n, int m, const double *V, int64_t ldv, const double *y, double alpha, double *x,
                    hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_gemv_n, dim3(nblk(n)), dim3(256), 0, st, n, m, V, ldv, y, alpha, x);
+}
+void launch_nrm2(int64_t n, const double *x, double *partials, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_nrm2_part, dim3(RED_BLOCKS), dim3(256), 0, st, n, x, partials);
+    hipLaunchKernelGGL(k_nrm2_final, dim3(1), dim3(64), 0, st, RED_BLOCKS, partials, out);
+}
+void launch_gemv(int64_t m, int64_t n, double alpha, const double *A, int64_t lda, const double *x, double beta,
+                 double *y, hipStream_t st) {
+    if (m > 0) hipLaunchKernelGGL(k_gemv, dim3(nblk(m)), dim3(256), 0, st, m, n, alpha, A, lda, x, beta, y);
+}
+void launch_spmv_t(int64_t n, const int64_t *trowptr, const int32_t *tcols, const int64_t *tpos, const double *vals,
+                   double alpha, const double *x, double beta, double *y, hipStream_t st) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_spmv_t, dim3(nblk(n)), dim3(256), 0, st, n, trowptr, tcols, tpos, vals, alpha, x, beta,
+                           y);
 }
 void launch_berr_parts(int64_t n, const int64_t *colptr, const int32_t *rows, const double *vals, const double *x,
                        const double *b, double *partials, double *out, hipStream_t st) {

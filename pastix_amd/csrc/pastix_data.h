@@ -41,6 +41,7 @@ struct pastix_data_s {
     pxa::Layout layout;
     pxa::Plan plan;
     int64_t n = 0;
+    int64_t nnz_in = 0;  // stored entries of the analysed input (DPARM_FILL_IN)
     int mtxtype = SpmGeneral;
     int flttype = SpmDouble;
     const spmatrix_t *csc = nullptr;

@@ -25,108 +25,6 @@ struct KeyEnt {
 };
 }  // namespace
 
-// Task list, dependency counts and row targets of the dependency-driven
-// solve.  The forward and backward product tiles are the same 64-row pieces
-// of the off-diagonal panels, in the same order.
-static void build_solve_tasks(Plan &P, const Symbol &s, const Layout &L) {
-    const int64_t cb = s.cblknbr();
-    const int32_t nl = P.nlevels;
-    std::vector<int32_t> cblk_of(s.n);
-    for (int64_t c = 0; c < cb; c++)
-        for (int64_t g = s.cblk_fcol[c]; g < s.cblk_fcol[c + 1]; g++) cblk_of[g] = (int32_t)c;
-    P.sneed.assign(3 * cb, 0);
-    P.itgt_ptr.assign(P.fwd.size() + 1, 0);
-    P.itgt.clear();
-    for (size_t i = 0; i < P.fwd.size(); i++) {
-        const FwdItem &f = P.fwd[i];
-        const BwdItem &b = P.bwd[i];
-        if (b.c != f.c || b.r0 != f.r0 || b.r1 != f.r1) throw std::runtime_error("solve plan: tile lists differ");
-        const int64_t rb = P.rowbase[f.c];
-        int32_t last = -1;
-        for (int32_t r = f.r0; r < f.r1; r++) {
-            const int32_t t = cblk_of[P.panel_grow[rb + r]];
-            if (t != last) {  // rows of one tile are sorted: targets come in runs
-                P.itgt.push_back(t);
-                P.sneed[t]++;
-                last = t;
-            }
-        }
-        P.itgt_ptr[i + 1] = (int32_t)P.itgt.size();
-    }
-    for (int64_t c = 0; c < cb; c++) {
-        const int64_t K = s.cblk_fcol[c + 1] - s.cblk_fcol[c];
-        P.sneed[cb + c] = (int32_t)((K + 63) / 64);
-    }
-    for (const BwdItem &b : P.bwd) {
-        const int64_t K = s.cblk_fcol[b.c + 1] - s.cblk_fcol[b.c];
-        P.sneed[2 * cb + b.c] += (int32_t)((K + 63) / 64);
-    }
-    P.stasks.clear();
-    for (int32_t lv = 0; lv < nl; lv++) {
-        for (int32_t d = P.lv_diag_ptr[lv]; d < P.lv_diag_ptr[lv + 1]; d++) P.stasks.push_back(SolveTask{0, d});
-        for (int32_t f = P.lv_fwd_ptr[lv]; f < P.lv_fwd_ptr[lv + 1]; f++) P.stasks.push_back(SolveTask{1, f});
-    }
-    for (int32_t lv = nl - 1; lv >= 0; lv--) {
-        for (int32_t b = P.lv_bwd_ptr[lv]; b < P.lv_bwd_ptr[lv + 1]; b++) {
-            const int64_t K = s.cblk_fcol[P.bwd[b].c + 1] - s.cblk_fcol[P.bwd[b].c];
-            for (int32_t kb = 0; kb * 64 < K; kb++) P.stasks.push_back(SolveTask{2 | (kb << 2), b});
-        }
-        for (int32_t d = P.lv_diag_ptr[lv]; d < P.lv_diag_ptr[lv + 1]; d++) P.stasks.push_back(SolveTask{3, d});
-    }
-    // Chunks: each run of tasks of one kind at one level is cut into pieces
-    // of up to 32 tasks (about 512 pieces per run), taken from the queue as a
-    // whole and done in order by one workgroup; short runs keep one task per
-    // piece.
-    P.schunk.assign(1, 0);
-    for (size_t t = 0; t < P.stasks.size();) {
-        size_t e = t;
-        auto key = [&](size_t i) {
-            const SolveTask &k = P.stasks[i];
-            const int kind = k.code & 3;
-            const int32_t c = (kind == 1) ? P.fwd[k.idx].c : (kind == 2) ? P.bwd[k.idx].c : P.diag[k.idx].c;
-            return std::make_pair(kind, L.cblk_level[c]);
-        };
-        const auto k0 = key(t);
-        while (e < P.stasks.size() && key(e) == k0) e++;
-        const size_t run = e - t;
-        const size_t g = std::max<size_t>(1, std::min<size_t>(32, run / 512));
-        for (size_t a = t; a < e; a += g) P.schunk.push_back((int32_t)std::min(e, a + g));
-        t = e;
-    }
-    // Replay the queue in list order on the host: every wait must already be
-    // met by the tasks before it (then the device queue always drains), and
-    // every counter must end at its expected count.
-    std::vector<int32_t> cnt(4 * cb, 0);
-    auto fail = [](const char *m) { throw std::runtime_error(std::string("solve plan: ") + m); };
-    for (const SolveTask &t : P.stasks) {
-        const int kind = t.code & 3;
-        if (kind == 0) {
-            const int32_t c = P.diag[t.idx].c;
-            if (cnt[c] < P.sneed[c]) fail("forward diagonal block before its products");
-            cnt[cb + c]++;
-        } else if (kind == 1) {
-            const int32_t c = P.fwd[t.idx].c;
-            if (cnt[cb + c] < P.sneed[cb + c]) fail("forward product before its diagonal blocks");
-            for (int32_t k = P.itgt_ptr[t.idx]; k < P.itgt_ptr[t.idx + 1]; k++) cnt[P.itgt[k]]++;
-        } else if (kind == 2) {
-            for (int32_t k = P.itgt_ptr[t.idx]; k < P.itgt_ptr[t.idx + 1]; k++) {
-                const int32_t f = P.itgt[k];
-                if (cnt[3 * cb + f] < P.sneed[cb + f]) fail("backward product before its targets' solutions");
-            }
-            cnt[2 * cb + P.bwd[t.idx].c]++;
-        } else {
-            const int32_t c = P.diag[t.idx].c;
-            if (cnt[cb + c] < P.sneed[cb + c] || cnt[2 * cb + c] < P.sneed[2 * cb + c])
-                fail("backward diagonal block before its inputs");
-            cnt[3 * cb + c]++;
-        }
-    }
-    for (int64_t c = 0; c < cb; c++)
-        if (cnt[c] != P.sneed[c] || cnt[cb + c] != P.sneed[cb + c] || cnt[2 * cb + c] != P.sneed[2 * cb + c] ||
-            cnt[3 * cb + c] != P.sneed[cb + c])
-            fail("counts differ from the expected ones");
-}
-
 Plan build_plan(const Symbol &s, const Layout &L) {
     const int64_t cb = s.cblknbr();
     Plan P;
@@ -486,7 +384,6 @@ Plan build_plan(const Symbol &s, const Layout &L) {
             P.lv_bwd_ptr[lv + 1] = (int32_t)P.bwd.size();
         }
     }
-    build_solve_tasks(P, s, L);
     return P;
 }
 

@@ -75,13 +75,6 @@ struct Plan {
     std::vector<int32_t> lv_diag_ptr;
     std::vector<DiagItem> diag;
     int64_t bwd_part = 0;  // partial buffer size per right-hand side (slots of all levels)
-    // dependency-driven solve: tasks in list order, target cblks of every
-    // product tile ([itgt_ptr[i], itgt_ptr[i+1]) for forward/backward item
-    // i), and per cblk the forward product tiles facing it, its diagonal
-    // blocks and its backward product tiles (3 x cblknbr)
-    std::vector<SolveTask> stasks;
-    std::vector<int32_t> schunk;         // queue pieces: tasks [schunk[k], schunk[k+1])
-    std::vector<int32_t> itgt_ptr, itgt, sneed;
     // panel row maps and inverse offsets
     std::vector<int32_t> panel_grow, panel_blk;
     std::vector<int64_t> rowbase, winv_off;

@@ -55,6 +55,7 @@ struct pastix_amd_server_s {
     std::deque<Req *> queue;
     std::unordered_map<int64_t, std::unique_ptr<Req>> reqs;
     std::map<int64_t, int64_t> live_solves;  // version -> queued or running solves
+    std::map<int64_t, int> failed;            // version -> status of its failed refactorisation
     int64_t next_id = 1, latest = 0;
     std::vector<int64_t> held;
     std::vector<char> ready;
@@ -132,6 +133,25 @@ void worker(pastix_amd_server_s *S, int k) {
                     S->queue.erase(it);
                     break;
                 }
+            if (rc != 0) {
+                // no factor of this version exists: its solves end with the
+                // refactorisation's status, later submissions bound to it
+                // too, and the slot is free again
+                S->failed[ref->version] = rc;
+                for (auto it = S->queue.begin(); it != S->queue.end();) {
+                    Req *r = *it;
+                    if (r->kind == 0 && r->state == 0 && r->version == ref->version) {
+                        r->rc = rc;
+                        r->state = 2;
+                        r->t_done = ref->t_done;
+                        S->live_solves[r->version]--;
+                        it = S->queue.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+                S->held[k] = -1;
+            }
             S->cv.notify_all();
             continue;
         }
@@ -211,8 +231,18 @@ extern "C" int pastix_amd_server_submit_solve(pastix_amd_server_t *S, pastix_int
     std::lock_guard<std::mutex> g(S->mu);
     r->id = S->next_id++;
     r->version = S->latest;
-    S->live_solves[r->version]++;
     *ticket = r->id;
+    const auto f = S->failed.find(r->version);
+    if (f != S->failed.end()) {
+        // the newest version failed to factor: the request ends at once
+        r->rc = f->second;
+        r->state = 2;
+        r->t_done = Clock::now();
+        S->reqs.emplace(r->id, std::move(r));
+        S->cv.notify_all();
+        return PASTIX_SUCCESS;
+    }
+    S->live_solves[r->version]++;
     S->queue.push_back(r.get());
     S->reqs.emplace(r->id, std::move(r));
     S->cv.notify_all();

@@ -94,7 +94,7 @@ class error:
 
 
 class amd_iparm:
-    factor_precision, device, level_order, resident_input, profile, graph, overlap, task_solve = 0, 1, 2, 3, 4, 5, 6, 7
+    factor_precision, device, level_order, resident_input, profile, graph, overlap = 0, 1, 2, 3, 4, 5, 6
 
 
 class PastixError(RuntimeError):
@@ -170,10 +170,9 @@ def _check(fn, rc):
         raise PastixError(fn, rc)
 
 
-def _getnrhs(nrhs, x):
-    if nrhs == -1:
-        return 1 if x.ndim == 1 else x.shape[1]
-    return nrhs
+def _columns(x, nrhs):
+    """Right-hand sides to process: all columns of x unless nrhs >= 0 says otherwise."""
+    return nrhs if nrhs >= 0 else (x.shape[1] if x.ndim > 1 else 1)
 
 
 # ---------------------------------------------------------- main API ----
@@ -219,16 +218,16 @@ def task_numfact(pastix_data, spmA):
 
 def task_solve(pastix_data, spmA, x, nrhs=-1):
     n = x.shape[0]
-    nrhs = _getnrhs(nrhs, x)
+    nrhs = _columns(x, nrhs)
     assert x.flags.f_contiguous and x.dtype == spmA.dtype
     _check("pastix_task_solve", lib.pastix_task_solve(pastix_data.ptr, nrhs, x.ctypes.data, n))
 
 
 def task_refine(pastix_data, spmA, b, x, nrhs=-1):
     n = x.shape[0]
-    nrhs = _getnrhs(nrhs, x)
+    nrhs = _columns(x, nrhs)
     if b.dtype != x.dtype:
-        raise TypeError("b and x must use the same arithmetic")
+        raise TypeError(f"task_refine: right-hand side ({b.dtype}) and solution ({x.dtype}) dtypes differ")
     assert x.flags.f_contiguous and b.flags.f_contiguous
     _check("pastix_task_refine", lib.pastix_task_refine(pastix_data.ptr, n, nrhs, b.ctypes.data, b.shape[0],
                                                         x.ctypes.data, x.shape[0]))
@@ -604,6 +603,52 @@ def solve_refine_device(pastix_data, db, dx, n, nrhs):
     one solve in the factor precision, then the instance's refinement."""
     _check("pastix_amd_solve_refine_device",
            lib.pastix_amd_solve_refine_device(pastix_data.ptr, nrhs, db.ptr, n, dx.ptr, n))
+
+
+for _f, _args in (("pastix_amd_bcsc_spmv", [_P, ct.c_int, ct.c_double, ct.c_void_p, ct.c_double, ct.c_void_p]),
+                  ("pastix_amd_bvec_dot", [_P, _I, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double)]),
+                  ("pastix_amd_bvec_nrm2", [_P, _I, ct.c_void_p, ct.POINTER(ct.c_double)]),
+                  ("pastix_amd_bvec_axpy", [_P, _I, ct.c_double, ct.c_void_p, ct.c_void_p]),
+                  ("pastix_amd_bvec_scal", [_P, _I, ct.c_double, ct.c_void_p]),
+                  ("pastix_amd_bvec_copy", [_P, _I, ct.c_void_p, ct.c_void_p]),
+                  ("pastix_amd_bvec_gemv", [_P, _I, _I, ct.c_double, ct.c_void_p, _I, ct.c_void_p, ct.c_double,
+                                            ct.c_void_p])):
+    getattr(lib, _f).argtypes = _args
+    getattr(lib, _f).restype = ct.c_int
+
+
+def bcsc_spmv(pastix_data, trans_, alpha, dx, beta, dy):
+    """dy = alpha op(P A P^T) dx + beta dy on FP64 DeviceArrays (permuted ordering)."""
+    _check("pastix_amd_bcsc_spmv", lib.pastix_amd_bcsc_spmv(pastix_data.ptr, trans_, alpha, dx.ptr, beta, dy.ptr))
+
+
+def bvec_dot(pastix_data, n, dx, dy):
+    out = ct.c_double()
+    _check("pastix_amd_bvec_dot", lib.pastix_amd_bvec_dot(pastix_data.ptr, n, dx.ptr, dy.ptr, ct.byref(out)))
+    return out.value
+
+
+def bvec_nrm2(pastix_data, n, dx):
+    out = ct.c_double()
+    _check("pastix_amd_bvec_nrm2", lib.pastix_amd_bvec_nrm2(pastix_data.ptr, n, dx.ptr, ct.byref(out)))
+    return out.value
+
+
+def bvec_axpy(pastix_data, n, alpha, dx, dy):
+    _check("pastix_amd_bvec_axpy", lib.pastix_amd_bvec_axpy(pastix_data.ptr, n, alpha, dx.ptr, dy.ptr))
+
+
+def bvec_scal(pastix_data, n, alpha, dx):
+    _check("pastix_amd_bvec_scal", lib.pastix_amd_bvec_scal(pastix_data.ptr, n, alpha, dx.ptr))
+
+
+def bvec_copy(pastix_data, n, dx, dy):
+    _check("pastix_amd_bvec_copy", lib.pastix_amd_bvec_copy(pastix_data.ptr, n, dx.ptr, dy.ptr))
+
+
+def bvec_gemv(pastix_data, m, n, alpha, dA, lda, dx, beta, dy):
+    _check("pastix_amd_bvec_gemv",
+           lib.pastix_amd_bvec_gemv(pastix_data.ptr, m, n, alpha, dA.ptr, lda, dx.ptr, beta, dy.ptr))
 
 
 class DeviceArray:

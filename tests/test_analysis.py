@@ -15,6 +15,7 @@ from tests.helpers import RawSpm, analyse
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LLT = pastix.factotype.LLT
+LU = pastix.factotype.LU
 
 
 def test_grid_order_golden_3x3x3():
@@ -227,3 +228,51 @@ def test_same_analysis_for_every_input_format(symmetric):
     for sym in syms[1:]:
         for k in ("permtab", "cblk_fcol", "cblk_bloknum", "blok_frow", "blok_lrow", "blok_fcblk"):
             assert np.array_equal(sym[k], syms[0][k]), k
+
+
+@pytest.mark.parametrize("case", ["llt_split", "llt_nosplit", "lu_split", "ldlt"])
+def test_thflops_and_nnzeros_equal_the_oracle_counts(case):
+    """DPARM_FACT_THFLOPS (the metric's denominator) equals the oracle's
+    reference count (symbol/symbol_cost.c:53-88 with common/flops.h, pinned
+    by brute-force operation counts in test_oracle.py) on the analysed,
+    split symbol, and IPARM_NNZEROS equals the strictly-lower entries of the
+    symbol (symbol/symbol_cost.c:327-361), doubled for LU
+    (blend/pastix_subtask_blend.c:535-539); DPARM_FILL_IN is their ratio to
+    the input's stored entries."""
+    import oracle
+    if case.startswith("lu"):
+        grid = (12, 11, 10)
+        A = spm.convection_diffusion27(*grid, seed=3)
+        facto, sym_in = LU, False
+    elif case == "ldlt":
+        grid = (9, 8, 7)
+        A = spm.laplacian7(*grid, diag=6.0)
+        facto, sym_in = pastix.factotype.LDLT, True
+    else:
+        grid = (16, 15, 14)
+        A = spm.laplacian7(*grid, diag=6.0)
+        facto, sym_in = LLT, True
+    bs = (1000, 2000) if case == "llt_nosplit" else (12, 24)
+    h, S, sym = analyse(A, sym_in, facto, grid=grid, min_bs=bs[0], max_bs=bs[1])
+    O = oracle.Oracle(sym)
+    th = float(h.dparm[pastix.dparm.fact_thflops])
+    if case != "ldlt":
+        assert th == pytest.approx(O.thflops(lu=facto == LU), rel=1e-14, abs=0)
+        assert th == float(pastix.flops(h)[0])
+    else:
+        # SYTRF counts (symbol_cost_flops.c): below the LU count of the same symbol
+        assert 0 < th < O.thflops(lu=True)
+    cnt = 0
+    for c in range(sym["cblknbr"]):
+        w = sym["cblk_fcol"][c + 1] - sym["cblk_fcol"][c]
+        cnt += w * (w - 1) // 2
+        for b in range(sym["cblk_bloknum"][c] + 1, sym["cblk_bloknum"][c + 1]):
+            cnt += (sym["blok_lrow"][b] - sym["blok_frow"][b] + 1) * w
+    nnz = int(h.iparm[pastix.iparm.nnzeros])
+    assert nnz == (2 * cnt if facto == LU else cnt)
+    stored = len(S.values)
+    assert float(h.dparm[pastix.dparm.fill_in]) == pytest.approx(nnz / stored, rel=1e-15)
+    if case == "llt_split":
+        # the split actually happened
+        assert max(sym["cblk_fcol"][1:] - sym["cblk_fcol"][:-1]) <= 24
+    pastix.finalize(h)

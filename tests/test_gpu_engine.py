@@ -358,13 +358,14 @@ def test_static_pivot_on_device():
     pastix.finalize(h)
 
 
-@pytest.mark.parametrize("method", ["SR", "BiCGSTAB"])
+@pytest.mark.parametrize("method", ["SR", "BiCGSTAB", "GMRES"])
 def test_refine_sr_bicgstab_fp64_factor_matches_oracle(method):
-    """Simple refinement and BiCGSTAB (reference refinement/z_refine_pivot.c,
-    z_refine_bicgstab.c) on the device with the FP64 factor: the oracle's
-    drivers on the oracle's factor take the same number of steps (the exact
-    preconditioner: 2 passes of simple refinement, 1 BiCGSTAB iteration), and
-    the iterates agree."""
+    """Simple refinement, BiCGSTAB and GMRES (reference
+    refinement/z_refine_pivot.c, z_refine_bicgstab.c, z_refine_gmres.c) on
+    the device with the FP64 factor: the oracle's drivers on the oracle's
+    factor take the same number of steps (the exact preconditioner: 2 passes
+    of simple refinement, 1 BiCGSTAB iteration, 1 GMRES step), and the
+    iterates agree."""
     grid = (10, 9, 8)
     A = spm.laplacian7(*grid, coef=(1.0, 1.0, 0.01))
     h, S, sym = analyse(A, True, LLT, grid=grid)
@@ -378,7 +379,7 @@ def test_refine_sr_bicgstab_fp64_factor_matches_oracle(method):
     p = sym["permtab"]
     bp = np.zeros(A.shape[0])
     bp[p] = b[:, 0]
-    fn = O.sr if method == "SR" else O.bicgstab
+    fn = {"SR": O.sr, "BiCGSTAB": O.bicgstab, "GMRES": O.gmres}[method]
     xo, ito, reso = fn(Lo, None, bc, bp, np.zeros(A.shape[0]), itermax=250, eps=1e-12)
     assert it == ito == (2 if method == "SR" else 1)
     assert np.abs(x[:, 0] - xo[p]).max() <= 1e-13 * np.abs(xo).max()
@@ -386,10 +387,10 @@ def test_refine_sr_bicgstab_fp64_factor_matches_oracle(method):
     pastix.finalize(h)
 
 
-@pytest.mark.parametrize("method", ["SR", "BiCGSTAB"])
+@pytest.mark.parametrize("method", ["SR", "BiCGSTAB", "GMRES"])
 @pytest.mark.parametrize("facto", [LLT, LU])
 def test_refine_sr_bicgstab_fp32_factor(method, facto):
-    """FP32 factor + FP64 simple refinement / BiCGSTAB reach berr <= 1e-12;
+    """FP32 factor + FP64 simple refinement / BiCGSTAB / GMRES reach berr <= 1e-12;
     the device-vector entry point runs the same recurrence (same iterate,
     same count), and the oracle's driver preconditioned with the same FP32
     factor values (applied in FP64) takes about as many steps."""
@@ -423,7 +424,7 @@ def test_refine_sr_bicgstab_fp32_factor(method, facto):
     bp[p] = b[:, 0]
     xs = bp.copy()
     O.solve(Lg.astype(np.float64), None if Ug is None else Ug.astype(np.float64), xs)
-    fn = O.sr if method == "SR" else O.bicgstab
+    fn = {"SR": O.sr, "BiCGSTAB": O.bicgstab, "GMRES": O.gmres}[method]
     xo, ito, reso = fn(Lg.astype(np.float64), None if Ug is None else Ug.astype(np.float64), bc, bp, xs, eps=1e-12)
     print(method, facto, "device steps", it, "oracle steps", ito, "oracle res", reso)
     assert reso <= 1e-12 and abs(it - ito) <= 1

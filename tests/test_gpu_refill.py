@@ -163,3 +163,29 @@ def test_changed_pattern_takes_whole_path():
     assert np.array_equal(L4, L5)
     for hh in (h, h2, h3):
         pastix.finalize(hh)
+
+
+def test_alternating_patterns_keep_device_memory_constant():
+    """An instance whose numfact inputs alternate between two patterns (the
+    second drops some off-diagonal pairs, so the analysed symbol still holds
+    it) frees the first pattern's device buffers before uploading the second:
+    the device footprint stays constant, and each factor equals a fresh
+    instance's."""
+    A, sym = _matrix(F.LLT)
+    C = sps.coo_matrix(A)
+    keep = ~((C.row != C.col) & ((np.minimum(C.row, C.col) % 5) == 0) & (np.abs(C.row - C.col) == 1))
+    B = sps.csc_matrix((C.data[keep], (C.row[keep], C.col[keep])), shape=A.shape)
+    assert B.nnz < A.nnz
+    h, SA, _ = analyse(A, True, F.LLT, grid=GRID, min_bs=12, max_bs=40)
+    SB = spm.Spm(B, symmetric=True)
+    sizes = []
+    for S in (SA, SB, SA, SB, SA, SB):
+        pastix.task_numfact(h, S)
+        sizes.append(pastix.device_bytes(h))
+    assert sizes[2:] == [sizes[0], sizes[1]] * 2, sizes
+    LB, _ = pastix.coeftab(h)
+    hB = _fresh(A, True, F.LLT, S=SB)
+    LB0, _ = pastix.coeftab(hB)
+    assert np.array_equal(LB, LB0)
+    pastix.finalize(hB)
+    pastix.finalize(h)

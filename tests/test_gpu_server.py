@@ -83,3 +83,41 @@ def test_server_mix_equals_alone():
         assert np.abs(x - xa).max() <= 1e-8 * np.abs(xa).max()
     for h, _ in slots:
         pastix.finalize(h)
+
+
+def test_server_failed_refactorisation_ends_its_solves():
+    """A refactorisation that fails (A - 7 I is not positive definite: LL^t
+    stops on a negative diagonal term) ends the solves bound to its version
+    with its error status instead of leaving them queued; solves of the
+    earlier version still succeed, and a later refactorisation serves its
+    solves again.  One slot, so a stuck version would stall everything."""
+    A = spm.laplacian7(*GRID, coef=(1.0, 1.0, 0.01))
+    n = A.shape[0]
+    h, S = _slot(A, spm.SpmFloat)
+    srv = pastix.Server([h], max_rhs=8)
+    rng = np.random.default_rng(5)
+    b0 = rng.uniform(-1.0, 1.0, size=(n, 2))
+    t0 = srv.submit_solve(b0.copy(order="F"))
+    tbad = srv.submit_refactor(-7.0)
+    b1 = rng.uniform(-1.0, 1.0, size=(n, 1))
+    t1 = srv.submit_solve(b1.copy(order="F"))
+    x0, _, v0 = srv.wait(t0)
+    assert v0 == 0 and all(_berr(A, x0[:, c], b0[:, c]) <= 1e-12 for c in range(2))
+    with pytest.raises(pastix.PastixError):
+        srv.wait(tbad)
+    with pytest.raises(pastix.PastixError):
+        srv.wait(t1)
+    # bound to the failed version after it failed: ends at once with its status
+    t2 = srv.submit_solve(b1.copy(order="F"))
+    with pytest.raises(pastix.PastixError):
+        srv.wait(t2)
+    tok = srv.submit_refactor(0.05)
+    b3 = rng.uniform(-1.0, 1.0, size=(n, 3))
+    t3 = srv.submit_solve(b3.copy(order="F"))
+    srv.wait(tok)
+    x3, _, v3 = srv.wait(t3)
+    assert v3 == 2
+    A2 = A + 0.05 * sps.eye(n)
+    assert all(_berr(A2, x3[:, c], b3[:, c]) <= 1e-12 for c in range(3))
+    srv.close()
+    pastix.finalize(h)

@@ -270,3 +270,87 @@ def test_bicgstab_refinement(lu):
     x2, it2, res2 = O.sr(Lp, Up, bc, b, np.zeros(A.shape[0]), itermax=50, eps=1e-12)
     assert res2 <= 1e-12 and np.abs(x2 - x).max() <= 1e-10 * np.abs(x).max()
     pastix.finalize(h)
+
+
+# ---- GMRES (reference refinement/z_refine_gmres.c) -------------------------
+# Any correct right-preconditioned GMRES cycle of k steps returns the point of
+# x0 + M^{-1} K_k(A M^{-1}, r0) with the smallest ||b - A x||_2; on A = I with
+# M = diag(c) and c taking m distinct values, A M^{-1} has a minimal
+# polynomial of degree m, so the exact solution is reached at step m.
+
+def _dense_precond(O, L, U):
+    """M^{-1} as a dense matrix, one oracle solve per unit vector."""
+    E = np.eye(O.n)
+    return O.solve(L, U, E.copy(order="F"))
+
+
+def _minres_cycle(A, Minv, b, x0, k):
+    r0 = b - A @ x0
+    AM = A @ Minv
+    K = np.empty((len(b), k))
+    v = r0.copy()
+    for j in range(k):
+        K[:, j] = v
+        v = AM @ v
+    Q, _ = np.linalg.qr(K)
+    c, *_ = np.linalg.lstsq(AM @ Q, r0, rcond=None)
+    return x0 + Minv @ (Q @ c)
+
+
+def test_gmres_exact_preconditioner_one_step():
+    for lu in (False, True):
+        grid = (7, 6, 5)
+        A = spm.convection_diffusion27(*grid, seed=4) if lu else spm.laplacian7(*grid, coef=(1.0, 1.0, 0.01))
+        h, S, sym = analyse(A, not lu, LU if lu else LLT, grid=grid)
+        O, bc, L, U = oracle_factor(A, sym, lu=lu)
+        b = np.random.default_rng(8).uniform(-1, 1, A.shape[0])
+        x, it, res = O.gmres(L, U, bc, b, np.zeros(A.shape[0]), itermax=50, im=25, eps=1e-12)
+        assert it == 1 and res <= 1e-12
+        assert np.abs(bc["B"] @ x - b).max() <= 1e-12
+        pastix.finalize(h)
+
+
+@pytest.mark.parametrize("values,steps", [((1.0, 2.0, 3.0), 3), ((1.5, 4.0), 2), ((2.0, 2.5, 3.0, 7.0, 9.0), 5)])
+def test_gmres_steps_equal_distinct_eigenvalues(values, steps):
+    O, bc, L1 = _identity_case(1.0)
+    c = np.array(values)[np.arange(O.n) % len(values)]
+    # permuted order: the factor's diagonal holds sqrt(c) (M = diag(c))
+    sym = O.sym
+    L = L1.copy()
+    for k in range(len(sym["cblk_stride"])):
+        fc, lc, st = sym["cblk_fcol"][k], sym["cblk_fcol"][k + 1], sym["cblk_stride"][k]
+        for j in range(lc - fc):
+            L[sym["cblk_coef"][k] + j + j * st] = np.sqrt(c[fc + j])
+    b = np.random.default_rng(9).uniform(-1, 1, O.n)
+    x, it, res = O.gmres(L, None, bc, b, np.zeros(O.n), itermax=50, im=25, eps=1e-12)
+    assert it == steps and res <= 1e-12
+    assert np.abs(x - b).max() <= 1e-11
+
+
+def test_gmres_iterates_are_minimal_residual_and_restart():
+    grid = (6, 5, 4)
+    A = spm.laplacian7(*grid, coef=(1.0, 1.0, 0.01))
+    h, S, sym = analyse(A, True, LLT, grid=grid)
+    O, bc, L, _ = oracle_factor(A, sym)
+    Lp = L * (1.0 + 1e-2 * np.random.default_rng(1).standard_normal(L.shape))
+    Minv = _dense_precond(O, Lp, None)
+    B = bc["B"].toarray()
+    b = np.random.default_rng(2).uniform(-1, 1, O.n)
+    x0 = np.zeros(O.n)
+    # one cycle cut at k steps by the step limit: the k-step minimal residual point
+    for k in (1, 2, 3):
+        x, it, _ = O.gmres(Lp, None, bc, b, x0, itermax=k, im=25, eps=1e-30)
+        xr = _minres_cycle(B, Minv, b, x0, k)
+        assert it == k
+        assert np.abs(x - xr).max() <= 1e-9 * np.abs(xr).max()
+    # GMRES(2), two cycles: the second restarts from the true residual of the first
+    x, it, _ = O.gmres(Lp, None, bc, b, x0, itermax=4, im=2, eps=1e-30)
+    xr = _minres_cycle(B, Minv, b, _minres_cycle(B, Minv, b, x0, 2), 2)
+    assert it == 4
+    assert np.abs(x - xr).max() <= 1e-9 * np.abs(xr).max()
+    # a nonzero initial guess enters through r0 = b - A x0
+    x1 = np.random.default_rng(3).uniform(-1, 1, O.n)
+    x, it, _ = O.gmres(Lp, None, bc, b, x1, itermax=2, im=25, eps=1e-30)
+    xr = _minres_cycle(B, Minv, b, x1, 2)
+    assert np.abs(x - xr).max() <= 1e-9 * np.abs(xr).max()
+    pastix.finalize(h)
