@@ -272,7 +272,7 @@ def subtask_applyorder(pastix_data, direction, b):
 
 def subtask_trsm(pastix_data, side_, uplo_, trans_, diag_, b, nrhs=-1):
     """One sweep of the solve on b in the permuted ordering, in place."""
-    nrhs = _getnrhs(nrhs, b)
+    nrhs = _columns(b, nrhs)
     flt = _spm.SpmDouble if b.dtype == np.float64 else _spm.SpmFloat
     _check("pastix_subtask_trsm", lib.pastix_subtask_trsm(pastix_data.ptr, flt, side_, uplo_, trans_, diag_, nrhs,
                                                           b.ctypes.data, b.shape[0]))
@@ -280,7 +280,7 @@ def subtask_trsm(pastix_data, side_, uplo_, trans_, diag_, b, nrhs=-1):
 
 def subtask_diag(pastix_data, b, nrhs=-1):
     """b /= the factor's diagonal (D of LDL^t), b in the permuted ordering."""
-    nrhs = _getnrhs(nrhs, b)
+    nrhs = _columns(b, nrhs)
     flt = _spm.SpmDouble if b.dtype == np.float64 else _spm.SpmFloat
     _check("pastix_subtask_diag", lib.pastix_subtask_diag(pastix_data.ptr, flt, nrhs, b.ctypes.data, b.shape[0]))
 
@@ -294,7 +294,7 @@ def getDiag(pastix_data, n, incD=1, dtype=np.float64):
 
 
 def subtask_solve(pastix_data, b, nrhs=-1):
-    nrhs = _getnrhs(nrhs, b)
+    nrhs = _columns(b, nrhs)
     _check("pastix_subtask_solve", lib.pastix_subtask_solve(pastix_data.ptr, nrhs, b.ctypes.data, b.shape[0]))
 
 
