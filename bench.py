@@ -78,8 +78,22 @@ def traffic_per_launch(kernel, grid):
     return None
 
 
-def cpu_baseline(pastix, spm, grid):
-    """Oracle (plain-C sequential restatement of the reference) on a bounded grid."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pastix, spm, grid, threads):
+    """The oracle's restatement of the reference static scheduler
+    (sopalin/sequential_zpotrf.c:71-120: per-thread task lists, ctrbcnt
+    waits, per-cblk locks) on `threads` host threads with a sequential CBLAS
+    per worker, on the same matrix as the GPU headline."""
     import oracle
     A = spm.laplacian7(*grid, diag=6.0)
     h, S = setup(pastix, spm, A, grid, resident=False)
@@ -87,12 +101,43 @@ def cpu_baseline(pastix, spm, grid):
     O = oracle.Oracle(sym)
     bc = oracle.bcsc(A, sym["permtab"])
     L, _, rc = O.fill(bc)
+    blas = oracle.use_blas()
+    thr = np.sqrt(np.ldexp(1.0, -53)) * np.linalg.norm(bc["vals"])
     t0 = time.perf_counter()
-    O.potrf(L, np.sqrt(np.ldexp(1.0, -53)) * np.linalg.norm(bc["vals"]))
+    rc, _ = O.factor_static(L, None, thr, threads)
     t = time.perf_counter() - t0
     th = O.thflops()
     pastix.finalize(h)
-    return th / t / 1e9, t
+    return th / t / 1e9, t, blas
+
+
+def cpu_sequential_c1(pastix, spm, grid=(20, 20, 20)):
+    """C1: the oracle's sequential scheduler (sopalin/sequential_zpotrf.c:34-68),
+    one thread, plain loops, with its backward error."""
+    import oracle
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S = setup(pastix, spm, A, grid, resident=False)
+    sym = pastix.symbol(h)
+    O = oracle.Oracle(sym)
+    bc = oracle.bcsc(A, sym["permtab"])
+    L, _, rc = O.fill(bc)
+    thr = np.sqrt(np.ldexp(1.0, -53)) * np.linalg.norm(bc["vals"])
+    t0 = time.perf_counter()
+    O.potrf(L, thr)
+    t = time.perf_counter() - t0
+    x0, b = spm.rhs_from_x(A, 1, seed=0)
+    p = sym["permtab"]
+    bp = np.zeros(A.shape[0])
+    bp[p] = b[:, 0]
+    s0 = time.perf_counter()
+    xp = O.solve(L, None, bp.copy())
+    ts = time.perf_counter() - s0
+    x = xp[p] if xp.ndim == 1 else xp[p, 0]
+    out = {"workload": f"laplacian7_{grid[0]}x{grid[1]}x{grid[2]}_potrf_fp64", "scheduler": "sequential",
+           "cores": 1, "numfact_s": t, "gflops": O.thflops() / t / 1e9, "solve_ms": ts * 1e3,
+           "berr": berr_inf(A, x, b[:, 0])}
+    pastix.finalize(h)
+    return out
 
 
 def leg_getrf(pastix, spm, grid, seed=5):
@@ -277,18 +322,33 @@ def leg_serving(pastix, spm, grid, nreq, seed=11, slots=2, max_rhs=64, load=0.6)
             time.sleep(dt)
         if kinds[i]:
             b = rng.uniform(-1.0, 1.0, size=(n, int(nrhs[i])))
-            tickets.append(("solve", srv.submit_solve(b), int(nrhs[i])))
+            # column 0 kept to check the result's backward error afterwards
+            tickets.append(("solve", srv.submit_solve(b), int(nrhs[i]), b[:, 0].copy()))
         else:
-            tickets.append(("refactor", srv.submit_refactor(float(rng.uniform(0.0, 0.1))), 0))
+            sh = float(rng.uniform(0.0, 0.1))
+            tickets.append(("refactor", srv.submit_refactor(sh), 0, sh))
     lat_s, lat_r, rhs_done = [], [], 0
-    for kind, t, r in tickets:
-        _, lat, _ = srv.wait(t)
+    shifts, checks = {0: 0.0}, []
+    for kind, t, r, extra in tickets:
+        x, lat, ver = srv.wait(t)
         if kind == "solve":
             lat_s.append(lat)
             rhs_done += r
+            checks.append((ver, extra, x[:, 0].copy()))
         else:
             lat_r.append(lat)
+            shifts[ver] = extra
     wall = time.perf_counter() - w0
+    # backward error of column 0 of every solve, against the matrix version
+    # the request was bound to
+    import scipy.sparse as sps
+    Ar = A.tocsr()
+    anorm = abs(Ar).sum(axis=1).max()
+    berr_max = 0.0
+    for ver, b0, x0 in checks:
+        r = b0 - (Ar @ x0 + shifts[ver] * x0)
+        berr_max = max(berr_max, float(np.abs(r).max() / ((anorm + abs(shifts[ver])) * np.abs(x0).max() +
+                                                         np.abs(b0).max())))
     st = srv.stats()
     srv.close()
     # plain FP64 path: the same kind of solves one at a time on one FP64 instance
@@ -312,6 +372,8 @@ def leg_serving(pastix, spm, grid, nreq, seed=11, slots=2, max_rhs=64, load=0.6)
             "solve_latency_p99_ms": float(np.percentile(lat_s, 99)) * 1e3 if lat_s else None,
             "refactor_latency_p50_ms": float(np.percentile(lat_r, 50)) * 1e3 if lat_r else None,
             "solves_per_s": len(lat_s) / wall, "rhs_per_s": rhs_done / wall, "batches": st["batches"],
+            "berr_max": berr_max, "berr_target_met": bool(berr_max <= 1e-12),
+            "berr_checked": "column 0 of every solve, against the matrix version it was bound to",
             "plain_fp64_rhs_per_s": prhs / t_plain,
             "plain_fp64_sample": f"{nsamp} requests solved one at a time with pastix_task_solve (host arrays, no refinement)"}
 
@@ -323,7 +385,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, nargs=3, default=[80, 80, 80])
     ap.add_argument("--solves", type=int, default=100)
-    ap.add_argument("--cpu-grid", type=int, nargs=3, default=[48, 48, 48])
+    ap.add_argument("--cpu-grid", type=int, nargs=3, default=None,
+                    help="grid of the CPU baseline (default: the headline grid)")
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="host threads of the CPU baseline (default: min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="only the timed factorisations (for rocprofv3)")
@@ -332,7 +397,7 @@ def main():
     ap.add_argument("--getrf-grid", type=int, nargs=3, default=[64, 64, 64])
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--serving-grid", type=int, nargs=3, default=[96, 96, 96])
-    ap.add_argument("--serving-requests", type=int, default=60)
+    ap.add_argument("--serving-requests", type=int, default=500)
     ap.add_argument("--overlap", type=int, default=1,
                     help="1: updates on a second stream beside the next level's panels; 0: one stream")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -493,12 +558,19 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cg = tuple(args.cpu_grid)
-        cpu_gf, cpu_t = cpu_baseline(pastix, spm, cg)
-        cpu = {"value": cpu_gf, "unit": "GFLOP/s", "cores": 1, "kind": "port",
-               "sample": f"oracle sequential LL^t numfact of the {cg[0]}x{cg[1]}x{cg[2]} 7-point Laplacian "
-                         f"(same stencil, smaller grid), {cpu_t:.1f} s of CPU work"}
-        log(f"cpu baseline {cpu_gf:.2f} GFLOP/s in {cpu_t:.1f}s")
+        cg = tuple(args.cpu_grid) if args.cpu_grid else grid
+        nt = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu_gf, cpu_t, blas = cpu_baseline(pastix, spm, cg, nt)
+        c1 = cpu_sequential_c1(pastix, spm)
+        cpu = {"value": cpu_gf, "unit": "GFLOP/s", "cores": nt, "kind": "port",
+               "sample": f"whole LL^t numfact of the {cg[0]}x{cg[1]}x{cg[2]} 7-point Laplacian (the headline "
+                         f"matrix) by the oracle's restatement of the reference static scheduler, {nt} threads, "
+                         f"{cpu_t:.1f} s",
+               "cpu_model": cpu_model(), "blas": blas or "plain loops", "numfact_s": cpu_t,
+               "gpu_speedup_numfact": (thflops / tmed) / (cpu_gf * 1e9),
+               "c1_sequential": c1}
+        log(f"cpu baseline {cpu_gf:.2f} GFLOP/s in {cpu_t:.1f}s on {nt} threads; C1 sequential "
+            f"{c1['gflops']:.2f} GFLOP/s berr {c1['berr']:.2e}")
 
     legs = {}
     for leg in [x for x in args.legs.split(",") if x and x != "none"]:

@@ -40,6 +40,12 @@ void pastixOrderExit ( pastix_order_t * const ordeptr );
 void pastixOrderBase ( pastix_order_t * const ordeptr, pastix_int_t baseval );
 int  pastixOrderCheck( const pastix_order_t * const ordeptr );
 
+/* reference order/order_io.c:132-181,294-330: the file named by the
+ * PASTIX_FILE_ORDER environment variable (default "ordername" to load,
+ * "ordergen" in the instance's output directory to save) */
+int  pastixOrderLoad( const pastix_data_t *pastix_data, pastix_order_t *ordeptr );
+int  pastixOrderSave( pastix_data_t *pastix_data, const pastix_order_t *ordeptr );
+
 /* reference order/order_grids.c:275-383 */
 int  pastixOrderGrid ( pastix_order_t **myorder, pastix_int_t nx,
                        pastix_int_t ny, pastix_int_t nz );

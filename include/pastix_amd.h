@@ -20,7 +20,8 @@ typedef enum pastix_amd_iparm_e {
     PXA_IPARM_FACTOR_PRECISION = 0,
     /* HIP device ordinal used by this instance (default 0).                  */
     PXA_IPARM_DEVICE           = 1,
-    /* Apply the reference's level reordering of supernodes (default 0).      */
+    /* Apply the reference's level reordering of supernodes, as its ordering
+     * step always does (default 1; 0 keeps the ordering as computed).      */
     PXA_IPARM_LEVEL_ORDER      = 2,
     /* Keep a device copy of the filled coefficients so that
      * pastix_amd_refactorize() can rerun the factorisation from HBM
@@ -239,6 +240,17 @@ int pastix_amd_server_wait(pastix_amd_server_t *srv, int64_t ticket, double *lat
 /* out[0] solve batches run, [1] right-hand sides they carried, [2] newest
  * version, [3] requests not finished. */
 int pastix_amd_server_stats(pastix_amd_server_t *srv, int64_t out[4]);
+/* HBM-budget admission: the device memory of the merged solves' work blocks
+ * (right-hand sides, solutions, refinement vectors) that all slots together
+ * may hold; by default IPARM_GPU_MEMORY_PERCENTAGE (of the first slot) of the
+ * device memory free when the server is created.  A batch is admitted only
+ * while it fits, shorter if need be; the oldest request otherwise waits
+ * while idle slots give their blocks back, and a request that cannot fit
+ * alone ends with PASTIX_ERR_OUTOFMEMORY. */
+int pastix_amd_server_set_budget(pastix_amd_server_t *srv, int64_t bytes);
+/* out[0] budget, [1] bytes the slots hold now, [2] the most they held at
+ * once, [3] times the oldest request waited for memory. */
+int pastix_amd_server_memory(pastix_amd_server_t *srv, int64_t out[4]);
 /* Finishes every admitted request, stops the workers, frees the server. */
 int pastix_amd_server_destroy(pastix_amd_server_t **srv);
 

@@ -50,7 +50,24 @@ def _lib():
         f.restype = ct.c_int
     lib.ora_gmres.argtypes = [P, ct.c_int, P, P, ct.c_int64, P, P, P, P, P, ct.c_int, ct.c_int, ct.c_double, P]
     lib.ora_gmres.restype = ct.c_int
+    lib.ora_factor_static.argtypes = [P, ct.c_int, P, P, ct.c_double, ct.c_int, P]
+    lib.ora_factor_static.restype = ct.c_int
+    lib.ora_use_blas.argtypes = [ct.c_char_p, ct.c_char_p]
+    lib.ora_use_blas.restype = ct.c_int
     return lib
+
+
+def use_blas():
+    """Give the static scheduler a sequential CBLAS (scipy's OpenBLAS, one
+    thread per call), as the reference's CPU path calls one; returns a
+    description, or None when it is not found (plain loops then)."""
+    import glob
+    import scipy
+    libdir = os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs")
+    for path in sorted(glob.glob(os.path.join(libdir, "libscipy_openblas*.so"))):
+        if _lib().ora_use_blas(path.encode(), b"scipy_") == 0:
+            return "OpenBLAS " + os.path.basename(path) + " (scipy wheel), 1 thread per worker"
+    return None
 
 
 def bcsc(A, permtab):
@@ -97,6 +114,14 @@ class Oracle:
     def getrf(self, L, U, crit):
         npiv = ct.c_int64()
         rc = self.lib.ora_getrf(ct.byref(self.s), L.ctypes.data, U.ctypes.data, crit, ct.byref(npiv))
+        return rc, npiv.value
+
+    def factor_static(self, L, U, crit, nthreads):
+        """The reference static scheduler on nthreads threads (sopalin/sequential_zpotrf.c:71-120);
+        LU when U is given.  Returns (rc, static pivots)."""
+        npiv = ct.c_int64()
+        rc = self.lib.ora_factor_static(ct.byref(self.s), 1 if U is not None else 0, L.ctypes.data,
+                                        U.ctypes.data if U is not None else None, crit, nthreads, ct.byref(npiv))
         return rc, npiv.value
 
     def solve(self, L, U, x):

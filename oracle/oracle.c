@@ -28,6 +28,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -204,10 +205,48 @@ static void getrfsp(int64_t n, double *A, int64_t lda, int64_t *npiv, double cri
     }
 }
 
+/* ------------------------------------------ optional CBLAS (CPU timing) --- */
+/* The static scheduler below times the reference's CPU path, which calls a
+ * sequential CBLAS (README.md:76-77) for its TRSMs and GEMMs; ora_use_blas()
+ * loads one (the cblas_dgemm / cblas_dtrsm of a given shared library, with its
+ * own threading set to one thread).  The sequential drivers above never use
+ * it: they stay plain loops. */
+#include <dlfcn.h>
+enum { CB_COL = 102, CB_NOTRANS = 111, CB_TRANS = 112, CB_UPPER = 121, CB_LOWER = 122, CB_NONUNIT = 131,
+       CB_UNIT = 132, CB_RIGHT = 142 };
+typedef void (*cblas_dgemm_t)(int, int, int, int, int, int, double, const double *, int, const double *, int,
+                              double, double *, int);
+typedef void (*cblas_dtrsm_t)(int, int, int, int, int, int, int, double, const double *, int, double *, int);
+static cblas_dgemm_t blas_dgemm = NULL;
+static cblas_dtrsm_t blas_dtrsm = NULL;
+
+int ora_use_blas(const char *path, const char *prefix) {
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1;
+    char name[128];
+    snprintf(name, sizeof name, "%scblas_dgemm", prefix);
+    cblas_dgemm_t g = (cblas_dgemm_t)dlsym(h, name);
+    snprintf(name, sizeof name, "%scblas_dtrsm", prefix);
+    cblas_dtrsm_t t = (cblas_dtrsm_t)dlsym(h, name);
+    snprintf(name, sizeof name, "%sopenblas_set_num_threads", prefix);
+    void (*nt)(int) = (void (*)(int))dlsym(h, name);
+    if (!g || !t) return -1;
+    if (nt) nt(1);
+    blas_dgemm = g;
+    blas_dtrsm = t;
+    return 0;
+}
+
 /* --------------------------------------------- Schur update (gemmsp) --- */
 /* shift = 0: iterblok >= blok (L side), 1: iterblok > blok (U side) */
+static void gemmsp_x(const ora_symbol_t *s, int shift, int64_t c, int64_t b, const double *A, const double *B,
+                     double *C, int blas);
 static void gemmsp(const ora_symbol_t *s, int shift, int64_t c, int64_t b, const double *A, const double *B,
                    double *C) {
+    gemmsp_x(s, shift, c, b, A, B, C, 0);
+}
+static void gemmsp_x(const ora_symbol_t *s, int shift, int64_t c, int64_t b, const double *A, const double *B,
+                     double *C, int blas) {
     const int64_t f = s->blok_fcblk[b];
     const int64_t K = width(s, c), st = s->cblk_stride[c], stf = s->cblk_stride[f];
     const int64_t N = brows(s, b);
@@ -219,7 +258,11 @@ static void gemmsp(const ora_symbol_t *s, int shift, int64_t c, int64_t b, const
         const double *Ai = A + s->cblk_coef[c] + s->blok_roff[i];
         double *Ci = C + s->cblk_coef[f] + s->blok_roff[fb] + (s->blok_frow[i] - s->blok_frow[fb]) +
                      (s->blok_frow[b] - s->cblk_fcol[f]) * stf;
-        gemm_nt(M, N, K, -1.0, Ai, st, Bb, st, Ci, stf);
+        if (blas && blas_dgemm)
+            blas_dgemm(CB_COL, CB_NOTRANS, CB_TRANS, (int)M, (int)N, (int)K, -1.0, Ai, (int)st, Bb, (int)st, 1.0, Ci,
+                       (int)stf);
+        else
+            gemm_nt(M, N, K, -1.0, Ai, st, Bb, st, Ci, stf);
     }
 }
 
@@ -561,4 +604,217 @@ int ora_gmres(const ora_symbol_t *s, int lu, const double *L, const double *U, i
     free(V); free(W); free(H); free(cs); free(sn); free(g);
     if (resid) *resid = rb;
     return iters;
+}
+
+/* ------------------------------------------------ static scheduler --- */
+/* Restatement of the reference static scheduler (sopalin/sequential_zpotrf.c:
+ * 71-120 thread_zpotrf_static, sequential_zgetrf.c the same shape) on
+ * nthreads POSIX threads:
+ *   - ctrbcnt[c] starts at the number of blocks facing c (blend/
+ *     solver_backup.c:95-101);
+ *   - each thread walks its own task list (one task per cblk, increasing
+ *     cblk order); before a task it waits until ctrbcnt reaches zero
+ *     (cpucblk_zincoming_deps, kernels/cpucblk_zmpi_coeftab.c:482-515),
+ *     factors the panel (diagonal block + TRSM), then for every off-diagonal
+ *     block applies the Schur update into the facing cblk under that cblk's
+ *     lock (cpucblk_zgemmsp with pastix_cblk_lock) and decrements its counter
+ *     (cpucblk_zrelease_deps :541-562).
+ * The task lists come from a proportional mapping of the cblk tree (parent =
+ * the cblk facing the first off-diagonal block) by flop cost, standing in for
+ * blend's proportional mapping and simulation (blend/propmap.c,
+ * blend/simu_run.c): a subtree whose share falls to one thread goes to that
+ * thread whole; a cblk shared by several candidates goes to the candidate
+ * with the least work so far.  Every task waits only on lower-numbered cblks,
+ * so the threads cannot deadlock; a failing panel (negative pivot in LL^t)
+ * raises a flag that ends every wait. */
+#include <pthread.h>
+#include <sched.h>
+
+typedef struct {
+    const ora_symbol_t *s;
+    double *L, *U;
+    double crit;
+    int lu;
+    int64_t **tasks, *ntasks;
+    int32_t *ctrb;
+    pthread_mutex_t *locks;
+    int64_t *npiv;
+    volatile int err;
+    int rank;
+} static_ctx_t;
+
+typedef struct {
+    static_ctx_t *g;
+    int rank;
+} static_arg_t;
+
+static double cblk_cost(const ora_symbol_t *s, int64_t c, int lu) {
+    const double N = (double)width(s, c);
+    double M = 0.0;
+    for (int64_t b = s->cblk_bloknum[c] + 1; b < s->cblk_bloknum[c + 1]; b++) M += (double)brows(s, b);
+    return (lu ? 2.0 : 1.0) * (N * N * N / 3.0 + M * N * N + M * M * N) + 1.0;
+}
+
+/* thread ranges [t0, t1) of the subtree rooted at c, children first */
+static void propmap(const ora_symbol_t *s, int64_t c, double t0, double t1, const int64_t *cptr,
+                    const int64_t *clist, const double *scost, const double *ccost, double *load, int nthreads,
+                    int64_t *owner) {
+    const int a = (int)t0, b = (int)(t1 - 1e-9);
+    if (b <= a) {
+        /* the whole subtree on one thread: iterate it explicitly */
+        int64_t stack_sz = 1, *stack = malloc((size_t)(s->cblknbr + 1) * sizeof(int64_t));
+        stack[0] = c;
+        while (stack_sz) {
+            const int64_t x = stack[--stack_sz];
+            owner[x] = a < nthreads ? a : nthreads - 1;
+            load[owner[x]] += ccost[x];
+            for (int64_t k = cptr[x]; k < cptr[x + 1]; k++) stack[stack_sz++] = clist[k];
+        }
+        free(stack);
+        return;
+    }
+    int best = a;
+    for (int t = a + 1; t <= b && t < nthreads; t++)
+        if (load[t] < load[best]) best = t;
+    owner[c] = best;
+    load[best] += ccost[c];
+    const double sub = scost[c] - ccost[c];
+    double at = t0;
+    for (int64_t k = cptr[c]; k < cptr[c + 1]; k++) {
+        const int64_t ch = clist[k];
+        const double w = sub > 0 ? (t1 - t0) * scost[ch] / sub : 0.0;
+        propmap(s, ch, at, at + w, cptr, clist, scost, ccost, load, nthreads, owner);
+        at += w;
+    }
+}
+
+static void *static_worker(void *argp) {
+    static_arg_t *arg = (static_arg_t *)argp;
+    static_ctx_t *g = arg->g;
+    const ora_symbol_t *s = g->s;
+    const int rank = arg->rank;
+    int64_t npiv = 0;
+    for (int64_t ii = 0; ii < g->ntasks[rank]; ii++) {
+        const int64_t c = g->tasks[rank][ii];
+        /* incoming dependencies */
+        while (__atomic_load_n(&g->ctrb[c], __ATOMIC_ACQUIRE) > 0) {
+            if (g->err) return NULL;
+            sched_yield();
+        }
+        if (g->err) return NULL;
+        const int64_t K = width(s, c), st = s->cblk_stride[c];
+        double *P = g->L + s->cblk_coef[c];
+        if (!g->lu) {
+            if (potrfsp(K, P, st, &npiv, g->crit)) {
+                g->err = 1;
+                return NULL;
+            }
+            if (st > K) {
+                if (blas_dtrsm)
+                    blas_dtrsm(CB_COL, CB_RIGHT, CB_LOWER, CB_TRANS, CB_NONUNIT, (int)(st - K), (int)K, 1.0, P,
+                               (int)st, P + K, (int)st);
+                else
+                    trsm_rltn(st - K, K, P, st, P + K, st);
+            }
+        } else {
+            double *Q = g->U + s->cblk_coef[c];
+            for (int64_t j = 0; j < K; j++)
+                for (int64_t i = 0; i < K; i++) P[i + j * st] += Q[j + i * st];
+            getrfsp(K, P, st, &npiv, g->crit);
+            for (int64_t j = 0; j < K; j++)
+                for (int64_t i = 0; i < K; i++) Q[i + j * st] = P[j + i * st];
+            if (st > K) {
+                if (blas_dtrsm) {
+                    blas_dtrsm(CB_COL, CB_RIGHT, CB_UPPER, CB_NOTRANS, CB_NONUNIT, (int)(st - K), (int)K, 1.0, P,
+                               (int)st, P + K, (int)st);
+                    blas_dtrsm(CB_COL, CB_RIGHT, CB_UPPER, CB_NOTRANS, CB_UNIT, (int)(st - K), (int)K, 1.0, Q,
+                               (int)st, Q + K, (int)st);
+                } else {
+                    trsm_run(st - K, K, P, st, 0, P + K, st);
+                    trsm_run(st - K, K, Q, st, 1, Q + K, st);
+                }
+            }
+        }
+        for (int64_t b = s->cblk_bloknum[c] + 1; b < s->cblk_bloknum[c + 1]; b++) {
+            const int64_t f = s->blok_fcblk[b];
+            pthread_mutex_lock(&g->locks[f]);
+            if (!g->lu) {
+                gemmsp_x(s, 0, c, b, g->L, g->L, g->L, 1);
+            } else {
+                gemmsp_x(s, 0, c, b, g->L, g->U, g->L, 1);
+                if (b + 1 < s->cblk_bloknum[c + 1]) gemmsp_x(s, 1, c, b, g->U, g->L, g->U, 1);
+            }
+            pthread_mutex_unlock(&g->locks[f]);
+            __atomic_sub_fetch(&g->ctrb[f], 1, __ATOMIC_RELEASE);
+        }
+    }
+    g->npiv[rank] = npiv;
+    return NULL;
+}
+
+int ora_factor_static(const ora_symbol_t *s, int lu, double *L, double *U, double crit, int nthreads,
+                      int64_t *npiv) {
+    const int64_t cb = s->cblknbr;
+    if (nthreads < 1) nthreads = 1;
+    *npiv = 0;
+    /* cblk tree: parent = cblk facing the first off-diagonal block */
+    int64_t *parent = malloc(cb * sizeof(int64_t)), *cptr = calloc(cb + 2, sizeof(int64_t));
+    int64_t *clist = malloc((cb + 1) * sizeof(int64_t)), *owner = malloc(cb * sizeof(int64_t));
+    double *ccost = malloc(cb * sizeof(double)), *scost = malloc(cb * sizeof(double));
+    double *load = calloc(nthreads, sizeof(double));
+    int32_t *ctrb = calloc(cb, sizeof(int32_t));
+    pthread_mutex_t *locks = malloc(cb * sizeof(pthread_mutex_t));
+    int64_t *ntasks = calloc(nthreads, sizeof(int64_t)), **tasks = malloc(nthreads * sizeof(int64_t *));
+    int64_t *tnpiv = calloc(nthreads, sizeof(int64_t));
+    for (int64_t c = 0; c < cb; c++) {
+        const int64_t b = s->cblk_bloknum[c] + 1;
+        parent[c] = (b < s->cblk_bloknum[c + 1]) ? s->blok_fcblk[b] : -1;
+        ccost[c] = cblk_cost(s, c, lu);
+        scost[c] = ccost[c];
+        pthread_mutex_init(&locks[c], NULL);
+        for (int64_t k = s->cblk_bloknum[c] + 1; k < s->cblk_bloknum[c + 1]; k++) ctrb[s->blok_fcblk[k]]++;
+    }
+    for (int64_t c = 0; c < cb; c++) /* parents have higher numbers: accumulate upward */
+        if (parent[c] >= 0) scost[parent[c]] += scost[c];
+    for (int64_t c = 0; c < cb; c++) cptr[(parent[c] >= 0 ? parent[c] : cb) + 1]++;
+    for (int64_t c = 0; c <= cb; c++) cptr[c + 1] += cptr[c];
+    {
+        int64_t *next = malloc((cb + 1) * sizeof(int64_t));
+        for (int64_t c = 0; c <= cb; c++) next[c] = cptr[c];
+        for (int64_t c = 0; c < cb; c++) clist[next[parent[c] >= 0 ? parent[c] : cb]++] = c;
+        free(next);
+    }
+    /* roots share the threads by cost */
+    double rtot = 0.0;
+    for (int64_t k = cptr[cb]; k < cptr[cb + 1]; k++) rtot += scost[clist[k]];
+    double at = 0.0;
+    for (int64_t k = cptr[cb]; k < cptr[cb + 1]; k++) {
+        const double w = rtot > 0 ? nthreads * scost[clist[k]] / rtot : 0.0;
+        propmap(s, clist[k], at, at + w, cptr, clist, scost, ccost, load, nthreads, owner);
+        at += w;
+    }
+    for (int64_t c = 0; c < cb; c++) ntasks[owner[c]]++;
+    for (int t = 0; t < nthreads; t++) {
+        tasks[t] = malloc((ntasks[t] + 1) * sizeof(int64_t));
+        ntasks[t] = 0;
+    }
+    for (int64_t c = 0; c < cb; c++) tasks[owner[c]][ntasks[owner[c]]++] = c;
+    static_ctx_t g = {s, L, U, crit, lu, tasks, ntasks, ctrb, locks, tnpiv, 0, 0};
+    pthread_t *th = malloc(nthreads * sizeof(pthread_t));
+    static_arg_t *args = malloc(nthreads * sizeof(static_arg_t));
+    for (int t = 0; t < nthreads; t++) {
+        args[t].g = &g;
+        args[t].rank = t;
+        pthread_create(&th[t], NULL, static_worker, &args[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    if (getenv("ORA_STATIC_DEBUG"))
+        for (int t = 0; t < nthreads; t++) fprintf(stderr, "thread %d: %ld tasks, cost %.3e\n", t, (long)ntasks[t], load[t]);
+    for (int t = 0; t < nthreads; t++) *npiv += tnpiv[t];
+    const int rc = g.err ? -1 : 0;
+    for (int64_t c = 0; c < cb; c++) pthread_mutex_destroy(&locks[c]);
+    for (int t = 0; t < nthreads; t++) free(tasks[t]);
+    free(th); free(args); free(tasks); free(ntasks); free(tnpiv); free(locks); free(ctrb); free(load);
+    free(scost); free(ccost); free(owner); free(clist); free(cptr); free(parent);
+    return rc;
 }

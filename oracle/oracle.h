@@ -31,4 +31,10 @@ int    ora_bicgstab(const ora_symbol_t *s, int lu, const double *L, const double
 int    ora_gmres(const ora_symbol_t *s, int lu, const double *L, const double *U, int64_t n,
                  const int64_t *rowptr, const int64_t *cols, const double *vals, const double *b, double *x,
                  int itermax, int im, double eps, double *resid);
+/* load cblas_dgemm / cblas_dtrsm (names prefixed by prefix) from a shared
+ * library for ora_factor_static; 0 on success */
+int    ora_use_blas(const char *path, const char *prefix);
+/* the reference static scheduler on nthreads POSIX threads (lu: GETRF with U) */
+int    ora_factor_static(const ora_symbol_t *s, int lu, double *L, double *U, double crit, int nthreads,
+                         int64_t *npiv);
 #endif

@@ -7,6 +7,8 @@
 #include "analysis.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <cmath>
 #include <cstdio>
 #include <functional>
@@ -366,28 +368,75 @@ void order_find_supernodes(const Graph &g, Order &o) {
     }
 }
 
-// Reference order/order_apply_level_order.c:270-330 (the branch without 2D
-// tasks): breadth-first walk from the roots, children pushed in reverse so
-// that brothers keep their order, then renumber supernodes bottom-up.
-void order_apply_level_order(Order &o) {
-    int64_t cb = o.cblknbr;
+// Reference order/order_apply_level_order.c:81-333.  Breadth-first walk of
+// the elimination tree from the roots, sons pushed in reverse so that
+// brothers keep their order; supernodes are then numbered backwards from
+// the end of the walk.  With level_tasks2d < 0 (the reference default) the
+// walk keeps two queues: the "2D" supernodes, at least width_tasks2d columns
+// wide together with their brothers and every ancestor and its brothers
+// (the PASTIX_BLEND_DEEPEST_DISTRIB marking, :155-201), come first in the
+// walk, so they take the last numbers; the others follow.
+void order_apply_level_order(Order &o, int64_t level_tasks2d, int64_t width_tasks2d) {
+    const int64_t cb = o.cblknbr;
     if (cb == 0) return;
-    std::vector<std::vector<int64_t>> sons(cb);
-    std::vector<int64_t> roots;
+    // elimination tree; index cb stands for the virtual root above the roots
+    std::vector<int64_t> fath(cb);
+    std::vector<std::vector<int64_t>> sons(cb + 1);
     for (int64_t i = 0; i < cb; i++) {
-        if (o.treetab[i] < 0) roots.push_back(i);
-        else sons[o.treetab[i]].push_back(i);
+        fath[i] = o.treetab[i] < 0 ? cb : o.treetab[i];
+        sons[fath[i]].push_back(i);
     }
-    std::vector<int64_t> sorted(roots);
+    std::vector<int64_t> sorted(cb);
     std::vector<int64_t> newfath(cb, -1);
-    for (size_t i = 0; i < sorted.size(); i++) {
-        int64_t node = sorted[i];
-        const auto &s = sons[node];
-        size_t pos = sorted.size();
-        sorted.resize(pos + s.size());
-        for (size_t k = 0; k < s.size(); k++) {
-            sorted[pos + s.size() - 1 - k] = s[k];
-            newfath[s[k]] = cb - (int64_t)i - 1;
+    if (level_tasks2d < 0) {
+        std::vector<char> is2d(cb + 1, 0);
+        int64_t tot2d = 0;
+        auto mark_sons = [&](int64_t f) {
+            for (int64_t s : sons[f])
+                if (!is2d[s]) {
+                    is2d[s] = 1;
+                    tot2d++;
+                }
+        };
+        for (int64_t node = 0; node < cb; node++) {
+            if (is2d[node]) continue;
+            const int64_t size = o.rangtab[node + 1] - o.rangtab[node];
+            if (size < width_tasks2d) continue;
+            int64_t f = fath[node];
+            mark_sons(f);
+            while (f != cb && !is2d[f]) {
+                f = fath[f];
+                mark_sons(f);
+            }
+        }
+        int64_t p2 = 0, p1 = tot2d;
+        for (int64_t r : sons[cb]) sorted[is2d[r] ? p2++ : p1++] = r;
+        for (int64_t i = 0; i < cb; i++) {
+            const int64_t node = sorted[i];
+            const auto &sn = sons[node];
+            int64_t n2 = 0;
+            for (int64_t s : sn) n2 += is2d[s];
+            const int64_t n1 = (int64_t)sn.size() - n2;
+            int64_t c2 = 0, c1 = 0;
+            for (int64_t s : sn) {
+                if (is2d[s]) sorted[p2 + n2 - (c2++) - 1] = s;
+                else sorted[p1 + n1 - (c1++) - 1] = s;
+                newfath[s] = cb - i - 1;
+            }
+            p2 += n2;
+            p1 += n1;
+        }
+    } else {
+        int64_t pos = 0;
+        for (int64_t r : sons[cb]) sorted[pos++] = r;
+        for (int64_t i = 0; i < cb; i++) {
+            const int64_t node = sorted[i];
+            const auto &sn = sons[node];
+            for (size_t k = 0; k < sn.size(); k++) {
+                sorted[pos + sn.size() - 1 - k] = sn[k];
+                newfath[sn[k]] = cb - i - 1;
+            }
+            pos += (int64_t)sn.size();
         }
     }
     std::vector<int64_t> rang(cb + 1, 0), tree(cb), peri(o.n);
@@ -402,6 +451,203 @@ void order_apply_level_order(Order &o) {
     o.rangtab = rang;
     o.treetab = tree;
     o.peritab = peri;
+    for (int64_t i = 0; i < o.n; i++) o.permtab[o.peritab[i]] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Symbol reordering (reference symbol/symbol_reordering.c:40-660): inside each
+// supernode, rows are reordered so that rows touched by the same
+// contributing supernodes become adjacent and the off-diagonal blocks facing
+// the supernode fewer and taller.  Each row of supernode c carries the set of
+// supernodes with a block facing c over that row, split in an "upper" set
+// (contributors no deeper than a split level of the elimination tree) and a
+// "lower" set; rows are inserted one at a time into a tour, at the position
+// that adds the least Hamming distance between lower sets, among the
+// positions whose neighbours' upper sets agree.
+// ---------------------------------------------------------------------------
+namespace {
+using SetVec = std::vector<std::vector<int64_t>>;
+
+// reference hamming_distance (:99-162): size of the symmetric difference of
+// two sorted sets, capped at stop; -1 is a ghost row with the empty set
+int64_t hamming(const SetVec &v, int64_t xi, int64_t xj, int64_t stop) {
+    if (xi == -1) return (int64_t)v[xj].size();
+    if (xj == -1) return (int64_t)v[xi].size();
+    const std::vector<int64_t> &a = v[xi], &b = v[xj];
+    const int64_t sa = (int64_t)a.size(), sb = (int64_t)b.size();
+    if (sa - sb >= stop) return stop;
+    if (sb - sa >= stop) return stop;
+    int64_t sum = 0, i = 0, j = 0;
+    while (i < sa && j < sb) {
+        if (a[i] == b[j]) {
+            i++;
+            j++;
+        } else if (a[i] < b[j]) {
+            while (i < sa && a[i] < b[j]) {
+                sum++;
+                i++;
+            }
+        } else {
+            while (j < sb && a[i] > b[j]) {
+                sum++;
+                j++;
+            }
+        }
+        if (sum >= stop) return stop;
+    }
+    sum += (sa - i) + (sb - j);
+    return sum >= stop ? stop : sum;
+}
+
+// reference symbol_reorder_tsp (:212-437): peri points at the supernode's
+// columns in the ordering's inverse permutation
+void reorder_tsp(int64_t size, int64_t *peri, const SetVec &lw, const SetVec &up, int64_t stop) {
+    if (size < 3) return;
+    std::vector<int64_t> tinvp(size + 1), tlen(size + 1, 0);
+    tinvp[0] = -1;
+    tinvp[1] = 0;
+    const int64_t distance = hamming(lw, 0, -1, stop);
+    tlen[0] = distance;
+    tlen[1] = distance;
+    for (int64_t i = 1; i < size; i++) {
+        int64_t lw_before = hamming(lw, i, tinvp[0], stop);
+        int64_t lw_after = hamming(lw, i, tinvp[1], stop);
+        int64_t up_after = hamming(up, i, tinvp[1], 1);
+        int64_t minl = lw_before + lw_after - tlen[0];
+        int64_t mpos = 1, min_cut = -1;
+        for (int64_t j = 1; j < i; j++) {
+            const int64_t up_before = up_after;
+            up_after = hamming(up, i, tinvp[j + 1], 1);
+            if (up_before < 1 || up_after < 1) {
+                // a skipped position leaves the previous distance unknown
+                lw_before = (lw_after == -1) ? hamming(lw, i, tinvp[j], stop) : lw_after;
+                lw_after = hamming(lw, i, tinvp[j + 1], stop);
+                const int64_t l = lw_before + lw_after - tlen[j];
+                // the same tour length: the smaller cut between two rows
+                if (l == minl) {
+                    if (lw_before < min_cut) {
+                        min_cut = lw_before;
+                        minl = l;
+                        mpos = j + 1;
+                    }
+                    if (lw_after < min_cut) {
+                        min_cut = lw_after;
+                        minl = l;
+                        mpos = j + 1;
+                    }
+                }
+                if (l < minl) {
+                    min_cut = lw_before;
+                    minl = l;
+                    mpos = j + 1;
+                    if (lw_after < min_cut) min_cut = lw_after;
+                }
+                // two equal rows: next to each other, search ends
+                if (lw_after == 0) {
+                    min_cut = 0;
+                    minl = l;
+                    mpos = j + 1;
+                    j = i;
+                }
+            } else {
+                lw_after = -1;
+            }
+        }
+        // between the last and the first
+        const int64_t first_pos = hamming(lw, i, tinvp[0], stop);
+        const int64_t last_pos = hamming(lw, i, tinvp[i], stop);
+        lw_before = hamming(lw, i, tinvp[mpos - 1], stop);
+        lw_after = hamming(lw, i, tinvp[mpos], stop);
+        const int64_t l = first_pos + last_pos - tlen[i];
+        if (l < minl) {
+            minl = l;
+            mpos = i + 1;
+        }
+        if (mpos > 0) tlen[mpos - 1] = lw_before;
+        if (mpos < i + 1) {
+            int64_t k = i, ll = lw_after;
+            for (int64_t j = mpos; j < i + 2; j++) {
+                std::swap(tinvp[j], k);
+                std::swap(tlen[j], ll);
+            }
+        } else {
+            tinvp[i + 1] = i;
+            tlen[i + 1] = first_pos;
+        }
+    }
+    // the tour starts after the ghost row (searched among the first size
+    // positions, as the reference does)
+    int64_t elected = 0;
+    for (int64_t i = 0; i < size; i++)
+        if (tinvp[i] == -1) elected = i;
+    std::vector<int64_t> sn(size);
+    for (int64_t i = 0; i < size; i++) sn[i] = peri[tinvp[(i + 1 + elected) % (size + 1)]];
+    std::copy(sn.begin(), sn.end(), peri);
+}
+}  // namespace
+
+void symbol_reordering(const Symbol &s, Order &o, int64_t split_level, int64_t stop, int64_t schurfcol) {
+    const int64_t cb = s.cblknbr();
+    if (cb != o.cblknbr) return;
+    // depth of every supernode in the elimination tree, roots at 1 (:46-62)
+    std::vector<int64_t> levels(cb, 0);
+    int64_t maxdepth = 0;
+    for (int64_t c = cb - 1; c >= 0; c--) {
+        levels[c] = o.treetab[c] < 0 ? 1 : levels[o.treetab[c]] + 1;
+        maxdepth = std::max(maxdepth, levels[c]);
+    }
+    // blocks facing each cblk, in increasing order of their owner (browtab)
+    std::vector<int64_t> fptr(cb + 1, 0);
+    for (int64_t c = 0; c < cb; c++)
+        for (int64_t b = s.cblk_bloknum[c] + 1; b < s.cblk_bloknum[c + 1]; b++) fptr[s.blok_fcblk[b] + 1]++;
+    for (int64_t c = 0; c < cb; c++) fptr[c + 1] += fptr[c];
+    std::vector<int64_t> fown(fptr[cb]), fblk(fptr[cb]);
+    {
+        std::vector<int64_t> pos(fptr.begin(), fptr.end() - 1);
+        for (int64_t c = 0; c < cb; c++)
+            for (int64_t b = s.cblk_bloknum[c] + 1; b < s.cblk_bloknum[c + 1]; b++) {
+                const int64_t f = s.blok_fcblk[b];
+                fown[pos[f]] = c;
+                fblk[pos[f]++] = b;
+            }
+    }
+    std::vector<int64_t> depthweight(std::max<int64_t>(maxdepth, 1));
+    for (int64_t c = 0; c < cb; c++) {
+        const int64_t fc = s.cblk_fcol[c], size = s.cblk_fcol[c + 1] - fc;
+        if (fc >= schurfcol) continue;
+        std::fill(depthweight.begin(), depthweight.end(), 0);
+        for (int64_t k = fptr[c]; k < fptr[c + 1]; k++)
+            depthweight[levels[fown[k]] - 1] += s.blok_lrow[fblk[k]] - s.blok_frow[fblk[k]] + 1;
+        // the split level, moved so that the upper set holds about a fifth
+        // of the contributions (:467-503)
+        int64_t lsl = split_level;
+        {
+            int sign = 0;
+            for (;;) {
+                int64_t up_total = 0, lw_total = 0, i = 0;
+                for (; i < lsl && i < maxdepth; i++) up_total += depthweight[i];
+                for (; i < maxdepth; i++) lw_total += depthweight[i];
+                if (lw_total < 5 * up_total && lw_total > 10 && up_total > 10 && sign <= 0) {
+                    lsl--;
+                    sign--;
+                    continue;
+                }
+                if (lw_total > 3 * up_total && lw_total > 10 && up_total > 10 && sign >= 0) {
+                    lsl++;
+                    sign++;
+                    continue;
+                }
+                break;
+            }
+        }
+        SetVec up(size), lw(size);
+        for (int64_t k = fptr[c]; k < fptr[c + 1]; k++) {
+            const int64_t b = fblk[k];
+            SetVec &v = (levels[fown[k]] <= lsl) ? up : lw;
+            for (int64_t r = s.blok_frow[b]; r <= s.blok_lrow[b]; r++) v[r - fc].push_back(fown[k]);
+        }
+        reorder_tsp(size, o.peritab.data() + o.rangtab[c], lw, up, stop);
+    }
     for (int64_t i = 0; i < o.n; i++) o.permtab[o.peritab[i]] = i;
 }
 
@@ -490,6 +736,52 @@ Symbol symbol_fax(const Graph &g, const Order &o) {
 // default cut search (computeSmallestSplit over blocks per line) is not
 // restated.  Every split piece keeps a dense block facing each later piece,
 // and every block facing a split supernode is cut at the new boundaries.
+// Cut rules of reference blend/splitsymbol.c: computeSmallestSplit (the
+// default, :195-241), computeSmallestSplit_max (PASTIX_BLEND_SPLIT=UPPER,
+// :265-299) and computeConstantSplit (PASTIX_BLEND_SPLIT=CONSTANT,
+// :301-313); nbpl points at the cblk's first column, step is the even cut,
+// the result is the last column of the piece relative to it.
+static int64_t split_smallest(const int64_t *nbpl, int64_t step, int64_t mn, int64_t mx, int64_t pct) {
+    const int64_t limit = (step * pct + 99) / 100;
+    if (step >= mx) return mx - 1;
+    const int64_t lavg = step - 1;
+    const int64_t lmin = std::max(lavg - limit - 1, mn), lmax = std::min(lavg + limit + 1, mx);
+    int64_t lcol = lavg, best = nbpl[lavg];
+    for (int64_t i = lavg + 1; i < lmax; i++)
+        if (nbpl[i] < best) {
+            lcol = i;
+            best = nbpl[i];
+        }
+    for (int64_t i = lavg - 1; i > lmin; i--)
+        if (nbpl[i] < best) {
+            lcol = i;
+            best = nbpl[i];
+        }
+    return lcol;
+}
+static int64_t split_smallest_upper(const int64_t *nbpl, int64_t step, int64_t mn, int64_t mx, int64_t pct) {
+    const int64_t limit = (step * pct + 99) / 100;
+    if (step >= mx) return mx - 1;
+    const int64_t lavg = step - 1;
+    const int64_t lmin = std::max(lavg - limit, mn), lmax = std::min(lavg + limit + 1, mx);
+    int64_t lcol = lmin, best = nbpl[lmin];
+    for (int64_t i = lmin; i < lmax; i++)
+        if (nbpl[i] <= best) {
+            lcol = i;
+            best = nbpl[i];
+        }
+    return lcol;
+}
+static int64_t split_constant(const int64_t *, int64_t step, int64_t, int64_t mx, int64_t) {
+    return step >= mx ? mx - 1 : step - 1;
+}
+
+// Reference blend/splitsymbol.c:79-100 (computeNbSplit) and splitSmart
+// (:316-441): each cblk wider than blcolmax is cut into pieces, the cut
+// before each piece chosen by the rule above from nblocksperline, the
+// number of symbol blocks spanning the boundary between a row and the next
+// (:104-139; counted once, then raised by pieces-1 on the rows of a split
+// cblk's off-diagonal blocks, which the split repeats).
 Symbol symbol_split(const Symbol &s, int64_t blcolmin, int64_t blcolmax) {
     const int64_t cb = s.cblknbr();
     auto nbsplit = [&](int64_t width) -> int64_t {
@@ -498,21 +790,48 @@ Symbol symbol_split(const Symbol &s, int64_t blcolmin, int64_t blcolmax) {
         if (nseq > 1 && width / nseq < blcolmin) nseq--;
         return std::max<int64_t>(nseq, 1);
     };
+    int64_t (*rule)(const int64_t *, int64_t, int64_t, int64_t, int64_t) = split_smallest;
+    if (const char *e = getenv("PASTIX_BLEND_SPLIT")) {
+        if (!strcmp(e, "CONSTANT")) rule = split_constant;
+        else if (!strcmp(e, "UPPER")) rule = split_smallest_upper;
+    }
+    int64_t pct = 10;
+    if (const char *e = getenv("PASTIX_BLEND_SPLIT_AUTORIZED_PERCENTAGE")) pct = atoll(e);
+    std::vector<int64_t> nbpl;  // per absolute row, from the first split cblk on
+    int64_t fsplit = -1;
     // New column partition.
     std::vector<int64_t> newf;               // first column of each new cblk
     std::vector<int64_t> first_new(cb + 1);  // first new cblk of each old cblk
     for (int64_t c = 0; c < cb; c++) {
         first_new[c] = (int64_t)newf.size();
-        int64_t f = s.cblk_fcol[c], l = s.cblk_fcol[c + 1] - 1;
+        const int64_t f = s.cblk_fcol[c], l = s.cblk_fcol[c + 1] - 1;
         int64_t width = l - f + 1;
-        int64_t fc = f;
-        while (fc <= l) {
-            int64_t nseq = nbsplit(width);
-            int64_t step = (nseq == 1) ? width : (width + nseq - 1) / nseq;
-            newf.push_back(fc);
-            fc += step;
-            width -= step;
+        if (nbsplit(width) <= 1) {
+            newf.push_back(f);
+            continue;
         }
+        if (fsplit < 0) {
+            fsplit = f;
+            nbpl.assign(s.n + 1, 0);
+            for (int64_t b = 0; b < s.bloknbr(); b++) {
+                if (s.blok_lrow[b] < fsplit) continue;
+                for (int64_t line = std::max(s.blok_frow[b], fsplit); line < s.blok_lrow[b]; line++) nbpl[line]++;
+            }
+        }
+        int64_t fc = f, pieces = 0;
+        while (fc <= l) {
+            const int64_t nseq = nbsplit(width);
+            newf.push_back(fc);
+            pieces++;
+            if (nseq == 1) break;
+            const int64_t step = (width + nseq - 1) / nseq;
+            const int64_t lwidth = width - blcolmin;
+            const int64_t lc = fc + rule(nbpl.data() + fc, step, blcolmin, lwidth, pct);
+            width -= lc - fc + 1;
+            fc = lc + 1;
+        }
+        for (int64_t b = s.cblk_bloknum[c] + 1; b < s.cblk_bloknum[c + 1]; b++)
+            for (int64_t line = s.blok_frow[b]; line < s.blok_lrow[b]; line++) nbpl[line] += pieces - 1;
     }
     first_new[cb] = (int64_t)newf.size();
     const int64_t ncb = (int64_t)newf.size();

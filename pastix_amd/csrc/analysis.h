@@ -69,9 +69,17 @@ Order order_nested_dissection(const Graph &g);
 // Supernode partition and tree for an ordering that has none
 // (reference order/order_find_supernodes.c, fundamental supernodes).
 void order_find_supernodes(const Graph &g, Order &o);
-// Breadth-first renumbering of supernodes by level
-// (reference order/order_apply_level_order.c:94-330, 1D branch).
-void order_apply_level_order(Order &o);
+// Breadth-first renumbering of supernodes by level (reference
+// order/order_apply_level_order.c:81-333): level_tasks2d < 0 puts the
+// supernodes of at least width_tasks2d columns, their brothers and their
+// ancestors' families in a queue of their own that takes the last numbers.
+void order_apply_level_order(Order &o, int64_t level_tasks2d, int64_t width_tasks2d);
+
+// Symbol reordering (reference symbol/symbol_reordering.c:617-660): the rows
+// of every supernode before schurfcol are reordered (o.peritab and permtab
+// change inside each supernode's range) to compact the blocks facing it;
+// s is the symbolic factorisation of o, before splitting.
+void symbol_reordering(const Symbol &s, Order &o, int64_t split_level, int64_t stop, int64_t schurfcol);
 
 // Supernodal symbolic factorisation with dense diagonal blocks
 // (reference symbol/symbol_fax_direct.c).

@@ -102,7 +102,7 @@ extern "C" void pastixInitWithAffinity(pastix_data_t **pastix_data, PASTIX_Comm,
     d->dparm = dparm;
     d->ext[PXA_IPARM_FACTOR_PRECISION] = -1;
     d->ext[PXA_IPARM_DEVICE] = 0;
-    d->ext[PXA_IPARM_LEVEL_ORDER] = 0;
+    d->ext[PXA_IPARM_LEVEL_ORDER] = 1;
     d->ext[PXA_IPARM_OVERLAP] = 1;
     d->steps = STEP_INIT;
     *pastix_data = d;
@@ -204,6 +204,16 @@ int pxa_api::spm_graph(const spmatrix_t *spm, pxa::Graph &g) {
     return PASTIX_SUCCESS;
 }
 
+// Numbering base of the input (spmFindBase): the first pointer of CSC/CSR,
+// the smallest index of IJV.
+int64_t pxa_api::spm_baseval(const spmatrix_t *spm) {
+    if (spm->fmttype == SpmCSC) return spm->colptr[0];
+    if (spm->fmttype == SpmCSR) return spm->rowptr[0];
+    int64_t base = INT64_MAX;
+    for (int64_t k = 0; k < spm->nnz; k++) base = std::min<int64_t>(base, std::min<int64_t>(spm->colptr[k], spm->rowptr[k]));
+    return spm->nnz > 0 ? base : 0;
+}
+
 // CSC (sorted, duplicates summed) of the n x n matrix given by triplets,
 // optionally permuted: B(perm[i], perm[j]) = A(i, j).
 void pxa_api::triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vector<int64_t> &colptr,
@@ -280,21 +290,71 @@ extern "C" int pastix_subtask_order(pastix_data_t *d, const spmatrix_t *spm, pas
     d->nnz_in = spm->gnnzexp > 0 ? spm->gnnzexp : spm->nnz;
     d->mtxtype = spm->mtxtype;
     d->flttype = spm->flttype;
+    // Schur unknowns (reference order/pastix_subtask_order.c:198-230,
+    // graph/graph_isolate.c): the ordering runs on the graph without them,
+    // and they are appended last, in increasing index order, as one more
+    // supernode that the roots of the elimination forest feed
+    // (order/order_add_isolate.c:55-130)
+    std::vector<int64_t> keep;  // old index of every remaining vertex, increasing
+    std::vector<int64_t> iso;   // the Schur unknowns, increasing
+    pxa::Graph sub;
+    const pxa::Graph *gord = &d->graph;
+    if (!d->schur_list.empty()) {
+        const int64_t base = spm_baseval(spm);
+        std::vector<char> mark(n, 0);
+        for (pastix_int_t v : d->schur_list) {
+            const int64_t i = v - base;
+            if (i < 0 || i >= n || mark[i]) { err_print("pastix_subtask_order", "invalid or repeated Schur unknown"); return PASTIX_ERR_BADPARAMETER; }
+            mark[i] = 1;
+        }
+        if ((int64_t)d->schur_list.size() >= n) { err_print("pastix_subtask_order", "the Schur complement must leave unknowns to factor"); return PASTIX_ERR_BADPARAMETER; }
+        std::vector<int64_t> newid(n, -1);
+        for (int64_t i = 0; i < n; i++) {
+            if (mark[i]) iso.push_back(i);
+            else {
+                newid[i] = (int64_t)keep.size();
+                keep.push_back(i);
+            }
+        }
+        sub.n = (int64_t)keep.size();
+        sub.colptr.assign(sub.n + 1, 0);
+        for (int64_t k = 0; k < sub.n; k++) {
+            const int64_t j = keep[k];
+            for (int64_t e = d->graph.colptr[j]; e < d->graph.colptr[j + 1]; e++)
+                if (newid[d->graph.rows[e]] >= 0) sub.rows.push_back(newid[d->graph.rows[e]]);
+            sub.colptr[k + 1] = (int64_t)sub.rows.size();
+        }
+        gord = &sub;
+    }
+    const int64_t no = gord->n;
     pxa::Order o;
-    o.n = n;
+    o.n = no;
     const pastix_int_t ord = d->iparm[IPARM_ORDERING];
+    pastix_order_t loaded;
+    memset(&loaded, 0, sizeof(loaded));
+    if (ord == PastixOrderPersonal && (d->iparm[IPARM_IO_STRATEGY] & PastixIOLoad)) {
+        // reference order/pastix_subtask_order.c:308-315: the ordering file
+        const int lrc = pastixOrderLoad(d, &loaded);
+        if (lrc) return lrc;
+        myorder = &loaded;
+    }
     if (ord == PastixOrderPersonal) {
-        o.permtab.resize(n);
-        o.peritab.resize(n);
+        o.permtab.resize(no);
+        o.peritab.resize(no);
         const int64_t base = (myorder && myorder->baseval) ? myorder->baseval : 0;
+        if (myorder && (myorder->permtab || myorder->peritab) && myorder->vertnbr != no) {
+            err_print("pastix_subtask_order", "the given ordering does not match the size of the graph to order (the matrix without its Schur unknowns)");
+            pastixOrderExit(&loaded);
+            return PASTIX_ERR_BADPARAMETER;
+        }
         if (myorder && myorder->permtab) {
-            for (int64_t i = 0; i < n; i++) o.permtab[i] = myorder->permtab[i] - base;
-            for (int64_t i = 0; i < n; i++) o.peritab[o.permtab[i]] = i;
+            for (int64_t i = 0; i < no; i++) o.permtab[i] = myorder->permtab[i] - base;
+            for (int64_t i = 0; i < no; i++) o.peritab[o.permtab[i]] = i;
         } else if (myorder && myorder->peritab) {
-            for (int64_t i = 0; i < n; i++) o.peritab[i] = myorder->peritab[i] - base;
-            for (int64_t i = 0; i < n; i++) o.permtab[o.peritab[i]] = i;
+            for (int64_t i = 0; i < no; i++) o.peritab[i] = myorder->peritab[i] - base;
+            for (int64_t i = 0; i < no; i++) o.permtab[o.peritab[i]] = i;
         } else {
-            for (int64_t i = 0; i < n; i++) o.permtab[i] = o.peritab[i] = i;
+            for (int64_t i = 0; i < no; i++) o.permtab[i] = o.peritab[i] = i;
         }
         if (myorder && myorder->rangtab && myorder->treetab && myorder->cblknbr > 0) {
             o.cblknbr = myorder->cblknbr;
@@ -304,18 +364,45 @@ extern "C" int pastix_subtask_order(pastix_data_t *d, const spmatrix_t *spm, pas
             for (int64_t c = 0; c < o.cblknbr; c++)
                 o.treetab[c] = myorder->treetab[c] < 0 ? -1 : myorder->treetab[c] - base;
         } else {
-            pxa::order_find_supernodes(d->graph, o);
+            pxa::order_find_supernodes(*gord, o);
         }
+        pastixOrderExit(&loaded);
     } else {
         // Scotch / METIS are not part of this build: graph nested dissection.
         if (d->iparm[IPARM_VERBOSE] > PastixVerboseNo)
             fprintf(stderr, "pastix_subtask_order: ordering library not available, using level-structure nested dissection\n");
-        o = pxa::order_nested_dissection(d->graph);
+        o = pxa::order_nested_dissection(*gord);
     }
     std::string err;
     if (!pxa::check_order(o, err)) { err_print("pastix_subtask_order", err.c_str()); return PASTIX_ERR_BADPARAMETER; }
-    if (d->ext[PXA_IPARM_LEVEL_ORDER]) pxa::order_apply_level_order(o);
+    // reference order/pastix_subtask_order.c:459-461, on every ordering
+    if (d->ext[PXA_IPARM_LEVEL_ORDER])
+        pxa::order_apply_level_order(o, d->iparm[IPARM_TASKS2D_LEVEL], d->iparm[IPARM_TASKS2D_WIDTH]);
+    if (!iso.empty()) {
+        pxa::Order f;
+        f.n = n;
+        f.permtab.assign(n, 0);
+        f.peritab.assign(n, 0);
+        for (int64_t k = 0; k < no; k++) f.permtab[keep[k]] = o.permtab[k];
+        for (size_t k = 0; k < iso.size(); k++) f.permtab[iso[k]] = no + (int64_t)k;
+        for (int64_t i = 0; i < n; i++) f.peritab[f.permtab[i]] = i;
+        f.cblknbr = o.cblknbr + 1;
+        f.rangtab = o.rangtab;
+        f.rangtab.push_back(n);
+        f.treetab = o.treetab;
+        for (int64_t &t : f.treetab)
+            if (t < 0) t = o.cblknbr;
+        f.treetab.push_back(-1);
+        if (!pxa::check_order(f, err)) { err_print("pastix_subtask_order", err.c_str()); return PASTIX_ERR_INTERNAL; }
+        o = std::move(f);
+    }
+    d->schurfcol = no;
     d->order = std::move(o);
+    // reference order/pastix_subtask_order.c:548-552
+    if (d->iparm[IPARM_IO_STRATEGY] & PastixIOSave) {
+        const int src = save_analysis_file(d, "ordergen", true);
+        if (src) return src;
+    }
     d->steps |= STEP_ORDERING;
     d->steps &= ~(STEP_SYMBFACT | STEP_ANALYSE | STEP_CSC2BCSC | STEP_BCSC2CTAB | STEP_NUMFACT | STEP_SOLVE | STEP_REFINE);
     d->engine.reset();
@@ -326,19 +413,57 @@ extern "C" int pastix_subtask_order(pastix_data_t *d, const spmatrix_t *spm, pas
 extern "C" int pastix_subtask_symbfact(pastix_data_t *d) {
     if (!d) return PASTIX_ERR_BADPARAMETER;
     if (!(d->steps & STEP_ORDERING)) { err_print("pastix_subtask_symbfact", "pastix_subtask_order() has to be called before"); return PASTIX_ERR_BADPARAMETER; }
-    d->symbol = pxa::symbol_fax(d->graph, d->order);
+    if (d->iparm[IPARM_IO_STRATEGY] & PastixIOLoad) {
+        // reference symbol/pastix_subtask_symbfact.c:159-166: "symbname"
+        FILE *f = fopen("symbname", "r");
+        if (!f) { err_print("pastix_subtask_symbfact", "cannot open symbname"); return PASTIX_ERR_FILE; }
+        const int lrc = symbol_read(f, d->symbol);
+        fclose(f);
+        if (lrc || d->symbol.n != d->n) { err_print("pastix_subtask_symbfact", "bad symbol file"); return PASTIX_ERR_FILE; }
+    } else {
+        d->symbol = pxa::symbol_fax(d->graph, d->order);
+    }
     std::string err;
     if (!pxa::check_symbol(d->symbol, err)) { err_print("pastix_subtask_symbfact", err.c_str()); return PASTIX_ERR_INTERNAL; }
     d->iparm[IPARM_NNZEROS] = pxa::symbol_nnz(d->symbol);
+    // reference symbol/pastix_subtask_symbfact.c:287-297
+    if (d->iparm[IPARM_IO_STRATEGY] & PastixIOSave) {
+        const int src = save_analysis_file(d, "symbgen", false);
+        if (src) return src;
+    }
     d->steps |= STEP_SYMBFACT;
     return PASTIX_SUCCESS;
 }
 
-// Reference symbol/pastix_subtask_reordering.c reorders rows inside
-// supernodes; this engine keeps the ordering as computed (documented divergence).
+// Reference symbol/pastix_subtask_reordering.c:59-135: the rows of every
+// supernode are reordered to compact the blocks facing it, and the symbolic
+// factorisation is computed again for the new ordering.
 extern "C" int pastix_subtask_reordering(pastix_data_t *d) {
     if (!d) return PASTIX_ERR_BADPARAMETER;
     if (!(d->steps & STEP_SYMBFACT)) { err_print("pastix_subtask_reordering", "pastix_subtask_symbfact() has to be called before"); return PASTIX_ERR_BADPARAMETER; }
+    // files saved by an analysis hold the reordered ordering and its symbol
+    if (d->iparm[IPARM_IO_STRATEGY] & PastixIOLoad) return PASTIX_SUCCESS;
+    pxa::symbol_reordering(d->symbol, d->order, d->iparm[IPARM_REORDERING_SPLIT], d->iparm[IPARM_REORDERING_STOP],
+                           d->schurfcol);
+    std::string err;
+    if (!pxa::check_order(d->order, err)) { err_print("pastix_subtask_reordering", err.c_str()); return PASTIX_ERR_INTERNAL; }
+    // reference symbol/pastix_subtask_reordering.c:100-107
+    if (d->iparm[IPARM_IO_STRATEGY] & PastixIOSave) {
+        const int src = save_analysis_file(d, "ordergen", true);
+        if (src) return src;
+    }
+    d->symbol = pxa::symbol_fax(d->graph, d->order);
+    if (!pxa::check_symbol(d->symbol, err)) { err_print("pastix_subtask_reordering", err.c_str()); return PASTIX_ERR_INTERNAL; }
+    d->iparm[IPARM_NNZEROS] = pxa::symbol_nnz(d->symbol);
+    // the symbolic factorisation runs again and saves its file again
+    // (symbol/pastix_subtask_reordering.c:116, pastix_subtask_symbfact.c:287-297)
+    if (d->iparm[IPARM_IO_STRATEGY] & PastixIOSave) {
+        const int src = save_analysis_file(d, "symbgen", false);
+        if (src) return src;
+    }
+    // the permuted matrix follows the new ordering
+    d->engine.reset();
+    d->bcsc = pxa::HostBcsc();
     return PASTIX_SUCCESS;
 }
 
@@ -352,7 +477,10 @@ extern "C" int pastix_subtask_blend(pastix_data_t *d) {
     std::string err;
     if (!pxa::check_symbol(d->symbol, err)) { err_print("pastix_subtask_blend", err.c_str()); return PASTIX_ERR_INTERNAL; }
     d->layout = pxa::build_layout(d->symbol);
-    d->plan = pxa::build_plan(d->symbol, d->layout);
+    // cblks from the Schur column on (reference CBLK_IN_SCHUR, blend/cand.c:725)
+    int64_t sc = d->symbol.cblknbr();
+    while (sc > 0 && d->symbol.cblk_fcol[sc - 1] >= d->schurfcol) sc--;
+    d->plan = pxa::build_plan(d->symbol, d->layout, sc);
     int facto = (int)d->iparm[IPARM_FACTORIZATION];
     int ft = (facto == PastixFactGETRF) ? 2 : is_ldlt(d) ? 1 : 0;
     d->dparm[DPARM_FACT_THFLOPS] = pxa::symbol_thflops(d->symbol, ft);
@@ -675,9 +803,23 @@ extern "C" int pastix_subtask_applyorder(pastix_data_t *d, pastix_coeftype_t flt
     return PASTIX_SUCCESS;
 }
 
+// Schur solve mode of the next solve (IPARM_SCHUR_SOLV_MODE, reference
+// include/pastix/api.h:212-236): Local and Interface; the Schur mode would
+// solve with the Schur part's coefficients, which numfact leaves unfactored.
+bool pxa_api::schur_mode(pastix_data_t *d, const char *fn) {
+    const pastix_int_t m = d->iparm[IPARM_SCHUR_SOLV_MODE];
+    if (d->engine) d->engine->schur_interface = (m == PastixSolvModeInterface);
+    if (d->schurfcol < d->n && m == PastixSolvModeSchur) {
+        err_print(fn, "the Schur complement is not factored: use PastixSolvModeLocal or PastixSolvModeInterface");
+        return false;
+    }
+    return true;
+}
+
 extern "C" int pastix_subtask_solve(pastix_data_t *d, pastix_int_t nrhs, void *b, pastix_int_t ldb) {
     if (!d || !b || nrhs < 0 || ldb < d->n) return PASTIX_ERR_BADPARAMETER;
     if (!(d->steps & STEP_NUMFACT)) { err_print("pastix_subtask_solve", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
+    if (!schur_mode(d, "pastix_subtask_solve")) return PASTIX_ERR_BADPARAMETER;
     // b is already in the permuted ordering: identity permutation for the engine
     std::vector<int64_t> id(d->n);
     for (int64_t i = 0; i < d->n; i++) id[i] = i;
@@ -691,6 +833,7 @@ extern "C" int pastix_task_solve(pastix_data_t *d, pastix_int_t nrhs, void *b, p
     if (!d) { err_print("pastix_task_solve", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
     if (!(d->steps & STEP_NUMFACT)) { err_print("pastix_task_solve", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
     if (!b || nrhs < 0 || ldb < d->n) return PASTIX_ERR_BADPARAMETER;
+    if (!schur_mode(d, "pastix_task_solve")) return PASTIX_ERR_BADPARAMETER;
     if (nrhs == 0) return PASTIX_SUCCESS;
     double secs = 0;
     if (d->engine->solve_host(d->order.permtab.data(), d->flttype, b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
@@ -727,6 +870,7 @@ extern "C" int pastix_subtask_trsm(pastix_data_t *d, pastix_coeftype_t flttype, 
         err_print("pastix_subtask_trsm", "this side/uplo/trans/diag is not a sweep of the factor");
         return PASTIX_ERR_BADPARAMETER;
     }
+    if (!schur_mode(d, "pastix_subtask_trsm")) return PASTIX_ERR_BADPARAMETER;
     if (nrhs == 0) return PASTIX_SUCCESS;
     double secs = 0;
     if (d->engine->solve_host_part(part, flttype, b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
@@ -741,6 +885,7 @@ extern "C" int pastix_subtask_diag(pastix_data_t *d, pastix_coeftype_t flttype, 
     if (!b) { err_print("pastix_subtask_diag", "wrong b parameter"); return PASTIX_ERR_BADPARAMETER; }
     if (!(d->steps & STEP_NUMFACT)) { err_print("pastix_subtask_diag", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function"); return PASTIX_ERR_BADPARAMETER; }
     if (nrhs < 0 || ldb < d->n || (flttype != SpmDouble && flttype != SpmFloat)) return PASTIX_ERR_BADPARAMETER;
+    if (!schur_mode(d, "pastix_subtask_diag")) return PASTIX_ERR_BADPARAMETER;
     if (nrhs == 0) return PASTIX_SUCCESS;
     double secs = 0;
     if (d->engine->solve_host_part(4, flttype, b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
@@ -753,6 +898,11 @@ extern "C" int pastix_subtask_diag(pastix_data_t *d, pastix_coeftype_t flttype, 
 static int refine_common(pastix_data_t *d, pastix_int_t nrhs, const void *b, pastix_int_t ldb, void *x,
                          pastix_int_t ldx, bool permute) {
     const int64_t n = d->n;
+    if (d->schurfcol < n) {
+        err_print("pastix_task_refine", "refinement needs the whole factor: not available with a Schur complement");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    d->engine->schur_interface = false;
     if (nrhs <= 0) return PASTIX_SUCCESS;
     const bool isf = d->flttype == SpmFloat;
     // FP64 copies in the caller's ordering; the permutation runs on the device
@@ -816,10 +966,74 @@ extern "C" int pastix(pastix_data_t **, PASTIX_Comm, pastix_int_t, pastix_int_t 
     err_print("pastix", "the former single-call interface is not provided; use the task interface");
     return PASTIX_ERR_NOTIMPLEMENTED;
 }
-extern "C" void pastixSetSchurUnknownList(pastix_data_t *, pastix_int_t, const pastix_int_t *) {
-    err_print("pastixSetSchurUnknownList", "Schur complement is not provided by this engine");
+// Reference sopalin/schur.c:50-58: the list is kept as given (the input's
+// numbering base); the next pastix_subtask_order orders these unknowns last.
+extern "C" void pastixSetSchurUnknownList(pastix_data_t *d, pastix_int_t n, const pastix_int_t *list) {
+    if (!d) return;
+    if (n > 0 && list) d->schur_list.assign(list, list + n);
 }
-extern "C" int pastixGetSchur(const pastix_data_t *, void *, pastix_int_t) { return PASTIX_ERR_NOTIMPLEMENTED; }
+// Reference sopalin/schur.c:90-148 with coeftab_zgetschur (sopalin/coeftab_z.c:
+// 413-437) and cpucblk_zgetschur_fr (kernels/cpucblk_zschur.c:113-154): the
+// Schur cblks' panels, never factored, are copied block by block into the
+// schur_n x schur_n array S (leading dimension lds), in the order the
+// analysis gave the Schur unknowns (increasing input index); for LU the
+// transposed U panels are added (S = L + U^T, the sum the factorisation's
+// geadd forms).  For LL^t and LDL^t only the lower triangle is defined (the
+// reference leaves partial sums above the diagonal); it is zero here.  The
+// values are in the arithmetic of the input.
+extern "C" int pastixGetSchur(const pastix_data_t *d, void *S, pastix_int_t lds) {
+    if (!d) { err_print("pastix_getSchur", "wrong pastix_data parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!S) { err_print("pastix_getSchur", "S parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (lds <= 0) { err_print("pastix_getSchur", "lds parameter"); return PASTIX_ERR_BADPARAMETER; }
+    if (!(d->steps & STEP_NUMFACT) || !d->engine) {
+        err_print("pastix_getSchur", "All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    const pxa::Symbol &sy = d->symbol;
+    const pxa::Layout &L = d->layout;
+    const int64_t f0 = d->schurfcol, ns = d->n - f0;
+    if (ns <= 0) { err_print("pastix_getSchur", "no Schur complement was requested"); return PASTIX_ERR_BADPARAMETER; }
+    if (lds < ns) { err_print("pastix_getSchur", "lds parameter"); return PASTIX_ERR_BADPARAMETER; }
+    int64_t c0 = sy.cblknbr();
+    while (c0 > 0 && sy.cblk_fcol[c0 - 1] >= f0) c0--;
+    const int64_t off = L.cblk_coef[c0], cnt = L.coefnbr - off;
+    const bool lu = d->engine->factotype() == 2 && !is_ldlt(d);
+    std::vector<double> hl(cnt), hu(lu ? cnt : 0);
+    if (d->engine->prec() == SpmFloat) {
+        std::vector<float> fl(cnt), fu(lu ? cnt : 0);
+        if (d->engine->download_coef_range(off, cnt, fl.data(), lu ? fu.data() : nullptr)) return PASTIX_ERR_INTERNAL;
+        std::copy(fl.begin(), fl.end(), hl.begin());
+        std::copy(fu.begin(), fu.end(), hu.begin());
+    } else if (d->engine->download_coef_range(off, cnt, hl.data(), lu ? hu.data() : nullptr)) {
+        return PASTIX_ERR_INTERNAL;
+    }
+    std::vector<double> dense((size_t)ns * ns, 0.0);
+    for (int64_t c = c0; c < sy.cblknbr(); c++) {
+        const int64_t fc = sy.cblk_fcol[c], K = sy.cblk_fcol[c + 1] - fc, st = L.cblk_stride[c];
+        const int64_t base = L.cblk_coef[c] - off;
+        for (int64_t b = sy.cblk_bloknum[c]; b < sy.cblk_bloknum[c + 1]; b++) {
+            const int64_t fr = sy.blok_frow[b], nr = sy.blok_lrow[b] - fr + 1, ro = L.blok_roff[b];
+            for (int64_t j = 0; j < K; j++)
+                for (int64_t i = 0; i < nr; i++) {
+                    const int64_t gi = fr + i - f0, gj = fc + j - f0;
+                    const double v = hl[base + ro + i + j * st];
+                    if (lu) {
+                        dense[gi + gj * ns] += v;
+                        dense[gj + gi * ns] += hu[base + ro + i + j * st];
+                    } else if (gi >= gj) {
+                        dense[gi + gj * ns] = v;
+                    }
+                }
+        }
+    }
+    for (int64_t j = 0; j < ns; j++)
+        for (int64_t i = 0; i < ns; i++) {
+            const double v = dense[i + j * ns];
+            if (d->flttype == SpmFloat) ((float *)S)[i + j * lds] = (float)v;
+            else ((double *)S)[i + j * lds] = v;
+        }
+    return PASTIX_SUCCESS;
+}
 extern "C" void pastixExpand(const pastix_data_t *, spmatrix_t *) {}
 // Diagonal of the factor, permuted order, at stride incD, in the instance's
 // arithmetic (reference sopalin/diag.c:55-102, coeftab_zgetdiag
@@ -934,20 +1148,28 @@ extern "C" int pastixOrderGrid(pastix_order_t **myorder, pastix_int_t nx, pastix
     for (int64_t c = 0; c < o.cblknbr; c++) r->treetab[c] = o.treetab[c];
     return PASTIX_SUCCESS;
 }
-extern "C" int pastixOrderApplyLevelOrder(pastix_order_t *o, pastix_int_t, pastix_int_t) {
-    if (!o || !o->rangtab || !o->treetab) return PASTIX_ERR_BADPARAMETER;
+// Reference order/order_apply_level_order.c:81-333, in the order's base.
+extern "C" int pastixOrderApplyLevelOrder(pastix_order_t *o, pastix_int_t level_tasks2d,
+                                          pastix_int_t width_tasks2d) {
+    if (!o || !o->rangtab || !o->treetab || !o->permtab || !o->peritab || o->cblknbr < 0)
+        return PASTIX_ERR_BADPARAMETER;
+    const pastix_int_t b = o->baseval;
     pxa::Order t;
     t.n = o->vertnbr;
     t.cblknbr = o->cblknbr;
-    t.permtab.assign(o->permtab, o->permtab + o->vertnbr);
-    t.peritab.assign(o->peritab, o->peritab + o->vertnbr);
-    t.rangtab.assign(o->rangtab, o->rangtab + o->cblknbr + 1);
-    t.treetab.assign(o->treetab, o->treetab + o->cblknbr);
-    pxa::order_apply_level_order(t);
-    std::copy(t.permtab.begin(), t.permtab.end(), o->permtab);
-    std::copy(t.peritab.begin(), t.peritab.end(), o->peritab);
-    std::copy(t.rangtab.begin(), t.rangtab.end(), o->rangtab);
-    std::copy(t.treetab.begin(), t.treetab.end(), o->treetab);
+    for (pastix_int_t i = 0; i < o->vertnbr; i++) {
+        t.permtab.push_back(o->permtab[i] - b);
+        t.peritab.push_back(o->peritab[i] - b);
+    }
+    for (pastix_int_t c = 0; c <= o->cblknbr; c++) t.rangtab.push_back(o->rangtab[c] - b);
+    for (pastix_int_t c = 0; c < o->cblknbr; c++) t.treetab.push_back(o->treetab[c] < b ? -1 : o->treetab[c] - b);
+    pxa::order_apply_level_order(t, level_tasks2d, width_tasks2d);
+    for (pastix_int_t i = 0; i < o->vertnbr; i++) {
+        o->permtab[i] = t.permtab[i] + b;
+        o->peritab[i] = t.peritab[i] + b;
+    }
+    for (pastix_int_t c = 0; c <= o->cblknbr; c++) o->rangtab[c] = t.rangtab[c] + b;
+    for (pastix_int_t c = 0; c < o->cblknbr; c++) o->treetab[c] = t.treetab[c] < 0 ? -1 : t.treetab[c] + b;
     return PASTIX_SUCCESS;
 }
 
@@ -1008,6 +1230,7 @@ extern "C" int pastix_amd_coeftab_export(const pastix_data_t *d, void *L, void *
 extern "C" int pastix_amd_solve_device(pastix_data_t *d, pastix_int_t nrhs, void *d_b, pastix_int_t ldb) {
     if (!d || !d_b || nrhs < 0 || ldb < d->n) return PASTIX_ERR_BADPARAMETER;
     if (!(d->steps & STEP_NUMFACT)) return PASTIX_ERR_BADPARAMETER;
+    if (!schur_mode(d, "pastix_amd_solve_device")) return PASTIX_ERR_BADPARAMETER;
     if (nrhs == 0) return PASTIX_SUCCESS;
     double secs = 0;
     if (d->engine->solve_user_device(d_b, ldb, (int)nrhs, &secs)) return PASTIX_ERR_INTERNAL;
@@ -1018,6 +1241,11 @@ static int refine_device(pastix_data_t *d, pastix_int_t nrhs, const double *d_b,
                          pastix_int_t ldx, bool initial_solve) {
     if (!d || !d_b || !d_x || nrhs < 0 || ldb < d->n || ldx < d->n) return PASTIX_ERR_BADPARAMETER;
     if (!(d->steps & STEP_NUMFACT) || !d->engine) return PASTIX_ERR_BADPARAMETER;
+    if (d->schurfcol < d->n) {
+        err_print("pastix_amd_refine_device", "refinement needs the whole factor: not available with a Schur complement");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    d->engine->schur_interface = false;
     if (nrhs == 0) return PASTIX_SUCCESS;
     const double t0 = wall();
     int it = 0;

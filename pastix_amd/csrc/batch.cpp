@@ -205,11 +205,18 @@ bool rhs_ok(const pastix_data_t *d, const pastix_int_t *nrhs, const void *const 
 double *pxa_api::device_block(pastix_data_t *d, int64_t count) {
     pxa::Engine &E = *d->engine;
     if (d->batch_dev_owner != &E || d->batch_dev_cap < count) {
+        if (d->batch_dev_owner == &E) E.release(d->batch_dev);
         d->batch_dev = (double *)E.alloc((size_t)count * sizeof(double));
         d->batch_dev_cap = d->batch_dev ? count : 0;
         d->batch_dev_owner = &E;
     }
     return d->batch_dev;
+}
+
+void pxa_api::release_device_block(pastix_data_t *d) {
+    if (d->engine && d->batch_dev_owner == d->engine.get()) d->engine->release(d->batch_dev);
+    d->batch_dev = nullptr;
+    d->batch_dev_cap = 0;
 }
 
 int pxa_api::refine_method(const pastix_data_t *d) {
@@ -264,6 +271,10 @@ extern "C" int pastix_amd_batch_analyze(pastix_data_t *d, pastix_int_t nsys, con
         rc = system_order(spms[s], orders ? orders[s] : nullptr, os[s]);
         if (rc) return rc;
         with_tree = with_tree && os[s].cblknbr > 0;
+        // each system's level order, as its own analysis would apply it; the
+        // union's would interleave the systems' supernodes
+        if (d->ext[PXA_IPARM_LEVEL_ORDER] && os[s].cblknbr > 0)
+            pxa::order_apply_level_order(os[s], d->iparm[IPARM_TASKS2D_LEVEL], d->iparm[IPARM_TASKS2D_WIDTH]);
     }
     rang.push_back(0);
     for (pastix_int_t s = 0; s < nsys; s++) {
@@ -286,10 +297,12 @@ extern "C" int pastix_amd_batch_analyze(pastix_data_t *d, pastix_int_t nsys, con
         uo.rangtab = rang.data();
         uo.treetab = tree.data();
     }
-    const pastix_int_t saved = d->iparm[IPARM_ORDERING];
+    const pastix_int_t saved = d->iparm[IPARM_ORDERING], saved_lo = d->ext[PXA_IPARM_LEVEL_ORDER];
     d->iparm[IPARM_ORDERING] = PastixOrderPersonal;
+    d->ext[PXA_IPARM_LEVEL_ORDER] = 0;
     rc = pastix_subtask_order(d, &d->u_spm, &uo);
     d->iparm[IPARM_ORDERING] = saved;
+    d->ext[PXA_IPARM_LEVEL_ORDER] = saved_lo;
     if (rc) return rc;
     if ((rc = pastix_subtask_symbfact(d)) || (rc = pastix_subtask_reordering(d)) || (rc = pastix_subtask_blend(d)))
         return rc;

@@ -119,6 +119,10 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     S_.browtab = upload_as<int32_t>(*this, L.browtab);
 
     lv_cblk_ptr_ = P.lv_cblk_ptr;
+    lv_fact_cnt_ = P.lv_fact_cnt;
+    schurfcol_ = (P.schur_cblk < cblknbr_) ? s.cblk_fcol[P.schur_cblk] : n_;
+    d_fact_cblks_ = upload(*this, P.fact_cblks);
+    n_fact_ = (int)P.fact_cblks.size();
     lv_upd_ptr_ = P.lv_upd_ptr;
     lv_upd_split_ = P.lv_upd_split;
     lv_upd_dnext_ = P.lv_upd_dnext;
@@ -236,6 +240,16 @@ int Engine::download_diag(double *h) {
     return 0;
 }
 
+int Engine::download_coef_range(int64_t off, int64_t count, void *hL, void *hU) {
+    if (off < 0 || count < 0 || off + count > coefnbr_) return fail("coefficient range outside the table");
+    const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
+    if (hL) HIPCHK(hipMemcpyAsync(hL, (char *)d_L_ + off * fsz, count * fsz, hipMemcpyDeviceToHost, stream_));
+    if (hU && facto_ == 2)
+        HIPCHK(hipMemcpyAsync(hU, (char *)d_U_ + off * fsz, count * fsz, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
 int Engine::download_coef(void *hL, void *hU) {
     const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
     if (hL) HIPCHK(hipMemcpyAsync(hL, d_L_, coefnbr_ * fsz, hipMemcpyDeviceToHost, stream_));
@@ -279,7 +293,8 @@ template <typename T> int Engine::factorize_t(double thr) {
     for (size_t lv = 0; lv < nl; lv++) {
         int pk = prof_begin(stream_);
         if (facto_ == 2) {
-            const int c0 = lv_cblk_ptr_[lv], nc = lv_cblk_ptr_[lv + 1] - c0;
+            // the level's factored cblks (the Schur ones keep their sums)
+            const int c0 = lv_cblk_ptr_[lv], nc = lv_fact_cnt_[lv];
             launch_lu_square_merge<T>(S_, d_lv_cblks_ + c0, nc, L, U, stream_);
         }
         // blocked panel: 64-column steps, each over every cblk of the level
@@ -390,7 +405,7 @@ template <typename T> int Engine::factorize_t(double thr) {
         // from U^T kept in the U panel's square, then the reference layout
         launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, L, W1, (T *)nullptr, stream_);
         launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, U, W2, (T *)d_WB_, stream_);
-        launch_lu_square_layout<T>(S_, d_lv_cblks_, (int)cblknbr_, L, U, stream_);
+        launch_lu_square_layout<T>(S_, d_fact_cblks_, n_fact_, L, U, stream_);
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -459,7 +474,7 @@ int Engine::enqueue_solve(void *dx, int64_t ldx, int nrhs, bool user) {
 
 hipGraphExec_t Engine::solve_graph(void *dx, int64_t ldx, int nrhs, bool user) {
     for (const SolveGraph &g : solve_graphs_)
-        if (g.ptr == dx && g.ld == ldx && g.nrhs == nrhs && g.user == user) return g.exec;
+        if (g.ptr == dx && g.ld == ldx && g.nrhs == nrhs && g.user == user && g.iface == schur_interface) return g.exec;
     if (solve_graphs_.size() >= 16) {
         (void)hipGraphExecDestroy(solve_graphs_.front().exec);
         solve_graphs_.erase(solve_graphs_.begin());
@@ -480,7 +495,7 @@ hipGraphExec_t Engine::solve_graph(void *dx, int64_t ldx, int nrhs, bool user) {
         fail(std::string("solve graph instantiation: ") + hipGetErrorString(ei));
         return nullptr;
     }
-    solve_graphs_.push_back(SolveGraph{dx, ldx, nrhs, user, ex});
+    solve_graphs_.push_back(SolveGraph{dx, ldx, nrhs, user, schur_interface, ex});
     return ex;
 }
 
@@ -490,9 +505,13 @@ int Engine::solve_user_device(void *db, int64_t ldb, int nrhs, double *seconds) 
     if (!d_perm_) return fail("permutation not uploaded");
     const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
     if (rhs_cap_ < (int64_t)nrhs * n_) {
+        release(d_rhs_);
         rhs_cap_ = (int64_t)nrhs * n_;
         d_rhs_ = alloc(rhs_cap_ * fsz);
-        if (!d_rhs_) return -1;
+        if (!d_rhs_) {
+            rhs_cap_ = 0;
+            return -1;
+        }
     }
     HIPCHK(hipEventRecord(ev0_, stream_));
     if (use_graph) {
@@ -597,9 +616,15 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
         // backward only, 3 backward after x *= D (LDL^t: L^{-t} = U^{-1} D),
         // 4 x /= diag(factor)
         if (part_ == 4) {
-            launch_diag_scale<T>(n_, nr, d_diagpos_, L, true, xx, ldx, stream_);
+            launch_diag_scale<T>(schurfcol_, nr, d_diagpos_, L, true, xx, ldx, stream_);
             continue;
         }
+        // Schur rows [sc, n): not solved; the forward sweep leaves them as
+        // they are (Local) or subtracts the factored part's products
+        // (Interface); the backward sweep ignores them (Local: read as zero,
+        // then restored) or reads them as the Schur solution (Interface)
+        const int64_t sc = schurfcol_;
+        const bool schur = sc < n_;
         if (part_ == 0 || part_ == 1) {
             for (int lv = 0; lv < nl; lv++) {
                 const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
@@ -608,19 +633,27 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
                                    lv_maxw_[lv], stream_);
                 launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, d_ydst_, ybuf, ldy, lv_maxw_[lv], stream_);
             }
+            if (schur)
+                launch_schur_rows<T>(sc, n_, nr, schur_interface ? 1 : 0, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz,
+                                     stream_);
         }
         if (part_ == 1) {
             launch_copy2d<T>(n_, nr, z, ldz, xx, ldx, stream_);
             continue;
         }
-        if (part_ == 3) launch_diag_scale<T>(n_, nr, d_diagpos_, L, false, xx, ldx, stream_);
+        if (part_ == 0 && schur) launch_schur_rows<T>(sc, n_, nr, 3, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, stream_);
+        if (part_ == 3) launch_diag_scale<T>(sc, nr, d_diagpos_, L, false, xx, ldx, stream_);
         if (part_ == 2 || part_ == 3) launch_copy2d<T>(n_, nr, xx, ldx, z, ldz, stream_);
+        if (schur && !schur_interface)
+            launch_schur_rows<T>(sc, n_, nr, 2, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, stream_);
         for (int lv = nl - 1; lv >= 0; lv--) {
             const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
             const int b0 = lv_bwd_ptr_[lv], nb = lv_bwd_ptr_[lv + 1] - b0;
             launch_bwd_prod<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, lv_maxw_[lv], stream_);
             launch_bwd_diag<T>(S_, d_diag_ + d0, nd, d_bwd_, z, ldz, xx, ldx, nr, part, WB, lv_maxw_[lv], stream_);
         }
+        if (schur && !schur_interface)
+            launch_schur_rows<T>(sc, n_, nr, 3, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, stream_);
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -648,9 +681,13 @@ int Engine::solve_device(void *dx, int64_t ldx, int nrhs, double *seconds) {
 int Engine::solve_host(const int64_t *perm, int flttype, void *b, int64_t ldb, int nrhs, double *seconds) {
     const size_t fsz = (prec_ == 2) ? sizeof(float) : sizeof(double);
     if (rhs_cap_ < (int64_t)nrhs * n_) {
+        release(d_rhs_);
         rhs_cap_ = (int64_t)nrhs * n_;
         d_rhs_ = alloc(rhs_cap_ * fsz);
-        if (!d_rhs_) return -1;
+        if (!d_rhs_) {
+            rhs_cap_ = 0;
+            return -1;
+        }
     }
     // permute on the host into the factor precision: xp[perm[i]] = b[i]
     std::vector<char> tmp((size_t)n_ * nrhs * fsz);
@@ -861,13 +898,37 @@ int Engine::precond(double *dv) {
     return 0;
 }
 
+void Engine::release_work() {
+    release(d_sr_);
+    release(d_bvec_);
+    release(d_bscal_);
+    release(d_bsvec_);
+    release(d_cpart_);
+    release(d_rhs_);
+    sr_cap_ = bvec_cap_ = bscal_cap_ = bsvec_cap_ = cpart_cap_ = rhs_cap_ = 0;
+}
+
+int64_t Engine::work_bytes(int R) const {
+    // refine_user: permuted B and X (2 n R), batched CG vectors (4 n R), the
+    // factor-precision copy of the preconditioned block (n R), per-column
+    // scalars and segmented-dot partials (one segment)
+    const int64_t nr = n_ * (int64_t)R;
+    const int64_t fsz = (prec_ == 2) ? 4 : 8;
+    const int64_t nchunk = (n_ + seg_dot_chunk() - 1) / seg_dot_chunk() + 1;
+    return 8 * (2 * nr + 4 * nr) + (prec_ == 2 ? fsz * nr : 0) + 8 * 8 * (int64_t)R + 8 * nchunk * R;
+}
+
 int Engine::precond_block(double *zb, int R) {
     if (prec_ == 3) return solve_t<double>(zb, n_, R);
     const int64_t nr = n_ * (int64_t)R;
     if (bsvec_cap_ < nr) {
+        release(d_bsvec_);
         bsvec_cap_ = nr;
         d_bsvec_ = (float *)alloc(nr * sizeof(float));
-        if (!d_bsvec_) return -1;
+        if (!d_bsvec_) {
+            bsvec_cap_ = 0;
+            return -1;
+        }
     }
     launch_d2s(nr, zb, d_bsvec_, stream_);
     const int rc = solve_t<float>(d_bsvec_, n_, R);
@@ -882,14 +943,22 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
     if (nseg < 1 || segoff.front() != 0 || segoff.back() != n_ || R < 1) return fail("bad batch segments");
     const int64_t nr = n_ * R, ns = (int64_t)nseg * R;
     if (bvec_cap_ < 4 * nr) {
+        release(d_bvec_);
         bvec_cap_ = 4 * nr;
         d_bvec_ = (double *)alloc(bvec_cap_ * sizeof(double));
-        if (!d_bvec_) return -1;
+        if (!d_bvec_) {
+            bvec_cap_ = 0;
+            return -1;
+        }
     }
     if (bscal_cap_ < 8 * ns) {
+        release(d_bscal_);
         bscal_cap_ = 8 * ns;
         d_bscal_ = (double *)alloc(bscal_cap_ * sizeof(double));
-        if (!d_bscal_) return -1;
+        if (!d_bscal_) {
+            bscal_cap_ = 0;
+            return -1;
+        }
     }
     if (segoff != segoff_host_) {
         if (!d_seg_) {
@@ -901,9 +970,13 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
             for (int64_t i = segoff[s]; i < segoff[s + 1]; i++) seg[i] = s;
         HIPCHK(hipMemcpy(d_seg_, seg.data(), n_ * sizeof(int32_t), hipMemcpyHostToDevice));
         if (segoff_dcap_ < (int64_t)segoff.size()) {
+            release(d_segoff_);
             segoff_dcap_ = (int64_t)segoff.size();
             d_segoff_ = (int64_t *)alloc(segoff.size() * sizeof(int64_t));
-            if (!d_segoff_) return -1;
+            if (!d_segoff_) {
+                segoff_dcap_ = 0;
+                return -1;
+            }
         }
         HIPCHK(hipMemcpy(d_segoff_, segoff.data(), segoff.size() * sizeof(int64_t), hipMemcpyHostToDevice));
         // chunks of the segmented dot products
@@ -916,9 +989,13 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
         }
         nchunk_ = coff[nseg];
         if (chunk_cap_ < nchunk_ + nseg + 1) {
+            release(d_coff_);
             chunk_cap_ = nchunk_ + nseg + 1;
             d_coff_ = (int32_t *)alloc(chunk_cap_ * sizeof(int32_t));
-            if (!d_coff_) return -1;
+            if (!d_coff_) {
+                chunk_cap_ = 0;
+                return -1;
+            }
         }
         d_cseg_ = d_coff_ + nseg + 1;
         HIPCHK(hipMemcpy(d_coff_, coff.data(), coff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -926,9 +1003,13 @@ int Engine::refine_cg_batch(double *X, const double *B, int R, const std::vector
         segoff_host_ = segoff;
     }
     if (cpart_cap_ < (int64_t)nchunk_ * R) {
+        release(d_cpart_);
         cpart_cap_ = (int64_t)nchunk_ * R;
         d_cpart_ = (double *)alloc(cpart_cap_ * sizeof(double));
-        if (!d_cpart_) return -1;
+        if (!d_cpart_) {
+            cpart_cap_ = 0;
+            return -1;
+        }
     }
     double *Rv = d_bvec_, *P = d_bvec_ + nr, *Z = d_bvec_ + 2 * nr, *AP = d_bvec_ + 3 * nr;
     double *dots = d_bscal_, *coef = d_bscal_ + 4 * ns;
@@ -1005,9 +1086,13 @@ int Engine::refine_user(const double *Bu, int64_t ldb, double *Xu, int64_t ldx, 
     if (user_order && !d_perm_) return fail("permutation not uploaded");
     const int64_t nr = n_ * (int64_t)R;
     if (sr_cap_ < 2 * nr) {
+        release(d_sr_);
         sr_cap_ = 2 * nr;
         d_sr_ = (double *)alloc(sr_cap_ * sizeof(double));
-        if (!d_sr_) return -1;
+        if (!d_sr_) {
+            sr_cap_ = 0;
+            return -1;
+        }
     }
     double *PB = d_sr_, *PX = d_sr_ + nr;
     auto in = [&](const double *src, int64_t ld, double *dst) {
@@ -1066,9 +1151,13 @@ int Engine::berr(const double *dx, const double *db, double out[3]) {
 int Engine::refine_cg(double *x, const double *b, int itermax, double eps, int *niter, double *resid) {
     if (vec_cap_ < 4 * n_) {
         vec_cap_ = 4 * n_;
+        release(d_vec_);
         d_vec_ = (double *)alloc(vec_cap_ * sizeof(double));
-        d_svec_ = (float *)alloc(n_ * sizeof(float));
-        if (!d_vec_ || !d_svec_) return -1;
+        if (!d_svec_) d_svec_ = (float *)alloc(n_ * sizeof(float));
+        if (!d_vec_ || !d_svec_) {
+            vec_cap_ = 0;
+            return -1;
+        }
     }
     double *r = d_vec_, *p = d_vec_ + n_, *z = d_vec_ + 2 * n_, *Ap = d_vec_ + 3 * n_;
     const double normb = std::sqrt(dot(b, b));
@@ -1117,9 +1206,13 @@ int Engine::refine_cg(double *x, const double *b, int itermax, double eps, int *
 int Engine::refine_sr(double *x, const double *b, int itermax, double eps, int *niter, double *resid) {
     if (vec_cap_ < 2 * n_) {
         vec_cap_ = 2 * n_;
+        release(d_vec_);
         d_vec_ = (double *)alloc(vec_cap_ * sizeof(double));
-        d_svec_ = (float *)alloc(n_ * sizeof(float));
-        if (!d_vec_ || !d_svec_) return -1;
+        if (!d_svec_) d_svec_ = (float *)alloc(n_ * sizeof(float));
+        if (!d_vec_ || !d_svec_) {
+            vec_cap_ = 0;
+            return -1;
+        }
     }
     double *r = d_vec_, *dx = d_vec_ + n_;
     const double normb = std::sqrt(dot(b, b));
@@ -1154,9 +1247,13 @@ int Engine::refine_sr(double *x, const double *b, int itermax, double eps, int *
 int Engine::refine_bicgstab(double *x, const double *b, int itermax, double eps, int *niter, double *resid) {
     if (vec_cap_ < 9 * n_) {
         vec_cap_ = 9 * n_;
+        release(d_vec_);
         d_vec_ = (double *)alloc(vec_cap_ * sizeof(double));
-        d_svec_ = (float *)alloc(n_ * sizeof(float));
-        if (!d_vec_ || !d_svec_) return -1;
+        if (!d_svec_) d_svec_ = (float *)alloc(n_ * sizeof(float));
+        if (!d_vec_ || !d_svec_) {
+            vec_cap_ = 0;
+            return -1;
+        }
     }
     double *r = d_vec_, *r2 = r + n_, *p = r2 + n_, *y = p + n_, *v = y + n_, *sv = v + n_, *z = sv + n_,
            *t = z + n_, *mt = t + n_;
@@ -1217,9 +1314,13 @@ int Engine::refine_gmres(double *x, const double *b, int itermax, int im, double
     const int64_t need = (int64_t)(2 * im + 1) * n_ + (im + 2);
     if (vec_cap_ < need) {
         vec_cap_ = need;
+        release(d_vec_);
         d_vec_ = (double *)alloc(vec_cap_ * sizeof(double));
-        d_svec_ = (float *)alloc(n_ * sizeof(float));
-        if (!d_vec_ || !d_svec_) return -1;
+        if (!d_svec_) d_svec_ = (float *)alloc(n_ * sizeof(float));
+        if (!d_vec_ || !d_svec_) {
+            vec_cap_ = 0;
+            return -1;
+        }
     }
     double *V = d_vec_;                            // n x (im+1)
     double *W = d_vec_ + (int64_t)(im + 1) * n_;   // n x im: M^{-1} v_i

@@ -28,6 +28,14 @@ class Engine {
 
     // Coefficient tables (1D layout, factor precision) to the host.
     int download_coef(void *hL, void *hU);
+    // Entries [off, off + count) of the tables (U only for LU), factor precision.
+    int download_coef_range(int64_t off, int64_t count, void *hL, void *hU);
+    // Permuted columns [schur_fcol(), n) hold the Schur complement (none when
+    // it equals n); schur_interface: the solve applies the updates of the
+    // factored part to the Schur rows and reads them in the backward sweep
+    // (PastixSolvModeInterface), else leaves them as they are (Local).
+    int64_t schur_fcol() const { return schurfcol_; }
+    bool schur_interface = false;
     // Diagonal of the factor's L panels (permuted order) to the host, FP64.
     int download_diag(double *h);
     // Pinned host staging block of at least count doubles, kept and grown
@@ -86,6 +94,17 @@ class Engine {
     hipStream_t stream() const { return stream_; }
     int64_t n() const { return n_; }
     void *alloc(size_t bytes);
+    // frees a buffer from alloc() (no-op on nullptr) and sets it to nullptr
+    template <typename P> void release(P *&p) {
+        release_ptr((void *)p);
+        p = nullptr;
+    }
+    void release_ptr(void *p);
+    // Work blocks of the refinement (grown on demand, kept between calls):
+    // the device bytes a call of refine_user on R columns of one system
+    // needs, and freeing them (the next call allocates them again).
+    int64_t work_bytes(int R) const;
+    void release_work();
     int64_t device_bytes() const { return bytes_; }
     const std::string &last_error() const { return err_; }
     int device() const { return dev_; }
@@ -161,17 +180,16 @@ class Engine {
     };
     std::vector<Owned> owned_;
     int64_t bytes_ = 0;
-    // frees a buffer from alloc() (no-op on nullptr) and sets it to nullptr
-    template <typename P> void release(P *&p) {
-        release_ptr((void *)p);
-        p = nullptr;
-    }
-    void release_ptr(void *p);
+
     std::string err_;
 
     int64_t n_ = 0, cblknbr_ = 0, coefnbr_ = 0, winvnbr_ = 0;
     int maxwidth_ = 0;
     std::vector<int> lv_maxw_;  // widest cblk per level
+    int64_t schurfcol_ = 0;
+    std::vector<int32_t> lv_fact_cnt_;
+    int32_t *d_fact_cblks_ = nullptr;
+    int n_fact_ = 0;
     DevSymbol S_{};
     // host copies of the level pointers
     std::vector<int32_t> lv_cblk_ptr_, lv_upd_ptr_, lv_upd_split_, lv_fwd_ptr_, lv_bwd_ptr_;
@@ -215,7 +233,7 @@ class Engine {
         const void *ptr;
         int64_t ld;
         int nrhs;
-        bool user;
+        bool user, iface;
         hipGraphExec_t exec;
     };
     std::vector<SolveGraph> solve_graphs_;

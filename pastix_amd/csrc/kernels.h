@@ -77,6 +77,11 @@ void launch_axpy(int64_t n, double alpha, const double *x, double *y, hipStream_
 void launch_axpy_dev(int64_t n, const double *alpha, double sign, const double *x, double *y, hipStream_t st);
 void launch_scal(int64_t n, double alpha, double *x, hipStream_t st);
 void launch_copy(int64_t n, const double *x, double *y, hipStream_t st);
+// Schur rows [r0, r1) of a solve: mode 0 z = x, 1 z = x - pending forward
+// products, 2 x = 0, 3 x = z
+template <typename T>
+void launch_schur_rows(int64_t r0, int64_t r1, int nrhs, int mode, const int64_t *yrow_ptr, const T *ybuf, int64_t ldy,
+                       T *x, int64_t ldx, T *z, int64_t ldz, hipStream_t st);
 // out[0] = ||x||_2 by scaled sums of squares (partials: 2 * 512 doubles)
 void launch_nrm2(int64_t n, const double *x, double *partials, double *out, hipStream_t st);
 // y = alpha A x + beta y, A m x n column-major

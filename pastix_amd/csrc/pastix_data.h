@@ -42,6 +42,12 @@ struct pastix_data_s {
     pxa::Plan plan;
     int64_t n = 0;
     int64_t nnz_in = 0;  // stored entries of the analysed input (DPARM_FILL_IN)
+    // Schur complement: the unknowns given to pastixSetSchurUnknownList (as
+    // given, in the input's numbering base), ordered last by the analysis;
+    // permuted columns [schurfcol, n) are the Schur part (schurfcol = n: none)
+    std::vector<pastix_int_t> schur_list;
+    std::string dir_global;  // output directory of the analysis files (IPARM_IO_STRATEGY)
+    int64_t schurfcol = 0;
     int mtxtype = SpmGeneral;
     int flttype = SpmDouble;
     const spmatrix_t *csc = nullptr;
@@ -73,6 +79,18 @@ struct Triplets {
 namespace pxa_api {
 double *device_pair(pastix_data_t *d);
 void err_print(const char *fn, const char *msg);
+int64_t spm_baseval(const spmatrix_t *spm);
+// analysis files (ordergen / symbgen formats); base: numbering base written
+int order_read(FILE *f, pxa::Order &o, int64_t &base);
+int order_write(FILE *f, const pxa::Order &o, int64_t base);
+int symbol_read(FILE *f, pxa::Symbol &s);
+int symbol_write(FILE *f, const pxa::Symbol &s);
+const char *output_dir(pastix_data_t *d);
+// writes the instance's ordering (order) or symbol into its output directory
+int save_analysis_file(pastix_data_t *d, const char *name, bool order);
+// sets the engine's Schur solve mode from IPARM_SCHUR_SOLV_MODE; false (with a
+// message) when the mode asks to solve with the unfactored Schur part
+bool schur_mode(pastix_data_t *d, const char *fn);
 int spm_triplets(const spmatrix_t *spm, Triplets &t, bool values);
 // adjacency graph of the input's pattern (symmetrised, without the diagonal)
 int spm_graph(const spmatrix_t *spm, pxa::Graph &g);
@@ -85,6 +103,8 @@ void triplets_to_csc(int64_t n, const Triplets &t, const int64_t *perm, std::vec
 // Device FP64 block of at least count entries, kept by the instance between
 // calls (grown on demand).
 double *device_block(pastix_data_t *d, int64_t count);
+// frees the block device_block() holds
+void release_device_block(pastix_data_t *d);
 // Refinement method code of the instance for Engine::refine_user: 1 CG, 0 GMRES.
 int refine_method(const pastix_data_t *d);
 // LDL^t requested (PastixFactSYTRF / PastixFactHETRF, the same in real arithmetic)

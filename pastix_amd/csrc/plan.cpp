@@ -25,9 +25,12 @@ struct KeyEnt {
 };
 }  // namespace
 
-Plan build_plan(const Symbol &s, const Layout &L) {
+Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
     const int64_t cb = s.cblknbr();
     Plan P;
+    const int64_t sc = std::min<int64_t>(std::max<int64_t>(schur_cblk, 0), cb);
+    P.schur_cblk = (int32_t)sc;
+    auto factored = [&](int64_t c) { return c < sc; };
     // panel row maps
     P.rowbase.assign(cb + 1, 0);
     for (int64_t c = 0; c < cb; c++) P.rowbase[c + 1] = P.rowbase[c] + L.cblk_stride[c];
@@ -70,9 +73,16 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     for (int64_t c = 0; c < cb; c++) P.lv_cblk_ptr[L.cblk_level[c] + 1]++;
     for (int32_t l = 0; l < nl; l++) P.lv_cblk_ptr[l + 1] += P.lv_cblk_ptr[l];
     P.lv_cblks.assign(cb, 0);
+    P.lv_fact_cnt.assign(nl, 0);
     {
+        // factored cblks first within each level, then the Schur ones
         std::vector<int32_t> pos(P.lv_cblk_ptr.begin(), P.lv_cblk_ptr.end() - 1);
-        for (int64_t c = 0; c < cb; c++) P.lv_cblks[pos[L.cblk_level[c]]++] = (int32_t)c;
+        for (int64_t c = 0; c < sc; c++) {
+            P.lv_cblks[pos[L.cblk_level[c]]++] = (int32_t)c;
+            P.lv_fact_cnt[L.cblk_level[c]]++;
+        }
+        for (int64_t c = sc; c < cb; c++) P.lv_cblks[pos[L.cblk_level[c]]++] = (int32_t)c;
+        for (int64_t c = 0; c < sc; c++) P.fact_cblks.push_back((int32_t)c);
     }
     auto width = [&](int64_t c) { return s.cblk_fcol[c + 1] - s.cblk_fcol[c]; };
     auto rows = [&](int64_t b) { return s.blok_lrow[b] - s.blok_frow[b] + 1; };
@@ -112,7 +122,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
     const int32_t FWD_ROWS = 64;  // panel rows of one forward product tile
     const int32_t BWD_ROWS = 64;  // panel rows of one backward product tile
 
-    for (int64_t c = 0; c < cb; c++)
+    for (int64_t c = 0; c < sc; c++)
         if (width(c) > 64) P.wide_cblks.push_back((int32_t)c);
     P.lv_step_ptr.assign(nl + 1, 0);
     P.step_cb_ptr.push_back(0);
@@ -121,9 +131,11 @@ Plan build_plan(const Symbol &s, const Layout &L) {
 
     for (int32_t lv = 0; lv < nl; lv++) {
         const int32_t l0 = P.lv_cblk_ptr[lv], l1 = P.lv_cblk_ptr[lv + 1];
+        // the factored cblks of the level: [l0, lf)
+        const int32_t lf = l0 + P.lv_fact_cnt[lv];
         // blocked panel steps of this level
         int64_t maxk = 0;
-        for (int32_t k = l0; k < l1; k++) maxk = std::max<int64_t>(maxk, width(P.lv_cblks[k]));
+        for (int32_t k = l0; k < lf; k++) maxk = std::max<int64_t>(maxk, width(P.lv_cblks[k]));
         const int64_t nsteps = (maxk + 63) / 64;
         // in-cblk updates: left-looking (the step's block takes all earlier
         // blocks' products, K up to the cblk width, one tile per row tile)
@@ -131,12 +143,12 @@ Plan build_plan(const Symbol &s, const Layout &L) {
         // (every later block takes the step's products, 64 deep) near the top
         // of the tree, where few row tiles would leave the device idle
         int64_t rowtiles = 0;
-        for (int32_t k = l0; k < l1; k++) rowtiles += (L.cblk_stride[P.lv_cblks[k]] + 63) / 64;
+        for (int32_t k = l0; k < lf; k++) rowtiles += (L.cblk_stride[P.lv_cblks[k]] + 63) / 64;
         const bool right = rowtiles < PANEL_RIGHT_ROWTILES;
         P.lv_panel_right[lv] = right;
         for (int64_t kb = 0; kb < nsteps; kb++) {
             const int64_t k0 = kb * 64;
-            for (int32_t k = l0; k < l1; k++) {
+            for (int32_t k = l0; k < lf; k++) {
                 const int32_t c = P.lv_cblks[k];
                 const int64_t K = width(c), st = L.cblk_stride[c];
                 if (K <= k0) continue;
@@ -163,7 +175,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
 
         // update tiles from the sources of this level
         ke.clear();
-        for (int32_t k = l0; k < l1; k++) {
+        for (int32_t k = l0; k < lf; k++) {
             const int64_t src = P.lv_cblks[k];
             const int64_t last = s.cblk_bloknum[src + 1];
             int64_t b = s.cblk_bloknum[src] + 1;
@@ -352,7 +364,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
         }
 
         // forward products of the sources of this level
-        for (int32_t k = l0; k < l1; k++) {
+        for (int32_t k = l0; k < lf; k++) {
             const int64_t c = P.lv_cblks[k];
             const int64_t K = width(c), st = L.cblk_stride[c];
             for (int64_t r0 = K; r0 < st; r0 += FWD_ROWS) {
@@ -369,6 +381,7 @@ Plan build_plan(const Symbol &s, const Layout &L) {
                 const int64_t c = P.lv_cblks[k];
                 const int32_t b0 = (int32_t)P.bwd.size();
                 P.bwd_itemptr[seg + (k - l0)] = b0;
+                if (!factored(c)) continue;  // the Schur part has no solve items
                 const int64_t K = width(c), st = L.cblk_stride[c];
                 for (int64_t r0 = K; r0 < st; r0 += BWD_ROWS) {
                     int64_t r1 = std::min<int64_t>(st, r0 + BWD_ROWS);

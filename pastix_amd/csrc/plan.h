@@ -20,6 +20,12 @@ struct Plan {
     int32_t nlevels = 0;
     // cblks of each level (increasing cblk index inside a level)
     std::vector<int32_t> lv_cblk_ptr, lv_cblks;
+    // cblks at and after schur_cblk hold the Schur complement: they receive
+    // updates but are neither factored nor sources; within each level they
+    // come after the factored cblks, of which there are lv_fact_cnt[lv]
+    int32_t schur_cblk = 0;
+    std::vector<int32_t> lv_fact_cnt;
+    std::vector<int32_t> fact_cblks;  // every factored cblk
     // blocked panel steps: level L owns steps [lv_step_ptr[L], lv_step_ptr[L+1]),
     // step s is column block kb = s - lv_step_ptr[L] of the level's cblks
     std::vector<int32_t> lv_step_ptr;
@@ -94,6 +100,7 @@ struct Plan {
     double flops_wtile = 0;
 };
 
-Plan build_plan(const Symbol &s, const Layout &L);
+// schur_cblk: first cblk of the Schur complement (cblknbr when there is none)
+Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk);
 
 }  // namespace pxa

@@ -17,6 +17,15 @@
 // slots overlap on the device.  No request waits for a later one, and a slot
 // holding version v is freed as soon as v's solves drain, so the loop always
 // makes progress.
+//
+// HBM-budget admission: the work blocks of a merged solve (right-hand sides,
+// solutions and refinement vectors: Engine::work_bytes plus the transfer
+// block) count against a budget, by default IPARM_GPU_MEMORY_PERCENTAGE of
+// the device memory left free once the slots hold their factors.  A slot
+// keeps its blocks between batches; a batch is admitted only while the
+// blocks of all slots fit the budget, shorter if need be.  When the oldest
+// request does not fit, it waits, and idle slots give their blocks back; a
+// request that cannot fit even alone ends with PASTIX_ERR_OUTOFMEMORY.
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -61,11 +70,31 @@ struct pastix_amd_server_s {
     std::vector<char> ready;
     std::vector<std::thread> workers;
     bool stop = false;
+    // HBM budget of the work blocks: bytes, and per slot the columns its
+    // blocks are sized for, the bytes they hold and whether it is running
+    int64_t budget = 0, peak = 0, waits = 0;
+    std::vector<int64_t> cap_cols, held_bytes;
+    std::vector<char> busy;
+    bool pressure = false;  // a slot waits for memory: pressure_slot
+    int pressure_slot = -1;
     // statistics
     int64_t batches = 0, batched_rhs = 0;
 };
 
 namespace {
+
+// device bytes a slot holds for a merged solve of R columns: the engine's
+// refinement blocks and the transfer block (B and X, FP64)
+int64_t work_need(pastix_amd_server_s *S, int k, int64_t R) {
+    if (R <= 0) return 0;
+    return S->slots[k]->engine->work_bytes((int)R) + 16 * S->n * R;
+}
+
+int64_t held_total(pastix_amd_server_s *S) {
+    int64_t t = 0;
+    for (int64_t b : S->held_bytes) t += b;
+    return t;
+}
 
 // One merged solve: the requests' columns are copied side by side into a
 // device block, solved and refined there (permutations on the device), and
@@ -155,25 +184,68 @@ void worker(pastix_amd_server_s *S, int k) {
             S->cv.notify_all();
             continue;
         }
-        // 2. a batch of queued solves of the version this slot holds
+        // 2. a batch of queued solves of the version this slot holds, as
+        // long as the slots' work blocks fit the HBM budget
         std::vector<Req *> batch;
-        if (S->held[k] >= 0 && S->ready[k]) {
+        Req *first = nullptr;
+        // while another slot waits for memory, this one takes no new batch
+        const bool yield = S->pressure && S->pressure_slot != k;
+        if (S->held[k] >= 0 && S->ready[k] && !yield) {
+            const int64_t avail = S->budget - (held_total(S) - S->held_bytes[k]);
             pastix_int_t R = 0;
             for (Req *r : S->queue) {
                 if (r->kind != 0 || r->state != 0 || r->version != S->held[k]) continue;
+                if (!first) first = r;
                 if (!batch.empty() && R + r->nrhs > S->max_rhs) break;
+                const int64_t cols = std::max<int64_t>(S->cap_cols[k], R + r->nrhs);
+                if (work_need(S, k, cols) > avail) break;
                 batch.push_back(r);
                 R += r->nrhs;
             }
+            if (batch.empty() && first) {
+                if (work_need(S, k, first->nrhs) > S->budget) {
+                    // cannot fit even alone
+                    first->rc = PASTIX_ERR_OUTOFMEMORY;
+                    first->state = 2;
+                    first->t_done = Clock::now();
+                    S->live_solves[first->version]--;
+                    for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
+                        if (*it == first) {
+                            S->queue.erase(it);
+                            break;
+                        }
+                    S->cv.notify_all();
+                    continue;
+                }
+                // wait for the other slots' blocks
+                if (!S->pressure) {
+                    S->waits++;
+                    S->pressure = true;
+                    S->pressure_slot = k;
+                    S->cv.notify_all();
+                }
+            }
+            if (S->pressure && S->pressure_slot == k && (!first || !batch.empty())) {
+                S->pressure = false;
+                S->pressure_slot = -1;
+                S->cv.notify_all();
+            }
         }
         if (!batch.empty()) {
+            pastix_int_t R = 0;
             for (Req *r : batch) {
                 r->state = 1;
                 r->slot = k;
+                R += r->nrhs;
             }
+            S->cap_cols[k] = std::max<int64_t>(S->cap_cols[k], R);
+            S->held_bytes[k] = work_need(S, k, S->cap_cols[k]);
+            S->peak = std::max(S->peak, held_total(S));
+            S->busy[k] = 1;
             lk.unlock();
             run_solves(S, k, batch);
             lk.lock();
+            S->busy[k] = 0;
             const auto now = Clock::now();
             for (Req *r : batch) {
                 r->state = 2;
@@ -190,6 +262,19 @@ void worker(pastix_amd_server_s *S, int k) {
             S->cv.notify_all();
             continue;
         }
+        // an idle slot gives its work blocks back when another one waits for memory
+        if (S->pressure && S->pressure_slot != k && S->held_bytes[k] > 0) {
+            S->busy[k] = 1;
+            lk.unlock();
+            release_device_block(S->slots[k]);
+            S->slots[k]->engine->release_work();
+            lk.lock();
+            S->busy[k] = 0;
+            S->held_bytes[k] = 0;
+            S->cap_cols[k] = 0;
+            S->cv.notify_all();
+            continue;
+        }
         S->cv.wait(lk);
     }
 }
@@ -203,7 +288,7 @@ extern "C" int pastix_amd_server_create(pastix_amd_server_t **srv, pastix_data_t
         pastix_data_t *d = slots[k];
         if (!d || !d->engine || !(d->steps & STEP_NUMFACT) || !d->ext[PXA_IPARM_RESIDENT_INPUT] ||
             d->n != slots[0]->n || d->symbol.cblknbr() != slots[0]->symbol.cblknbr() ||
-            d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF || is_ldlt(d))
+            d->iparm[IPARM_FACTORIZATION] == PastixFactGETRF || is_ldlt(d) || d->schurfcol < d->n)
             return PASTIX_ERR_BADPARAMETER;
     }
     auto *S = new pastix_amd_server_s();
@@ -212,6 +297,17 @@ extern "C" int pastix_amd_server_create(pastix_amd_server_t **srv, pastix_data_t
     S->n = slots[0]->n;
     S->held.assign(nslots, -1);
     S->ready.assign(nslots, 0);
+    S->cap_cols.assign(nslots, 0);
+    S->held_bytes.assign(nslots, 0);
+    S->busy.assign(nslots, 0);
+    {
+        // the work budget: IPARM_GPU_MEMORY_PERCENTAGE of the memory left free
+        size_t freeb = 0, totb = 0;
+        (void)hipSetDevice((int)slots[0]->ext[PXA_IPARM_DEVICE]);
+        if (hipMemGetInfo(&freeb, &totb) != hipSuccess) freeb = 0;
+        const pastix_int_t pct = std::max<pastix_int_t>(1, std::min<pastix_int_t>(100, slots[0]->iparm[IPARM_GPU_MEMORY_PERCENTAGE]));
+        S->budget = (int64_t)((double)freeb * (double)pct / 100.0);
+    }
     S->held[0] = 0;  // slot 0 serves the matrix it was factored on
     S->ready[0] = 1;
     for (int k = 0; k < nslots; k++) S->workers.emplace_back(worker, S, k);
@@ -277,6 +373,24 @@ extern "C" int pastix_amd_server_wait(pastix_amd_server_t *S, int64_t ticket, do
     const int rc = r->rc;
     S->reqs.erase(it);
     return rc;
+}
+
+extern "C" int pastix_amd_server_set_budget(pastix_amd_server_t *S, int64_t bytes) {
+    if (!S || bytes < 0) return PASTIX_ERR_BADPARAMETER;
+    std::lock_guard<std::mutex> g(S->mu);
+    S->budget = bytes;
+    S->cv.notify_all();
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int pastix_amd_server_memory(pastix_amd_server_t *S, int64_t out[4]) {
+    if (!S || !out) return PASTIX_ERR_BADPARAMETER;
+    std::lock_guard<std::mutex> g(S->mu);
+    out[0] = S->budget;
+    out[1] = held_total(S);
+    out[2] = S->peak;
+    out[3] = S->waits;
+    return PASTIX_SUCCESS;
 }
 
 extern "C" int pastix_amd_server_stats(pastix_amd_server_t *S, int64_t out[4]) {

@@ -293,6 +293,30 @@ def getDiag(pastix_data, n, incD=1, dtype=np.float64):
     return D
 
 
+lib.pastixSetSchurUnknownList.argtypes = [_P, _I, ct.c_void_p]
+lib.pastixSetSchurUnknownList.restype = None
+lib.pastixGetSchur.argtypes = [_P, ct.c_void_p, _I]
+lib.pastixGetSchur.restype = ct.c_int
+
+
+class solv_mode:
+    Local, Interface, Schur = 0, 1, 2
+
+
+def setSchurUnknownList(pastix_data, schur_list):
+    """Unknowns (in the matrix's numbering base) to keep out of the
+    factorisation; the next analysis orders them last."""
+    lst = np.ascontiguousarray(schur_list, dtype=np.int64)
+    lib.pastixSetSchurUnknownList(pastix_data.ptr, len(lst), lst.ctypes.data)
+
+
+def getSchur(pastix_data, S):
+    """Fill the Fortran-ordered square array S (leading dimension S.shape[0])
+    with the Schur complement after pastix_task_numfact."""
+    assert S.flags.f_contiguous
+    _check("pastixGetSchur", lib.pastixGetSchur(pastix_data.ptr, S.ctypes.data, S.shape[0]))
+
+
 def subtask_solve(pastix_data, b, nrhs=-1):
     nrhs = _columns(b, nrhs)
     _check("pastix_subtask_solve", lib.pastix_subtask_solve(pastix_data.ptr, nrhs, b.ctypes.data, b.shape[0]))
@@ -518,6 +542,10 @@ lib.pastix_amd_server_wait.argtypes = [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_
 lib.pastix_amd_server_wait.restype = ct.c_int
 lib.pastix_amd_server_stats.argtypes = [ct.c_void_p, _i64p]
 lib.pastix_amd_server_stats.restype = ct.c_int
+lib.pastix_amd_server_memory.argtypes = [ct.c_void_p, _i64p]
+lib.pastix_amd_server_memory.restype = ct.c_int
+lib.pastix_amd_server_set_budget.argtypes = [ct.c_void_p, ct.c_int64]
+lib.pastix_amd_server_set_budget.restype = ct.c_int
 lib.pastix_amd_server_destroy.argtypes = [ct.POINTER(ct.c_void_p)]
 lib.pastix_amd_server_destroy.restype = ct.c_int
 
@@ -565,6 +593,15 @@ class Server:
         if rc != 0:
             raise PastixError("pastix_amd_server_wait", rc)
         return self._bufs.pop(ticket, None), float(lat.value), int(ver.value)
+
+    def set_budget(self, nbytes):
+        """Device memory the merged solves' work blocks may hold (all slots)."""
+        _check("pastix_amd_server_set_budget", lib.pastix_amd_server_set_budget(self.ptr, int(nbytes)))
+
+    def memory(self):
+        out = np.zeros(4, dtype=np.int64)
+        _check("pastix_amd_server_memory", lib.pastix_amd_server_memory(self.ptr, out.ctypes.data_as(_i64p)))
+        return {"budget": int(out[0]), "held": int(out[1]), "peak": int(out[2]), "waits": int(out[3])}
 
     def stats(self):
         out = np.zeros(4, dtype=np.int64)

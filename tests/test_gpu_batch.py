@@ -88,3 +88,33 @@ def test_batch_fp32_factor_fp64_cg(graph):
         assert np.abs(x - xp).max() <= 1e-8 * np.abs(xp).max()
         pastix.finalize(h1)
     pastix.finalize(h)
+
+
+def test_c4_batch_512_systems_full_size():
+    """C4 at its size: 512 shifted 2D Laplacians on k x k grids, k uniform in
+    [16, 128], 1-8 right-hand sides each; one FP32 factor of the union (graph
+    replay) + batched FP64 CG: every column of every system reaches berr <=
+    1e-12, and a sample of systems agrees with plain per-system FP64 solves
+    to cond(A) x 1e-12 (<= 1e-8 relative)."""
+    rng = np.random.default_rng(7)
+    ks = [int(k) for k in rng.integers(16, 129, size=512)]
+    mats, bs = _systems(ks, 77)
+    h, S = _batch(mats, ks, prec=spm.SpmFloat, graph=True)
+    xs = pastix.batch_solve_refine(h, bs)
+    worst, cols = 0.0, 0
+    for A, b, x in zip(mats, bs, xs):
+        R = b - A @ x
+        anorm = abs(A).sum(axis=1).max()
+        for c in range(b.shape[1]):
+            worst = max(worst, float(np.abs(R[:, c]).max() / (anorm * np.abs(x[:, c]).max() + np.abs(b[:, c]).max())))
+            cols += 1
+    print(f"C4: 512 systems, {cols} right-hand sides, worst berr {worst:.2e}, CG iterations "
+          f"{int(h.iparm[pastix.iparm.nbiter])}")
+    assert worst <= 1e-12
+    for i in range(0, 512, 64):
+        xd = np.linalg.solve(mats[i].toarray(), bs[i]) if ks[i] <= 64 else None
+        if xd is None:
+            import scipy.sparse.linalg as spla
+            xd = spla.spsolve(mats[i].tocsc(), bs[i]).reshape(bs[i].shape)
+        assert np.abs(xs[i] - xd).max() <= 1e-8 * np.abs(xd).max()
+    pastix.finalize(h)

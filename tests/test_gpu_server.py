@@ -121,3 +121,53 @@ def test_server_failed_refactorisation_ends_its_solves():
     assert all(_berr(A2, x3[:, c], b3[:, c]) <= 1e-12 for c in range(3))
     srv.close()
     pastix.finalize(h)
+
+
+def test_server_hbm_budget_admission():
+    """HBM-budget admission: with a work budget of about 20 right-hand sides,
+    two slots cannot both hold 16-column batches; merged solves are admitted
+    as memory frees up (the most the slots held at once stays within the
+    budget, and requests waited for memory), every result is right for its
+    matrix version, and a request larger than the whole budget ends with
+    PASTIX_ERR_OUTOFMEMORY instead of stalling the queue."""
+    A = spm.laplacian7(*GRID, coef=(1.0, 1.0, 0.01))
+    n = A.shape[0]
+    slots = [_slot(A, spm.SpmFloat) for _ in range(2)]
+    srv = pastix.Server([h for h, _ in slots], max_rhs=64)
+    assert srv.memory()["budget"] > 0
+    # one column's blocks, measured
+    rng = np.random.default_rng(41)
+    srv.wait(srv.submit_solve(rng.uniform(-1, 1, (n, 1))))
+    one = srv.memory()["held"]
+    assert one > 0
+    srv.set_budget(20 * one)
+    shifts = {0: 0.0}
+    reqs = []
+    for k in range(16):
+        if k == 3:
+            sh = 0.05
+            reqs.append(("refactor", srv.submit_refactor(sh), sh, None))
+        else:
+            nrhs = int(rng.choice([4, 16]))
+            b = rng.uniform(-1.0, 1.0, size=(n, nrhs))
+            reqs.append(("solve", srv.submit_solve(b.copy(order="F")), None, b))
+    big = rng.uniform(-1.0, 1.0, size=(n, 64))
+    tbig = srv.submit_solve(big.copy(order="F"))
+    for kind, t, sh, b in reqs:
+        x, lat, ver = srv.wait(t)
+        if kind == "refactor":
+            shifts[ver] = sh
+        else:
+            Av = A + shifts[ver] * sps.eye(n)
+            for c in range(b.shape[1]):
+                assert _berr(Av, x[:, c], b[:, c]) <= 1e-12
+    with pytest.raises(pastix.PastixError) as e:
+        srv.wait(tbig)
+    assert e.value.code == pastix.error.OUTOFMEMORY
+    mem = srv.memory()
+    print("server memory", mem, "one column", one)
+    assert mem["peak"] <= 20 * one or mem["peak"] == one
+    assert mem["waits"] >= 1
+    srv.close()
+    for h, _ in slots:
+        pastix.finalize(h)

@@ -354,3 +354,50 @@ def test_gmres_iterates_are_minimal_residual_and_restart():
     xr = _minres_cycle(B, Minv, b, x1, 2)
     assert np.abs(x - xr).max() <= 1e-9 * np.abs(xr).max()
     pastix.finalize(h)
+
+
+# ---- static scheduler (sopalin/sequential_zpotrf.c:71-120) -----------------
+@pytest.mark.parametrize("lu", [False, True])
+def test_static_scheduler_matches_sequential(lu):
+    """One thread walks the cblks in order: the sequential factor bit for bit.
+    Several threads apply a target's updates in another order: the same factor
+    to rounding, and (LL^t) the dense Cholesky of P A P^T."""
+    grid = (14, 13, 12)
+    A = spm.convection_diffusion27(*grid, seed=2) if lu else spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, not lu, LU if lu else LLT, grid=grid, min_bs=8, max_bs=24)
+    O = oracle.Oracle(sym)
+    bc = oracle.bcsc(A, sym["permtab"])
+    crit = np.sqrt(np.ldexp(1.0, -53)) * np.linalg.norm(bc["vals"])
+    L0, U0, _ = O.fill(bc, lu=lu)
+    L1, U1 = L0.copy(), None if U0 is None else U0.copy()
+    if lu:
+        O.getrf(L1, U1, crit)
+    else:
+        O.potrf(L1, crit)
+    L2, U2 = L0.copy(), None if U0 is None else U0.copy()
+    rc, npiv = O.factor_static(L2, U2, crit, 1)
+    assert rc == 0 and npiv == 0
+    assert np.array_equal(L1, L2) and (U1 is None or np.array_equal(U1, U2))
+    for nt in (3, 8):
+        L3, U3 = L0.copy(), None if U0 is None else U0.copy()
+        rc, npiv = O.factor_static(L3, U3, crit, nt)
+        assert rc == 0
+        assert rel_frob(L3, L1) <= 1e-13 and (U1 is None or rel_frob(U3, U1) <= 1e-13)
+    if not lu:
+        D = O.to_dense(L3)
+        Bd = bc["B"].toarray()
+        assert rel_frob(D, np.linalg.cholesky(Bd)) <= 1e-13
+    pastix.finalize(h)
+
+
+def test_static_scheduler_negative_pivot_ends_every_thread():
+    grid = (10, 9, 8)
+    A = spm.laplacian7(*grid, diag=6.0)
+    h, S, sym = analyse(A, True, LLT, grid=grid, min_bs=8, max_bs=24)
+    import scipy.sparse as sps
+    O = oracle.Oracle(sym)
+    bc = oracle.bcsc(A - 7.0 * sps.eye(A.shape[0]), sym["permtab"])
+    L, _, _ = O.fill(bc)
+    rc, _ = O.factor_static(L, None, 1e-8, 6)
+    assert rc == -1
+    pastix.finalize(h)
