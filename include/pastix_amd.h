@@ -94,6 +94,19 @@ int pastix_amd_solve_refine_device(pastix_data_t *pastix_data, pastix_int_t nrhs
 int pastix_amd_backward_error(pastix_data_t *pastix_data, const double *b,
                               const double *x, double *berr);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pastix_amd_backward_error on nrhs FP64 device columns
+ * d_b (ldb) and d_x (ldx) in the user ordering, results into the device array
+ * d_berr[nrhs].  Allocates nothing: the partial maxima use a library-static
+ * device array of 768 KiB per GPU (512 blocks x 3 maxima x 64 columns),
+ * held until the process ends, so nrhs is at most 64 (larger values return
+ * PASTIX_ERR_BADPARAMETER).  Calls that overlap on one GPU share that array
+ * and race; the call returns once the results are written. */
+int pastix_amd_backward_error_device_noalloc(pastix_data_t *pastix_data, pastix_int_t nrhs,
+                                             const double *d_b, pastix_int_t ldb,
+                                             const double *d_x, pastix_int_t ldx, double *d_berr);
+
 /* ---- Residual SpMV and vector kernels of the refinement ------------------
  * The operations the reference's refinement drivers call through their
  * z_solver table (refinement/z_refine_functions.c:115-148), on FP64 device

@@ -5,6 +5,7 @@
 // sopalin/pastix_task_sopalin.c:90-101,206-217,367-388), iparm/dparm inputs and
 // outputs, in-place right-hand sides, and PASTIX_ERR_* return codes.
 #include "pastix_data.h"
+#include "kernels.h"
 #include <algorithm>
 #include <atomic>
 #include <functional>
@@ -1403,6 +1404,27 @@ extern "C" int pastix_amd_backward_error(pastix_data_t *d, const double *b, cons
     *berr = den > 0 ? o[0] / den : o[0];
     return PASTIX_SUCCESS;
 }
+// No reference counterpart: pastix_amd_backward_error for callers that cannot
+// allocate (HIP graph capture, serving loops).  Device columns in the user
+// ordering, results into a device array; the partial maxima live in the
+// library-static scratch of the device, so at most 64 columns per call.
+extern "C" int pastix_amd_backward_error_device_noalloc(pastix_data_t *d, pastix_int_t nrhs, const double *d_b,
+                                                        pastix_int_t ldb, const double *d_x, pastix_int_t ldx,
+                                                        double *d_berr) {
+    if (!d || !d->engine || !(d->steps & STEP_BCSC2CTAB)) {
+        err_print("pastix_amd_backward_error_device_noalloc", "the matrix has not been factorised");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if (nrhs < 1 || nrhs > pxa::NOALLOC_MAX_RHS) {
+        err_print("pastix_amd_backward_error_device_noalloc",
+                  "nrhs must be between 1 and 64 (the size of the library's static scratch)");
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if (!d_b || !d_x || !d_berr || ldb < d->n || ldx < d->n) return PASTIX_ERR_BADPARAMETER;
+    if (d->engine->berr_noalloc(d_x, ldx, d_b, ldb, (int)nrhs, d->bcsc.normInf, d_berr)) return PASTIX_ERR_INTERNAL;
+    return PASTIX_SUCCESS;
+}
+
 extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[10]) {
     if (!d || !d->engine) return PASTIX_ERR_BADPARAMETER;
     for (int c = 0; c < pxa::Engine::KC_N; c++) {

@@ -79,6 +79,8 @@ template <typename T, typename S> static T *upload_as(Engine &E, const std::vect
 int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, int factotype, int prec) {
     dev_ = device;
     HIPCHK(hipSetDevice(dev_));
+    // the no-allocation entry points' scratch: its address on this device
+    if (noalloc_scratch_init(dev_)) return fail("cannot resolve the device scratch array");
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0_));
     HIPCHK(hipEventCreate(&ev1_));
@@ -445,6 +447,12 @@ int Engine::upload_perm(const std::vector<int64_t> &permtab) {
         std::vector<int32_t> p(permtab.begin(), permtab.end());
         d_perm_ = upload(*this, p);
         if (!d_perm_) return -1;
+        // the inverse (peritab), for kernels that read user-ordered vectors
+        // in the permuted row order
+        std::vector<int32_t> q(p.size());
+        for (size_t i = 0; i < p.size(); i++) q[p[i]] = (int32_t)i;
+        d_iperm_ = upload(*this, q);
+        if (!d_iperm_) return -1;
     }
     return 0;
 }
@@ -1140,6 +1148,24 @@ int Engine::refine_user(const double *Bu, int64_t ldb, double *Xu, int64_t ldx, 
     HIPCHK(hipStreamSynchronize(stream_));
     if (niter) *niter = it;
     if (resid) *resid = res;
+    return 0;
+}
+
+// Backward errors of nrhs device columns in the user ordering into the device
+// array dberr, with no allocation: the partial maxima go to the library-static
+// scratch of this device (pastix_amd_backward_error_device_noalloc).
+int Engine::berr_noalloc(const double *dx, int64_t ldx, const double *db, int64_t ldb, int nrhs, double normInf,
+                         double *dberr) {
+    if (!have_matrix_) return fail("matrix not uploaded");
+    if (!d_iperm_) return fail("permutation not uploaded");
+    if (nrhs < 1 || nrhs > NOALLOC_MAX_RHS) return fail("nrhs outside [1, 64]");
+    double *scr = noalloc_scratch(dev_);
+    if (!scr) return fail("no-allocation scratch not set up on this device");
+    HIPCHK(hipSetDevice(dev_));
+    launch_berr_user(n_, d_rowptr_, d_cols_, d_vals_, d_iperm_, dx, ldx, db, ldb, nrhs, normInf, scr, dberr,
+                     stream_);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
     return 0;
 }
 

@@ -90,6 +90,8 @@ class Engine {
                     double *resid);
     // Normwise backward error pieces: [max|b-Ax|, max|x|, max|b|].
     int berr(const double *dx, const double *db, double out[3]);
+    int berr_noalloc(const double *dx, int64_t ldx, const double *db, int64_t ldb, int nrhs, double normInf,
+                     double *dberr);
 
     hipStream_t stream() const { return stream_; }
     int64_t n() const { return n_; }
@@ -221,6 +223,7 @@ class Engine {
     void *d_z_ = nullptr;  // forward-sweep result, n x nrhs_chunk
     BwdItem *d_bwd_ = nullptr;
     int32_t *d_perm_ = nullptr;  // permtab as int32 for device permutations
+    int32_t *d_iperm_ = nullptr; // peritab as int32
     void *d_L0_ = nullptr, *d_U0_ = nullptr;  // snapshot of the filled coefficients
     std::vector<hipEvent_t> pev_;
     // captured graphs: numfact (re-captured when the pivot threshold changes)

@@ -130,6 +130,19 @@ void launch_gemv_n(int64_t n, int m, const double *V, int64_t ldv, const double 
                    hipStream_t st);
 void launch_berr_parts(int64_t n, const int64_t *colptr, const int32_t *rows, const double *vals, const double *x,
                        const double *b, double *partials, double *out, hipStream_t st);
+// Backward error without allocation: partial maxima in a library-static
+// device array.  NOALLOC_MAX_RHS columns at most; noalloc_scratch_init
+// resolves the array's address on the current device (call it once per
+// device with that device current), noalloc_scratch returns it (nullptr
+// before the init).  partials: NOALLOC_SCRATCH_DOUBLES doubles.
+constexpr int NOALLOC_MAX_RHS = 64;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)512 * 3 * NOALLOC_MAX_RHS;
+int noalloc_scratch_init(int device);
+double *noalloc_scratch(int device);
+void launch_berr_user(int64_t n, const int64_t *rowptr, const int32_t *cols, const double *vals,
+                      const int32_t *peri, const double *x, int64_t ldx, const double *b, int64_t ldb, int nrhs,
+                      double normInf, double *partials, double *berr, hipStream_t st);
 // Permute rows of a column-major m x nrhs block: y[perm[i]] = x[i] (scatter) or y[i] = x[perm[i]] (gather).
 template <typename T>
 void launch_permute(int64_t m, int nrhs, const int32_t *perm, bool scatter, const T *x, int64_t ldx, T *y,

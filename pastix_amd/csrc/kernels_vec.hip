@@ -3,6 +3,8 @@
 // bitwise reproducible from run to run.
 #include "dev_common.h"
 #include "dev_types.h"
+#include <mutex>
+
 #include "kernels.h"
 
 namespace pxa {
@@ -541,6 +543,100 @@ void launch_berr_parts(int64_t n, const int64_t *colptr, const int32_t *rows, co
                        const double *b, double *partials, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_berr_part, dim3(RED_BLOCKS), dim3(256), 0, st, n, colptr, rows, vals, x, b, partials);
     hipLaunchKernelGGL(k_max_final, dim3(1), dim3(64), 0, st, RED_BLOCKS, 3, partials, out);
+}
+
+// ---- backward error without allocation --------------------------------------
+// Library-static scratch of pastix_amd_backward_error_device_noalloc: the
+// per-block partial maxima (|b - A x|, |x|, |b|) of up to NOALLOC_MAX_RHS
+// right-hand sides.  Every GPU holds its own copy for the life of the
+// process.  Host code never passes the array's name to a kernel (on the host
+// it is only a stand-in for the symbol): it uses the address that
+// hipGetSymbolAddress returned for the device, resolved once per device by
+// noalloc_scratch_init.  Every block writes its partials on every call, so
+// nothing needs clearing between calls; calls that overlap on one GPU race.
+static_assert(NOALLOC_SCRATCH_DOUBLES == (size_t)RED_BLOCKS * 3 * NOALLOC_MAX_RHS, "one partial triple per block and column");
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+static std::mutex scratch_mu;
+static double *scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_scratch_init(int device) {
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return -1;
+    if (scratch_dev[device]) return 0;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+    scratch_dev[device] = static_cast<double *>(p);
+    return 0;
+}
+
+double *noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> g(scratch_mu);
+    return (device >= 0 && device < NOALLOC_MAX_DEVICES) ? scratch_dev[device] : nullptr;
+}
+
+// k_berr_part on right-hand sides and solutions in the user's ordering: row
+// j of the permuted matrix is user row peri[j], and its column m reads x at
+// peri[m], so the products and their order are those of k_berr_part on the
+// permuted vectors.  Column c of the block (blockIdx.y) writes its partials
+// at partials[(c * gridDim.x + blockIdx.x) * 3 + t].
+__global__ __launch_bounds__(256) void k_berr_part_user(int64_t n, const int64_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ cols,
+                                                        const double *__restrict__ vals,
+                                                        const int32_t *__restrict__ peri, const double *__restrict__ x,
+                                                        int64_t ldx, const double *__restrict__ b, int64_t ldb,
+                                                        double *__restrict__ partials) {
+    __shared__ double sh[3][4];
+    const int c = blockIdx.y;
+    x += (int64_t)c * ldx;
+    b += (int64_t)c * ldb;
+    double r = 0, xm = 0, bm = 0;
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256) {
+        double s = 0;
+        for (int64_t k = rowptr[j]; k < rowptr[j + 1]; k++) s += vals[k] * x[peri[cols[k]]];
+        const int64_t u = peri[j];
+        r = fmax(r, fabs(b[u] - s));
+        xm = fmax(xm, fabs(x[u]));
+        bm = fmax(bm, fabs(b[u]));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        r = fmax(r, __shfl_xor(r, o, 64));
+        xm = fmax(xm, __shfl_xor(xm, o, 64));
+        bm = fmax(bm, __shfl_xor(bm, o, 64));
+    }
+    if (lane == 0) {
+        sh[0][wave] = r;
+        sh[1][wave] = xm;
+        sh[2][wave] = bm;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0;
+        for (int w = 0; w < 4; w++) t = fmax(t, sh[threadIdx.x][w]);
+        partials[((int64_t)c * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = t;
+    }
+}
+
+// berr[c] = max|b - A x| / (normInf max|x| + max|b|) of column c (thread c),
+// rounded step by step as the host computes it (no contraction)
+__global__ void k_berr_final_user(int nparts, int nrhs, const double *__restrict__ partials, double normInf,
+                                  double *__restrict__ berr) {
+    const int c = threadIdx.x;
+    if (c >= nrhs) return;
+    double t[3] = {0, 0, 0};
+    for (int b = 0; b < nparts; b++)
+        for (int i = 0; i < 3; i++) t[i] = fmax(t[i], partials[((int64_t)c * nparts + b) * 3 + i]);
+    const double den = __dadd_rn(__dmul_rn(normInf, t[1]), t[2]);
+    berr[c] = den > 0 ? __ddiv_rn(t[0], den) : t[0];
+}
+
+void launch_berr_user(int64_t n, const int64_t *rowptr, const int32_t *cols, const double *vals,
+                      const int32_t *peri, const double *x, int64_t ldx, const double *b, int64_t ldb, int nrhs,
+                      double normInf, double *partials, double *berr, hipStream_t st) {
+    if (nrhs < 1) return;
+    hipLaunchKernelGGL(k_berr_part_user, dim3(RED_BLOCKS, nrhs), dim3(256), 0, st, n, rowptr, cols, vals, peri, x,
+                       ldx, b, ldb, partials);
+    hipLaunchKernelGGL(k_berr_final_user, dim3(1), dim3(64), 0, st, RED_BLOCKS, nrhs, partials, normInf, berr);
 }
 
 #define INST(T)                                                                                               \

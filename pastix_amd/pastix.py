@@ -387,6 +387,19 @@ def backward_error(pastix_data, b, x):
     return out.value
 
 
+lib.pastix_amd_backward_error_device_noalloc.argtypes = [_P, _I, ct.c_void_p, _I, ct.c_void_p, _I, ct.c_void_p]
+lib.pastix_amd_backward_error_device_noalloc.restype = ct.c_int
+
+
+def backward_error_device_noalloc(pastix_data, db, dx, nrhs, dberr):
+    """Backward errors of the nrhs columns of the DeviceArrays dx against db
+    (n x nrhs, user ordering) into the DeviceArray dberr, without any device
+    allocation (library-static scratch, nrhs <= 64)."""
+    n = db.shape[0]
+    _check("pastix_amd_backward_error_device_noalloc",
+           lib.pastix_amd_backward_error_device_noalloc(pastix_data.ptr, nrhs, db.ptr, n, dx.ptr, n, dberr.ptr))
+
+
 lib.pastix_amd_fact_times.argtypes = [_P, _dp]
 lib.pastix_amd_fact_times.restype = ct.c_int
 lib.pastix_amd_flops.argtypes = [_P, _dp]
