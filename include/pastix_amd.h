@@ -39,7 +39,15 @@ typedef enum pastix_amd_iparm_e {
      * HIP stream, beside the next level's panel steps (default 1; 0: one
      * stream, level after level).  Results are identical either way.       */
     PXA_IPARM_OVERLAP          = 6,
-    PXA_IPARM_SIZE             = 7
+    /* LL^t: factor the leaf region (the levels from the leaves up whose
+     * supernodes are at most 64 columns wide) in one persistent launch whose
+     * tasks are released by device-side dependency counters, as the
+     * reference's dynamic scheduler releases them (default 0: level by
+     * level like the levels above; 1 is not validated on the device yet).   */
+    PXA_IPARM_LEAF_TASKS       = 7,
+    /* Workgroups of that persistent launch (default 0: 512).                */
+    PXA_IPARM_LEAF_GRID        = 8,
+    PXA_IPARM_SIZE             = 9
 } pastix_amd_iparm_t;
 
 /* Set / get an engine parameter.  Returns PASTIX_SUCCESS or
@@ -146,10 +154,11 @@ int pastix_amd_fact_times(const pastix_data_t *pastix_data, double out[8]);
  * update of one level per span), 1 single-source update tiles
  * (k_update_direct), 2 multi-source product tiles (k_update_wgemm), 3
  * multi-source ordered sums (k_update_gather), 4 single-pass multi-source
- * update (k_update_acc, used when a level's products exceed the workspace).
+ * update (k_update_acc, used when a level's products exceed the workspace),
+ * 5 the leaf region's persistent task launch (PXA_IPARM_LEAF_TASKS).
  * Every span of classes 1-4 is exactly one kernel launch (LU runs the L
  * and U sides as separate launches). */
-int pastix_amd_kernel_times(const pastix_data_t *pastix_data, double out[10]);
+int pastix_amd_kernel_times(const pastix_data_t *pastix_data, double out[12]);
 
 /* Schur-update flops by kernel, counting only the (p, q) pairs the
  * reference's per-block GEMMs compute: out[0] single-source tiles and out[1]
@@ -157,8 +166,11 @@ int pastix_amd_kernel_times(const pastix_data_t *pastix_data, double out[10]);
  * L side (block(p) >= block(q)), out[2], out[3] the same for the U side of
  * LU (block(p) > block(q)), out[4] the flops of the multi-source product
  * tiles as computed (one side), out[5], out[6] the multi-source tiles of
- * single-pass levels (k_update_acc), L and U side. */
-int pastix_amd_update_flops(const pastix_data_t *pastix_data, double out[7]);
+ * single-pass levels (k_update_acc), L and U side, out[7] the updates inside
+ * the leaf region's persistent launch.  With that launch on, out[0..6] count
+ * what the level kernels run: the region's levels leave them and its
+ * updates into the supernodes above it join out[0] and out[5]. */
+int pastix_amd_update_flops(const pastix_data_t *pastix_data, double out[8]);
 
 /* Algorithmic flops of the analysed factorisation: out[0]=DPARM_FACT_THFLOPS,
  * [1]=diagonal factorisations, [2]=panel TRSMs, [3]=Schur updates counted

@@ -105,6 +105,8 @@ extern "C" void pastixInitWithAffinity(pastix_data_t **pastix_data, PASTIX_Comm,
     d->ext[PXA_IPARM_DEVICE] = 0;
     d->ext[PXA_IPARM_LEVEL_ORDER] = 1;
     d->ext[PXA_IPARM_OVERLAP] = 1;
+    // off: its first GPU runs did not finish (the cause is not found yet)
+    d->ext[PXA_IPARM_LEAF_TASKS] = 0;
     d->steps = STEP_INIT;
     *pastix_data = d;
 }
@@ -632,6 +634,21 @@ extern "C" int pastix_subtask_spm2bcsc(pastix_data_t *d, spmatrix_t *spm) {
     bc.normF = std::sqrt(f2);
     bc.normInf = ninf;
     d->dparm[DPARM_A_NORM] = bc.normF;
+    if (d->batch_off.size() > 2 && d->batch_off.back() == d->n) {
+        // a batch: the static-pivot threshold sqrt(eps) ||A|| (reference
+        // sopalin/pastix_task_sopalin.c:401-425) takes the smallest of the
+        // systems' own norms, so that no system sees a larger threshold than
+        // it would alone (the union's norm grows with the number of systems);
+        // the ordering keeps each system in its own row range
+        double mn = -1;
+        for (size_t q = 0; q + 1 < d->batch_off.size(); q++) {
+            double g2 = 0;
+            for (int64_t i = d->batch_off[q]; i < d->batch_off[q + 1]; i++)
+                for (int64_t k = bc.rowptr[i]; k < bc.rowptr[i + 1]; k++) g2 += bc.tvals[k] * bc.tvals[k];
+            if (g2 > 0 && (mn < 0 || g2 < mn)) mn = g2;
+        }
+        if (mn > 0) d->dparm[DPARM_A_NORM] = std::sqrt(mn);
+    }
     d->flttype = spm->flttype == SpmPattern ? SpmDouble : spm->flttype;
     d->csc = spm;
     d->steps |= STEP_CSC2BCSC;
@@ -686,6 +703,8 @@ static void engine_flags(pastix_data_t *d) {
     d->engine->profile = d->ext[PXA_IPARM_PROFILE] != 0;
     d->engine->use_graph = d->ext[PXA_IPARM_GRAPH] != 0;
     d->engine->overlap = d->ext[PXA_IPARM_OVERLAP] != 0;
+    d->engine->use_leaf = d->ext[PXA_IPARM_LEAF_TASKS] != 0;
+    d->engine->leaf_grid = (int)std::max<pastix_int_t>(0, d->ext[PXA_IPARM_LEAF_GRID]);
 }
 
 static int factor_prec(pastix_data_t *d) {
@@ -1425,7 +1444,7 @@ extern "C" int pastix_amd_backward_error_device_noalloc(pastix_data_t *d, pastix
     return PASTIX_SUCCESS;
 }
 
-extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[10]) {
+extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[12]) {
     if (!d || !d->engine) return PASTIX_ERR_BADPARAMETER;
     for (int c = 0; c < pxa::Engine::KC_N; c++) {
         out[2 * c] = d->engine->kt[c];
@@ -1434,7 +1453,7 @@ extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[10]) {
     return PASTIX_SUCCESS;
 }
 
-extern "C" int pastix_amd_update_flops(const pastix_data_t *d, double out[7]) {
+extern "C" int pastix_amd_update_flops(const pastix_data_t *d, double out[8]) {
     if (!d || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
     const pxa::Plan &P = d->plan;
     out[0] = P.flops_kind[0][0];
@@ -1444,6 +1463,16 @@ extern "C" int pastix_amd_update_flops(const pastix_data_t *d, double out[7]) {
     out[4] = P.flops_wtile;
     out[5] = P.flops_kind[0][2];
     out[6] = P.flops_kind[1][2];
+    out[7] = 0;
+    if (leaf_tasks_on(d)) {
+        // the region's levels run as one persistent launch: their tiles leave
+        // the level kernels' counts, its updates above it join them
+        out[0] += P.flops_leaf_out[0] - P.flops_kind_leaf[0][0];
+        out[1] -= P.flops_kind_leaf[0][1];
+        out[4] -= P.flops_wtile_leaf;
+        out[5] += P.flops_leaf_out[1] - P.flops_kind_leaf[0][2];
+        out[7] = P.flops_leaf_upd;
+    }
     return PASTIX_SUCCESS;
 }
 

@@ -73,6 +73,19 @@ struct DiagItem {
     int32_t c, ib, b0, b1;
 };
 
+// Leaf-region task lists on the device (see Plan::leaf_*): incoming update
+// tiles per region cblk, the region cblks each one faces, the initial
+// counters and the initially ready cblks.
+struct LeafDev {
+    const int32_t *in_ptr;   // cblknbr + 1
+    const UpdItem *in;
+    const int32_t *tgt_ptr;  // cblknbr + 1
+    const int32_t *tgt;
+    const int32_t *ctr0;     // cblknbr
+    const int32_t *init;     // ninit
+    int32_t ninit, nregion, cblknbr, pad;
+};
+
 // Blocked panel step tile: rows [r0, r0+64) (and columns [c0, c0+64) for the
 // trailing update) of cblk c's panel.
 struct PanelItem {

@@ -125,6 +125,34 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     schurfcol_ = (P.schur_cblk < cblknbr_) ? s.cblk_fcol[P.schur_cblk] : n_;
     d_fact_cblks_ = upload(*this, P.fact_cblks);
     n_fact_ = (int)P.fact_cblks.size();
+    // leaf region of LL^t factorisations
+    leaf_ = LeafDev{};
+    leaf_levels_ = 0;
+    if (factotype != 2 && P.leaf_levels > 0 && P.leaf_n > 0) {
+        leaf_levels_ = P.leaf_levels;
+        int rate_khz = 0;
+        if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev_) != hipSuccess || rate_khz <= 0)
+            rate_khz = 100000;
+        leaf_maxwait_ = 2000LL * rate_khz;
+        leaf_.in_ptr = upload(*this, P.leaf_in_ptr);
+        leaf_.in = upload(*this, P.leaf_in);
+        leaf_.tgt_ptr = upload(*this, P.leaf_tgt_ptr);
+        leaf_.tgt = upload(*this, P.leaf_tgt);
+        leaf_.ctr0 = upload(*this, P.leaf_ctr0);
+        leaf_.init = upload(*this, P.leaf_init);
+        leaf_.ninit = (int32_t)P.leaf_init.size();
+        leaf_.nregion = P.leaf_n;
+        leaf_.cblknbr = (int32_t)cblknbr_;
+        d_leaf_ctr_ = (int32_t *)alloc(std::max<int64_t>(cblknbr_, 1) * sizeof(int32_t));
+        d_leaf_queue_ = (int32_t *)alloc(std::max<int32_t>(P.leaf_n, 1) * sizeof(int32_t));
+        d_leaf_sync_ = (int32_t *)alloc(4 * sizeof(int32_t));
+        d_leaf_out_ = upload(*this, P.leaf_out);
+        n_leaf_out_ = (int32_t)P.leaf_out.size();
+        leaf_out_split_ = P.leaf_out_split;
+        if (!leaf_.in_ptr || !leaf_.tgt_ptr || !leaf_.ctr0 || !leaf_.init || !d_leaf_ctr_ || !d_leaf_queue_ ||
+            !d_leaf_sync_)
+            return fail("leaf region lists");
+    }
     lv_upd_ptr_ = P.lv_upd_ptr;
     lv_upd_split_ = P.lv_upd_split;
     lv_upd_dnext_ = P.lv_upd_dnext;
@@ -297,7 +325,30 @@ template <typename T> int Engine::factorize_t(double thr) {
     // streams are then disjoint at every moment, and every target tile still
     // receives its contributions level by level, as with one stream.
     bool side_open = false;
-    for (size_t lv = 0; lv < nl; lv++) {
+    size_t lv_first = 0;
+    if (leaf_active()) {
+        // the leaf region in one persistent launch, then its updates into the
+        // cblks above it; the level schedule takes over from its last level
+        int pk = prof_begin(stream_);
+        launch_leaf_init(leaf_, d_leaf_ctr_, d_leaf_queue_, d_leaf_sync_, stream_);
+        launch_leaf_tasks<T>(S_, leaf_, d_ents_, d_shift_, L, W1, (T *)d_WB_, thr, d_stat_, d_leaf_ctr_,
+                             d_leaf_queue_, d_leaf_sync_, leaf_grid > 0 ? leaf_grid : LEAF_GRID, leaf_maxwait_,
+                             stream_);
+        prof_end(KC_LEAF, pk, stream_);
+        if (leaf_out_split_ > 0) {
+            pk = prof_begin(stream_);
+            launch_update<T>(S_, 0, true, d_leaf_out_, leaf_out_split_, d_ents_, d_shift_, L, L, L, stream_);
+            prof_end(KC_DIRECT, pk, stream_);
+        }
+        if (n_leaf_out_ > leaf_out_split_) {
+            pk = prof_begin(stream_);
+            launch_update<T>(S_, 0, false, d_leaf_out_ + leaf_out_split_, n_leaf_out_ - leaf_out_split_, d_ents_,
+                             d_shift_, L, L, L, stream_);
+            prof_end(KC_ACC, pk, stream_);
+        }
+        lv_first = (size_t)leaf_levels_;
+    }
+    for (size_t lv = lv_first; lv < nl; lv++) {
         int pk = prof_begin(stream_);
         if (facto_ == 2) {
             // the level's factored cblks (the Schur ones keep their sums)
@@ -551,7 +602,8 @@ int Engine::enqueue_factorize(double thr) {
 int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) {
     if (!inited_) return fail("engine not initialised");
     const bool graph = use_graph && !profile;
-    if (graph && (!fact_exec_ || thr != fact_thr_ || overlap != fact_overlap_)) {
+    const int leaf_key = leaf_active() ? 1 + leaf_grid : 0;
+    if (graph && (!fact_exec_ || thr != fact_thr_ || overlap != fact_overlap_ || leaf_key != fact_leaf_)) {
         if (fact_exec_) {
             HIPCHK(hipGraphExecDestroy(fact_exec_));
             fact_exec_ = nullptr;
@@ -572,6 +624,7 @@ int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) 
         }
         fact_thr_ = thr;
         fact_overlap_ = overlap;
+        fact_leaf_ = leaf_key;
     }
     HIPCHK(hipEventRecord(ev0_, stream_));
     if (graph) {
@@ -595,13 +648,14 @@ int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) 
             kt[sp.cls] += a * 1e-3;
             kn[sp.cls]++;
         }
-        t_diag = kt[KC_PANEL];
-        n_diag = kn[KC_PANEL];
+        t_diag = kt[KC_PANEL] + kt[KC_LEAF];
+        n_diag = kn[KC_PANEL] + kn[KC_LEAF];
         t_upd = kt[KC_DIRECT] + kt[KC_WGEMM] + kt[KC_GATHER] + kt[KC_ACC];
         n_upd = kn[KC_DIRECT] + kn[KC_WGEMM] + kn[KC_GATHER] + kn[KC_ACC];
     }
     int st[4];
     HIPCHK(hipMemcpy(st, d_stat_, sizeof(st), hipMemcpyDeviceToHost));
+    if (st[2]) return fail("leaf-region task queue did not drain (code " + std::to_string(st[2]) + ")");
     if (npiv) *npiv = st[0];
     if (negdiag) *negdiag = st[1];
     if (seconds) *seconds = ms * 1e-3;

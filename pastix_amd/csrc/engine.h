@@ -151,7 +151,7 @@ class Engine {
     // update classes time one launch per span; the panel class times the
     // panel steps of one level (diagonal blocks, panel TRSM, trailing panel
     // update) as one span.
-    enum KClass { KC_PANEL = 0, KC_DIRECT, KC_WGEMM, KC_GATHER, KC_ACC, KC_N };
+    enum KClass { KC_PANEL = 0, KC_DIRECT, KC_WGEMM, KC_GATHER, KC_ACC, KC_LEAF, KC_N };
     bool profile = false;
     // Replay numfact and device solves from captured HIP graphs (one graph
     // launch per factorisation / solve instead of one launch per kernel).
@@ -159,8 +159,15 @@ class Engine {
     // Overlap each level's updates into later levels with the next level's
     // panel steps (second stream); off: one stream, level by level.
     bool overlap = true;
-    double kt[KC_N] = {0, 0, 0, 0, 0};
-    int kn[KC_N] = {0, 0, 0, 0, 0};
+    // Factor the leaf region (LL^t) in one persistent launch driven by
+    // device-side dependency counters; off: level by level like the rest.
+    bool use_leaf = true;
+    int leaf_grid = 0;  // persistent workgroups of that launch (0: LEAF_GRID)
+    static constexpr int LEAF_GRID = 512;
+
+    bool leaf_active() const { return use_leaf && facto_ != 2 && leaf_.nregion > 0; }
+    double kt[KC_N] = {0, 0, 0, 0, 0, 0};
+    int kn[KC_N] = {0, 0, 0, 0, 0, 0};
     double t_diag = 0, t_upd = 0;
     int n_diag = 0, n_upd = 0;
 
@@ -217,6 +224,14 @@ class Engine {
     std::vector<uint8_t> lv_single_pass_;
     void *d_wspace_ = nullptr;
     int64_t wcap_ = 0;  // workspace entries
+    // leaf region (LL^t): task lists, counters, ready queue, the updates it
+    // sends above itself
+    LeafDev leaf_{};
+    int32_t leaf_levels_ = 0;
+    long long leaf_maxwait_ = 0;  // wall-clock ticks of one wait in that launch (two seconds)
+    int32_t *d_leaf_ctr_ = nullptr, *d_leaf_queue_ = nullptr, *d_leaf_sync_ = nullptr;
+    UpdItem *d_leaf_out_ = nullptr;
+    int32_t n_leaf_out_ = 0, leaf_out_split_ = 0;
     FwdItem *d_fwd_ = nullptr;
     DiagItem *d_diag_ = nullptr;
     std::vector<int32_t> lv_diag_ptr_;
@@ -232,6 +247,7 @@ class Engine {
     hipGraphExec_t fact_exec_ = nullptr;
     double fact_thr_ = 0;
     bool fact_overlap_ = true;
+    int fact_leaf_ = -1;
     struct SolveGraph {
         const void *ptr;
         int64_t ld;

@@ -130,6 +130,15 @@ void launch_gemv_n(int64_t n, int m, const double *V, int64_t ldv, const double 
                    hipStream_t st);
 void launch_berr_parts(int64_t n, const int64_t *colptr, const int32_t *rows, const double *vals, const double *x,
                        const double *b, double *partials, double *out, hipStream_t st);
+// Leaf region: counters and queue set up, then the persistent task launch
+// (grid workgroups at most; a wait gives up after maxwait ticks of the
+// device's constant-rate wall clock and sets stat[2]).
+void launch_leaf_init(const LeafDev &LP, int32_t *ctr, int32_t *queue, int32_t *sync, hipStream_t st);
+template <typename T>
+void launch_leaf_tasks(const DevSymbol &S, const LeafDev &LP, const UpdEntry *ents, const int32_t *shift, T *coef,
+                       T *winv, T *winvt, double thr, int *stat, int32_t *ctr, int32_t *queue, int32_t *sync,
+                       int grid, long long maxwait, hipStream_t st);
+
 // Backward error without allocation: partial maxima in a library-static
 // device array.  NOALLOC_MAX_RHS columns at most; noalloc_scratch_init
 // resolves the array's address on the current device (call it once per

@@ -15,8 +15,11 @@
 // k_winv_offdiag completes the inverse of the whole diagonal block for the
 // solve once the panel is factored.  Inverses are stored twice: W (lower,
 // column-major) for the forward sweep and WT = W^T for the backward sweep.
+#include <algorithm>
+
 #include "dev_common.h"
 #include "dev_types.h"
+#include "dev_update.h"
 #include "kernels.h"
 
 namespace pxa {
@@ -212,13 +215,22 @@ __device__ __forceinline__ void lower_inverse(T (*Ls)[65], T (*Xs)[65], T *Ts, c
 // The inverse X = inv(L) comes from lower_inverse (recursive doubling, in
 // place of L in LDS).  W receives X (lower, column-major, ld = K) and WT its
 // transpose, both with explicit zeros in the other triangle of the sub-block.
+// LDS of one diagonal-block factorisation: the published columns (then the
+// inverse's scratch), the factor and the pivots' reciprocals.
+template <typename T> struct PotrfLds {
+    T fac[1024];
+    T Ls[64][65];
+    T dinv[64];
+};
+
 template <typename T>
 __device__ __forceinline__ void potrf64_block(const DevSymbol &S, const int c, int kb, T *__restrict__ coef,
-                                              T *__restrict__ winv, T *__restrict__ winvt, double thr, int *stat) {
-    __shared__ T fac[1024];
+                                              T *__restrict__ winv, T *__restrict__ winvt, double thr, int *stat,
+                                              PotrfLds<T> &lds) {
+    T *fac = lds.fac;
     T(*colb)[4][64] = reinterpret_cast<T(*)[4][64]>(fac);  // [parity][w][row]
-    __shared__ T Ls[64][65];
-    __shared__ T dinv[64];
+    T(*Ls)[65] = lds.Ls;
+    T *dinv = lds.dinv;
     T *Ts = fac;  // inverse scratch once the factor is done
     const int tid = threadIdx.x, i = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -329,20 +341,18 @@ __global__ __launch_bounds__(256) void k_potrf64(DevSymbol S, const int32_t *__r
                                                  T *__restrict__ coef, T *__restrict__ winv, T *__restrict__ winvt,
                                                  double thr, int *stat) {
     __builtin_amdgcn_s_setprio(3);  // panel steps lead the critical path
-    potrf64_block<T>(S, cblks[blockIdx.x], kb, coef, winv, winvt, thr, stat);
+    __shared__ PotrfLds<T> lds;
+    potrf64_block<T>(S, cblks[blockIdx.x], kb, coef, winv, winvt, thr, stat, lds);
 }
 
-// rows [r0, r0+64) below the diagonal sub-block kb: X = A * W_kk^T
+// rows [r0, r0+64) of cblk c below the diagonal sub-block kb: X = A * W_kk^T
+// (all 256 threads; contains barriers)
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
-                                                    int kb, T *__restrict__ coef, const T *__restrict__ winv) {
-    __builtin_amdgcn_s_setprio(3);
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    if (id >= nitems) return;
+__device__ __forceinline__ void panel_trsm_tile(const DevSymbol &S, const int c, const int r0, int kb,
+                                                T *__restrict__ coef, const T *__restrict__ winv,
+                                                StageBuf<T, 16> &sb) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    const PanelItem it = items[id];
-    const int c = it.c, r0 = it.r0;
     const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
     const int stride = S.cblk_stride[c];
     const int64_t ld = stride;
@@ -350,7 +360,6 @@ __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem
     T *P = coef + S.cblk_coef[c];
     const T *W = winv + S.cblk_winv[c];
     const int mr = min(64, stride - r0);
-    __shared__ StageBuf<T, 16> sb;
     Quad<T> q;
     q.zero();
     // the row tile is read completely (staged through LDS) before it is overwritten
@@ -358,6 +367,17 @@ __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem
     if (wr < mr && wc < nb) {
         QFOR(if (qrow < mr - wr && qcol < nb - wc) P[(r0 + wr + qrow) + (int64_t)(k0 + wc + qcol) * ld] = q.c[m_][n_][r_];)
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
+                                                    int kb, T *__restrict__ coef, const T *__restrict__ winv) {
+    __builtin_amdgcn_s_setprio(3);
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    __shared__ StageBuf<T, 16> sb;
+    const PanelItem it = items[id];
+    panel_trsm_tile<T>(S, it.c, it.r0, kb, coef, winv, sb);
 }
 
 // In-cblk update of column block c0 of the panel (item), 64-row tile r0:
@@ -429,7 +449,8 @@ __global__ __launch_bounds__(256) void k_panel_upd_potrf(DevSymbol S, const int3
         const int c = cblks[id];
         panel_upd_tile<T>(S, PanelItem{c, k1, k1, 0}, kb, right, coef, coef, coef, 0);
         __syncthreads();  // the tile's stores reach the whole workgroup
-        potrf64_block<T>(S, c, kf, coef, winv, winvt, thr, stat);
+        __shared__ PotrfLds<T> lds;
+        potrf64_block<T>(S, c, kf, coef, winv, winvt, thr, stat, lds);
         return;
     }
     if (id - npot >= nitems) return;
@@ -684,6 +705,121 @@ __global__ __launch_bounds__(256) void k_lu_square_layout(DevSymbol S, const int
     }
 }
 
+// ---------------------------------------------------------------------------
+// Leaf region: one persistent launch whose tasks are driven by device-side
+// dependency counters (reference kernels/cpucblk_zmpi_coeftab.c:482-562,
+// cpucblk_zincoming_deps / cpucblk_zrelease_deps, with counters as
+// blend/solver_backup.c:95-101 sets them, and the ready queue of the dynamic
+// scheduler, sopalin/sequential_zpotrf.c:129-234).
+//
+// A task is one region cblk c (at most 64 columns): it pulls the updates of
+// every region cblk facing it (its incoming tiles, entries in source order,
+// owner computes: no other workgroup writes c), factors the diagonal block
+// with its inverse, solves the rows below, then decrements the counter of
+// every region cblk it faces; the decrement that reaches zero appends that
+// cblk to the ready queue.  Workgroups take queue slots in order with one
+// atomic; a slot's cblk is published with a release store once its counter
+// reaches zero.  Every cblk enters the queue exactly once, and a taken slot
+// whose cblk is not yet published waits only for tasks taken before it, all
+// of them running on workgroups that wait for nothing: the queue drains
+// whatever the number of resident workgroups.  Every wait is bounded: after
+// `maxwait` ticks of the constant-rate wall clock without its slot being
+// published a workgroup sets stat[2], every workgroup leaves, and the host
+// reports the error.
+//
+// Memory ordering: a task's stores are released at agent scope by every wave
+// (the writes of other compute units' L2 become visible to the consumer),
+// the counter decrement a release, the last decrement's thread acquires the
+// earlier ones before the release store of the queue slot, and the consumer
+// polls the slot with relaxed loads, then takes one acquire fence; the other
+// waves of its workgroup are ordered after that fence by the barrier.
+// sync[0]: next queue slot to take, sync[1]: next slot to publish.
+template <typename T> union LeafLds {
+    UpdAccLds<T> u;
+    PotrfLds<T> p;
+};
+
+__global__ __launch_bounds__(256) void k_leaf_init(LeafDev LP, int32_t *__restrict__ ctr,
+                                                   int32_t *__restrict__ queue, int32_t *__restrict__ sync) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < LP.cblknbr) ctr[i] = LP.ctr0[i];
+    if (i < LP.nregion) queue[i] = (i < LP.ninit) ? LP.init[i] : -1;
+    if (i == 0) {
+        sync[0] = 0;
+        sync[1] = LP.ninit;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_leaf_tasks(DevSymbol S, LeafDev LP, const UpdEntry *__restrict__ ents,
+                                                    const int32_t *__restrict__ shift, T *__restrict__ coef,
+                                                    T *__restrict__ winv, T *__restrict__ winvt, double thr,
+                                                    int *stat, int32_t *__restrict__ ctr,
+                                                    int32_t *__restrict__ queue, int32_t *__restrict__ sync,
+                                                    long long maxwait) {
+    __shared__ LeafLds<T> lds;
+    __shared__ int task;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) {
+            int c = -1;
+            const int t = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t < LP.nregion) {
+                // relaxed polls (an acquire load would invalidate the L2 on
+                // every poll), one acquire fence once the slot is published;
+                // the wait is bounded in time (constant-rate wall clock)
+                const unsigned long long t0 = wall_clock64();
+                for (;;) {
+                    c = __hip_atomic_load(&queue[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (c >= 0) break;
+                    if (__hip_atomic_load(&stat[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                        wall_clock64() - t0 > (unsigned long long)maxwait) {
+                        __hip_atomic_store(&stat[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        c = -1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(8);
+                }
+                if (c >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            task = c;
+        }
+        __syncthreads();
+        const int c = task;
+        __syncthreads();  // every thread holds the task before thread 0 takes the next one
+        if (c < 0 || c >= LP.cblknbr) break;
+        // incoming updates (each tile ends with a barrier after its stores)
+        for (int k = LP.in_ptr[c]; k < LP.in_ptr[c + 1]; k++)
+            update_acc_tile<T, 0>(S, LP.in[k], ents, shift, coef, coef, coef, lds.u);
+        // the diagonal block and its inverse, then the rows below it
+        potrf64_block<T>(S, c, 0, coef, winv, winvt, thr, stat, lds.p);
+        __syncthreads();
+        const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+        const int stride = S.cblk_stride[c];
+        for (int r0 = K; r0 < stride; r0 += 64) {
+            panel_trsm_tile<T>(S, c, r0, 0, coef, winv, lds.u.sb);
+            __syncthreads();
+        }
+        // release the panel and the inverse, then the counters it feeds
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) {
+            for (int k = LP.tgt_ptr[c]; k < LP.tgt_ptr[c + 1]; k++) {
+                const int f = LP.tgt[k];
+                // release: this task's writes (fenced above by every wave);
+                // the last decrement also acquires the earlier ones before it
+                // publishes f
+                if (__hip_atomic_fetch_add(&ctr[f], -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == 1) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    const int pos = __hip_atomic_fetch_add(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (pos < LP.nregion) __hip_atomic_store(&queue[pos], f, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    else __hip_atomic_store(&stat[2], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------- launchers ---------------------------------
 static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
 
@@ -737,7 +873,24 @@ void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool t
                        winv, winvt);
 }
 
+void launch_leaf_init(const LeafDev &LP, int32_t *ctr, int32_t *queue, int32_t *sync, hipStream_t st) {
+    const int n = std::max(std::max(LP.cblknbr, LP.nregion), 1);
+    hipLaunchKernelGGL(k_leaf_init, dim3((n + 255) / 256), dim3(256), 0, st, LP, ctr, queue, sync);
+}
+template <typename T>
+void launch_leaf_tasks(const DevSymbol &S, const LeafDev &LP, const UpdEntry *ents, const int32_t *shift, T *coef,
+                       T *winv, T *winvt, double thr, int *stat, int32_t *ctr, int32_t *queue, int32_t *sync,
+                       int grid, long long maxwait, hipStream_t st) {
+    if (LP.nregion <= 0) return;
+    const int g = std::max(1, std::min(grid, LP.nregion));
+    hipLaunchKernelGGL(k_leaf_tasks<T>, dim3(g), dim3(256), 0, st, S, LP, ents, shift, coef, winv, winvt, thr, stat,
+                       ctr, queue, sync, maxwait);
+}
+
 #define INST(T)                                                                                                    \
+    template void launch_leaf_tasks<T>(const DevSymbol &, const LeafDev &, const UpdEntry *, const int32_t *, T *, \
+                                       T *, T *, double, int *, int32_t *, int32_t *, int32_t *, int, long long,   \
+                                       hipStream_t);                                                               \
     template void launch_getrf64<T>(const DevSymbol &, const int32_t *, int, int, T *, T *, T *, T *, T *, double, \
                                     int *, hipStream_t);                                                           \
     template void launch_lu_square_layout<T>(const DevSymbol &, const int32_t *, int, T *, T *, hipStream_t);      \

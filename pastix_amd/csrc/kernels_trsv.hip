@@ -118,18 +118,29 @@ __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem
     const int fc = S.cblk_fcol[c];
     const int Kc = S.cblk_fcol[c + 1] - fc;
     const int nrow = min(Kc, r0 + 64);
-    for (int t = tid; t < nrow * nrhs; t += NW * 64) {
-        const int i = t % nrow, u = t / nrow;
-        const int g = fc + i;
-        const T *yb = ybuf + u * ldy;
-        T v = b[g + u * ldb];
-        int64_t p = yrow_ptr[g];
-        const int64_t pe = yrow_ptr[g + 1];
+    // y = b minus the pending products of rows [0, nrow).  A row's entries
+    // are contiguous in the update buffer, and so are the rows of one cblk:
+    // gl consecutive lanes per row (the power of two at most the cblk's
+    // first row's entry count, up to a whole wave), lane j summing entries
+    // j, j + gl, ... (coalesced across the lanes of a row), the lanes' sums
+    // combined by xor shuffles in a fixed order
+    const int tasks = nrow * nrhs;
+    const int ne = (int)(yrow_ptr[fc + 1] - yrow_ptr[fc]);
+    int gl = 1;
+    while (gl < 64 && gl * 2 <= ne) gl *= 2;
 #pragma unroll 4
-        for (; p + 4 <= pe; p += 4)
-            v -= ((ld_co<CO>(yb + p) + ld_co<CO>(yb + p + 1)) + (ld_co<CO>(yb + p + 2) + ld_co<CO>(yb + p + 3)));
-        for (; p < pe; p++) v -= ld_co<CO>(yb + p);
-        y[i + u * Kc] = v;
+    for (int base = 0; base < tasks * gl; base += NW * 64) {
+        const int t = base + tid, task = t / gl, j = t % gl;
+        const int i = task % nrow, u = task / nrow;
+        T v = 0;
+        if (task < tasks) {
+            const T *yb = ybuf + u * ldy;
+            const int64_t pe = yrow_ptr[fc + i + 1];
+#pragma unroll 8
+            for (int64_t p = yrow_ptr[fc + i] + j; p < pe; p += gl) v += ld_co<CO>(yb + p);
+        }
+        for (int o = gl >> 1; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+        if (task < tasks && j == 0) y[i + u * Kc] = b[fc + i + u * ldb] - v;
     }
     __syncthreads();
     const int i = r0 + r;
@@ -202,9 +213,12 @@ __device__ __forceinline__ void fwd_prod_body(const DevSymbol &S, const FwdItem 
     wave_sum_store_rows<CO, T, NR, NW, E>(acc, red, r, q, r < nr, nrhs, ybuf + dst, ldy);
 }
 
-// one 64x64 tile (column block k0/64) of the backward products, 4 waves
-// nrhs right-hand sides starting at column u0 of the partials (laid out for
-// nrhs_tot columns); x points at column u0 already
+// one strip (panel rows [r0, r1), any multiple of 64 of them, column block
+// k0/64) of the backward products, 4 waves: partial[j] = sum over the strip's
+// rows of L[r, k0+j] x[row(r)], 64 rows at a time, the next chunk's loads in
+// flight while the current one is summed.  nrhs right-hand sides starting at
+// column u0 of the partials (laid out for nrhs_tot columns); x points at
+// column u0 already
 template <bool CO, typename T, int NR>
 __device__ __forceinline__ void bwd_prod_body(const DevSymbol &S, const BwdItem it, int k0,
                                               const T *__restrict__ coef, const T *__restrict__ x, int64_t ldx,
@@ -215,33 +229,49 @@ __device__ __forceinline__ void bwd_prod_body(const DevSymbol &S, const BwdItem 
     const int Kc = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
     const int64_t ld = S.cblk_stride[c];
     const int64_t rb = S.cblk_rowbase[c];
-    const int nr = it.r1 - it.r0;
-    for (int t = tid; t < 64 * NR; t += 256) {
-        const int row = t & 63, u = t >> 6;
-        xr[t] = (row < nr && u < nrhs) ? ld_co<CO>(x + S.panel_grow[rb + it.r0 + row] + u * ldx) : (T)0;
-    }
-    const T *P = coef + S.cblk_coef[c] + it.r0;
-    {
-        // E consecutive rows per lane (one 16-byte load), 4E columns per pass
-        constexpr int E = Vec<T>::E, G = 64 / E;
-        const int rr = E * (lane % G), kc = lane / G;
+    // E consecutive rows per lane (one 16-byte load), 4E columns per pass
+    constexpr int E = Vec<T>::E, G = 64 / E, NP = 64 / (4 * E);
+    const int rr = E * (lane % G), kc = lane / G;
+    const T *P = coef + S.cblk_coef[c];
+    T a[NP][E], xv[NR];
+    auto gload = [&](int r0) {
+        const int nr = min(64, it.r1 - r0);
 #pragma unroll
-        for (int cc = q * E + kc; cc < 64; cc += 4 * E) {
-            T a[E];
-            ld_rows(P + rr + (int64_t)(k0 + cc) * ld, rr, (k0 + cc < Kc) ? nr : 0, a);
-#pragma unroll
-            for (int i = 0; i < E; i++) tile[rr + i][cc] = a[i];
+        for (int u = 0; u < NP; u++) {
+            const int cc = q * E + kc + 4 * E * u;
+            ld_rows(P + r0 + rr + (int64_t)(k0 + cc) * ld, rr, (k0 + cc < Kc) ? nr : 0, a[u]);
         }
-    }
-    __syncthreads();
+        if (tid < 64) {
+#pragma unroll
+            for (int v = 0; v < NR; v++)
+                xv[v] = (tid < nr && v < nrhs) ? ld_co<CO>(x + S.panel_grow[rb + r0 + tid] + v * ldx) : (T)0;
+        }
+    };
     T acc[NR];
 #pragma unroll
     for (int u = 0; u < NR; u++) acc[u] = 0;
-#pragma unroll 4
-    for (int row = q * 16; row < q * 16 + 16; row++) {
-        const T a = tile[row][lane];
+    gload(it.r0);
+    for (int r0 = it.r0; r0 < it.r1; r0 += 64) {
+        // the chunk in registers goes to LDS, the next one is fetched
 #pragma unroll
-        for (int u = 0; u < NR; u++) acc[u] += a * xr[row + u * 64];
+        for (int u = 0; u < NP; u++) {
+            const int cc = q * E + kc + 4 * E * u;
+#pragma unroll
+            for (int i = 0; i < E; i++) tile[rr + i][cc] = a[u][i];
+        }
+        if (tid < 64) {
+#pragma unroll
+            for (int v = 0; v < NR; v++) xr[tid + v * 64] = xv[v];
+        }
+        __syncthreads();
+        if (r0 + 64 < it.r1) gload(r0 + 64);
+#pragma unroll 4
+        for (int row = q * 16; row < q * 16 + 16; row++) {
+            const T t = tile[row][lane];
+#pragma unroll
+            for (int u = 0; u < NR; u++) acc[u] += t * xr[row + u * 64];
+        }
+        __syncthreads();
     }
     T *out = part + (int64_t)it.slot * nrhs_tot + (int64_t)u0 * Kc + k0 + lane;
     wave_sum_store<CO, T, NR, 4>(acc, red, lane, q, k0 + lane < Kc, nrhs, out, Kc, (const T *)nullptr, 0);

@@ -15,6 +15,7 @@
 // iterblok >= blok / iterblok > blok loops.
 #include "dev_common.h"
 #include "dev_types.h"
+#include "dev_update.h"
 #include "kernels.h"
 
 namespace pxa {
@@ -23,35 +24,6 @@ namespace pxa {
 // faster than 32 on gfx950: the smaller LDS footprint keeps 6 workgroups per
 // CU resident).
 constexpr int UPD_CH = 16;
-
-#define UQFOR(...)                                                       \
-    for (int m_ = 0; m_ < 2; m_++)                                       \
-        for (int n_ = 0; n_ < 2; n_++)                                   \
-            for (int r_ = 0; r_ < 4; r_++) {                             \
-                const int qrow = m_ * 16 + Mfma<T>::drow(lane, r_);       \
-                const int qcol = n_ * 16 + (lane & 15);                   \
-                __VA_ARGS__                                              \
-            }
-
-// Row/column maps of one entry into the target tile.
-__device__ __forceinline__ void update_maps(const DevSymbol &S, const UpdEntry &en, const int32_t *shift, int r0,
-                                            int fcol_c0, int *rmap, int *cmap, int *rblk, int *qblk, int tid) {
-    const int64_t rb = S.cblk_rowbase[en.s];
-    if (tid < 64) {
-        const int p = en.p0 + tid;
-        if (p < en.p1) {
-            const int bk = S.panel_blk[rb + p];
-            rmap[tid] = S.panel_grow[rb + p] + shift[en.sb + bk - en.ib0] - r0;
-            rblk[tid] = bk;
-        }
-    } else if (tid < 128) {
-        const int q = en.q0 + tid - 64;
-        if (q < en.q1) {
-            cmap[tid - 64] = S.panel_grow[rb + q] - fcol_c0;
-            qblk[tid - 64] = S.panel_blk[rb + q];
-        }
-    }
-}
 
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_update_direct(DevSymbol S, const UpdItem *__restrict__ items, int nitems,
@@ -95,52 +67,10 @@ __global__ __launch_bounds__(256) void k_update_acc(DevSymbol S, const UpdItem *
                                                     const UpdEntry *__restrict__ ents, const int32_t *__restrict__ shift,
                                                     const T *__restrict__ Acoef, const T *__restrict__ Bcoef,
                                                     T *__restrict__ Ccoef) {
-    __shared__ T acc[64][65];
-    __shared__ StageBuf<T, 16> sb;
-    __shared__ int rmap[64], cmap[64], rblk[64], qblk[64];
+    __shared__ UpdAccLds<T> lds;
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    const UpdItem it = items[id];
-    const int f = it.f, r0 = it.r0, c0 = it.c0;
-    const int fcol = S.cblk_fcol[f];
-    for (int e = tid; e < 64 * 65; e += 256) (&acc[0][0])[e] = (T)0;
-    for (int e = it.e0; e < it.e1; e++) {
-        const UpdEntry en = ents[e];
-        update_maps(S, en, shift, r0, fcol + c0, rmap, cmap, rblk, qblk, tid);
-        const int s = en.s;
-        const int np = en.p1 - en.p0, nq = en.q1 - en.q0;
-        const int K = S.cblk_fcol[s + 1] - S.cblk_fcol[s];
-        const int64_t ld = S.cblk_stride[s];
-        Quad<T> q;
-        q.zero();
-        // wg_gemm_lds starts with a barrier-protected LDS fill: the maps are
-        // visible to every thread once it returns
-        wg_gemm_lds<T, 16>(q, Acoef + S.cblk_coef[s] + en.p0, np, ld, Bcoef + S.cblk_coef[s] + en.q0, nq, ld, K, sb,
-                       tid);
-        if (wr < np && wc < nq) {
-            UQFOR({
-                const int pi = wr + qrow, qj = wc + qcol;
-                if (pi < np && qj < nq) {
-                    const bool keep = (MODE == 0) ? (rblk[pi] >= qblk[qj]) : (rblk[pi] > qblk[qj]);
-                    if (keep) acc[rmap[pi]][cmap[qj]] += q.c[m_][n_][r_];
-                }
-            })
-        }
-        __syncthreads();
-    }
-    const int Kf = S.cblk_fcol[f + 1] - fcol;
-    const int stf = S.cblk_stride[f];
-    T *C = Ccoef + S.cblk_coef[f];
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int i = e & 63, j = e >> 6;
-        const int r = r0 + i, cc = c0 + j;
-        if (r < stf && cc < Kf) {
-            const T v = acc[i][j];
-            if (v != (T)0) C[r + (int64_t)cc * stf] -= v;
-        }
-    }
+    update_acc_tile<T, MODE>(S, items[id], ents, shift, Acoef, Bcoef, Ccoef, lds);
 }
 
 // Phase 1 of the multi-source tiles: dense 128 (p) x 64 (q) tile of a

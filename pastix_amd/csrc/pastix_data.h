@@ -112,6 +112,12 @@ inline bool is_ldlt(const pastix_data_t *d) {
     return d->iparm[IPARM_FACTORIZATION] == PastixFactSYTRF || d->iparm[IPARM_FACTORIZATION] == PastixFactHETRF;
 }
 double refine_eps(const pastix_data_t *d);
+// the leaf region runs as one persistent launch: LL^t, a region analysed,
+// PXA_IPARM_LEAF_TASKS on
+inline bool leaf_tasks_on(const pastix_data_t *d) {
+    return d->ext[PXA_IPARM_LEAF_TASKS] != 0 && d->iparm[IPARM_FACTORIZATION] != PastixFactGETRF && !is_ldlt(d) &&
+           d->plan.leaf_levels > 0;
+}
 // Host FP64 blocks (n x R, ld n, user ordering of the instance): X = M^{-1} B
 // when initial_solve (else X holds the guesses), then batched CG over the
 // row segments segoff (method 1) or nothing (method -1).

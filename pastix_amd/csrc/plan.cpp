@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <functional>
 #include <map>
 
 namespace pxa {
@@ -120,63 +121,21 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
     int64_t bslot = 0;            // backward partial slots handed out so far
     const int64_t WT_P = 128;     // source rows (p) of one product tile; 64 columns (q)
     const int32_t FWD_ROWS = 64;  // panel rows of one forward product tile
-    const int32_t BWD_ROWS = 64;  // panel rows of one backward product tile
+    // panel rows of one backward product strip: 64 on the levels of narrow
+    // cblks, BWD_ROWS_WIDE where a cblk is wider than 64 columns (its
+    // diagonal step then sums a quarter of the partials, while the level
+    // still has a few hundred strips)
+    const int32_t BWD_ROWS = 64, BWD_ROWS_WIDE = 256;
 
-    for (int64_t c = 0; c < sc; c++)
-        if (width(c) > 64) P.wide_cblks.push_back((int32_t)c);
-    P.lv_step_ptr.assign(nl + 1, 0);
-    P.step_cb_ptr.push_back(0);
-    P.step_trsm_ptr.push_back(0);
-    P.step_upd_ptr.push_back(0);
-
-    for (int32_t lv = 0; lv < nl; lv++) {
-        const int32_t l0 = P.lv_cblk_ptr[lv], l1 = P.lv_cblk_ptr[lv + 1];
-        // the factored cblks of the level: [l0, lf)
-        const int32_t lf = l0 + P.lv_fact_cnt[lv];
-        // blocked panel steps of this level
-        int64_t maxk = 0;
-        for (int32_t k = l0; k < lf; k++) maxk = std::max<int64_t>(maxk, width(P.lv_cblks[k]));
-        const int64_t nsteps = (maxk + 63) / 64;
-        // in-cblk updates: left-looking (the step's block takes all earlier
-        // blocks' products, K up to the cblk width, one tile per row tile)
-        // where the level has rows enough to fill the device; right-looking
-        // (every later block takes the step's products, 64 deep) near the top
-        // of the tree, where few row tiles would leave the device idle
-        int64_t rowtiles = 0;
-        for (int32_t k = l0; k < lf; k++) rowtiles += (L.cblk_stride[P.lv_cblks[k]] + 63) / 64;
-        const bool right = rowtiles < PANEL_RIGHT_ROWTILES;
-        P.lv_panel_right[lv] = right;
-        for (int64_t kb = 0; kb < nsteps; kb++) {
-            const int64_t k0 = kb * 64;
-            for (int32_t k = l0; k < lf; k++) {
-                const int32_t c = P.lv_cblks[k];
-                const int64_t K = width(c), st = L.cblk_stride[c];
-                if (K <= k0) continue;
-                const int64_t nb = std::min<int64_t>(64, K - k0);
-                P.step_cblks.push_back(c);
-                for (int64_t r0 = k0 + nb; r0 < st; r0 += 64) P.ptrsm.push_back(PanelItem{c, (int32_t)r0, 0, 0});
-                if (right) {
-                    // every later column block by this one, from each
-                    // block's diagonal tile down
-                    for (int64_t c0 = k0 + 64; c0 < K; c0 += 64)
-                        for (int64_t r0 = c0; r0 < st; r0 += 64)
-                            P.pupd.push_back(PanelItem{c, (int32_t)r0, (int32_t)c0, 0});
-                } else if (kb > 0) {
-                    // this column block by the blocks before it, from the
-                    // diagonal tile down
-                    for (int64_t r0 = k0; r0 < st; r0 += 64) P.pupd.push_back(PanelItem{c, (int32_t)r0, (int32_t)k0, 0});
-                }
-            }
-            P.step_cb_ptr.push_back((int32_t)P.step_cblks.size());
-            P.step_trsm_ptr.push_back((int32_t)P.ptrsm.size());
-            P.step_upd_ptr.push_back((int32_t)P.pupd.size());
-        }
-        P.lv_step_ptr[lv + 1] = P.lv_step_ptr[lv] + (int32_t)nsteps;
-
-        // update tiles from the sources of this level
+    // Update tiles of target panels from the sources `srcs` (increasing cblk
+    // index), for the targets `keep` accepts: every run of a source's blocks
+    // facing one target contributes entries to the 64x64 target tiles it
+    // lands in; a tile's entries stay in source order.  Shifts and entries are
+    // appended to P.shift and P.ents, the tiles to `out`.
+    auto build_items = [&](const std::vector<int32_t> &srcs, const std::function<bool(int64_t)> &keep,
+                           std::vector<UpdItem> &out) {
         ke.clear();
-        for (int32_t k = l0; k < lf; k++) {
-            const int64_t src = P.lv_cblks[k];
+        for (const int32_t src : srcs) {
             const int64_t last = s.cblk_bloknum[src + 1];
             int64_t b = s.cblk_bloknum[src] + 1;
             while (b < last) {
@@ -184,6 +143,7 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
                 const int64_t i0 = b;
                 while (b < last && s.blok_fcblk[b] == f) b++;
                 const int64_t i1 = b;
+                if (!keep(f)) continue;
                 const int32_t sb = (int32_t)P.shift.size();
                 int64_t fb = s.cblk_bloknum[f];
                 for (int64_t i = i0; i < last; i++) {
@@ -240,8 +200,98 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
             for (size_t t = k; t < e; t++) P.ents.push_back(ke[t].e);
             it.e1 = (int32_t)P.ents.size();
             it.pad0 = it.pad1 = it.pad2 = 0;
-            P.upd.push_back(it);
+            out.push_back(it);
             k = e;
+        }
+    };
+    // exact flops of the kept (p, q) pairs of one entry: [0] block(p) >=
+    // block(q) (LL^t, L side of LU), [1] block(p) > block(q) (U side)
+    auto entry_flops = [&](const UpdEntry &en, double &f0, double &f1) {
+        const int64_t rb = P.rowbase[en.s];
+        const double K2 = 2.0 * (double)width(en.s);
+        int64_t qa = en.q0, qb = en.q0;  // first q with blk(q) >= / > blk(p)
+        double c0 = 0, c1 = 0;
+        for (int64_t p = en.p0; p < en.p1; p++) {
+            const int32_t bp = P.panel_blk[rb + p];
+            while (qa < en.q1 && P.panel_blk[rb + qa] < bp) qa++;
+            while (qb < en.q1 && P.panel_blk[rb + qb] <= bp) qb++;
+            c0 += (double)(qb - en.q0);
+            c1 += (double)(qa - en.q0);
+        }
+        f0 = K2 * c0;
+        f1 = K2 * c1;
+    };
+    // leaf region: the longest run of levels from 0 whose cblks are all
+    // factored and at most LEAF_MAX_WIDTH wide (downward closed: a cblk's
+    // sources sit on lower levels)
+    {
+        int32_t lr = 0;
+        for (; lr < nl; lr++) {
+            bool ok = P.lv_cblk_ptr[lr + 1] > P.lv_cblk_ptr[lr];
+            for (int32_t k = P.lv_cblk_ptr[lr]; k < P.lv_cblk_ptr[lr + 1] && ok; k++) {
+                const int32_t c = P.lv_cblks[k];
+                ok = factored(c) && width(c) <= LEAF_MAX_WIDTH;
+            }
+            if (!ok) break;
+        }
+        P.leaf_levels = lr;
+    }
+
+    for (int64_t c = 0; c < sc; c++)
+        if (width(c) > 64) P.wide_cblks.push_back((int32_t)c);
+    P.lv_step_ptr.assign(nl + 1, 0);
+    P.step_cb_ptr.push_back(0);
+    P.step_trsm_ptr.push_back(0);
+    P.step_upd_ptr.push_back(0);
+
+    for (int32_t lv = 0; lv < nl; lv++) {
+        const int32_t l0 = P.lv_cblk_ptr[lv], l1 = P.lv_cblk_ptr[lv + 1];
+        // the factored cblks of the level: [l0, lf)
+        const int32_t lf = l0 + P.lv_fact_cnt[lv];
+        // blocked panel steps of this level
+        int64_t maxk = 0;
+        for (int32_t k = l0; k < lf; k++) maxk = std::max<int64_t>(maxk, width(P.lv_cblks[k]));
+        const int64_t nsteps = (maxk + 63) / 64;
+        // in-cblk updates: left-looking (the step's block takes all earlier
+        // blocks' products, K up to the cblk width, one tile per row tile)
+        // where the level has rows enough to fill the device; right-looking
+        // (every later block takes the step's products, 64 deep) near the top
+        // of the tree, where few row tiles would leave the device idle
+        int64_t rowtiles = 0;
+        for (int32_t k = l0; k < lf; k++) rowtiles += (L.cblk_stride[P.lv_cblks[k]] + 63) / 64;
+        const bool right = rowtiles < PANEL_RIGHT_ROWTILES;
+        P.lv_panel_right[lv] = right;
+        for (int64_t kb = 0; kb < nsteps; kb++) {
+            const int64_t k0 = kb * 64;
+            for (int32_t k = l0; k < lf; k++) {
+                const int32_t c = P.lv_cblks[k];
+                const int64_t K = width(c), st = L.cblk_stride[c];
+                if (K <= k0) continue;
+                const int64_t nb = std::min<int64_t>(64, K - k0);
+                P.step_cblks.push_back(c);
+                for (int64_t r0 = k0 + nb; r0 < st; r0 += 64) P.ptrsm.push_back(PanelItem{c, (int32_t)r0, 0, 0});
+                if (right) {
+                    // every later column block by this one, from each
+                    // block's diagonal tile down
+                    for (int64_t c0 = k0 + 64; c0 < K; c0 += 64)
+                        for (int64_t r0 = c0; r0 < st; r0 += 64)
+                            P.pupd.push_back(PanelItem{c, (int32_t)r0, (int32_t)c0, 0});
+                } else if (kb > 0) {
+                    // this column block by the blocks before it, from the
+                    // diagonal tile down
+                    for (int64_t r0 = k0; r0 < st; r0 += 64) P.pupd.push_back(PanelItem{c, (int32_t)r0, (int32_t)k0, 0});
+                }
+            }
+            P.step_cb_ptr.push_back((int32_t)P.step_cblks.size());
+            P.step_trsm_ptr.push_back((int32_t)P.ptrsm.size());
+            P.step_upd_ptr.push_back((int32_t)P.pupd.size());
+        }
+        P.lv_step_ptr[lv + 1] = P.lv_step_ptr[lv] + (int32_t)nsteps;
+
+        // update tiles from the sources of this level
+        {
+            std::vector<int32_t> srcs(P.lv_cblks.begin() + l0, P.lv_cblks.begin() + lf);
+            build_items(srcs, [](int64_t) { return true; }, P.upd);
         }
         // single-entry tiles first (register-direct kernel), then the others
         std::stable_partition(P.upd.begin() + P.lv_upd_ptr[lv], P.upd.end(),
@@ -272,20 +322,14 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
         for (int32_t t = P.lv_upd_ptr[lv]; t < P.lv_upd_ptr[lv + 1]; t++) {
             const int kind = (t < split) ? 0 : (single ? 2 : 1);
             for (int32_t e = P.upd[t].e0; e < P.upd[t].e1; e++) {
-                const UpdEntry &en = P.ents[e];
-                const int64_t rb = P.rowbase[en.s];
-                const double K2 = 2.0 * (double)width(en.s);
-                int64_t qa = en.q0, qb = en.q0;  // first q with blk(q) >= / > blk(p)
-                double c0 = 0, c1 = 0;
-                for (int64_t p = en.p0; p < en.p1; p++) {
-                    const int32_t bp = P.panel_blk[rb + p];
-                    while (qa < en.q1 && P.panel_blk[rb + qa] < bp) qa++;
-                    while (qb < en.q1 && P.panel_blk[rb + qb] <= bp) qb++;
-                    c0 += (double)(qb - en.q0);
-                    c1 += (double)(qa - en.q0);
+                double f0, f1;
+                entry_flops(P.ents[e], f0, f1);
+                P.flops_kind[0][kind] += f0;
+                P.flops_kind[1][kind] += f1;
+                if (lv < P.leaf_levels) {
+                    P.flops_kind_leaf[0][kind] += f0;
+                    P.flops_kind_leaf[1][kind] += f1;
                 }
-                P.flops_kind[0][kind] += K2 * c0;
-                P.flops_kind[1][kind] += K2 * c1;
             }
         }
 
@@ -359,6 +403,7 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
                 w.wofs = G.wofs + pt * WT_P + qt * ts * (int64_t)G.ldw;
                 P.wtiles.push_back(w);
                 P.flops_wtile += 2.0 * w.mp * w.mq * (double)width(w.s);
+                if (lv < P.leaf_levels) P.flops_wtile_leaf += 2.0 * w.mp * w.mq * (double)width(w.s);
             }
             P.lv_wt_ptr[lv + 1] = (int32_t)P.wtiles.size();
         }
@@ -376,6 +421,8 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
         // backward gather items for the cblks of this level (partial slots
         // numbered over all levels, so that several levels may be in flight)
         {
+            bool lv_wide = false;
+            for (int32_t k = l0; k < l1; k++) lv_wide = lv_wide || width(P.lv_cblks[k]) > 64;
             const int32_t seg = l0 + lv;
             for (int32_t k = l0; k < l1; k++) {
                 const int64_t c = P.lv_cblks[k];
@@ -383,8 +430,9 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
                 P.bwd_itemptr[seg + (k - l0)] = b0;
                 if (!factored(c)) continue;  // the Schur part has no solve items
                 const int64_t K = width(c), st = L.cblk_stride[c];
-                for (int64_t r0 = K; r0 < st; r0 += BWD_ROWS) {
-                    int64_t r1 = std::min<int64_t>(st, r0 + BWD_ROWS);
+                const int64_t brows = (lv_wide ? BWD_ROWS_WIDE : BWD_ROWS);
+                for (int64_t r0 = K; r0 < st; r0 += brows) {
+                    int64_t r1 = std::min<int64_t>(st, r0 + brows);
                     P.bwd.push_back(BwdItem{(int32_t)c, (int32_t)r0, (int32_t)r1, (int32_t)bslot});
                     bslot += K;
                 }
@@ -396,6 +444,92 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
             P.bwd_part = std::max(P.bwd_part, bslot);
             P.lv_bwd_ptr[lv + 1] = (int32_t)P.bwd.size();
         }
+    }
+    // leaf region task lists
+    if (P.leaf_levels > 0) {
+        const int32_t nreg = P.lv_cblk_ptr[P.leaf_levels];
+        std::vector<char> in_region(cb, 0);
+        std::vector<int32_t> srcs(P.lv_cblks.begin(), P.lv_cblks.begin() + nreg);
+        std::sort(srcs.begin(), srcs.end());
+        for (const int32_t c : srcs) in_region[c] = 1;
+        P.leaf_n = nreg;
+        P.leaf_ctr0.assign(cb, 0);
+        P.leaf_tgt_ptr.assign(cb + 1, 0);
+        for (int64_t c = 0; c < cb; c++) {
+            if (in_region[c]) {
+                int64_t prev = -1;
+                for (int64_t b = s.cblk_bloknum[c] + 1; b < s.cblk_bloknum[c + 1]; b++) {
+                    const int64_t f = s.blok_fcblk[b];
+                    if (f == prev) continue;  // blocks facing one cblk are consecutive
+                    prev = f;
+                    if (!in_region[f]) continue;
+                    P.leaf_tgt.push_back((int32_t)f);
+                    P.leaf_ctr0[f]++;
+                }
+            }
+            P.leaf_tgt_ptr[c + 1] = (int32_t)P.leaf_tgt.size();
+        }
+        for (const int32_t c : srcs)
+            if (P.leaf_ctr0[c] == 0) P.leaf_init.push_back(c);
+        std::vector<UpdItem> in;
+        build_items(srcs, [&](int64_t f) { return in_region[f] != 0; }, in);
+        // grouped by target (the tile key orders by target first)
+        P.leaf_in_ptr.assign(cb + 1, 0);
+        for (const UpdItem &it : in) P.leaf_in_ptr[it.f + 1]++;
+        for (int64_t c = 0; c < cb; c++) P.leaf_in_ptr[c + 1] += P.leaf_in_ptr[c];
+        P.leaf_in = in;
+        for (const UpdItem &it : in)
+            for (int32_t e = it.e0; e < it.e1; e++) {
+                double f0, f1;
+                entry_flops(P.ents[e], f0, f1);
+                P.flops_leaf_upd += f0;
+            }
+        // replay the task protocol on the host: every region cblk must become
+        // ready exactly once, after every source its tiles read is done; a
+        // list that fails leaves the region to the level schedule
+        {
+            std::vector<int32_t> ctr(P.leaf_ctr0), order(P.leaf_init);
+            std::vector<char> done(cb, 0);
+            bool ok = true;
+            for (size_t i = 0; i < order.size() && ok; i++) {
+                const int32_t c = order[i];
+                for (int32_t k = P.leaf_in_ptr[c]; k < P.leaf_in_ptr[c + 1] && ok; k++)
+                    for (int32_t e = P.leaf_in[k].e0; e < P.leaf_in[k].e1; e++)
+                        if (!done[P.ents[e].s] || P.leaf_in[k].f != c) ok = false;
+                done[c] = 1;
+                for (int32_t k = P.leaf_tgt_ptr[c]; k < P.leaf_tgt_ptr[c + 1]; k++) {
+                    const int32_t f = P.leaf_tgt[k];
+                    if (--ctr[f] == 0) order.push_back(f);
+                    if (ctr[f] < 0) ok = false;
+                }
+            }
+            if (!ok || (int32_t)order.size() != nreg) {
+                fprintf(stderr, "pastix_amd: leaf-region task lists failed their replay; using the level schedule\n");
+                P.leaf_levels = 0;
+                P.leaf_n = 0;
+                P.leaf_ctr0.clear();
+                P.leaf_init.clear();
+                P.leaf_in_ptr.clear();
+                P.leaf_in.clear();
+                P.leaf_tgt_ptr.clear();
+                P.leaf_tgt.clear();
+                P.flops_leaf_upd = 0;
+                for (int a = 0; a < 2; a++)
+                    for (int k = 0; k < 3; k++) P.flops_kind_leaf[a][k] = 0;
+                P.flops_wtile_leaf = 0;
+                return P;
+            }
+        }
+        build_items(srcs, [&](int64_t f) { return in_region[f] == 0; }, P.leaf_out);
+        P.leaf_out_split = (int32_t)(std::stable_partition(P.leaf_out.begin(), P.leaf_out.end(),
+                                                           [](const UpdItem &it) { return it.e1 - it.e0 == 1; }) -
+                                     P.leaf_out.begin());
+        for (size_t t = 0; t < P.leaf_out.size(); t++)
+            for (int32_t e = P.leaf_out[t].e0; e < P.leaf_out[t].e1; e++) {
+                double f0, f1;
+                entry_flops(P.ents[e], f0, f1);
+                P.flops_leaf_out[(int32_t)t < P.leaf_out_split ? 0 : 1] += f0;
+            }
     }
     return P;
 }

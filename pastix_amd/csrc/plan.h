@@ -15,6 +15,8 @@ constexpr double SINGLE_PASS_ENTS = 2.05;
 // levels with fewer 64-row panel tiles than this update their panels
 // right-looking (64-deep products over all later blocks)
 constexpr int64_t PANEL_RIGHT_ROWTILES = 200;
+// widest cblk of the leaf region (one diagonal sub-block per cblk)
+constexpr int64_t LEAF_MAX_WIDTH = 64;
 
 struct Plan {
     int32_t nlevels = 0;
@@ -85,6 +87,35 @@ struct Plan {
     std::vector<int32_t> panel_grow, panel_blk;
     std::vector<int64_t> rowbase, winv_off;
     int64_t winvnbr = 0;
+    // Leaf region: the cblks of levels [0, leaf_levels), all factored and at
+    // most LEAF_MAX_WIDTH wide, factored by one persistent launch in which a
+    // cblk is a task that becomes ready when its dependency counter (the
+    // number of region cblks updating it, reference ctrbcnt) reaches zero:
+    // it pulls its incoming updates, factors its diagonal block and solves
+    // its rows below, then decrements the counters of the region cblks it
+    // faces.  leaf_levels = 0: no region.
+    int32_t leaf_levels = 0;
+    int32_t leaf_n = 0;                  // region cblks
+    std::vector<int32_t> leaf_ctr0;      // cblknbr: initial counter (region cblks)
+    std::vector<int32_t> leaf_init;      // region cblks with no incoming update, increasing
+    // incoming update tiles of region cblk f: leaf_in[leaf_in_ptr[f] ..
+    // leaf_in_ptr[f+1]), entries of every region source in source order
+    std::vector<int32_t> leaf_in_ptr;    // cblknbr + 1
+    std::vector<UpdItem> leaf_in;
+    // region cblks each region cblk faces (its counters to decrement)
+    std::vector<int32_t> leaf_tgt_ptr;   // cblknbr + 1
+    std::vector<int32_t> leaf_tgt;
+    // updates of the region's cblks into the cblks above it, applied after
+    // the persistent launch: [0, leaf_out_split) single-entry tiles, the
+    // rest several entries
+    std::vector<UpdItem> leaf_out;
+    int32_t leaf_out_split = 0;
+    // reference flops of the region: its update pairs (LL^t) and the share
+    // of flops_kind that levels [0, leaf_levels) hold
+    double flops_leaf_upd = 0;
+    double flops_kind_leaf[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double flops_leaf_out[2] = {0, 0};  // leaf_out single-entry and multi-entry tiles (LL^t)
+    double flops_wtile_leaf = 0;        // flops_wtile of levels [0, leaf_levels)
     // algorithmic flops per kernel family (reference flop conventions):
     // diag = FLOPS_DPOTRF/GETRF(K), trsm = FLOPS_DTRSM(Right, M, K) per cblk,
     // update = sum over (cblk, blok) of FLOPS_DGEMM(m, n, k) with the m, n, k

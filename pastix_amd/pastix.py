@@ -95,6 +95,7 @@ class error:
 
 class amd_iparm:
     factor_precision, device, level_order, resident_input, profile, graph, overlap = 0, 1, 2, 3, 4, 5, 6
+    leaf_tasks, leaf_grid = 7, 8
 
 
 class PastixError(RuntimeError):
@@ -435,12 +436,12 @@ def fact_times(pastix_data):
     return out
 
 
-KERNEL_CLASSES = ("panel", "update_direct", "update_wgemm", "update_gather", "update_acc")
+KERNEL_CLASSES = ("panel", "update_direct", "update_wgemm", "update_gather", "update_acc", "leaf_tasks")
 
 
 def kernel_times(pastix_data):
     """{class: (seconds, timed spans)} of the last numfact (profiling on)."""
-    out = np.zeros(10)
+    out = np.zeros(12)
     _check("pastix_amd_kernel_times", lib.pastix_amd_kernel_times(pastix_data.ptr, out))
     return {k: (float(out[2 * i]), int(out[2 * i + 1])) for i, k in enumerate(KERNEL_CLASSES)}
 
@@ -449,11 +450,11 @@ def update_flops(pastix_data):
     """Schur-update flops per kernel: L-side single/multi-source tiles, U-side
     single/multi-source tiles, multi-source product tiles as computed, and
     the multi-source tiles of single-pass levels (L, U side)."""
-    out = np.zeros(7)
+    out = np.zeros(8)
     _check("pastix_amd_update_flops", lib.pastix_amd_update_flops(pastix_data.ptr, out))
     return {"direct": float(out[0]), "multi": float(out[1]), "direct_u": float(out[2]),
             "multi_u": float(out[3]), "wtile": float(out[4]), "single": float(out[5]),
-            "single_u": float(out[6])}
+            "single_u": float(out[6]), "leaf": float(out[7])}
 
 
 def _ptr_array(objs):

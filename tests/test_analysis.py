@@ -160,12 +160,24 @@ def test_update_flops_partition_the_reference_count(grid):
     # the per-kernel split of the Schur-update flops (single-source tiles,
     # two-phase multi-source tiles, single-pass multi-source tiles) covers
     # every (p, q) pair of the reference's per-block GEMMs exactly once
+    # (and, with the leaf region's persistent launch on, its in-launch
+    # updates and the level kernels' share together)
     A = spm.laplacian7(*grid, diag=6.0)
     h, S, sym = analyse(A, True, LLT, grid=grid, min_bs=20, max_bs=40)
-    uf = pastix.update_flops(h)
     fl = pastix.flops(h)
-    total = uf["direct"] + uf["multi"] + uf["single"]
+    pastix.set_amd_iparm(h, pastix.amd_iparm.leaf_tasks, 0)
+    off = pastix.update_flops(h)
+    assert off["leaf"] == 0
+    assert off["direct"] + off["multi"] + off["single"] == pytest.approx(fl[3], rel=1e-12)
+    # the accounting of the (off by default) leaf-region launch
+    pastix.set_amd_iparm(h, pastix.amd_iparm.leaf_tasks, 1)
+    uf = pastix.update_flops(h)
+    assert uf["leaf"] > 0
+    total = uf["direct"] + uf["multi"] + uf["single"] + uf["leaf"]
     assert total == pytest.approx(fl[3], rel=1e-12)
+    # the level kernels' split (leaf launch off) for the U side of LU
+    uf = off
+    total = uf["direct"] + uf["multi"] + uf["single"]
     # the U side of LU drops the pairs of a source's diagonal-facing block
     assert 0 < uf["direct_u"] + uf["multi_u"] + uf["single_u"] < total
     # product tiles cover at least the two-phase pairs they serve (one side)
