@@ -38,9 +38,16 @@ def berr_inf(A, x, b):
 
 
 OVERLAP = 1  # second stream for the updates beside the next panels (--overlap)
+# level-schedule choices written by pastix_amd/autotune.py (--no-tuning: the plan's rules)
+TUNING = os.path.join(ROOT, "pastix_amd", "tuning", "table.txt")
+TUNED = {}  # instance label -> whether a table entry applied
 
 
-def setup(pastix, spm, A, grid, prec=None, resident=True):
+def apply_tuning(pastix, h, label):
+    TUNED[label] = bool(TUNING and os.path.exists(TUNING) and pastix.load_tuning(h, TUNING))
+
+
+def setup(pastix, spm, A, grid, prec=None, resident=True, label=None):
     S = spm.Spm(A, symmetric=True)
     ip, dp = pastix.initParam()
     ip[pastix.iparm.factorization] = pastix.factotype.LLT
@@ -58,6 +65,8 @@ def setup(pastix, spm, A, grid, prec=None, resident=True):
     pastix.subtask_symbfact(h)
     pastix.subtask_reordering(h)
     pastix.subtask_blend(h)
+    if label:
+        apply_tuning(pastix, h, label)
     return h, S
 
 
@@ -164,6 +173,7 @@ def leg_getrf(pastix, spm, grid, seed=5):
         pastix.subtask_symbfact(h)
         pastix.subtask_reordering(h)
         pastix.subtask_blend(h)
+        apply_tuning(pastix, h, "getrf_" + label)
         pastix.task_numfact(h, S)
         times = []
         for _ in range(3):
@@ -394,6 +404,7 @@ def main():
                     help="host threads of the CPU baseline (default: min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-tuning", action="store_true", help="the plan's rules, not the tuning table")
     ap.add_argument("--profile-only", action="store_true", help="only the timed factorisations (for rocprofv3)")
     ap.add_argument("--legs", default="getrf,batch,serving",
                     help="extra workloads reported beside the headline: getrf, batch, serving (comma list or none)")
@@ -406,8 +417,10 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only three device-to-device copies of 256 MiB (reference dispatch for HBM counters)")
     args = ap.parse_args()
-    global OVERLAP
+    global OVERLAP, TUNING
     OVERLAP = args.overlap
+    if args.no_tuning:
+        TUNING = None
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU)", file=sys.stderr)
         sys.exit(2)
@@ -428,7 +441,7 @@ def main():
     log(f"building the {grid} 7-point Laplacian")
     A = spm.laplacian7(*grid, diag=6.0)
     t0 = time.perf_counter()
-    h, S = setup(pastix, spm, A, grid)
+    h, S = setup(pastix, spm, A, grid, label="fp64")
     t_an = time.perf_counter() - t0
     sym = pastix.symbol(h)
     fl = pastix.flops(h)
@@ -521,7 +534,7 @@ def main():
 
     fp32 = None
     if not args.no_fp32:
-        h32, S32 = setup(pastix, spm, A, grid, prec=spm.SpmFloat)
+        h32, S32 = setup(pastix, spm, A, grid, prec=spm.SpmFloat, label="fp32")
         pastix.task_numfact(h32, S32)
         t32 = []
         for _ in range(max(2, args.steps // 2)):
@@ -611,6 +624,7 @@ def main():
         "numfact_split_ms": {"panel": ft[1] * 1e3, "update": ft[3] * 1e3,
                              "note": "one-stream profiled run (overlap off)"},
         "overlap": bool(OVERLAP),
+        "level_schedule_tuning": {k: ("table entry" if v else "plan rules") for k, v in TUNED.items()},
         "kernels": kern,
         "cpu_baseline": cpu,
     }

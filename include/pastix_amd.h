@@ -172,6 +172,26 @@ int pastix_amd_kernel_times(const pastix_data_t *pastix_data, double out[12]);
  * updates into the supernodes above it join out[0] and out[5]. */
 int pastix_amd_update_flops(const pastix_data_t *pastix_data, double out[8]);
 
+/* ---- Level-schedule tuning (no reference counterpart) --------------------
+ * The level schedule makes two choices per dependency level: whether its
+ * multi-source update tiles are summed in one pass (1) or through the
+ * product workspace (0), and whether its panels are updated right-looking
+ * (1) or left-looking (0).  The plan's rules pick them unless per-level
+ * values are given (-1 keeps the rule).  pastix_amd/autotune.py times both
+ * choices per level on the device and writes a tuning table; a table is read
+ * by pastix_amd_load_tuning, or at analysis from the file PASTIX_AMD_TUNING
+ * names, for an entry matching the analysed partition and factorisation. */
+int pastix_amd_set_level_knobs(pastix_data_t *pastix_data, int nlevels, const int8_t *single_pass,
+                               const int8_t *panel_right);
+int pastix_amd_load_tuning(pastix_data_t *pastix_data, const char *path, int *applied);
+int pastix_amd_save_tuning(const pastix_data_t *pastix_data, const char *path);
+/* Per level: 4 ints (cblks, multi-source tiles, single-pass, right-looking);
+ * returns the number of levels written (-1 before the analysis). */
+int pastix_amd_level_info(const pastix_data_t *pastix_data, int32_t *out, int maxlevels);
+/* Per level of the last numfact run with PXA_IPARM_PROFILE: 2 doubles (panel
+ * steps, Schur updates, seconds); returns the number of levels written. */
+int pastix_amd_level_times(const pastix_data_t *pastix_data, double *out, int maxlevels);
+
 /* Algorithmic flops of the analysed factorisation: out[0]=DPARM_FACT_THFLOPS,
  * [1]=diagonal factorisations, [2]=panel TRSMs, [3]=Schur updates counted
  * as the reference's per-block GEMMs (kernels/core_zgemmsp.c:1352-1395). */

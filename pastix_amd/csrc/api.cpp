@@ -20,6 +20,59 @@ double *device_pair(pastix_data_t *d) {
 }
 
 void err_print(const char *fn, const char *msg) { fprintf(stderr, "%s: %s\n", fn, msg); }
+
+// Tuning tables: text files of entries
+//   entry <key>
+//   single_pass <one value per level: -1, 0 or 1>
+//   panel_right <one value per level>
+//   end
+// keyed by the analysed partition and the factorisation it is tuned for.
+std::string tuning_key(const pastix_data_t *d) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "n=%lld cblknbr=%lld bloknbr=%lld nlevels=%d facto=%lld prec=%lld",
+             (long long)d->n, (long long)d->symbol.cblknbr(), (long long)d->symbol.bloknbr(), (int)d->layout.nlevels,
+             (long long)d->iparm[IPARM_FACTORIZATION], (long long)d->ext[PXA_IPARM_FACTOR_PRECISION]);
+    return buf;
+}
+
+// 1: an entry for this analysis was read into k, 0: none, -1: unreadable file
+int read_tuning(const pastix_data_t *d, const char *path, pxa::PlanKnobs &k) {
+    FILE *f = fopen(path, "r");
+    if (!f) return -1;
+    const std::string key = "entry " + tuning_key(d);
+    const int nl = d->layout.nlevels;
+    char line[1 << 16];
+    bool in = false, found = false;
+    pxa::PlanKnobs r;
+    while (fgets(line, sizeof(line), f)) {
+        std::string l(line);
+        while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+        if (!in) {
+            if (l == key) in = true;
+            continue;
+        }
+        if (l == "end") {
+            found = (int)r.single_pass.size() == nl && (int)r.panel_right.size() == nl;
+            break;
+        }
+        std::vector<int8_t> *dst = nullptr;
+        size_t pos = 0;
+        if (l.rfind("single_pass", 0) == 0) dst = &r.single_pass, pos = 11;
+        else if (l.rfind("panel_right", 0) == 0) dst = &r.panel_right, pos = 11;
+        if (!dst) continue;
+        const char *p = l.c_str() + pos;
+        char *e = nullptr;
+        for (;;) {
+            const long v = strtol(p, &e, 10);
+            if (e == p) break;
+            dst->push_back((int8_t)std::max(-1L, std::min(1L, v)));
+            p = e;
+        }
+    }
+    fclose(f);
+    if (found) k = r;
+    return found ? 1 : 0;
+}
 }  // namespace pxa_api
 using namespace pxa_api;
 
@@ -483,7 +536,14 @@ extern "C" int pastix_subtask_blend(pastix_data_t *d) {
     // cblks from the Schur column on (reference CBLK_IN_SCHUR, blend/cand.c:725)
     int64_t sc = d->symbol.cblknbr();
     while (sc > 0 && d->symbol.cblk_fcol[sc - 1] >= d->schurfcol) sc--;
-    d->plan = pxa::build_plan(d->symbol, d->layout, sc);
+    d->schur_cblk = sc;
+    // a tuning table named by PASTIX_AMD_TUNING supplies per-level choices
+    // for a matching analysis
+    if (const char *tf = getenv("PASTIX_AMD_TUNING")) {
+        pxa::PlanKnobs k;
+        if (read_tuning(d, tf, k) == 1) d->knobs = k;
+    }
+    d->plan = pxa::build_plan(d->symbol, d->layout, sc, d->knobs);
     int facto = (int)d->iparm[IPARM_FACTORIZATION];
     int ft = (facto == PastixFactGETRF) ? 2 : is_ldlt(d) ? 1 : 0;
     d->dparm[DPARM_FACT_THFLOPS] = pxa::symbol_thflops(d->symbol, ft);
@@ -1451,6 +1511,88 @@ extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[12]) {
         out[2 * c + 1] = d->engine->kn[c];
     }
     return PASTIX_SUCCESS;
+}
+
+// Rebuild the level schedule with new per-level choices (no reference
+// counterpart: the tuner's and the tuning table's hook).  The next numfact
+// sets the engine up again.
+static int replan(pastix_data_t *d) {
+    d->plan = pxa::build_plan(d->symbol, d->layout, d->schur_cblk, d->knobs);
+    d->engine.reset();
+    d->steps &= ~(STEP_BCSC2CTAB | STEP_NUMFACT | STEP_SOLVE | STEP_REFINE);
+    return PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_set_level_knobs(pastix_data_t *d, int nlevels, const int8_t *single_pass,
+                                          const int8_t *panel_right) {
+    if (!d || nlevels < 0 || (nlevels > 0 && (!single_pass || !panel_right))) return PASTIX_ERR_BADPARAMETER;
+    d->knobs.single_pass.assign(single_pass, single_pass + nlevels);
+    d->knobs.panel_right.assign(panel_right, panel_right + nlevels);
+    return (d->steps & STEP_ANALYSE) ? replan(d) : PASTIX_SUCCESS;
+}
+extern "C" int pastix_amd_load_tuning(pastix_data_t *d, const char *path, int *applied) {
+    if (!d || !path || !applied) return PASTIX_ERR_BADPARAMETER;
+    *applied = 0;
+    if (!(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
+    pxa::PlanKnobs k;
+    const int rc = read_tuning(d, path, k);
+    if (rc < 0) return PASTIX_ERR_FILE;
+    if (rc == 0) return PASTIX_SUCCESS;
+    d->knobs = k;
+    *applied = 1;
+    return replan(d);
+}
+extern "C" int pastix_amd_save_tuning(const pastix_data_t *d, const char *path) {
+    if (!d || !path || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
+    const std::string key = "entry " + tuning_key(d);
+    std::vector<std::string> keep;
+    if (FILE *f = fopen(path, "r")) {
+        char line[1 << 16];
+        bool skip = false;
+        while (fgets(line, sizeof(line), f)) {
+            std::string l(line);
+            while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+            if (l == key) skip = true;
+            if (!skip) keep.push_back(l);
+            if (skip && l == "end") skip = false;
+        }
+        fclose(f);
+    }
+    FILE *f = fopen(path, "w");
+    if (!f) return PASTIX_ERR_FILE;
+    for (const std::string &l : keep) fprintf(f, "%s\n", l.c_str());
+    const pxa::Plan &P = d->plan;
+    fprintf(f, "%s\nsingle_pass", key.c_str());
+    for (int lv = 0; lv < P.nlevels; lv++) fprintf(f, " %d", (int)P.lv_single_pass[lv]);
+    fprintf(f, "\npanel_right");
+    for (int lv = 0; lv < P.nlevels; lv++) fprintf(f, " %d", (int)P.lv_panel_right[lv]);
+    fprintf(f, "\nend\n");
+    return fclose(f) == 0 ? PASTIX_SUCCESS : PASTIX_ERR_FILE;
+}
+// Per level: cblks, multi-source update tiles, single-pass (1) or product
+// workspace (0), right-looking (1) or left-looking (0) panels.
+extern "C" int pastix_amd_level_info(const pastix_data_t *d, int32_t *out, int maxlevels) {
+    if (!d || !out || !(d->steps & STEP_ANALYSE)) return -1;
+    const pxa::Plan &P = d->plan;
+    const int nl = std::min<int>(maxlevels, P.nlevels);
+    for (int lv = 0; lv < nl; lv++) {
+        out[4 * lv] = P.lv_cblk_ptr[lv + 1] - P.lv_cblk_ptr[lv];
+        out[4 * lv + 1] = P.lv_upd_ptr[lv + 1] - P.lv_upd_split[lv];
+        out[4 * lv + 2] = P.lv_single_pass[lv];
+        out[4 * lv + 3] = P.lv_panel_right[lv];
+    }
+    return nl;
+}
+// Per level of the last numfact run with PXA_IPARM_PROFILE: panel steps and
+// Schur updates (seconds).
+extern "C" int pastix_amd_level_times(const pastix_data_t *d, double *out, int maxlevels) {
+    if (!d || !out || !d->engine) return -1;
+    const pxa::Engine &E = *d->engine;
+    const int nl = std::min<int>(maxlevels, (int)E.lv_panel_s.size());
+    for (int lv = 0; lv < nl; lv++) {
+        out[2 * lv] = E.lv_panel_s[lv];
+        out[2 * lv + 1] = E.lv_upd_s[lv];
+    }
+    return nl;
 }
 
 extern "C" int pastix_amd_update_flops(const pastix_data_t *d, double out[8]) {

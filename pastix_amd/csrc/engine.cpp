@@ -309,7 +309,7 @@ int Engine::prof_begin(hipStream_t st) {
 void Engine::prof_end(int cls, int k, hipStream_t st) {
     if (k < 0) return;
     if (hipEventRecord(pev_[nev_], st) != hipSuccess) return;
-    spans_.push_back(Span{cls, k, nev_});
+    spans_.push_back(Span{cls, k, nev_, cur_lv_});
     nev_++;
 }
 
@@ -326,6 +326,7 @@ template <typename T> int Engine::factorize_t(double thr) {
     // receives its contributions level by level, as with one stream.
     bool side_open = false;
     size_t lv_first = 0;
+    cur_lv_ = 0;
     if (leaf_active()) {
         // the leaf region in one persistent launch, then its updates into the
         // cblks above it; the level schedule takes over from its last level
@@ -349,6 +350,7 @@ template <typename T> int Engine::factorize_t(double thr) {
         lv_first = (size_t)leaf_levels_;
     }
     for (size_t lv = lv_first; lv < nl; lv++) {
+        cur_lv_ = (int)lv;
         int pk = prof_begin(stream_);
         if (facto_ == 2) {
             // the level's factored cblks (the Schur ones keep their sums)
@@ -642,11 +644,18 @@ int Engine::factorize(double thr, int64_t *npiv, int *negdiag, double *seconds) 
             kt[c] = 0;
             kn[c] = 0;
         }
+        lv_panel_s.assign(lv_cblk_ptr_.size() - 1, 0.0);
+        lv_upd_s.assign(lv_cblk_ptr_.size() - 1, 0.0);
         for (const Span &sp : spans_) {
             float a = 0;
             HIPCHK(hipEventElapsedTime(&a, pev_[sp.e0], pev_[sp.e1]));
             kt[sp.cls] += a * 1e-3;
             kn[sp.cls]++;
+            const int lv = std::max(sp.lv, 0);
+            if (lv < (int)lv_panel_s.size()) {
+                if (sp.cls == KC_PANEL || sp.cls == KC_LEAF) lv_panel_s[lv] += a * 1e-3;
+                else lv_upd_s[lv] += a * 1e-3;
+            }
         }
         t_diag = kt[KC_PANEL] + kt[KC_LEAF];
         n_diag = kn[KC_PANEL] + kn[KC_LEAF];

@@ -168,6 +168,9 @@ class Engine {
     bool leaf_active() const { return use_leaf && facto_ != 2 && leaf_.nregion > 0; }
     double kt[KC_N] = {0, 0, 0, 0, 0, 0};
     int kn[KC_N] = {0, 0, 0, 0, 0, 0};
+    // per dependency level of the last profiled factorisation: panel steps
+    // and Schur updates (seconds; the leaf launch counts as level 0's panel)
+    std::vector<double> lv_panel_s, lv_upd_s;
     double t_diag = 0, t_upd = 0;
     int n_diag = 0, n_upd = 0;
 
@@ -260,8 +263,9 @@ class Engine {
     int enqueue_solve(void *dx, int64_t ldx, int nrhs, bool user);
     hipGraphExec_t solve_graph(void *dx, int64_t ldx, int nrhs, bool user);
     struct Span {
-        int cls, e0, e1;
+        int cls, e0, e1, lv;
     };
+    int cur_lv_ = -1;  // level whose launches are being issued (per-level profiles)
     std::vector<Span> spans_;
     int nev_ = 0;
     int prof_begin(hipStream_t st);

@@ -118,17 +118,30 @@ __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem
     const int fc = S.cblk_fcol[c];
     const int Kc = S.cblk_fcol[c + 1] - fc;
     const int nrow = min(Kc, r0 + 64);
-    // y = b minus the pending products of rows [0, nrow).  A row's entries
-    // are contiguous in the update buffer, and so are the rows of one cblk:
-    // gl consecutive lanes per row (the power of two at most the cblk's
-    // first row's entry count, up to a whole wave), lane j summing entries
-    // j, j + gl, ... (coalesced across the lanes of a row), the lanes' sums
-    // combined by xor shuffles in a fixed order
+    // y = b minus the pending products of rows [0, nrow) (a row's entries
+    // are contiguous in the update buffer): gl consecutive lanes per row, as
+    // many as fill the workgroup in one pass (at most 64, at most the cblk's
+    // first row's entry count), lane j summing the entries j, j + gl, ... of
+    // its row, UL loads issued together, in order; the lanes' sums combined
+    // by xor shuffles in a fixed order
     const int tasks = nrow * nrhs;
     const int ne = (int)(yrow_ptr[fc + 1] - yrow_ptr[fc]);
     int gl = 1;
-    while (gl < 64 && gl * 2 <= ne) gl *= 2;
-#pragma unroll 4
+    while (gl < 64 && gl * 2 <= ne && tasks * gl * 2 <= NW * 64) gl *= 2;
+    // rows of the inverse: E consecutive per lane (16-byte loads); the first
+    // PF column groups are fetched before the gather, so that their latency
+    // overlaps it
+    constexpr int E = Vec<T>::E, G = 64 / E, PF = 4;
+    const int rr = E * (r % G), kc = r / G;
+    const bool mrow = r0 + rr < Kc;
+    const T *M = w1 + S.cblk_winv[c] + r0 + rr;
+    T mpre[PF][E];
+#pragma unroll
+    for (int f = 0; f < PF; f++) {
+        const int k = q * E + kc + f * NW * E;
+        if (mrow && k < nrow) ld_rows(M + (int64_t)k * Kc, rr, Kc - r0, mpre[f]);
+    }
+    constexpr int UL = 16;
     for (int base = 0; base < tasks * gl; base += NW * 64) {
         const int t = base + tid, task = t / gl, j = t % gl;
         const int i = task % nrow, u = task / nrow;
@@ -136,26 +149,41 @@ __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem
         if (task < tasks) {
             const T *yb = ybuf + u * ldy;
             const int64_t pe = yrow_ptr[fc + i + 1];
-#pragma unroll 8
-            for (int64_t p = yrow_ptr[fc + i] + j; p < pe; p += gl) v += ld_co<CO>(yb + p);
+            for (int64_t p = yrow_ptr[fc + i] + j; p < pe; p += (int64_t)UL * gl) {
+                T a[UL];
+#pragma unroll
+                for (int k = 0; k < UL; k++) a[k] = (p + (int64_t)k * gl < pe) ? ld_co<CO>(yb + p + (int64_t)k * gl) : (T)0;
+#pragma unroll
+                for (int k = 0; k < UL; k++) v += a[k];
+            }
         }
         for (int o = gl >> 1; o >= 1; o >>= 1) v += __shfl_xor(v, o);
         if (task < tasks && j == 0) y[i + u * Kc] = b[fc + i + u * ldb] - v;
     }
     __syncthreads();
     const int i = r0 + r;
-    // rows of the inverse: E consecutive per lane (16-byte loads)
-    constexpr int E = Vec<T>::E, G = 64 / E;
-    const int rr = E * (r % G), kc = r / G;
     T acc[E][NR];
 #pragma unroll
     for (int e = 0; e < E; e++)
 #pragma unroll
         for (int u = 0; u < NR; u++) acc[e][u] = 0;
-    if (r0 + rr < Kc) {
-        const T *M = w1 + S.cblk_winv[c] + r0 + rr;
+    if (mrow) {
+#pragma unroll
+        for (int f = 0; f < PF; f++) {
+            const int k = q * E + kc + f * NW * E;
+            if (k < nrow) {
+#pragma unroll
+                for (int u = 0; u < NR; u++) {
+                    if (u < nrhs) {
+                        const T yv = y[k + u * Kc];
+#pragma unroll
+                        for (int e = 0; e < E; e++) acc[e][u] += mpre[f][e] * yv;
+                    }
+                }
+            }
+        }
 #pragma unroll 4
-        for (int k = q * E + kc; k < nrow; k += NW * E) {
+        for (int k = q * E + kc + PF * NW * E; k < nrow; k += NW * E) {
             T m[E];
             ld_rows(M + (int64_t)k * Kc, rr, Kc - r0, m);
 #pragma unroll

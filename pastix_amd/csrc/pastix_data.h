@@ -64,6 +64,10 @@ struct pastix_data_s {
     std::vector<spm_int_t> u_colptr, u_rows;
     std::vector<double> u_vals;
     spmatrix_t u_spm{};
+    // per-level schedule choices from a tuning table or the tuner
+    // (pastix_amd_set_level_knobs / pastix_amd_load_tuning)
+    pxa::PlanKnobs knobs;
+    int64_t schur_cblk = 0;  // first Schur cblk of the analysed partition
     double *batch_dev = nullptr;  // 2 x n x R FP64 blocks of the batched refinement
     int64_t batch_dev_cap = 0;
     pxa::Engine *batch_dev_owner = nullptr;

@@ -26,7 +26,7 @@ struct KeyEnt {
 };
 }  // namespace
 
-Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
+Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk, const PlanKnobs &knobs) {
     const int64_t cb = s.cblknbr();
     Plan P;
     const int64_t sc = std::min<int64_t>(std::max<int64_t>(schur_cblk, 0), cb);
@@ -259,7 +259,7 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
         // of the tree, where few row tiles would leave the device idle
         int64_t rowtiles = 0;
         for (int32_t k = l0; k < lf; k++) rowtiles += (L.cblk_stride[P.lv_cblks[k]] + 63) / 64;
-        const bool right = rowtiles < PANEL_RIGHT_ROWTILES;
+        const bool right = knobs.pr(lv) >= 0 ? knobs.pr(lv) != 0 : rowtiles < PANEL_RIGHT_ROWTILES;
         P.lv_panel_right[lv] = right;
         for (int64_t kb = 0; kb < nsteps; kb++) {
             const int64_t k0 = kb * 64;
@@ -315,7 +315,8 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk) {
                 nm++;
                 nme += P.upd[t].e1 - P.upd[t].e0;
             }
-            P.lv_single_pass[lv] = nm > 0 && (double)nme <= SINGLE_PASS_ENTS * (double)nm;
+            P.lv_single_pass[lv] = nm > 0 && (knobs.sp(lv) >= 0 ? knobs.sp(lv) != 0
+                                                                : (double)nme <= SINGLE_PASS_ENTS * (double)nm);
         }
         const bool single = P.lv_single_pass[lv] != 0;
         // exact flops of the kept (p, q) pairs of every entry

@@ -131,7 +131,17 @@ struct Plan {
     double flops_wtile = 0;
 };
 
+// Per-level schedule choices that override the plan's rules (a tuning table
+// supplies them): -1 keeps the rule; single_pass 1 sums the level's
+// multi-source tiles in one pass, 0 through the product workspace;
+// panel_right 1 updates the level's panels right-looking, 0 left-looking.
+struct PlanKnobs {
+    std::vector<int8_t> single_pass, panel_right;
+    int8_t sp(int64_t lv) const { return lv < (int64_t)single_pass.size() ? single_pass[lv] : -1; }
+    int8_t pr(int64_t lv) const { return lv < (int64_t)panel_right.size() ? panel_right[lv] : -1; }
+};
+
 // schur_cblk: first cblk of the Schur complement (cblknbr when there is none)
-Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk);
+Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk, const PlanKnobs &knobs = PlanKnobs());
 
 }  // namespace pxa

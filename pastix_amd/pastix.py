@@ -401,6 +401,59 @@ def backward_error_device_noalloc(pastix_data, db, dx, nrhs, dberr):
            lib.pastix_amd_backward_error_device_noalloc(pastix_data.ptr, nrhs, db.ptr, n, dx.ptr, n, dberr.ptr))
 
 
+lib.pastix_amd_set_level_knobs.argtypes = [_P, ct.c_int, ct.c_void_p, ct.c_void_p]
+lib.pastix_amd_set_level_knobs.restype = ct.c_int
+lib.pastix_amd_load_tuning.argtypes = [_P, ct.c_char_p, ct.POINTER(ct.c_int)]
+lib.pastix_amd_load_tuning.restype = ct.c_int
+lib.pastix_amd_save_tuning.argtypes = [_P, ct.c_char_p]
+lib.pastix_amd_save_tuning.restype = ct.c_int
+lib.pastix_amd_level_info.argtypes = [_P, ct.c_void_p, ct.c_int]
+lib.pastix_amd_level_info.restype = ct.c_int
+lib.pastix_amd_level_times.argtypes = [_P, ct.c_void_p, ct.c_int]
+lib.pastix_amd_level_times.restype = ct.c_int
+
+
+def set_level_knobs(pastix_data, single_pass, panel_right):
+    """Per-level schedule choices (-1 keeps the plan's rule); rebuilds the
+    level schedule of an analysed instance."""
+    sp = np.ascontiguousarray(single_pass, dtype=np.int8)
+    pr = np.ascontiguousarray(panel_right, dtype=np.int8)
+    if sp.shape != pr.shape:
+        raise ValueError("set_level_knobs: one value per level in both arrays")
+    _check("pastix_amd_set_level_knobs",
+           lib.pastix_amd_set_level_knobs(pastix_data.ptr, len(sp), sp.ctypes.data, pr.ctypes.data))
+
+
+def load_tuning(pastix_data, path):
+    """Apply a tuning table's entry for this analysis; True when one matched."""
+    applied = ct.c_int(0)
+    _check("pastix_amd_load_tuning", lib.pastix_amd_load_tuning(pastix_data.ptr, str(path).encode(), ct.byref(applied)))
+    return bool(applied.value)
+
+
+def save_tuning(pastix_data, path):
+    """Write (or replace) this analysis's entry with its current choices."""
+    _check("pastix_amd_save_tuning", lib.pastix_amd_save_tuning(pastix_data.ptr, str(path).encode()))
+
+
+def level_info(pastix_data):
+    """Per level: (cblks, multi-source tiles, single-pass, right-looking)."""
+    out = np.zeros((4096, 4), dtype=np.int32)
+    nl = lib.pastix_amd_level_info(pastix_data.ptr, out.ctypes.data, 4096)
+    if nl < 0:
+        raise PastixError("pastix_amd_level_info", nl)
+    return out[:nl].copy()
+
+
+def level_times(pastix_data):
+    """Per level of the last profiled numfact: (panel s, update s)."""
+    out = np.zeros((4096, 2))
+    nl = lib.pastix_amd_level_times(pastix_data.ptr, out.ctypes.data, 4096)
+    if nl < 0:
+        raise PastixError("pastix_amd_level_times", nl)
+    return out[:nl].copy()
+
+
 lib.pastix_amd_fact_times.argtypes = [_P, _dp]
 lib.pastix_amd_fact_times.restype = ct.c_int
 lib.pastix_amd_flops.argtypes = [_P, _dp]
