@@ -87,6 +87,21 @@ def traffic_per_launch(kernel, grid):
     return None
 
 
+def solve_traffic(grid):
+    """HBM bytes of one 1-RHS device solve (all its kernels) from the
+    committed counter summary (line 'traffic_bytes_per_solve <bytes>'), or
+    None when there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", f"laplacian7_{grid[0]}cube_solve_fp64_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 2 and parts[0] == "traffic_bytes_per_solve":
+                return float(parts[1])
+    return None
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -406,6 +421,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-tuning", action="store_true", help="the plan's rules, not the tuning table")
     ap.add_argument("--profile-only", action="store_true", help="only the timed factorisations (for rocprofv3)")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="the timed factorisations, then --solves device solves (for rocprofv3)")
     ap.add_argument("--legs", default="getrf,batch,serving",
                     help="extra workloads reported beside the headline: getrf, batch, serving (comma list or none)")
     ap.add_argument("--getrf-grid", type=int, nargs=3, default=[64, 64, 64])
@@ -466,6 +483,19 @@ def main():
     gflops = thflops / tmed / 1e9
     if args.profile_only:
         print(json.dumps({"numfact_ms": tmed * 1e3, "gflops": gflops}))
+        return
+    if args.solve_only:
+        # only device solves of 1 right-hand side after the factorisations
+        # above (for rocprofv3 runs of the solve kernels)
+        x0, b = spm.rhs_from_x(A, 1, seed=2024)
+        db = pastix.DeviceArray(b)
+        st = []
+        for _ in range(args.solves):
+            db.upload(b)
+            pastix.solve_device(h, db, A.shape[0], 1)
+            st.append(float(h.dparm[pastix.dparm.solv_time]))
+        db.free()
+        print(json.dumps({"solves": args.solves, "solve_p50_ms": float(np.median(st)) * 1e3}))
         return
 
     # pastix_task_numfact on new values of the analysed pattern: permuted
@@ -528,7 +558,8 @@ def main():
     p50, p99 = float(np.percentile(stimes, 50)) * 1e3, float(np.percentile(stimes, 99)) * 1e3
     log(f"solve p50 {p50:.3f} ms p99 {p99:.3f} ms berr {berr64:.2e}")
     nnzL = float(h.iparm[pastix.iparm.nnzeros])
-    solve_gbs = (2 * (nnzL + A.shape[0]) * 8 + 2 * 2 * A.shape[0] * 8) / (p50 * 1e-3) / 1e9
+    solve_bytes = 2 * (nnzL + A.shape[0]) * 8 + 2 * 2 * A.shape[0] * 8
+    solve_gbs = solve_bytes / (p50 * 1e-3) / 1e9
     db.free()
     pastix.finalize(h)
 
@@ -610,6 +641,11 @@ def main():
         "gflops_pastix_2p30": thflops / tmed / 2 ** 30,
         "wall_ms_per_step": wall / args.steps * 1e3,
         "solve_p50_ms": p50, "solve_p99_ms": p99, "solve_gbs_p50": solve_gbs, "solve_berr": berr64,
+        "solve_roofline": {"bound": "hbm", "achieved": solve_gbs, "peak": 8000.0, "unit": "GB/s",
+                           "frac": solve_gbs / 8000.0, "algorithmic_bytes": solve_bytes,
+                           "traffic": solve_traffic(grid),
+                           "traffic_unit": "HBM bytes per 1-RHS solve, all solve kernels (rocprofv3 FETCH_SIZE "
+                                           "and WRITE_SIZE, copy-calibrated; profiles/)"},
         "fp32_ir": fp32,
         "refill": refill,
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": kern[dom]["tflops"],
