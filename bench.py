@@ -38,6 +38,7 @@ def berr_inf(A, x, b):
 
 
 OVERLAP = 1  # second stream for the updates beside the next panels (--overlap)
+NODE_MAX = -1  # widest solve node (--solve-node-max; -1 the library default, 0 one node per cblk)
 # level-schedule choices written by pastix_amd/autotune.py (--no-tuning: the plan's rules)
 TUNING = os.path.join(ROOT, "pastix_amd", "tuning", "table.txt")
 TUNED = {}  # instance label -> whether a table entry applied
@@ -59,6 +60,7 @@ def setup(pastix, spm, A, grid, prec=None, resident=True, label=None):
     if resident:
         pastix.set_amd_iparm(h, pastix.amd_iparm.resident_input, 1)
     pastix.set_amd_iparm(h, pastix.amd_iparm.overlap, OVERLAP)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.solve_node_max, NODE_MAX)
     o = pastix.orderGrid(*grid)
     h.order = o
     pastix.subtask_order(h, S, o)
@@ -431,11 +433,14 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=500)
     ap.add_argument("--overlap", type=int, default=1,
                     help="1: updates on a second stream beside the next level's panels; 0: one stream")
+    ap.add_argument("--solve-node-max", type=int, default=-1,
+                    help="widest solve node in columns (-1: the library default; 0: one node per cblk)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only three device-to-device copies of 256 MiB (reference dispatch for HBM counters)")
     args = ap.parse_args()
-    global OVERLAP, TUNING
+    global OVERLAP, TUNING, NODE_MAX
     OVERLAP = args.overlap
+    NODE_MAX = args.solve_node_max
     if args.no_tuning:
         TUNING = None
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -47,7 +47,14 @@ typedef enum pastix_amd_iparm_e {
     PXA_IPARM_LEAF_TASKS       = 7,
     /* Workgroups of that persistent launch (default 0: 512).                */
     PXA_IPARM_LEAF_GRID        = 8,
-    PXA_IPARM_SIZE             = 9
+    /* Widest solve node, in columns (default -1: 1024; 0 or 1: every
+     * supernode its own node).  A solve node is a chain of supernodes (each
+     * the parent of the one before, coupled to all later ones by full
+     * blocks) that the solve treats as one diagonal block with its whole
+     * inverse: one level of the solve instead of one per supernode.  Setting
+     * it after the analysis rebuilds the schedule.                          */
+    PXA_IPARM_SOLVE_NODE_MAX   = 9,
+    PXA_IPARM_SIZE             = 10
 } pastix_amd_iparm_t;
 
 /* Set / get an engine parameter.  Returns PASTIX_SUCCESS or
@@ -188,6 +195,14 @@ int pastix_amd_save_tuning(const pastix_data_t *pastix_data, const char *path);
 /* Per level: 4 ints (cblks, multi-source tiles, single-pass, right-looking);
  * returns the number of levels written (-1 before the analysis). */
 int pastix_amd_level_info(const pastix_data_t *pastix_data, int32_t *out, int maxlevels);
+/* Solve structure after the analysis: out[0] solve nodes, [1] nodes of
+ * several supernodes, [2] solve levels, [3] merge rounds of the nodes'
+ * diagonal-block inverses (summed over the batches the factorisation issues
+ * them in), [4] their 64x64 tiles, [5] entries of the
+ * inverses of the nodes of several supernodes, [6] flops of the merge rounds
+ * (one triangular factor), [7] factorisation levels.  No reference
+ * counterpart (the solve nodes are this engine's). */
+int pastix_amd_solve_info(const pastix_data_t *pastix_data, double out[8]);
 /* Per level of the last numfact run with PXA_IPARM_PROFILE: 2 doubles (panel
  * steps, Schur updates, seconds); returns the number of levels written. */
 int pastix_amd_level_times(const pastix_data_t *pastix_data, double *out, int maxlevels);

@@ -160,6 +160,7 @@ extern "C" void pastixInitWithAffinity(pastix_data_t **pastix_data, PASTIX_Comm,
     d->ext[PXA_IPARM_OVERLAP] = 1;
     // off: its first GPU runs did not finish (the cause is not found yet)
     d->ext[PXA_IPARM_LEAF_TASKS] = 0;
+    d->ext[PXA_IPARM_SOLVE_NODE_MAX] = -1;
     d->steps = STEP_INIT;
     *pastix_data = d;
 }
@@ -543,6 +544,7 @@ extern "C" int pastix_subtask_blend(pastix_data_t *d) {
         pxa::PlanKnobs k;
         if (read_tuning(d, tf, k) == 1) d->knobs = k;
     }
+    d->knobs.node_max = (int32_t)d->ext[PXA_IPARM_SOLVE_NODE_MAX];
     d->plan = pxa::build_plan(d->symbol, d->layout, sc, d->knobs);
     int facto = (int)d->iparm[IPARM_FACTORIZATION];
     int ft = (facto == PastixFactGETRF) ? 2 : is_ldlt(d) ? 1 : 0;
@@ -1256,11 +1258,17 @@ extern "C" int pastixOrderApplyLevelOrder(pastix_order_t *o, pastix_int_t level_
 // --------------------------------------------------------------------------
 // Engine extensions (pastix_amd.h)
 // --------------------------------------------------------------------------
+static int replan(pastix_data_t *d);
 extern "C" int pastix_amd_set_iparm(pastix_data_t *d, int key, pastix_int_t value) {
     if (!d || key < 0 || key >= PXA_IPARM_SIZE) return PASTIX_ERR_BADPARAMETER;
     if (key == PXA_IPARM_FACTOR_PRECISION && value != -1 && value != SpmFloat && value != SpmDouble)
         return PASTIX_ERR_BADPARAMETER;
+    const bool replan_nodes = key == PXA_IPARM_SOLVE_NODE_MAX && d->ext[key] != value;
     d->ext[key] = value;
+    if (replan_nodes) {
+        d->knobs.node_max = (int32_t)value;
+        if (d->steps & STEP_ANALYSE) return replan(d);
+    }
     if (d->engine) engine_flags(d);
     return PASTIX_SUCCESS;
 }
@@ -1537,6 +1545,7 @@ extern "C" int pastix_amd_load_tuning(pastix_data_t *d, const char *path, int *a
     const int rc = read_tuning(d, path, k);
     if (rc < 0) return PASTIX_ERR_FILE;
     if (rc == 0) return PASTIX_SUCCESS;
+    k.node_max = d->knobs.node_max;  // the table holds per-level choices only
     d->knobs = k;
     *applied = 1;
     return replan(d);
@@ -1581,6 +1590,27 @@ extern "C" int pastix_amd_level_info(const pastix_data_t *d, int32_t *out, int m
         out[4 * lv + 3] = P.lv_panel_right[lv];
     }
     return nl;
+}
+// Solve structure (no reference counterpart: the solve nodes are this
+// engine's): out[0] nodes, [1] nodes of several cblks, [2] solve levels,
+// [3] merge rounds of the nodes' inverses (summed over the batches), [4] their tiles, [5] entries of
+// the inverses of the nodes of several cblks, [6] products of the merge
+// rounds (flops, one side), [7] factorisation levels.
+extern "C" int pastix_amd_solve_info(const pastix_data_t *d, double out[8]) {
+    if (!d || !out || !(d->steps & STEP_ANALYSE)) return PASTIX_ERR_BADPARAMETER;
+    const pxa::Plan &P = d->plan;
+    const int64_t nn = (int64_t)P.node_cblk.size() - 1;
+    int64_t groups = 0;
+    for (int64_t k = 0; k < nn; k++) groups += (P.node_cblk[k + 1] - P.node_cblk[k]) > 1;
+    out[0] = (double)nn;
+    out[1] = (double)groups;
+    out[2] = (double)P.nslevels;
+    out[3] = (double)(P.winv_round_ptr.empty() ? 0 : P.winv_round_ptr.size() - 1);
+    out[4] = (double)P.winv_tiles.size();
+    out[5] = (double)(P.winvnbr - P.winv_off.back());
+    out[6] = P.flops_winv;
+    out[7] = (double)P.nlevels;
+    return PASTIX_SUCCESS;
 }
 // Per level of the last numfact run with PXA_IPARM_PROFILE: panel steps and
 // Schur updates (seconds).

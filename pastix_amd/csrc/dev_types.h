@@ -24,6 +24,10 @@ struct DevSymbol {
     const int32_t *panel_blk;     // block index of every panel row
     const int32_t *cblk_brow;     // cblknbr+1
     const int32_t *browtab;
+    // solve nodes (runs of cblks solved as one diagonal block)
+    const int32_t *node_cblk;     // nnodes+1: first cblk of every node
+    const int64_t *node_winv;     // nnodes: offset of the node's K x K inverse
+    const int32_t *cblk_btile;    // 2 cblknbr: backward strips [tb, te) of every cblk
 };
 
 // One target tile of a Schur-complement update: rows [r0,r0+64) x columns
@@ -67,10 +71,32 @@ struct BwdItem {
     int32_t c, r0, r1, slot;
 };
 
-// Diagonal step of the solve: row block [64*ib, 64*ib+64) of cblk c, whose
-// backward tiles are items [b0, b1).
+// Update buffer of the forward solve, sliced by 64 permuted rows: entry j
+// of row g at base[g / 64] + 64 j + g % 64 (times the right-hand side's
+// leading dimension), cnt[g] entries.
+struct YSlices {
+    const int64_t *base;
+    const int32_t *cnt;
+};
+
+// Diagonal step of the solve: row block [64*ib, 64*ib+64) of solve node nd.
 struct DiagItem {
-    int32_t c, ib, b0, b1;
+    int32_t nd, ib, pad0, pad1;
+};
+
+// One 64x64 output tile of a merge round of the diagonal-block inverses:
+// node-local column runs [a, m) and [m, b) of node nd with known inverses
+// X11 and X22; rows [i0, i0+64) of the run [m, b), columns [j0, j0+64) of
+// [a, m).  Phase 1: T = M21 X11 (stored transposed at toff, leading
+// dimension m - a); phase 2: X21 = -X22 T.
+struct WinvTile {
+    int32_t nd, a, m, b, i0, j0;
+    int64_t toff;
+};
+
+// Leaf ib (64-column block) of member cblk c of a node of several cblks.
+struct WinvLeaf {
+    int32_t nd, c, ib, pad;
 };
 
 // Leaf-region task lists on the device (see Plan::leaf_*): incoming update

@@ -22,6 +22,7 @@ namespace pxa {
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (side_) (void)hipStreamSynchronize(side_);
+    if (winv_) (void)hipStreamSynchronize(winv_);
     if (fact_exec_) (void)hipGraphExecDestroy(fact_exec_);
     for (SolveGraph &g : solve_graphs_) (void)hipGraphExecDestroy(g.exec);
     for (const Owned &o : owned_) (void)hipFree(o.p);
@@ -32,6 +33,8 @@ Engine::~Engine() {
     if (ev_panel_) (void)hipEventDestroy(ev_panel_);
     if (ev_side_) (void)hipEventDestroy(ev_side_);
     if (side_) (void)hipStreamDestroy(side_);
+    if (ev_winv_) (void)hipEventDestroy(ev_winv_);
+    if (winv_) (void)hipStreamDestroy(winv_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -87,6 +90,8 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_panel_, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
+    HIPCHK(hipStreamCreateWithFlags(&winv_, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_winv_, hipEventDisableTiming));
     facto_ = factotype;
     prec_ = prec;
     n_ = s.n;
@@ -94,14 +99,10 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     coefnbr_ = L.coefnbr;
     winvnbr_ = P.winvnbr;
     maxwidth_ = (int)L.maxwidth;
-    // widest cblk of each level: sizes the solve kernels' workgroups and
-    // staging (the leaf levels hold thousands of narrow cblks)
-    lv_maxw_.assign(P.lv_cblk_ptr.size() - 1, 1);
-    for (size_t lv = 0; lv + 1 < P.lv_cblk_ptr.size(); lv++)
-        for (int32_t k = P.lv_cblk_ptr[lv]; k < P.lv_cblk_ptr[lv + 1]; k++) {
-            const int c = P.lv_cblks[k];
-            lv_maxw_[lv] = std::max<int>(lv_maxw_[lv], (int)(s.cblk_fcol[c + 1] - s.cblk_fcol[c]));
-        }
+    // widest node and cblk of each solve level: size the solve kernels'
+    // workgroups and staging (the leaf levels hold thousands of narrow cblks)
+    slv_maxw_.assign(P.slv_maxw.begin(), P.slv_maxw.end());
+    slv_maxcw_.assign(P.slv_maxcw.begin(), P.slv_maxcw.end());
     if (n_ >= (int64_t)INT32_MAX) return fail("matrix order exceeds the 32-bit device index range");
 
     S_.cblk_fcol = upload_as<int32_t>(*this, s.cblk_fcol);
@@ -119,6 +120,9 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     S_.panel_blk = upload(*this, P.panel_blk);
     S_.cblk_brow = upload_as<int32_t>(*this, L.cblk_brow);
     S_.browtab = upload_as<int32_t>(*this, L.browtab);
+    S_.node_cblk = upload(*this, P.node_cblk);
+    S_.node_winv = upload(*this, P.node_winv);
+    S_.cblk_btile = upload(*this, P.cblk_btile);
 
     lv_cblk_ptr_ = P.lv_cblk_ptr;
     lv_fact_cnt_ = P.lv_fact_cnt;
@@ -158,8 +162,8 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     lv_upd_dnext_ = P.lv_upd_dnext;
     lv_upd_mnext_ = P.lv_upd_mnext;
     lv_wt_next_ = P.lv_wt_next;
-    lv_fwd_ptr_ = P.lv_fwd_ptr;
-    lv_bwd_ptr_ = P.lv_bwd_ptr;
+    slv_fwd_ptr_ = P.slv_fwd_ptr;
+    slv_bwd_ptr_ = P.slv_bwd_ptr;
     d_lv_cblks_ = upload(*this, P.lv_cblks);
     lv_step_ptr_ = P.lv_step_ptr;
     step_cb_ptr_ = P.step_cb_ptr;
@@ -169,10 +173,15 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     d_step_cblks_ = upload(*this, P.step_cblks);
     d_ptrsm_ = upload(*this, P.ptrsm);
     d_pupd_ = upload(*this, P.pupd);
-    d_wide_ = upload(*this, P.wide_cblks);
-    n_wide_ = (int)P.wide_cblks.size();
-    d_bwd_itemptr_ = upload(*this, P.bwd_itemptr);
-    d_yrow_ptr_ = upload(*this, P.yrow_ptr);
+    d_winv_leaves_ = upload(*this, P.winv_leaves);
+    n_winv_leaves_ = (int)P.winv_leaves.size();
+    d_winv_tiles_ = upload(*this, P.winv_tiles);
+    winv_round_ptr_ = P.winv_round_ptr;
+    winv_batch_lv_ = P.winv_batch_lv;
+    winv_batch_leaf_ = P.winv_batch_leaf;
+    winv_batch_round_ = P.winv_batch_round;
+    ys_.base = upload(*this, P.ybase);
+    ys_.cnt = upload(*this, P.ycnt);
     d_ydst_ = upload(*this, P.ydst);
     nyrows_ = P.nyrows;
     d_shift_ = upload(*this, P.shift);
@@ -180,7 +189,7 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
     d_ents_ = upload(*this, P.ents);
     d_fwd_ = upload(*this, P.fwd);
     d_diag_ = upload(*this, P.diag);
-    lv_diag_ptr_ = P.lv_diag_ptr;
+    slv_diag_ptr_ = P.slv_diag_ptr;
     d_bwd_ = upload(*this, P.bwd);
     bwd_part_ = P.bwd_part;
     {
@@ -224,6 +233,8 @@ int Engine::init(int device, const Symbol &s, const Layout &L, const Plan &P, in
         d_W2_ = d_W1_;
     }
     d_stat_ = (int *)alloc(4 * sizeof(int));
+    // merge rounds of the solve nodes' inverses: the largest round's products
+    if (P.winv_tmax > 0 && !(d_tbuf_ = alloc((size_t)P.winv_tmax * fsz))) return -1;
     d_WB_ = alloc(winvnbr_ * fsz);
     if (d_WB_) HIPCHK(hipMemsetAsync(d_WB_, 0, winvnbr_ * fsz, stream_));
     // right-hand sides of one sweep: up to 32, the kernels taking them in
@@ -327,6 +338,37 @@ template <typename T> int Engine::factorize_t(double thr) {
     bool side_open = false;
     size_t lv_first = 0;
     cur_lv_ = 0;
+    // Whole inverses of the solve nodes' diagonal blocks (needed only by the
+    // solve), batch by batch: a batch's nodes are complete once the level of
+    // their last cblk is factored.  With the second stream they run on a
+    // third one from that point on, beside the factorisation; otherwise all
+    // at the end, in stream order.
+    const int nbatch = (int)winv_batch_lv_.size();
+    int kb_next = 0;
+    bool winv_open = false;
+    auto winv_batch = [&](int k, hipStream_t st) {
+        const int l0 = winv_batch_leaf_[k], nlv = winv_batch_leaf_[k + 1] - l0;
+        const int r0 = winv_batch_round_[k], nr = winv_batch_round_[k + 1] - r0;
+        const int32_t *rp = winv_round_ptr_.data() + r0;
+        T *tb = (T *)d_tbuf_;
+        if (facto_ != 2) {
+            launch_winv_nodes<T>(S_, d_winv_leaves_ + l0, nlv, d_winv_tiles_, rp, nr, L, W1, (T *)d_WB_, tb, st);
+        } else {
+            // inv(L) from the unit lower L, inv(U^T) (and its transpose
+            // inv(U)) from U^T kept in the U panel's square and below
+            launch_winv_nodes<T>(S_, d_winv_leaves_ + l0, nlv, d_winv_tiles_, rp, nr, L, W1, (T *)nullptr, tb, st);
+            launch_winv_nodes<T>(S_, d_winv_leaves_ + l0, nlv, d_winv_tiles_, rp, nr, U, W2, (T *)d_WB_, tb, st);
+        }
+    };
+    // batches complete before level lv (all factored): onto the third stream
+    auto winv_upto = [&](size_t lv) -> int {
+        if (!overlap || kb_next >= nbatch || (size_t)winv_batch_lv_[kb_next] >= lv) return 0;
+        HIPCHK(hipEventRecord(ev_winv_, stream_));
+        HIPCHK(hipStreamWaitEvent(winv_, ev_winv_, 0));
+        while (kb_next < nbatch && (size_t)winv_batch_lv_[kb_next] < lv) winv_batch(kb_next++, winv_);
+        winv_open = true;
+        return 0;
+    };
     if (leaf_active()) {
         // the leaf region in one persistent launch, then its updates into the
         // cblks above it; the level schedule takes over from its last level
@@ -348,6 +390,7 @@ template <typename T> int Engine::factorize_t(double thr) {
             prof_end(KC_ACC, pk, stream_);
         }
         lv_first = (size_t)leaf_levels_;
+        if (const int rc = winv_upto(lv_first)) return rc;
     }
     for (size_t lv = lv_first; lv < nl; lv++) {
         cur_lv_ = (int)lv;
@@ -448,6 +491,7 @@ template <typename T> int Engine::factorize_t(double thr) {
             continue;
         }
         const int dn = lv_upd_dnext_[lv], mn = lv_upd_mnext_[lv], wn = lv_wt_next_[lv];
+        if (const int rc = winv_upto(lv + 1)) return rc;
         HIPCHK(hipEventRecord(ev_panel_, stream_));
         if (side_open) HIPCHK(hipStreamWaitEvent(stream_, ev_side_, 0));
         sides(stream_, u0, dn, us, mn, w0, wn);
@@ -457,16 +501,20 @@ template <typename T> int Engine::factorize_t(double thr) {
         side_open = true;
     }
     if (side_open) HIPCHK(hipStreamWaitEvent(stream_, ev_side_, 0));
-    // full inverses of the wide diagonal blocks, needed only by the solve
-    if (facto_ != 2) {
-        launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, L, W1, (T *)d_WB_, stream_);
-    } else {
-        // inv(L) from the unit lower L, inv(U^T) (and its transpose inv(U))
-        // from U^T kept in the U panel's square, then the reference layout
-        launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, L, W1, (T *)nullptr, stream_);
-        launch_winv_offdiag<T>(S_, d_wide_, n_wide_, false, U, W2, (T *)d_WB_, stream_);
-        launch_lu_square_layout<T>(S_, d_fact_cblks_, n_fact_, L, U, stream_);
+    // the remaining inverse batches (all of them on one stream), profiled
+    // with the panel steps, outside every level
+    if (kb_next < nbatch) {
+        cur_lv_ = (int)nl;
+        const int pk = prof_begin(stream_);
+        while (kb_next < nbatch) winv_batch(kb_next++, stream_);
+        prof_end(KC_PANEL, pk, stream_);
     }
+    if (winv_open) {
+        HIPCHK(hipEventRecord(ev_winv_, winv_));
+        HIPCHK(hipStreamWaitEvent(stream_, ev_winv_, 0));
+    }
+    // LU: the squares in the reference layout once every inverse is formed
+    if (facto_ == 2) launch_lu_square_layout<T>(S_, d_fact_cblks_, n_fact_, L, U, stream_);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -683,7 +731,7 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
     T *ybuf = (T *)d_ybuf_;
     T *z = (T *)d_z_;
     const int64_t ldy = std::max<int64_t>(nyrows_, 1), ldz = std::max<int64_t>(n_, 1);
-    const int nl = (int)lv_cblk_ptr_.size() - 1;
+    const int nl = (int)slv_maxw_.size();  // solve levels
     const int chunk = nrhs_chunk_;
     for (int r0 = 0; r0 < nrhs; r0 += chunk) {
         const int nr = std::min(chunk, nrhs - r0);
@@ -703,33 +751,33 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
         const bool schur = sc < n_;
         if (part_ == 0 || part_ == 1) {
             for (int lv = 0; lv < nl; lv++) {
-                const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
-                const int f0 = lv_fwd_ptr_[lv], nf = lv_fwd_ptr_[lv + 1] - f0;
-                launch_fwd_diag<T>(S_, d_diag_ + d0, nd, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, nr, W1,
-                                   lv_maxw_[lv], stream_);
-                launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, d_ydst_, ybuf, ldy, lv_maxw_[lv], stream_);
+                const int d0 = slv_diag_ptr_[lv], nd = slv_diag_ptr_[lv + 1] - d0;
+                const int f0 = slv_fwd_ptr_[lv], nf = slv_fwd_ptr_[lv + 1] - f0;
+                launch_fwd_diag<T>(S_, d_diag_ + d0, nd, ys_, ybuf, ldy, xx, ldx, z, ldz, nr, W1,
+                                   slv_maxw_[lv], stream_);
+                launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, d_ydst_, ybuf, ldy, slv_maxcw_[lv], stream_);
             }
             if (schur)
-                launch_schur_rows<T>(sc, n_, nr, schur_interface ? 1 : 0, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz,
+                launch_schur_rows<T>(sc, n_, nr, schur_interface ? 1 : 0, ys_, ybuf, ldy, xx, ldx, z, ldz,
                                      stream_);
         }
         if (part_ == 1) {
             launch_copy2d<T>(n_, nr, z, ldz, xx, ldx, stream_);
             continue;
         }
-        if (part_ == 0 && schur) launch_schur_rows<T>(sc, n_, nr, 3, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, stream_);
+        if (part_ == 0 && schur) launch_schur_rows<T>(sc, n_, nr, 3, ys_, ybuf, ldy, xx, ldx, z, ldz, stream_);
         if (part_ == 3) launch_diag_scale<T>(sc, nr, d_diagpos_, L, false, xx, ldx, stream_);
         if (part_ == 2 || part_ == 3) launch_copy2d<T>(n_, nr, xx, ldx, z, ldz, stream_);
         if (schur && !schur_interface)
-            launch_schur_rows<T>(sc, n_, nr, 2, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, stream_);
+            launch_schur_rows<T>(sc, n_, nr, 2, ys_, ybuf, ldy, xx, ldx, z, ldz, stream_);
         for (int lv = nl - 1; lv >= 0; lv--) {
-            const int d0 = lv_diag_ptr_[lv], nd = lv_diag_ptr_[lv + 1] - d0;
-            const int b0 = lv_bwd_ptr_[lv], nb = lv_bwd_ptr_[lv + 1] - b0;
-            launch_bwd_prod<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, lv_maxw_[lv], stream_);
-            launch_bwd_diag<T>(S_, d_diag_ + d0, nd, d_bwd_, z, ldz, xx, ldx, nr, part, WB, lv_maxw_[lv], stream_);
+            const int d0 = slv_diag_ptr_[lv], nd = slv_diag_ptr_[lv + 1] - d0;
+            const int b0 = slv_bwd_ptr_[lv], nb = slv_bwd_ptr_[lv + 1] - b0;
+            launch_bwd_prod<T>(S_, d_bwd_ + b0, nb, U, xx, ldx, nr, part, slv_maxcw_[lv], stream_);
+            launch_bwd_diag<T>(S_, d_diag_ + d0, nd, d_bwd_, z, ldz, xx, ldx, nr, part, WB, slv_maxw_[lv], stream_);
         }
         if (schur && !schur_interface)
-            launch_schur_rows<T>(sc, n_, nr, 3, d_yrow_ptr_, ybuf, ldy, xx, ldx, z, ldz, stream_);
+            launch_schur_rows<T>(sc, n_, nr, 3, ys_, ybuf, ldy, xx, ldx, z, ldz, stream_);
     }
     HIPCHK(hipGetLastError());
     return 0;

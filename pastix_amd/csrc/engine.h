@@ -185,7 +185,9 @@ class Engine {
     // numfact issues the updates that the next level's panels do not wait
     // for on a second stream (joined back before the level after)
     hipStream_t side_ = nullptr;
-    hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev_panel_ = nullptr, ev_side_ = nullptr;
+    // the solve nodes' inverse merges, beside the factorisation
+    hipStream_t winv_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev_panel_ = nullptr, ev_side_ = nullptr, ev_winv_ = nullptr;
     struct Owned {
         void *p;
         size_t bytes;
@@ -197,25 +199,32 @@ class Engine {
 
     int64_t n_ = 0, cblknbr_ = 0, coefnbr_ = 0, winvnbr_ = 0;
     int maxwidth_ = 0;
-    std::vector<int> lv_maxw_;  // widest cblk per level
+    // solve levels: widest node (diagonal kernels), widest cblk (products)
+    std::vector<int> slv_maxw_, slv_maxcw_;
     int64_t schurfcol_ = 0;
     std::vector<int32_t> lv_fact_cnt_;
     int32_t *d_fact_cblks_ = nullptr;
     int n_fact_ = 0;
     DevSymbol S_{};
     // host copies of the level pointers
-    std::vector<int32_t> lv_cblk_ptr_, lv_upd_ptr_, lv_upd_split_, lv_fwd_ptr_, lv_bwd_ptr_;
+    std::vector<int32_t> lv_cblk_ptr_, lv_upd_ptr_, lv_upd_split_, slv_fwd_ptr_, slv_bwd_ptr_;
     std::vector<int32_t> lv_upd_dnext_, lv_upd_mnext_, lv_wt_next_;
     // device plan
-    int32_t *d_lv_cblks_ = nullptr, *d_bwd_itemptr_ = nullptr, *d_shift_ = nullptr;
-    int64_t *d_yrow_ptr_ = nullptr;
+    int32_t *d_lv_cblks_ = nullptr, *d_shift_ = nullptr;
+    YSlices ys_{};  // update-buffer slices
     int32_t *d_ydst_ = nullptr;
     int64_t nyrows_ = 0;
     void *d_ybuf_ = nullptr, *d_WB_ = nullptr;
     std::vector<int32_t> lv_step_ptr_, step_cb_ptr_, step_trsm_ptr_, step_upd_ptr_;
     std::vector<uint8_t> lv_panel_right_;
-    int32_t *d_step_cblks_ = nullptr, *d_wide_ = nullptr;
-    int n_wide_ = 0;
+    int32_t *d_step_cblks_ = nullptr;
+    // inverses of the solve nodes' diagonal blocks: leaves of the nodes of
+    // several cblks, merge-round tiles in batches, the rounds' product scratch
+    WinvLeaf *d_winv_leaves_ = nullptr;
+    int n_winv_leaves_ = 0;
+    WinvTile *d_winv_tiles_ = nullptr;
+    std::vector<int32_t> winv_round_ptr_, winv_batch_lv_, winv_batch_leaf_, winv_batch_round_;
+    void *d_tbuf_ = nullptr;
     PanelItem *d_ptrsm_ = nullptr, *d_pupd_ = nullptr;
     UpdItem *d_upd_ = nullptr;
     UpdEntry *d_ents_ = nullptr;
@@ -237,7 +246,7 @@ class Engine {
     int32_t n_leaf_out_ = 0, leaf_out_split_ = 0;
     FwdItem *d_fwd_ = nullptr;
     DiagItem *d_diag_ = nullptr;
-    std::vector<int32_t> lv_diag_ptr_;
+    std::vector<int32_t> slv_diag_ptr_;
     void *d_z_ = nullptr;  // forward-sweep result, n x nrhs_chunk
     BwdItem *d_bwd_ = nullptr;
     int32_t *d_perm_ = nullptr;  // permtab as int32 for device permutations

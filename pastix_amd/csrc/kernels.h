@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 #include "dev_types.h"
 
 namespace pxa {
@@ -42,15 +43,20 @@ template <typename T>
 void launch_lu_square_merge(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, hipStream_t st);
 template <typename T>
 void launch_lu_square_layout(const DevSymbol &S, const int32_t *cblks, int n, T *lcoef, T *ucoef, hipStream_t st);
-// winvt may be null (no transposed copy).
+// Inverses of the solve nodes' diagonal blocks from the 64-column inverses
+// the diagonal-block kernels leave in w (and wt): the leaves of nodes of
+// several cblks copied into the nodes' inverses, then nrounds merge rounds,
+// round r the tiles [round_ptr[r], round_ptr[r+1]) (round_ptr a host array),
+// reading the factor in coef.  wt may be null (no transposed copy); tbuf
+// holds the largest round's products.
 template <typename T>
-void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool transposed_factor, const T *coef,
-                         T *winv, T *winvt, hipStream_t st);
+void launch_winv_nodes(const DevSymbol &S, const WinvLeaf *leaves, int nleaves, const WinvTile *tiles,
+                       const int32_t *round_ptr, int nrounds, const T *coef, T *w, T *wt, T *tbuf, hipStream_t st);
 
 // Supernodal triangular solves (reference kernels/solve_ztrsmsp.c:332-557).
 // nrhs <= 8 per launch; the forward sweep writes z, the backward sweep x.
 template <typename T>
-void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int64_t *yrow_ptr, const T *ybuf,
+void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const YSlices &ys, const T *ybuf,
                      int64_t ldy, const T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
                      hipStream_t st);
 template <typename T>
@@ -80,7 +86,7 @@ void launch_copy(int64_t n, const double *x, double *y, hipStream_t st);
 // Schur rows [r0, r1) of a solve: mode 0 z = x, 1 z = x - pending forward
 // products, 2 x = 0, 3 x = z
 template <typename T>
-void launch_schur_rows(int64_t r0, int64_t r1, int nrhs, int mode, const int64_t *yrow_ptr, const T *ybuf, int64_t ldy,
+void launch_schur_rows(int64_t r0, int64_t r1, int nrhs, int mode, const YSlices &ys, const T *ybuf, int64_t ldy,
                        T *x, int64_t ldx, T *z, int64_t ldz, hipStream_t st);
 // out[0] = ||x||_2 by scaled sums of squares (partials: 2 * 512 doubles)
 void launch_nrm2(int64_t n, const double *x, double *partials, double *out, hipStream_t st);

@@ -12,8 +12,8 @@
 //                 blocks before it into the step's block (left-looking), or,
 //                 on levels with few rows, after the step, of the step's block
 //                 into every later block (right-looking).
-// k_winv_offdiag completes the inverse of the whole diagonal block for the
-// solve once the panel is factored.  Inverses are stored twice: W (lower,
+// k_winv_merge completes the inverses of the solve nodes' diagonal blocks
+// once the panels are factored.  Inverses are stored twice: W (lower,
 // column-major) for the forward sweep and WT = W^T for the backward sweep.
 #include <algorithm>
 
@@ -459,53 +459,167 @@ __global__ __launch_bounds__(256) void k_panel_upd_potrf(DevSymbol S, const int3
     panel_upd_tile<T>(S, it, kb, right, coef, coef, coef, 0);
 }
 
-// Off-diagonal 64-blocks of inv(L_cc) (and of its transpose) for the solve:
-//   W[ib,jb] = -W[ib,ib] * sum_{kb=jb}^{ib-1} L[ib,kb] W[kb,jb]
+// Inverses of the solve nodes' diagonal blocks, by recursive doubling.  The
+// diagonal-block kernels leave the inverse of every 64-column sub-block of a
+// cblk in place (W, and WT = W^T); a merge round joins adjacent column runs
+// [a, m) and [m, b) of a node whose inverses X11 and X22 are known:
+//   X21 = -X22 M21 X11
+// (M21: rows [m, b), columns [a, m) of the node's lower-triangular diagonal
+// block), on 64x64 tiles of X21 in two phases: T = M21 X11, stored
+// transposed, then X21 = -X22 T, written to W (column-major, lower) and to
+// WT.  M is read in the members' panels: node column j lies in member c
+// (o = its first node column), node row i >= o is panel row i - o of c
+// (a node's members are coupled by full blocks).  k_winv_leaves first copies
+// the members' 64-column inverses into the W and WT of nodes of several
+// cblks (a node of one cblk already holds them in place).
+
+// D(64x64) += X(64xK) Y(64xK)^T as wg_gemm_lds, where xcol(k) points at
+// X(0, k) (rows contiguous; called with increasing k, also for k >= K, whose
+// rows are not read) and Y(r, k) = Y[k + r*ldy] (consecutive k contiguous)
+template <typename T, int CH, class XCol>
+__device__ __forceinline__ void wg_gemm_lds_xcol_yt(Quad<T> &q, XCol &xcol, int mx, const T *Y, int my, int64_t ldy,
+                                                    int K, StageBuf<T, CH> &sb, int tid) {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int i = lane & 15, kq = lane >> 4;
+    constexpr int E = Vec<T>::E, NP = CH / (4 * E), NY = CH / 4;
+    static_assert(CH % (4 * E) == 0 && CH % 4 == 0, "chunk of whole passes");
+    const int re = E * (tid % (64 / E)), ke = tid / (64 / E);
+    // Y: thread (row yr, k group yk) fetches NY consecutive k of its row
+    const int yr = tid & 63, yk = (tid >> 6) * NY;
+    T gx[NP][E], gy[NY];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+            const int k = k0 + ke + 4 * E * u;
+            ld_rows(xcol(k) + re, re, k < K ? mx : 0, gx[u]);
+        }
+#pragma unroll
+        for (int e = 0; e < NY; e++) {
+            const int k = k0 + yk + e;
+            gy[e] = (k < K && yr < my) ? Y[k + (int64_t)yr * ldy] : (T)0;
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int u = 0; u < NP; u++) st_rows(&sb.xs[ke + 4 * E * u][re], gx[u]);
+#pragma unroll
+        for (int e = 0; e < NY; e++) sb.ys[yk + e][yr] = gy[e];
+    };
+    const bool active = (wr < mx) && (wc < my);
+    gload(0);
+    sstore();
+    __syncthreads();
+    for (int k0 = 0; k0 < K; k0 += CH) {
+        const bool more = k0 + CH < K;
+        if (more) gload(k0 + CH);
+        if (active) {
+#pragma unroll
+            for (int s = 0; s < CH / 4; s++) {
+                if (k0 + 4 * s < K) {
+                    const int kr = 4 * s + kq;
+                    const T a0 = sb.xs[kr][wr + i], a1 = sb.xs[kr][wr + 16 + i];
+                    const T b0 = sb.ys[kr][wc + i], b1 = sb.ys[kr][wc + 16 + i];
+                    q.c[0][0] = Mfma<T>::mma(a0, b0, q.c[0][0]);
+                    q.c[0][1] = Mfma<T>::mma(a0, b1, q.c[0][1]);
+                    q.c[1][0] = Mfma<T>::mma(a1, b0, q.c[1][0]);
+                    q.c[1][1] = Mfma<T>::mma(a1, b1, q.c[1][1]);
+                }
+            }
+        }
+        __syncthreads();
+        if (more) sstore();
+        __syncthreads();
+    }
+}
+
+// One workgroup per leaf (64-column block ib of member c of node nd, a node
+// of several cblks): the leaf's inverse block into the node's W (and WT).
 template <typename T>
-__global__ __launch_bounds__(256) void k_winv_offdiag(DevSymbol S, const int32_t *__restrict__ cblks, int lstride,
-                                                      int lrs, int lcs_is_ld, const T *__restrict__ coef,
-                                                      T *__restrict__ winv, T *__restrict__ winvt) {
-    __shared__ T Sm[64][65];
+__global__ __launch_bounds__(256) void k_winv_leaves(DevSymbol S, const WinvLeaf *__restrict__ leaves,
+                                                     T *__restrict__ w, T *__restrict__ wt) {
+    const WinvLeaf lf = leaves[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int nd = lf.nd, c = lf.c;
+    const int f0 = S.cblk_fcol[S.node_cblk[nd]];
+    const int K = S.cblk_fcol[S.node_cblk[nd + 1]] - f0;
+    const int o = S.cblk_fcol[c] - f0, wc = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
+    const int j0 = 64 * lf.ib, bw = min(64, wc - j0);
+    const int64_t xo = S.node_winv[nd] + (o + j0) + (int64_t)(o + j0) * K;
+    const int64_t mo = S.cblk_winv[c] + j0 + (int64_t)j0 * wc;
+    // 16 entries per thread, loads first
+    T v[16], vt[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int t = tid + 256 * u, i = t & 63, j = t >> 6;
+        const bool ok = i < bw && j < bw;
+        v[u] = ok ? w[mo + i + (int64_t)j * wc] : (T)0;
+        vt[u] = (ok && wt) ? wt[mo + i + (int64_t)j * wc] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int t = tid + 256 * u, i = t & 63, j = t >> 6;
+        if (i < bw && j < bw) {
+            w[xo + i + (int64_t)j * K] = v[u];
+            if (wt) wt[xo + i + (int64_t)j * K] = vt[u];
+        }
+    }
+}
+
+// 64-deep staging chunks: the late rounds have few tiles per launch, so each
+// workgroup's chunk latency is not hidden by others
+constexpr int WINV_CH = 16;
+
+template <typename T, int PHASE>
+__global__ __launch_bounds__(256) void k_winv_merge(DevSymbol S, const WinvTile *__restrict__ tiles,
+                                                    const T *__restrict__ coef, T *__restrict__ w,
+                                                    T *__restrict__ wt, T *__restrict__ tbuf) {
+    __shared__ StageBuf<T, WINV_CH> sb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    const int c = cblks[blockIdx.x];
-    const int K = S.cblk_fcol[c + 1] - S.cblk_fcol[c];
-    const int64_t ld = S.cblk_stride[c];
-    const T *P = coef + S.cblk_coef[c];
-    // L(r, k) = P[r + k*ld] (lower factor) or P[k + r*ld] (transposed storage)
-    const int64_t rs = lcs_is_ld ? ld : 1, cs = lcs_is_ld ? 1 : ld;
-    (void)lstride;
-    (void)lrs;
-    T *W = winv + S.cblk_winv[c];
-    T *WT = winvt ? winvt + S.cblk_winv[c] : nullptr;
-    for (int jb0 = 0; jb0 < K; jb0 += 64) {
-        const int nj = min(64, K - jb0);
-        for (int ib0 = jb0 + 64; ib0 < K; ib0 += 64) {
-            const int ni = min(64, K - ib0);
-            Quad<T> q;
-            q.zero();
-            if (wr < ni && wc < nj) {
-                View<T> A{P + (int64_t)(ib0 + wr) * rs + (int64_t)jb0 * cs, rs, cs};
-                View<T> B{W + jb0 + (int64_t)(jb0 + wc) * K, K, 1};
-                quad_gemm_nt(q, A, ni - wr, B, nj - wc, ib0 - jb0, lane);
+    const WinvTile t = tiles[blockIdx.x];
+    const int nd = t.nd, c0 = S.node_cblk[nd];
+    const int f0 = S.cblk_fcol[c0];
+    const int K = S.cblk_fcol[S.node_cblk[nd + 1]] - f0;
+    const int h1 = t.m - t.a, h2 = t.b - t.m;
+    const int mi = min(64, h2 - t.i0), mj = min(64, h1 - t.j0);
+    T *X = w + S.node_winv[nd];
+    Quad<T> q;
+    q.zero();
+    if constexpr (PHASE == 1) {
+        // T[i0+i, j0+j] = sum_{k >= j0} M[m+i0+i, a+k] X[a+k, a+j0+j], M in
+        // the members' panels (the member of a column followed as k grows)
+        const int row0 = t.m + t.i0;
+        int mc = c0, mend = S.cblk_fcol[c0 + 1] - f0, mo = 0;
+        const T *mbase = coef + S.cblk_coef[c0] + row0;
+        int64_t mld = S.cblk_stride[c0];
+        auto xcol = [&](int k) -> const T * {
+            const int gc = min(t.a + t.j0 + k, t.m - 1);
+            while (gc >= mend) {
+                mc++;
+                mo = mend;
+                mend = S.cblk_fcol[mc + 1] - f0;
+                mbase = coef + S.cblk_coef[mc] + (row0 - mo);
+                mld = S.cblk_stride[mc];
             }
-            QFOR(Sm[wr + qrow][wc + qcol] = q.c[m_][n_][r_];)
-            __syncthreads();
-            Quad<T> q2;
-            q2.zero();
-            if (wr < ni && wc < nj) {
-                View<T> A{W + (ib0 + wr) + (int64_t)ib0 * K, 1, K};
-                View<T> B{&Sm[0][wc], 1, 65};
-                quad_gemm_nt(q2, A, ni - wr, B, nj - wc, ni, lane);
-                QFOR(if (qrow < ni - wr && qcol < nj - wc) {
-                    const int gi = ib0 + wr + qrow, gj = jb0 + wc + qcol;
-                    const T v = -q2.c[m_][n_][r_];
-                    W[gi + (int64_t)gj * K] = v;
-                    if (winvt) WT[gj + (int64_t)gi * K] = v;
-                })
-            }
-            __syncthreads();
-        }
+            return mbase + (int64_t)(gc - mo) * mld;
+        };
+        wg_gemm_lds_xcol_yt<T, WINV_CH>(q, xcol, mi, X + (t.a + t.j0) + (int64_t)(t.a + t.j0) * K, mj, K, h1 - t.j0, sb,
+                                   tid);
+        T *Tt = tbuf + t.toff;
+        QFOR(if (wr + qrow < mi && wc + qcol < mj)
+                 Tt[(t.j0 + wc + qcol) + (int64_t)(t.i0 + wr + qrow) * h1] = q.c[m_][n_][r_];)
+    } else {
+        // X[m+i0+i, a+j0+j] = -sum_{k <= i0+i} X[m+i0+i, m+k] T[k, j0+j]
+        wg_gemm_lds<T, WINV_CH>(q, X + (t.m + t.i0) + (int64_t)t.m * K, mi, K, tbuf + t.toff + t.j0, mj, h1,
+                           min(t.i0 + 64, h2), sb, tid);
+        T *XT = wt ? wt + S.node_winv[nd] : nullptr;
+        QFOR(if (wr + qrow < mi && wc + qcol < mj) {
+            const int gi = t.m + t.i0 + wr + qrow, gj = t.a + t.j0 + wc + qcol;
+            const T v = -q.c[m_][n_][r_];
+            X[gi + (int64_t)gj * K] = v;
+            if (XT) XT[gj + (int64_t)gi * K] = v;
+        })
     }
 }
 
@@ -866,11 +980,15 @@ void launch_panel_upd_potrf(const DevSymbol &S, const int32_t *cblks, int npot, 
                        right ? 1 : 0, coef, winv, winvt, thr, stat);
 }
 template <typename T>
-void launch_winv_offdiag(const DevSymbol &S, const int32_t *cblks, int n, bool transposed_factor, const T *coef,
-                         T *winv, T *winvt, hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_winv_offdiag<T>, dim3(n), dim3(256), 0, st, S, cblks, 0, 0, transposed_factor ? 1 : 0, coef,
-                       winv, winvt);
+void launch_winv_nodes(const DevSymbol &S, const WinvLeaf *leaves, int nleaves, const WinvTile *tiles,
+                       const int32_t *round_ptr, int nrounds, const T *coef, T *w, T *wt, T *tbuf, hipStream_t st) {
+    if (nleaves > 0) hipLaunchKernelGGL(k_winv_leaves<T>, dim3(nleaves), dim3(256), 0, st, S, leaves, w, wt);
+    for (int r = 0; r < nrounds; r++) {
+        const int t0 = round_ptr[r], nt = round_ptr[r + 1] - t0;
+        if (nt <= 0) continue;
+        hipLaunchKernelGGL((k_winv_merge<T, 1>), dim3(nt), dim3(256), 0, st, S, tiles + t0, coef, w, wt, tbuf);
+        hipLaunchKernelGGL((k_winv_merge<T, 2>), dim3(nt), dim3(256), 0, st, S, tiles + t0, coef, w, wt, tbuf);
+    }
 }
 
 void launch_leaf_init(const LeafDev &LP, int32_t *ctr, int32_t *queue, int32_t *sync, hipStream_t st) {
@@ -902,8 +1020,8 @@ void launch_leaf_tasks(const DevSymbol &S, const LeafDev &LP, const UpdEntry *en
                                       const T *, bool, hipStream_t);                                             \
     template void launch_panel_upd_potrf<T>(const DevSymbol &, const int32_t *, int, const PanelItem *, int, int,  \
                                             bool, T *, T *, T *, double, int *, hipStream_t);                    \
-    template void launch_winv_offdiag<T>(const DevSymbol &, const int32_t *, int, bool, const T *, T *, T *,       \
-                                         hipStream_t);
+    template void launch_winv_nodes<T>(const DevSymbol &, const WinvLeaf *, int, const WinvTile *,               \
+                                       const int32_t *, int, const T *, T *, T *, T *, hipStream_t);
 INST(double)
 INST(float)
 

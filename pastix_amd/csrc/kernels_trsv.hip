@@ -4,18 +4,23 @@
 // The sweeps read b from x and leave the solution in x; the forward result
 // z = inv(L) b lives in a separate buffer so that every workgroup of a cblk
 // can read the cblk's input while others write their rows of the output.
+// The sweeps go over solve nodes: a node is one cblk, or a run of cblks
+// whose diagonal blocks and coupling blocks form one dense lower-triangular
+// block, solved with that block's whole inverse.
 // Forward sweep, level by level:
-//   k_fwd_diag : workgroup (c, ib) gathers, for rows [0, 64*ib+64) of cblk c,
-//                b minus the update-buffer entries of these rows (stored
+//   k_fwd_diag : workgroup (nd, ib) gathers, for rows [0, 64*ib+64) of node
+//                nd, b minus the update-buffer entries of these rows (stored
 //                contiguously per row, sources in increasing order), then
-//                writes rows [64*ib, 64*ib+64) of z_c = inv(L_cc) y;
-//   k_fwd_prod : L_off z_c on 64-row tiles, each product row stored at its
-//                entry of the update buffer.
+//                writes rows [64*ib, 64*ib+64) of z_nd = inv(L_nd) y;
+//   k_fwd_prod : the products of the members' panel rows outside their node
+//                on 64-row tiles, each product row stored at its entry of
+//                the update buffer.
 // Backward sweep, levels from the root down:
-//   k_bwd_prod : partial[j] = sum over a 64x64 tile of L[r, j] x[row(r)];
-//   k_bwd_diag : workgroup (c, ib) forms y = z_c - the tiles' partials for
+//   k_bwd_prod : partial[j] = sum over a strip of L[r, j] x[row(r)] (panel
+//                rows outside the node);
+//   k_bwd_diag : workgroup (nd, ib) forms y = z_nd - the strips' partials for
 //                columns [64*ib, K) and writes rows [64*ib, 64*ib+64) of
-//                x_c = inv(L_cc)^T y.
+//                x_nd = inv(L_nd)^T y.
 // The diagonal inverses are stored whole (zeros in the other triangle), so
 // the triangular products run over full 64-column blocks.  Every output
 // element is produced in a fixed order by one workgroup: no atomics,
@@ -110,22 +115,23 @@ __device__ __forceinline__ void wave_sum_store_rows(T (&acc)[E][NR], T *red, int
 // ---------------------------------------------------------------------------
 template <bool CO, typename T, int NR, int NW>
 __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem it,
-                                              const int64_t *__restrict__ yrow_ptr, const T *__restrict__ ybuf,
+                                              const YSlices ys, const T *__restrict__ ybuf,
                                               int64_t ldy, const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
                                               int64_t ldz, int nrhs, const T *__restrict__ w1, T *y, T *red) {
     const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
-    const int c = it.c, r0 = it.ib * 64;
-    const int fc = S.cblk_fcol[c];
-    const int Kc = S.cblk_fcol[c + 1] - fc;
+    const int nd = it.nd, r0 = it.ib * 64;
+    const int fc = S.cblk_fcol[S.node_cblk[nd]];
+    const int Kc = S.cblk_fcol[S.node_cblk[nd + 1]] - fc;
     const int nrow = min(Kc, r0 + 64);
-    // y = b minus the pending products of rows [0, nrow) (a row's entries
-    // are contiguous in the update buffer): gl consecutive lanes per row, as
-    // many as fill the workgroup in one pass (at most 64, at most the cblk's
-    // first row's entry count), lane j summing the entries j, j + gl, ... of
-    // its row, UL loads issued together, in order; the lanes' sums combined
-    // by xor shuffles in a fixed order
+    // y = b minus the pending products of rows [0, nrow): gl consecutive
+    // lanes per row, as many as fill the workgroup in one pass (at most 64,
+    // at most the node's first row's entry count), lane j summing the
+    // entries j, j + gl, ... of its row, UL loads issued together, in order;
+    // the lanes' sums combined by xor shuffles in a fixed order.  With one
+    // lane per row, consecutive lanes read consecutive rows' entries, which
+    // the sliced update buffer keeps side by side.
     const int tasks = nrow * nrhs;
-    const int ne = (int)(yrow_ptr[fc + 1] - yrow_ptr[fc]);
+    const int ne = ys.cnt[fc];
     int gl = 1;
     while (gl < 64 && gl * 2 <= ne && tasks * gl * 2 <= NW * 64) gl *= 2;
     // rows of the inverse: E consecutive per lane (16-byte loads); the first
@@ -134,7 +140,7 @@ __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem
     constexpr int E = Vec<T>::E, G = 64 / E, PF = 4;
     const int rr = E * (r % G), kc = r / G;
     const bool mrow = r0 + rr < Kc;
-    const T *M = w1 + S.cblk_winv[c] + r0 + rr;
+    const T *M = w1 + S.node_winv[nd] + r0 + rr;
     T mpre[PF][E];
 #pragma unroll
     for (int f = 0; f < PF; f++) {
@@ -147,12 +153,12 @@ __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem
         const int i = task % nrow, u = task / nrow;
         T v = 0;
         if (task < tasks) {
-            const T *yb = ybuf + u * ldy;
-            const int64_t pe = yrow_ptr[fc + i + 1];
-            for (int64_t p = yrow_ptr[fc + i] + j; p < pe; p += (int64_t)UL * gl) {
+            const int g = fc + i, ne_i = ys.cnt[g];
+            const T *yb = ybuf + u * ldy + ys.base[g >> 6] + (g & 63);
+            for (int e = j; e < ne_i; e += UL * gl) {
                 T a[UL];
 #pragma unroll
-                for (int k = 0; k < UL; k++) a[k] = (p + (int64_t)k * gl < pe) ? ld_co<CO>(yb + p + (int64_t)k * gl) : (T)0;
+                for (int k = 0; k < UL; k++) a[k] = (e + k * gl < ne_i) ? ld_co<CO>(yb + (int64_t)(e + k * gl) * 64) : (T)0;
 #pragma unroll
                 for (int k = 0; k < UL; k++) v += a[k];
             }
@@ -314,11 +320,29 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
                                               int u0, const T *__restrict__ part, const T *__restrict__ wb, T *y,
                                               T *red) {
     const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
-    const int c = it.c, r0 = it.ib * 64;
-    const int fc = S.cblk_fcol[c];
-    const int Kc = S.cblk_fcol[c + 1] - fc;
-    // the tiles of one cblk own consecutive partial slots, Kc apart
-    const int64_t s0 = (it.b1 > it.b0) ? (int64_t)bitems[it.b0].slot : 0;
+    const int nd = it.nd, r0 = it.ib * 64;
+    const int cb0 = S.node_cblk[nd], cb1 = S.node_cblk[nd + 1];
+    const int fc = S.cblk_fcol[cb0];
+    const int Kc = S.cblk_fcol[cb1] - fc;
+    // Column k of the node belongs to member cblk m (column k - o of it, o
+    // = fcol[m] - fc); m's strips [tb, te) own consecutive partial slots, w
+    // apart, the first at bitems[tb].slot.  pcol returns the address of the
+    // partial of right-hand side 0 in m's first strip; the strips follow at
+    // stride w * nrhs_tot.
+    auto pcol = [&](int k, int &tb, int &te, int64_t &stride) -> const T * {
+        int lo = cb0, hi = cb1 - 1;  // the last member whose first column is <= k
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.cblk_fcol[mid] - fc <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int o = S.cblk_fcol[lo] - fc, w = S.cblk_fcol[lo + 1] - S.cblk_fcol[lo];
+        tb = S.cblk_btile[2 * lo];
+        te = S.cblk_btile[2 * lo + 1];
+        stride = (int64_t)w * nrhs_tot;
+        const int64_t s0 = (te > tb) ? (int64_t)bitems[tb].slot : 0;
+        return part + s0 * nrhs_tot + (int64_t)u0 * w + (k - o);
+    };
     // y = z_c - the tiles' partials for columns [r0, Kc).  Several
     // right-hand sides: 64 columns at a time, the waves splitting the tiles.
     if constexpr (NR > 1) {
@@ -328,12 +352,16 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
 #pragma unroll
             for (int u = 0; u < NR; u++) acc[u] = 0;
             if (k < Kc) {
+                int tb, te;
+                int64_t stride;
+                const T *p0 = pcol(k, tb, te, stride);
+                const int w = (int)(stride / nrhs_tot);
 #pragma unroll 4
-                for (int j = it.b0 + q; j < it.b1; j += NW) {
-                    const T *pp = part + (s0 + (int64_t)(j - it.b0) * Kc) * nrhs_tot + (int64_t)u0 * Kc + k;
+                for (int j = tb + q; j < te; j += NW) {
+                    const T *pp = p0 + (int64_t)(j - tb) * stride;
 #pragma unroll
                     for (int u = 0; u < NR; u++)
-                        if (u < nrhs) acc[u] += ld_co<CO>(pp + u * Kc);
+                        if (u < nrhs) acc[u] += ld_co<CO>(pp + u * w);
                 }
             }
             // y lives in LDS: a plain store; z comes from other tasks
@@ -354,12 +382,16 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
 #pragma unroll
         for (int u = 0; u < NR; u++) acc[u] = 0;
         if (live) {
+            int tb, te;
+            int64_t stride;
+            const T *p0 = pcol(k, tb, te, stride);
+            const int w = (int)(stride / nrhs_tot);
 #pragma unroll 4
-            for (int j = it.b0 + h; j < it.b1; j += L) {
-                const T *pp = part + (s0 + (int64_t)(j - it.b0) * Kc) * nrhs_tot + (int64_t)u0 * Kc + k;
+            for (int j = tb + h; j < te; j += L) {
+                const T *pp = p0 + (int64_t)(j - tb) * stride;
 #pragma unroll
                 for (int u = 0; u < NR; u++)
-                    if (u < nrhs) acc[u] += ld_co<CO>(pp + u * Kc);
+                    if (u < nrhs) acc[u] += ld_co<CO>(pp + u * w);
             }
         }
         if (h < L) {
@@ -391,7 +423,7 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
         for (int u = 0; u < NR; u++) acc2[e][u] = 0;
     if (r0 + rr < Kc) {
         // wb holds inv(L_cc)^T (or inv(U_cc)) column-major: upper triangular
-        const T *M = wb + S.cblk_winv[c] + r0 + rr;
+        const T *M = wb + S.node_winv[nd] + r0 + rr;
 #pragma unroll 4
         for (int k = r0 + q * E + kc; k < Kc; k += NW * E) {
             T m[E];
@@ -412,8 +444,7 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
 // ------------------------------ level kernels -------------------------------
 template <typename T, int NR, int NW>
 __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
-                                                      const int64_t *__restrict__ yrow_ptr,
-                                                      const T *__restrict__ ybuf, int64_t ldy,
+                                                      const YSlices ys, const T *__restrict__ ybuf, int64_t ldy,
                                                       const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
                                                       int64_t ldz, int nrhs, const T *__restrict__ w1) {
     extern __shared__ char smem_raw[];
@@ -422,7 +453,7 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
     if (id >= nitems) return;
     // right-hand sides [NR*blockIdx.y, NR*blockIdx.y + NR)
     const int u0 = NR * blockIdx.y;
-    fwd_diag_body<false, T, NR, NW>(S, items[id], yrow_ptr, ybuf + u0 * ldy, ldy, b + u0 * ldb, ldb, z + u0 * ldz,
+    fwd_diag_body<false, T, NR, NW>(S, items[id], ys, ybuf + u0 * ldy, ldy, b + u0 * ldb, ldb, z + u0 * ldz,
                                     ldz, min(NR, nrhs - u0), w1, reinterpret_cast<T *>(smem_raw), red);
 }
 
@@ -477,7 +508,7 @@ __global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagIte
 
 // ------------------------------- launchers ---------------------------------
 template <typename T>
-void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int64_t *yrow_ptr, const T *ybuf,
+void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const YSlices &ys, const T *ybuf,
                      int64_t ldy, const T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
                      hipStream_t st) {
     if (n <= 0) return;
@@ -489,21 +520,21 @@ void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const int
     if (nrhs == 1) {
         if (maxwidth <= SOLVE_TINY)
             hipLaunchKernelGGL((k_fwd_diag<T, 1, 1>), dim3(grid8(n)), dim3(64), (size_t)maxwidth * sizeof(T), st, S,
-                               items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+                               items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
         else if (narrow)
             hipLaunchKernelGGL((k_fwd_diag<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
-                               items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+                               items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
         else
             hipLaunchKernelGGL((k_fwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st,
-                               S, items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+                               S, items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
     } else {
         if (narrow)
             hipLaunchKernelGGL((k_fwd_diag<T, 2, 4>), dim3(grid8(n), (nrhs + 1) / 2), dim3(256),
-                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz,
+                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, ys, ybuf, ldy, b, ldb, z, ldz,
                                nrhs, w1);
         else
             hipLaunchKernelGGL((k_fwd_diag<T, 2, 16>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
-                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, yrow_ptr, ybuf, ldy, b, ldb, z, ldz,
+                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, ys, ybuf, ldy, b, ldb, z, ldz,
                                nrhs, w1);
     }
 }
@@ -564,7 +595,7 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
 
 
 #define INST(T)                                                                                                   \
-    template void launch_fwd_diag<T>(const DevSymbol &, const DiagItem *, int, const int64_t *, const T *,       \
+    template void launch_fwd_diag<T>(const DevSymbol &, const DiagItem *, int, const YSlices &, const T *,       \
                                      int64_t, const T *, int64_t, T *, int64_t, int, const T *, int,             \
                                      hipStream_t);                                                               \
     template void launch_fwd_prod<T>(const DevSymbol &, const FwdItem *, int, const T *, const T *, int64_t, int, \

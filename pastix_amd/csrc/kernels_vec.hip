@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void k_spmv_t(int64_t n, const int64_t *__rest
 // order), 2 x = 0, 3 x = z.
 template <typename T>
 __global__ __launch_bounds__(256) void k_schur_rows(int64_t r0, int64_t r1, int nrhs, int mode,
-                                                    const int64_t *__restrict__ yrow_ptr, const T *__restrict__ ybuf,
+                                                    const YSlices ys, const T *__restrict__ ybuf,
                                                     int64_t ldy, T *__restrict__ x, int64_t ldx, T *__restrict__ z,
                                                     int64_t ldz) {
     const int64_t i = r0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -263,7 +263,8 @@ __global__ __launch_bounds__(256) void k_schur_rows(int64_t r0, int64_t r1, int 
             z[i + u * ldz] = x[i + u * ldx];
         } else if (mode == 1) {
             T v = x[i + u * ldx];
-            for (int64_t p = yrow_ptr[i]; p < yrow_ptr[i + 1]; p++) v -= ybuf[p + u * ldy];
+            const T *yb = ybuf + u * ldy + ys.base[i >> 6] + (i & 63);
+            for (int j = 0; j < ys.cnt[i]; j++) v -= yb[(int64_t)j * 64];
             z[i + u * ldz] = v;
         } else if (mode == 2) {
             x[i + u * ldx] = (T)0;
@@ -515,15 +516,15 @@ void launch_gemv_n(int64_t n, int m, const double *V, int64_t ldv, const double 
     if (n > 0) hipLaunchKernelGGL(k_gemv_n, dim3(nblk(n)), dim3(256), 0, st, n, m, V, ldv, y, alpha, x);
 }
 template <typename T>
-void launch_schur_rows(int64_t r0, int64_t r1, int nrhs, int mode, const int64_t *yrow_ptr, const T *ybuf, int64_t ldy,
+void launch_schur_rows(int64_t r0, int64_t r1, int nrhs, int mode, const YSlices &ys, const T *ybuf, int64_t ldy,
                        T *x, int64_t ldx, T *z, int64_t ldz, hipStream_t st) {
     if (r1 > r0)
-        hipLaunchKernelGGL(k_schur_rows<T>, dim3(nblk(r1 - r0)), dim3(256), 0, st, r0, r1, nrhs, mode, yrow_ptr, ybuf,
+        hipLaunchKernelGGL(k_schur_rows<T>, dim3(nblk(r1 - r0)), dim3(256), 0, st, r0, r1, nrhs, mode, ys, ybuf,
                            ldy, x, ldx, z, ldz);
 }
-template void launch_schur_rows<double>(int64_t, int64_t, int, int, const int64_t *, const double *, int64_t, double *,
+template void launch_schur_rows<double>(int64_t, int64_t, int, int, const YSlices &, const double *, int64_t, double *,
                                         int64_t, double *, int64_t, hipStream_t);
-template void launch_schur_rows<float>(int64_t, int64_t, int, int, const int64_t *, const float *, int64_t, float *,
+template void launch_schur_rows<float>(int64_t, int64_t, int, int, const YSlices &, const float *, int64_t, float *,
                                        int64_t, float *, int64_t, hipStream_t);
 void launch_nrm2(int64_t n, const double *x, double *partials, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_nrm2_part, dim3(RED_BLOCKS), dim3(256), 0, st, n, x, partials);

@@ -93,39 +93,13 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk, const Plan
     P.lv_upd_dnext.assign(nl, 0);
     P.lv_upd_mnext.assign(nl, 0);
     P.lv_wt_next.assign(nl, 0);
-    P.lv_fwd_ptr.assign(nl + 1, 0);
-    P.lv_bwd_ptr.assign(nl + 1, 0);
     P.lv_wt_ptr.assign(nl + 1, 0);
-    P.lv_diag_ptr.assign(nl + 1, 0);
     P.lv_wsize.assign(nl, 0);
     P.lv_single_pass.assign(nl, 0);
     P.lv_panel_right.assign(nl, 0);
-    P.bwd_itemptr.assign(cb + nl, 0);
-    // update buffer: one row per off-diagonal panel row; CSR by permuted row
-    std::vector<int64_t> yoff(cb + 1, 0);
-    for (int64_t c = 0; c < cb; c++) yoff[c + 1] = yoff[c] + (L.cblk_stride[c] - width(c));
-    P.nyrows = yoff[cb];
-    P.yrow_ptr.assign(s.n + 1, 0);
-    for (int64_t c = 0; c < cb; c++)
-        for (int64_t r = width(c); r < L.cblk_stride[c]; r++) P.yrow_ptr[P.panel_grow[P.rowbase[c] + r] + 1]++;
-    for (int64_t g = 0; g < s.n; g++) P.yrow_ptr[g + 1] += P.yrow_ptr[g];
-    P.ydst.assign(P.nyrows, 0);
-    {
-        std::vector<int64_t> pos(P.yrow_ptr.begin(), P.yrow_ptr.end() - 1);
-        for (int64_t c = 0; c < cb; c++)
-            for (int64_t r = width(c); r < L.cblk_stride[c]; r++)
-                P.ydst[yoff[c] + r - width(c)] = (int32_t)pos[P.panel_grow[P.rowbase[c] + r]]++;
-    }
     std::vector<KeyEnt> ke;
     std::vector<Range> rr, cr;
-    int64_t bslot = 0;            // backward partial slots handed out so far
     const int64_t WT_P = 128;     // source rows (p) of one product tile; 64 columns (q)
-    const int32_t FWD_ROWS = 64;  // panel rows of one forward product tile
-    // panel rows of one backward product strip: 64 on the levels of narrow
-    // cblks, BWD_ROWS_WIDE where a cblk is wider than 64 columns (its
-    // diagonal step then sums a quarter of the partials, while the level
-    // still has a few hundred strips)
-    const int32_t BWD_ROWS = 64, BWD_ROWS_WIDE = 256;
 
     // Update tiles of target panels from the sources `srcs` (increasing cblk
     // index), for the targets `keep` accepts: every run of a source's blocks
@@ -245,7 +219,7 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk, const Plan
     P.step_upd_ptr.push_back(0);
 
     for (int32_t lv = 0; lv < nl; lv++) {
-        const int32_t l0 = P.lv_cblk_ptr[lv], l1 = P.lv_cblk_ptr[lv + 1];
+        const int32_t l0 = P.lv_cblk_ptr[lv];
         // the factored cblks of the level: [l0, lf)
         const int32_t lf = l0 + P.lv_fact_cnt[lv];
         // blocked panel steps of this level
@@ -409,41 +383,233 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk, const Plan
             P.lv_wt_ptr[lv + 1] = (int32_t)P.wtiles.size();
         }
 
-        // forward products of the sources of this level
-        for (int32_t k = l0; k < lf; k++) {
-            const int64_t c = P.lv_cblks[k];
-            const int64_t K = width(c), st = L.cblk_stride[c];
-            for (int64_t r0 = K; r0 < st; r0 += FWD_ROWS) {
-                int64_t r1 = std::min<int64_t>(st, r0 + FWD_ROWS);
-                P.fwd.push_back(FwdItem{(int32_t)c, (int32_t)r0, (int32_t)r1, (int32_t)(yoff[c] + r0 - K)});
+    }
+    // ---- solve nodes, solve levels and their items ----
+    {
+        const int32_t nmax = knobs.node_max < 0 ? SOLVE_NODE_MAX : knobs.node_max;
+        // no merged runs beside a Schur part (its cblks are not solved)
+        const bool merge = nmax > 1 && sc == cb;
+        P.node_cblk.assign(1, 0);
+        P.cblk_node.assign(cb, 0);
+        P.cblk_intra.assign(cb, 0);
+        for (int64_t c = 0; c < cb;) {
+            int64_t e = c + 1, K = width(c);  // members [c, e), K columns
+            while (merge && e < cb) {
+                // e is the parent of the last member (its first off-diagonal
+                // block faces e) ...
+                const int64_t fb = s.cblk_bloknum[e - 1] + 1;
+                if (fb >= s.cblk_bloknum[e] || s.blok_fcblk[fb] != e || K + width(e) > nmax) break;
+                // ... and every member's off-diagonal rows go on with e's columns
+                bool ok = true;
+                for (int64_t k = c; k < e && ok; k++) {
+                    const int64_t r0 = width(k) + P.cblk_intra[k];
+                    if (r0 + width(e) > L.cblk_stride[k]) {
+                        ok = false;
+                        break;
+                    }
+                    const int32_t *g = &P.panel_grow[P.rowbase[k] + r0];
+                    for (int64_t t = 0; t < width(e) && ok; t++) ok = g[t] == s.cblk_fcol[e] + t;
+                }
+                if (!ok) break;
+                for (int64_t k = c; k < e; k++) P.cblk_intra[k] += (int32_t)width(e);
+                K += width(e);
+                e++;
+            }
+            const int32_t nd = (int32_t)P.node_cblk.size() - 1;
+            for (int64_t k = c; k < e; k++) P.cblk_node[k] = nd;
+            P.node_cblk.push_back((int32_t)e);
+            c = e;
+        }
+        const int32_t nn = (int32_t)P.node_cblk.size() - 1;
+        auto nfcol = [&](int32_t nd) { return s.cblk_fcol[P.node_cblk[nd]]; };
+        auto nwidth = [&](int32_t nd) { return s.cblk_fcol[P.node_cblk[nd + 1]] - nfcol(nd); };
+        auto nfact = [&](int32_t nd) { return factored(P.node_cblk[nd]); };
+        P.node_winv.assign(nn, 0);
+        for (int32_t nd = 0; nd < nn; nd++) {
+            const int32_t c0 = P.node_cblk[nd];
+            if (P.node_cblk[nd + 1] - c0 == 1) {
+                P.node_winv[nd] = P.winv_off[c0];
+            } else {
+                const int64_t K = nwidth(nd);
+                P.node_winv[nd] = P.winvnbr;
+                P.winvnbr += K * K;
             }
         }
-        P.lv_fwd_ptr[lv + 1] = (int32_t)P.fwd.size();
-        // backward gather items for the cblks of this level (partial slots
-        // numbered over all levels, so that several levels may be in flight)
-        {
-            bool lv_wide = false;
-            for (int32_t k = l0; k < l1; k++) lv_wide = lv_wide || width(P.lv_cblks[k]) > 64;
-            const int32_t seg = l0 + lv;
-            for (int32_t k = l0; k < l1; k++) {
-                const int64_t c = P.lv_cblks[k];
-                const int32_t b0 = (int32_t)P.bwd.size();
-                P.bwd_itemptr[seg + (k - l0)] = b0;
-                if (!factored(c)) continue;  // the Schur part has no solve items
-                const int64_t K = width(c), st = L.cblk_stride[c];
-                const int64_t brows = (lv_wide ? BWD_ROWS_WIDE : BWD_ROWS);
-                for (int64_t r0 = K; r0 < st; r0 += brows) {
-                    int64_t r1 = std::min<int64_t>(st, r0 + brows);
-                    P.bwd.push_back(BwdItem{(int32_t)c, (int32_t)r0, (int32_t)r1, (int32_t)bslot});
-                    bslot += K;
-                }
-                for (int64_t ib = 0; ib * 64 < K; ib++)
-                    P.diag.push_back(DiagItem{(int32_t)c, (int32_t)ib, b0, (int32_t)P.bwd.size()});
+        // solve levels over the nodes (sources: factored cblks; every source
+        // of a node precedes the node's first cblk)
+        std::vector<int32_t> nlev(nn, 0);
+        for (int64_t c = 0; c < sc; c++) {
+            const int32_t nd = P.cblk_node[c];
+            for (int64_t b = s.cblk_bloknum[c] + 1; b < s.cblk_bloknum[c + 1]; b++) {
+                const int32_t t = P.cblk_node[s.blok_fcblk[b]];
+                if (t != nd) nlev[t] = std::max(nlev[t], nlev[nd] + 1);
             }
-            P.lv_diag_ptr[lv + 1] = (int32_t)P.diag.size();
-            P.bwd_itemptr[seg + (l1 - l0)] = (int32_t)P.bwd.size();
-            P.bwd_part = std::max(P.bwd_part, bslot);
-            P.lv_bwd_ptr[lv + 1] = (int32_t)P.bwd.size();
+        }
+        int32_t nsl = 0;
+        for (int32_t nd = 0; nd < nn; nd++) nsl = std::max(nsl, nlev[nd] + 1);
+        P.nslevels = nsl;
+        std::vector<int32_t> lptr(nsl + 1, 0), lnodes(nn);
+        for (int32_t nd = 0; nd < nn; nd++) lptr[nlev[nd] + 1]++;
+        for (int32_t l = 0; l < nsl; l++) lptr[l + 1] += lptr[l];
+        {
+            std::vector<int32_t> pos(lptr.begin(), lptr.end() - 1);
+            for (int32_t nd = 0; nd < nn; nd++) lnodes[pos[nlev[nd]]++] = nd;
+        }
+        // update buffer: one entry per off-diagonal panel row outside the
+        // cblk's node, sliced by 64 permuted rows (slice = the rows' j-th
+        // entries side by side, padded to the slice's longest row)
+        std::vector<int64_t> yoff(cb + 1, 0);
+        for (int64_t c = 0; c < cb; c++) yoff[c + 1] = yoff[c] + (L.cblk_stride[c] - width(c));
+        P.ycnt.assign(s.n, 0);
+        for (int64_t c = 0; c < cb; c++)
+            for (int64_t r = width(c) + P.cblk_intra[c]; r < L.cblk_stride[c]; r++)
+                P.ycnt[P.panel_grow[P.rowbase[c] + r]]++;
+        const int64_t nblk = (s.n + 63) / 64;
+        P.ybase.assign(nblk, 0);
+        int64_t ytot = 0;
+        for (int64_t k = 0; k < nblk; k++) {
+            int32_t mx = 0;
+            for (int64_t g = 64 * k; g < std::min<int64_t>(s.n, 64 * k + 64); g++) mx = std::max(mx, P.ycnt[g]);
+            P.ybase[k] = ytot;
+            ytot += 64 * (int64_t)mx;
+        }
+        P.nyrows = ytot;
+        if (ytot > (int64_t)INT32_MAX) throw std::runtime_error("update buffer exceeds 32-bit entries");
+        P.ydst.assign(yoff[cb], 0);
+        {
+            std::vector<int32_t> pos(s.n, 0);
+            for (int64_t c = 0; c < cb; c++)
+                for (int64_t r = width(c) + P.cblk_intra[c]; r < L.cblk_stride[c]; r++) {
+                    const int64_t g = P.panel_grow[P.rowbase[c] + r];
+                    P.ydst[yoff[c] + r - width(c)] = (int32_t)(P.ybase[g >> 6] + 64 * (int64_t)pos[g]++ + (g & 63));
+                }
+        }
+        const int32_t FWD_ROWS = 64;  // panel rows of one forward product tile
+        // panel rows of one backward product strip: 64 on the levels of
+        // narrow cblks, BWD_ROWS_WIDE where a cblk is wider than 64 columns
+        // (its diagonal step then sums a quarter of the partials, while the
+        // level still has a few hundred strips)
+        const int32_t BWD_ROWS = 64, BWD_ROWS_WIDE = 256;
+        P.slv_fwd_ptr.assign(nsl + 1, 0);
+        P.slv_bwd_ptr.assign(nsl + 1, 0);
+        P.slv_diag_ptr.assign(nsl + 1, 0);
+        P.slv_maxw.assign(nsl, 1);
+        P.slv_maxcw.assign(nsl, 1);
+        P.cblk_btile.assign(2 * cb, 0);
+        int64_t bslot = 0;  // backward partial slots (numbered over all levels)
+        for (int32_t lv = 0; lv < nsl; lv++) {
+            bool lv_wide = false;
+            for (int32_t k = lptr[lv]; k < lptr[lv + 1]; k++) {
+                const int32_t nd = lnodes[k];
+                P.slv_maxw[lv] = std::max<int32_t>(P.slv_maxw[lv], (int32_t)nwidth(nd));
+                for (int32_t c = P.node_cblk[nd]; c < P.node_cblk[nd + 1]; c++) {
+                    P.slv_maxcw[lv] = std::max<int32_t>(P.slv_maxcw[lv], (int32_t)width(c));
+                    lv_wide = lv_wide || width(c) > 64;
+                }
+            }
+            for (int32_t k = lptr[lv]; k < lptr[lv + 1]; k++) {
+                const int32_t nd = lnodes[k];
+                if (!nfact(nd)) continue;  // the Schur part has no solve items
+                for (int32_t c = P.node_cblk[nd]; c < P.node_cblk[nd + 1]; c++) {
+                    const int64_t K = width(c), st = L.cblk_stride[c];
+                    for (int64_t r0 = K + P.cblk_intra[c]; r0 < st; r0 += FWD_ROWS) {
+                        const int64_t r1 = std::min<int64_t>(st, r0 + FWD_ROWS);
+                        P.fwd.push_back(FwdItem{c, (int32_t)r0, (int32_t)r1, (int32_t)(yoff[c] + r0 - K)});
+                    }
+                }
+            }
+            P.slv_fwd_ptr[lv + 1] = (int32_t)P.fwd.size();
+            const int64_t brows = lv_wide ? BWD_ROWS_WIDE : BWD_ROWS;
+            for (int32_t k = lptr[lv]; k < lptr[lv + 1]; k++) {
+                const int32_t nd = lnodes[k];
+                if (!nfact(nd)) continue;
+                for (int32_t c = P.node_cblk[nd]; c < P.node_cblk[nd + 1]; c++) {
+                    const int64_t K = width(c), st = L.cblk_stride[c];
+                    P.cblk_btile[2 * c] = (int32_t)P.bwd.size();
+                    for (int64_t r0 = K + P.cblk_intra[c]; r0 < st; r0 += brows) {
+                        const int64_t r1 = std::min<int64_t>(st, r0 + brows);
+                        P.bwd.push_back(BwdItem{c, (int32_t)r0, (int32_t)r1, (int32_t)bslot});
+                        bslot += K;
+                    }
+                    P.cblk_btile[2 * c + 1] = (int32_t)P.bwd.size();
+                }
+                for (int64_t ib = 0; ib * 64 < nwidth(nd); ib++) P.diag.push_back(DiagItem{nd, (int32_t)ib, 0, 0});
+            }
+            P.slv_diag_ptr[lv + 1] = (int32_t)P.diag.size();
+            P.slv_bwd_ptr[lv + 1] = (int32_t)P.bwd.size();
+        }
+        P.bwd_part = bslot;
+        if (bslot > (int64_t)INT32_MAX) throw std::runtime_error("backward partial slots exceed 32 bits");
+
+        // merge rounds of the nodes' inverses: leaves are the members'
+        // 64-column blocks; each round merges adjacent pairs of runs.  The
+        // nodes are batched by the factorisation level of their last cblk
+        // (after which their panels and leaf inverses are final): a batch
+        // gathers the nodes of consecutive levels until it holds
+        // WINV_BATCH_TILES tiles (fewer, larger launches) or a node of
+        // several cblks (whose merges are long: they start as soon as the
+        // node is complete), rounds numbered within each batch.
+        const size_t WINV_BATCH_TILES = 1024;
+        std::vector<std::vector<int32_t>> by_level(nl);
+        for (int32_t nd = 0; nd < nn; nd++)
+            if (nfact(nd)) by_level[L.cblk_level[P.node_cblk[nd + 1] - 1]].push_back(nd);
+        P.winv_round_ptr.assign(1, 0);
+        P.winv_batch_leaf.assign(1, 0);
+        P.winv_batch_round.assign(1, 0);
+        std::vector<std::vector<WinvTile>> rounds;
+        std::vector<int64_t> rsize;
+        size_t btiles = 0, leaves0 = 0;
+        for (int32_t lv = 0; lv < nl; lv++) {
+            bool bgroup = false;
+            for (const int32_t nd : by_level[lv]) {
+                const bool group = P.node_cblk[nd + 1] - P.node_cblk[nd] > 1;
+                bgroup = bgroup || group;
+                std::vector<std::pair<int32_t, int32_t>> runs;
+                for (int32_t c = P.node_cblk[nd]; c < P.node_cblk[nd + 1]; c++) {
+                    const int32_t o = (int32_t)(s.cblk_fcol[c] - nfcol(nd)), w = (int32_t)width(c);
+                    for (int32_t ib = 0; ib * 64 < w; ib++) {
+                        runs.push_back({o + 64 * ib, o + std::min(w, 64 * ib + 64)});
+                        if (group) P.winv_leaves.push_back(WinvLeaf{nd, c, ib, 0});
+                    }
+                }
+                for (size_t r = 0; runs.size() > 1; r++) {
+                    if (rounds.size() <= r) {
+                        rounds.emplace_back();
+                        rsize.push_back(0);
+                    }
+                    std::vector<std::pair<int32_t, int32_t>> next;
+                    for (size_t i = 0; i < runs.size(); i += 2) {
+                        if (i + 1 == runs.size()) {
+                            next.push_back(runs[i]);
+                            break;
+                        }
+                        const int32_t a = runs[i].first, m = runs[i + 1].first, b = runs[i + 1].second;
+                        for (int32_t i0 = 0; i0 < b - m; i0 += 64)
+                            for (int32_t j0 = 0; j0 < m - a; j0 += 64) {
+                                rounds[r].push_back(WinvTile{nd, a, m, b, i0, j0, rsize[r]});
+                                btiles++;
+                            }
+                        rsize[r] += (int64_t)(b - m) * (m - a);
+                        P.flops_winv += (double)(b - m) * (m - a) * ((m - a) + (b - m));
+                        next.push_back({a, b});
+                    }
+                    runs.swap(next);
+                }
+            }
+            if (rounds.empty() && P.winv_leaves.size() == leaves0) continue;
+            if (btiles < WINV_BATCH_TILES && !bgroup && lv + 1 < nl) continue;
+            for (size_t r = 0; r < rounds.size(); r++) {
+                P.winv_tiles.insert(P.winv_tiles.end(), rounds[r].begin(), rounds[r].end());
+                P.winv_round_ptr.push_back((int32_t)P.winv_tiles.size());
+                P.winv_tmax = std::max(P.winv_tmax, rsize[r]);
+            }
+            P.winv_batch_lv.push_back(lv);
+            P.winv_batch_leaf.push_back((int32_t)P.winv_leaves.size());
+            P.winv_batch_round.push_back((int32_t)P.winv_round_ptr.size() - 1);
+            rounds.clear();
+            rsize.clear();
+            btiles = 0;
+            leaves0 = P.winv_leaves.size();
         }
     }
     // leaf region task lists

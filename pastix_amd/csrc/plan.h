@@ -17,6 +17,8 @@ constexpr double SINGLE_PASS_ENTS = 2.05;
 constexpr int64_t PANEL_RIGHT_ROWTILES = 200;
 // widest cblk of the leaf region (one diagonal sub-block per cblk)
 constexpr int64_t LEAF_MAX_WIDTH = 64;
+// widest solve node by default (columns of a merged run of cblks)
+constexpr int32_t SOLVE_NODE_MAX = 1024;
 
 struct Plan {
     int32_t nlevels = 0;
@@ -66,23 +68,59 @@ struct Plan {
     // tiles, lv_wsize 0): those averaging at most SINGLE_PASS_ENTS entries
     std::vector<uint8_t> lv_single_pass;
     int64_t wmax = 0;
-    // forward products (sources of level L) into the update buffer, whose
-    // entries are grouped by permuted row: row g owns [yrow_ptr[g], yrow_ptr[g+1])
-    std::vector<int32_t> lv_fwd_ptr;
+    // ---- solve ----
+    // Solve nodes: runs of consecutive factored cblks solved as one diagonal
+    // block.  A run is a chain: each member's parent is the next member and
+    // each member's first off-diagonal panel rows are exactly the columns of
+    // the members after it, so the run's diagonal block is dense lower
+    // triangular; the solve applies its whole inverse at once (one level
+    // instead of one per member).  A node of one cblk is the cblk itself.
+    std::vector<int32_t> node_cblk;      // nnodes + 1: first cblk of every node
+    std::vector<int32_t> cblk_node;      // cblknbr
+    std::vector<int64_t> node_winv;      // nnodes: offset of the node's K x K inverse in the W arrays
+    std::vector<int32_t> cblk_intra;     // cblknbr: off-diagonal panel rows of a cblk inside its node
+    std::vector<int32_t> cblk_btile;     // 2 cblknbr: backward strips [tb, te) of every cblk
+    // solve levels: a node's level is one more than the highest level of
+    // the nodes that update it
+    int32_t nslevels = 0;
+    std::vector<int32_t> slv_maxw;       // widest node of each solve level (diagonal kernels)
+    std::vector<int32_t> slv_maxcw;      // widest cblk of each solve level (product kernels)
+    // forward products of the nodes of solve level L (their members' panel
+    // rows outside the node) into the update buffer.  Its entries are sliced
+    // by 64 permuted rows: entry j of row g (its j-th source in increasing
+    // order) sits at ybase[g / 64] + 64 j + g % 64, so that consecutive rows'
+    // j-th entries are contiguous; row g has ycnt[g] entries.
+    std::vector<int32_t> slv_fwd_ptr;
     std::vector<FwdItem> fwd;
-    int64_t nyrows = 0;                  // update-buffer rows = all off-diagonal panel rows
-    std::vector<int64_t> yrow_ptr;       // n+1
-    std::vector<int32_t> ydst;           // nyrows: update-buffer entry of every off-diagonal
-                                         // panel row (entries grouped by permuted row, sources
-                                         // in increasing order within a row)
-    // backward gather items
-    std::vector<int32_t> lv_bwd_ptr;
+    int64_t nyrows = 0;                  // update-buffer entries (slices padded to their longest row)
+    std::vector<int64_t> ybase;          // (n + 63) / 64
+    std::vector<int32_t> ycnt;           // n
+    std::vector<int32_t> ydst;           // update-buffer entry of every off-diagonal panel row
+                                         // outside its node, at yoff[c] + r - width(c) (rows
+                                         // inside the node have none)
+    // backward strips of solve level L
+    std::vector<int32_t> slv_bwd_ptr;
     std::vector<BwdItem> bwd;
-    std::vector<int32_t> bwd_itemptr;
-    // diagonal solve steps: every 64-row block of the level's cblks
-    std::vector<int32_t> lv_diag_ptr;
+    // diagonal solve steps: every 64-row block of the level's nodes
+    std::vector<int32_t> slv_diag_ptr;
     std::vector<DiagItem> diag;
     int64_t bwd_part = 0;  // partial buffer size per right-hand side (slots of all levels)
+    // Inverses of the nodes' diagonal blocks, by recursive doubling over
+    // 64-column leaves whose inverses the diagonal-block kernels leave in
+    // place: round r merges pairs of adjacent runs [a, m), [m, b) whose
+    // inverses are known, tiles [winv_round_ptr[r], winv_round_ptr[r+1])
+    // (each tile in both phases: T = M21 X11, then X21 = -X22 T).  Batch k
+    // holds the nodes whose last cblk is factored at level winv_batch_lv[k]:
+    // leaves [winv_batch_leaf[k], winv_batch_leaf[k+1]) and rounds
+    // [winv_batch_round[k], winv_batch_round[k+1]).
+    std::vector<WinvTile> winv_tiles;
+    std::vector<int32_t> winv_round_ptr;
+    std::vector<int32_t> winv_batch_lv, winv_batch_leaf, winv_batch_round;
+    int64_t winv_tmax = 0;               // T scratch entries of the largest round
+    // leaves of the nodes of several cblks (their inverses are copied from
+    // the members' into the node's)
+    std::vector<WinvLeaf> winv_leaves;
+    double flops_winv = 0;               // products of the merge rounds (one side)
     // panel row maps and inverse offsets
     std::vector<int32_t> panel_grow, panel_blk;
     std::vector<int64_t> rowbase, winv_off;
@@ -137,6 +175,9 @@ struct Plan {
 // panel_right 1 updates the level's panels right-looking, 0 left-looking.
 struct PlanKnobs {
     std::vector<int8_t> single_pass, panel_right;
+    // widest solve node (columns): -1 the default SOLVE_NODE_MAX, 0 or 1
+    // every cblk its own node
+    int32_t node_max = -1;
     int8_t sp(int64_t lv) const { return lv < (int64_t)single_pass.size() ? single_pass[lv] : -1; }
     int8_t pr(int64_t lv) const { return lv < (int64_t)panel_right.size() ? panel_right[lv] : -1; }
 };

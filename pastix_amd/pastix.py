@@ -95,7 +95,7 @@ class error:
 
 class amd_iparm:
     factor_precision, device, level_order, resident_input, profile, graph, overlap = 0, 1, 2, 3, 4, 5, 6
-    leaf_tasks, leaf_grid = 7, 8
+    leaf_tasks, leaf_grid, solve_node_max = 7, 8, 9
 
 
 class PastixError(RuntimeError):
@@ -409,6 +409,8 @@ lib.pastix_amd_save_tuning.argtypes = [_P, ct.c_char_p]
 lib.pastix_amd_save_tuning.restype = ct.c_int
 lib.pastix_amd_level_info.argtypes = [_P, ct.c_void_p, ct.c_int]
 lib.pastix_amd_level_info.restype = ct.c_int
+lib.pastix_amd_solve_info.argtypes = [_P, ct.c_void_p]
+lib.pastix_amd_solve_info.restype = ct.c_int
 lib.pastix_amd_level_times.argtypes = [_P, ct.c_void_p, ct.c_int]
 lib.pastix_amd_level_times.restype = ct.c_int
 
@@ -443,6 +445,17 @@ def level_info(pastix_data):
     if nl < 0:
         raise PastixError("pastix_amd_level_info", nl)
     return out[:nl].copy()
+
+
+def solve_info(pastix_data):
+    """Solve structure: nodes, nodes of several cblks, solve levels, merge
+    rounds of the nodes' inverses, their tiles, entries of the merged nodes'
+    inverses, merge flops (one side), factorisation levels."""
+    out = np.zeros(8)
+    _check("pastix_amd_solve_info", lib.pastix_amd_solve_info(pastix_data.ptr, out.ctypes.data))
+    keys = ["nodes", "merged_nodes", "solve_levels", "merge_rounds", "merge_tiles", "merged_entries",
+            "merge_flops", "levels"]
+    return {k: (float(v) if k == "merge_flops" else int(v)) for k, v in zip(keys, out)}
 
 
 def level_times(pastix_data):

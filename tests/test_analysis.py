@@ -540,3 +540,119 @@ def test_ordergen_golden_file(tmp_path, monkeypatch):
     assert [o.peritab[i] for i in range(4)] == [3, 1, 4, 2]
     assert [o.treetab[i] for i in range(2)] == [2, -1]
     pastix.orderExit(o)
+
+
+def _solve_nodes_from_symbol(sym, nmax):
+    """Solve nodes restated from the symbol: runs of consecutive cblks, each
+    the parent of the one before (its first off-diagonal block faces the
+    next), every member's off-diagonal rows starting with exactly the columns
+    of the members after it, at most nmax columns; then the solve levels over
+    the nodes and the merge rounds of their inverses (64-column leaves,
+    adjacent pairs merged per round), summed over the batches they are
+    issued in."""
+    fc, bn = sym["cblk_fcol"], sym["cblk_bloknum"]
+    fr, lr, fb = sym["blok_frow"], sym["blok_lrow"], sym["blok_fcblk"]
+    cb = sym["cblknbr"]
+    w = np.diff(fc)
+
+    def offrows(c):  # off-diagonal rows of c in block order
+        return np.concatenate([np.arange(fr[b], lr[b] + 1) for b in range(bn[c] + 1, bn[c + 1])] or [[]])
+    rows = [offrows(c) for c in range(cb)]
+    nodes, c = [], 0
+    while c < cb:
+        e, K = c + 1, w[c]
+        while nmax > 1 and e < cb:
+            if bn[e] == bn[e - 1] + 1 or fb[bn[e - 1] + 1] != e or K + w[e] > nmax:
+                break
+            ok = all(len(rows[k]) >= fc[e + 1] - fc[k + 1] and
+                     np.array_equal(rows[k][:fc[e + 1] - fc[k + 1]], np.arange(fc[k + 1], fc[e + 1]))
+                     for k in range(c, e))
+            if not ok:
+                break
+            K += w[e]
+            e += 1
+        nodes.append((c, e))
+        c = e
+    node_of = np.zeros(cb, int)
+    for k, (a, b) in enumerate(nodes):
+        node_of[a:b] = k
+    lev = np.zeros(len(nodes), int)
+    for c in range(cb):
+        for b in range(bn[c] + 1, bn[c + 1]):
+            t = node_of[fb[b]]
+            if t != node_of[c]:
+                lev[t] = max(lev[t], lev[node_of[c]] + 1)
+    # factorisation level of every cblk; a node is complete with its last cblk
+    clev = np.zeros(cb, int)
+    for c in range(cb):
+        for b in range(bn[c] + 1, bn[c + 1]):
+            clev[fb[b]] = max(clev[fb[b]], clev[c] + 1)
+    nrounds, tiles, flops, extra = {}, 0, 0.0, 0
+    ntiles = {}
+    for a, b in nodes:
+        runs = []
+        for c in range(a, b):
+            o = fc[c] - fc[a]
+            runs += [(o + s, o + min(w[c], s + 64)) for s in range(0, w[c], 64)]
+        r, nt = 0, 0
+        while len(runs) > 1:
+            nxt = []
+            for i in range(0, len(runs), 2):
+                if i + 1 == len(runs):
+                    nxt.append(runs[i])
+                    break
+                x0, m, x1 = runs[i][0], runs[i + 1][0], runs[i + 1][1]
+                nt += -(-(x1 - m) // 64) * -(-(m - x0) // 64)
+                flops += (x1 - m) * (m - x0) * ((m - x0) + (x1 - m))
+                nxt.append((x0, x1))
+            runs, r = nxt, r + 1
+        nrounds[(a, b)], ntiles[(a, b)] = r, nt
+        tiles += nt
+        if b - a > 1:
+            extra += (fc[b] - fc[a]) ** 2
+    # batches: nodes by the level of their last cblk, consecutive levels
+    # gathered until 1024 tiles, a node of several cblks or the last level;
+    # the rounds of a batch are those of its deepest node
+    rounds, brounds, btiles, bleaves, bgroup = 0, 0, 0, 0, False
+    nlev = int(clev.max()) + 1
+    for lv in range(nlev):
+        bgroup = False
+        for a, b in nodes:
+            if clev[b - 1] != lv:
+                continue
+            brounds, btiles = max(brounds, nrounds[(a, b)]), btiles + ntiles[(a, b)]
+            if b - a > 1:
+                bgroup = True
+                bleaves += sum(-(-w[c] // 64) for c in range(a, b))
+        if brounds == 0 and bleaves == 0:
+            continue
+        if btiles < 1024 and not bgroup and lv + 1 < nlev:
+            continue
+        rounds += brounds
+        brounds, btiles, bleaves = 0, 0, 0
+    return nodes, int(lev.max()) + 1, rounds, tiles, flops, extra
+
+
+@pytest.mark.parametrize("grid,nmax", [((20, 20, 20), -1), ((30, 30, 30), -1), ((30, 30, 30), 400),
+                                       ((30, 30, 30), 0), ((14, 9, 11), -1)])
+def test_solve_nodes_follow_their_rule(grid, nmax):
+    A = spm.laplacian7(*grid)
+    h, S, sym = analyse(A, True, pastix.factotype.LLT, grid=grid)
+    pastix.set_amd_iparm(h, pastix.amd_iparm.solve_node_max, nmax)
+    info = pastix.solve_info(h)
+    sym = pastix.symbol(h)
+    nodes, nsl, rounds, tiles, flops, extra = _solve_nodes_from_symbol(sym, 1024 if nmax < 0 else nmax)
+    print(grid, nmax, info)
+    assert info["nodes"] == len(nodes)
+    assert info["merged_nodes"] == sum(1 for a, b in nodes if b - a > 1)
+    assert info["solve_levels"] == nsl
+    assert info["merge_rounds"] == rounds
+    assert info["merge_tiles"] == tiles
+    assert info["merge_flops"] == flops
+    assert info["merged_entries"] == extra
+    assert info["levels"] == sym["nlevels"]
+    if nmax == 0:
+        assert info["nodes"] == sym["cblknbr"] and info["solve_levels"] == info["levels"]
+    elif nmax < 0 and grid[0] >= 20:
+        assert info["merged_nodes"] >= 1 and info["solve_levels"] < info["levels"]
+    pastix.finalize(h)
