@@ -55,9 +55,12 @@ void launch_winv_nodes(const DevSymbol &S, const WinvLeaf *leaves, int nleaves, 
 
 // Supernodal triangular solves (reference kernels/solve_ztrsmsp.c:332-557).
 // nrhs <= 8 per launch; the forward sweep writes z, the backward sweep x.
+// On levels of wide nodes the forward diagonal step leaves b minus the
+// pending products in b's node rows, the backward one z minus the
+// partials in z.
 template <typename T>
 void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const YSlices &ys, const T *ybuf,
-                     int64_t ldy, const T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
+                     int64_t ldy, T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
                      hipStream_t st);
 template <typename T>
 void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *z, int64_t ldz,
@@ -66,7 +69,7 @@ template <typename T>
 void launch_bwd_prod(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
                      int nrhs, T *part, int maxwidth, hipStream_t st);
 template <typename T>
-void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const BwdItem *bitems, const T *z,
+void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const BwdItem *bitems, T *z,
                      int64_t ldz, T *x, int64_t ldx, int nrhs, const T *part, const T *wb, int maxwidth,
                      hipStream_t st);
 

@@ -40,6 +40,9 @@ static inline int grid8(int n) { return ((n + 7) / 8) * 8; }
 constexpr int SOLVE_NARROW = 64;
 // and one-wave workgroups at most this many (the leaf levels)
 constexpr int SOLVE_TINY = 16;
+// levels whose widest node has more columns gather once per 64-row block
+// before the diagonal kernels (k_fwd_gather, k_bwd_reduce)
+constexpr int SOLVE_SPLIT = 512;
 
 // Loads and stores of the data one solve task hands to another (update
 // buffer, forward result, solution rows, backward partials).  CO: device-
@@ -113,7 +116,7 @@ __device__ __forceinline__ void wave_sum_store_rows(T (&acc)[E][NR], T *red, int
 // Task bodies, each run by one whole workgroup of NW waves (the level
 // kernels below and the dependency-driven solve share them).
 // ---------------------------------------------------------------------------
-template <bool CO, typename T, int NR, int NW>
+template <bool CO, typename T, int NR, int NW, bool PRE = false>
 __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem it,
                                               const YSlices ys, const T *__restrict__ ybuf,
                                               int64_t ldy, const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
@@ -148,6 +151,13 @@ __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem
         if (mrow && k < nrow) ld_rows(M + (int64_t)k * Kc, rr, Kc - r0, mpre[f]);
     }
     constexpr int UL = 16;
+    if constexpr (PRE) {
+        // the rows' pending products were subtracted in b already (k_fwd_gather)
+        for (int t = tid; t < tasks; t += NW * 64) {
+            const int i = t % nrow, u = t / nrow;
+            y[i + u * Kc] = b[fc + i + u * ldb];
+        }
+    } else
     for (int base = 0; base < tasks * gl; base += NW * 64) {
         const int t = base + tid, task = t / gl, j = t % gl;
         const int i = task % nrow, u = task / nrow;
@@ -313,7 +323,7 @@ __device__ __forceinline__ void bwd_prod_body(const DevSymbol &S, const BwdItem 
 
 // nrhs right-hand sides starting at column u0 of the partials, which are laid
 // out for nrhs_tot columns (z and x point at column u0 already)
-template <bool CO, typename T, int NR, int NW>
+template <bool CO, typename T, int NR, int NW, bool PRE = false>
 __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem it,
                                               const BwdItem *__restrict__ bitems, const T *__restrict__ z,
                                               int64_t ldz, T *__restrict__ x, int64_t ldx, int nrhs, int nrhs_tot,
@@ -343,9 +353,17 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
         const int64_t s0 = (te > tb) ? (int64_t)bitems[tb].slot : 0;
         return part + s0 * nrhs_tot + (int64_t)u0 * w + (k - o);
     };
+    if constexpr (PRE) {
+        // the partials were subtracted from z already (k_bwd_reduce)
+        for (int t = tid; t < (Kc - r0) * nrhs; t += NW * 64) {
+            const int k = r0 + t % (Kc - r0), u = t / (Kc - r0);
+            y[k + u * Kc] = z[fc + k + u * ldz];
+        }
+        __syncthreads();
+    }
     // y = z_c - the tiles' partials for columns [r0, Kc).  Several
     // right-hand sides: 64 columns at a time, the waves splitting the tiles.
-    if constexpr (NR > 1) {
+    if constexpr (NR > 1 && !PRE) {
         for (int kc = r0; kc < Kc; kc += 64) {
             const int k = kc + r;
             T acc[NR];
@@ -372,7 +390,7 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
     // One right-hand side: all columns at once (cpp per pass), each summed
     // over every L-th tile by L threads, whose sums are then added in thread
     // order; one barrier pair per pass instead of one per 64 columns.
-    const int ncol = (NR > 1) ? 0 : Kc - r0;
+    const int ncol = (NR > 1 || PRE) ? 0 : Kc - r0;
     const int cpp = max(1, min(ncol, NW * 64)), L = (NW * 64) / cpp;
     for (int c0 = 0; c0 < ncol; c0 += cpp) {
         const int cc = tid % cpp, h = tid / cpp;
@@ -442,7 +460,7 @@ __device__ __forceinline__ void bwd_diag_body(const DevSymbol &S, const DiagItem
 }
 
 // ------------------------------ level kernels -------------------------------
-template <typename T, int NR, int NW>
+template <typename T, int NR, int NW, bool PRE = false>
 __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
                                                       const YSlices ys, const T *__restrict__ ybuf, int64_t ldy,
                                                       const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
@@ -453,7 +471,7 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
     if (id >= nitems) return;
     // right-hand sides [NR*blockIdx.y, NR*blockIdx.y + NR)
     const int u0 = NR * blockIdx.y;
-    fwd_diag_body<false, T, NR, NW>(S, items[id], ys, ybuf + u0 * ldy, ldy, b + u0 * ldb, ldb, z + u0 * ldz,
+    fwd_diag_body<false, T, NR, NW, PRE>(S, items[id], ys, ybuf + u0 * ldy, ldy, b + u0 * ldb, ldb, z + u0 * ldz,
                                     ldz, min(NR, nrhs - u0), w1, reinterpret_cast<T *>(smem_raw), red);
 }
 
@@ -490,7 +508,47 @@ __global__ __launch_bounds__(256) void k_bwd_prod(DevSymbol S, const BwdItem *__
                                 red);
 }
 
-template <typename T, int NR, int NW>
+// Backward strips of narrow cblks (at most C columns), one right-hand side:
+// four strips per wave, 16 lanes each; lane l sums rows l, l + 16, ... of
+// every column in registers, the 16 lanes combine by xor shuffles, lane j
+// stores column j.
+template <typename T, int C>
+__global__ __launch_bounds__(64) void k_bwd_prod_narrow(DevSymbol S, const BwdItem *__restrict__ items, int nitems,
+                                                        const T *__restrict__ coef, const T *__restrict__ x,
+                                                        T *__restrict__ part) {
+    const int lane = threadIdx.x, grp = lane >> 4, l = lane & 15;
+    const int id = xcd_remap(blockIdx.x, gridDim.x) * 4 + grp;
+    BwdItem it{0, 0, 0, 0};
+    int Kc = 0;
+    int64_t ld = 0, rb = 0;
+    const T *P = coef;
+    if (id < nitems) {
+        it = items[id];
+        Kc = S.cblk_fcol[it.c + 1] - S.cblk_fcol[it.c];
+        ld = S.cblk_stride[it.c];
+        rb = S.cblk_rowbase[it.c];
+        P = coef + S.cblk_coef[it.c];
+    }
+    T acc[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) acc[k] = 0;
+    for (int r = it.r0 + l; r < it.r1; r += 16) {
+        const T xr = x[S.panel_grow[rb + r]];
+#pragma unroll
+        for (int k = 0; k < C; k++)
+            if (k < Kc) acc[k] += P[r + (int64_t)k * ld] * xr;
+    }
+#pragma unroll
+    for (int off = 1; off < 16; off *= 2)
+#pragma unroll
+        for (int k = 0; k < C; k++) acc[k] += __shfl_xor(acc[k], off);
+    // lane j of the group stores columns j, j + 16, ...
+#pragma unroll
+    for (int k = 0; k < C; k++)
+        if ((k & 15) == l && k < Kc) part[(int64_t)it.slot + k] = acc[k];
+}
+
+template <typename T, int NR, int NW, bool PRE = false>
 __global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
                                                       const BwdItem *__restrict__ bitems, const T *__restrict__ z,
                                                       int64_t ldz, T *__restrict__ x, int64_t ldx, int nrhs,
@@ -501,17 +559,189 @@ __global__ __launch_bounds__(NW * 64) void k_bwd_diag(DevSymbol S, const DiagIte
     if (id >= nitems) return;
     // right-hand sides [NR*blockIdx.y, NR*blockIdx.y + NR)
     const int u0 = NR * blockIdx.y;
-    bwd_diag_body<false, T, NR, NW>(S, items[id], bitems, z + u0 * ldz, ldz, x + u0 * ldx, ldx, min(NR, nrhs - u0),
+    bwd_diag_body<false, T, NR, NW, PRE>(S, items[id], bitems, z + u0 * ldz, ldz, x + u0 * ldx, ldx, min(NR, nrhs - u0),
                                     nrhs, u0, part, wb, reinterpret_cast<T *>(smem_raw), red);
 }
 
 
+// Levels of wide nodes (more than SOLVE_SPLIT columns): the gathers run once
+// per 64-row block in a launch of their own instead of in every diagonal
+// workgroup that needs the rows (workgroup ib needs rows [0, 64 ib + 64)).
+// Forward: b minus the rows' update-buffer entries, in place in b; wave q
+// sums the entries q, q + 4, ... of the block's rows (lane = row, so the
+// sliced buffer's loads are contiguous), the waves combined in order.  One
+// right-hand side per grid row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_gather(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
+                                                    const YSlices ys, const T *__restrict__ ybuf, int64_t ldy,
+                                                    T *__restrict__ b, int64_t ldb) {
+    __shared__ T red[4][64];
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const int u = blockIdx.y;
+    const DiagItem it = items[id];
+    const int fc = S.cblk_fcol[S.node_cblk[it.nd]];
+    const int K = S.cblk_fcol[S.node_cblk[it.nd + 1]] - fc;
+    const int i = it.ib * 64 + lane;
+    constexpr int UL = 8;
+    T v = 0;
+    if (i < K) {
+        const int g = fc + i, ne = ys.cnt[g];
+        const T *yb = ybuf + u * ldy + ys.base[g >> 6] + (g & 63);
+        for (int e = q; e < ne; e += 4 * UL) {
+            T a[UL];
+#pragma unroll
+            for (int k = 0; k < UL; k++) a[k] = (e + 4 * k < ne) ? yb[(int64_t)(e + 4 * k) * 64] : (T)0;
+#pragma unroll
+            for (int k = 0; k < UL; k++) v += a[k];
+        }
+    }
+    red[q][lane] = v;
+    __syncthreads();
+    if (q == 0 && i < K) b[fc + i + u * ldb] -= ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+// Backward: z minus the strips' partials of the block's columns, in place in
+// z; wave q sums the strips q, q + 4, ... of each column's member (lane =
+// column), the waves combined in order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_reduce(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
+                                                    const BwdItem *__restrict__ bitems, T *__restrict__ z,
+                                                    int64_t ldz, const T *__restrict__ part, int nrhs_tot) {
+    __shared__ T red[4][64];
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const int u = blockIdx.y;
+    const DiagItem it = items[id];
+    const int cb0 = S.node_cblk[it.nd], cb1 = S.node_cblk[it.nd + 1];
+    const int fc = S.cblk_fcol[cb0];
+    const int K = S.cblk_fcol[cb1] - fc;
+    const int k = it.ib * 64 + lane;
+    T v = 0;
+    if (k < K) {
+        int lo = cb0, hi = cb1 - 1;  // the last member whose first column is <= k
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.cblk_fcol[mid] - fc <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int o = S.cblk_fcol[lo] - fc, w = S.cblk_fcol[lo + 1] - S.cblk_fcol[lo];
+        const int tb = S.cblk_btile[2 * lo], te = S.cblk_btile[2 * lo + 1];
+        if (te > tb) {
+            const int64_t stride = (int64_t)w * nrhs_tot;
+            const T *p0 = part + (int64_t)bitems[tb].slot * nrhs_tot + (int64_t)u * w + (k - o);
+            for (int j = q; j < te - tb; j += 4) v += p0[(int64_t)j * stride];
+        }
+    }
+    red[q][lane] = v;
+    __syncthreads();
+    if (q == 0 && k < K) z[fc + k + u * ldz] -= ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+// Levels of narrow nodes (at most SOLVE_TINY columns), one right-hand side:
+// four nodes per wave, 16 lanes each, lane i owning row (forward) or column
+// (backward) i of its node; the node's inverse is read into registers
+// before the gathers, the product takes y by lane shuffles.
+template <typename T>
+__global__ __launch_bounds__(64) void k_fwd_diag_tiny(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
+                                                      const YSlices ys, const T *__restrict__ ybuf,
+                                                      const T *__restrict__ b, T *__restrict__ z,
+                                                      const T *__restrict__ w1) {
+    const int lane = threadIdx.x, grp = lane >> 4, i = lane & 15;
+    const int id = xcd_remap(blockIdx.x, gridDim.x) * 4 + grp;
+    int fc = 0, K = 0;
+    int64_t wo = 0;
+    if (id < nitems) {
+        const int nd = items[id].nd;
+        fc = S.cblk_fcol[S.node_cblk[nd]];
+        K = S.cblk_fcol[S.node_cblk[nd + 1]] - fc;
+        wo = S.node_winv[nd];
+    }
+    const bool row = i < K;
+    T wv[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) wv[k] = (row && k < K) ? w1[wo + i + (int64_t)k * K] : (T)0;
+    T y = 0;
+    if (row) {
+        const int g = fc + i, ne = ys.cnt[g];
+        const T *yb = ybuf + ys.base[g >> 6] + (g & 63);
+        T v = 0;
+        for (int e = 0; e < ne; e += 8) {
+            T a[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) a[k] = (e + k < ne) ? yb[(int64_t)(e + k) * 64] : (T)0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) v += a[k];
+        }
+        y = b[g] - v;
+    }
+    T acc = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) acc += wv[k] * __shfl(y, (grp << 4) + k);
+    if (row) z[fc + i] = acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_bwd_diag_tiny(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
+                                                      const BwdItem *__restrict__ bitems, const T *__restrict__ z,
+                                                      T *__restrict__ x, const T *__restrict__ part,
+                                                      const T *__restrict__ wb) {
+    const int lane = threadIdx.x, grp = lane >> 4, k = lane & 15;
+    const int id = xcd_remap(blockIdx.x, gridDim.x) * 4 + grp;
+    int fc = 0, K = 0, cb0 = 0, cb1 = 0;
+    int64_t wo = 0;
+    if (id < nitems) {
+        const int nd = items[id].nd;
+        cb0 = S.node_cblk[nd];
+        cb1 = S.node_cblk[nd + 1];
+        fc = S.cblk_fcol[cb0];
+        K = S.cblk_fcol[cb1] - fc;
+        wo = S.node_winv[nd];
+    }
+    const bool col = k < K;
+    // row k of inv(L_nd)^T: WB[k + j*K], j >= k
+    T wv[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) wv[j] = (col && j < K) ? wb[wo + k + (int64_t)j * K] : (T)0;
+    T y = 0;
+    if (col) {
+        // member m of column k, its strips' partials (consecutive slots, w apart)
+        int m = cb0;
+        while (m + 1 < cb1 && S.cblk_fcol[m + 1] - fc <= k) m++;
+        const int o = S.cblk_fcol[m] - fc, w = S.cblk_fcol[m + 1] - S.cblk_fcol[m];
+        const int tb = S.cblk_btile[2 * m], te = S.cblk_btile[2 * m + 1];
+        T v = 0;
+        if (te > tb) {
+            const T *pp = part + (int64_t)bitems[tb].slot + (k - o);
+            for (int j = 0; j < te - tb; j++) v += pp[(int64_t)j * w];
+        }
+        y = z[fc + k] - v;
+    }
+    T acc = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc += wv[j] * __shfl(y, (grp << 4) + j);
+    if (col) x[fc + k] = acc;
+}
+
 // ------------------------------- launchers ---------------------------------
 template <typename T>
 void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const YSlices &ys, const T *ybuf,
-                     int64_t ldy, const T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
+                     int64_t ldy, T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
                      hipStream_t st) {
     if (n <= 0) return;
+    if (maxwidth > SOLVE_SPLIT) {
+        hipLaunchKernelGGL(k_fwd_gather<T>, dim3(grid8(n), nrhs), dim3(256), 0, st, S, items, n, ys, ybuf, ldy, b, ldb);
+        if (nrhs == 1)
+            hipLaunchKernelGGL((k_fwd_diag<T, 1, 16, true>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T),
+                               st, S, items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+        else
+            hipLaunchKernelGGL((k_fwd_diag<T, 2, 16, true>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
+                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, ys, ybuf, ldy, b, ldb, z, ldz,
+                               nrhs, w1);
+        return;
+    }
     // several right-hand sides: pairs of columns on the grid's y dimension,
     // so that the few diagonal blocks of the top levels use more of the chip;
     // levels of narrow cblks (maxwidth = the level's widest) take 4-wave
@@ -519,8 +749,8 @@ void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const YSl
     const bool narrow = maxwidth <= SOLVE_NARROW;
     if (nrhs == 1) {
         if (maxwidth <= SOLVE_TINY)
-            hipLaunchKernelGGL((k_fwd_diag<T, 1, 1>), dim3(grid8(n)), dim3(64), (size_t)maxwidth * sizeof(T), st, S,
-                               items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+            hipLaunchKernelGGL(k_fwd_diag_tiny<T>, dim3(grid8((n + 3) / 4)), dim3(64), 0, st, S, items, n, ys, ybuf, b,
+                               z, w1);
         else if (narrow)
             hipLaunchKernelGGL((k_fwd_diag<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
                                items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
@@ -559,6 +789,11 @@ template <typename T>
 void launch_bwd_prod(const DevSymbol &S, const BwdItem *items, int n, const T *coef, const T *x, int64_t ldx,
                      int nrhs, T *part, int maxwidth, hipStream_t st) {
     if (n <= 0) return;
+    if (nrhs == 1 && maxwidth <= SOLVE_TINY) {
+        hipLaunchKernelGGL((k_bwd_prod_narrow<T, SOLVE_TINY>), dim3(grid8((n + 3) / 4)), dim3(64), 0, st, S, items, n,
+                           coef, x, part);
+        return;
+    }
     const dim3 g(grid8(n), (maxwidth + 63) / 64, nrhs == 1 ? 1 : (nrhs + 7) / 8);
     if (nrhs == 1)
         hipLaunchKernelGGL((k_bwd_prod<T, 1>), g, dim3(256), 0, st, S, items, n, coef, x, ldx, nrhs, part);
@@ -566,15 +801,27 @@ void launch_bwd_prod(const DevSymbol &S, const BwdItem *items, int n, const T *c
         hipLaunchKernelGGL((k_bwd_prod<T, 8>), g, dim3(256), 0, st, S, items, n, coef, x, ldx, nrhs, part);
 }
 template <typename T>
-void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const BwdItem *bitems, const T *z,
+void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const BwdItem *bitems, T *z,
                      int64_t ldz, T *x, int64_t ldx, int nrhs, const T *part, const T *wb, int maxwidth,
                      hipStream_t st) {
     if (n <= 0) return;
+    if (maxwidth > SOLVE_SPLIT) {
+        hipLaunchKernelGGL(k_bwd_reduce<T>, dim3(grid8(n), nrhs), dim3(256), 0, st, S, items, n, bitems, z, ldz, part,
+                           nrhs);
+        if (nrhs == 1)
+            hipLaunchKernelGGL((k_bwd_diag<T, 1, 16, true>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T),
+                               st, S, items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
+        else
+            hipLaunchKernelGGL((k_bwd_diag<T, 2, 16, true>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
+                               (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, bitems, z, ldz, x, ldx, nrhs, part,
+                               wb);
+        return;
+    }
     const bool narrow = maxwidth <= SOLVE_NARROW;
     if (nrhs == 1) {
         if (maxwidth <= SOLVE_TINY)
-            hipLaunchKernelGGL((k_bwd_diag<T, 1, 1>), dim3(grid8(n)), dim3(64), (size_t)maxwidth * sizeof(T), st, S,
-                               items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
+            hipLaunchKernelGGL(k_bwd_diag_tiny<T>, dim3(grid8((n + 3) / 4)), dim3(64), 0, st, S, items, n, bitems, z, x,
+                               part, wb);
         else if (narrow)
             hipLaunchKernelGGL((k_bwd_diag<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
                                items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
@@ -596,14 +843,14 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
 
 #define INST(T)                                                                                                   \
     template void launch_fwd_diag<T>(const DevSymbol &, const DiagItem *, int, const YSlices &, const T *,       \
-                                     int64_t, const T *, int64_t, T *, int64_t, int, const T *, int,             \
+                                     int64_t, T *, int64_t, T *, int64_t, int, const T *, int,                   \
                                      hipStream_t);                                                               \
     template void launch_fwd_prod<T>(const DevSymbol &, const FwdItem *, int, const T *, const T *, int64_t, int, \
                                      const int32_t *, T *, int64_t, int, hipStream_t);                           \
     template void launch_bwd_prod<T>(const DevSymbol &, const BwdItem *, int, const T *, const T *, int64_t, int, \
                                      T *, int, hipStream_t);                                                     \
-    template void launch_bwd_diag<T>(const DevSymbol &, const DiagItem *, int, const BwdItem *, const T *,        \
-                                     int64_t, T *, int64_t, int, const T *, const T *, int, hipStream_t);
+    template void launch_bwd_diag<T>(const DevSymbol &, const DiagItem *, int, const BwdItem *, T *, int64_t,     \
+                                     T *, int64_t, int, const T *, const T *, int, hipStream_t);
 INST(double)
 INST(float)
 
