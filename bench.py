@@ -533,12 +533,12 @@ def main():
     kern = {}
     for cls, name, fl_ in (("update_direct", "k_update_direct<double,0>", uf["direct"]),
                            ("update_wgemm", "k_update_wgemm<double>", uf["multi"]),
-                           ("update_acc", "k_update_acc<double,0>", uf["single"])):
+                           ("update_acc", "k_update_seq<double,0>", uf["single"])):
         t_, n_ = kt[cls]
         if n_ > 0:
             kern[name] = {"ms": t_ * 1e3, "launches": n_, "avg_launch_ms": t_ / n_ * 1e3,
                           "flops": fl_, "tflops": fl_ / t_ / 1e12}
-    for cls, name in (("panel", "panel steps (k_potrf64 + k_panel_trsm + k_panel_upd)"),
+    for cls, name in (("panel", "panel steps (k_potrf64 + k_panel_trsm + k_panel_upd) and the solve nodes' inverses"),
                       ("update_gather", "k_update_gather<double,0>")):
         t_, n_ = kt[cls]
         if n_ > 0:

@@ -1,8 +1,9 @@
 """Level-schedule autotuner.
 
 The level schedule makes two choices per dependency level: its multi-source
-update tiles are summed in one pass (k_update_acc) or through the product
-workspace (k_update_wgemm + k_update_gather), and its panels are updated
+update tiles are summed in one pass (k_update_seq in FP64, k_update_acc in
+FP32) or through the product workspace (k_update_wgemm + k_update_gather),
+and its panels are updated
 left-looking or right-looking.  The plan's rules pick them from counts
 (average entries per tile, row tiles per level); this tuner times both
 choices of every level on the device and keeps the faster one per level:

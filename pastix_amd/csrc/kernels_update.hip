@@ -8,8 +8,11 @@
 //   k_update_direct : tiles fed by a single source: the product is kept in
 //                     registers (computed transposed so that stores walk down
 //                     target columns) and subtracted straight into C;
-//   k_update_acc    : tiles fed by several sources: contributions are summed
-//                     in an LDS tile, written once at the end.
+//   k_update_seq    : tiles fed by several sources (FP64): each entry's
+//                     product as in k_update_direct, subtracted into C in
+//                     entry order;
+//   k_update_acc    : tiles fed by several sources (FP32): contributions are
+//                     summed in an LDS tile, written once at the end.
 // MODE 0 (LL^t and the L side of LU) keeps pairs with block(p) >= block(q),
 // MODE 1 (U side of LU) keeps block(p) > block(q), as the reference's
 // iterblok >= blok / iterblok > blok loops.
@@ -59,6 +62,54 @@ __global__ __launch_bounds__(256) void k_update_direct(DevSymbol S, const UpdIte
                 if (keep) C[rmap[pi] + (int64_t)cmap[qj] * stf] -= q.c[m_][n_][r_];
             }
         })
+    }
+}
+
+// Tiles fed by several sources: the entries in order, each one's product
+// computed in registers as in k_update_direct and subtracted straight into
+// C (C -= p1, then C -= p2, ..., the reference's source order), with a
+// barrier between entries so that the next entry's read-modify-write of an
+// element another wave wrote sees it.  The direct kernel's LDS footprint
+// (no 64x64 accumulator), so six waves per SIMD instead of three.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_update_seq(DevSymbol S, const UpdItem *__restrict__ items, int nitems,
+                                                    const UpdEntry *__restrict__ ents, const int32_t *__restrict__ shift,
+                                                    const T *__restrict__ Acoef, const T *__restrict__ Bcoef,
+                                                    T *__restrict__ Ccoef) {
+    __shared__ StageBuf<T, UPD_CH> sb;
+    __shared__ int rmap[64], cmap[64], rblk[64], qblk[64];
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    if (id >= nitems) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const UpdItem it = items[id];
+    const int f = it.f, r0 = it.r0, c0 = it.c0;
+    const int fcol = S.cblk_fcol[f];
+    const int64_t stf = S.cblk_stride[f];
+    T *C = Ccoef + S.cblk_coef[f] + r0 + (int64_t)c0 * stf;
+    for (int e = it.e0; e < it.e1; e++) {
+        // the previous entry's stores and map reads are done by every wave
+        if (e > it.e0) __syncthreads();
+        const UpdEntry en = ents[e];
+        update_maps(S, en, shift, r0, fcol + c0, rmap, cmap, rblk, qblk, tid);
+        const int s = en.s;
+        const int np = en.p1 - en.p0, nq = en.q1 - en.q0;
+        const int K = S.cblk_fcol[s + 1] - S.cblk_fcol[s];
+        const int64_t ld = S.cblk_stride[s];
+        Quad<T> q;
+        q.zero();
+        // D(j, i) = B(q_j) . A(p_i): quadrant rows index columns of the target
+        wg_gemm_lds<T, UPD_CH>(q, Bcoef + S.cblk_coef[s] + en.q0, nq, ld, Acoef + S.cblk_coef[s] + en.p0, np, ld, K,
+                               sb, tid);
+        if (wr < nq && wc < np) {
+            UQFOR({
+                const int qj = wr + qrow, pi = wc + qcol;
+                if (qj < nq && pi < np) {
+                    const bool keep = (MODE == 0) ? (rblk[pi] >= qblk[qj]) : (rblk[pi] > qblk[qj]);
+                    if (keep) C[rmap[pi] + (int64_t)cmap[qj] * stf] -= q.c[m_][n_][r_];
+                }
+            })
+        }
     }
 }
 
@@ -260,8 +311,17 @@ void launch_update(const DevSymbol &S, int mode, bool direct, const UpdItem *ite
         if (mode == 0) hipLaunchKernelGGL((k_update_direct<T, 0>), g, b, 0, st, S, items, n, ents, shift, A, B, C);
         else hipLaunchKernelGGL((k_update_direct<T, 1>), g, b, 0, st, S, items, n, ents, shift, A, B, C);
     } else {
-        if (mode == 0) hipLaunchKernelGGL((k_update_acc<T, 0>), g, b, 0, st, S, items, n, ents, shift, A, B, C);
-        else hipLaunchKernelGGL((k_update_acc<T, 1>), g, b, 0, st, S, items, n, ents, shift, A, B, C);
+        // FP64: entries subtracted in turn (six waves per SIMD; the LDS
+        // accumulator's 52 KB allow three): 56.0 -> 54.3 ms numfact on the
+        // 80^3 Laplacian.  FP32: the accumulator (26 KB, six waves already)
+        // saves the extra read-modify-writes of C: 33.1 against 33.5 ms.
+        if constexpr (sizeof(T) == 8) {
+            if (mode == 0) hipLaunchKernelGGL((k_update_seq<T, 0>), g, b, 0, st, S, items, n, ents, shift, A, B, C);
+            else hipLaunchKernelGGL((k_update_seq<T, 1>), g, b, 0, st, S, items, n, ents, shift, A, B, C);
+        } else {
+            if (mode == 0) hipLaunchKernelGGL((k_update_acc<T, 0>), g, b, 0, st, S, items, n, ents, shift, A, B, C);
+            else hipLaunchKernelGGL((k_update_acc<T, 1>), g, b, 0, st, S, items, n, ents, shift, A, B, C);
+        }
     }
 }
 
