@@ -640,6 +640,58 @@ __global__ __launch_bounds__(256) void k_bwd_reduce(DevSymbol S, const DiagItem 
     if (q == 0 && k < K) z[fc + k + u * ldz] -= ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
+// Wide levels, one right-hand side, after the gathers: the diagonal GEMV on
+// 16-row blocks (four workgroups per 64-row item), so that a node's inverse
+// is read by four times as many compute units.  Forward (UPPER false): rows
+// [r, r+16) of z = W y, y = b's node rows (already gathered), columns [0,
+// r+16); backward (UPPER true): rows of x = WB y, y = z's node rows,
+// columns [r, K).  Thread t keeps E consecutive rows (16-byte loads) and the
+// columns c, c + 64/... of its column phase; the phases combine through LDS
+// in a fixed order.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void k_diag_gemv16(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
+                                                     const T *__restrict__ src, T *__restrict__ dst,
+                                                     const T *__restrict__ w) {
+    constexpr int E = Vec<T>::E, RL = 16 / E, NC = 256 / RL;  // lanes per column, column phases
+    extern __shared__ char smem_raw[];
+    T *y = reinterpret_cast<T *>(smem_raw);
+    __shared__ T red[NC][17];
+    const int tid = threadIdx.x;
+    const int gid = xcd_remap(blockIdx.x, gridDim.x);
+    const int id = gid >> 2, sub = gid & 3;
+    if (id >= nitems) return;
+    const DiagItem it = items[id];
+    const int fc = S.cblk_fcol[S.node_cblk[it.nd]];
+    const int K = S.cblk_fcol[S.node_cblk[it.nd + 1]] - fc;
+    const int r = it.ib * 64 + sub * 16;
+    if (r >= K) return;  // uniform: the whole workgroup leaves
+    const int nr = min(16, K - r);
+    const int k0 = UPPER ? r : 0, k1 = UPPER ? K : min(K, r + 16);
+    for (int k = k0 + tid; k < k1; k += 256) y[k - k0] = src[fc + k];
+    __syncthreads();
+    const int rr = E * (tid % RL), c = tid / RL;
+    const T *M = w + S.node_winv[it.nd] + r + rr;
+    T acc[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) acc[e] = 0;
+#pragma unroll 4
+    for (int k = k0 + c; k < k1; k += NC) {
+        T m[E];
+        ld_rows(M + (int64_t)k * K, rr, nr, m);
+        const T yv = y[k - k0];
+#pragma unroll
+        for (int e = 0; e < E; e++) acc[e] += m[e] * yv;
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) red[c][rr + e] = acc[e];
+    __syncthreads();
+    if (tid < nr) {
+        T v = red[0][tid];
+        for (int q = 1; q < NC; q++) v += red[q][tid];
+        dst[fc + r + tid] = v;
+    }
+}
+
 // Levels of narrow nodes (at most SOLVE_TINY columns), one right-hand side:
 // four nodes per wave, 16 lanes each, lane i owning row (forward) or column
 // (backward) i of its node; the node's inverse is read into registers
@@ -734,8 +786,8 @@ void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const YSl
     if (maxwidth > SOLVE_SPLIT) {
         hipLaunchKernelGGL(k_fwd_gather<T>, dim3(grid8(n), nrhs), dim3(256), 0, st, S, items, n, ys, ybuf, ldy, b, ldb);
         if (nrhs == 1)
-            hipLaunchKernelGGL((k_fwd_diag<T, 1, 16, true>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T),
-                               st, S, items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+            hipLaunchKernelGGL((k_diag_gemv16<T, false>), dim3(grid8(4 * n)), dim3(256), (size_t)maxwidth * sizeof(T),
+                               st, S, items, n, (const T *)b, z, w1);
         else
             hipLaunchKernelGGL((k_fwd_diag<T, 2, 16, true>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
                                (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, ys, ybuf, ldy, b, ldb, z, ldz,
@@ -809,8 +861,8 @@ void launch_bwd_diag(const DevSymbol &S, const DiagItem *items, int n, const Bwd
         hipLaunchKernelGGL(k_bwd_reduce<T>, dim3(grid8(n), nrhs), dim3(256), 0, st, S, items, n, bitems, z, ldz, part,
                            nrhs);
         if (nrhs == 1)
-            hipLaunchKernelGGL((k_bwd_diag<T, 1, 16, true>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T),
-                               st, S, items, n, bitems, z, ldz, x, ldx, nrhs, part, wb);
+            hipLaunchKernelGGL((k_diag_gemv16<T, true>), dim3(grid8(4 * n)), dim3(256), (size_t)maxwidth * sizeof(T),
+                               st, S, items, n, (const T *)z, x, wb);
         else
             hipLaunchKernelGGL((k_bwd_diag<T, 2, 16, true>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
                                (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, bitems, z, ldz, x, ldx, nrhs, part,
