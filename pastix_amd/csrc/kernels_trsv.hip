@@ -467,7 +467,9 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
                                                       int64_t ldz, int nrhs, const T *__restrict__ w1) {
     extern __shared__ char smem_raw[];
     __shared__ T red[NW * 64 * NR];
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    // the blocks with the most rows to gather and columns to read (the last
+    // ones of a node) take the lowest workgroup ids, so that they start first
+    const int id = (int)gridDim.x - 1 - xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     // right-hand sides [NR*blockIdx.y, NR*blockIdx.y + NR)
     const int u0 = NR * blockIdx.y;
@@ -657,7 +659,10 @@ __global__ __launch_bounds__(256) void k_diag_gemv16(DevSymbol S, const DiagItem
     T *y = reinterpret_cast<T *>(smem_raw);
     __shared__ T red[NC][17];
     const int tid = threadIdx.x;
-    const int gid = xcd_remap(blockIdx.x, gridDim.x);
+    // forward: the blocks with the most columns (the last rows of a node)
+    // take the lowest workgroup ids, so that they start first
+    const int g0 = xcd_remap(blockIdx.x, gridDim.x);
+    const int gid = UPPER ? g0 : (int)gridDim.x - 1 - g0;
     const int id = gid >> 2, sub = gid & 3;
     if (id >= nitems) return;
     const DiagItem it = items[id];
