@@ -7,6 +7,8 @@
 #include "pastix_data.h"
 #include "kernels.h"
 #include <algorithm>
+#include <new>
+#include <stdexcept>
 #include <atomic>
 #include <functional>
 #include <thread>
@@ -524,6 +526,21 @@ extern "C" int pastix_subtask_reordering(pastix_data_t *d) {
     return PASTIX_SUCCESS;
 }
 
+// The level schedule of the analysed symbol; its failures (host memory, lists
+// beyond the devices' 32-bit indices) come back as error codes.
+static int make_plan(pastix_data_t *d, int64_t sc, const char *who) {
+    try {
+        d->plan = pxa::build_plan(d->symbol, d->layout, sc, d->knobs);
+    } catch (const std::bad_alloc &) {
+        err_print(who, "out of host memory building the level schedule");
+        return PASTIX_ERR_OUTOFMEMORY;
+    } catch (const std::exception &e) {
+        err_print(who, e.what());
+        return PASTIX_ERR_INTERNAL;
+    }
+    return PASTIX_SUCCESS;
+}
+
 extern "C" int pastix_subtask_blend(pastix_data_t *d) {
     if (!d) return PASTIX_ERR_BADPARAMETER;
     if (!(d->steps & STEP_SYMBFACT)) { err_print("pastix_subtask_blend", "pastix_subtask_symbfact() has to be called before"); return PASTIX_ERR_BADPARAMETER; }
@@ -545,7 +562,7 @@ extern "C" int pastix_subtask_blend(pastix_data_t *d) {
         if (read_tuning(d, tf, k) == 1) d->knobs = k;
     }
     d->knobs.node_max = (int32_t)d->ext[PXA_IPARM_SOLVE_NODE_MAX];
-    d->plan = pxa::build_plan(d->symbol, d->layout, sc, d->knobs);
+    if (const int prc = make_plan(d, sc, "pastix_subtask_blend")) return prc;
     int facto = (int)d->iparm[IPARM_FACTORIZATION];
     int ft = (facto == PastixFactGETRF) ? 2 : is_ldlt(d) ? 1 : 0;
     d->dparm[DPARM_FACT_THFLOPS] = pxa::symbol_thflops(d->symbol, ft);
@@ -1525,9 +1542,12 @@ extern "C" int pastix_amd_kernel_times(const pastix_data_t *d, double out[12]) {
 // counterpart: the tuner's and the tuning table's hook).  The next numfact
 // sets the engine up again.
 static int replan(pastix_data_t *d) {
-    d->plan = pxa::build_plan(d->symbol, d->layout, d->schur_cblk, d->knobs);
     d->engine.reset();
     d->steps &= ~(STEP_BCSC2CTAB | STEP_NUMFACT | STEP_SOLVE | STEP_REFINE);
+    if (const int prc = make_plan(d, d->schur_cblk, "pastix_amd_replan")) {
+        d->steps &= ~STEP_ANALYSE;
+        return prc;
+    }
     return PASTIX_SUCCESS;
 }
 extern "C" int pastix_amd_set_level_knobs(pastix_data_t *d, int nlevels, const int8_t *single_pass,

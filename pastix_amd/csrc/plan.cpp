@@ -118,6 +118,9 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk, const Plan
                 while (b < last && s.blok_fcblk[b] == f) b++;
                 const int64_t i1 = b;
                 if (!keep(f)) continue;
+                // shift and entry positions are 32-bit in the device lists
+                if (P.shift.size() + (size_t)(last - i0) > (size_t)INT32_MAX)
+                    throw std::runtime_error("update tile lists exceed 32-bit indices");
                 const int32_t sb = (int32_t)P.shift.size();
                 int64_t fb = s.cblk_bloknum[f];
                 for (int64_t i = i0; i < last; i++) {
@@ -170,6 +173,8 @@ Plan build_plan(const Symbol &s, const Layout &L, int64_t schur_cblk, const Plan
             it.f = (int32_t)(ke[k].key >> 40);
             it.r0 = (int32_t)((ke[k].key >> 16) & 0xFFFFFF) * 64;
             it.c0 = (int32_t)(ke[k].key & 0xFFFF) * 64;
+            if (P.ents.size() + (e - k) > (size_t)INT32_MAX)
+                throw std::runtime_error("update tile lists exceed 32-bit indices");
             it.e0 = (int32_t)P.ents.size();
             for (size_t t = k; t < e; t++) P.ents.push_back(ke[t].e);
             it.e1 = (int32_t)P.ents.size();
