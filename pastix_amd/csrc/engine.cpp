@@ -415,17 +415,11 @@ template <typename T> int Engine::factorize_t(double thr) {
                 // L -= L U^T and U^T -= U^T L^T on the tiles; rows below the
                 // sub-block: L side times inv(U_kk), U side (stored
                 // transposed) times inv(L_kk)^T
-                if (!right) {
-                    launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, false, L, L, U, false, stream_);
-                    launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, false, U, U, L, true, stream_);
-                }
+                // (each pair of sides in one launch)
+                if (!right) launch_panel_upd_lu<T>(S_, d_pupd_ + q0, nq, kb, false, L, U, stream_);
                 launch_getrf64<T>(S_, d_step_cblks_ + b0, nb, kb, L, U, W1, W2, (T *)d_WB_, thr, d_stat_, stream_);
-                launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, L, W2, stream_);
-                launch_panel_trsm<T>(S_, d_ptrsm_ + p0, np, kb, U, W1, stream_);
-                if (right) {
-                    launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, true, L, L, U, false, stream_);
-                    launch_panel_upd<T>(S_, d_pupd_ + q0, nq, kb, true, U, U, L, true, stream_);
-                }
+                launch_panel_trsm2<T>(S_, d_ptrsm_ + p0, np, kb, L, W2, U, W1, stream_);
+                if (right) launch_panel_upd_lu<T>(S_, d_pupd_ + q0, nq, kb, true, L, U, stream_);
             } else if (!right) {
                 // (one launch for the update and the factorisation is slower
                 // here: it reserves the factorisation's LDS for every update

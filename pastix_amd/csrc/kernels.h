@@ -26,9 +26,18 @@ void launch_potrf64(const DevSymbol &S, const int32_t *cblks, int n, int kb, T *
 template <typename T>
 void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, const T *winv,
                        hipStream_t st);
+// Both sides of an LU step's TRSM in one launch: (coef, winv), (coef2, winv2)
+template <typename T>
+void launch_panel_trsm2(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, const T *winv, T *coef2,
+                        const T *winv2, hipStream_t st);
 template <typename T>
 void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, bool right, T *C, const T *X,
                       const T *Y, bool strict, hipStream_t st);
+// Both sides of an LU step's in-cblk update in one launch: (C, X, Y) = (L, L,
+// U) and (U, U, L), the U side strict
+template <typename T>
+void launch_panel_upd_lu(const DevSymbol &S, const PanelItem *items, int n, int kb, bool right, T *L, T *U,
+                         hipStream_t st);
 // An LL^t step's in-cblk update and the diagonal factorisation waiting on it
 // in one launch (right-looking: update of step kb, factorisation of block
 // kb+1; left-looking: update and factorisation of block kb).

@@ -369,15 +369,20 @@ __device__ __forceinline__ void panel_trsm_tile(const DevSymbol &S, const int c,
     }
 }
 
+// blockIdx.y selects the side: (coef, winv), or (coef2, winv2) for the
+// second panel of LU (both sides of a step in one launch: they write
+// different panels and read only the step's diagonal inverses)
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_trsm(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
-                                                    int kb, T *__restrict__ coef, const T *__restrict__ winv) {
+                                                    int kb, T *__restrict__ coef, const T *__restrict__ winv,
+                                                    T *__restrict__ coef2, const T *__restrict__ winv2) {
     __builtin_amdgcn_s_setprio(3);
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
     __shared__ StageBuf<T, 16> sb;
     const PanelItem it = items[id];
-    panel_trsm_tile<T>(S, it.c, it.r0, kb, coef, winv, sb);
+    if (blockIdx.y == 0) panel_trsm_tile<T>(S, it.c, it.r0, kb, coef, winv, sb);
+    else panel_trsm_tile<T>(S, it.c, it.r0, kb, coef2, winv2, sb);
 }
 
 // In-cblk update of column block c0 of the panel (item), 64-row tile r0:
@@ -416,15 +421,19 @@ __device__ __forceinline__ void panel_upd_tile(const DevSymbol &S, const PanelIt
     }
 }
 
+// blockIdx.y = 1: the other side of LU, C = Y = ycoef's panel, X = xcoef's,
+// strict (the two sides write different panels and read only the blocks
+// the step does not write)
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_upd(DevSymbol S, const PanelItem *__restrict__ items, int nitems,
                                                    int kb, int right, T *__restrict__ ccoef,
-                                                   const T *__restrict__ xcoef, const T *__restrict__ ycoef,
+                                                   const T *__restrict__ xcoef, T *__restrict__ ycoef,
                                                    int strict) {
     __builtin_amdgcn_s_setprio(3);
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     if (id >= nitems) return;
-    panel_upd_tile<T>(S, items[id], kb, right, ccoef, xcoef, ycoef, strict);
+    if (blockIdx.y == 0) panel_upd_tile<T>(S, items[id], kb, right, ccoef, xcoef, ycoef, strict);
+    else panel_upd_tile<T>(S, items[id], kb, right, ycoef, ycoef, xcoef, 1);
 }
 
 // One launch for an LL^t panel step's in-cblk update and the diagonal
@@ -963,14 +972,28 @@ template <typename T>
 void launch_panel_trsm(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, const T *winv,
                        hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_panel_trsm<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, coef, winv);
+    hipLaunchKernelGGL(k_panel_trsm<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, coef, winv, coef, winv);
+}
+template <typename T>
+void launch_panel_trsm2(const DevSymbol &S, const PanelItem *items, int n, int kb, T *coef, const T *winv, T *coef2,
+                        const T *winv2, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_panel_trsm<T>, dim3(grid8(n), 2), dim3(256), 0, st, S, items, n, kb, coef, winv, coef2,
+                       winv2);
 }
 template <typename T>
 void launch_panel_upd(const DevSymbol &S, const PanelItem *items, int n, int kb, bool right, T *C, const T *X,
                       const T *Y, bool strict, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, right ? 1 : 0, C, X, Y,
-                       strict ? 1 : 0);
+    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n)), dim3(256), 0, st, S, items, n, kb, right ? 1 : 0, C, X,
+                       const_cast<T *>(Y), strict ? 1 : 0);
+}
+template <typename T>
+void launch_panel_upd_lu(const DevSymbol &S, const PanelItem *items, int n, int kb, bool right, T *L, T *U,
+                         hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_panel_upd<T>, dim3(grid8(n), 2), dim3(256), 0, st, S, items, n, kb, right ? 1 : 0, L, L, U,
+                       0);
 }
 template <typename T>
 void launch_panel_upd_potrf(const DevSymbol &S, const int32_t *cblks, int npot, const PanelItem *items, int n, int kb,
@@ -1016,8 +1039,12 @@ void launch_leaf_tasks(const DevSymbol &S, const LeafDev &LP, const UpdEntry *en
     template void launch_potrf64<T>(const DevSymbol &, const int32_t *, int, int, T *, T *, T *, double, int *,    \
                                     hipStream_t);                                                                  \
     template void launch_panel_trsm<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, hipStream_t); \
+    template void launch_panel_trsm2<T>(const DevSymbol &, const PanelItem *, int, int, T *, const T *, T *,         \
+                                        const T *, hipStream_t);                                                 \
     template void launch_panel_upd<T>(const DevSymbol &, const PanelItem *, int, int, bool, T *, const T *,       \
                                       const T *, bool, hipStream_t);                                             \
+    template void launch_panel_upd_lu<T>(const DevSymbol &, const PanelItem *, int, int, bool, T *, T *,          \
+                                         hipStream_t);                                                           \
     template void launch_panel_upd_potrf<T>(const DevSymbol &, const int32_t *, int, const PanelItem *, int, int,  \
                                             bool, T *, T *, T *, double, int *, hipStream_t);                    \
     template void launch_winv_nodes<T>(const DevSymbol &, const WinvLeaf *, int, const WinvTile *,               \
