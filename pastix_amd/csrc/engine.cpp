@@ -721,6 +721,9 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
     const T *L = (const T *)d_L_;
     const T *U = (facto_ == 2) ? (const T *)d_U_ : L;
     const T *W1 = (const T *)d_W1_, *WB = (const T *)d_WB_;
+    // LL^t: WB = inv(L)^T, the forward inverse transposed (read down its
+    // columns by the wide levels' forward GEMV)
+    const T *W1T = (facto_ != 2) ? WB : nullptr;
     T *part = (T *)d_part_;
     T *ybuf = (T *)d_ybuf_;
     T *z = (T *)d_z_;
@@ -747,7 +750,7 @@ template <typename T> int Engine::solve_t(T *x, int64_t ldx, int nrhs) {
             for (int lv = 0; lv < nl; lv++) {
                 const int d0 = slv_diag_ptr_[lv], nd = slv_diag_ptr_[lv + 1] - d0;
                 const int f0 = slv_fwd_ptr_[lv], nf = slv_fwd_ptr_[lv + 1] - f0;
-                launch_fwd_diag<T>(S_, d_diag_ + d0, nd, ys_, ybuf, ldy, xx, ldx, z, ldz, nr, W1,
+                launch_fwd_diag<T>(S_, d_diag_ + d0, nd, ys_, ybuf, ldy, xx, ldx, z, ldz, nr, W1, W1T,
                                    slv_maxw_[lv], stream_);
                 launch_fwd_prod<T>(S_, d_fwd_ + f0, nf, L, z, ldz, nr, d_ydst_, ybuf, ldy, slv_maxcw_[lv], stream_);
             }

@@ -69,8 +69,8 @@ void launch_winv_nodes(const DevSymbol &S, const WinvLeaf *leaves, int nleaves, 
 // partials in z.
 template <typename T>
 void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const YSlices &ys, const T *ybuf,
-                     int64_t ldy, T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, int maxwidth,
-                     hipStream_t st);
+                     int64_t ldy, T *b, int64_t ldb, T *z, int64_t ldz, int nrhs, const T *w1, const T *w1t,
+                     int maxwidth, hipStream_t st);
 template <typename T>
 void launch_fwd_prod(const DevSymbol &S, const FwdItem *items, int n, const T *coef, const T *z, int64_t ldz,
                      int nrhs, const int32_t *ydst, T *ybuf, int64_t ldy, int maxwidth, hipStream_t st);
