@@ -120,7 +120,8 @@ template <bool CO, typename T, int NR, int NW, bool PRE = false>
 __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem it,
                                               const YSlices ys, const T *__restrict__ ybuf,
                                               int64_t ldy, const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
-                                              int64_t ldz, int nrhs, const T *__restrict__ w1, T *y, T *red) {
+                                              int64_t ldz, int nrhs, const T *__restrict__ w1,
+                                              const T *__restrict__ w1t, T *y, T *red) {
     const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
     const int nd = it.nd, r0 = it.ib * 64;
     const int fc = S.cblk_fcol[S.node_cblk[nd]];
@@ -144,11 +145,21 @@ __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem
     const int rr = E * (r % G), kc = r / G;
     const bool mrow = r0 + rr < Kc;
     const T *M = w1 + S.node_winv[nd] + r0 + rr;
+    // one right-hand side with W^T at hand (LL^t): row i of W is read as
+    // column i of W^T (contiguous), NW lanes per row (row r0 + jt, lane lt)
+    const bool tw = NR == 1 && w1t != nullptr;
+    const int jt = tid / NW, lt = tid % NW, it_row = r0 + jt;
+    const T *MT = tw ? w1t + S.node_winv[nd] + (int64_t)it_row * Kc : nullptr;
     T mpre[PF][E];
 #pragma unroll
     for (int f = 0; f < PF; f++) {
-        const int k = q * E + kc + f * NW * E;
-        if (mrow && k < nrow) ld_rows(M + (int64_t)k * Kc, rr, Kc - r0, mpre[f]);
+        if (tw) {
+            const int k = E * lt + f * NW * E;
+            if (it_row < Kc && k <= it_row) ld_rows(MT + k, k, it_row + 1, mpre[f]);
+        } else {
+            const int k = q * E + kc + f * NW * E;
+            if (mrow && k < nrow) ld_rows(M + (int64_t)k * Kc, rr, Kc - r0, mpre[f]);
+        }
     }
     constexpr int UL = 16;
     if constexpr (PRE) {
@@ -177,6 +188,36 @@ __device__ __forceinline__ void fwd_diag_body(const DevSymbol &S, const DiagItem
         if (task < tasks && j == 0) y[i + u * Kc] = b[fc + i + u * ldb] - v;
     }
     __syncthreads();
+    if (tw) {
+        // W[i, k] = W^T[k, i] is nonzero for k <= i; y holds rows [0, nrow)
+        T a2[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) a2[e] = 0;
+        if (it_row < Kc) {
+#pragma unroll
+            for (int f = 0; f < PF; f++) {
+                const int k = E * lt + f * NW * E;
+                if (k <= it_row) {
+#pragma unroll
+                    for (int e = 0; e < E; e++) a2[e] += mpre[f][e] * y[min(k + e, it_row)];
+                }
+            }
+#pragma unroll 4
+            for (int k = E * lt + PF * NW * E; k <= it_row; k += NW * E) {
+                T m[E];
+                ld_rows(MT + k, k, it_row + 1, m);
+#pragma unroll
+                for (int e = 0; e < E; e++) a2[e] += m[e] * y[min(k + e, it_row)];
+            }
+        }
+        T v = 0;
+#pragma unroll
+        for (int e = 0; e < E; e++) v += a2[e];
+#pragma unroll
+        for (int o = NW / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, NW);
+        if (lt == 0 && it_row < Kc) st_co<CO>(z + fc + it_row, v);
+        return;  // uniform: tw is the same for the whole workgroup
+    }
     const int i = r0 + r;
     T acc[E][NR];
 #pragma unroll
@@ -464,7 +505,8 @@ template <typename T, int NR, int NW, bool PRE = false>
 __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagItem *__restrict__ items, int nitems,
                                                       const YSlices ys, const T *__restrict__ ybuf, int64_t ldy,
                                                       const T *__restrict__ b, int64_t ldb, T *__restrict__ z,
-                                                      int64_t ldz, int nrhs, const T *__restrict__ w1) {
+                                                      int64_t ldz, int nrhs, const T *__restrict__ w1,
+                                                      const T *__restrict__ w1t) {
     extern __shared__ char smem_raw[];
     __shared__ T red[NW * 64 * NR];
     // the blocks with the most rows to gather and columns to read (the last
@@ -474,7 +516,7 @@ __global__ __launch_bounds__(NW * 64) void k_fwd_diag(DevSymbol S, const DiagIte
     // right-hand sides [NR*blockIdx.y, NR*blockIdx.y + NR)
     const int u0 = NR * blockIdx.y;
     fwd_diag_body<false, T, NR, NW, PRE>(S, items[id], ys, ybuf + u0 * ldy, ldy, b + u0 * ldb, ldb, z + u0 * ldz,
-                                    ldz, min(NR, nrhs - u0), w1, reinterpret_cast<T *>(smem_raw), red);
+                                    ldz, min(NR, nrhs - u0), w1, w1t, reinterpret_cast<T *>(smem_raw), red);
 }
 
 template <typename T, int NR, int NW>
@@ -851,7 +893,7 @@ void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const YSl
         else
             hipLaunchKernelGGL((k_fwd_diag<T, 2, 16, true>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
                                (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, ys, ybuf, ldy, b, ldb, z, ldz,
-                               nrhs, w1);
+                               nrhs, w1, (const T *)nullptr);
         return;
     }
     // several right-hand sides: pairs of columns on the grid's y dimension,
@@ -865,19 +907,19 @@ void launch_fwd_diag(const DevSymbol &S, const DiagItem *items, int n, const YSl
                                z, w1);
         else if (narrow)
             hipLaunchKernelGGL((k_fwd_diag<T, 1, 4>), dim3(grid8(n)), dim3(256), (size_t)maxwidth * sizeof(T), st, S,
-                               items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+                               items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1, w1t);
         else
             hipLaunchKernelGGL((k_fwd_diag<T, 1, 16>), dim3(grid8(n)), dim3(1024), (size_t)maxwidth * sizeof(T), st,
-                               S, items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1);
+                               S, items, n, ys, ybuf, ldy, b, ldb, z, ldz, nrhs, w1, w1t);
     } else {
         if (narrow)
             hipLaunchKernelGGL((k_fwd_diag<T, 2, 4>), dim3(grid8(n), (nrhs + 1) / 2), dim3(256),
                                (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, ys, ybuf, ldy, b, ldb, z, ldz,
-                               nrhs, w1);
+                               nrhs, w1, (const T *)nullptr);
         else
             hipLaunchKernelGGL((k_fwd_diag<T, 2, 16>), dim3(grid8(n), (nrhs + 1) / 2), dim3(1024),
                                (size_t)maxwidth * 2 * sizeof(T), st, S, items, n, ys, ybuf, ldy, b, ldb, z, ldz,
-                               nrhs, w1);
+                               nrhs, w1, (const T *)nullptr);
     }
 }
 template <typename T>
